@@ -1,0 +1,94 @@
+// kfx_common.h — internal helpers shared by the HIP translation units of
+// libkfx: error reporting, the library stream, launch checks and the device
+// functions for the cyclic TSDF volume (address and voxel codec).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/kfx.h"
+
+namespace kfx {
+
+// Library stream (created by kfx_init, default priority) on which every
+// entry point enqueues its work.
+hipStream_t stream();
+int current_device();
+bool initialised();
+
+// Records a formatted message for kfx_last_error() and returns `code`.
+int set_error(int code, const char* fmt, ...);
+
+// Brackets a kernel launch with HIP events on the library stream while
+// kfx_profile_enable(1) is in force (no-op otherwise).
+struct ProfScope {
+    explicit ProfScope(const char* name);
+    ~ProfScope();
+    int slot;
+};
+
+}  // namespace kfx
+
+#define KFX_CHECK_HIP(expr)                                                    \
+    do {                                                                       \
+        hipError_t kfx_e_ = (expr);                                            \
+        if (kfx_e_ != hipSuccess)                                              \
+            return ::kfx::set_error(KFX_ERR_HIP, "%s failed: %s", #expr,       \
+                                    hipGetErrorString(kfx_e_));                \
+    } while (0)
+
+#define KFX_CHECK_LAUNCH() KFX_CHECK_HIP(hipGetLastError())
+
+#define KFX_REQUIRE(cond, ...)                                                 \
+    do {                                                                       \
+        if (!(cond)) return ::kfx::set_error(KFX_ERR_INVALID_ARG, __VA_ARGS__); \
+    } while (0)
+
+#define KFX_REQUIRE_INIT()                                                     \
+    do {                                                                       \
+        if (!::kfx::initialised())                                             \
+            return ::kfx::set_error(KFX_ERR_NOT_INITIALISED,                   \
+                                    "kfx_init has not been called");           \
+    } while (0)
+
+namespace kfx {
+
+// ---------------------------------------------------------------------------
+// Cyclic volume addressing (reference: GPU/opencl/raycast.cl:29-47).
+// Per-axis (coordinate + origin) modulo the axis size, linearised x-fastest,
+// plus start_index, folded modulo buffer_length when strictly greater than it
+// (the reference's `>` test is kept as is).
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint32_t cyclic_address(int x, int y, int z,
+                                                            const kfx_round_buffer& rb) {
+    const int rx = (x + rb.origin_GRID.x) % rb.voxels_size.x;
+    const int ry = (y + rb.origin_GRID.y) % rb.voxels_size.y;
+    const int rz = (z + rb.origin_GRID.z) % rb.voxels_size.z;
+    uint64_t a = (uint64_t)(uint32_t)(rb.voxels_size.x * rb.voxels_size.y * rz) +
+                 (uint64_t)(uint32_t)(rb.voxels_size.x * ry) + (uint64_t)(uint32_t)rx;
+    a += (uint64_t)(int64_t)rb.start_index;
+    if (a > (uint64_t)(int64_t)rb.buffer_length) a %= (uint64_t)(int64_t)rb.buffer_length;
+    return (uint32_t)a;
+}
+
+// (v + o) wrapped once into [0, n); valid when 0 <= v + o < 2n, which holds
+// for local coordinates in [0, n) and origins normalised into [0, n).
+__host__ __device__ __forceinline__ int wrap_once(int v, int n) { return v >= n ? v - n : v; }
+
+// Voxel codec (raycast.cl:9-27): decode value / 32767, encode rtz then clamp.
+__device__ __forceinline__ float tsdf_decode(int16_t v) {
+    return (float)v / (float)KFX_MAX_DEPTH_VALUE;
+}
+__device__ __forceinline__ int16_t tsdf_encode(float tsdf) {
+    // rtz conversion of a finite float; saturate before the int cast so the
+    // conversion is defined for every input.
+    float s = tsdf * (float)KFX_MAX_DEPTH_VALUE;
+    s = fminf(fmaxf(s, -2147483520.0f), 2147483520.0f);
+    int fixedp = (int)s;  // truncation toward zero
+    fixedp = max(-KFX_MAX_DEPTH_VALUE, min(KFX_MAX_DEPTH_VALUE, fixedp));
+    return (int16_t)fixedp;
+}
+
+__host__ __device__ __forceinline__ unsigned divup(unsigned a, unsigned b) { return (a + b - 1) / b; }
+
+}  // namespace kfx

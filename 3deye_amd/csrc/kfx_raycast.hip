@@ -1,0 +1,235 @@
+// kfx_raycast.hip — per-pixel ray march through the cyclic TSDF volume.
+//
+// Follows the reference kernel's algorithm (GPU/opencl/raycast.cl:182-405):
+// the ray starts at the camera, is clipped to the volume box [0, volume_size],
+// advances in steps of 0.8 * truncation sampling the nearest voxel, stops at
+// the first + to - sign change, refines the crossing linearly from two
+// trilinear samples, and takes the normal as the normalised central
+// difference of six trilinear samples one cell apart.  Arithmetic is plain
+// IEEE single precision in the reference's operation order (this file is
+// built without floating-point contraction), so the CPU oracle restates it
+// bit for bit.
+//
+// Layout: one lane per pixel, 64 x 4 pixel tiles per workgroup, so the 64
+// rays of a wave are neighbours and their samples fall in the same cache
+// lines of the volume.
+#include <math.h>
+
+#include "kfx_common.h"
+
+namespace kfx {
+
+struct RaycastParams {
+    float r[9];
+    float tx, ty, tz;
+    float csx, csy, csz;
+    float fx, fy, cx, cy;
+    int cols, rows;
+    float time_step;
+    kfx_round_buffer rb;
+};
+
+struct f3 { float x, y, z; };
+
+__device__ __forceinline__ float dot3(f3 a, f3 b) {
+    float r = 0.0f;  // the reference accumulates from 0 (elementary_point_template.h:135-143)
+    r = r + a.x * b.x;
+    r = r + a.y * b.y;
+    r = r + a.z * b.z;
+    return r;
+}
+
+__device__ __forceinline__ f3 normalized3(f3 a) {
+    const float rs = 1.0f / sqrtf(dot3(a, a));
+    return f3{a.x * rs, a.y * rs, a.z * rs};
+}
+
+__device__ __forceinline__ float read_tsdf(const kfx_tsdf* __restrict__ vol, const kfx_round_buffer& rb,
+                                           int x, int y, int z) {
+    return tsdf_decode(vol[cyclic_address(x, y, z, rb)].value);
+}
+
+// floor(p / cell) with saturation (convert_int_sat_rtn).
+__device__ __forceinline__ int voxel_floor(float p, float cell) {
+    float q = floorf(p / cell);
+    if (q != q) return 0;
+    q = fminf(fmaxf(q, -2147483648.0f), 2147483520.0f);
+    return (int)q;
+}
+
+// interpolateTrilineary (raycast.cl:95-145): NaN outside [1, N-2].
+__device__ float interpolate_trilinear(const kfx_tsdf* __restrict__ vol, f3 pt, const RaycastParams& p) {
+    const kfx_round_buffer& rb = p.rb;
+    int gx = voxel_floor(pt.x, p.csx);
+    int gy = voxel_floor(pt.y, p.csy);
+    int gz = voxel_floor(pt.z, p.csz);
+    if (gx <= 0 || gx >= rb.voxels_size.x - 1) return NAN;
+    if (gy <= 0 || gy >= rb.voxels_size.y - 1) return NAN;
+    if (gz <= 0 || gz >= rb.voxels_size.z - 1) return NAN;
+    const float vx = ((float)gx + 0.5f) * p.csx;
+    const float vy = ((float)gy + 0.5f) * p.csy;
+    const float vz = ((float)gz + 0.5f) * p.csz;
+    gx = (pt.x < vx) ? (gx - 1) : gx;
+    gy = (pt.y < vy) ? (gy - 1) : gy;
+    gz = (pt.z < vz) ? (gz - 1) : gz;
+    float a = pt.x / p.csx - ((float)gx + 0.5f);
+    if (a < 0.0f) { gx--; a += 1.0f; }
+    float b = pt.y / p.csy - ((float)gy + 0.5f);
+    if (b < 0.0f) { gy--; b += 1.0f; }
+    float c = pt.z / p.csz - ((float)gz + 0.5f);
+    if (c < 0.0f) { gz--; c += 1.0f; }
+    const float t000 = read_tsdf(vol, rb, gx + 0, gy + 0, gz + 0);
+    const float t001 = read_tsdf(vol, rb, gx + 0, gy + 0, gz + 1);
+    const float t010 = read_tsdf(vol, rb, gx + 0, gy + 1, gz + 0);
+    const float t011 = read_tsdf(vol, rb, gx + 0, gy + 1, gz + 1);
+    const float t100 = read_tsdf(vol, rb, gx + 1, gy + 0, gz + 0);
+    const float t101 = read_tsdf(vol, rb, gx + 1, gy + 0, gz + 1);
+    const float t110 = read_tsdf(vol, rb, gx + 1, gy + 1, gz + 0);
+    const float t111 = read_tsdf(vol, rb, gx + 1, gy + 1, gz + 1);
+    return (1.0f - a) * ((1.0f - b) * (t000 * (1.0f - c) + t001 * c) + b * (t010 * (1.0f - c) + t011 * c)) +
+           a * ((1.0f - b) * (t100 * (1.0f - c) + t101 * c) + b * (t110 * (1.0f - c) + t111 * c));
+}
+
+__device__ __forceinline__ f3 ray_at(f3 o, f3 d, float t) {
+    return f3{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
+}
+
+__global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict__ vol,
+                                                      kfx_xyz* __restrict__ vmap,
+                                                      kfx_xyz* __restrict__ nmap, RaycastParams p) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= p.cols || y >= p.rows) return;
+    const int idx = p.cols * y + x;
+    kfx_xyz vout{NAN, NAN, NAN};
+    kfx_xyz nout{NAN, NAN, NAN};
+
+    const kfx_round_buffer& rb = p.rb;
+    const f3 vol_m{rb.volume_size.x, rb.volume_size.y, rb.volume_size.z};
+    const f3 start{p.tx, p.ty, p.tz};
+    // get_ray_next, R * ray, + t, then (ray_next - start) normalised.
+    const f3 rn{((float)x - p.cx) / p.fx, ((float)y - p.cy) / p.fy, 1.0f};
+    const f3 rr{dot3(f3{p.r[0], p.r[1], p.r[2]}, rn), dot3(f3{p.r[3], p.r[4], p.r[5]}, rn),
+                dot3(f3{p.r[6], p.r[7], p.r[8]}, rn)};
+    const f3 next{rr.x + start.x, rr.y + start.y, rr.z + start.z};
+    f3 dir = normalized3(f3{next.x - start.x, next.y - start.y, next.z - start.z});
+    dir.x = (dir.x == 0.0f) ? 1e-15f : dir.x;
+    dir.y = (dir.y == 0.0f) ? 1e-15f : dir.y;
+    dir.z = (dir.z == 0.0f) ? 1e-15f : dir.z;
+
+    // getMinTime / getMaxTime (raycast.cl:153-171).
+    const float txmin = ((dir.x > 0.0f ? 0.0f : vol_m.x) - start.x) / dir.x;
+    const float tymin = ((dir.y > 0.0f ? 0.0f : vol_m.y) - start.y) / dir.y;
+    const float tzmin = ((dir.z > 0.0f ? 0.0f : vol_m.z) - start.z) / dir.z;
+    float t_enter = fmaxf(fmaxf(txmin, tymin), tzmin);
+    const float txmax = ((dir.x > 0.0f ? vol_m.x : 0.0f) - start.x) / dir.x;
+    const float tymax = ((dir.y > 0.0f ? vol_m.y : 0.0f) - start.y) / dir.y;
+    const float tzmax = ((dir.z > 0.0f ? vol_m.z : 0.0f) - start.z) / dir.z;
+    const float t_exit = fminf(fminf(txmax, tymax), tzmax);
+
+    t_enter = fmaxf(t_enter, 0.0f);
+    if (!(t_enter < t_exit)) {
+        vmap[idx] = vout;
+        nmap[idx] = nout;
+        return;
+    }
+
+    float time_curr = t_enter;
+    const int nxv = rb.voxels_size.x, nyv = rb.voxels_size.y, nzv = rb.voxels_size.z;
+    {
+        const f3 q = ray_at(start, dir, time_curr);
+        int gx = voxel_floor(q.x, p.csx), gy = voxel_floor(q.y, p.csy), gz = voxel_floor(q.z, p.csz);
+        gx = max(0, min(gx, nxv - 1));
+        gy = max(0, min(gy, nyv - 1));
+        gz = max(0, min(gz, nzv - 1));
+        float tsdf = read_tsdf(vol, rb, gx, gy, gz);
+        const float max_time = 3.0f * (vol_m.x + vol_m.y + vol_m.z);
+        const float step = p.time_step;
+
+        for (; time_curr < max_time; time_curr += step) {
+            const float tsdf_prev = tsdf;
+            const float t_next = time_curr + step;
+            const f3 q1 = ray_at(start, dir, t_next);
+            const int vx = voxel_floor(q1.x, p.csx), vy = voxel_floor(q1.y, p.csy), vz = voxel_floor(q1.z, p.csz);
+            if (!(vx >= 0 && vy >= 0 && vz >= 0 && vx < nxv && vy < nyv && vz < nzv)) break;
+            tsdf = read_tsdf(vol, rb, vx, vy, vz);
+            if (tsdf_prev < 0.0f && tsdf > 0.0f) break;
+            if (tsdf_prev > 0.0f && tsdf < 0.0f) {
+                const float ftdt = interpolate_trilinear(vol, q1, p);
+                if (isnan(ftdt)) break;
+                const f3 q2 = ray_at(start, dir, time_curr);
+                const float ft = interpolate_trilinear(vol, q2, p);
+                if (isnan(ft)) break;
+                const float ts = time_curr - step * ft / (ftdt - ft);
+                const f3 vert = ray_at(start, dir, ts);
+                vout = kfx_xyz{vert.x, vert.y, vert.z};
+
+                const f3 q3 = ray_at(start, dir, time_curr);
+                const int hx = voxel_floor(q3.x, p.csx), hy = voxel_floor(q3.y, p.csy), hz = voxel_floor(q3.z, p.csz);
+                if (hx > 1 && hy > 1 && hz > 1 && hx < nxv - 2 && hy < nyv - 2 && hz < nzv - 2) {
+                    f3 t = vert;
+                    t.x += p.csx;
+                    const float fx1 = interpolate_trilinear(vol, t, p);
+                    t = vert;
+                    t.x -= p.csx;
+                    const float fx2 = interpolate_trilinear(vol, t, p);
+                    t = vert;
+                    t.y += p.csy;
+                    const float fy1 = interpolate_trilinear(vol, t, p);
+                    t = vert;
+                    t.y -= p.csy;
+                    const float fy2 = interpolate_trilinear(vol, t, p);
+                    t = vert;
+                    t.z += p.csz;
+                    const float fz1 = interpolate_trilinear(vol, t, p);
+                    t = vert;
+                    t.z -= p.csz;
+                    const float fz2 = interpolate_trilinear(vol, t, p);
+                    const f3 n = normalized3(f3{fx1 - fx2, fy1 - fy2, fz1 - fz2});
+                    nout = kfx_xyz{n.x, n.y, n.z};
+                }
+                break;
+            }
+        }
+    }
+    vmap[idx] = vout;
+    nmap[idx] = nout;
+}
+
+int check_round_buffer(const kfx_round_buffer* rb, const char* who);
+
+}  // namespace kfx
+
+using namespace kfx;
+
+extern "C" int kfx_raycast(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                           const kfx_round_buffer* rb, const kfx_point_3f* translation,
+                           const kfx_matrix_3f* rotation, float truncation_distance,
+                           const kfx_intrinsics* intr, const kfx_point_3f* cell_size) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(volume && vertex_map && normal_map && translation && rotation && intr && cell_size,
+                "kfx_raycast: null pointer");
+    int rc = check_round_buffer(rb, "kfx_raycast");
+    if (rc) return rc;
+    KFX_REQUIRE(intr->cols > 0 && intr->rows > 0, "kfx_raycast: empty image");
+    RaycastParams p;
+    for (int r = 0; r < 3; ++r) {
+        p.r[3 * r + 0] = rotation->data[r].x;
+        p.r[3 * r + 1] = rotation->data[r].y;
+        p.r[3 * r + 2] = rotation->data[r].z;
+    }
+    p.tx = translation->x; p.ty = translation->y; p.tz = translation->z;
+    p.csx = cell_size->x; p.csy = cell_size->y; p.csz = cell_size->z;
+    p.fx = intr->fx; p.fy = intr->fy; p.cx = intr->cx; p.cy = intr->cy;
+    p.cols = intr->cols; p.rows = intr->rows;
+    // time_step = Truncation_Distance * 0.8 (raycast.cl:209), the product
+    // taken in double as a C compiler would with the double literal.
+    p.time_step = (float)((double)truncation_distance * 0.8);
+    p.rb = *rb;
+    ProfScope prof("raycast");
+    dim3 block(64, 4);
+    dim3 grid(divup(intr->cols, 64), divup(intr->rows, 4));
+    raycast_kernel<<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}

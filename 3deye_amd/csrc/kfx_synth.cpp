@@ -1,0 +1,213 @@
+// kfx_synth.cpp — deterministic synthetic input: analytic scene (ground plane,
+// boxes, spheres), camera trajectories and Kinect-like depth frames.
+//
+// Rays are intersected with the primitives in closed form (plane, slab test,
+// sphere quadratic) in double precision, so one frame of 640x480 costs a few
+// milliseconds of host time and every caller (tests, benchmark, denoiser
+// training) sees bit-identical inputs for a given seed.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+#include "../../include/kfx.h"
+
+namespace {
+
+inline uint64_t mix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Uniform double in [0, 1) from the top 53 bits.
+inline double unit(uint64_t h) { return (double)(h >> 11) * (1.0 / 9007199254740992.0); }
+
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    double next() { s = mix(s); return unit(s); }
+    double uniform(double a, double b) { return a + (b - a) * next(); }
+    double normal() {
+        double u1 = next(), u2 = next();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
+};
+
+void look_rotation(const double f_in[3], double R[9]) {
+    // Columns: x right = normalize(down x f), y = f x x, z = f; down = +y.
+    double f[3] = {f_in[0], f_in[1], f_in[2]};
+    double n = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (double& v : f) v /= n;
+    const double down[3] = {0.0, 1.0, 0.0};
+    double x[3] = {down[1] * f[2] - down[2] * f[1], down[2] * f[0] - down[0] * f[2], down[0] * f[1] - down[1] * f[0]};
+    n = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    for (double& v : x) v /= n;
+    double y[3] = {f[1] * x[2] - f[2] * x[1], f[2] * x[0] - f[0] * x[2], f[0] * x[1] - f[1] * x[0]};
+    for (int r = 0; r < 3; ++r) {
+        R[3 * r + 0] = x[r];
+        R[3 * r + 1] = y[r];
+        R[3 * r + 2] = f[r];
+    }
+}
+
+// Smallest positive ray parameter s of o + s d against the scene (d has
+// camera z component 1, so s is the camera z-depth).  0 = no hit.
+double intersect(const kfx_synth_scene& sc, const double o[3], const double d[3]) {
+    double best = INFINITY;
+    if (d[1] > 1e-12) {
+        const double s = (sc.ground_y - o[1]) / d[1];
+        if (s > 0 && s < best) best = s;
+    }
+    for (int i = 0; i < sc.n_objects; ++i) {
+        const kfx_synth_object& ob = sc.objects[i];
+        if (ob.shape == KFX_SYNTH_SPHERE) {
+            const double oc[3] = {o[0] - ob.center[0], o[1] - ob.center[1], o[2] - ob.center[2]};
+            const double a = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double b = oc[0] * d[0] + oc[1] * d[1] + oc[2] * d[2];
+            const double c = oc[0] * oc[0] + oc[1] * oc[1] + oc[2] * oc[2] - ob.half[0] * ob.half[0];
+            const double disc = b * b - a * c;
+            if (disc < 0) continue;
+            const double sq = sqrt(disc);
+            double s = (-b - sq) / a;
+            if (s <= 0) s = (-b + sq) / a;
+            if (s > 0 && s < best) best = s;
+        } else {
+            double t0 = -INFINITY, t1 = INFINITY;
+            bool miss = false;
+            for (int k = 0; k < 3; ++k) {
+                const double lo = ob.center[k] - ob.half[k], hi = ob.center[k] + ob.half[k];
+                if (fabs(d[k]) < 1e-15) {
+                    if (o[k] < lo || o[k] > hi) { miss = true; break; }
+                    continue;
+                }
+                double a = (lo - o[k]) / d[k], b = (hi - o[k]) / d[k];
+                if (a > b) std::swap(a, b);
+                t0 = std::max(t0, a);
+                t1 = std::min(t1, b);
+            }
+            if (miss || t0 > t1) continue;
+            const double s = t0 > 0 ? t0 : t1;
+            if (s > 0 && s < best) best = s;
+        }
+    }
+    return std::isfinite(best) ? best : 0.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t kfx_splitmix64(uint64_t x) { return mix(x); }
+
+int kfx_synth_make_scene(uint64_t seed, int n_boxes, int n_spheres, const double region[4], double ground_y,
+                         kfx_synth_scene* out) {
+    if (!out || !region || n_boxes < 0 || n_spheres < 0 || n_boxes + n_spheres > KFX_SYNTH_MAX_OBJECTS)
+        return KFX_ERR_INVALID_ARG;
+    memset(out, 0, sizeof(*out));
+    out->ground_y = ground_y;
+    Rng rng(mix(seed ^ 0x5CE4E5CE4Eull));
+    int k = 0;
+    for (int i = 0; i < n_boxes; ++i, ++k) {
+        kfx_synth_object& ob = out->objects[k];
+        ob.shape = KFX_SYNTH_BOX;
+        for (int a = 0; a < 3; ++a) ob.half[a] = 0.5 * rng.uniform(0.1, 0.5);
+        ob.center[0] = rng.uniform(region[0], region[1]);
+        ob.center[2] = rng.uniform(region[2], region[3]);
+        ob.center[1] = ground_y - ob.half[1];
+    }
+    for (int i = 0; i < n_spheres; ++i, ++k) {
+        kfx_synth_object& ob = out->objects[k];
+        ob.shape = KFX_SYNTH_SPHERE;
+        ob.half[0] = rng.uniform(0.1, 0.5);
+        ob.half[1] = ob.half[2] = ob.half[0];
+        ob.center[0] = rng.uniform(region[0], region[1]);
+        ob.center[2] = rng.uniform(region[2], region[3]);
+        ob.center[1] = ground_y - ob.half[0];
+    }
+    out->n_objects = k;
+    return KFX_OK;
+}
+
+int kfx_synth_orbit(const double center[3], double radius, double dy, double angle0, double dangle, int n,
+                    kfx_pose* out) {
+    if (!center || !out || n < 0) return KFX_ERR_INVALID_ARG;
+    for (int i = 0; i < n; ++i) {
+        const double a = angle0 + dangle * i;
+        double* t = out[i].t;
+        t[0] = center[0] + radius * sin(a);
+        t[1] = center[1] + dy;
+        t[2] = center[2] - radius * cos(a);
+        const double f[3] = {center[0] - t[0], center[1] - t[1], center[2] - t[2]};
+        look_rotation(f, out[i].R);
+    }
+    return KFX_OK;
+}
+
+int kfx_synth_walk(uint64_t seed, const double start[3], double step, double pitch, int n, kfx_pose* out) {
+    if (!start || !out || n < 0) return KFX_ERR_INVALID_ARG;
+    Rng rng(mix(seed ^ 0x3A1C3A1Cull));
+    double p[3] = {start[0], start[1], start[2]};
+    double heading = 0.0, rate = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double yaw = heading + 0.15 * sin(0.05 * i);
+        const double pt = pitch + 0.05 * sin(0.031 * i);
+        const double f[3] = {cos(yaw) * cos(pt), sin(pt), sin(yaw) * cos(pt)};
+        look_rotation(f, out[i].R);
+        out[i].t[0] = p[0];
+        out[i].t[1] = p[1] + 0.03 * sin(0.07 * i);
+        out[i].t[2] = p[2];
+        rate = 0.9 * rate + 0.1 * 0.05 * rng.normal();
+        heading = std::max(-0.5, std::min(0.5, heading + rate));
+        p[0] += step * cos(heading);
+        p[2] += step * sin(heading);
+    }
+    return KFX_OK;
+}
+
+int kfx_synth_render(const kfx_synth_scene* sc, const kfx_pose* pose, const kfx_intrinsics* in, uint64_t seed,
+                     int frame, int flags, int threads, uint16_t* depth_mm, float* clean_m) {
+    if (!sc || !pose || !in || !depth_mm || in->cols <= 0 || in->rows <= 0) return KFX_ERR_INVALID_ARG;
+    const int W = in->cols, H = in->rows;
+    const uint64_t fkey = mix(mix(seed) ^ mix((uint64_t)frame + 0x1000));
+    int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nt = std::min(nt, H);
+    auto rows = [&](int y0, int y1) {
+        for (int v = y0; v < y1; ++v)
+            for (int u = 0; u < W; ++u) {
+                const double dc[3] = {((double)u - in->cx) / in->fx, ((double)v - in->cy) / in->fy, 1.0};
+                const double* R = pose->R;
+                const double dw[3] = {R[0] * dc[0] + R[1] * dc[1] + R[2] * dc[2],
+                                      R[3] * dc[0] + R[4] * dc[1] + R[5] * dc[2],
+                                      R[6] * dc[0] + R[7] * dc[1] + R[8] * dc[2]};
+                const double z = intersect(*sc, pose->t, dw);
+                const int i = v * W + u;
+                if (clean_m) clean_m[i] = (float)z;
+                double zm = z;
+                if (z > 0 && (flags & KFX_SYNTH_NOISE)) {
+                    const uint64_t h = mix(fkey ^ mix((uint64_t)i));
+                    Rng r(h);
+                    const double sigma_mm = 1.2 + 1.9 * (z - 0.4) * (z - 0.4);
+                    zm = z + 1e-3 * sigma_mm * r.normal();
+                    if (r.next() < 0.02) zm = 0.0;
+                }
+                uint16_t d = 0;
+                if (zm >= 0.4 && zm <= 6.0) d = (uint16_t)lrint(zm * 1000.0);
+                depth_mm[i] = d;
+            }
+    };
+    if (nt <= 1) {
+        rows(0, H);
+    } else {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < nt; ++k) pool.emplace_back(rows, H * k / nt, H * (k + 1) / nt);
+        for (auto& th : pool) th.join();
+    }
+    return KFX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+// kfx_volume.hip — the cyclic TSDF volume: address evaluation, reset and
+// projective integration of one depth frame.
+//
+// Volume layout in HBM: kfx_tsdf {int16 value, int16 weight}, 4 B per voxel,
+// x fastest then y then z after the cyclic offset (raycast.cl:29-47).  A wave
+// of 64 lanes walks 64 neighbouring x columns, so every z step of a wave is
+// one 256-byte contiguous segment; each lane sweeps a run of z for its
+// column ("z-major sweep"), the reference's (Nx, Ny) launch with the z loop
+// split over several workgroups so the chip has enough waves in flight.
+#include "kfx_common.h"
+
+namespace kfx {
+
+// Voxels of one column that a thread handles; grid.z = Nz / kIntegrateZRun.
+constexpr int kIntegrateZRun = 32;
+constexpr int kIntegrateBlockX = 64;
+constexpr int kIntegrateBlockY = 4;
+// Voxels of a run whose loads are issued before any of them is updated.
+constexpr int kIntegrateBatch = 8;
+
+struct IntegrateParams {
+    float r[9];          // rotation_inverse, row-major
+    float tx, ty, tz;    // camera position in volume coordinates
+    float csx, csy, csz; // cell size
+    float fx, fy, cx, cy;
+    int cols, rows;
+    float trunc, inv_trunc;
+    int nx, ny, nz;
+    int ox, oy, oz;      // cyclic origin, each in [0, n)
+    uint32_t slice;      // nx * ny
+};
+
+// scale_depth (opencl_wrappers.cpp:37-43; kernel absent from the reference,
+// KinectFusion convention): metres along the pixel's ray,
+//   lambda = sqrt(((u-cx)/fx)^2 + ((v-cy)/fy)^2 + 1),  D = d_mm * lambda / 1000.
+__global__ void __launch_bounds__(256) scale_depth_kernel(const uint16_t* __restrict__ depth,
+                                                          float* __restrict__ scaled,
+                                                          kfx_intrinsics intr) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y * blockDim.y + threadIdx.y;
+    if (u >= intr.cols || v >= intr.rows) return;
+    const float xl = ((float)u - intr.cx) / intr.fx;
+    const float yl = ((float)v - intr.cy) / intr.fy;
+    const float lambda = sqrtf(xl * xl + yl * yl + 1.0f);
+    const int idx = v * intr.cols + u;
+    scaled[idx] = (float)depth[idx] * lambda / 1000.0f;
+}
+
+// One voxel of the update rule.  Returns true when the voxel's stored value
+// changes and has to be written back.
+__device__ __forceinline__ bool integrate_voxel(const IntegrateParams& p, float gx, float gy,
+                                                float part, int z, const float* __restrict__ scaled,
+                                                float& out_dp, bool& in_band) {
+    const float gz = ((float)z + 0.5f) * p.csz - p.tz;
+    const float camx = p.r[0] * gx + p.r[1] * gy + p.r[2] * gz;
+    const float camy = p.r[3] * gx + p.r[4] * gy + p.r[5] * gz;
+    const float camz = p.r[6] * gx + p.r[7] * gy + p.r[8] * gz;
+    in_band = false;
+    if (!(camz > 0.0f)) return false;
+    const float inv_z = 1.0f / camz;
+    const float fu = rintf(camx * p.fx * inv_z + p.cx);
+    const float fv = rintf(camy * p.fy * inv_z + p.cy);
+    if (!(fu >= 0.0f && fu < (float)p.cols && fv >= 0.0f && fv < (float)p.rows)) return false;
+    const float dp = scaled[(int)fv * p.cols + (int)fu];
+    if (dp == 0.0f) return false;
+    const float sdf = dp - sqrtf(gz * gz + part);
+    if (!(sdf >= -p.trunc)) return false;
+    out_dp = fminf(1.0f, sdf * p.inv_trunc);
+    in_band = true;
+    return true;
+}
+
+__global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
+integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume, IntegrateParams p) {
+    const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
+    const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
+    if (x >= p.nx || y >= p.ny) return;
+    const int z_begin = blockIdx.z * kIntegrateZRun;
+    const int z_end = min(z_begin + kIntegrateZRun, p.nz);
+
+    const float gx = ((float)x + 0.5f) * p.csx - p.tx;
+    const float gy = ((float)y + 0.5f) * p.csy - p.ty;
+    const float part = gx * gx + gy * gy;
+    const uint32_t col = (uint32_t)wrap_once(x + p.ox, p.nx) + (uint32_t)p.nx * (uint32_t)wrap_once(y + p.oy, p.ny);
+
+    for (int z0 = z_begin; z0 < z_end; z0 += kIntegrateBatch) {
+        float tsdf_new[kIntegrateBatch];
+        bool upd[kIntegrateBatch];
+        kfx_tsdf prev[kIntegrateBatch];
+        uint32_t addr[kIntegrateBatch];
+        // Phase 1: projection, depth gather and voxel loads for the batch.
+#pragma unroll
+        for (int k = 0; k < kIntegrateBatch; ++k) {
+            const int z = z0 + k;
+            bool band = false;
+            upd[k] = (z < z_end) && integrate_voxel(p, gx, gy, part, z, scaled, tsdf_new[k], band);
+            addr[k] = col + p.slice * (uint32_t)wrap_once(z + p.oz, p.nz);
+            if (upd[k]) prev[k] = volume[addr[k]];
+        }
+        // Phase 2: running weighted average, quantise, write what changed.
+#pragma unroll
+        for (int k = 0; k < kIntegrateBatch; ++k) {
+            if (!upd[k]) continue;
+            const float tp = tsdf_decode(prev[k].value);
+            const int w = prev[k].weight;
+            const float avg = (tp * (float)w + tsdf_new[k]) / (float)(w + 1);
+            const int w_new = min(w + 1, KFX_MAX_TSDF_WEIGHT);
+            const int16_t v_new = tsdf_encode(avg);
+            if (v_new != prev[k].value || w_new != w) {
+                kfx_tsdf out;
+                out.value = v_new;
+                out.weight = (int16_t)w_new;
+                volume[addr[k]] = out;
+            }
+        }
+    }
+}
+
+// Read-only pass of the update rule: counts the voxels integrate_kernel
+// updates (one 64-bit atomic per wave).
+__global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
+integrate_count_kernel(const float* __restrict__ scaled, IntegrateParams p, unsigned long long* __restrict__ count) {
+    const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
+    const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
+    unsigned n = 0;
+    if (x < p.nx && y < p.ny) {
+        const int z_begin = blockIdx.z * kIntegrateZRun;
+        const int z_end = min(z_begin + kIntegrateZRun, p.nz);
+        const float gx = ((float)x + 0.5f) * p.csx - p.tx;
+        const float gy = ((float)y + 0.5f) * p.csy - p.ty;
+        const float part = gx * gx + gy * gy;
+        for (int z = z_begin; z < z_end; ++z) {
+            float tn;
+            bool band;
+            n += integrate_voxel(p, gx, gy, part, z, scaled, tn, band) ? 1u : 0u;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(count, (unsigned long long)n);
+}
+
+__global__ void cyclic_address_kernel(const int32_t* __restrict__ xyz, uint32_t* __restrict__ out,
+                                      size_t n, kfx_round_buffer rb) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = cyclic_address(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], rb);
+}
+
+// Checks shared by every kernel that sweeps the volume with wrap_once().
+int check_round_buffer(const kfx_round_buffer* rb, const char* who) {
+    KFX_REQUIRE(rb != nullptr, "%s: null round_buffer", who);
+    const kfx_point_3i n = rb->voxels_size;
+    KFX_REQUIRE(n.x > 0 && n.y > 0 && n.z > 0, "%s: voxels_size must be positive", who);
+    KFX_REQUIRE((int64_t)n.x * n.y * n.z == (int64_t)rb->buffer_length,
+                "%s: buffer_length %d != Nx*Ny*Nz", who, rb->buffer_length);
+    KFX_REQUIRE(rb->start_index == 0,
+                "%s: start_index must be 0 (the reference always sets 0; its '>' fold would "
+                "address one voxel past the end otherwise)", who);
+    KFX_REQUIRE(rb->origin_GRID.x >= 0 && rb->origin_GRID.x < n.x && rb->origin_GRID.y >= 0 &&
+                    rb->origin_GRID.y < n.y && rb->origin_GRID.z >= 0 && rb->origin_GRID.z < n.z,
+                "%s: origin_GRID (%d,%d,%d) must lie in [0, N) per axis", who, rb->origin_GRID.x,
+                rb->origin_GRID.y, rb->origin_GRID.z);
+    return KFX_OK;
+}
+
+}  // namespace kfx
+
+using namespace kfx;
+
+extern "C" {
+
+uint32_t kfx_cyclic_address(int x, int y, int z, const kfx_round_buffer* rb) {
+    return cyclic_address(x, y, z, *rb);
+}
+
+int kfx_cyclic_address_batch(const int32_t* xyz, uint32_t* out, size_t n, const kfx_round_buffer* rb) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(rb != nullptr, "kfx_cyclic_address_batch: null round_buffer");
+    KFX_REQUIRE(rb->voxels_size.x > 0 && rb->voxels_size.y > 0 && rb->voxels_size.z > 0 && rb->buffer_length > 0,
+                "kfx_cyclic_address_batch: sizes must be positive");
+    if (n == 0) return KFX_OK;
+    KFX_REQUIRE(xyz && out, "kfx_cyclic_address_batch: null pointer");
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    cyclic_address_kernel<<<blocks, 256, 0, stream()>>>(xyz, out, n, *rb);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_init_tsdf(kfx_tsdf* volume, const kfx_round_buffer* rb) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(volume && rb, "kfx_init_tsdf: null pointer");
+    KFX_REQUIRE(rb->buffer_length > 0, "kfx_init_tsdf: empty volume");
+    return kfx_fill_u8(volume, 0, (size_t)rb->buffer_length * sizeof(kfx_tsdf));
+}
+
+}  // extern "C"
+
+namespace kfx {
+
+static int integrate_setup(const char* who, const uint16_t* depth, float* depth_scaled,
+                           const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                           const kfx_point_3f* translation, float truncation_distance,
+                           const kfx_point_3f* cell_size, const kfx_round_buffer* rb, IntegrateParams& p) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(depth && depth_scaled && intr && rotation_inverse && translation && cell_size,
+                "%s: null pointer", who);
+    int rc = check_round_buffer(rb, who);
+    if (rc) return rc;
+    KFX_REQUIRE(intr->cols > 0 && intr->rows > 0, "%s: empty image", who);
+    KFX_REQUIRE(truncation_distance > 0.0f, "%s: truncation distance must be > 0", who);
+
+    {
+        ProfScope prof("scale_depth");
+        dim3 sblock(64, 4);
+        dim3 sgrid(divup(intr->cols, 64), divup(intr->rows, 4));
+        scale_depth_kernel<<<sgrid, sblock, 0, stream()>>>(depth, depth_scaled, *intr);
+        KFX_CHECK_LAUNCH();
+    }
+
+    for (int r = 0; r < 3; ++r) {
+        p.r[3 * r + 0] = rotation_inverse->data[r].x;
+        p.r[3 * r + 1] = rotation_inverse->data[r].y;
+        p.r[3 * r + 2] = rotation_inverse->data[r].z;
+    }
+    p.tx = translation->x; p.ty = translation->y; p.tz = translation->z;
+    p.csx = cell_size->x; p.csy = cell_size->y; p.csz = cell_size->z;
+    p.fx = intr->fx; p.fy = intr->fy; p.cx = intr->cx; p.cy = intr->cy;
+    p.cols = intr->cols; p.rows = intr->rows;
+    p.trunc = truncation_distance;
+    p.inv_trunc = 1.0f / truncation_distance;
+    p.nx = rb->voxels_size.x; p.ny = rb->voxels_size.y; p.nz = rb->voxels_size.z;
+    p.ox = rb->origin_GRID.x; p.oy = rb->origin_GRID.y; p.oz = rb->origin_GRID.z;
+    p.slice = (uint32_t)p.nx * (uint32_t)p.ny;
+    return KFX_OK;
+}
+
+}  // namespace kfx
+
+extern "C" {
+
+int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
+                       const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                       const kfx_point_3f* translation, float truncation_distance,
+                       const kfx_point_3f* cell_size, const kfx_round_buffer* rb) {
+    KFX_REQUIRE(volume != nullptr, "kfx_integrate_tsdf: null volume");
+    IntegrateParams p;
+    int rc = integrate_setup("kfx_integrate_tsdf", depth, depth_scaled, intr, rotation_inverse, translation,
+                             truncation_distance, cell_size, rb, p);
+    if (rc) return rc;
+    ProfScope prof("integrate");
+    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
+    integrate_kernel<<<grid, block, 0, stream()>>>(depth_scaled, volume, p);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_integrate_count_updates(const uint16_t* depth, float* depth_scaled, const kfx_intrinsics* intr,
+                                const kfx_matrix_3f* rotation_inverse, const kfx_point_3f* translation,
+                                float truncation_distance, const kfx_point_3f* cell_size,
+                                const kfx_round_buffer* rb, uint64_t* count) {
+    KFX_REQUIRE(count != nullptr, "kfx_integrate_count_updates: null output");
+    IntegrateParams p;
+    int rc = integrate_setup("kfx_integrate_count_updates", depth, depth_scaled, intr, rotation_inverse,
+                             translation, truncation_distance, cell_size, rb, p);
+    if (rc) return rc;
+    unsigned long long* dcount = nullptr;
+    KFX_CHECK_HIP(hipMallocAsync((void**)&dcount, sizeof(unsigned long long), stream()));
+    hipError_t e = hipMemsetAsync(dcount, 0, sizeof(unsigned long long), stream());
+    if (e == hipSuccess) {
+        dim3 block(kIntegrateBlockX, kIntegrateBlockY);
+        dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
+        integrate_count_kernel<<<grid, block, 0, stream()>>>(depth_scaled, p, dcount);
+        e = hipGetLastError();
+    }
+    unsigned long long host = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&host, dcount, sizeof(host), hipMemcpyDeviceToHost, stream());
+    hipError_t e2 = hipFreeAsync(dcount, stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    KFX_CHECK_HIP(e);
+    KFX_CHECK_HIP(e2);
+    *count = host;
+    return KFX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,334 @@
+"""ctypes binding of libkfx (include/kfx.h) for Python callers: the tests,
+bench.py and __graft_entry__.
+
+The binding is a thin mirror of the C ABI: ctypes structures with the exact
+layouts of include/kfx_types.h, a status check that raises KfxError with
+kfx_last_error(), and a small device-buffer helper that moves numpy arrays in
+and out of HIP memory through kfx_memcpy_*.  There is no fallback: if
+libkfx.so is missing the import of this module fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkfx.so")
+HEADER_DIR = os.path.join(os.path.dirname(HERE), "include")
+
+
+class KfxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"kfx status {code}: {msg}")
+        self.code = code
+
+
+# --------------------------------------------------------------------------
+# Structures (include/kfx_types.h)
+# --------------------------------------------------------------------------
+class Point3f(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class Point3i(C.Structure):
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32)]
+
+
+class Matrix3f(C.Structure):
+    _fields_ = [("data", Point3f * 3)]
+
+
+class Intrinsics(C.Structure):
+    _fields_ = [("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+                ("cols", C.c_int32), ("rows", C.c_int32)]
+
+    def level(self, i):
+        """intrinsics::operator()(i) (common/src/intrinsics.cpp:7-18)."""
+        d = float(1 << i)
+        return Intrinsics(self.fx / d, self.fy / d, self.cx / d, self.cy / d, self.cols >> i, self.rows >> i)
+
+
+class RoundBuffer(C.Structure):
+    _fields_ = [("start_index", C.c_int32), ("end_index", C.c_int32), ("buffer_length", C.c_int32),
+                ("origin_GRID", Point3i), ("origin_GRID_global", Point3f), ("origin_metric", Point3f),
+                ("volume_size", Point3f), ("voxels_size", Point3i)]
+
+
+class SynthObject(C.Structure):
+    _fields_ = [("shape", C.c_int32), ("reserved", C.c_int32), ("center", C.c_double * 3),
+                ("half", C.c_double * 3)]
+
+
+class SynthScene(C.Structure):
+    _fields_ = [("ground_y", C.c_double), ("n_objects", C.c_int32), ("reserved", C.c_int32),
+                ("objects", SynthObject * 64)]
+
+
+class Pose(C.Structure):
+    _fields_ = [("R", C.c_double * 9), ("t", C.c_double * 3)]
+
+    def matrices(self):
+        return np.array(self.R[:], dtype=np.float64).reshape(3, 3), np.array(self.t[:], dtype=np.float64)
+
+
+assert C.sizeof(Intrinsics) == 24 and C.sizeof(RoundBuffer) == 72 and C.sizeof(Matrix3f) == 36
+
+TSDF_DTYPE = np.dtype([("value", "<i2"), ("weight", "<i2")])
+XYZ_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4")])
+
+
+def point3f(v):
+    return Point3f(float(v[0]), float(v[1]), float(v[2]))
+
+
+def point3i(v):
+    return Point3i(int(v[0]), int(v[1]), int(v[2]))
+
+
+def matrix3f(m):
+    m = np.asarray(m, dtype=np.float64).reshape(3, 3)
+    return Matrix3f((Point3f * 3)(*[Point3f(*[float(c) for c in m[r]]) for r in range(3)]))
+
+
+def round_buffer(voxels, volume_m, origin=(0, 0, 0)):
+    """round_buffer(point_3i voxels_size, point_3f metric_size)
+    (common/src/round_buffer.cpp:65-74): start_index 0, buffer_length Nx*Ny*Nz."""
+    n = [int(v) for v in (voxels if hasattr(voxels, "__len__") else (voxels,) * 3)]
+    s = [float(v) for v in (volume_m if hasattr(volume_m, "__len__") else (volume_m,) * 3)]
+    rb = RoundBuffer()
+    rb.start_index = 0
+    rb.end_index = 0
+    rb.buffer_length = n[0] * n[1] * n[2]
+    rb.origin_GRID = point3i(origin)
+    rb.origin_GRID_global = Point3f(float(origin[0]), float(origin[1]), float(origin[2]))
+    rb.origin_metric = Point3f(0.0, 0.0, 0.0)
+    rb.volume_size = point3f(s)
+    rb.voxels_size = point3i(n)
+    return rb
+
+
+def cell_size(rb):
+    return Point3f(rb.volume_size.x / rb.voxels_size.x, rb.volume_size.y / rb.voxels_size.y,
+                   rb.volume_size.z / rb.voxels_size.z)
+
+
+# --------------------------------------------------------------------------
+# Library loading and prototypes
+# --------------------------------------------------------------------------
+_P = C.c_void_p
+_I = C.c_int
+_SZ = C.c_size_t
+
+
+def header_functions():
+    """Names of every function declared in include/*.h."""
+    names = []
+    for fn in sorted(os.listdir(HEADER_DIR)):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(HEADER_DIR, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w]*\s*\**\s+\**(kfx_\w+)\s*\(", text, flags=re.M)
+    return names
+
+
+_PROTOS = {
+    "kfx_init": (_I, [_I]),
+    "kfx_last_error": (C.c_char_p, []),
+    "kfx_device_count": (_I, [C.POINTER(_I)]),
+    "kfx_synchronize": (_I, []),
+    "kfx_get_stream": (_I, [C.POINTER(_P)]),
+    "kfx_malloc": (_I, [C.POINTER(_P), _SZ]),
+    "kfx_free": (_I, [_P]),
+    "kfx_memcpy_htod": (_I, [_P, _P, _SZ]),
+    "kfx_memcpy_dtoh": (_I, [_P, _P, _SZ]),
+    "kfx_memcpy_dtod": (_I, [_P, _P, _SZ]),
+    "kfx_fill_u8": (_I, [_P, C.c_uint8, _SZ]),
+    "kfx_event_create": (_I, [C.POINTER(_P)]),
+    "kfx_event_destroy": (_I, [_P]),
+    "kfx_event_record": (_I, [_P]),
+    "kfx_event_elapsed_ms": (_I, [_P, _P, C.POINTER(C.c_float)]),
+    "kfx_cyclic_address": (C.c_uint32, [_I, _I, _I, C.POINTER(RoundBuffer)]),
+    "kfx_cyclic_address_batch": (_I, [_P, _P, _SZ, C.POINTER(RoundBuffer)]),
+    "kfx_init_tsdf": (_I, [_P, C.POINTER(RoundBuffer)]),
+    "kfx_integrate_tsdf": (_I, [_P, _P, _P, C.POINTER(Intrinsics), C.POINTER(Matrix3f), C.POINTER(Point3f),
+                                C.c_float, C.POINTER(Point3f), C.POINTER(RoundBuffer)]),
+    "kfx_raycast": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), C.POINTER(Point3f), C.POINTER(Matrix3f), C.c_float,
+                         C.POINTER(Intrinsics), C.POINTER(Point3f)]),
+    "kfx_profile_enable": (_I, [_I]),
+    "kfx_profile_reset": (_I, []),
+    "kfx_profile_query": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "kfx_integrate_count_updates": (_I, [_P, _P, C.POINTER(Intrinsics), C.POINTER(Matrix3f), C.POINTER(Point3f),
+                                         C.c_float, C.POINTER(Point3f), C.POINTER(RoundBuffer),
+                                         C.POINTER(C.c_uint64)]),
+    "kfx_splitmix64": (C.c_uint64, [C.c_uint64]),
+    "kfx_synth_make_scene": (_I, [C.c_uint64, _I, _I, C.POINTER(C.c_double), C.c_double, C.POINTER(SynthScene)]),
+    "kfx_synth_orbit": (_I, [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.c_double, _I,
+                             C.POINTER(Pose)]),
+    "kfx_synth_walk": (_I, [C.c_uint64, C.POINTER(C.c_double), C.c_double, C.c_double, _I, C.POINTER(Pose)]),
+    "kfx_synth_render": (_I, [C.POINTER(SynthScene), C.POINTER(Pose), C.POINTER(Intrinsics), C.c_uint64, _I, _I, _I,
+                              _P, _P]),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libkfx.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in _PROTOS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def register(name, restype, argtypes):
+    """Declare the prototype of a further entry point."""
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+    _PROTOS[name] = (restype, argtypes)
+    return f
+
+
+def check(rc):
+    if rc != 0:
+        raise KfxError(rc, lib.kfx_last_error().decode(errors="replace"))
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(lib, name)(*args))
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = lib.kfx_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def init(device=0):
+    check(lib.kfx_init(device))
+
+
+def synchronize():
+    check(lib.kfx_synchronize())
+
+
+class Buffer:
+    """Device allocation owned by Python (gpu_array_base counterpart)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib.kfx_malloc(C.byref(p), self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib.kfx_memcpy_htod(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def download(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib.kfx_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def fill(self, value=0):
+        check(lib.kfx_fill_u8(self.ptr, value, self.nbytes))
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            check(lib.kfx_free(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None) is not None and self.ptr.value:
+                lib.kfx_free(self.ptr)
+        except Exception:
+            pass
+
+
+class Timer:
+    """HIP-event timer on the library stream."""
+
+    def __init__(self):
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        check(lib.kfx_event_create(C.byref(self.a)))
+        check(lib.kfx_event_create(C.byref(self.b)))
+
+    def start(self):
+        check(lib.kfx_event_record(self.a))
+
+    def stop(self):
+        check(lib.kfx_event_record(self.b))
+
+    def ms(self):
+        v = C.c_float()
+        check(lib.kfx_event_elapsed_ms(self.a, self.b, C.byref(v)))
+        return v.value
+
+
+def profile_enable(on=True):
+    check(lib.kfx_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(lib.kfx_profile_reset())
+
+
+def profile_query(name):
+    """(total milliseconds, launches) of a named kernel since the last reset."""
+    ms, n = C.c_double(), C.c_uint64()
+    check(lib.kfx_profile_query(name.encode(), C.byref(ms), C.byref(n)))
+    return ms.value, n.value
+
+
+# --------------------------------------------------------------------------
+# Synthetic input helpers
+# --------------------------------------------------------------------------
+def make_scene(seed, n_boxes, n_spheres, region, ground_y):
+    sc = SynthScene()
+    reg = (C.c_double * 4)(*[float(v) for v in region])
+    check(lib.kfx_synth_make_scene(seed, n_boxes, n_spheres, reg, float(ground_y), C.byref(sc)))
+    return sc
+
+
+def orbit(center, radius, dy, angle0, dangle, n):
+    poses = (Pose * n)()
+    check(lib.kfx_synth_orbit((C.c_double * 3)(*center), radius, dy, angle0, dangle, n, poses))
+    return poses
+
+
+def walk(seed, start, step, pitch, n):
+    poses = (Pose * n)()
+    check(lib.kfx_synth_walk(seed, (C.c_double * 3)(*start), step, pitch, n, poses))
+    return poses
+
+
+def render(scene, pose, intr, seed, frame, noise=True, threads=0, want_clean=False):
+    depth = np.zeros((intr.rows, intr.cols), dtype=np.uint16)
+    clean = np.zeros((intr.rows, intr.cols), dtype=np.float32) if want_clean else None
+    check(lib.kfx_synth_render(C.byref(scene), C.byref(pose), C.byref(intr), seed, frame, 1 if noise else 0, threads,
+                               depth.ctypes.data_as(C.c_void_p),
+                               clean.ctypes.data_as(C.c_void_p) if clean is not None else None))
+    return (depth, clean) if want_clean else depth
+
+
+def kinect_intrinsics(cols=640, rows=480, f=525.0):
+    return Intrinsics(f, f, (cols - 1) / 2.0, (rows - 1) / 2.0, cols, rows)

@@ -1,0 +1,176 @@
+/*
+ * kfx.h — C ABI of the MI355X-native KinectFusion per-frame pipeline.
+ *
+ * Plain pointers and sizes only.  Every function returns a status code
+ * (KFX_OK == 0); kfx_last_error() holds the message of the most recent
+ * failure on the calling thread.  Device pointers are HIP device memory of
+ * the GPU selected by kfx_init().  Work is enqueued on the library stream and
+ * is asynchronous unless the comment says otherwise; kfx_synchronize() waits
+ * for it.
+ *
+ * Each entry point names the reference interface it replaces
+ * (dbolshak/3deye, common/inc/opencl_wrappers.h and common/src/...).
+ */
+#ifndef KFX_H
+#define KFX_H
+
+#include "kfx_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum kfx_status {
+    KFX_OK = 0,
+    KFX_ERR_INVALID_ARG = 1,
+    KFX_ERR_HIP = 2,
+    KFX_ERR_CAPACITY = 3,       /* output buffer too small; required size reported */
+    KFX_ERR_NOT_INITIALISED = 4,
+    KFX_ERR_SINGULAR = 5,       /* ICP normal equations not positive definite */
+    KFX_ERR_UNSUPPORTED = 6
+};
+
+/* ------------------------------------------------------------------------
+ * Runtime.  Replaces the OpenCL singleton and buffer layer:
+ * opencl_helper::Get_Instance / init_opencl (common/src/opencl_helper.cpp:49-72,169-174),
+ * gpu_array_base::dev_alloc / dev_free / memcpy_* (common/inc/gpu_array_base.h:17-123,
+ * common/src/opencl_helper.cpp:351-462,524-534).
+ * ------------------------------------------------------------------------ */
+
+/* Select the GPU and create the library stream.  Idempotent per device. */
+int kfx_init(int device);
+/* Message of the last failure on this thread ("" if none). */
+const char* kfx_last_error(void);
+/* Number of visible GPUs. */
+int kfx_device_count(int* count);
+/* Wait for all work enqueued on the library stream. */
+int kfx_synchronize(void);
+/* The library stream as a hipStream_t (for callers that interleave work). */
+int kfx_get_stream(void** hip_stream);
+
+int kfx_malloc(void** dev_ptr, size_t bytes);
+int kfx_free(void* dev_ptr);
+/* Blocking copies, as the reference's CL_TRUE transfers. */
+int kfx_memcpy_htod(void* dev_dst, const void* host_src, size_t bytes);
+int kfx_memcpy_dtoh(void* host_dst, const void* dev_src, size_t bytes);
+/* Asynchronous device-to-device copy and byte fill on the library stream. */
+int kfx_memcpy_dtod(void* dev_dst, const void* dev_src, size_t bytes);
+int kfx_fill_u8(void* dev_dst, uint8_t value, size_t bytes);
+
+/* Events on the library stream, for timing (replaces the OpenCL event timing
+ * of opencl_helper::execute_kernel, common/src/opencl_helper.cpp:675-685). */
+int kfx_event_create(void** event);
+int kfx_event_destroy(void* event);
+int kfx_event_record(void* event);
+int kfx_event_elapsed_ms(void* start, void* stop, float* ms);
+
+
+/* Per-kernel timing on the library stream (replaces the per-kernel OpenCL
+ * event printout of opencl_helper::execute_kernel, opencl_helper.cpp:675-685).
+ * While enabled, the library brackets the launches of its named kernels with
+ * HIP events on its own stream; kfx_profile_query sums their durations.
+ * Names: "integrate", "scale_depth", "raycast", ... */
+int kfx_profile_enable(int on);
+int kfx_profile_reset(void);
+int kfx_profile_query(const char* kernel, double* total_ms, uint64_t* launches);
+
+/* ------------------------------------------------------------------------
+ * Cyclic TSDF volume.
+ * ------------------------------------------------------------------------ */
+
+/* Linear voxel address of local grid coordinates (x, y, z), computed on the
+ * host with the device's formula (GPU/opencl/raycast.cl:29-47). */
+uint32_t kfx_cyclic_address(int x, int y, int z, const kfx_round_buffer* rb);
+
+/* Device-side evaluation of the same formula for n coordinate triples
+ * (xyz: 3*n ints on the device, out: n uint32 on the device). */
+int kfx_cyclic_address_batch(const int32_t* xyz, uint32_t* out, size_t n,
+                             const kfx_round_buffer* rb);
+
+/* Reset every voxel to {value 0, weight 0}: the reference's init_tsdf kernel
+ * (listed in common/inc/constants.h:168), for buffer_length voxels. */
+int kfx_init_tsdf(kfx_tsdf* volume, const kfx_round_buffer* rb);
+
+/* Integrate_Tsdf_Data (common/inc/opencl_wrappers.h:10-21,
+ * common/src/opencl_wrappers.cpp:21-64): scale_depth (uint16 mm -> metres
+ * along the pixel ray) into depth_scaled (cols*rows floats, device), then the
+ * projective TSDF update of every voxel column.  rotation_inverse maps world
+ * (volume) coordinates to camera coordinates; translation is the camera
+ * position in volume coordinates. */
+int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
+                       const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                       const kfx_point_3f* translation, float truncation_distance,
+                       const kfx_point_3f* cell_size, const kfx_round_buffer* rb);
+
+
+/* Number of voxels the update rule of kfx_integrate_tsdf changes for this
+ * depth frame and pose (in-image projection, valid depth, sdf >= -trunc).
+ * The rule does not read the volume, so the count is independent of its
+ * state.  Read-only pass; blocking; depth_scaled is written as in
+ * kfx_integrate_tsdf. */
+int kfx_integrate_count_updates(const uint16_t* depth, float* depth_scaled, const kfx_intrinsics* intr,
+                                const kfx_matrix_3f* rotation_inverse, const kfx_point_3f* translation,
+                                float truncation_distance, const kfx_point_3f* cell_size,
+                                const kfx_round_buffer* rb, uint64_t* count);
+
+/* Raycast (common/inc/opencl_wrappers.h:23-33, GPU/opencl/raycast.cl:182-405):
+ * per-pixel march through the cyclic TSDF; writes the surface vertex and the
+ * normal (NaN where there is none) in volume coordinates, cols*rows XYZ each. */
+int kfx_raycast(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                const kfx_round_buffer* rb, const kfx_point_3f* translation,
+                const kfx_matrix_3f* rotation, float truncation_distance,
+                const kfx_intrinsics* intr, const kfx_point_3f* cell_size);
+
+
+/* ------------------------------------------------------------------------
+ * Synthetic input (host only, deterministic): the seeded analytic scene,
+ * camera trajectories and Kinect-like depth frames every test, benchmark and
+ * training run uses.  The reference reads a Kinect through OpenNI
+ * (common/src/Capture.cpp); this generator replaces the sensor.
+ * World frame: y points down (gravity), the ground is the plane y = ground_y.
+ * ------------------------------------------------------------------------ */
+#define KFX_SYNTH_MAX_OBJECTS 64
+enum kfx_synth_shape { KFX_SYNTH_SPHERE = 0, KFX_SYNTH_BOX = 1 };
+typedef struct kfx_synth_object {
+    int32_t shape;     /* kfx_synth_shape */
+    int32_t reserved;
+    double center[3];  /* metres */
+    double half[3];    /* box half extents; sphere radius in half[0] */
+} kfx_synth_object;
+typedef struct kfx_synth_scene {
+    double ground_y;
+    int32_t n_objects;
+    int32_t reserved;
+    kfx_synth_object objects[KFX_SYNTH_MAX_OBJECTS];
+} kfx_synth_scene;
+/* Camera-to-world rigid transform: p_world = R p_cam + t (R row-major). */
+typedef struct kfx_pose { double R[9]; double t[3]; } kfx_pose;
+
+uint64_t kfx_splitmix64(uint64_t x);
+/* n_boxes boxes and n_spheres spheres resting on the ground, sizes and radii
+ * uniform in [0.1, 0.5] m, centres uniform in region {x0, x1, z0, z1}. */
+int kfx_synth_make_scene(uint64_t seed, int n_boxes, int n_spheres, const double region[4],
+                         double ground_y, kfx_synth_scene* out);
+/* n poses on a circle of `radius` around `center`, height offset dy (negative
+ * is up), starting at angle0 and advancing dangle radians per frame, each
+ * looking at `center`. */
+int kfx_synth_orbit(const double center[3], double radius, double dy, double angle0, double dangle,
+                    int n, kfx_pose* out);
+/* Seeded smooth random walk: `step` metres per frame from `start`, heading
+ * wandering around +x, looking ahead and `pitch` radians down. */
+int kfx_synth_walk(uint64_t seed, const double start[3], double step, double pitch, int n, kfx_pose* out);
+/* Render one depth frame (uint16 mm, row-major cols x rows).  flags bit 0:
+ * Kinect noise sigma(z) = 1.2 + 1.9 (z - 0.4)^2 mm, 1 mm quantisation, 2 %
+ * dropout, seeded by splitmix64(seed, frame); without it the exact depth is
+ * rounded to the millimetre.  Depth outside [0.4, 6.0] m reads 0.  clean_m
+ * (optional, cols*rows floats) receives the exact z-depth in metres (0 = no
+ * hit).  threads <= 0 uses every host core. */
+int kfx_synth_render(const kfx_synth_scene* scene, const kfx_pose* pose, const kfx_intrinsics* intr,
+                     uint64_t seed, int frame, int flags, int threads, uint16_t* depth_mm, float* clean_m);
+#define KFX_SYNTH_NOISE 1
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* KFX_H */
