@@ -1,0 +1,490 @@
+// kfx_icp.hip — projective point-to-plane ICP normal equations.
+//
+// Two paths share the per-pixel correspondence and Jacobian row:
+//
+// 1. The reference's three-stage path (Estimate_Combined,
+//    common/src/opencl_wrappers.cpp:364-438): estimate_combined writes one
+//    single-precision partial sum per 32x8 pixel block for each of the 27
+//    entries of the upper triangle of [A | b] into gbuf; transform_estimation
+//    (GPU/opencl/transform_estimation.cl:4-53) reduces each entry with 256
+//    lanes; the host downloads the 27 floats and unpacks A and b.
+//
+// 2. The fused path (kfx_icp_iterate): one launch per ICP iteration.  Each
+//    workgroup accumulates its pixels' 27 products with compensated (TwoSum /
+//    Kahan) single-precision arithmetic and stores (sum, compensation) pairs;
+//    the last workgroup to finish (atomic ticket) adds every pair in double
+//    precision in a fixed order, solves the 6x6 system by Cholesky in double
+//    precision and composes the increment into the device-resident pose.  No
+//    host round trip between iterations, so a whole frame's 19 iterations can
+//    be enqueued (or captured in a graph) at once.
+#include <math.h>
+
+#include "kfx_common.h"
+
+namespace kfx {
+
+struct IcpFrame {
+    float rprev_inv[9];
+    float tprev[3];
+    float fx, fy, cx, cy;
+    int cols, rows;
+    float dist_thr, angle_thr;
+};
+
+// Device-resident ICP state of the fused path.
+struct IcpState {
+    double R[9], t[3];       // current estimate, camera -> volume
+    float Rf[9], tf[3];      // single-precision copy read by the Jacobian
+    double A[36], b[6];      // last normal equations
+    double x[6];             // last increment
+    int status;              // KFX_OK or KFX_ERR_SINGULAR (last solve)
+    int singular_count;
+    int converged_tag;       // tag of the level that has converged (-1 none)
+    int iterations;          // solves performed
+    unsigned int ticket;     // last-workgroup counter, reset by the last workgroup
+    int pad;
+};
+
+// The 27 products: row[i] * row[j] for i <= j, j in [i, 6], row[6] = b
+// (unpacking order of opencl_wrappers.cpp:426-437).
+__device__ __forceinline__ void icp_row(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
+                                        const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev,
+                                        const float* Rc, const float* tc, const IcpFrame& f, int x, int y,
+                                        float row[7]) {
+    for (int k = 0; k < 7; ++k) row[k] = 0.0f;
+    const int i = y * f.cols + x;
+    const kfx_xyz n = ncurr[i];
+    if (isnan(n.x)) return;
+    const kfx_xyz v = vcurr[i];
+    if (isnan(v.x)) return;
+    const float gx = Rc[0] * v.x + Rc[1] * v.y + Rc[2] * v.z + tc[0];
+    const float gy = Rc[3] * v.x + Rc[4] * v.y + Rc[5] * v.z + tc[1];
+    const float gz = Rc[6] * v.x + Rc[7] * v.y + Rc[8] * v.z + tc[2];
+    const float dx = gx - f.tprev[0], dy = gy - f.tprev[1], dz = gz - f.tprev[2];
+    const float px = f.rprev_inv[0] * dx + f.rprev_inv[1] * dy + f.rprev_inv[2] * dz;
+    const float py = f.rprev_inv[3] * dx + f.rprev_inv[4] * dy + f.rprev_inv[5] * dz;
+    const float pz = f.rprev_inv[6] * dx + f.rprev_inv[7] * dy + f.rprev_inv[8] * dz;
+    if (!(pz > 0.0f)) return;
+    const float fu = rintf(px * f.fx / pz + f.cx);
+    const float fv = rintf(py * f.fy / pz + f.cy);
+    if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)f.cols && fv < (float)f.rows)) return;
+    const int j = (int)fv * f.cols + (int)fu;
+    const kfx_xyz np = nprev[j];
+    if (isnan(np.x)) return;
+    const kfx_xyz vp = vprev[j];
+    if (isnan(vp.x)) return;
+    const float ex = vp.x - gx, ey = vp.y - gy, ez = vp.z - gz;
+    if (!(sqrtf(ex * ex + ey * ey + ez * ez) <= f.dist_thr)) return;
+    const float cnx = Rc[0] * n.x + Rc[1] * n.y + Rc[2] * n.z;
+    const float cny = Rc[3] * n.x + Rc[4] * n.y + Rc[5] * n.z;
+    const float cnz = Rc[6] * n.x + Rc[7] * n.y + Rc[8] * n.z;
+    const float sx = cny * np.z - cnz * np.y, sy = cnz * np.x - cnx * np.z, sz = cnx * np.y - cny * np.x;
+    if (!(sqrtf(sx * sx + sy * sy + sz * sz) < f.angle_thr)) return;
+    row[0] = gy * np.z - gz * np.y;
+    row[1] = gz * np.x - gx * np.z;
+    row[2] = gx * np.y - gy * np.x;
+    row[3] = np.x;
+    row[4] = np.y;
+    row[5] = np.z;
+    row[6] = np.x * ex + np.y * ey + np.z * ez;
+}
+
+// ------------------------------------------------------------------------
+// Reference path.
+// ------------------------------------------------------------------------
+constexpr int kEcBX = 32, kEcBY = 8;
+
+__global__ void __launch_bounds__(kEcBX * kEcBY)
+estimate_combined_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
+                         const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev, IcpFrame f,
+                         kfx_matrix_3f Rc_m, kfx_point_3f tc_p, float* __restrict__ gbuf, int stride) {
+    __shared__ float red[kEcBX * kEcBY / 64][27];
+    const int x = blockIdx.x * kEcBX + threadIdx.x;
+    const int y = blockIdx.y * kEcBY + threadIdx.y;
+    const float Rc[9] = {Rc_m.data[0].x, Rc_m.data[0].y, Rc_m.data[0].z, Rc_m.data[1].x, Rc_m.data[1].y,
+                         Rc_m.data[1].z, Rc_m.data[2].x, Rc_m.data[2].y, Rc_m.data[2].z};
+    const float tc[3] = {tc_p.x, tc_p.y, tc_p.z};
+    float row[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (x < f.cols && y < f.rows) icp_row(vcurr, ncurr, vprev, nprev, Rc, tc, f, x, y, row);
+    const int tid = threadIdx.y * kEcBX + threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    int k = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 7; ++j, ++k) {
+            float v = row[i] * row[j];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) red[wave][k] = v;
+        }
+    __syncthreads();
+    if (tid < 27) {
+        float s = 0.0f;
+        for (int w = 0; w < kEcBX * kEcBY / 64; ++w) s += red[w][tid];
+        gbuf[tid * stride + blockIdx.y * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// transform_estimation.cl:4-53 with the row stride as a parameter (the
+// reference hard-codes 1200, correct only for 640x480 level 0).
+__global__ void __launch_bounds__(256) transform_estimation_kernel(const float* __restrict__ gbuf,
+                                                                   float* __restrict__ out, int buf_length,
+                                                                   int stride) {
+    __shared__ float smem[256];
+    const int tid = threadIdx.x;
+    const int start = blockIdx.x * stride;
+    const int end = start + buf_length;
+    float sum = 0.0f;
+    for (int id = start + tid; id < end; id += 256) sum += gbuf[id];
+    smem[tid] = sum;
+    __syncthreads();
+    float val = smem[tid];
+    if (tid < 128) smem[tid] = val = val + smem[tid + 128];
+    __syncthreads();
+    if (tid < 64) smem[tid] = val = val + smem[tid + 64];
+    __syncthreads();
+    for (int off = 32; off >= 1; off >>= 1) {
+        if (tid < 32) smem[tid] = val = val + smem[tid + off];
+        __syncthreads();
+    }
+    if (tid == 0) out[blockIdx.x] = smem[0];
+}
+
+__global__ void clean_buff_kernel(float* __restrict__ g, int width) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= width) return;
+    for (int i = 0; i < 27; ++i) g[i * width + id] = 0.0f;
+}
+
+// ------------------------------------------------------------------------
+// Fused path.
+// ------------------------------------------------------------------------
+constexpr int kIcpThreads = 256;
+constexpr int kIcpPixelsPerThread = 4;
+constexpr int kIcpPixelsPerBlock = kIcpThreads * kIcpPixelsPerThread;
+
+// Error-free sum (Knuth TwoSum): s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated add of a value into (sum, comp).
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    float e;
+    two_sum(s, v, s, e);
+    c += e;
+}
+
+// Compensated merge of two (sum, comp) pairs.
+__device__ __forceinline__ void pair_merge(float& s, float& c, float s2, float c2) {
+    float e;
+    two_sum(s, s2, s, e);
+    c = c + c2 + e;
+}
+
+__device__ void solve_and_update(IcpState* st, const double* sums, int tag, double conv_rot, double conv_trans) {
+    double A[36], b[6];
+    int k = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 7; ++j, ++k) {
+            if (j == 6) b[i] = sums[k];
+            else A[j * 6 + i] = A[i * 6 + j] = sums[k];
+        }
+    for (int i = 0; i < 36; ++i) st->A[i] = A[i];
+    for (int i = 0; i < 6; ++i) st->b[i] = b[i];
+    // Cholesky A = L L^T (lower, in place).
+    double L[36];
+    bool ok = true;
+    for (int i = 0; i < 6 && ok; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            double s = A[i * 6 + j];
+            for (int p = 0; p < j; ++p) s -= L[i * 6 + p] * L[j * 6 + p];
+            if (i == j) {
+                if (!(s > 1e-300) || !isfinite(s)) { ok = false; break; }
+                L[i * 6 + i] = sqrt(s);
+            } else {
+                L[i * 6 + j] = s / L[j * 6 + j];
+            }
+        }
+    }
+    st->iterations += 1;
+    if (!ok) {
+        st->status = KFX_ERR_SINGULAR;
+        st->singular_count += 1;
+        for (int i = 0; i < 6; ++i) st->x[i] = 0.0;
+        return;
+    }
+    double yv[6], x[6];
+    for (int i = 0; i < 6; ++i) {
+        double s = b[i];
+        for (int p = 0; p < i; ++p) s -= L[i * 6 + p] * yv[p];
+        yv[i] = s / L[i * 6 + i];
+    }
+    for (int i = 5; i >= 0; --i) {
+        double s = yv[i];
+        for (int p = i + 1; p < 6; ++p) s -= L[p * 6 + i] * x[p];
+        x[i] = s / L[i * 6 + i];
+    }
+    for (int i = 0; i < 6; ++i) st->x[i] = x[i];
+    st->status = KFX_OK;
+    // Increment: R_inc = Rz(x2) Ry(x1) Rx(x0), t_inc = x[3..5];
+    // t <- R_inc t + t_inc, R <- R_inc R.
+    const double ca = cos(x[0]), sa = sin(x[0]), cb = cos(x[1]), sb = sin(x[1]), cg = cos(x[2]), sg = sin(x[2]);
+    const double Ri[9] = {cg * cb, cg * sb * sa - sg * ca, cg * sb * ca + sg * sa,
+                          sg * cb, sg * sb * sa + cg * ca, sg * sb * ca - cg * sa,
+                          -sb, cb * sa, cb * ca};
+    double R[9], t[3];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c)
+            R[3 * r + c] = Ri[3 * r + 0] * st->R[c] + Ri[3 * r + 1] * st->R[3 + c] + Ri[3 * r + 2] * st->R[6 + c];
+        t[r] = Ri[3 * r + 0] * st->t[0] + Ri[3 * r + 1] * st->t[1] + Ri[3 * r + 2] * st->t[2] + x[3 + r];
+    }
+    for (int i = 0; i < 9; ++i) { st->R[i] = R[i]; st->Rf[i] = (float)R[i]; }
+    for (int i = 0; i < 3; ++i) { st->t[i] = t[i]; st->tf[i] = (float)t[i]; }
+    const double rot = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    const double tr = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+    if (conv_rot > 0.0 && rot < conv_rot && tr < conv_trans) st->converged_tag = tag;
+}
+
+__global__ void __launch_bounds__(kIcpThreads)
+icp_fused_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
+                 const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev, IcpFrame f,
+                 IcpState* __restrict__ st, float2* __restrict__ partials, int tag, double conv_rot,
+                 double conv_trans) {
+    __shared__ float2 wred[kIcpThreads / 64][27];
+    __shared__ double dred[kIcpThreads / 64][27];
+    __shared__ bool last;
+    if (tag >= 0 && st->converged_tag == tag) return;  // uniform across the grid
+    float Rc[9], tc[3];
+    for (int i = 0; i < 9; ++i) Rc[i] = st->Rf[i];
+    for (int i = 0; i < 3; ++i) tc[i] = st->tf[i];
+
+    float s[27], c[27];
+    for (int k = 0; k < 27; ++k) { s[k] = 0.0f; c[k] = 0.0f; }
+    const int npix = f.cols * f.rows;
+    const int base = blockIdx.x * kIcpPixelsPerBlock + threadIdx.x;
+    for (int q = 0; q < kIcpPixelsPerThread; ++q) {
+        const int p = base + q * kIcpThreads;
+        if (p >= npix) break;
+        float row[7];
+        icp_row(vcurr, ncurr, vprev, nprev, Rc, tc, f, p % f.cols, p / f.cols, row);
+        int k = 0;
+        for (int i = 0; i < 6; ++i)
+            for (int j = i; j < 7; ++j, ++k) kahan_add(s[k], c[k], row[i] * row[j]);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < 27; ++k) {
+        float a = s[k], e = c[k];
+        for (int off = 32; off > 0; off >>= 1) {
+            const float a2 = __shfl_xor(a, off), e2 = __shfl_xor(e, off);
+            pair_merge(a, e, a2, e2);
+        }
+        if (lane == 0) wred[wave][k] = make_float2(a, e);
+    }
+    __syncthreads();
+    if (threadIdx.x < 27) {
+        float a = wred[0][threadIdx.x].x, e = wred[0][threadIdx.x].y;
+        for (int w = 1; w < kIcpThreads / 64; ++w) pair_merge(a, e, wred[w][threadIdx.x].x, wred[w][threadIdx.x].y);
+        partials[blockIdx.x * 27 + threadIdx.x] = make_float2(a, e);
+    }
+    // Publish the partials, then take a ticket; the last workgroup reduces.
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = (atomicAdd(&st->ticket, 1u) == gridDim.x - 1);
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const int nblk = gridDim.x;
+    // Fixed-order double-precision sum: lane l of wave w handles entry k =
+    // w + 4 i over blocks l, l + 64, ...; then a shuffle tree per entry.
+    for (int k = wave; k < 27; k += kIcpThreads / 64) {
+        double acc = 0.0;
+        for (int b = lane; b < nblk; b += 64) {
+            const float2 v = partials[b * 27 + k];
+            acc += (double)v.x + (double)v.y;
+        }
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) dred[0][k] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        solve_and_update(st, &dred[0][0], tag, conv_rot, conv_trans);
+        st->ticket = 0u;
+        __threadfence();
+    }
+}
+
+__global__ void icp_reset_kernel(IcpState* st, kfx_pose pose) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int i = 0; i < 9; ++i) { st->R[i] = pose.R[i]; st->Rf[i] = (float)pose.R[i]; }
+    for (int i = 0; i < 3; ++i) { st->t[i] = pose.t[i]; st->tf[i] = (float)pose.t[i]; }
+    st->status = KFX_OK;
+    st->singular_count = 0;
+    st->converged_tag = -1;
+    st->iterations = 0;
+    st->ticket = 0u;
+}
+
+IcpFrame make_frame(const kfx_matrix_3f* rprev_inv, const kfx_point_3f* tprev, const kfx_intrinsics* intr,
+                    float dist_thr, float angle_thr) {
+    IcpFrame f;
+    for (int r = 0; r < 3; ++r) {
+        f.rprev_inv[3 * r + 0] = rprev_inv->data[r].x;
+        f.rprev_inv[3 * r + 1] = rprev_inv->data[r].y;
+        f.rprev_inv[3 * r + 2] = rprev_inv->data[r].z;
+    }
+    f.tprev[0] = tprev->x; f.tprev[1] = tprev->y; f.tprev[2] = tprev->z;
+    f.fx = intr->fx; f.fy = intr->fy; f.cx = intr->cx; f.cy = intr->cy;
+    f.cols = intr->cols; f.rows = intr->rows;
+    f.dist_thr = dist_thr;
+    f.angle_thr = angle_thr;
+    return f;
+}
+
+}  // namespace kfx
+
+struct kfx_icp {
+    kfx::IcpState* state;
+    float2* partials;
+    int max_blocks;
+};
+
+using namespace kfx;
+
+extern "C" {
+
+int kfx_estimate_combined(const kfx_matrix_3f* rotation_current, const kfx_point_3f* translation_current,
+                          const kfx_xyz* vmap_current, const kfx_xyz* nmap_current,
+                          const kfx_matrix_3f* rotation_previous_inverse, const kfx_point_3f* translation_previous,
+                          const kfx_intrinsics* intr, const kfx_xyz* vmap_previous, const kfx_xyz* nmap_previous,
+                          float distance_threshold, float angle_threshold, float* gbuf, int gbuf_stride,
+                          float* mbuf, double* A, double* b) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(rotation_current && translation_current && vmap_current && nmap_current && rotation_previous_inverse &&
+                    translation_previous && intr && vmap_previous && nmap_previous && gbuf && mbuf && A && b,
+                "kfx_estimate_combined: null pointer");
+    KFX_REQUIRE(intr->cols > 0 && intr->rows > 0, "kfx_estimate_combined: empty image");
+    const int gx = (int)divup(intr->cols, kEcBX), gy = (int)divup(intr->rows, kEcBY);
+    const int buf_len = gx * gy;
+    KFX_REQUIRE(gbuf_stride >= buf_len, "kfx_estimate_combined: gbuf stride %d < %d partials per entry",
+                gbuf_stride, buf_len);
+    IcpFrame f = make_frame(rotation_previous_inverse, translation_previous, intr, distance_threshold,
+                            angle_threshold);
+    {
+        ProfScope prof("estimate_combined");
+        estimate_combined_kernel<<<dim3(gx, gy), dim3(kEcBX, kEcBY), 0, stream()>>>(
+            vmap_current, nmap_current, vmap_previous, nmap_previous, f, *rotation_current, *translation_current,
+            gbuf, gbuf_stride);
+        KFX_CHECK_LAUNCH();
+    }
+    {
+        ProfScope prof("transform_estimation");
+        transform_estimation_kernel<<<27, 256, 0, stream()>>>(gbuf, mbuf, buf_len, gbuf_stride);
+        KFX_CHECK_LAUNCH();
+    }
+    float h[27];
+    KFX_CHECK_HIP(hipMemcpyAsync(h, mbuf, sizeof(h), hipMemcpyDeviceToHost, stream()));
+    KFX_CHECK_HIP(hipStreamSynchronize(stream()));
+    int shift = 0;
+    for (int k = 0; k < 6; ++k)
+        for (int j = k; j < 7; ++j) {
+            const double value = h[shift++];
+            if (j == 6) b[k] = value;
+            else A[j * 6 + k] = A[k * 6 + j] = value;
+        }
+    return KFX_OK;
+}
+
+int kfx_clean_buff(float* gbuf, int width) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(gbuf != nullptr && width > 0, "kfx_clean_buff: bad arguments");
+    clean_buff_kernel<<<divup(width, 256), 256, 0, stream()>>>(gbuf, width);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_icp_create(int max_pixels, kfx_icp** out) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(out != nullptr && max_pixels > 0, "kfx_icp_create: bad arguments");
+    kfx_icp* h = new kfx_icp();
+    h->max_blocks = (int)divup((unsigned)max_pixels, (unsigned)kIcpPixelsPerBlock);
+    hipError_t e = hipMalloc((void**)&h->state, sizeof(IcpState));
+    if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(float2) * 27 * (size_t)h->max_blocks);
+    if (e == hipSuccess) e = hipMemsetAsync(h->state, 0, sizeof(IcpState), stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    if (e != hipSuccess) {
+        (void)hipFree(h->state);
+        (void)hipFree(h->partials);
+        delete h;
+        return set_error(KFX_ERR_HIP, "kfx_icp_create: %s", hipGetErrorString(e));
+    }
+    *out = h;
+    return KFX_OK;
+}
+
+int kfx_icp_destroy(kfx_icp* h) {
+    if (h == nullptr) return KFX_OK;
+    KFX_REQUIRE_INIT();
+    KFX_CHECK_HIP(hipStreamSynchronize(stream()));
+    (void)hipFree(h->state);
+    (void)hipFree(h->partials);
+    delete h;
+    return KFX_OK;
+}
+
+int kfx_icp_reset(kfx_icp* h, const kfx_pose* pose) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(h && pose, "kfx_icp_reset: null pointer");
+    icp_reset_kernel<<<1, 64, 0, stream()>>>(h->state, *pose);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap_current,
+                    const kfx_xyz* vmap_previous, const kfx_xyz* nmap_previous,
+                    const kfx_matrix_3f* rotation_previous_inverse, const kfx_point_3f* translation_previous,
+                    const kfx_intrinsics* intr, float distance_threshold, float angle_threshold, int level_tag,
+                    double converge_rotation, double converge_translation) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(h && vmap_current && nmap_current && vmap_previous && nmap_previous && rotation_previous_inverse &&
+                    translation_previous && intr,
+                "kfx_icp_iterate: null pointer");
+    KFX_REQUIRE(intr->cols > 0 && intr->rows > 0, "kfx_icp_iterate: empty image");
+    const int nblk = (int)divup((unsigned)(intr->cols * intr->rows), (unsigned)kIcpPixelsPerBlock);
+    KFX_REQUIRE(nblk <= h->max_blocks, "kfx_icp_iterate: image of %d pixels exceeds the handle's capacity",
+                intr->cols * intr->rows);
+    IcpFrame f = make_frame(rotation_previous_inverse, translation_previous, intr, distance_threshold,
+                            angle_threshold);
+    ProfScope prof("icp_fused");
+    icp_fused_kernel<<<nblk, kIcpThreads, 0, stream()>>>(vmap_current, nmap_current, vmap_previous, nmap_previous, f,
+                                                         h->state, h->partials, level_tag, converge_rotation,
+                                                         converge_translation);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_icp_read(kfx_icp* h, kfx_icp_result* out) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(h && out, "kfx_icp_read: null pointer");
+    IcpState s;
+    KFX_CHECK_HIP(hipMemcpyAsync(&s, h->state, sizeof(s), hipMemcpyDeviceToHost, stream()));
+    KFX_CHECK_HIP(hipStreamSynchronize(stream()));
+    for (int i = 0; i < 9; ++i) out->pose.R[i] = s.R[i];
+    for (int i = 0; i < 3; ++i) out->pose.t[i] = s.t[i];
+    for (int i = 0; i < 36; ++i) out->A[i] = s.A[i];
+    for (int i = 0; i < 6; ++i) { out->b[i] = s.b[i]; out->x[i] = s.x[i]; }
+    out->status = s.status;
+    out->singular_count = s.singular_count;
+    out->iterations = s.iterations;
+    out->converged_tag = s.converged_tag;
+    return KFX_OK;
+}
+
+int kfx_icp_pose_device(kfx_icp* h, const void** dev_R_f32, const void** dev_t_f32) {
+    KFX_REQUIRE(h && dev_R_f32 && dev_t_f32, "kfx_icp_pose_device: null pointer");
+    *dev_R_f32 = &h->state->Rf[0];
+    *dev_t_f32 = &h->state->tf[0];
+    return KFX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,363 @@
+// kfx_preprocess.hip — per-frame depth preprocessing: bilateral filter,
+// depth pyramid, vertex and normal maps, rigid transform of maps and the 2x
+// downsampling of raycast maps for the coarser ICP levels.
+//
+// Images are row-major.  Depth is uint16 millimetres; vertex and normal maps
+// are arrays of 12-byte XYZ structures (the reference's Point_XYZ layout),
+// NaN in x marking an invalid pixel.  The stencil kernels stage a tile plus
+// halo in LDS so each depth/vertex sample is read from HBM once per tile.
+#include <math.h>
+
+#include "kfx_common.h"
+
+namespace kfx {
+
+// --------------------------------------------------------------------------
+// Bilateral filter (GPU/opencl/compute_bilateral_filtering.cl:8-57).
+// Window [c-6, min(c+7, dim-1)) per axis: the far-edge row and column of the
+// image never enter a window, as in the reference.  Weight
+//   exp(-(space2 * 0.5/sigma_s^2 + color2 * 0.5/sigma_r^2)),
+// output clamp(rint(sum(w d) / sum(w)), 0, 32767), zeroed above
+// max_distance_mm.  (The reference kernel stores rint(sum(w d)) instead of
+// the normalised value it computes; this is its intended output.)
+// --------------------------------------------------------------------------
+constexpr int kBilR = 6;
+constexpr int kBilTX = 64, kBilTY = 4;
+constexpr int kBilLX = kBilTX + 2 * kBilR, kBilLY = kBilTY + 2 * kBilR;
+
+__global__ void __launch_bounds__(kBilTX * kBilTY)
+bilateral_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, int cols, int rows,
+                 float space_coef, float color_coef, uint32_t max_distance) {
+    __shared__ uint16_t tile[kBilLY][kBilLX];
+    const int bx = blockIdx.x * kBilTX, by = blockIdx.y * kBilTY;
+    const int tid = threadIdx.y * kBilTX + threadIdx.x;
+    for (int i = tid; i < kBilLX * kBilLY; i += kBilTX * kBilTY) {
+        const int ly = i / kBilLX, lx = i - ly * kBilLX;
+        const int gx = bx + lx - kBilR, gy = by + ly - kBilR;
+        tile[ly][lx] = (gx >= 0 && gx < cols && gy >= 0 && gy < rows) ? src[gy * cols + gx] : (uint16_t)0;
+    }
+    __syncthreads();
+    const int c = bx + threadIdx.x, r = by + threadIdx.y;
+    if (c >= cols || r >= rows) return;
+    const int value = tile[threadIdx.y + kBilR][threadIdx.x + kBilR];
+    const int max_x = min(c + kBilR + 1, cols - 1), max_y = min(r + kBilR + 1, rows - 1);
+    const int min_x = max(c - kBilR, 0), min_y = max(r - kBilR, 0);
+    float sum1 = 0.0f, sum2 = 0.0f;
+    for (int cy = min_y; cy < max_y; ++cy) {
+        const uint16_t* row = &tile[cy - by + kBilR][0];
+        const float dy2 = (float)((r - cy) * (r - cy));
+        for (int cx = min_x; cx < max_x; ++cx) {
+            const int tmp = row[cx - bx + kBilR];
+            const float space2 = (float)((c - cx) * (c - cx)) + dy2;
+            const float diff = (float)(value - tmp);
+            const float w = __expf(-(space2 * space_coef + diff * diff * color_coef));
+            sum1 += (float)tmp * w;
+            sum2 += w;
+        }
+    }
+    int res = (int)rintf(sum1 / sum2);
+    res = max(0, min(res, KFX_MAX_DEPTH_VALUE));
+    if ((uint32_t)res > max_distance) res = 0;
+    dst[r * cols + c] = (uint16_t)res;
+}
+
+// --------------------------------------------------------------------------
+// Depth pyramid (compute_depth_pyramid; call site opencl_wrappers.cpp:244-271,
+// kernel absent — KinectFusion pyrDown): dst(x, y) averages the 5x5
+// neighbourhood of src(2x, 2y) with separable weights {1,4,6,4,1}/16 over the
+// samples within 3 sigma_r (90 mm) of the centre, truncated to integer mm.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) pyr_down_kernel(const uint16_t* __restrict__ src, int scols, int srows,
+                                                       uint16_t* __restrict__ dst, int dcols, int drows,
+                                                       float sigma_color) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= dcols || y >= drows) return;
+    const float wt[3] = {0.375f, 0.25f, 0.0625f};
+    const int center = src[(2 * y) * scols + 2 * x];
+    const int x_lo = max(0, 2 * x - 2) - 2 * x, y_lo = max(0, 2 * y - 2) - 2 * y;
+    const int x_hi = min(scols, 2 * x + 3) - 2 * x, y_hi = min(srows, 2 * y + 3) - 2 * y;
+    const float thr = 3.0f * sigma_color;
+    float sum = 0.0f, wall = 0.0f;
+    for (int yi = y_lo; yi < y_hi; ++yi)
+        for (int xi = x_lo; xi < x_hi; ++xi) {
+            const int val = src[(2 * y + yi) * scols + 2 * x + xi];
+            if ((float)abs(val - center) < thr) {
+                const float w = wt[abs(xi)] * wt[abs(yi)];
+                sum += (float)val * w;
+                wall += w;
+            }
+        }
+    dst[y * dcols + x] = (uint16_t)(int)(sum / wall);
+}
+
+// --------------------------------------------------------------------------
+// Vertex map (compute_vertex_map; call site opencl_wrappers.cpp:281-305,
+// kernel absent): z = d / 1000 m; v = (z (u - cx) / fx, z (v - cy) / fy, z);
+// NaN where d == 0.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) vertex_map_kernel(const uint16_t* __restrict__ depth,
+                                                         kfx_xyz* __restrict__ vmap, kfx_intrinsics in,
+                                                         float fx_inv, float fy_inv) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y * blockDim.y + threadIdx.y;
+    if (u >= in.cols || v >= in.rows) return;
+    const int i = v * in.cols + u;
+    const float z = (float)depth[i] / 1000.0f;
+    kfx_xyz out{NAN, NAN, NAN};
+    if (z != 0.0f) out = kfx_xyz{z * ((float)u - in.cx) * fx_inv, z * ((float)v - in.cy) * fy_inv, z};
+    vmap[i] = out;
+}
+
+// --------------------------------------------------------------------------
+// Normal map (compute_normal_map_eigen; call site opencl_wrappers.cpp:315-343,
+// kernel absent — KinectFusion "eigen" normals): over the window
+// [u-3, min(u+4, cols-1)) x [v-3, min(v+4, rows-1)) of valid vertices (at
+// least 24 of them), the normal is the eigenvector of the smallest eigenvalue
+// of the covariance about the centroid, oriented toward the camera (n . v < 0).
+// --------------------------------------------------------------------------
+constexpr int kNmR = 3;
+constexpr int kNmTX = 32, kNmTY = 8;
+constexpr int kNmLX = kNmTX + 2 * kNmR + 1, kNmLY = kNmTY + 2 * kNmR + 1;
+
+// Eigenvector of the smallest eigenvalue of a symmetric 3x3 matrix
+// {a00, a01, a02, a11, a12, a22}: eigenvalues from the trigonometric form of
+// the characteristic cubic (O. K. Smith, CACM 4(4), 1961), eigenvector as the
+// largest cross product of two rows of (A - lambda I).
+__device__ void smallest_eigvec(const float* cov, float& nx, float& ny, float& nz) {
+    float scale = fmaxf(fmaxf(fabsf(cov[0]), fabsf(cov[1])),
+                        fmaxf(fmaxf(fabsf(cov[2]), fabsf(cov[3])), fmaxf(fabsf(cov[4]), fabsf(cov[5]))));
+    if (!(scale > 0.0f)) { nx = ny = nz = NAN; return; }
+    const float inv = 1.0f / scale;
+    const float a00 = cov[0] * inv, a01 = cov[1] * inv, a02 = cov[2] * inv;
+    const float a11 = cov[3] * inv, a12 = cov[4] * inv, a22 = cov[5] * inv;
+    const float q = (a00 + a11 + a22) * (1.0f / 3.0f);
+    const float p1 = a01 * a01 + a02 * a02 + a12 * a12;
+    const float b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
+    const float p2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0f * p1;
+    float lambda;
+    if (p2 <= 1e-30f) {
+        lambda = q;
+    } else {
+        const float p = sqrtf(p2 * (1.0f / 6.0f));
+        const float ip = 1.0f / p;
+        const float c00 = b00 * ip, c11 = b11 * ip, c22 = b22 * ip;
+        const float c01 = a01 * ip, c02 = a02 * ip, c12 = a12 * ip;
+        const float det = c00 * (c11 * c22 - c12 * c12) - c01 * (c01 * c22 - c12 * c02) + c02 * (c01 * c12 - c11 * c02);
+        const float r = fminf(1.0f, fmaxf(-1.0f, 0.5f * det));
+        const float phi = acosf(r) * (1.0f / 3.0f);
+        lambda = q + 2.0f * p * cosf(phi + 2.0943951023931957f);
+    }
+    const float r0x = a00 - lambda, r0y = a01, r0z = a02;
+    const float r1x = a01, r1y = a11 - lambda, r1z = a12;
+    const float r2x = a02, r2y = a12, r2z = a22 - lambda;
+    const float c0x = r0y * r1z - r0z * r1y, c0y = r0z * r1x - r0x * r1z, c0z = r0x * r1y - r0y * r1x;
+    const float c1x = r0y * r2z - r0z * r2y, c1y = r0z * r2x - r0x * r2z, c1z = r0x * r2y - r0y * r2x;
+    const float c2x = r1y * r2z - r1z * r2y, c2y = r1z * r2x - r1x * r2z, c2z = r1x * r2y - r1y * r2x;
+    const float d0 = c0x * c0x + c0y * c0y + c0z * c0z;
+    const float d1 = c1x * c1x + c1y * c1y + c1z * c1z;
+    const float d2 = c2x * c2x + c2y * c2y + c2z * c2z;
+    float vx = c0x, vy = c0y, vz = c0z, dm = d0;
+    if (d1 > dm) { vx = c1x; vy = c1y; vz = c1z; dm = d1; }
+    if (d2 > dm) { vx = c2x; vy = c2y; vz = c2z; dm = d2; }
+    if (!(dm > 0.0f)) { nx = ny = nz = NAN; return; }
+    const float s = 1.0f / sqrtf(dm);
+    nx = vx * s; ny = vy * s; nz = vz * s;
+}
+
+__global__ void __launch_bounds__(kNmTX * kNmTY)
+normal_map_kernel(const kfx_xyz* __restrict__ vmap, kfx_xyz* __restrict__ nmap, int cols, int rows) {
+    __shared__ float sx[kNmLY][kNmLX], sy[kNmLY][kNmLX], sz[kNmLY][kNmLX];
+    const int bx = blockIdx.x * kNmTX, by = blockIdx.y * kNmTY;
+    const int tid = threadIdx.y * kNmTX + threadIdx.x;
+    for (int i = tid; i < kNmLX * kNmLY; i += kNmTX * kNmTY) {
+        const int ly = i / kNmLX, lx = i - ly * kNmLX;
+        const int gx = bx + lx - kNmR, gy = by + ly - kNmR;
+        kfx_xyz v{NAN, NAN, NAN};
+        if (gx >= 0 && gx < cols && gy >= 0 && gy < rows) v = vmap[gy * cols + gx];
+        sx[ly][lx] = v.x; sy[ly][lx] = v.y; sz[ly][lx] = v.z;
+    }
+    __syncthreads();
+    const int u = bx + threadIdx.x, v = by + threadIdx.y;
+    if (u >= cols || v >= rows) return;
+    kfx_xyz out{NAN, NAN, NAN};
+    const int lu = threadIdx.x + kNmR, lv = threadIdx.y + kNmR;
+    if (!isnan(sx[lv][lu])) {
+        const int ty = min(v + kNmR + 1, rows - 1), tx = min(u + kNmR + 1, cols - 1);
+        const int y0 = max(v - kNmR, 0), x0 = max(u - kNmR, 0);
+        float cxs = 0.0f, cys = 0.0f, czs = 0.0f;
+        int count = 0;
+        for (int cy = y0; cy < ty; ++cy)
+            for (int cx = x0; cx < tx; ++cx) {
+                const int ly = cy - by + kNmR, lx = cx - bx + kNmR;
+                const float px = sx[ly][lx];
+                if (!isnan(px)) { cxs += px; cys += sy[ly][lx]; czs += sz[ly][lx]; ++count; }
+            }
+        if (count >= (2 * kNmR + 1) * (2 * kNmR + 1) / 2) {
+            const float ic = 1.0f / (float)count;
+            cxs *= ic; cys *= ic; czs *= ic;
+            float cov[6] = {0, 0, 0, 0, 0, 0};
+            for (int cy = y0; cy < ty; ++cy)
+                for (int cx = x0; cx < tx; ++cx) {
+                    const int ly = cy - by + kNmR, lx = cx - bx + kNmR;
+                    const float px = sx[ly][lx];
+                    if (isnan(px)) continue;
+                    const float dx = px - cxs, dy = sy[ly][lx] - cys, dz = sz[ly][lx] - czs;
+                    cov[0] += dx * dx; cov[1] += dx * dy; cov[2] += dx * dz;
+                    cov[3] += dy * dy; cov[4] += dy * dz; cov[5] += dz * dz;
+                }
+            float nx, ny, nz;
+            smallest_eigvec(cov, nx, ny, nz);
+            if (!isnan(nx)) {
+                if (nx * sx[lv][lu] + ny * sy[lv][lu] + nz * sz[lv][lu] > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+                out = kfx_xyz{nx, ny, nz};
+            }
+        }
+    }
+    nmap[v * cols + u] = out;
+}
+
+// --------------------------------------------------------------------------
+// transform_map (GPU/opencl/transform_map.cl:6-65): v' = R v + t, n' = R n,
+// NaN passthrough keyed on the x component.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) transform_maps_kernel(const kfx_xyz* __restrict__ vs,
+                                                             const kfx_xyz* __restrict__ ns, kfx_matrix_3f R,
+                                                             kfx_point_3f t, int cols, int rows,
+                                                             kfx_xyz* __restrict__ vd, kfx_xyz* __restrict__ nd) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= cols || y >= rows) return;
+    const int i = x + cols * y;
+    auto mul = [&](kfx_xyz a) {
+        kfx_xyz o;
+        float s;
+        s = 0.0f; s = s + R.data[0].x * a.x; s = s + R.data[0].y * a.y; s = s + R.data[0].z * a.z; o.x = s;
+        s = 0.0f; s = s + R.data[1].x * a.x; s = s + R.data[1].y * a.y; s = s + R.data[1].z * a.z; o.y = s;
+        s = 0.0f; s = s + R.data[2].x * a.x; s = s + R.data[2].y * a.y; s = s + R.data[2].z * a.z; o.z = s;
+        return o;
+    };
+    const kfx_xyz v = vs[i];
+    kfx_xyz vo{NAN, NAN, NAN};
+    if (!isnan(v.x)) {
+        vo = mul(v);
+        vo.x = vo.x + t.x; vo.y = vo.y + t.y; vo.z = vo.z + t.z;
+    }
+    vd[i] = vo;
+    const kfx_xyz n = ns[i];
+    kfx_xyz no{NAN, NAN, NAN};
+    if (!isnan(n.x)) no = mul(n);
+    nd[i] = no;
+}
+
+// --------------------------------------------------------------------------
+// resize_vertex_map / resize_normal_map (call sites opencl_wrappers.cpp:97-146,
+// kernels absent — KinectFusion resizeMap): out(x, y) is the mean of the 2x2
+// block at (2x, 2y) when all four are valid, NaN otherwise; normals are
+// renormalised.
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) resize_map_kernel(const kfx_xyz* __restrict__ in, int icols,
+                                                         kfx_xyz* __restrict__ out, int ocols, int orows,
+                                                         int normalize) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= ocols || y >= orows) return;
+    const kfx_xyz a = in[(2 * y) * icols + 2 * x], b = in[(2 * y) * icols + 2 * x + 1];
+    const kfx_xyz c = in[(2 * y + 1) * icols + 2 * x], d = in[(2 * y + 1) * icols + 2 * x + 1];
+    kfx_xyz o{NAN, NAN, NAN};
+    if (!(isnan(a.x) || isnan(b.x) || isnan(c.x) || isnan(d.x))) {
+        o.x = (a.x + b.x + c.x + d.x) / 4.0f;
+        o.y = (a.y + b.y + c.y + d.y) / 4.0f;
+        o.z = (a.z + b.z + c.z + d.z) / 4.0f;
+        if (normalize) {
+            const float s = 1.0f / sqrtf(o.x * o.x + o.y * o.y + o.z * o.z);
+            o.x *= s; o.y *= s; o.z *= s;
+        }
+    }
+    out[y * ocols + x] = o;
+}
+
+}  // namespace kfx
+
+using namespace kfx;
+
+extern "C" {
+
+int kfx_bilateral_filter(const uint16_t* src, uint16_t* dst, int cols, int rows, float sigma_space,
+                         float sigma_color, uint32_t max_distance_mm) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(src && dst, "kfx_bilateral_filter: null pointer");
+    KFX_REQUIRE(src != dst, "kfx_bilateral_filter: in-place filtering is not supported");
+    KFX_REQUIRE(cols > 0 && rows > 0, "kfx_bilateral_filter: empty image");
+    KFX_REQUIRE(sigma_space > 0.0f && sigma_color > 0.0f, "kfx_bilateral_filter: sigmas must be > 0");
+    ProfScope prof("bilateral");
+    const float space_coef = 0.5f / (sigma_space * sigma_space);
+    const float color_coef = 0.5f / (sigma_color * sigma_color);
+    dim3 block(kBilTX, kBilTY), grid(divup(cols, kBilTX), divup(rows, kBilTY));
+    bilateral_kernel<<<grid, block, 0, stream()>>>(src, dst, cols, rows, space_coef, color_coef, max_distance_mm);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_pyr_down(const uint16_t* src, int src_cols, int src_rows, uint16_t* dst, float sigma_color) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(src && dst, "kfx_pyr_down: null pointer");
+    KFX_REQUIRE(src_cols >= 2 && src_rows >= 2, "kfx_pyr_down: source smaller than 2x2");
+    ProfScope prof("pyr_down");
+    const int dc = src_cols / 2, dr = src_rows / 2;
+    dim3 block(64, 4), grid(divup(dc, 64), divup(dr, 4));
+    pyr_down_kernel<<<grid, block, 0, stream()>>>(src, src_cols, src_rows, dst, dc, dr, sigma_color);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_vertex_map(const uint16_t* depth, kfx_xyz* vmap, const kfx_intrinsics* intr) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(depth && vmap && intr, "kfx_vertex_map: null pointer");
+    KFX_REQUIRE(intr->cols > 0 && intr->rows > 0, "kfx_vertex_map: empty image");
+    ProfScope prof("vertex_map");
+    dim3 block(64, 4), grid(divup(intr->cols, 64), divup(intr->rows, 4));
+    vertex_map_kernel<<<grid, block, 0, stream()>>>(depth, vmap, *intr, 1.0f / intr->fx, 1.0f / intr->fy);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_normal_map(const kfx_xyz* vmap, kfx_xyz* nmap, int cols, int rows) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(vmap && nmap, "kfx_normal_map: null pointer");
+    KFX_REQUIRE(vmap != nmap, "kfx_normal_map: in-place is not supported");
+    KFX_REQUIRE(cols > 0 && rows > 0, "kfx_normal_map: empty image");
+    ProfScope prof("normal_map");
+    dim3 block(kNmTX, kNmTY), grid(divup(cols, kNmTX), divup(rows, kNmTY));
+    normal_map_kernel<<<grid, block, 0, stream()>>>(vmap, nmap, cols, rows);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_transform_maps(const kfx_xyz* vmap_src, const kfx_xyz* nmap_src, const kfx_matrix_3f* rotation,
+                       const kfx_point_3f* translation, int cols, int rows, kfx_xyz* vmap_dst, kfx_xyz* nmap_dst) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(vmap_src && nmap_src && rotation && translation && vmap_dst && nmap_dst,
+                "kfx_transform_maps: null pointer");
+    KFX_REQUIRE(cols > 0 && rows > 0, "kfx_transform_maps: empty image");
+    ProfScope prof("transform_maps");
+    dim3 block(64, 4), grid(divup(cols, 64), divup(rows, 4));
+    transform_maps_kernel<<<grid, block, 0, stream()>>>(vmap_src, nmap_src, *rotation, *translation, cols, rows,
+                                                        vmap_dst, nmap_dst);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_resize_map(const kfx_xyz* in, int in_cols, int in_rows, kfx_xyz* out, int normalize) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(in && out, "kfx_resize_map: null pointer");
+    KFX_REQUIRE(in_cols >= 2 && in_rows >= 2, "kfx_resize_map: source smaller than 2x2");
+    ProfScope prof("resize_map");
+    const int oc = in_cols / 2, orow = in_rows / 2;
+    dim3 block(64, 4), grid(divup(oc, 64), divup(orow, 4));
+    resize_map_kernel<<<grid, block, 0, stream()>>>(in, in_cols, out, oc, orow, normalize);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+}  // extern "C"

@@ -12,7 +12,11 @@
 namespace kfx {
 
 // Voxels of one column that a thread handles; grid.z = Nz / kIntegrateZRun.
-constexpr int kIntegrateZRun = 32;
+constexpr int kIntegrateZRun = 64;
+// Frustum side planes are pushed out by this many pixels and the clipped z
+// range by one voxel on each side, so the clip never removes a voxel the
+// per-voxel projection test would accept.
+constexpr float kClipMarginPx = 2.0f;
 constexpr int kIntegrateBlockX = 64;
 constexpr int kIntegrateBlockY = 4;
 // Voxels of a run whose loads are issued before any of them is updated.
@@ -28,7 +32,42 @@ struct IntegrateParams {
     int nx, ny, nz;
     int ox, oy, oz;      // cyclic origin, each in [0, n)
     uint32_t slice;      // nx * ny
+    // Frustum planes through the camera centre in volume coordinates:
+    // a voxel centre g (relative to the camera) can project into the image
+    // only if dot(plane[k], g) >= 0 for every k (near, left, right, top, bottom).
+    float plane[5][3];
 };
+
+// Range [lo, hi] of z in [z0, z1) whose voxel centres of column (gx, gy) lie
+// inside every frustum plane, widened by one voxel; lo > hi when empty.
+__device__ __forceinline__ void clip_column(const IntegrateParams& p, float gx, float gy, int z0, int z1,
+                                            int& lo, int& hi) {
+    float zl = (float)z0, zh = (float)(z1 - 1);
+    bool empty = false;
+    const float base_z = 0.5f * p.csz - p.tz;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const float a = p.plane[k][0] * gx + p.plane[k][1] * gy + p.plane[k][2] * base_z;
+        const float b = p.plane[k][2] * p.csz;
+        if (b > 0.0f) zl = fmaxf(zl, -a * __builtin_amdgcn_rcpf(b));
+        else if (b < 0.0f) zh = fminf(zh, -a * __builtin_amdgcn_rcpf(b));
+        else if (a < 0.0f) empty = true;
+    }
+    zl = fmaxf(zl, (float)z0 - 2.0f);
+    zh = fminf(zh, (float)z1 + 2.0f);
+    lo = max(z0, (int)floorf(zl) - 1);
+    hi = min(z1 - 1, (int)ceilf(zh) + 1);
+    if (empty || !(zl <= zh + 2.0f)) { lo = z1; hi = z0 - 1; }
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
 
 // scale_depth (opencl_wrappers.cpp:37-43; kernel absent from the reference,
 // KinectFusion convention): metres along the pixel's ray,
@@ -74,16 +113,25 @@ __global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
 integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume, IntegrateParams p) {
     const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
     const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
-    if (x >= p.nx || y >= p.ny) return;
+    const bool inside = x < p.nx && y < p.ny;
     const int z_begin = blockIdx.z * kIntegrateZRun;
     const int z_end = min(z_begin + kIntegrateZRun, p.nz);
 
     const float gx = ((float)x + 0.5f) * p.csx - p.tx;
     const float gy = ((float)y + 0.5f) * p.csy - p.ty;
     const float part = gx * gx + gy * gy;
-    const uint32_t col = (uint32_t)wrap_once(x + p.ox, p.nx) + (uint32_t)p.nx * (uint32_t)wrap_once(y + p.oy, p.ny);
+    const uint32_t col = inside ? (uint32_t)wrap_once(x + p.ox, p.nx) + (uint32_t)p.nx * (uint32_t)wrap_once(y + p.oy, p.ny) : 0u;
 
-    for (int z0 = z_begin; z0 < z_end; z0 += kIntegrateBatch) {
+    // Clip the run to the frustum; the wave walks the union of its lanes'
+    // ranges so that every z step stays one contiguous 256-byte access.
+    int lo, hi;
+    clip_column(p, gx, gy, z_begin, z_end, lo, hi);
+    if (!inside) { lo = z_end; hi = z_begin - 1; }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if (!inside) return;
+
+    for (int z0 = lo; z0 <= hi; z0 += kIntegrateBatch) {
         float tsdf_new[kIntegrateBatch];
         bool upd[kIntegrateBatch];
         kfx_tsdf prev[kIntegrateBatch];
@@ -93,7 +141,7 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
         for (int k = 0; k < kIntegrateBatch; ++k) {
             const int z = z0 + k;
             bool band = false;
-            upd[k] = (z < z_end) && integrate_voxel(p, gx, gy, part, z, scaled, tsdf_new[k], band);
+            upd[k] = (z <= hi) && integrate_voxel(p, gx, gy, part, z, scaled, tsdf_new[k], band);
             addr[k] = col + p.slice * (uint32_t)wrap_once(z + p.oz, p.nz);
             if (upd[k]) prev[k] = volume[addr[k]];
         }
@@ -129,7 +177,9 @@ integrate_count_kernel(const float* __restrict__ scaled, IntegrateParams p, unsi
         const float gx = ((float)x + 0.5f) * p.csx - p.tx;
         const float gy = ((float)y + 0.5f) * p.csy - p.ty;
         const float part = gx * gx + gy * gy;
-        for (int z = z_begin; z < z_end; ++z) {
+        int lo, hi;
+        clip_column(p, gx, gy, z_begin, z_end, lo, hi);
+        for (int z = lo; z <= hi; ++z) {
             float tn;
             bool band;
             n += integrate_voxel(p, gx, gy, part, z, scaled, tn, band) ? 1u : 0u;
@@ -231,6 +281,28 @@ static int integrate_setup(const char* who, const uint16_t* depth, float* depth_
     p.nx = rb->voxels_size.x; p.ny = rb->voxels_size.y; p.nz = rb->voxels_size.z;
     p.ox = rb->origin_GRID.x; p.oy = rb->origin_GRID.y; p.oz = rb->origin_GRID.z;
     p.slice = (uint32_t)p.nx * (uint32_t)p.ny;
+    // Frustum planes: camera coordinates c = R^-1 g; a voxel can project
+    // inside the image only if c.z > 0 and -0.5 - m <= f c.xy / c.z + c0 <
+    // size - 0.5 + m.  In volume coordinates each plane normal is the
+    // corresponding combination of rows of R^-1.
+    {
+        double ri[3][3];
+        for (int r = 0; r < 3; ++r) {
+            ri[r][0] = rotation_inverse->data[r].x;
+            ri[r][1] = rotation_inverse->data[r].y;
+            ri[r][2] = rotation_inverse->data[r].z;
+        }
+        const double m = kClipMarginPx;
+        const double cl = intr->cx + 0.5 + m, cr = intr->cols - 0.5 + m - intr->cx;
+        const double ct = intr->cy + 0.5 + m, cb = intr->rows - 0.5 + m - intr->cy;
+        for (int k = 0; k < 3; ++k) {
+            p.plane[0][k] = (float)ri[2][k];
+            p.plane[1][k] = (float)(intr->fx * ri[0][k] + cl * ri[2][k]);
+            p.plane[2][k] = (float)(-intr->fx * ri[0][k] + cr * ri[2][k]);
+            p.plane[3][k] = (float)(intr->fy * ri[1][k] + ct * ri[2][k]);
+            p.plane[4][k] = (float)(-intr->fy * ri[1][k] + cb * ri[2][k]);
+        }
+    }
     return KFX_OK;
 }
 

@@ -74,6 +74,12 @@ class Pose(C.Structure):
         return np.array(self.R[:], dtype=np.float64).reshape(3, 3), np.array(self.t[:], dtype=np.float64)
 
 
+class IcpResult(C.Structure):
+    _fields_ = [("pose", Pose), ("A", C.c_double * 36), ("b", C.c_double * 6), ("x", C.c_double * 6),
+                ("status", C.c_int32), ("singular_count", C.c_int32), ("iterations", C.c_int32),
+                ("converged_tag", C.c_int32)]
+
+
 assert C.sizeof(Intrinsics) == 24 and C.sizeof(RoundBuffer) == 72 and C.sizeof(Matrix3f) == 36
 
 TSDF_DTYPE = np.dtype([("value", "<i2"), ("weight", "<i2")])
@@ -164,6 +170,23 @@ _PROTOS = {
     "kfx_integrate_count_updates": (_I, [_P, _P, C.POINTER(Intrinsics), C.POINTER(Matrix3f), C.POINTER(Point3f),
                                          C.c_float, C.POINTER(Point3f), C.POINTER(RoundBuffer),
                                          C.POINTER(C.c_uint64)]),
+    "kfx_bilateral_filter": (_I, [_P, _P, _I, _I, C.c_float, C.c_float, C.c_uint32]),
+    "kfx_pyr_down": (_I, [_P, _I, _I, _P, C.c_float]),
+    "kfx_vertex_map": (_I, [_P, _P, C.POINTER(Intrinsics)]),
+    "kfx_normal_map": (_I, [_P, _P, _I, _I]),
+    "kfx_transform_maps": (_I, [_P, _P, C.POINTER(Matrix3f), C.POINTER(Point3f), _I, _I, _P, _P]),
+    "kfx_resize_map": (_I, [_P, _I, _I, _P, _I]),
+    "kfx_estimate_combined": (_I, [C.POINTER(Matrix3f), C.POINTER(Point3f), _P, _P, C.POINTER(Matrix3f),
+                                   C.POINTER(Point3f), C.POINTER(Intrinsics), _P, _P, C.c_float, C.c_float, _P, _I,
+                                   _P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "kfx_clean_buff": (_I, [_P, _I]),
+    "kfx_icp_create": (_I, [_I, C.POINTER(_P)]),
+    "kfx_icp_destroy": (_I, [_P]),
+    "kfx_icp_reset": (_I, [_P, C.POINTER(Pose)]),
+    "kfx_icp_iterate": (_I, [_P, _P, _P, _P, _P, C.POINTER(Matrix3f), C.POINTER(Point3f), C.POINTER(Intrinsics),
+                             C.c_float, C.c_float, _I, C.c_double, C.c_double]),
+    "kfx_icp_read": (_I, [_P, C.POINTER(IcpResult)]),
+    "kfx_icp_pose_device": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "kfx_splitmix64": (C.c_uint64, [C.c_uint64]),
     "kfx_synth_make_scene": (_I, [C.c_uint64, _I, _I, C.POINTER(C.c_double), C.c_double, C.POINTER(SynthScene)]),
     "kfx_synth_orbit": (_I, [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.c_double, _I,

@@ -20,6 +20,9 @@
 extern "C" {
 #endif
 
+/* Rigid camera transform in double precision: p_out = R p_cam + t (R row-major). */
+typedef struct kfx_pose { double R[9]; double t[3]; } kfx_pose;
+
 enum kfx_status {
     KFX_OK = 0,
     KFX_ERR_INVALID_ARG = 1,
@@ -123,6 +126,96 @@ int kfx_raycast(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map
 
 
 /* ------------------------------------------------------------------------
+ * Depth preprocessing.  Images are row-major; depth is uint16 millimetres;
+ * vertex/normal maps are cols*rows kfx_xyz, NaN x = invalid.
+ * ------------------------------------------------------------------------ */
+
+/* Apply_Billateral_Filtration (common/inc/opencl_wrappers.h:72-77,
+ * common/src/opencl_wrappers.cpp:216-235, GPU/opencl/compute_bilateral_filtering.cl:8-57):
+ * 13x13 bilateral filter, output clamp(rint(sum(w d)/sum(w)), 0, 32767), zero
+ * above max_distance_mm (pass 0xFFFFFFFF for none: the reference wrapper binds
+ * the float truncation distance's bits to this argument, so it never fires). */
+int kfx_bilateral_filter(const uint16_t* src, uint16_t* dst, int cols, int rows, float sigma_space,
+                         float sigma_color, uint32_t max_distance_mm);
+/* One level of Create_Pyramid (opencl_wrappers.h:79-83, opencl_wrappers.cpp:244-271):
+ * dst is (src_cols/2) x (src_rows/2). */
+int kfx_pyr_down(const uint16_t* src, int src_cols, int src_rows, uint16_t* dst, float sigma_color);
+/* One level of Create_Vertex_Maps (opencl_wrappers.h:85-90, opencl_wrappers.cpp:281-305);
+ * intr is the level's intrinsics (intrinsics::operator(), intrinsics.cpp:7-18). */
+int kfx_vertex_map(const uint16_t* depth, kfx_xyz* vmap, const kfx_intrinsics* intr);
+/* One level of Create_Normal_Maps (opencl_wrappers.h:92-97, opencl_wrappers.cpp:315-343). */
+int kfx_normal_map(const kfx_xyz* vmap, kfx_xyz* nmap, int cols, int rows);
+/* Transform_Maps (opencl_wrappers.h:49-57, opencl_wrappers.cpp:148-179,
+ * GPU/opencl/transform_map.cl:6-65): v' = R v + t, n' = R n. */
+int kfx_transform_maps(const kfx_xyz* vmap_src, const kfx_xyz* nmap_src, const kfx_matrix_3f* rotation,
+                       const kfx_point_3f* translation, int cols, int rows, kfx_xyz* vmap_dst, kfx_xyz* nmap_dst);
+/* Resize_Vertex_Map (normalize = 0) / Resize_Normal_Map (normalize = 1)
+ * (opencl_wrappers.h:37-47, opencl_wrappers.cpp:97-146): 2x2 mean, NaN if any
+ * of the four is invalid; out is (in_cols/2) x (in_rows/2). */
+int kfx_resize_map(const kfx_xyz* in, int in_cols, int in_rows, kfx_xyz* out, int normalize);
+
+/* ------------------------------------------------------------------------
+ * ICP normal equations and pose update.
+ * ------------------------------------------------------------------------ */
+
+/* Estimate_Combined (common/inc/opencl_wrappers.h:105-122,
+ * common/src/opencl_wrappers.cpp:364-438): projective point-to-plane
+ * correspondences between the current level's maps (camera coordinates) and
+ * the previous raycast maps (volume coordinates), per-32x8-block partial sums
+ * of the 27 entries of [A | b] into gbuf (entry k at gbuf[k * gbuf_stride +
+ * block]), the 256-lane reduction of transform_estimation.cl into mbuf (27
+ * floats, device), then a blocking download and the unpacking of
+ * opencl_wrappers.cpp:426-437 into A (6x6, row-major, symmetric) and b (6).
+ * gbuf_stride >= DIVUP(cols,32)*DIVUP(rows,8) (the reference hard-codes 1200). */
+int kfx_estimate_combined(const kfx_matrix_3f* rotation_current, const kfx_point_3f* translation_current,
+                          const kfx_xyz* vmap_current, const kfx_xyz* nmap_current,
+                          const kfx_matrix_3f* rotation_previous_inverse, const kfx_point_3f* translation_previous,
+                          const kfx_intrinsics* intr, const kfx_xyz* vmap_previous, const kfx_xyz* nmap_previous,
+                          float distance_threshold, float angle_threshold, float* gbuf, int gbuf_stride,
+                          float* mbuf, double* A, double* b);
+
+/* Clean_Buff (opencl_wrappers.h:222-224, GPU/opencl/clean_buff.cl:4-11):
+ * zero gbuf[27][width]. */
+int kfx_clean_buff(float* gbuf, int width);
+
+/* Fused ICP (no reference counterpart as one call: it replaces the
+ * Estimate_Combined + host solve + pose update loop of the absent fusion
+ * application).  The handle owns the device-resident pose (double), the
+ * block partials and the last-workgroup ticket. */
+typedef struct kfx_icp kfx_icp;
+typedef struct kfx_icp_result {
+    kfx_pose pose;         /* current estimate, camera -> volume */
+    double A[36], b[6];    /* normal equations of the last solve */
+    double x[6];           /* last increment: rx, ry, rz, tx, ty, tz */
+    int32_t status;        /* KFX_OK or KFX_ERR_SINGULAR (last solve) */
+    int32_t singular_count;
+    int32_t iterations;    /* solves performed since kfx_icp_reset */
+    int32_t converged_tag; /* level tag that met the convergence test, -1 none */
+} kfx_icp_result;
+
+int kfx_icp_create(int max_pixels, kfx_icp** out);
+int kfx_icp_destroy(kfx_icp* icp);
+/* Set the estimate (asynchronous, on the library stream). */
+int kfx_icp_reset(kfx_icp* icp, const kfx_pose* pose);
+/* One iteration: FP32 Jacobian rows, FP32 compensated (TwoSum) workgroup
+ * sums, FP64 cross-workgroup sum in the last workgroup, FP64 6x6 Cholesky
+ * solve and pose composition on the GPU.  A singular system leaves the pose
+ * unchanged and records KFX_ERR_SINGULAR.  When level_tag >= 0 and an
+ * iteration's increment is below (converge_rotation rad, converge_translation
+ * m), later iterations with the same tag return immediately (early exit
+ * without a host round trip); converge_rotation <= 0 disables the test. */
+int kfx_icp_iterate(kfx_icp* icp, const kfx_xyz* vmap_current, const kfx_xyz* nmap_current,
+                    const kfx_xyz* vmap_previous, const kfx_xyz* nmap_previous,
+                    const kfx_matrix_3f* rotation_previous_inverse, const kfx_point_3f* translation_previous,
+                    const kfx_intrinsics* intr, float distance_threshold, float angle_threshold, int level_tag,
+                    double converge_rotation, double converge_translation);
+/* Blocking read of the state. */
+int kfx_icp_read(kfx_icp* icp, kfx_icp_result* out);
+/* Device addresses of the single-precision pose copy (9 + 3 floats), for
+ * kernels that consume the estimate without a host round trip. */
+int kfx_icp_pose_device(kfx_icp* icp, const void** dev_R_f32, const void** dev_t_f32);
+
+/* ------------------------------------------------------------------------
  * Synthetic input (host only, deterministic): the seeded analytic scene,
  * camera trajectories and Kinect-like depth frames every test, benchmark and
  * training run uses.  The reference reads a Kinect through OpenNI
@@ -143,8 +236,6 @@ typedef struct kfx_synth_scene {
     int32_t reserved;
     kfx_synth_object objects[KFX_SYNTH_MAX_OBJECTS];
 } kfx_synth_scene;
-/* Camera-to-world rigid transform: p_world = R p_cam + t (R row-major). */
-typedef struct kfx_pose { double R[9]; double t[3]; } kfx_pose;
 
 uint64_t kfx_splitmix64(uint64_t x);
 /* n_boxes boxes and n_spheres spheres resting on the ground, sizes and radii

@@ -62,6 +62,33 @@ void orc_raycast_f32(const kfx_tsdf* volume, kfx_xyz* vmap, kfx_xyz* nmap, const
 float orc_interpolate_trilinear_f32(const kfx_tsdf* volume, float px, float py, float pz,
                                     const kfx_point_3f* cell, const kfx_round_buffer* rb);
 
+/* compute_bilateral_filtering.cl:8-57, intended normalised output. */
+void orc_bilateral(const uint16_t* src, uint16_t* dst, int cols, int rows, float sigma_space, float sigma_color,
+                   uint32_t max_distance);
+/* compute_depth_pyramid (absent): KinectFusion pyrDown. */
+void orc_pyr_down(const uint16_t* src, int scols, int srows, uint16_t* dst, float sigma_color);
+/* compute_vertex_map (absent). */
+void orc_vertex_map(const uint16_t* depth, kfx_xyz* vmap, const kfx_intrinsics* intr);
+/* compute_normal_map_eigen (absent), double precision, Jacobi eigensolver. */
+void orc_normal_map_f64(const kfx_xyz* vmap, kfx_xyz* nmap, int cols, int rows);
+/* transform_map.cl:6-65. */
+void orc_transform_maps(const kfx_xyz* vs, const kfx_xyz* ns, const kfx_matrix_3f* R, const kfx_point_3f* t,
+                        int cols, int rows, kfx_xyz* vd, kfx_xyz* nd);
+/* resize_vertex_map / resize_normal_map (absent). */
+void orc_resize_map(const kfx_xyz* in, int icols, int irows, kfx_xyz* out, int normalize);
+
+/* ICP sums of the 27 products (precision 0: single-precision rows as the GPU,
+ * double accumulation; 1: double throughout).  Returns the correspondences. */
+int orc_icp_sums(const kfx_xyz* vc, const kfx_xyz* nc, const kfx_xyz* vp, const kfx_xyz* np, const double* Rc,
+                 const double* tc, const double* Rprev_inv, const double* tprev, const kfx_intrinsics* in,
+                 double dthr, double athr, int precision, double out[27]);
+/* Reference reduction path (estimate_combined + transform_estimation.cl), bit for bit. */
+void orc_estimate_combined_f32(const kfx_xyz* vc, const kfx_xyz* nc, const kfx_xyz* vp, const kfx_xyz* np,
+                               const float* Rc, const float* tc, const float* Rpi, const float* tp,
+                               const kfx_intrinsics* in, float dthr, float athr, float mbuf[27]);
+/* Cholesky solve + KinectFusion pose increment in double; 1 if singular. */
+int orc_icp_solve_update(const double sums[27], double* R, double* t, double x_out[6]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,15 @@ _PROTOS = {
     "orc_integrate_f64": (C.c_uint64, [_P, _P, IN, PD, PD, C.c_double, PD, RB]),
     "orc_raycast_f32": (None, [_P, _P, _P, RB, P3, M3, C.c_float, IN, P3]),
     "orc_interpolate_trilinear_f32": (C.c_float, [_P, C.c_float, C.c_float, C.c_float, P3, RB]),
+    "orc_bilateral": (None, [_P, _P, _I, _I, C.c_float, C.c_float, C.c_uint32]),
+    "orc_pyr_down": (None, [_P, _I, _I, _P, C.c_float]),
+    "orc_vertex_map": (None, [_P, _P, IN]),
+    "orc_normal_map_f64": (None, [_P, _P, _I, _I]),
+    "orc_transform_maps": (None, [_P, _P, M3, P3, _I, _I, _P, _P]),
+    "orc_resize_map": (None, [_P, _I, _I, _P, _I]),
+    "orc_icp_sums": (_I, [_P, _P, _P, _P, PD, PD, PD, PD, IN, C.c_double, C.c_double, _I, PD]),
+    "orc_estimate_combined_f32": (None, [_P, _P, _P, _P, _P, _P, _P, _P, IN, C.c_float, C.c_float, _P]),
+    "orc_icp_solve_update": (_I, [PD, PD, PD, PD]),
 }
 
 
@@ -107,3 +116,95 @@ def raycast_f32(vol, rb, t, R, trunc, intr):
     lib.orc_raycast_f32(ptr(vol), ptr(vmap), ptr(nmap), C.byref(rb), C.byref(kfx.point3f(t)),
                         C.byref(kfx.matrix3f(R)), float(trunc), C.byref(intr), C.byref(cs))
     return vmap, nmap
+
+
+NO_MAX_DISTANCE = 0xFFFFFFFF
+
+
+def bilateral(depth, sigma_space=4.5, sigma_color=30.0, max_distance=NO_MAX_DISTANCE):
+    depth = np.ascontiguousarray(depth, np.uint16)
+    out = np.zeros_like(depth)
+    lib.orc_bilateral(ptr(depth), ptr(out), depth.shape[1], depth.shape[0], sigma_space, sigma_color, max_distance)
+    return out
+
+
+def pyr_down(depth, sigma_color=30.0):
+    depth = np.ascontiguousarray(depth, np.uint16)
+    out = np.zeros((depth.shape[0] // 2, depth.shape[1] // 2), np.uint16)
+    lib.orc_pyr_down(ptr(depth), depth.shape[1], depth.shape[0], ptr(out), sigma_color)
+    return out
+
+
+def vertex_map(depth, intr):
+    depth = np.ascontiguousarray(depth, np.uint16)
+    out = np.zeros(depth.shape + (3,), np.float32)
+    lib.orc_vertex_map(ptr(depth), ptr(out), C.byref(intr))
+    return out
+
+
+def normal_map_f64(vmap):
+    vmap = np.ascontiguousarray(vmap, np.float32)
+    out = np.zeros_like(vmap)
+    lib.orc_normal_map_f64(ptr(vmap), ptr(out), vmap.shape[1], vmap.shape[0])
+    return out
+
+
+def transform_maps(vmap, nmap, R, t):
+    vmap = np.ascontiguousarray(vmap, np.float32)
+    nmap = np.ascontiguousarray(nmap, np.float32)
+    vd, nd = np.zeros_like(vmap), np.zeros_like(nmap)
+    lib.orc_transform_maps(ptr(vmap), ptr(nmap), C.byref(kfx.matrix3f(R)), C.byref(kfx.point3f(t)),
+                           vmap.shape[1], vmap.shape[0], ptr(vd), ptr(nd))
+    return vd, nd
+
+
+def resize_map(m, normalize):
+    m = np.ascontiguousarray(m, np.float32)
+    out = np.zeros((m.shape[0] // 2, m.shape[1] // 2, 3), np.float32)
+    lib.orc_resize_map(ptr(m), m.shape[1], m.shape[0], ptr(out), int(normalize))
+    return out
+
+
+def _d(a, n):
+    a = np.ascontiguousarray(np.asarray(a, np.float64).reshape(n))
+    return a, a.ctypes.data_as(PD)
+
+
+def icp_sums(vc, nc, vp, np_, Rc, tc, Rprev_inv, tprev, intr, dthr, athr, precision=0):
+    keep = [_d(Rc, 9), _d(tc, 3), _d(Rprev_inv, 9), _d(tprev, 3)]
+    out = np.zeros(27, np.float64)
+    n = lib.orc_icp_sums(ptr(vc), ptr(nc), ptr(vp), ptr(np_), keep[0][1], keep[1][1], keep[2][1], keep[3][1],
+                         C.byref(intr), float(dthr), float(athr), int(precision), out.ctypes.data_as(PD))
+    return out, n
+
+
+def estimate_combined_f32(vc, nc, vp, np_, Rc, tc, Rprev_inv, tprev, intr, dthr, athr):
+    f = [np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1)) for a in (Rc, tc, Rprev_inv, tprev)]
+    out = np.zeros(27, np.float32)
+    lib.orc_estimate_combined_f32(ptr(vc), ptr(nc), ptr(vp), ptr(np_), ptr(f[0]), ptr(f[1]), ptr(f[2]), ptr(f[3]),
+                                  C.byref(intr), float(dthr), float(athr), ptr(out))
+    return out
+
+
+def icp_solve_update(sums, R, t):
+    s, sp = _d(sums, 27)
+    R = np.array(R, np.float64).reshape(9)
+    t = np.array(t, np.float64).reshape(3)
+    x = np.zeros(6, np.float64)
+    rc = lib.orc_icp_solve_update(sp, R.ctypes.data_as(PD), t.ctypes.data_as(PD), x.ctypes.data_as(PD))
+    return rc, R.reshape(3, 3), t, x
+
+
+def unpack_sums(s):
+    """27 sums -> (A, b) as opencl_wrappers.cpp:426-437."""
+    A = np.zeros((6, 6))
+    b = np.zeros(6)
+    k = 0
+    for i in range(6):
+        for j in range(i, 7):
+            if j == 6:
+                b[i] = s[k]
+            else:
+                A[i, j] = A[j, i] = s[k]
+            k += 1
+    return A, b

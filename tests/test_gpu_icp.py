@@ -1,0 +1,131 @@
+"""GPU parity of the ICP paths against the oracle: the reference-style
+Estimate_Combined reduction bit for bit; the fused FP32-compensated / FP64
+path's normal equations, solve and pose update against double precision; a
+coarse-to-fine run within 1e-5 rad / 1e-5 m of the oracle's FP64 ICP; the
+singular-system and early-exit behaviour."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import icp_inputs as ii
+
+pytestmark = pytest.mark.gpu
+
+
+def _upload(kfx, maps):
+    return [kfx.Buffer.from_array(np.ascontiguousarray(m, np.float32)) for m in maps]
+
+
+@pytest.fixture(scope="module")
+def pair(gpu, orc):
+    intr = gpu.kinect_intrinsics()
+    maps, prev, cur = ii.icp_pair(orc, intr, noise=True)
+    return intr, maps, prev, cur
+
+
+def test_estimate_combined_bit_exact(gpu, orc, pair):
+    kfx = gpu
+    intr, maps, (Rp, tp), (Rc, tc) = pair
+    bufs = _upload(kfx, maps)
+    stride = ((intr.cols + 31) // 32) * ((intr.rows + 7) // 8)
+    gbuf = kfx.Buffer(27 * stride * 4)
+    mbuf = kfx.Buffer(27 * 4)
+    A = (C.c_double * 36)()
+    b = (C.c_double * 6)()
+    kfx.call("kfx_clean_buff", gbuf.ptr, stride)
+    kfx.call("kfx_estimate_combined", C.byref(kfx.matrix3f(Rp)), C.byref(kfx.point3f(tp)), bufs[0].ptr, bufs[1].ptr,
+             C.byref(kfx.matrix3f(Rp.T)), C.byref(kfx.point3f(tp)), C.byref(intr), bufs[2].ptr, bufs[3].ptr,
+             ii.DIST_THR, ii.ANGLE_THR, gbuf.ptr, stride, mbuf.ptr, A, b)
+    got = mbuf.download(np.float32, 27)
+    want = orc.estimate_combined_f32(*maps, Rp, tp, Rp.T, tp, intr, ii.DIST_THR, ii.ANGLE_THR)
+    assert np.array_equal(got, want)
+    Aw, bw = orc.unpack_sums(want.astype(np.float64))
+    assert np.array_equal(np.array(A[:]).reshape(6, 6), Aw) and np.array_equal(np.array(b[:]), bw)
+
+
+def _icp_handle(kfx, npix):
+    h = C.c_void_p()
+    kfx.call("kfx_icp_create", npix, C.byref(h))
+    return h
+
+
+def _pose(kfx, R, t):
+    p = kfx.Pose()
+    p.R[:] = [float(v) for v in np.asarray(R).reshape(9)]
+    p.t[:] = [float(v) for v in np.asarray(t).reshape(3)]
+    return p
+
+
+def test_fused_iteration_matches_double(gpu, orc, pair):
+    kfx = gpu
+    intr, maps, (Rp, tp), (Rc, tc) = pair
+    bufs = _upload(kfx, maps)
+    h = _icp_handle(kfx, intr.cols * intr.rows)
+    kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+    kfx.call("kfx_icp_iterate", h, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, C.byref(kfx.matrix3f(Rp.T)),
+             C.byref(kfx.point3f(tp)), C.byref(intr), ii.DIST_THR, ii.ANGLE_THR, -1, 0.0, 0.0)
+    res = kfx.IcpResult()
+    kfx.call("kfx_icp_read", h, C.byref(res))
+    s, n = orc.icp_sums(*maps, Rp, tp, Rp.T, tp, intr, ii.DIST_THR, ii.ANGLE_THR, precision=0)
+    A, b = orc.unpack_sums(s)
+    np.testing.assert_allclose(np.array(res.A[:]).reshape(6, 6), A, rtol=1e-9, atol=1e-9 * np.abs(A).max())
+    np.testing.assert_allclose(np.array(res.b[:]), b, rtol=1e-9, atol=1e-9 * np.abs(b).max())
+    rc, R, t, x = orc.icp_solve_update(s, Rp, tp)
+    assert res.status == 0 and rc == 0
+    np.testing.assert_allclose(np.array(res.x[:]), x, atol=1e-9)
+    np.testing.assert_allclose(np.array(res.pose.R[:]).reshape(3, 3), R, atol=1e-9)
+    np.testing.assert_allclose(np.array(res.pose.t[:]), t, atol=1e-9)
+    kfx.call("kfx_icp_destroy", h)
+
+
+def test_fused_coarse_to_fine_matches_oracle_pose(gpu, orc, pair):
+    kfx = gpu
+    intr, maps, (Rp, tp), (Rc, tc) = pair
+    pyr = ii.pyramid_maps(orc, maps)
+    dev = [_upload(kfx, m) for m in pyr]
+    h = _icp_handle(kfx, intr.cols * intr.rows)
+    kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+    for level in (2, 1, 0):
+        li = intr.level(level)
+        b = dev[level]
+        for _ in range((10, 5, 4)[level]):
+            kfx.call("kfx_icp_iterate", h, b[0].ptr, b[1].ptr, b[2].ptr, b[3].ptr, C.byref(kfx.matrix3f(Rp.T)),
+                     C.byref(kfx.point3f(tp)), C.byref(li), ii.DIST_THR, ii.ANGLE_THR, -1, 0.0, 0.0)
+    res = kfx.IcpResult()
+    kfx.call("kfx_icp_read", h, C.byref(res))
+    assert res.iterations == 19 and res.singular_count == 0
+    R, t = ii.oracle_icp(orc, pyr, intr, Rp, tp, Rp, tp, precision=1)
+    Rg = np.array(res.pose.R[:]).reshape(3, 3)
+    tg = np.array(res.pose.t[:])
+    from scipy.spatial.transform import Rotation
+    assert np.linalg.norm(Rotation.from_matrix(Rg @ R.T).as_rotvec()) < 1e-5
+    assert np.linalg.norm(tg - t) < 1e-5
+    # and it tracked the true motion
+    assert np.linalg.norm(tg - tc) < 2e-3
+    kfx.call("kfx_icp_destroy", h)
+
+
+def test_fused_singular_keeps_pose_and_early_exit(gpu, orc, pair):
+    kfx = gpu
+    intr, maps, (Rp, tp), (Rc, tc) = pair
+    bufs = _upload(kfx, maps)
+    nan = kfx.Buffer.from_array(np.full(maps[0].shape, np.nan, np.float32))
+    h = _icp_handle(kfx, intr.cols * intr.rows)
+    kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+    for _ in range(3):  # an all-invalid frame: no correspondences
+        kfx.call("kfx_icp_iterate", h, nan.ptr, nan.ptr, bufs[2].ptr, bufs[3].ptr, C.byref(kfx.matrix3f(Rp.T)),
+                 C.byref(kfx.point3f(tp)), C.byref(intr), ii.DIST_THR, ii.ANGLE_THR, -1, 0.0, 0.0)
+    res = kfx.IcpResult()
+    kfx.call("kfx_icp_read", h, C.byref(res))
+    assert res.status == 5 and res.singular_count == 3
+    assert np.array_equal(np.array(res.pose.t[:]), tp) and np.array_equal(np.array(res.pose.R[:]), Rp.reshape(9))
+    # Early exit: a loose convergence test stops the level after one solve.
+    kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+    for _ in range(6):
+        kfx.call("kfx_icp_iterate", h, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr,
+                 C.byref(kfx.matrix3f(Rp.T)), C.byref(kfx.point3f(tp)), C.byref(intr), ii.DIST_THR, ii.ANGLE_THR,
+                 7, 1.0, 1.0)
+    kfx.call("kfx_icp_read", h, C.byref(res))
+    assert res.iterations == 1 and res.converged_tag == 7
+    kfx.call("kfx_icp_destroy", h)

@@ -1,0 +1,106 @@
+"""GPU parity of the preprocessing kernels against the oracle on C1 frames:
+bilateral (within 1 mm, valid mask bit-exact), pyramid, vertex maps,
+transform and resize (bit-exact), normal maps (mask bit-exact, direction
+against the double-precision oracle)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1_frame(gpu):
+    kfx = gpu
+    intr = kfx.kinect_intrinsics()
+    pose = scenes.c1_poses(30)[12]
+    return intr, kfx.render(scenes.c1_scene(), pose, intr, scenes.C1_SEED, 12, noise=True)
+
+
+def test_bilateral_matches_oracle(gpu, orc, c1_frame):
+    kfx = gpu
+    intr, d = c1_frame
+    src = kfx.Buffer.from_array(d)
+    dst = kfx.Buffer(d.nbytes)
+    kfx.call("kfx_bilateral_filter", src.ptr, dst.ptr, intr.cols, intr.rows, 4.5, 30.0, 0xFFFFFFFF)
+    got = dst.download(np.uint16, d.shape)
+    want = orc.bilateral(d)
+    assert np.array_equal(got > 0, want > 0)
+    assert np.array_equal(got > 0, d > 0)
+    diff = np.abs(got.astype(int) - want)
+    assert diff.max() <= 1
+    assert np.mean(diff == 0) > 0.999
+
+
+def test_pyramid_vertex_maps_bit_exact(gpu, orc, c1_frame):
+    kfx = gpu
+    intr, d = c1_frame
+    lvl = [d]
+    for i in range(1, 3):
+        lvl.append(orc.pyr_down(lvl[-1]))
+    src = kfx.Buffer.from_array(d)
+    cur = src
+    for i in range(1, 3):
+        h, w = lvl[i - 1].shape
+        nxt = kfx.Buffer(lvl[i].nbytes)
+        kfx.call("kfx_pyr_down", cur.ptr, w, h, nxt.ptr, 30.0)
+        got = nxt.download(np.uint16, lvl[i].shape)
+        assert np.array_equal(got, lvl[i]), f"pyramid level {i}"
+        cur = nxt
+    for i in range(3):
+        li = intr.level(i)
+        db = kfx.Buffer.from_array(lvl[i])
+        vb = kfx.Buffer(lvl[i].size * 12)
+        kfx.call("kfx_vertex_map", db.ptr, vb.ptr, C.byref(li))
+        got = vb.download(np.float32, lvl[i].shape + (3,))
+        want = orc.vertex_map(lvl[i], li)
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        ok = ~np.isnan(want)
+        assert np.array_equal(got[ok], want[ok]), f"vertex map level {i}"
+
+
+def test_normal_map_matches_double_oracle(gpu, orc, c1_frame):
+    kfx = gpu
+    intr, d = c1_frame
+    vm = orc.vertex_map(orc.bilateral(d), intr)
+    vb = kfx.Buffer.from_array(vm)
+    nb = kfx.Buffer(vm.nbytes)
+    kfx.call("kfx_normal_map", vb.ptr, nb.ptr, intr.cols, intr.rows)
+    got = nb.download(np.float32, vm.shape)
+    want = orc.normal_map_f64(vm)
+    assert np.array_equal(np.isnan(got[..., 0]), np.isnan(want[..., 0]))
+    ok = ~np.isnan(want[..., 0])
+    assert ok.mean() > 0.5
+    cosang = np.sum(got[ok] * want[ok], axis=-1)
+    assert np.all(cosang > 0.99), np.min(cosang)
+    assert np.mean(cosang > 1 - 1e-5) > 0.999  # within 4.5 mrad on 99.9 %
+
+
+def test_transform_and_resize_bit_exact(gpu, orc, c1_frame):
+    kfx = gpu
+    intr, d = c1_frame
+    vm = orc.vertex_map(d, intr)
+    nm = orc.normal_map_f64(vm)
+    R = scenes.c1_poses(30)[5].matrices()[0]
+    t = np.array([0.3, -0.1, 1.2])
+    vb, nb = kfx.Buffer.from_array(vm), kfx.Buffer.from_array(nm)
+    vo, no = kfx.Buffer(vm.nbytes), kfx.Buffer(vm.nbytes)
+    kfx.call("kfx_transform_maps", vb.ptr, nb.ptr, C.byref(kfx.matrix3f(R)), C.byref(kfx.point3f(t)),
+             intr.cols, intr.rows, vo.ptr, no.ptr)
+    wv, wn = orc.transform_maps(vm, nm, R, t)
+    for buf, want in ((vo, wv), (no, wn)):
+        got = buf.download(np.float32, vm.shape)
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        ok = ~np.isnan(want)
+        assert np.array_equal(got[ok], want[ok])
+    for normalize, src, m in ((0, vb, vm), (1, nb, nm)):
+        out = kfx.Buffer(vm.nbytes // 4)
+        kfx.call("kfx_resize_map", src.ptr, intr.cols, intr.rows, out.ptr, normalize)
+        got = out.download(np.float32, (intr.rows // 2, intr.cols // 2, 3))
+        want = orc.resize_map(m, normalize)
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        ok = ~np.isnan(want)
+        assert np.array_equal(got[ok], want[ok])

@@ -115,3 +115,40 @@ def test_raycast_matches_oracle_c1(gpu, orc, origin):
         nh = ~np.isnan(rn[..., 0])
         assert np.max(np.abs(nm[nh] - rn[nh])) <= 1e-4
         assert np.array_equal(vm[hit], rv[hit]), "raycast vertices not bit-identical"
+
+
+def _random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_integrate_frustum_clip_random_poses(gpu, orc, seed):
+    # Arbitrary camera orientations and positions (inside and outside the
+    # cube) exercise every sign of the frustum-plane clip; the result must
+    # equal the oracle's full sweep bit for bit.
+    kfx = gpu
+    rng = np.random.default_rng(100 + seed)
+    intr = kfx.kinect_intrinsics(160, 120, 131.25)
+    rb = kfx.round_buffer(96, 1.92, origin=tuple(int(v) for v in rng.integers(0, 96, 3)))
+    R = _random_rotation(rng)
+    t = rng.uniform(-0.5, 2.4, size=3)
+    depth = rng.integers(300, 3000, size=(120, 160)).astype(np.uint16)
+    depth[rng.random(depth.shape) < 0.05] = 0
+    ref = scenes.empty_volume(rb)
+    n_ref = orc.integrate_f32(ref, depth, intr, R.T, t, scenes.TRUNC, rb)
+    dv = kfx.Buffer(rb.buffer_length * 4)
+    kfx.call("kfx_init_tsdf", dv.ptr, C.byref(rb))
+    ds = _integrate_gpu(kfx, dv, depth, intr, R.T, t, scenes.TRUNC, rb)
+    got = dv.download(kfx.TSDF_DTYPE, rb.buffer_length)
+    assert np.array_equal(got, ref)
+    dd = kfx.Buffer.from_array(depth)
+    n = C.c_uint64()
+    cs = kfx.cell_size(rb)
+    kfx.call("kfx_integrate_count_updates", dd.ptr, ds.ptr, C.byref(intr), C.byref(kfx.matrix3f(R.T)),
+             C.byref(kfx.point3f(t)), scenes.TRUNC, C.byref(cs), C.byref(rb), C.byref(n))
+    assert n.value == n_ref
