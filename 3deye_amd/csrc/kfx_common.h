@@ -53,6 +53,12 @@ struct ProfScope {
 
 namespace kfx {
 
+// Single-precision pose passed by value to kernels.
+struct kfx_pose_f {
+    float R[9];
+    float t[3];
+};
+
 // ---------------------------------------------------------------------------
 // Cyclic volume addressing (reference: GPU/opencl/raycast.cl:29-47).
 // Per-axis (coordinate + origin) modulo the axis size, linearised x-fastest,

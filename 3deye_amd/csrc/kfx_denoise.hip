@@ -1,0 +1,844 @@
+// kfx_denoise.hip — learned residual depth denoiser (no reference
+// counterpart): forward, backward and Adam, all hand-written for gfx950.
+//
+// Network: input x0 = depth in metres (0 where invalid), x1 = valid mask;
+// four 3x3 convolutions with zero padding, 2 -> 16 -> 16 -> 16 -> 1
+// channels, ReLU after the first three; output y = x0 + conv4 on valid pixels.
+// 5,089 parameters, flat fp32 (per layer weights [co][ci][ky][kx], then
+// biases [co]).  Weights are used in bfloat16, accumulation is fp32.
+//
+// Layout: hidden activations and their gradients are bfloat16 NHWC with 16
+// channels (32 bytes per pixel).  The 16 -> 16 layers are implicit GEMMs on
+// the matrix cores (v_mfma_f32_16x16x16_bf16): one wave computes 16 output
+// channels x 16 consecutive pixels, one MFMA per 3x3 tap; the B operand of a
+// lane is 4 channels of one pixel (one 8-byte load), the A operand 4 input
+// channels of one output channel's tap weights.  Weight gradients are GEMMs
+// over pixels (K = pixels) accumulated per wave, then reduced in a fixed
+// order.  The 2 -> 16 and 16 -> 1 layers (18 and 144 MACs per output) run on
+// the vector ALUs.
+#include <math.h>
+
+#include <algorithm>
+
+#include "kfx_common.h"
+
+namespace kfx {
+
+constexpr int kDnC = 16;
+constexpr int kDnParams = 5089;
+constexpr int kOffW1 = 0, kOffB1 = 288;
+constexpr int kOffW2 = 304, kOffB2 = 2608;
+constexpr int kOffW3 = 2624, kOffB3 = 4928;
+constexpr int kOffW4 = 4944, kOffB4 = 5088;
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct DnPacked {
+    float w1[16 * 2 * 9];        // [co][ci][tap], bf16-rounded values
+    float b1[16];
+    uint16_t w2[9][16][16];      // [tap][co][ci] bf16 bits (forward A operand)
+    uint16_t w2t[9][16][16];     // [tap][ci][co] (input-gradient A operand)
+    float b2[16];
+    uint16_t w3[9][16][16];
+    uint16_t w3t[9][16][16];
+    float b3[16];
+    float w4[16 * 9];            // [ci][tap], bf16-rounded values
+    float b4;
+};
+
+__host__ __device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t b = __float_as_uint(f);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+__host__ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__global__ void dn_pack_kernel(const float* __restrict__ m, DnPacked* __restrict__ pk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 288) pk->w1[i] = bf2f(f2bf(m[kOffW1 + i]));
+    if (i < 16) {
+        pk->b1[i] = m[kOffB1 + i];
+        pk->b2[i] = m[kOffB2 + i];
+        pk->b3[i] = m[kOffB3 + i];
+    }
+    if (i < 144) pk->w4[i] = bf2f(f2bf(m[kOffW4 + i]));
+    if (i == 0) pk->b4 = m[kOffB4];
+    if (i < 2304) {
+        // i = ((co * 16 + ci) * 9 + tap)
+        const int tap = i % 9, ci = (i / 9) % 16, co = i / 144;
+        pk->w2[tap][co][ci] = f2bf(m[kOffW2 + i]);
+        pk->w2t[tap][ci][co] = f2bf(m[kOffW2 + i]);
+        pk->w3[tap][co][ci] = f2bf(m[kOffW3 + i]);
+        pk->w3t[tap][ci][co] = f2bf(m[kOffW3 + i]);
+    }
+}
+
+__device__ __forceinline__ float depth_m(uint16_t d) { return (float)d * 0.001f; }
+
+// Layer 1 (2 -> 16), one lane per pixel.
+__global__ void __launch_bounds__(256) dn_l1_fwd(const uint16_t* __restrict__ depth, int W, int H,
+                                                 const DnPacked* __restrict__ pk, uint16_t* __restrict__ a1) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    float in[2][9];
+    for (int ky = 0; ky < 3; ++ky)
+        for (int kx = 0; kx < 3; ++kx) {
+            const int qx = x + kx - 1, qy = y + ky - 1;
+            uint16_t d = 0;
+            if (qx >= 0 && qy >= 0 && qx < W && qy < H) d = depth[qy * W + qx];
+            in[0][ky * 3 + kx] = d ? depth_m(d) : 0.0f;
+            in[1][ky * 3 + kx] = d ? 1.0f : 0.0f;
+        }
+    uint32_t packed[8];
+    for (int co = 0; co < 16; co += 2) {
+        float z[2];
+        for (int h = 0; h < 2; ++h) {
+            float s = pk->b1[co + h];
+            for (int ci = 0; ci < 2; ++ci)
+                for (int t = 0; t < 9; ++t) s += pk->w1[((co + h) * 2 + ci) * 9 + t] * in[ci][t];
+            z[h] = s > 0.0f ? s : 0.0f;
+        }
+        packed[co / 2] = (uint32_t)f2bf(z[0]) | ((uint32_t)f2bf(z[1]) << 16);
+    }
+    uint4* dst = reinterpret_cast<uint4*>(a1 + (size_t)(y * W + x) * kDnC);
+    dst[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+    dst[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
+}
+
+// 4 bf16 channels [c0, c0+4) of pixel (qx, qy) as an MFMA operand; zero
+// outside the image.
+__device__ __forceinline__ v4s load4(const uint16_t* __restrict__ act, int W, int H, int qx, int qy, int c0) {
+    v4s r = {0, 0, 0, 0};
+    if (qx >= 0 && qy >= 0 && qx < W && qy < H) {
+        const uint2 u = *reinterpret_cast<const uint2*>(act + (size_t)(qy * W + qx) * kDnC + c0);
+        r[0] = (short)(u.x & 0xffffu);
+        r[1] = (short)(u.x >> 16);
+        r[2] = (short)(u.y & 0xffffu);
+        r[3] = (short)(u.y >> 16);
+    }
+    return r;
+}
+
+__device__ __forceinline__ v4s wload4(const uint16_t* w) {
+    const uint2 u = *reinterpret_cast<const uint2*>(w);
+    v4s r;
+    r[0] = (short)(u.x & 0xffffu);
+    r[1] = (short)(u.x >> 16);
+    r[2] = (short)(u.y & 0xffffu);
+    r[3] = (short)(u.y >> 16);
+    return r;
+}
+
+// Hidden layer (16 -> 16), forward: out[p][co] = relu(b[co] + sum_tap
+// sum_ci W[tap][co][ci] in[p + off(tap)][ci]).  Wave = 16 pixels of a row.
+// D[m = co][n = pixel]: lane l holds channels 4*(l/16)..+3 of pixel l%16.
+__global__ void __launch_bounds__(256) dn_mid_fwd(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int W,
+                                                  int H, const uint16_t* __restrict__ w, const float* __restrict__ b) {
+    const int lane = threadIdx.x & 63;
+    const int x0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    const int y = blockIdx.y;
+    if (x0 >= W) return;
+    const int n = lane & 15, kq = (lane >> 4) * 4;
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int ky = tap / 3, kx = tap % 3;
+        const v4s a = wload4(w + (tap * 16 + n) * 16 + kq);          // A[m = co = n][k = ci]
+        const v4s bb = load4(in, W, H, x0 + n + kx - 1, y + ky - 1, kq);  // B[k = ci][n = pixel]
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc, 0, 0, 0);
+    }
+    const int px = x0 + n;
+    if (px >= W) return;
+    uint32_t lo = 0, hi = 0;
+    float v[4];
+    for (int i = 0; i < 4; ++i) {
+        const float z = acc[i] + b[kq + i];
+        v[i] = z > 0.0f ? z : 0.0f;
+    }
+    const float v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
+    lo = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+    hi = (uint32_t)f2bf(v2) | ((uint32_t)f2bf(v3) << 16);
+    *reinterpret_cast<uint2*>(out + (size_t)(y * W + px) * kDnC + kq) = make_uint2(lo, hi);
+}
+
+// Layer 4 (16 -> 1) and the residual: y = x0 + b4 + sum w4 a3 on valid
+// pixels.  Writes y (metres, optional) and the depth in mm (optional).
+__global__ void __launch_bounds__(256) dn_l4_fwd(const uint16_t* __restrict__ depth, const uint16_t* __restrict__ a3,
+                                                 int W, int H, const DnPacked* __restrict__ pk,
+                                                 float* __restrict__ ypred, uint16_t* __restrict__ out) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int yy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || yy >= H) return;
+    const int p = yy * W + x;
+    const uint16_t d = depth[p];
+    float s = pk->b4;
+    for (int ky = 0; ky < 3; ++ky)
+        for (int kx = 0; kx < 3; ++kx) {
+            const int qx = x + kx - 1, qy = yy + ky - 1;
+            if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+            const uint4* src = reinterpret_cast<const uint4*>(a3 + (size_t)(qy * W + qx) * kDnC);
+            const uint4 u0 = src[0], u1 = src[1];
+            const uint32_t words[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+            const int t = ky * 3 + kx;
+            for (int c = 0; c < 16; ++c) {
+                const uint16_t h = (uint16_t)((words[c >> 1] >> ((c & 1) * 16)) & 0xffffu);
+                s += pk->w4[c * 9 + t] * bf2f(h);
+            }
+        }
+    const float y = d ? depth_m(d) + s : 0.0f;
+    if (ypred) ypred[p] = y;
+    if (out) {
+        int mm = 0;
+        if (d) mm = (int)fminf(fmaxf(rintf(y * 1000.0f), 1.0f), 65535.0f);
+        out[p] = (uint16_t)mm;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Training.
+// ---------------------------------------------------------------------------
+
+// Loss gradient of the output: dz4[p] = 2 (y - t) on pixels with a valid
+// input, a valid target and |x0 - t| < outlier; per-block sums of squared
+// error and counts (reduced by dn_loss_reduce).
+__global__ void __launch_bounds__(256) dn_loss_kernel(const uint16_t* __restrict__ depth,
+                                                      const float* __restrict__ ypred,
+                                                      const float* __restrict__ target, int P, float outlier,
+                                                      float* __restrict__ dz4, double* __restrict__ blk_sse,
+                                                      unsigned* __restrict__ blk_cnt) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    double e2 = 0.0;
+    unsigned c = 0;
+    if (p < P) {
+        const uint16_t d = depth[p];
+        const float t = target[p];
+        float g = 0.0f;
+        if (d && t > 0.0f && fabsf(depth_m(d) - t) < outlier) {
+            const float e = ypred[p] - t;
+            g = 2.0f * e;
+            e2 = (double)e * e;
+            c = 1;
+        }
+        dz4[p] = g;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        e2 += __shfl_xor(e2, off);
+        c += __shfl_xor(c, off);
+    }
+    __shared__ double se[4];
+    __shared__ unsigned sc[4];
+    if ((threadIdx.x & 63) == 0) {
+        se[threadIdx.x >> 6] = e2;
+        sc[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        blk_sse[blockIdx.x] = se[0] + se[1] + se[2] + se[3];
+        blk_cnt[blockIdx.x] = sc[0] + sc[1] + sc[2] + sc[3];
+    }
+}
+
+// stats[0] = sum of squared error, stats[1] = count (as double), stats[2] =
+// gradient scale 1 / count (0 when empty).
+__global__ void __launch_bounds__(256) dn_loss_reduce(const double* __restrict__ blk_sse,
+                                                      const unsigned* __restrict__ blk_cnt, int nblk,
+                                                      double* __restrict__ stats) {
+    double s = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 256) {
+        s += blk_sse[i];
+        c += (double)blk_cnt[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        c += __shfl_xor(c, off);
+    }
+    __shared__ double ss[4], cc[4];
+    if ((threadIdx.x & 63) == 0) {
+        ss[threadIdx.x >> 6] = s;
+        cc[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double S = ss[0] + ss[1] + ss[2] + ss[3], Cn = cc[0] + cc[1] + cc[2] + cc[3];
+        stats[0] = S;
+        stats[1] = Cn;
+        stats[2] = Cn > 0.0 ? 1.0 / Cn : 0.0;
+    }
+}
+
+// Input gradient of layer 4 through the ReLU of layer 3:
+// dz3[p][ci] = (a3[p][ci] > 0) * sum_tap dz4[p - off(tap)] w4[ci][tap].
+__global__ void __launch_bounds__(256) dn_l4_bwd_input(const float* __restrict__ dz4, const uint16_t* __restrict__ a3,
+                                                       const uint16_t* __restrict__ depth, int W, int H,
+                                                       const DnPacked* __restrict__ pk, uint16_t* __restrict__ dz3) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    float g[9];
+    for (int ky = 0; ky < 3; ++ky)
+        for (int kx = 0; kx < 3; ++kx) {
+            // output pixel q = p - (kx-1, ky-1) used input p through tap (ky, kx)
+            const int qx = x - (kx - 1), qy = y - (ky - 1);
+            float v = 0.0f;
+            if (qx >= 0 && qy >= 0 && qx < W && qy < H && depth[qy * W + qx]) v = dz4[qy * W + qx];
+            g[ky * 3 + kx] = v;
+        }
+    const size_t base = (size_t)(y * W + x) * kDnC;
+    uint32_t packed[8];
+    for (int c = 0; c < 16; c += 2) {
+        float r[2];
+        for (int h = 0; h < 2; ++h) {
+            float s = 0.0f;
+            for (int t = 0; t < 9; ++t) s += g[t] * pk->w4[(c + h) * 9 + t];
+            r[h] = (a3[base + c + h] & 0x7fffu) != 0 ? s : 0.0f;
+        }
+        packed[c / 2] = (uint32_t)f2bf(r[0]) | ((uint32_t)f2bf(r[1]) << 16);
+    }
+    uint4* dst = reinterpret_cast<uint4*>(dz3 + base);
+    dst[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+    dst[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
+}
+
+// Hidden-layer input gradient through the previous ReLU (MFMA):
+// dz_in[p][ci] = (a_in[p][ci] > 0) * sum_tap sum_co W[tap][co][ci] dz[p - off(tap)][co].
+// A[m = ci][k = co] from the transposed pack, B[k = co][n = pixel].
+__global__ void __launch_bounds__(256) dn_mid_bwd_input(const uint16_t* __restrict__ dz, const uint16_t* __restrict__ a_in,
+                                                        int W, int H, const uint16_t* __restrict__ wt,
+                                                        uint16_t* __restrict__ dz_in) {
+    const int lane = threadIdx.x & 63;
+    const int x0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    const int y = blockIdx.y;
+    if (x0 >= W) return;
+    const int n = lane & 15, kq = (lane >> 4) * 4;
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int ky = tap / 3, kx = tap % 3;
+        const v4s a = wload4(wt + (tap * 16 + n) * 16 + kq);
+        const v4s bb = load4(dz, W, H, x0 + n - (kx - 1), y - (ky - 1), kq);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc, 0, 0, 0);
+    }
+    const int px = x0 + n;
+    if (px >= W) return;
+    const size_t base = (size_t)(y * W + px) * kDnC + kq;
+    const uint2 am = *reinterpret_cast<const uint2*>(a_in + base);
+    const uint16_t ah[4] = {(uint16_t)(am.x & 0xffffu), (uint16_t)(am.x >> 16), (uint16_t)(am.y & 0xffffu),
+                            (uint16_t)(am.y >> 16)};
+    uint16_t r[4];
+    for (int i = 0; i < 4; ++i) r[i] = (ah[i] & 0x7fffu) != 0 ? f2bf(acc[i]) : (uint16_t)0;
+    *reinterpret_cast<uint2*>(dz_in + base) =
+        make_uint2((uint32_t)r[0] | ((uint32_t)r[1] << 16), (uint32_t)r[2] | ((uint32_t)r[3] << 16));
+}
+
+// Hidden-layer weight gradient (MFMA, K = pixels): per wave, over the
+// 16-pixel groups [g0, g1) of row y:
+//   dW[tap][co][ci] += sum_p dz[p][co] a_in[p + off(tap)][ci],  db[co] += sum_p dz[p][co].
+// A[m = co][k = pixel], B[k = pixel][n = ci].  Writes the wave's partial
+// (9*256 weights in [co][ci][tap] order, then 16 biases) to part[wave].
+constexpr int kMidGroupsPerWave = 10;
+__global__ void __launch_bounds__(64) dn_mid_wgrad(const uint16_t* __restrict__ dz, const uint16_t* __restrict__ a_in,
+                                                   int W, int H, float* __restrict__ part) {
+    const int lane = threadIdx.x;
+    const int groups = (W + 15) / 16;
+    const int seg_per_row = (groups + kMidGroupsPerWave - 1) / kMidGroupsPerWave;
+    const int wave = blockIdx.x;
+    const int y = wave / seg_per_row, seg = wave % seg_per_row;
+    float* out = part + (size_t)wave * 2320;
+    if (y >= H) return;
+    const int g0 = seg * kMidGroupsPerWave, g1 = min(groups, g0 + kMidGroupsPerWave);
+    const int m = lane & 15, kq = (lane >> 4) * 4;
+    v4f acc[9];
+    for (int t = 0; t < 9; ++t) acc[t] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
+    for (int g = g0; g < g1; ++g) {
+        const int xb = g * 16;
+        // A: dz[pixel xb + kq + j][co = m], j = 0..3
+        v4s a;
+        for (int j = 0; j < 4; ++j) {
+            const int px = xb + kq + j;
+            const uint16_t h = px < W ? dz[(size_t)(y * W + px) * kDnC + m] : (uint16_t)0;
+            a[j] = (short)h;
+            bsum += bf2f(h);
+        }
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+            v4s bb;
+            for (int j = 0; j < 4; ++j) {
+                const int px = xb + kq + j, qx = px + kx - 1, qy = y + ky - 1;
+                uint16_t h = 0;
+                if (px < W && qx >= 0 && qy >= 0 && qx < W && qy < H) h = a_in[(size_t)(qy * W + qx) * kDnC + m];
+                bb[j] = (short)h;
+            }
+            acc[tap] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[tap], 0, 0, 0);
+        }
+    }
+    // D[co = kq + i][ci = m] per tap
+    for (int tap = 0; tap < 9; ++tap)
+        for (int i = 0; i < 4; ++i) out[((kq + i) * 16 + m) * 9 + tap] = acc[tap][i];
+    // bias: lanes with the same m hold partial sums over disjoint pixels
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+    if (lane < 16) out[2304 + lane] = bsum;
+}
+
+// Small-layer weight gradients on the vector ALUs, one wave per row segment:
+// layer 4: dW4[ci][tap] = sum_p dz4[p] a3[p + off][ci], db4 = sum dz4 (145);
+// layer 1: dW1[co][ci][tap] = sum_p dz1[p][co] x[p + off][ci], db1[co] (304),
+// split in two halves of 8 output channels (grid.y).
+constexpr int kSmallPixelsPerWave = 320;
+__global__ void __launch_bounds__(64) dn_l4_wgrad(const float* __restrict__ dz4, const uint16_t* __restrict__ a3,
+                                                  const uint16_t* __restrict__ depth, int W, int H,
+                                                  float* __restrict__ part) {
+    const int lane = threadIdx.x;
+    const int seg_per_row = (W + kSmallPixelsPerWave - 1) / kSmallPixelsPerWave;
+    const int y = blockIdx.x / seg_per_row, seg = blockIdx.x % seg_per_row;
+    float* out = part + (size_t)blockIdx.x * 145;
+    if (y >= H) return;
+    float acc[145];
+    for (int i = 0; i < 145; ++i) acc[i] = 0.0f;
+    for (int x = seg * kSmallPixelsPerWave + lane; x < min(W, (seg + 1) * kSmallPixelsPerWave); x += 64) {
+        const int p = y * W + x;
+        const float g = depth[p] ? dz4[p] : 0.0f;
+        if (g == 0.0f) continue;
+        acc[144] += g;
+        for (int ky = 0; ky < 3; ++ky)
+            for (int kx = 0; kx < 3; ++kx) {
+                const int qx = x + kx - 1, qy = y + ky - 1;
+                if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+                const uint16_t* src = a3 + (size_t)(qy * W + qx) * kDnC;
+                for (int c = 0; c < 16; ++c) acc[c * 9 + ky * 3 + kx] += g * bf2f(src[c]);
+            }
+    }
+    for (int i = 0; i < 145; ++i) {
+        float v = acc[i];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) out[i] = v;
+    }
+}
+
+__global__ void __launch_bounds__(64) dn_l1_wgrad(const uint16_t* __restrict__ dz1, const uint16_t* __restrict__ depth,
+                                                  int W, int H, float* __restrict__ part) {
+    const int lane = threadIdx.x;
+    const int half = blockIdx.y;  // output channels [8 half, 8 half + 8)
+    const int seg_per_row = (W + kSmallPixelsPerWave - 1) / kSmallPixelsPerWave;
+    const int y = blockIdx.x / seg_per_row, seg = blockIdx.x % seg_per_row;
+    float* out = part + ((size_t)blockIdx.x * 2 + half) * 152;
+    if (y >= H) return;
+    float acc[152];  // [co 8][ci 2][tap 9] then 8 biases
+    for (int i = 0; i < 152; ++i) acc[i] = 0.0f;
+    for (int x = seg * kSmallPixelsPerWave + lane; x < min(W, (seg + 1) * kSmallPixelsPerWave); x += 64) {
+        const int p = y * W + x;
+        float g[8];
+        bool any = false;
+        for (int c = 0; c < 8; ++c) {
+            g[c] = bf2f(dz1[(size_t)p * kDnC + 8 * half + c]);
+            any |= g[c] != 0.0f;
+        }
+        if (!any) continue;
+        for (int c = 0; c < 8; ++c) acc[144 + c] += g[c];
+        for (int ky = 0; ky < 3; ++ky)
+            for (int kx = 0; kx < 3; ++kx) {
+                const int qx = x + kx - 1, qy = y + ky - 1;
+                if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+                const uint16_t d = depth[qy * W + qx];
+                if (!d) continue;
+                const float x0 = depth_m(d);
+                for (int c = 0; c < 8; ++c) {
+                    acc[(c * 2 + 0) * 9 + ky * 3 + kx] += g[c] * x0;
+                    acc[(c * 2 + 1) * 9 + ky * 3 + kx] += g[c];
+                }
+            }
+    }
+    for (int i = 0; i < 152; ++i) {
+        float v = acc[i];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) out[i] = v;
+    }
+}
+
+// Input gradient of layer 2 into layer 1's activation: the hidden-layer
+// kernel with layer 2's transposed weights (dn_mid_bwd_input).
+
+// Sums the per-wave partials of one layer in wave order (deterministic) and
+// accumulates scale * sum into grad (fp32).  Layout maps the partial index to
+// the flat parameter index.
+__global__ void __launch_bounds__(256) dn_reduce_mid(const float* __restrict__ part, int nwave, int woff, int boff,
+                                                     const double* __restrict__ stats, float* __restrict__ grad) {
+    const int i = blockIdx.x * 256 + threadIdx.x;  // 0 .. 2319
+    if (i >= 2320) return;
+    double s = 0.0;
+    for (int w = 0; w < nwave; ++w) s += part[(size_t)w * 2320 + i];
+    const int dst = i < 2304 ? woff + i : boff + (i - 2304);
+    grad[dst] += (float)(s * stats[2]);
+}
+
+__global__ void __launch_bounds__(256) dn_reduce_l4(const float* __restrict__ part, int nwave,
+                                                    const double* __restrict__ stats, float* __restrict__ grad) {
+    const int i = blockIdx.x * 256 + threadIdx.x;  // 0 .. 144
+    if (i >= 145) return;
+    double s = 0.0;
+    for (int w = 0; w < nwave; ++w) s += part[(size_t)w * 145 + i];
+    grad[i < 144 ? kOffW4 + i : kOffB4] += (float)(s * stats[2]);
+}
+
+__global__ void __launch_bounds__(256) dn_reduce_l1(const float* __restrict__ part, int nwave,
+                                                    const double* __restrict__ stats, float* __restrict__ grad) {
+    const int i = blockIdx.x * 256 + threadIdx.x;  // 0 .. 303
+    if (i >= 304) return;
+    // parameter i -> (half, local index)
+    int half, local;
+    if (i < 288) {
+        const int co = i / 18, rem = i % 18;  // [co][ci][tap]
+        half = co / 8;
+        local = (co % 8) * 18 + rem;
+    } else {
+        const int co = i - 288;
+        half = co / 8;
+        local = 144 + co % 8;
+    }
+    double s = 0.0;
+    for (int w = 0; w < nwave; ++w) s += part[((size_t)w * 2 + half) * 152 + local];
+    grad[kOffW1 + i] += (float)(s * stats[2]);
+}
+
+// Adam (Kingma & Ba, 2015) on the fp32 master weights; grad is the sum of
+// per-frame mean gradients over `frames` frames and is cleared afterwards.
+__global__ void dn_adam_kernel(float* __restrict__ w, float* __restrict__ grad, float* __restrict__ m,
+                               float* __restrict__ v, int n, float lr, float b1, float b2, float eps, float c1,
+                               float c2, float inv_frames) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = grad[i] * inv_frames;
+    const float mi = b1 * m[i] + (1.0f - b1) * g;
+    const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    grad[i] = 0.0f;
+}
+
+}  // namespace kfx
+
+struct kfx_denoiser {
+    int W = 0, H = 0;
+    float* master = nullptr;      // kDnParams
+    float* grad = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    kfx::DnPacked* pk = nullptr;
+    uint16_t *a1 = nullptr, *a2 = nullptr, *a3 = nullptr;  // activations
+    uint16_t *g1 = nullptr, *g2 = nullptr, *g3 = nullptr;  // gradients of the hidden layers
+    float* ypred = nullptr;
+    float* dz4 = nullptr;
+    double* blk_sse = nullptr;
+    unsigned* blk_cnt = nullptr;
+    double* stats = nullptr;
+    double* host_stats = nullptr;
+    float* part = nullptr;
+    size_t part_floats = 0;
+    int adam_t = 0;
+};
+
+using namespace kfx;
+
+namespace {
+
+int mid_waves(int W, int H) {
+    const int groups = (W + 15) / 16;
+    return H * ((groups + kMidGroupsPerWave - 1) / kMidGroupsPerWave);
+}
+int small_waves(int W, int H) { return H * ((W + kSmallPixelsPerWave - 1) / kSmallPixelsPerWave); }
+
+int dn_forward(kfx_denoiser* d, const uint16_t* depth, uint16_t* out, float* ypred) {
+    const int W = d->W, H = d->H;
+    dim3 pix_grid(divup(W, 64), divup(H, 4));
+    dim3 mid_grid(divup(W, 64), H);
+    ProfScope prof("denoise_forward");
+    dn_l1_fwd<<<pix_grid, 256, 0, stream()>>>(depth, W, H, d->pk, d->a1);
+    KFX_CHECK_LAUNCH();
+    dn_mid_fwd<<<mid_grid, 256, 0, stream()>>>(d->a1, d->a2, W, H, &d->pk->w2[0][0][0], d->pk->b2);
+    KFX_CHECK_LAUNCH();
+    dn_mid_fwd<<<mid_grid, 256, 0, stream()>>>(d->a2, d->a3, W, H, &d->pk->w3[0][0][0], d->pk->b3);
+    KFX_CHECK_LAUNCH();
+    dn_l4_fwd<<<pix_grid, 256, 0, stream()>>>(depth, d->a3, W, H, d->pk, ypred, out);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int dn_pack(kfx_denoiser* d) {
+    dn_pack_kernel<<<divup(2304, 256), 256, 0, stream()>>>(d->master, d->pk);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kfx_denoiser_param_count(void) { return kDnParams; }
+
+int kfx_denoiser_create(int cols, int rows, kfx_denoiser** out) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(out && cols > 0 && rows > 0, "kfx_denoiser_create: bad arguments");
+    kfx_denoiser* d = new kfx_denoiser();
+    d->W = cols;
+    d->H = rows;
+    const size_t P = (size_t)cols * rows;
+    const int nblk = (int)divup((unsigned)P, 256u);
+    d->part_floats = std::max({(size_t)mid_waves(cols, rows) * 2320, (size_t)small_waves(cols, rows) * 145,
+                               (size_t)small_waves(cols, rows) * 2 * 152});
+    hipError_t e = hipMalloc((void**)&d->master, sizeof(float) * kDnParams);
+    if (e == hipSuccess) e = hipMalloc((void**)&d->grad, sizeof(float) * kDnParams);
+    if (e == hipSuccess) e = hipMalloc((void**)&d->adam_m, sizeof(float) * kDnParams);
+    if (e == hipSuccess) e = hipMalloc((void**)&d->adam_v, sizeof(float) * kDnParams);
+    if (e == hipSuccess) e = hipMalloc((void**)&d->pk, sizeof(DnPacked));
+    uint16_t** acts[6] = {&d->a1, &d->a2, &d->a3, &d->g1, &d->g2, &d->g3};
+    for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipMalloc((void**)acts[i], 2 * kDnC * P);
+    if (e == hipSuccess) e = hipMalloc((void**)&d->ypred, sizeof(float) * P);
+    if (e == hipSuccess) e = hipMalloc((void**)&d->dz4, sizeof(float) * P);
+    if (e == hipSuccess) e = hipMalloc((void**)&d->blk_sse, sizeof(double) * nblk);
+    if (e == hipSuccess) e = hipMalloc((void**)&d->blk_cnt, sizeof(unsigned) * nblk);
+    if (e == hipSuccess) e = hipMalloc((void**)&d->stats, sizeof(double) * 4);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&d->host_stats, sizeof(double) * 4, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void**)&d->part, sizeof(float) * d->part_floats);
+    if (e == hipSuccess) e = hipMemsetAsync(d->master, 0, sizeof(float) * kDnParams, stream());
+    if (e == hipSuccess) e = hipMemsetAsync(d->grad, 0, sizeof(float) * kDnParams, stream());
+    if (e == hipSuccess) e = hipMemsetAsync(d->adam_m, 0, sizeof(float) * kDnParams, stream());
+    if (e == hipSuccess) e = hipMemsetAsync(d->adam_v, 0, sizeof(float) * kDnParams, stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    if (e != hipSuccess) {
+        kfx_denoiser_destroy(d);
+        return set_error(KFX_ERR_HIP, "kfx_denoiser_create: %s", hipGetErrorString(e));
+    }
+    int rc = dn_pack(d);
+    if (rc) {
+        kfx_denoiser_destroy(d);
+        return rc;
+    }
+    *out = d;
+    return KFX_OK;
+}
+
+int kfx_denoiser_destroy(kfx_denoiser* d) {
+    if (!d) return KFX_OK;
+    if (initialised()) (void)hipStreamSynchronize(stream());
+    void* ptrs[] = {d->master, d->grad, d->adam_m, d->adam_v, d->pk, d->a1, d->a2, d->a3, d->g1, d->g2, d->g3,
+                    d->ypred, d->dz4, d->blk_sse, d->blk_cnt, d->stats, d->part};
+    for (void* p : ptrs) (void)hipFree(p);
+    if (d->host_stats) (void)hipHostFree(d->host_stats);
+    delete d;
+    return KFX_OK;
+}
+
+int kfx_denoiser_set_params(kfx_denoiser* d, const float* host_params) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(d && host_params, "kfx_denoiser_set_params: null pointer");
+    KFX_CHECK_HIP(hipMemcpyAsync(d->master, host_params, sizeof(float) * kDnParams, hipMemcpyHostToDevice, stream()));
+    KFX_CHECK_HIP(hipStreamSynchronize(stream()));
+    d->adam_t = 0;
+    KFX_CHECK_HIP(hipMemsetAsync(d->adam_m, 0, sizeof(float) * kDnParams, stream()));
+    KFX_CHECK_HIP(hipMemsetAsync(d->adam_v, 0, sizeof(float) * kDnParams, stream()));
+    KFX_CHECK_HIP(hipMemsetAsync(d->grad, 0, sizeof(float) * kDnParams, stream()));
+    return dn_pack(d);
+}
+
+int kfx_denoiser_get_params(kfx_denoiser* d, float* host_params) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(d && host_params, "kfx_denoiser_get_params: null pointer");
+    KFX_CHECK_HIP(hipMemcpyAsync(host_params, d->master, sizeof(float) * kDnParams, hipMemcpyDeviceToHost, stream()));
+    KFX_CHECK_HIP(hipStreamSynchronize(stream()));
+    return KFX_OK;
+}
+
+int kfx_denoiser_forward(kfx_denoiser* d, const uint16_t* depth, uint16_t* out_depth, float* out_metres) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(d && depth, "kfx_denoiser_forward: null pointer");
+    return dn_forward(d, depth, out_depth, out_metres);
+}
+
+int kfx_denoiser_accumulate(kfx_denoiser* d, const uint16_t* depth, const float* target_m, float outlier_m,
+                            double* loss, uint64_t* count) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(d && depth && target_m, "kfx_denoiser_accumulate: null pointer");
+    const int W = d->W, H = d->H, P = W * H;
+    int rc = dn_forward(d, depth, nullptr, d->ypred);
+    if (rc) return rc;
+    ProfScope prof("denoise_backward");
+    const int nblk = (int)divup((unsigned)P, 256u);
+    dn_loss_kernel<<<nblk, 256, 0, stream()>>>(depth, d->ypred, target_m, P, outlier_m, d->dz4, d->blk_sse, d->blk_cnt);
+    KFX_CHECK_LAUNCH();
+    dn_loss_reduce<<<1, 256, 0, stream()>>>(d->blk_sse, d->blk_cnt, nblk, d->stats);
+    KFX_CHECK_LAUNCH();
+    dim3 pix_grid(divup(W, 64), divup(H, 4));
+    dim3 mid_grid(divup(W, 64), H);
+    const int ms = mid_waves(W, H), ss = small_waves(W, H);
+    // layer 4
+    dn_l4_wgrad<<<ss, 64, 0, stream()>>>(d->dz4, d->a3, depth, W, H, d->part);
+    KFX_CHECK_LAUNCH();
+    dn_reduce_l4<<<1, 256, 0, stream()>>>(d->part, ss, d->stats, d->grad);
+    KFX_CHECK_LAUNCH();
+    dn_l4_bwd_input<<<pix_grid, 256, 0, stream()>>>(d->dz4, d->a3, depth, W, H, d->pk, d->g3);
+    KFX_CHECK_LAUNCH();
+    // layer 3: weights, then input gradient into layer 2's activation
+    dn_mid_wgrad<<<ms, 64, 0, stream()>>>(d->g3, d->a2, W, H, d->part);
+    KFX_CHECK_LAUNCH();
+    dn_reduce_mid<<<divup(2320, 256), 256, 0, stream()>>>(d->part, ms, kOffW3, kOffB3, d->stats, d->grad);
+    KFX_CHECK_LAUNCH();
+    dn_mid_bwd_input<<<mid_grid, 256, 0, stream()>>>(d->g3, d->a2, W, H, &d->pk->w3t[0][0][0], d->g2);
+    KFX_CHECK_LAUNCH();
+    // layer 2
+    dn_mid_wgrad<<<ms, 64, 0, stream()>>>(d->g2, d->a1, W, H, d->part);
+    KFX_CHECK_LAUNCH();
+    dn_reduce_mid<<<divup(2320, 256), 256, 0, stream()>>>(d->part, ms, kOffW2, kOffB2, d->stats, d->grad);
+    KFX_CHECK_LAUNCH();
+    dn_mid_bwd_input<<<mid_grid, 256, 0, stream()>>>(d->g2, d->a1, W, H, &d->pk->w2t[0][0][0], d->g1);
+    KFX_CHECK_LAUNCH();
+    // layer 1
+    dn_l1_wgrad<<<dim3(ss, 2), 64, 0, stream()>>>(d->g1, depth, W, H, d->part);
+    KFX_CHECK_LAUNCH();
+    dn_reduce_l1<<<divup(304, 256), 256, 0, stream()>>>(d->part, ss, d->stats, d->grad);
+    KFX_CHECK_LAUNCH();
+    if (loss || count) {
+        KFX_CHECK_HIP(hipMemcpyAsync(d->host_stats, d->stats, sizeof(double) * 3, hipMemcpyDeviceToHost, stream()));
+        KFX_CHECK_HIP(hipStreamSynchronize(stream()));
+        if (loss) *loss = d->host_stats[1] > 0 ? d->host_stats[0] / d->host_stats[1] : 0.0;
+        if (count) *count = (uint64_t)d->host_stats[1];
+    }
+    return KFX_OK;
+}
+
+int kfx_denoiser_get_grad(kfx_denoiser* d, float* host_grad) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(d && host_grad, "kfx_denoiser_get_grad: null pointer");
+    KFX_CHECK_HIP(hipMemcpyAsync(host_grad, d->grad, sizeof(float) * kDnParams, hipMemcpyDeviceToHost, stream()));
+    KFX_CHECK_HIP(hipStreamSynchronize(stream()));
+    return KFX_OK;
+}
+
+int kfx_denoiser_zero_grad(kfx_denoiser* d) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(d != nullptr, "kfx_denoiser_zero_grad: null pointer");
+    return kfx_fill_u8(d->grad, 0, sizeof(float) * kDnParams);
+}
+
+int kfx_denoiser_adam_step(kfx_denoiser* d, float lr, int frames) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(d && frames > 0 && lr > 0.0f, "kfx_denoiser_adam_step: bad arguments");
+    d->adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f - powf(b1, (float)d->adam_t), c2 = 1.0f - powf(b2, (float)d->adam_t);
+    dn_adam_kernel<<<divup(kDnParams, 256), 256, 0, stream()>>>(d->master, d->grad, d->adam_m, d->adam_v, kDnParams,
+                                                                lr, b1, b2, eps, c1, c2, 1.0f / (float)frames);
+    KFX_CHECK_LAUNCH();
+    return dn_pack(d);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Model-depth helpers: training targets and the fallback gate.
+// ---------------------------------------------------------------------------
+namespace kfx {
+
+// z-depth in the camera of each raycast vertex (volume coordinates) at the
+// pose (R camera -> volume, t); 0 where the raycast found no surface.
+__global__ void __launch_bounds__(256) model_depth_kernel(const kfx_xyz* __restrict__ vmap, int P, kfx_pose_f pose,
+                                                          float* __restrict__ out) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= P) return;
+    const kfx_xyz v = vmap[p];
+    float z = 0.0f;
+    if (!isnan(v.x)) {
+        const float dx = v.x - pose.t[0], dy = v.y - pose.t[1], dz = v.z - pose.t[2];
+        z = pose.R[2] * dx + pose.R[5] * dy + pose.R[8] * dz;  // third row of R^T
+        if (!(z > 0.0f)) z = 0.0f;
+    }
+    out[p] = z;
+}
+
+// Consistency of a depth frame with the model: every raycast vertex is
+// brought into the camera at the device pose and projected; where the frame
+// has depth at that pixel and the two differ by less than max_diff, the
+// squared difference is accumulated.  stats: [sum of squares (m^2), count].
+__global__ void __launch_bounds__(256) depth_consistency_kernel(const kfx_xyz* __restrict__ vmap,
+                                                                const float* __restrict__ dpose,
+                                                                const uint16_t* __restrict__ depth,
+                                                                kfx_intrinsics in, float max_diff,
+                                                                double* __restrict__ stats) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    double s = 0.0, c = 0.0;
+    if (p < in.cols * in.rows) {
+        const kfx_xyz v = vmap[p];
+        if (!isnan(v.x)) {
+            const float dx = v.x - dpose[9], dy = v.y - dpose[10], dz = v.z - dpose[11];
+            const float cx = dpose[0] * dx + dpose[3] * dy + dpose[6] * dz;
+            const float cy = dpose[1] * dx + dpose[4] * dy + dpose[7] * dz;
+            const float cz = dpose[2] * dx + dpose[5] * dy + dpose[8] * dz;
+            if (cz > 0.0f) {
+                const float fu = rintf(cx * in.fx / cz + in.cx), fv = rintf(cy * in.fy / cz + in.cy);
+                if (fu >= 0.0f && fv >= 0.0f && fu < (float)in.cols && fv < (float)in.rows) {
+                    const uint16_t d = depth[(int)fv * in.cols + (int)fu];
+                    if (d) {
+                        const float e = (float)d * 0.001f - cz;
+                        if (fabsf(e) < max_diff) {
+                            s = (double)e * e;
+                            c = 1.0;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        c += __shfl_xor(c, off);
+    }
+    if ((threadIdx.x & 63) == 0 && c > 0.0) {
+        atomicAdd(&stats[0], s);
+        atomicAdd(&stats[1], c);
+    }
+}
+
+__global__ void zero_doubles_kernel(double* p, int n) {
+    const int i = threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+int depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
+                      const kfx_intrinsics* intr, float max_diff, double* dev_stats) {
+    zero_doubles_kernel<<<1, 64, 0, stream()>>>(dev_stats, 2);
+    KFX_CHECK_LAUNCH();
+    const int P = intr->cols * intr->rows;
+    depth_consistency_kernel<<<divup(P, 256), 256, 0, stream()>>>(vmap_model, dev_pose, depth, *intr, max_diff,
+                                                                  dev_stats);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+}  // namespace kfx
+
+extern "C" {
+
+int kfx_model_depth(const kfx_xyz* vmap_model, const kfx_pose* pose_volume, int cols, int rows, float* out_m) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(vmap_model && pose_volume && out_m && cols > 0 && rows > 0, "kfx_model_depth: bad arguments");
+    kfx_pose_f pf;
+    for (int i = 0; i < 9; ++i) pf.R[i] = (float)pose_volume->R[i];
+    for (int i = 0; i < 3; ++i) pf.t[i] = (float)pose_volume->t[i];
+    const int P = cols * rows;
+    model_depth_kernel<<<divup(P, 256), 256, 0, stream()>>>(vmap_model, P, pf, out_m);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
+                          const kfx_intrinsics* intr, float max_diff_m, double* dev_stats) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(vmap_model && dev_pose && depth && intr && dev_stats, "kfx_depth_consistency: null pointer");
+    return depth_consistency(vmap_model, dev_pose, depth, intr, max_diff_m, dev_stats);
+}
+
+}  // extern "C"

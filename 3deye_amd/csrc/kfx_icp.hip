@@ -29,6 +29,7 @@ struct IcpFrame {
     float fx, fy, cx, cy;
     int cols, rows;
     float dist_thr, angle_thr;
+    int prev_from_state;     // take rprev_inv / tprev from the state (set by begin_frame)
 };
 
 // Device-resident ICP state of the fused path.
@@ -37,6 +38,7 @@ struct IcpState {
     float Rf[9], tf[3];      // single-precision copy read by the Jacobian
     double A[36], b[6];      // last normal equations
     double x[6];             // last increment
+    float Rpi[9], tp[3];     // previous frame's pose: R^T and t (begin_frame)
     int status;              // KFX_OK or KFX_ERR_SINGULAR (last solve)
     int singular_count;
     int converged_tag;       // tag of the level that has converged (-1 none)
@@ -255,6 +257,10 @@ icp_fused_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ 
     __shared__ double dred[kIcpThreads / 64][27];
     __shared__ bool last;
     if (tag >= 0 && st->converged_tag == tag) return;  // uniform across the grid
+    if (f.prev_from_state) {
+        for (int i = 0; i < 9; ++i) f.rprev_inv[i] = st->Rpi[i];
+        for (int i = 0; i < 3; ++i) f.tprev[i] = st->tp[i];
+    }
     float Rc[9], tc[3];
     for (int i = 0; i < 9; ++i) Rc[i] = st->Rf[i];
     for (int i = 0; i < 3; ++i) tc[i] = st->tf[i];
@@ -314,10 +320,32 @@ icp_fused_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ 
     }
 }
 
+// Start of a frame: the current estimate becomes the previous pose (as R^T,
+// t in single precision) and the per-frame counters reset.
+__global__ void icp_begin_frame_kernel(IcpState* st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) st->Rpi[3 * r + c] = (float)st->R[3 * c + r];
+    for (int i = 0; i < 3; ++i) st->tp[i] = (float)st->t[i];
+    st->status = KFX_OK;
+    st->singular_count = 0;
+    st->converged_tag = -1;
+    st->iterations = 0;
+}
+
+__global__ void icp_translate_kernel(IcpState* st, double dx, double dy, double dz) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    st->t[0] += dx; st->t[1] += dy; st->t[2] += dz;
+    for (int i = 0; i < 3; ++i) st->tf[i] = (float)st->t[i];
+}
+
 __global__ void icp_reset_kernel(IcpState* st, kfx_pose pose) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     for (int i = 0; i < 9; ++i) { st->R[i] = pose.R[i]; st->Rf[i] = (float)pose.R[i]; }
     for (int i = 0; i < 3; ++i) { st->t[i] = pose.t[i]; st->tf[i] = (float)pose.t[i]; }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) st->Rpi[3 * r + c] = (float)pose.R[3 * c + r];
+    for (int i = 0; i < 3; ++i) st->tp[i] = (float)pose.t[i];
     st->status = KFX_OK;
     st->singular_count = 0;
     st->converged_tag = -1;
@@ -328,12 +356,15 @@ __global__ void icp_reset_kernel(IcpState* st, kfx_pose pose) {
 IcpFrame make_frame(const kfx_matrix_3f* rprev_inv, const kfx_point_3f* tprev, const kfx_intrinsics* intr,
                     float dist_thr, float angle_thr) {
     IcpFrame f;
+    f.prev_from_state = (rprev_inv == nullptr);
     for (int r = 0; r < 3; ++r) {
-        f.rprev_inv[3 * r + 0] = rprev_inv->data[r].x;
-        f.rprev_inv[3 * r + 1] = rprev_inv->data[r].y;
-        f.rprev_inv[3 * r + 2] = rprev_inv->data[r].z;
+        f.rprev_inv[3 * r + 0] = rprev_inv ? rprev_inv->data[r].x : 0.0f;
+        f.rprev_inv[3 * r + 1] = rprev_inv ? rprev_inv->data[r].y : 0.0f;
+        f.rprev_inv[3 * r + 2] = rprev_inv ? rprev_inv->data[r].z : 0.0f;
     }
-    f.tprev[0] = tprev->x; f.tprev[1] = tprev->y; f.tprev[2] = tprev->z;
+    f.tprev[0] = tprev ? tprev->x : 0.0f;
+    f.tprev[1] = tprev ? tprev->y : 0.0f;
+    f.tprev[2] = tprev ? tprev->z : 0.0f;
     f.fx = intr->fx; f.fy = intr->fy; f.cx = intr->cx; f.cy = intr->cy;
     f.cols = intr->cols; f.rows = intr->rows;
     f.dist_thr = dist_thr;
@@ -446,9 +477,10 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
                     const kfx_intrinsics* intr, float distance_threshold, float angle_threshold, int level_tag,
                     double converge_rotation, double converge_translation) {
     KFX_REQUIRE_INIT();
-    KFX_REQUIRE(h && vmap_current && nmap_current && vmap_previous && nmap_previous && rotation_previous_inverse &&
-                    translation_previous && intr,
+    KFX_REQUIRE(h && vmap_current && nmap_current && vmap_previous && nmap_previous && intr,
                 "kfx_icp_iterate: null pointer");
+    KFX_REQUIRE((rotation_previous_inverse == nullptr) == (translation_previous == nullptr),
+                "kfx_icp_iterate: give both previous-pose arguments or neither");
     KFX_REQUIRE(intr->cols > 0 && intr->rows > 0, "kfx_icp_iterate: empty image");
     const int nblk = (int)divup((unsigned)(intr->cols * intr->rows), (unsigned)kIcpPixelsPerBlock);
     KFX_REQUIRE(nblk <= h->max_blocks, "kfx_icp_iterate: image of %d pixels exceeds the handle's capacity",
@@ -459,6 +491,22 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
     icp_fused_kernel<<<nblk, kIcpThreads, 0, stream()>>>(vmap_current, nmap_current, vmap_previous, nmap_previous, f,
                                                          h->state, h->partials, level_tag, converge_rotation,
                                                          converge_translation);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_icp_begin_frame(kfx_icp* h) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(h != nullptr, "kfx_icp_begin_frame: null handle");
+    icp_begin_frame_kernel<<<1, 64, 0, stream()>>>(h->state);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_icp_translate(kfx_icp* h, double dx, double dy, double dz) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(h != nullptr, "kfx_icp_translate: null handle");
+    icp_translate_kernel<<<1, 64, 0, stream()>>>(h->state, dx, dy, dz);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
@@ -480,10 +528,12 @@ int kfx_icp_read(kfx_icp* h, kfx_icp_result* out) {
     return KFX_OK;
 }
 
-int kfx_icp_pose_device(kfx_icp* h, const void** dev_R_f32, const void** dev_t_f32) {
-    KFX_REQUIRE(h && dev_R_f32 && dev_t_f32, "kfx_icp_pose_device: null pointer");
-    *dev_R_f32 = &h->state->Rf[0];
-    *dev_t_f32 = &h->state->tf[0];
+int kfx_icp_pose_device(kfx_icp* h, const float** dev_pose, const float** dev_prev_pose_inv) {
+    KFX_REQUIRE(h && dev_pose && dev_prev_pose_inv, "kfx_icp_pose_device: null pointer");
+    static_assert(offsetof(IcpState, tf) == offsetof(IcpState, Rf) + 9 * sizeof(float), "pose layout");
+    static_assert(offsetof(IcpState, tp) == offsetof(IcpState, Rpi) + 9 * sizeof(float), "prev pose layout");
+    *dev_pose = &h->state->Rf[0];
+    *dev_prev_pose_inv = &h->state->Rpi[0];
     return KFX_OK;
 }
 

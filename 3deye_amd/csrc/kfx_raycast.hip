@@ -94,9 +94,15 @@ __device__ __forceinline__ f3 ray_at(f3 o, f3 d, float t) {
     return f3{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
 }
 
+template <bool kDevicePose>
 __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict__ vol,
                                                       kfx_xyz* __restrict__ vmap,
-                                                      kfx_xyz* __restrict__ nmap, RaycastParams p) {
+                                                      kfx_xyz* __restrict__ nmap, RaycastParams p,
+                                                      const float* __restrict__ dpose) {
+    if (kDevicePose) {
+        for (int i = 0; i < 9; ++i) p.r[i] = dpose[i];
+        p.tx = dpose[9]; p.ty = dpose[10]; p.tz = dpose[11];
+    }
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= p.cols || y >= p.rows) return;
@@ -202,10 +208,10 @@ int check_round_buffer(const kfx_round_buffer* rb, const char* who);
 
 using namespace kfx;
 
-extern "C" int kfx_raycast(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
-                           const kfx_round_buffer* rb, const kfx_point_3f* translation,
-                           const kfx_matrix_3f* rotation, float truncation_distance,
-                           const kfx_intrinsics* intr, const kfx_point_3f* cell_size) {
+static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                          const kfx_round_buffer* rb, const kfx_point_3f* translation,
+                          const kfx_matrix_3f* rotation, const float* dev_pose, float truncation_distance,
+                          const kfx_intrinsics* intr, const kfx_point_3f* cell_size) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(volume && vertex_map && normal_map && translation && rotation && intr && cell_size,
                 "kfx_raycast: null pointer");
@@ -229,7 +235,28 @@ extern "C" int kfx_raycast(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz*
     ProfScope prof("raycast");
     dim3 block(64, 4);
     dim3 grid(divup(intr->cols, 64), divup(intr->rows, 4));
-    raycast_kernel<<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p);
+    if (dev_pose)
+        raycast_kernel<true><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, dev_pose);
+    else
+        raycast_kernel<false><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, nullptr);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
+}
+
+extern "C" int kfx_raycast(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                           const kfx_round_buffer* rb, const kfx_point_3f* translation,
+                           const kfx_matrix_3f* rotation, float truncation_distance,
+                           const kfx_intrinsics* intr, const kfx_point_3f* cell_size) {
+    return raycast_launch(volume, vertex_map, normal_map, rb, translation, rotation, nullptr, truncation_distance,
+                          intr, cell_size);
+}
+
+extern "C" int kfx_raycast_posed(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                                 const kfx_round_buffer* rb, const float* dev_pose, float truncation_distance,
+                                 const kfx_intrinsics* intr, const kfx_point_3f* cell_size) {
+    KFX_REQUIRE(dev_pose != nullptr, "kfx_raycast_posed: null pose");
+    const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    const kfx_point_3f zero = {0, 0, 0};
+    return raycast_launch(volume, vertex_map, normal_map, rb, &zero, &ident, dev_pose, truncation_distance, intr,
+                          cell_size);
 }

@@ -54,6 +54,8 @@ hipEvent_t pooled_event() {
 
 ProfScope::ProfScope(const char* name) : slot(-1) {
     if (!g_prof_on) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(g_stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     std::lock_guard<std::mutex> lk(g_prof_mu);
     hipEvent_t a = pooled_event(), b = pooled_event();
     if (!a || !b) return;

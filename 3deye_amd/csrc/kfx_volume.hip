@@ -109,8 +109,44 @@ __device__ __forceinline__ bool integrate_voxel(const IntegrateParams& p, float 
     return true;
 }
 
+// Fills the pose-dependent fields of p from a device-resident pose (R
+// camera -> volume row-major, then t: 12 floats); false when the camera moved
+// less than motion_thr since the previous pose (R^T, t), the KinectFusion
+// integration metric (|rotation angle| + |translation|) / 2.
+__device__ __forceinline__ bool load_device_pose(IntegrateParams& p, const float* __restrict__ dpose,
+                                                 const float* __restrict__ dprev, float motion_thr) {
+    float R[9], t[3];
+    for (int i = 0; i < 9; ++i) R[i] = dpose[i];
+    for (int i = 0; i < 3; ++i) t[i] = dpose[9 + i];
+    if (dprev != nullptr && motion_thr > 0.0f) {
+        float tr = 0.0f;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) tr += R[3 * i + j] * dprev[3 * j + i];
+        const float ang = acosf(fminf(1.0f, fmaxf(-1.0f, 0.5f * (tr - 1.0f))));
+        const float dx = t[0] - dprev[9], dy = t[1] - dprev[10], dz = t[2] - dprev[11];
+        if (0.5f * (ang + sqrtf(dx * dx + dy * dy + dz * dz)) < motion_thr) return false;
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) p.r[3 * r + c] = R[3 * c + r];
+    p.tx = t[0]; p.ty = t[1]; p.tz = t[2];
+    const float m = kClipMarginPx;
+    const float cl = p.cx + 0.5f + m, cr = (float)p.cols - 0.5f + m - p.cx;
+    const float ct = p.cy + 0.5f + m, cb = (float)p.rows - 0.5f + m - p.cy;
+    for (int k = 0; k < 3; ++k) {
+        p.plane[0][k] = p.r[6 + k];
+        p.plane[1][k] = p.fx * p.r[k] + cl * p.r[6 + k];
+        p.plane[2][k] = -p.fx * p.r[k] + cr * p.r[6 + k];
+        p.plane[3][k] = p.fy * p.r[3 + k] + ct * p.r[6 + k];
+        p.plane[4][k] = -p.fy * p.r[3 + k] + cb * p.r[6 + k];
+    }
+    return true;
+}
+
+template <bool kDevicePose>
 __global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
-integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume, IntegrateParams p) {
+integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume, IntegrateParams p,
+                 const float* __restrict__ dpose, const float* __restrict__ dprev, float motion_thr) {
+    if (kDevicePose && !load_device_pose(p, dpose, dprev, motion_thr)) return;
     const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
     const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
     const bool inside = x < p.nx && y < p.ny;
@@ -322,7 +358,27 @@ int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* vol
     ProfScope prof("integrate");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
     dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
-    integrate_kernel<<<grid, block, 0, stream()>>>(depth_scaled, volume, p);
+    integrate_kernel<false><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, nullptr, nullptr, 0.0f);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
+                             const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
+                             float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
+                             const kfx_round_buffer* rb) {
+    KFX_REQUIRE(volume != nullptr && dev_pose != nullptr, "kfx_integrate_tsdf_posed: null pointer");
+    const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    const kfx_point_3f zero = {0, 0, 0};
+    IntegrateParams p;
+    int rc = integrate_setup("kfx_integrate_tsdf_posed", depth, depth_scaled, intr, &ident, &zero,
+                             truncation_distance, cell_size, rb, p);
+    if (rc) return rc;
+    ProfScope prof("integrate");
+    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
+    integrate_kernel<true><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, dev_pose, dev_prev_pose_inv,
+                                                         motion_threshold);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }

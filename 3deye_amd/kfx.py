@@ -80,6 +80,37 @@ class IcpResult(C.Structure):
                 ("converged_tag", C.c_int32)]
 
 
+class Xyzi(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class PipelineConfig(C.Structure):
+    _fields_ = [("intr", Intrinsics), ("voxels", Point3i), ("volume_size", Point3f), ("truncation", C.c_float),
+                ("sigma_space", C.c_float), ("sigma_color", C.c_float), ("distance_threshold", C.c_float),
+                ("angle_threshold", C.c_float), ("icp_iterations", C.c_int32 * 3), ("mode", C.c_int32),
+                ("use_graph", C.c_int32), ("denoiser", C.c_int32), ("fallback_rms_mm", C.c_float),
+                ("shift_threshold", C.c_float), ("integrate_threshold", C.c_float),
+                ("converge_rotation", C.c_double), ("converge_translation", C.c_double),
+                ("extract_on_shift", C.c_int32), ("reserved", C.c_int32)]
+
+
+class FrameInfo(C.Structure):
+    _fields_ = [("pose_world", Pose), ("pose_volume", Pose), ("icp_iterations", C.c_int32), ("icp_singular", C.c_int32),
+                ("integrated", C.c_int32), ("shifted", C.c_int32), ("extracted_points", C.c_uint64),
+                ("fallback", C.c_int32), ("denoise_rms_mm", C.c_float)]
+
+
+class PipelineView(C.Structure):
+    _fields_ = [("volume", C.c_void_p), ("round_buffer", RoundBuffer), ("cell_size", Point3f),
+                ("origin_world", C.c_double * 3), ("depth_raw", C.c_void_p), ("depth_filtered", C.c_void_p),
+                ("vmap_model", C.c_void_p), ("nmap_model", C.c_void_p), ("vmap_current", C.c_void_p),
+                ("nmap_current", C.c_void_p), ("pose_volume", Pose), ("frames", C.c_int64), ("shifts", C.c_int32),
+                ("fallbacks", C.c_int32), ("extracted_points", C.c_uint64), ("last_points", C.c_void_p),
+                ("last_colors", C.c_void_p)]
+
+
+PLAIN, FUSED = 0, 1
+
 assert C.sizeof(Intrinsics) == 24 and C.sizeof(RoundBuffer) == 72 and C.sizeof(Matrix3f) == 36
 
 TSDF_DTYPE = np.dtype([("value", "<i2"), ("weight", "<i2")])
@@ -164,6 +195,10 @@ _PROTOS = {
                                 C.c_float, C.POINTER(Point3f), C.POINTER(RoundBuffer)]),
     "kfx_raycast": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), C.POINTER(Point3f), C.POINTER(Matrix3f), C.c_float,
                          C.POINTER(Intrinsics), C.POINTER(Point3f)]),
+    "kfx_integrate_tsdf_posed": (_I, [_P, _P, _P, C.POINTER(Intrinsics), _P, _P, C.c_float, C.c_float,
+                                      C.POINTER(Point3f), C.POINTER(RoundBuffer)]),
+    "kfx_raycast_posed": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), _P, C.c_float, C.POINTER(Intrinsics),
+                               C.POINTER(Point3f)]),
     "kfx_profile_enable": (_I, [_I]),
     "kfx_profile_reset": (_I, []),
     "kfx_profile_query": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -186,7 +221,34 @@ _PROTOS = {
     "kfx_icp_iterate": (_I, [_P, _P, _P, _P, _P, C.POINTER(Matrix3f), C.POINTER(Point3f), C.POINTER(Intrinsics),
                              C.c_float, C.c_float, _I, C.c_double, C.c_double]),
     "kfx_icp_read": (_I, [_P, C.POINTER(IcpResult)]),
+    "kfx_icp_begin_frame": (_I, [_P]),
+    "kfx_icp_translate": (_I, [_P, C.c_double, C.c_double, C.c_double]),
     "kfx_icp_pose_device": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    "kfx_determine_extract_step": (_I, [_I, C.POINTER(_I), C.POINTER(_I)]),
+    "kfx_clean_tsdf_slice": (_I, [C.POINTER(Point3i), C.POINTER(Point3i), C.POINTER(RoundBuffer), _P]),
+    "kfx_clean_color_slice": (_I, [C.POINTER(Point3i), C.POINTER(Point3i), C.POINTER(RoundBuffer), _P]),
+    "kfx_slicer_create": (_I, [C.POINTER(RoundBuffer), C.POINTER(_P)]),
+    "kfx_slicer_destroy": (_I, [_P]),
+    "kfx_extract_slice": (_I, [_P, C.POINTER(Point3i), C.POINTER(Point3i), C.POINTER(RoundBuffer), _P, _P, _P, _P,
+                               C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kfx_pipeline_default_config": (_I, [C.POINTER(PipelineConfig)]),
+    "kfx_pipeline_create": (_I, [C.POINTER(PipelineConfig), C.POINTER(Pose), C.POINTER(C.c_double), C.POINTER(_P)]),
+    "kfx_pipeline_destroy": (_I, [_P]),
+    "kfx_pipeline_process": (_I, [_P, _P, C.POINTER(FrameInfo)]),
+    "kfx_pipeline_state": (_I, [_P, C.POINTER(PipelineView)]),
+    "kfx_pipeline_set_denoiser": (_I, [_P, _P]),
+    "kfx_denoiser_param_count": (_I, []),
+    "kfx_denoiser_create": (_I, [_I, _I, C.POINTER(_P)]),
+    "kfx_denoiser_destroy": (_I, [_P]),
+    "kfx_denoiser_set_params": (_I, [_P, _P]),
+    "kfx_denoiser_get_params": (_I, [_P, _P]),
+    "kfx_denoiser_forward": (_I, [_P, _P, _P, _P]),
+    "kfx_denoiser_accumulate": (_I, [_P, _P, _P, C.c_float, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "kfx_denoiser_get_grad": (_I, [_P, _P]),
+    "kfx_denoiser_zero_grad": (_I, [_P]),
+    "kfx_denoiser_adam_step": (_I, [_P, C.c_float, _I]),
+    "kfx_model_depth": (_I, [_P, C.POINTER(Pose), _I, _I, _P]),
+    "kfx_depth_consistency": (_I, [_P, _P, _P, C.POINTER(Intrinsics), C.c_float, _P]),
     "kfx_splitmix64": (C.c_uint64, [C.c_uint64]),
     "kfx_synth_make_scene": (_I, [C.c_uint64, _I, _I, C.POINTER(C.c_double), C.c_double, C.POINTER(SynthScene)]),
     "kfx_synth_orbit": (_I, [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.c_double, _I,
@@ -355,3 +417,53 @@ def render(scene, pose, intr, seed, frame, noise=True, threads=0, want_clean=Fal
 
 def kinect_intrinsics(cols=640, rows=480, f=525.0):
     return Intrinsics(f, f, (cols - 1) / 2.0, (rows - 1) / 2.0, cols, rows)
+
+
+class Pipeline:
+    """Owner of a kfx_pipeline handle."""
+
+    def __init__(self, cfg, initial_pose_world, volume_origin_world):
+        self.h = C.c_void_p()
+        check(lib.kfx_pipeline_create(C.byref(cfg), C.byref(initial_pose_world),
+                                      (C.c_double * 3)(*[float(v) for v in volume_origin_world]), C.byref(self.h)))
+        self.cfg = cfg
+
+    def process(self, depth_dev_ptr):
+        info = FrameInfo()
+        check(lib.kfx_pipeline_process(self.h, depth_dev_ptr, C.byref(info)))
+        return info
+
+    def view(self):
+        v = PipelineView()
+        check(lib.kfx_pipeline_state(self.h, C.byref(v)))
+        return v
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            check(lib.kfx_pipeline_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and self.h.value:
+                lib.kfx_pipeline_destroy(self.h)
+        except Exception:
+            pass
+
+
+def default_config(**kw):
+    cfg = PipelineConfig()
+    check(lib.kfx_pipeline_default_config(C.byref(cfg)))
+    for k, v in kw.items():
+        if k == "icp_iterations":
+            cfg.icp_iterations[:] = list(v)
+        else:
+            setattr(cfg, k, v)
+    return cfg
+
+
+def ate_rmse(est, gt):
+    """Absolute trajectory error (RMSE of camera positions, metres) of two
+    pose lists with a common first frame (no alignment)."""
+    d = [np.array(e.t[:]) - np.array(g.t[:]) for e, g in zip(est, gt)]
+    return float(np.sqrt(np.mean([np.dot(v, v) for v in d])))

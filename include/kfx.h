@@ -116,6 +116,17 @@ int kfx_integrate_count_updates(const uint16_t* depth, float* depth_scaled, cons
                                 float truncation_distance, const kfx_point_3f* cell_size,
                                 const kfx_round_buffer* rb, uint64_t* count);
 
+/* kfx_integrate_tsdf with the pose read from device memory (12 floats: R
+ * camera -> volume row-major, then t), e.g. the fused ICP's estimate, so the
+ * call can follow ICP without a host round trip (no reference counterpart).
+ * When dev_prev_pose_inv (R^T, t of the previous frame) is given and
+ * motion_threshold > 0, the update is skipped if the camera moved less than
+ * the threshold ((rotation angle + translation) / 2, the KinectFusion rule). */
+int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
+                             const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
+                             float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
+                             const kfx_round_buffer* rb);
+
 /* Raycast (common/inc/opencl_wrappers.h:23-33, GPU/opencl/raycast.cl:182-405):
  * per-pixel march through the cyclic TSDF; writes the surface vertex and the
  * normal (NaN where there is none) in volume coordinates, cols*rows XYZ each. */
@@ -123,6 +134,10 @@ int kfx_raycast(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map
                 const kfx_round_buffer* rb, const kfx_point_3f* translation,
                 const kfx_matrix_3f* rotation, float truncation_distance,
                 const kfx_intrinsics* intr, const kfx_point_3f* cell_size);
+/* kfx_raycast with the pose read from device memory (12 floats, as above). */
+int kfx_raycast_posed(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                      const kfx_round_buffer* rb, const float* dev_pose, float truncation_distance,
+                      const kfx_intrinsics* intr, const kfx_point_3f* cell_size);
 
 
 /* ------------------------------------------------------------------------
@@ -209,11 +224,179 @@ int kfx_icp_iterate(kfx_icp* icp, const kfx_xyz* vmap_current, const kfx_xyz* nm
                     const kfx_matrix_3f* rotation_previous_inverse, const kfx_point_3f* translation_previous,
                     const kfx_intrinsics* intr, float distance_threshold, float angle_threshold, int level_tag,
                     double converge_rotation, double converge_translation);
+/* With rotation_previous_inverse and translation_previous both NULL,
+ * kfx_icp_iterate takes the previous pose from the state (set by
+ * kfx_icp_begin_frame), so a frame's iterations need no host arguments. */
+/* Start a frame: the current estimate becomes the previous pose; counters reset. */
+int kfx_icp_begin_frame(kfx_icp* icp);
+/* Add (dx, dy, dz) to the estimate's translation (re-expression after a
+ * cyclic shift of the volume). */
+int kfx_icp_translate(kfx_icp* icp, double dx, double dy, double dz);
 /* Blocking read of the state. */
 int kfx_icp_read(kfx_icp* icp, kfx_icp_result* out);
-/* Device addresses of the single-precision pose copy (9 + 3 floats), for
+/* Device addresses of the single-precision current pose (R row-major camera
+ * -> volume, then t: 12 floats) and of the previous pose (R^T then t), for
  * kernels that consume the estimate without a host round trip. */
-int kfx_icp_pose_device(kfx_icp* icp, const void** dev_R_f32, const void** dev_t_f32);
+int kfx_icp_pose_device(kfx_icp* icp, const float** dev_pose, const float** dev_prev_pose_inv);
+
+/* ------------------------------------------------------------------------
+ * Cyclic shift: black-zone clean and slice extraction.  A local voxel
+ * (x, y, z) is in the black zone when, on any axis, x + origin_GRID.x lies in
+ * [min_bounds.x, max_bounds.x] (inclusive, unwrapped; clean_color_slice.cl:29-40).
+ * ------------------------------------------------------------------------ */
+
+/* determine_extract_step (common/src/mem_utils.cpp:43-85, divisor 4):
+ * chunk depth depth/8 (512 -> 64, 256 -> 32, 128 -> 16), extended to any
+ * depth divisible by 8 (one chunk otherwise; the reference throws). */
+int kfx_determine_extract_step(int depth, int* step, int* num_of_steps);
+/* Clean_Tsdf_Slice (opencl_wrappers.h:180-185, opencl_wrappers.cpp:835-854):
+ * reset black-zone voxels to {0, 0}. */
+int kfx_clean_tsdf_slice(const kfx_point_3i* min_bounds, const kfx_point_3i* max_bounds, const kfx_round_buffer* rb,
+                         kfx_tsdf* volume);
+/* Clean_Color_Slice (opencl_wrappers.h:186-191, opencl_wrappers.cpp:814-833,
+ * GPU/opencl/clean_color_slice.cl:41-67). */
+int kfx_clean_color_slice(const kfx_point_3i* min_bounds, const kfx_point_3i* max_bounds,
+                          const kfx_round_buffer* rb, kfx_color* volume);
+/* Reusable scratch for slice extraction of one volume size. */
+typedef struct kfx_slicer kfx_slicer;
+int kfx_slicer_create(const kfx_round_buffer* rb, kfx_slicer** out);
+int kfx_slicer_destroy(kfx_slicer* slicer);
+/* Extract_Slice (V + C overload; opencl_wrappers.h:200-211,
+ * opencl_wrappers.cpp:856-964, GPU/opencl/scan_slice_of_both_volumes.cl:76-208,
+ * GPU/opencl/extract_slice_as_cloud.cl:72-202): black-zone zero crossings
+ * (x/y/z neighbour, both weights != 0, F != 1, sign change; the z neighbour
+ * only inside the current chunk) and set colour voxels, as XYZI points (local
+ * grid coordinates of the neighbour, w = 1) with their colours, in the
+ * reference's order.  color_volume may be NULL (no colour volume: plane 3 is
+ * empty and colours are zero).  Blocking: *count receives the number of
+ * points; if it exceeds capacity the call returns KFX_ERR_CAPACITY and
+ * writes nothing. */
+int kfx_extract_slice(kfx_slicer* slicer, const kfx_point_3i* min_bounds, const kfx_point_3i* max_bounds,
+                      const kfx_round_buffer* rb, const kfx_tsdf* tsdf_volume, const kfx_color* color_volume,
+                      kfx_xyzi* points, kfx_color* colors, uint64_t capacity, uint64_t* count);
+
+/* ------------------------------------------------------------------------
+ * Learned depth denoiser (no reference counterpart; it stands in for the
+ * bilateral filter of Apply_Billateral_Filtration).  Residual network of four
+ * 3x3 convolutions, 2 -> 16 -> 16 -> 16 -> 1 channels (input: depth in
+ * metres and the valid mask; output: depth + correction on valid pixels,
+ * invalid pixels stay 0), 5,089 fp32 master parameters (per layer weights
+ * [co][ci][ky][kx] then biases), used in bfloat16 with fp32 accumulation on
+ * the matrix cores.  Trained by the kernels below with Adam.
+ * ------------------------------------------------------------------------ */
+typedef struct kfx_denoiser kfx_denoiser;
+int kfx_denoiser_param_count(void);
+/* All parameters start at zero (the identity map); load trained ones with
+ * kfx_denoiser_set_params. */
+int kfx_denoiser_create(int cols, int rows, kfx_denoiser** out);
+int kfx_denoiser_destroy(kfx_denoiser* dn);
+/* Host <-> device copies of the master parameters (blocking); setting them
+ * also resets the Adam state. */
+int kfx_denoiser_set_params(kfx_denoiser* dn, const float* host_params);
+int kfx_denoiser_get_params(kfx_denoiser* dn, float* host_params);
+/* Forward pass on a device depth frame: out_depth (uint16 mm) and/or
+ * out_metres (float) may be NULL. */
+int kfx_denoiser_forward(kfx_denoiser* dn, const uint16_t* depth, uint16_t* out_depth, float* out_metres);
+/* Forward + backward on one (raw depth, target depth in metres) pair: adds
+ * the gradient of the frame's mean squared error (pixels with a valid input,
+ * a target > 0 and |input - target| < outlier_m) to the gradient buffer.
+ * loss / count (optional, blocking when given) receive the frame's mean
+ * squared error and pixel count. */
+int kfx_denoiser_accumulate(kfx_denoiser* dn, const uint16_t* depth, const float* target_m, float outlier_m,
+                            double* loss, uint64_t* count);
+int kfx_denoiser_get_grad(kfx_denoiser* dn, float* host_grad);
+int kfx_denoiser_zero_grad(kfx_denoiser* dn);
+/* Adam step with the accumulated gradient divided by `frames`; clears it. */
+int kfx_denoiser_adam_step(kfx_denoiser* dn, float lr, int frames);
+/* Training target: z-depth (metres) in the camera at pose_volume of each
+ * raycast vertex (volume coordinates); 0 where there is no surface. */
+int kfx_model_depth(const kfx_xyz* vmap_model, const kfx_pose* pose_volume, int cols, int rows, float* out_m);
+/* Fallback gate: raycast vertices (volume coordinates, previous frame)
+ * projected into the camera at the device pose (12 floats, R camera ->
+ * volume then t); where the depth frame has a value within max_diff_m of the
+ * vertex depth, dev_stats[0] += squared difference, dev_stats[1] += 1
+ * (dev_stats: 2 doubles on the device, cleared first). */
+int kfx_depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
+                          const kfx_intrinsics* intr, float max_diff_m, double* dev_stats);
+
+/* ------------------------------------------------------------------------
+ * Per-frame pipeline: the driver of the reference's absent fusion
+ * application (Makefile:3,19) — preprocessing, coarse-to-fine ICP,
+ * integration, raycast and cyclic shift with slice extraction.
+ * ------------------------------------------------------------------------ */
+enum kfx_pipeline_mode {
+    KFX_PIPELINE_PLAIN = 0, /* reference control flow: sync per step, host solve per ICP iteration */
+    KFX_PIPELINE_FUSED = 1  /* device-resident pose, fused ICP iterations, optional graph replay */
+};
+typedef struct kfx_pipeline_config {
+    kfx_intrinsics intr;          /* level-0 camera; cols, rows multiples of 4 */
+    kfx_point_3i voxels;          /* voxels per axis */
+    kfx_point_3f volume_size;     /* metres per axis */
+    float truncation;             /* 0.03 m */
+    float sigma_space, sigma_color; /* bilateral: 4.5 px, 30 mm */
+    float distance_threshold;     /* ICP correspondence distance, 0.1 m */
+    float angle_threshold;        /* ICP normal angle, sin(20 deg) */
+    int32_t icp_iterations[3];    /* level 0 (finest), 1, 2: 10, 5, 4 */
+    int32_t mode;                 /* kfx_pipeline_mode */
+    int32_t use_graph;            /* fused mode: replay each frame as one HIP graph */
+    int32_t denoiser;             /* 1: learned denoiser instead of the bilateral filter */
+    float fallback_rms_mm;        /* denoiser fallback threshold, 15 mm */
+    float shift_threshold;        /* cyclic shift trigger, 1.5 m from the cube centre */
+    float integrate_threshold;    /* integrate only after (angle + translation) / 2 >= this, 0.001 */
+    double converge_rotation;     /* fused early exit per level (rad); 0 disables */
+    double converge_translation;  /* fused early exit per level (m) */
+    int32_t extract_on_shift;     /* extract the leaving slab before clearing it */
+    int32_t reserved;
+} kfx_pipeline_config;
+
+typedef struct kfx_frame_info {
+    kfx_pose pose_world;          /* camera -> world after this frame */
+    kfx_pose pose_volume;         /* camera -> volume used to integrate this frame (before any shift) */
+    int32_t icp_iterations;       /* ICP solves performed */
+    int32_t icp_singular;         /* solves rejected as singular */
+    int32_t integrated;           /* the frame was integrated */
+    int32_t shifted;              /* a cyclic shift followed this frame */
+    uint64_t extracted_points;    /* points extracted by that shift */
+    int32_t fallback;             /* denoiser fell back to the bilateral filter */
+    float denoise_rms_mm;         /* denoiser output vs raycast model, RMS */
+} kfx_frame_info;
+
+typedef struct kfx_pipeline_view {
+    kfx_tsdf* volume;
+    kfx_round_buffer round_buffer;
+    kfx_point_3f cell_size;
+    double origin_world[3];       /* world position of local voxel (0,0,0)'s corner */
+    uint16_t* depth_raw;          /* last input frame (device) */
+    uint16_t* depth_filtered;     /* level-0 filtered depth (device) */
+    kfx_xyz* vmap_model;          /* last raycast, volume coordinates (device) */
+    kfx_xyz* nmap_model;
+    kfx_xyz* vmap_current;        /* level-0 maps of the last frame, camera coordinates */
+    kfx_xyz* nmap_current;
+    kfx_pose pose_volume;         /* camera -> volume */
+    int64_t frames;
+    int32_t shifts;
+    int32_t fallbacks;            /* frames that fell back from the denoiser to the bilateral filter */
+    uint64_t extracted_points;    /* total over all shifts */
+    kfx_xyzi* last_points;        /* last shift's extracted points (device) */
+    kfx_color* last_colors;
+} kfx_pipeline_view;
+
+typedef struct kfx_pipeline kfx_pipeline;
+/* Defaults of the 512^3, 640x480 configuration (fx = fy = 525, 3.072 m). */
+int kfx_pipeline_default_config(kfx_pipeline_config* cfg);
+/* volume_origin_world: world position of local voxel (0,0,0)'s corner;
+ * initial_pose_world: the first frame's camera -> world pose. */
+int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_pose_world,
+                        const double volume_origin_world[3], kfx_pipeline** out);
+int kfx_pipeline_destroy(kfx_pipeline* pipeline);
+/* Process one depth frame (device pointer, cols x rows uint16 mm).  Returns
+ * when the frame's pose is known on the host (one synchronisation per frame
+ * in fused mode; one per step in plain mode). */
+int kfx_pipeline_process(kfx_pipeline* pipeline, const uint16_t* depth, kfx_frame_info* info);
+int kfx_pipeline_state(kfx_pipeline* pipeline, kfx_pipeline_view* view);
+/* Attach a trained denoiser (fused mode; used when cfg.denoiser is set).  The
+ * pipeline does not own it. */
+int kfx_pipeline_set_denoiser(kfx_pipeline* pipeline, kfx_denoiser* denoiser);
 
 /* ------------------------------------------------------------------------
  * Synthetic input (host only, deterministic): the seeded analytic scene,

@@ -1,0 +1,209 @@
+// denoise.cpp — oracle for the learned depth denoiser: forward pass, loss and
+// backward pass in double precision.  Test infrastructure only (see oracle.h).
+//
+// Network (no reference counterpart): input channels x0 = depth in metres
+// (0 where invalid), x1 = valid mask; four 3x3 convolutions with zero padding,
+// 2 -> 16 -> 16 -> 16 -> 1 channels, ReLU after the first three; output
+// y = x0 + conv4 on valid pixels (residual), 0 elsewhere.  Parameters (5,089,
+// flat fp32): per layer the weights [co][ci][ky][kx] then the biases [co].
+// Loss: mean over pixels with a valid input, a valid target and
+// |x0 - target| < outlier of (y - target)^2.
+//
+// With bf16 != 0 the oracle rounds what the GPU stores in bfloat16 (weights
+// and the three hidden activations, round to nearest even) before using it,
+// so the comparison with the GPU isolates accumulation-order differences;
+// with bf16 == 0 everything is double (for finite-difference checks).  The
+// GPU also stores the hidden layers' gradients in bfloat16; it propagates the
+// unscaled output gradient 2 (y - t) and applies 1 / count at the end, which
+// the bf16 mode follows.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "oracle.h"
+
+namespace {
+
+const int kC = 16;
+const int kLayerCin[4] = {2, 16, 16, 16};
+const int kLayerCout[4] = {16, 16, 16, 1};
+
+int layer_offset(int l) {
+    int o = 0;
+    for (int k = 0; k < l; ++k) o += kLayerCout[k] * kLayerCin[k] * 9 + kLayerCout[k];
+    return o;
+}
+
+double bf16_round(double v) {
+    float f = (float)v;
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    b &= 0xFFFF0000u;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+struct Net {
+    int W, H;
+    bool bf16;
+    std::vector<double> w;                  // parameters (rounded if bf16)
+    std::vector<double> x;                  // [p][2]
+    std::vector<double> a[3];               // hidden activations [p][16]
+    std::vector<double> y;                  // output
+    std::vector<uint8_t> valid;
+};
+
+double in_val(const Net& n, int l, const std::vector<double>* src, int px, int py, int ci) {
+    if (px < 0 || py < 0 || px >= n.W || py >= n.H) return 0.0;
+    const int p = py * n.W + px;
+    if (l == 0) return n.x[2 * p + ci];
+    return (*src)[(size_t)p * kC + ci];
+}
+
+void forward(Net& n) {
+    const int P = n.W * n.H;
+    for (int l = 0; l < 4; ++l) {
+        const int cin = kLayerCin[l], cout = kLayerCout[l], off = layer_offset(l);
+        const std::vector<double>* src = l == 0 ? nullptr : &n.a[l - 1];
+        std::vector<double> out((size_t)P * cout);
+        for (int py = 0; py < n.H; ++py)
+            for (int px = 0; px < n.W; ++px)
+                for (int co = 0; co < cout; ++co) {
+                    double s = n.w[off + cout * cin * 9 + co];
+                    for (int ci = 0; ci < cin; ++ci)
+                        for (int ky = 0; ky < 3; ++ky)
+                            for (int kx = 0; kx < 3; ++kx)
+                                s += n.w[off + ((co * cin + ci) * 3 + ky) * 3 + kx] *
+                                     in_val(n, l, src, px + kx - 1, py + ky - 1, ci);
+                    out[(size_t)(py * n.W + px) * cout + co] = s;
+                }
+        if (l < 3) {
+            for (double& v : out) {
+                v = v > 0.0 ? v : 0.0;
+                if (n.bf16) v = bf16_round(v);
+            }
+            n.a[l] = out;
+        } else {
+            n.y.assign(P, 0.0);
+            for (int p = 0; p < P; ++p) n.y[p] = n.valid[p] ? n.x[2 * p] + out[p] : 0.0;
+        }
+    }
+}
+
+void setup(Net& n, const uint16_t* depth, int W, int H, const float* params, int bf16) {
+    n.W = W;
+    n.H = H;
+    n.bf16 = bf16 != 0;
+    const int np = layer_offset(4);
+    n.w.resize(np);
+    for (int i = 0; i < np; ++i) n.w[i] = n.bf16 ? bf16_round(params[i]) : (double)params[i];
+    // Biases stay fp32 on the GPU.
+    for (int l = 0; l < 4; ++l) {
+        const int off = layer_offset(l) + kLayerCout[l] * kLayerCin[l] * 9;
+        for (int co = 0; co < kLayerCout[l]; ++co) n.w[off + co] = params[off + co];
+    }
+    const int P = W * H;
+    n.x.assign(2 * (size_t)P, 0.0);
+    n.valid.assign(P, 0);
+    for (int p = 0; p < P; ++p)
+        if (depth[p]) {
+            n.valid[p] = 1;
+            n.x[2 * p] = (double)((float)depth[p] * 0.001f);
+            n.x[2 * p + 1] = 1.0;
+        }
+}
+
+}  // namespace
+
+extern "C" int orc_dn_param_count(void) { return layer_offset(4); }
+
+// Forward pass: y (metres, 0 on invalid pixels); hidden activations optional
+// (3 * W * H * 16 doubles, [layer][p][c]).
+extern "C" void orc_dn_forward(const uint16_t* depth, int W, int H, const float* params, int bf16, double* y,
+                               double* hidden) {
+    Net n;
+    setup(n, depth, W, H, params, bf16);
+    forward(n);
+    memcpy(y, n.y.data(), sizeof(double) * W * H);
+    if (hidden)
+        for (int l = 0; l < 3; ++l) memcpy(hidden + (size_t)l * W * H * kC, n.a[l].data(), sizeof(double) * W * H * kC);
+}
+
+// Loss and its gradient with respect to every parameter (5,089 doubles).
+// Returns the loss; *count receives the number of pixels in it.
+extern "C" double orc_dn_loss_grad(const uint16_t* depth, const float* target_m, int W, int H, const float* params,
+                                   int bf16, double outlier_m, double* grad, int* count) {
+    Net n;
+    setup(n, depth, W, H, params, bf16);
+    forward(n);
+    const int P = W * H;
+    std::vector<double> dy(P, 0.0);
+    int cnt = 0;
+    double loss = 0.0;
+    for (int p = 0; p < P; ++p) {
+        const double t = target_m[p];
+        if (!n.valid[p] || !(t > 0.0) || !(fabs(n.x[2 * p] - t) < outlier_m)) continue;
+        ++cnt;
+    }
+    for (int p = 0; p < P; ++p) {
+        const double t = target_m[p];
+        if (!n.valid[p] || !(t > 0.0) || !(fabs(n.x[2 * p] - t) < outlier_m)) continue;
+        const double e = n.y[p] - t;
+        loss += e * e;
+        dy[p] = n.bf16 ? (double)(2.0f * (float)e) : 2.0 * e;
+    }
+    if (count) *count = cnt;
+    if (cnt == 0) {
+        memset(grad, 0, sizeof(double) * layer_offset(4));
+        return 0.0;
+    }
+    loss /= cnt;
+    memset(grad, 0, sizeof(double) * layer_offset(4));
+    // dz of the output layer: dy on valid pixels (y = x0 + z).
+    std::vector<double> dz(dy);
+    for (int p = 0; p < P; ++p)
+        if (!n.valid[p]) dz[p] = 0.0;
+    for (int l = 3; l >= 0; --l) {
+        const int cin = kLayerCin[l], cout = kLayerCout[l], off = layer_offset(l);
+        const std::vector<double>* src = l == 0 ? nullptr : &n.a[l - 1];
+        // parameter gradients
+        for (int py = 0; py < H; ++py)
+            for (int px = 0; px < W; ++px)
+                for (int co = 0; co < cout; ++co) {
+                    const double g = dz[(size_t)(py * W + px) * cout + co];
+                    if (g == 0.0) continue;
+                    grad[off + cout * cin * 9 + co] += g;
+                    for (int ci = 0; ci < cin; ++ci)
+                        for (int ky = 0; ky < 3; ++ky)
+                            for (int kx = 0; kx < 3; ++kx)
+                                grad[off + ((co * cin + ci) * 3 + ky) * 3 + kx] +=
+                                    g * in_val(n, l, src, px + kx - 1, py + ky - 1, ci);
+                }
+        if (l == 0) break;
+        // input gradient, through the ReLU of the previous layer
+        std::vector<double> da((size_t)P * cin, 0.0);
+        for (int py = 0; py < H; ++py)
+            for (int px = 0; px < W; ++px)
+                for (int co = 0; co < cout; ++co) {
+                    const double g = dz[(size_t)(py * W + px) * cout + co];
+                    if (g == 0.0) continue;
+                    for (int ky = 0; ky < 3; ++ky)
+                        for (int kx = 0; kx < 3; ++kx) {
+                            const int qx = px + kx - 1, qy = py + ky - 1;
+                            if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+                            for (int ci = 0; ci < cin; ++ci)
+                                da[(size_t)(qy * W + qx) * cin + ci] += g * n.w[off + ((co * cin + ci) * 3 + ky) * 3 + kx];
+                        }
+                }
+        const std::vector<double>& a = n.a[l - 1];
+        for (size_t i = 0; i < da.size(); ++i) {
+            da[i] = a[i] > 0.0 ? da[i] : 0.0;
+            if (n.bf16) da[i] = bf16_round(da[i]);
+        }
+        dz.swap(da);
+    }
+    for (int i = 0; i < layer_offset(4); ++i) grad[i] /= cnt;
+    return loss;
+}

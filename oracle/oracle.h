@@ -89,6 +89,21 @@ void orc_estimate_combined_f32(const kfx_xyz* vc, const kfx_xyz* nc, const kfx_x
 /* Cholesky solve + KinectFusion pose increment in double; 1 if singular. */
 int orc_icp_solve_update(const double sums[27], double* R, double* t, double x_out[6]);
 
+/* clean_color_slice.cl:41-67 pattern (TSDF or colour volume, elem_bytes 4). */
+void orc_clean_slice(const kfx_point_3i* mn, const kfx_point_3i* mx, const kfx_round_buffer* rb, void* vol,
+                     int elem_bytes);
+/* Extract_Slice V+C (opencl_wrappers.cpp:856-964 with its two kernels). */
+uint64_t orc_extract_slice(const kfx_point_3i* mn, const kfx_point_3i* mx, const kfx_round_buffer* rb,
+                           const kfx_tsdf* vol, const kfx_color* col, kfx_xyzi* pts, kfx_color* cols,
+                           uint64_t capacity);
+
+/* Learned denoiser (no reference counterpart): forward and loss gradient in
+ * double precision, optionally emulating the GPU's bfloat16 storage. */
+int orc_dn_param_count(void);
+void orc_dn_forward(const uint16_t* depth, int W, int H, const float* params, int bf16, double* y, double* hidden);
+double orc_dn_loss_grad(const uint16_t* depth, const float* target_m, int W, int H, const float* params, int bf16,
+                        double outlier_m, double* grad, int* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,12 @@ _PROTOS = {
     "orc_icp_sums": (_I, [_P, _P, _P, _P, PD, PD, PD, PD, IN, C.c_double, C.c_double, _I, PD]),
     "orc_estimate_combined_f32": (None, [_P, _P, _P, _P, _P, _P, _P, _P, IN, C.c_float, C.c_float, _P]),
     "orc_icp_solve_update": (_I, [PD, PD, PD, PD]),
+    "orc_dn_param_count": (_I, []),
+    "orc_dn_forward": (None, [_P, _I, _I, _P, _I, _P, _P]),
+    "orc_dn_loss_grad": (C.c_double, [_P, _P, _I, _I, _P, _I, C.c_double, _P, C.POINTER(_I)]),
+    "orc_clean_slice": (None, [C.POINTER(kfx.Point3i), C.POINTER(kfx.Point3i), RB, _P, _I]),
+    "orc_extract_slice": (C.c_uint64, [C.POINTER(kfx.Point3i), C.POINTER(kfx.Point3i), RB, _P, _P, _P, _P,
+                                       C.c_uint64]),
 }
 
 
@@ -208,3 +214,43 @@ def unpack_sums(s):
                 A[i, j] = A[j, i] = s[k]
             k += 1
     return A, b
+
+
+COLOR_DTYPE = np.dtype([("r", "u1"), ("g", "u1"), ("b", "u1"), ("w", "u1")])
+
+
+def clean_slice(vol, mn, mx, rb):
+    lib.orc_clean_slice(C.byref(kfx.point3i(mn)), C.byref(kfx.point3i(mx)), C.byref(rb), ptr(vol), vol.dtype.itemsize)
+
+
+def extract_slice(vol, col, mn, mx, rb):
+    """-> (points float32 [n,4], colors uint8 [n,4])."""
+    n = lib.orc_extract_slice(C.byref(kfx.point3i(mn)), C.byref(kfx.point3i(mx)), C.byref(rb), ptr(vol),
+                              ptr(col) if col is not None else None, None, None, 0)
+    pts = np.zeros((n, 4), np.float32)
+    cols = np.zeros((n, 4), np.uint8)
+    lib.orc_extract_slice(C.byref(kfx.point3i(mn)), C.byref(kfx.point3i(mx)), C.byref(rb), ptr(vol),
+                          ptr(col) if col is not None else None, ptr(pts), ptr(cols), n)
+    return pts, cols
+
+
+def dn_forward(depth, params, bf16=True, hidden=False):
+    depth = np.ascontiguousarray(depth, np.uint16)
+    H, W = depth.shape
+    params = np.ascontiguousarray(params, np.float32)
+    y = np.zeros((H, W), np.float64)
+    hid = np.zeros((3, H, W, 16), np.float64) if hidden else None
+    lib.orc_dn_forward(ptr(depth), W, H, ptr(params), int(bf16), ptr(y), ptr(hid) if hidden else None)
+    return (y, hid) if hidden else y
+
+
+def dn_loss_grad(depth, target, params, bf16=True, outlier=0.05):
+    depth = np.ascontiguousarray(depth, np.uint16)
+    H, W = depth.shape
+    target = np.ascontiguousarray(target, np.float32)
+    params = np.ascontiguousarray(params, np.float32)
+    g = np.zeros(params.size, np.float64)
+    n = C.c_int()
+    loss = lib.orc_dn_loss_grad(ptr(depth), ptr(target), W, H, ptr(params), int(bf16), float(outlier), ptr(g),
+                                C.byref(n))
+    return loss, g, n.value

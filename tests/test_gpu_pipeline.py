@@ -1,0 +1,93 @@
+"""End-to-end pipeline on the GPU: plain (reference control flow) and fused
+modes track the C1 orbit, agree with each other, the graph replay equals the
+eager fused path bit for bit, and the cyclic shift keeps tracking while
+extracting the leaving slab."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(kfx, pipe, depths):
+    dd = kfx.Buffer.from_array(depths)
+    return [pipe.process(C.c_void_p(dd.ptr.value + i * depths[0].nbytes)) for i in range(len(depths))]
+
+
+def _c1_run(kfx, mode, use_graph=0, frames=30, n=256, **kw):
+    # The C1 orbit keeps the camera 1.5 m from the cube centre: no shifts.
+    intr = kfx.kinect_intrinsics()
+    scene, poses = scenes.c1_scene(), scenes.c1_poses(frames)
+    depths = np.stack([kfx.render(scene, poses[i], intr, scenes.C1_SEED, i, noise=True) for i in range(frames)])
+    kw.setdefault("shift_threshold", 100.0)
+    cfg = kfx.default_config(intr=intr, voxels=kfx.Point3i(n, n, n), volume_size=kfx.Point3f(3.0, 3.0, 3.0),
+                             mode=mode, use_graph=use_graph, **kw)
+    pipe = kfx.Pipeline(cfg, poses[0], (0.0, 0.0, 0.0))
+    return pipe, _run(kfx, pipe, depths), poses
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_pipeline_tracks_c1(gpu, mode):
+    kfx = gpu
+    pipe, infos, gt = _c1_run(kfx, mode)
+    est = [i.pose_world for i in infos]
+    ate = kfx.ate_rmse(est, gt)
+    print(f"mode {mode}: ATE {ate * 1000:.2f} mm over {len(gt)} frames")
+    assert ate < 0.01
+    assert all(i.icp_singular == 0 for i in infos[1:])
+    assert all(i.icp_iterations == 19 for i in infos[1:])
+    pipe.close()
+
+
+def test_fused_matches_plain_and_graph_is_exact(gpu):
+    kfx = gpu
+    p0, i0, gt = _c1_run(kfx, 0, frames=12)
+    p1, i1, _ = _c1_run(kfx, 1, frames=12)
+    p2, i2, _ = _c1_run(kfx, 1, use_graph=1, frames=12)
+    for a, b, c in zip(i0, i1, i2):
+        ta, tb = np.array(a.pose_world.t[:]), np.array(b.pose_world.t[:])
+        assert np.linalg.norm(ta - tb) < 5e-4  # FP32 host-unpacked sums vs compensated/FP64
+        assert np.array_equal(np.array(b.pose_world.t[:]), np.array(c.pose_world.t[:]))
+        assert np.array_equal(np.array(b.pose_world.R[:]), np.array(c.pose_world.R[:]))
+    for p in (p0, p1, p2):
+        p.close()
+
+
+def test_early_exit_reduces_iterations(gpu):
+    kfx = gpu
+    pipe, infos, gt = _c1_run(kfx, 1, frames=15, converge_rotation=1e-5, converge_translation=1e-5)
+    its = [i.icp_iterations for i in infos[1:]]
+    assert max(its) <= 19 and np.mean(its) < 19
+    assert kfx.ate_rmse([i.pose_world for i in infos], gt) < 0.01
+    pipe.close()
+
+
+@pytest.mark.parametrize("mode,use_graph", [(0, 0), (1, 1)])
+def test_cyclic_shift_keeps_tracking(gpu, mode, use_graph):
+    # Random walk of 2 cm per frame (the 512^3 workload's trajectory) in a
+    # 256^3, 3.072 m cube with a 0.4 m shift threshold: several shifts in 80
+    # frames, tracking continues across them.
+    kfx = gpu
+    intr = kfx.kinect_intrinsics()
+    scene = kfx.make_scene(2, 10, 10, (-0.5, 7.5, -1.8, 1.8), 0.0)
+    gt = kfx.walk(2, (0.0, -1.2, 0.0), 0.02, 0.35, 80)
+    origin = (gt[0].t[0] - 1.536, -2.5, gt[0].t[2] - 1.536)
+    depths = np.stack([kfx.render(scene, gt[i], intr, 2, i, noise=True) for i in range(80)])
+    cfg = kfx.default_config(intr=intr, voxels=kfx.Point3i(256, 256, 256),
+                             volume_size=kfx.Point3f(3.072, 3.072, 3.072), mode=mode, use_graph=use_graph,
+                             shift_threshold=0.4)
+    pipe = kfx.Pipeline(cfg, gt[0], origin)
+    infos = _run(kfx, pipe, depths)
+    v = pipe.view()
+    shifted = [i for i in infos if i.shifted]
+    assert len(shifted) >= 1 and v.shifts == len(shifted)
+    assert v.extracted_points > 0
+    o = v.round_buffer.origin_GRID
+    assert (o.x, o.y, o.z) != (0, 0, 0)
+    ate = kfx.ate_rmse([i.pose_world for i in infos], gt)
+    print(f"mode {mode}: {len(shifted)} shifts, ATE {ate * 1000:.2f} mm")
+    assert ate < 0.02
+    pipe.close()
