@@ -6,6 +6,8 @@
 // milliseconds of host time and every caller (tests, benchmark, denoiser
 // training) sees bit-identical inputs for a given seed.
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -207,6 +209,105 @@ int kfx_synth_render(const kfx_synth_scene* sc, const kfx_pose* pose, const kfx_
         for (int k = 0; k < nt; ++k) pool.emplace_back(rows, H * k / nt, H * (k + 1) / nt);
         for (auto& th : pool) th.join();
     }
+    return KFX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Trajectory file and error metrics.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+// Text format of the reference's write_isometry_to_file
+// (common/src/string_utils.cpp:26-74): per pose, its index on one line, the
+// three rows of R as "axbxc " (single precision, 6 significant digits), then
+// t the same way.
+int kfx_trajectory_write(const char* path, const kfx_pose* poses, int n) {
+    if (!path || (n > 0 && !poses) || n < 0) return KFX_ERR_INVALID_ARG;
+    FILE* f = fopen(path, "w");
+    if (!f) return KFX_ERR_INVALID_ARG;
+    for (int i = 0; i < n; ++i) {
+        fprintf(f, "%d\n", i);
+        for (int r = 0; r < 3; ++r)
+            fprintf(f, "%gx%gx%g %s\n", (float)poses[i].R[3 * r], (float)poses[i].R[3 * r + 1],
+                    (float)poses[i].R[3 * r + 2], r == 2 ? " " : "");
+        fprintf(f, "%gx%gx%g \n", (float)poses[i].t[0], (float)poses[i].t[1], (float)poses[i].t[2]);
+    }
+    return fclose(f) == 0 ? KFX_OK : KFX_ERR_INVALID_ARG;
+}
+
+// Reads a file in that format; returns the number of poses read (at most
+// max_poses) in *n.  As the reference's stream operators do, the 'x'
+// delimiters are replaced by blanks before parsing (a bare "0x..." would
+// otherwise read as a hexadecimal number).
+int kfx_trajectory_read(const char* path, kfx_pose* poses, int max_poses, int* n) {
+    if (!path || !n || (max_poses > 0 && !poses)) return KFX_ERR_INVALID_ARG;
+    FILE* f = fopen(path, "r");
+    if (!f) return KFX_ERR_INVALID_ARG;
+    int count = 0;
+    char line[512];
+    std::vector<double> vals;
+    while (count < max_poses && fgets(line, sizeof(line), f)) {
+        for (char* c = line; *c; ++c)
+            if (*c == 'x') *c = ' ';
+        char* p = line;
+        char* end = nullptr;
+        for (double v = strtod(p, &end); end != p; v = strtod(p, &end)) {
+            vals.push_back(v);
+            p = end;
+        }
+        if (vals.size() == 13) {  // index, 9 rotation entries, 3 translation entries
+            for (int k = 0; k < 9; ++k) poses[count].R[k] = vals[1 + k];
+            for (int k = 0; k < 3; ++k) poses[count].t[k] = vals[10 + k];
+            ++count;
+            vals.clear();
+        }
+    }
+    fclose(f);
+    *n = count;
+    return KFX_OK;
+}
+
+// ATE: RMSE of camera positions (no alignment: both trajectories share the
+// first frame's world frame).  RPE over consecutive frames: RMSE of the
+// relative translation error (m) and mean relative rotation error (rad).
+int kfx_trajectory_errors(const kfx_pose* est, const kfx_pose* gt, int n, double* ate, double* rpe_trans,
+                          double* rpe_rot) {
+    if (!est || !gt || n <= 0) return KFX_ERR_INVALID_ARG;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const double d = est[i].t[k] - gt[i].t[k];
+            s += d * d;
+        }
+    if (ate) *ate = sqrt(s / n);
+    double st = 0.0, sr = 0.0;
+    int m = 0;
+    for (int i = 1; i < n; ++i) {
+        // relative motions D = P_{i-1}^-1 P_i for both, error E = Dgt^-1 Dest
+        double De[12], Dg[12];
+        const kfx_pose* P[2][2] = {{&est[i - 1], &est[i]}, {&gt[i - 1], &gt[i]}};
+        double* D[2] = {De, Dg};
+        for (int w = 0; w < 2; ++w) {
+            const kfx_pose &a = *P[w][0], &b = *P[w][1];
+            for (int r = 0; r < 3; ++r) {
+                for (int c = 0; c < 3; ++c)
+                    D[w][3 * r + c] = a.R[r] * b.R[c] + a.R[3 + r] * b.R[3 + c] + a.R[6 + r] * b.R[6 + c];
+                D[w][9 + r] = a.R[r] * (b.t[0] - a.t[0]) + a.R[3 + r] * (b.t[1] - a.t[1]) + a.R[6 + r] * (b.t[2] - a.t[2]);
+            }
+        }
+        double tr = 0.0, et[3];
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) tr += Dg[3 * c + r] * De[3 * c + r];  // trace(Dg^T De)
+            et[r] = Dg[r] * (De[9] - Dg[9]) + Dg[3 + r] * (De[10] - Dg[10]) + Dg[6 + r] * (De[11] - Dg[11]);
+        }
+        st += et[0] * et[0] + et[1] * et[1] + et[2] * et[2];
+        sr += acos(std::max(-1.0, std::min(1.0, 0.5 * (tr - 1.0))));
+        ++m;
+    }
+    if (rpe_trans) *rpe_trans = m ? sqrt(st / m) : 0.0;
+    if (rpe_rot) *rpe_rot = m ? sr / m : 0.0;
     return KFX_OK;
 }
 

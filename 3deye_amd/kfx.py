@@ -249,6 +249,10 @@ _PROTOS = {
     "kfx_denoiser_adam_step": (_I, [_P, C.c_float, _I]),
     "kfx_model_depth": (_I, [_P, C.POINTER(Pose), _I, _I, _P]),
     "kfx_depth_consistency": (_I, [_P, _P, _P, C.POINTER(Intrinsics), C.c_float, _P]),
+    "kfx_trajectory_write": (_I, [C.c_char_p, C.POINTER(Pose), _I]),
+    "kfx_trajectory_read": (_I, [C.c_char_p, C.POINTER(Pose), _I, C.POINTER(_I)]),
+    "kfx_trajectory_errors": (_I, [C.POINTER(Pose), C.POINTER(Pose), _I, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "kfx_splitmix64": (C.c_uint64, [C.c_uint64]),
     "kfx_synth_make_scene": (_I, [C.c_uint64, _I, _I, C.POINTER(C.c_double), C.c_double, C.POINTER(SynthScene)]),
     "kfx_synth_orbit": (_I, [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.c_double, _I,
@@ -462,8 +466,16 @@ def default_config(**kw):
     return cfg
 
 
+def trajectory_errors(est, gt):
+    """(ATE, RPE translation, RPE rotation) of two pose lists with a common
+    first frame (kfx_trajectory_errors)."""
+    n = min(len(est), len(gt))
+    a, b = (Pose * n)(*est[:n]), (Pose * n)(*gt[:n])
+    ate, rt, rr = C.c_double(), C.c_double(), C.c_double()
+    check(lib.kfx_trajectory_errors(a, b, n, C.byref(ate), C.byref(rt), C.byref(rr)))
+    return ate.value, rt.value, rr.value
+
+
 def ate_rmse(est, gt):
-    """Absolute trajectory error (RMSE of camera positions, metres) of two
-    pose lists with a common first frame (no alignment)."""
-    d = [np.array(e.t[:]) - np.array(g.t[:]) for e, g in zip(est, gt)]
-    return float(np.sqrt(np.mean([np.dot(v, v) for v in d])))
+    """Absolute trajectory error (RMSE of camera positions, metres)."""
+    return trajectory_errors(est, gt)[0]

@@ -443,6 +443,17 @@ int kfx_synth_render(const kfx_synth_scene* scene, const kfx_pose* pose, const k
                      uint64_t seed, int frame, int flags, int threads, uint16_t* depth_mm, float* clean_m);
 #define KFX_SYNTH_NOISE 1
 
+/* Trajectory text file in the reference's format (write_isometry_to_file,
+ * common/src/string_utils.cpp:26-74): per pose its index, the three rows of
+ * R and then t, each as "axbxc " in single precision. */
+int kfx_trajectory_write(const char* path, const kfx_pose* poses, int n);
+int kfx_trajectory_read(const char* path, kfx_pose* poses, int max_poses, int* n);
+/* ATE (RMSE of positions, common first frame) and RPE between consecutive
+ * frames (RMSE of relative translation error in m, mean relative rotation
+ * error in rad); any output may be NULL. */
+int kfx_trajectory_errors(const kfx_pose* est, const kfx_pose* gt, int n, double* ate, double* rpe_trans,
+                          double* rpe_rot);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -102,3 +102,26 @@ def test_synth_noise_statistics(kfx):
     sigma = 1.2 + 1.9 * (2.0 - 0.4) ** 2
     assert abs(valid.mean() - 2000.0) < 0.2
     assert abs(valid.std() - np.sqrt(sigma ** 2 + 1 / 12.0)) < 0.1
+
+
+def test_trajectory_file_round_trip_and_errors(kfx, tmp_path):
+    gt = kfx.orbit((1.5, 2.0, 1.5), 1.5, -0.6, -0.3, 0.02, 20)
+    path = str(tmp_path / "traj.txt").encode()
+    kfx.call("kfx_trajectory_write", path, (kfx.Pose * 20)(*gt), 20)
+    text = open(path.decode()).read().splitlines()
+    assert text[0] == "0" and text[1].count("x") == 2  # reference layout: index, 3 rows, t
+    back = (kfx.Pose * 30)()
+    n = C.c_int()
+    kfx.call("kfx_trajectory_read", path, back, 30, C.byref(n))
+    assert n.value == 20
+    for a, b in zip(back[:20], gt):
+        assert np.allclose(a.R[:], b.R[:], atol=1e-5) and np.allclose(a.t[:], b.t[:], atol=1e-5)
+    # errors: identical -> 0; a constant 1 cm offset after frame 10 -> ATE, RPE closed forms
+    ate0, rt0, rr0 = kfx.trajectory_errors(list(gt), list(gt))
+    assert ate0 == 0.0 and rt0 < 1e-15 and rr0 < 1e-7  # acos near 1 resolves ~1e-8 rad
+    est = (kfx.Pose * 20)(*gt)
+    for i in range(10, 20):
+        est[i].t[0] += 0.01
+    ate, rt, rr = kfx.trajectory_errors(list(est), list(gt))
+    assert abs(ate - 0.01 * np.sqrt(10 / 20)) < 1e-12
+    assert abs(rt - 0.01 * np.sqrt(1 / 19)) < 1e-9 and rr < 1e-7
