@@ -81,6 +81,17 @@ __host__ __device__ __forceinline__ uint32_t cyclic_address(int x, int y, int z,
 // for local coordinates in [0, n) and origins normalised into [0, n).
 __host__ __device__ __forceinline__ int wrap_once(int v, int n) { return v >= n ? v - n : v; }
 
+// cyclic_address for local coordinates in [0, N) per axis with the volume
+// kernels' preconditions (origin_GRID in [0, N), start_index == 0, checked on
+// the host): equal to cyclic_address there, without integer division.
+__host__ __device__ __forceinline__ uint32_t cyclic_address_in_range(int x, int y, int z,
+                                                                     const kfx_round_buffer& rb) {
+    const uint32_t rx = (uint32_t)wrap_once(x + rb.origin_GRID.x, rb.voxels_size.x);
+    const uint32_t ry = (uint32_t)wrap_once(y + rb.origin_GRID.y, rb.voxels_size.y);
+    const uint32_t rz = (uint32_t)wrap_once(z + rb.origin_GRID.z, rb.voxels_size.z);
+    return ((uint32_t)rb.voxels_size.y * rz + ry) * (uint32_t)rb.voxels_size.x + rx;
+}
+
 // Voxel codec (raycast.cl:9-27): decode value / 32767, encode rtz then clamp.
 __device__ __forceinline__ float tsdf_decode(int16_t v) {
     return (float)v / (float)KFX_MAX_DEPTH_VALUE;

@@ -287,14 +287,19 @@ int run_part(kfx_pipeline* p, int (*body)(kfx_pipeline*), hipGraph_t* graph, hip
     return KFX_OK;
 }
 
-// Cyclic shift: when the camera is more than shift_threshold from the cube's
-// centre, move the cube by whole voxels so its centre comes to the camera.
-// The slab leaving the cube (black zone, in unwrapped grid coordinates) is
-// extracted, then cleared; origin_GRID advances modulo N; the pose and the
-// world origin are re-expressed.  Voxels keep their memory cells.
+// Cyclic shift (KinectFusion large-scale convention): the tracked point is
+// the camera's target, shift_target_distance metres along its optical axis
+// (the camera centre itself when 0).  When it is more than shift_threshold
+// from the cube's centre, the cube moves by whole voxels so that its centre
+// comes to the target.  The slab leaving the cube (black zone, in unwrapped
+// grid coordinates) is extracted, then cleared; origin_GRID advances modulo
+// N; the pose and the world origin are re-expressed.  Voxels keep their
+// memory cells.
 int maybe_shift(kfx_pipeline* p, kfx_frame_info* info) {
     const double cx = 0.5 * p->rb.volume_size.x, cy = 0.5 * p->rb.volume_size.y, cz = 0.5 * p->rb.volume_size.z;
-    const double dx = p->t[0] - cx, dy = p->t[1] - cy, dz = p->t[2] - cz;
+    const double f = p->cfg.shift_target_distance;
+    const double tx = p->t[0] + f * p->R[2], ty = p->t[1] + f * p->R[5], tz = p->t[2] + f * p->R[8];
+    const double dx = tx - cx, dy = ty - cy, dz = tz - cz;
     if (sqrt(dx * dx + dy * dy + dz * dz) <= p->cfg.shift_threshold) return KFX_OK;
     const int n[3] = {p->rb.voxels_size.x, p->rb.voxels_size.y, p->rb.voxels_size.z};
     const double cs[3] = {p->cell.x, p->cell.y, p->cell.z};
@@ -371,6 +376,7 @@ int kfx_pipeline_default_config(kfx_pipeline_config* c) {
     c->denoiser = 0;
     c->fallback_rms_mm = 15.0f;
     c->shift_threshold = KFX_DEFAULT_SHIFT_THRESHOLD;
+    c->shift_target_distance = 1.5;
     c->integrate_threshold = 0.001f;
     c->converge_rotation = 0.0;
     c->converge_translation = 0.0;

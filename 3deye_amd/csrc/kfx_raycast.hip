@@ -44,9 +44,17 @@ __device__ __forceinline__ f3 normalized3(f3 a) {
     return f3{a.x * rs, a.y * rs, a.z * rs};
 }
 
+// Every read below is at coordinates inside [0, N) on each axis: march
+// samples pass the bounds test first, the first sample is clamped, and
+// trilinear taps lie in [g-1, g+1] with g in [1, N-2].
+__device__ __forceinline__ int16_t read_raw(const kfx_tsdf* __restrict__ vol, const kfx_round_buffer& rb,
+                                            int x, int y, int z) {
+    return vol[cyclic_address_in_range(x, y, z, rb)].value;
+}
+
 __device__ __forceinline__ float read_tsdf(const kfx_tsdf* __restrict__ vol, const kfx_round_buffer& rb,
                                            int x, int y, int z) {
-    return tsdf_decode(vol[cyclic_address(x, y, z, rb)].value);
+    return tsdf_decode(read_raw(vol, rb, x, y, z));
 }
 
 // floor(p / cell) with saturation (convert_int_sat_rtn).
@@ -148,19 +156,22 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
         gx = max(0, min(gx, nxv - 1));
         gy = max(0, min(gy, nyv - 1));
         gz = max(0, min(gz, nzv - 1));
-        float tsdf = read_tsdf(vol, rb, gx, gy, gz);
+        // The march tests only the signs of the samples; the sign of
+        // value / 32767 is the sign of the stored int16, so the samples stay
+        // undecoded here.
+        int tsdf = read_raw(vol, rb, gx, gy, gz);
         const float max_time = 3.0f * (vol_m.x + vol_m.y + vol_m.z);
         const float step = p.time_step;
 
         for (; time_curr < max_time; time_curr += step) {
-            const float tsdf_prev = tsdf;
+            const int tsdf_prev = tsdf;
             const float t_next = time_curr + step;
             const f3 q1 = ray_at(start, dir, t_next);
             const int vx = voxel_floor(q1.x, p.csx), vy = voxel_floor(q1.y, p.csy), vz = voxel_floor(q1.z, p.csz);
             if (!(vx >= 0 && vy >= 0 && vz >= 0 && vx < nxv && vy < nyv && vz < nzv)) break;
-            tsdf = read_tsdf(vol, rb, vx, vy, vz);
-            if (tsdf_prev < 0.0f && tsdf > 0.0f) break;
-            if (tsdf_prev > 0.0f && tsdf < 0.0f) {
+            tsdf = read_raw(vol, rb, vx, vy, vz);
+            if (tsdf_prev < 0 && tsdf > 0) break;
+            if (tsdf_prev > 0 && tsdf < 0) {
                 const float ftdt = interpolate_trilinear(vol, q1, p);
                 if (isnan(ftdt)) break;
                 const f3 q2 = ray_at(start, dir, time_curr);

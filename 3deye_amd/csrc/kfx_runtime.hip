@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -83,6 +84,14 @@ __global__ void fill_u8_kernel(uint8_t* __restrict__ dst, uint32_t word, size_t 
         reinterpret_cast<uint4*>(dst)[i] = w4;
     for (size_t i = n16 * 16 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < bytes; i += stride)
         dst[i] = (uint8_t)(word & 0xffu);
+}
+
+// Streaming copy (16-byte vector loads and stores, grid-stride): the
+// reference point for achievable HBM bandwidth and for calibrating the
+// profiler's FETCH_SIZE / WRITE_SIZE counters.
+__global__ void __launch_bounds__(256) copy16_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
 }  // namespace kfx
@@ -218,6 +227,22 @@ int kfx_profile_query(const char* kernel, double* total_ms, uint64_t* launches) 
     }
     *total_ms = sum;
     *launches = n;
+    return KFX_OK;
+}
+
+int kfx_copy_kernel(void* dev_dst, const void* dev_src, size_t bytes) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(dev_dst && dev_src, "kfx_copy_kernel: null pointer");
+    KFX_REQUIRE(((uintptr_t)dev_dst & 15u) == 0 && ((uintptr_t)dev_src & 15u) == 0 && bytes % 16 == 0,
+                "kfx_copy_kernel: pointers and size must be multiples of 16 bytes");
+    if (bytes == 0) return KFX_OK;
+    ProfScope prof("copy");
+    const size_t n16 = bytes / 16;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    size_t blocks = std::min((n16 + 255) / 256, (size_t)cus * 8);
+    copy16_kernel<<<(unsigned)blocks, 256, 0, g_stream>>>((uint4*)dev_dst, (const uint4*)dev_src, n16);
+    KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
 

@@ -56,9 +56,12 @@ __device__ __forceinline__ int row_ranges(const SliceParams& p, int y, int z, in
     return 1;
 }
 
-__device__ __forceinline__ bool crossing(float F, const kfx_tsdf n) {
-    const float Fn = tsdf_decode(n.value);
-    return (n.weight != 0 && Fn != 1.0f) && ((F > 0.0f && Fn < 0.0f) || (F < 0.0f && Fn > 0.0f));
+// Zero crossing between a voxel of value v and its neighbour n
+// (scan_slice_of_both_volumes.cl:125-160).  With F = value / 32767 decoded,
+// F != 1 is value != 32767 and the signs of F are those of the int16, so the
+// test runs on the stored integers.
+__device__ __forceinline__ bool crossing(int v, const kfx_tsdf n) {
+    return (n.weight != 0 && n.value != KFX_MAX_DEPTH_VALUE) && ((v > 0 && n.value < 0) || (v < 0 && n.value > 0));
 }
 
 __device__ __forceinline__ bool color_set(kfx_color c) { return c.r != 0 || c.g != 0 || c.b != 0 || c.w != 0; }
@@ -67,14 +70,13 @@ __device__ __forceinline__ bool color_set(kfx_color c) { return c.r != 0 || c.g 
 __device__ __forceinline__ unsigned predicates(const SliceParams& p, const kfx_tsdf* __restrict__ vol,
                                                const kfx_color* __restrict__ col, int x, int y, int z, int chunk_end) {
     unsigned m = 0;
-    const kfx_tsdf v = vol[cyclic_address(x, y, z, p.rb)];
-    const float F = tsdf_decode(v.value);
-    if (v.weight != 0 && F != 1.0f) {
-        if (x + 1 < p.rb.voxels_size.x && crossing(F, vol[cyclic_address(x + 1, y, z, p.rb)])) m |= 1u;
-        if (y + 1 < p.rb.voxels_size.y && crossing(F, vol[cyclic_address(x, y + 1, z, p.rb)])) m |= 2u;
-        if (z + 1 < chunk_end && crossing(F, vol[cyclic_address(x, y, z + 1, p.rb)])) m |= 4u;
+    const kfx_tsdf v = vol[cyclic_address_in_range(x, y, z, p.rb)];
+    if (v.weight != 0 && v.value != KFX_MAX_DEPTH_VALUE) {
+        if (x + 1 < p.rb.voxels_size.x && crossing(v.value, vol[cyclic_address_in_range(x + 1, y, z, p.rb)])) m |= 1u;
+        if (y + 1 < p.rb.voxels_size.y && crossing(v.value, vol[cyclic_address_in_range(x, y + 1, z, p.rb)])) m |= 2u;
+        if (z + 1 < chunk_end && crossing(v.value, vol[cyclic_address_in_range(x, y, z + 1, p.rb)])) m |= 4u;
     }
-    if (col != nullptr && color_set(col[cyclic_address(x, y, z, p.rb)])) m |= 8u;
+    if (col != nullptr && color_set(col[cyclic_address_in_range(x, y, z, p.rb)])) m |= 8u;
     return m;
 }
 
@@ -236,7 +238,7 @@ __global__ void __launch_bounds__(256) slice_emit_kernel(const kfx_tsdf* __restr
                     const int px = x + (k == 0), py = y + (k == 1), pz = z + (k == 2);
                     out_pts[idx] = kfx_xyzi{(float)px, (float)py, (float)pz, 1.0f};
                     kfx_color c{0, 0, 0, 0};
-                    if (col != nullptr) c = col[cyclic_address(px, py, pz, p.rb)];
+                    if (col != nullptr) c = col[cyclic_address_in_range(px, py, pz, p.rb)];
                     out_col[idx] = c;
                 }
                 base[k] += (uint32_t)__popcll(ballot);
@@ -261,7 +263,7 @@ __global__ void __launch_bounds__(256) clean_slice_kernel(T* __restrict__ vol, S
         z1 = min(nz - 1, p.maxb[2] - p.rb.origin_GRID.z);
     }
     const T zero{};
-    for (int z = z0; z <= z1; ++z) vol[cyclic_address(x, y, z, p.rb)] = zero;
+    for (int z = z0; z <= z1; ++z) vol[cyclic_address_in_range(x, y, z, p.rb)] = zero;
 }
 
 int determine_step(int depth) {

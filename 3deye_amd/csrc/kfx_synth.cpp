@@ -164,7 +164,7 @@ int kfx_synth_walk(uint64_t seed, const double start[3], double step, double pit
         out[i].t[1] = p[1] + 0.03 * sin(0.07 * i);
         out[i].t[2] = p[2];
         rate = 0.9 * rate + 0.1 * 0.05 * rng.normal();
-        heading = std::max(-0.5, std::min(0.5, heading + rate));
+        heading = std::max(-0.3, std::min(0.3, heading + rate));
         p[0] += step * cos(heading);
         p[2] += step * sin(heading);
     }

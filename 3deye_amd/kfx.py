@@ -91,7 +91,7 @@ class PipelineConfig(C.Structure):
                 ("use_graph", C.c_int32), ("denoiser", C.c_int32), ("fallback_rms_mm", C.c_float),
                 ("shift_threshold", C.c_float), ("integrate_threshold", C.c_float),
                 ("converge_rotation", C.c_double), ("converge_translation", C.c_double),
-                ("extract_on_shift", C.c_int32), ("reserved", C.c_int32)]
+                ("extract_on_shift", C.c_int32), ("reserved", C.c_int32), ("shift_target_distance", C.c_double)]
 
 
 class FrameInfo(C.Structure):
@@ -184,6 +184,7 @@ _PROTOS = {
     "kfx_memcpy_dtoh": (_I, [_P, _P, _SZ]),
     "kfx_memcpy_dtod": (_I, [_P, _P, _SZ]),
     "kfx_fill_u8": (_I, [_P, C.c_uint8, _SZ]),
+    "kfx_copy_kernel": (_I, [_P, _P, _SZ]),
     "kfx_event_create": (_I, [C.POINTER(_P)]),
     "kfx_event_destroy": (_I, [_P]),
     "kfx_event_record": (_I, [_P]),

@@ -38,9 +38,10 @@ HELDOUT_SEED_BASE = 5000
 
 def sequence(seed, frames):
     intr = kfx.kinect_intrinsics(W, H, 525.0)
-    scene = kfx.make_scene(seed, 10, 10, (-0.5, 7.5, -1.8, 1.8), 0.0)
+    scene = kfx.make_scene(seed, 10, 10, (0.5, 9.0, -2.0, 2.0), 0.0)
     poses = kfx.walk(seed, (0.0, -1.2, 0.0), 0.02, 0.35, frames)
-    origin = (poses[0].t[0] - VOL_M / 2, -2.5, poses[0].t[2] - VOL_M / 2)
+    R, t = poses[0].matrices()
+    origin = t + 1.5 * R[:, 2] - VOL_M / 2  # cube centred on the first target point
     return intr, scene, poses, origin
 
 

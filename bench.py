@@ -6,8 +6,9 @@ cx=319.5, cy=239.5) rendered from a seeded analytic scene (ground plane, 10
 boxes, 10 spheres, sizes 0.1-0.5 m) along a 300-frame seeded random walk of
 2 cm per frame (6 m), Kinect noise sigma(z) = 1.2 + 1.9 (z - 0.4)^2 mm, 1 mm
 quantisation, 2 % dropout; truncation 0.03 m; ICP 3 levels x 10/5/4;
-cyclic shift when the camera is more than 1.5 m from the cube centre, with
-the leaving slab extracted.  Depth frames are rendered on the host before
+cyclic shift when the camera's target point (1.5 m along its optical axis)
+is more than 1.5 m from the cube centre, with the leaving slab extracted; the
+cube starts centred on the first frame's target point.  Depth frames are rendered on the host before
 timing and are resident in HBM when the timed region starts.
 
 One step = one frame through the pipeline (preprocess, ICP, integrate,
@@ -41,13 +42,19 @@ SCENE_SEED = 2
 N_FRAMES = 300
 
 
+def cube_origin(pose, target_distance=1.5):
+    """World position of the cube's corner voxel such that the cube's centre
+    is the camera's target point, target_distance along its optical axis
+    (where the pipeline re-centres the cube on every cyclic shift)."""
+    R, t = pose.matrices()
+    return t + target_distance * R[:, 2] - VOL_M / 2
+
+
 def workload_inputs(n_frames):
     intr = kfx.kinect_intrinsics(W, H, 525.0)
-    scene = kfx.make_scene(SCENE_SEED, 10, 10, (-0.5, 7.5, -1.8, 1.8), 0.0)
+    scene = kfx.make_scene(SCENE_SEED, 10, 10, (0.5, 9.0, -2.0, 2.0), 0.0)
     poses = kfx.walk(SCENE_SEED, (0.0, -1.2, 0.0), 0.02, 0.35, n_frames)
-    # Cube placement: the first camera at the centre of the cube in x and z,
-    # the ground plane 2.5 m below the cube's top face.
-    origin = np.array([poses[0].t[0] - VOL_M / 2, -2.5, poses[0].t[2] - VOL_M / 2])
+    origin = cube_origin(poses[0])
     depths = np.stack([kfx.render(scene, poses[i], intr, SCENE_SEED, i, noise=True) for i in range(n_frames)])
     return intr, scene, poses, origin, depths
 
@@ -130,6 +137,48 @@ def cpu_baseline(intr, depths, poses, origin, frames=12):
                       f"19 ICP iterations, integrate, raycast), first {frames} frames of the 512^3 workload"}
 
 
+def kernel_probe(frames=20):
+    """Short, fixed workload for hardware-counter runs (rocprofv3 --pmc):
+    three 256 MiB streaming copies (known traffic, for calibrating the
+    FETCH_SIZE / WRITE_SIZE counters), then `frames` integrations of the
+    512^3 workload at the ground-truth poses, then their raycasts."""
+    intr, scene, poses, origin, depths = workload_inputs(frames)
+    kfx.init(0)
+    nbytes = 256 << 20
+    a, b = kfx.Buffer(nbytes), kfx.Buffer(nbytes)
+    a.fill(1)
+    for _ in range(3):
+        kfx.call("kfx_copy_kernel", b.ptr, a.ptr, nbytes)
+    kfx.synchronize()
+    rb = kfx.round_buffer(N_VOX, VOL_M)
+    cs = kfx.cell_size(rb)
+    vol = kfx.Buffer(rb.buffer_length * 4)
+    kfx.call("kfx_init_tsdf", vol.ptr, C.byref(rb))
+    dd = kfx.Buffer.from_array(depths)
+    scaled = kfx.Buffer(W * H * 4)
+    vm, nm = kfx.Buffer(W * H * 12), kfx.Buffer(W * H * 12)
+    n_upd = 0
+    for i in range(frames):
+        R, t = poses[i].matrices()
+        tv = t - origin
+        dp = C.c_void_p(dd.ptr.value + i * depths[0].nbytes)
+        kfx.call("kfx_integrate_tsdf", dp, scaled.ptr, vol.ptr, C.byref(intr), C.byref(kfx.matrix3f(R.T)),
+                 C.byref(kfx.point3f(tv)), TRUNC, C.byref(cs), C.byref(rb))
+        kfx.call("kfx_raycast", vol.ptr, vm.ptr, nm.ptr, C.byref(rb), C.byref(kfx.point3f(tv)),
+                 C.byref(kfx.matrix3f(R)), TRUNC, C.byref(intr), C.byref(cs))
+    kfx.synchronize()
+    for i in range(frames):
+        R, t = poses[i].matrices()
+        n = C.c_uint64()
+        dp = C.c_void_p(dd.ptr.value + i * depths[0].nbytes)
+        kfx.call("kfx_integrate_count_updates", dp, scaled.ptr, C.byref(intr), C.byref(kfx.matrix3f(R.T)),
+                 C.byref(kfx.point3f(t - origin)), TRUNC, C.byref(cs), C.byref(rb), C.byref(n))
+        n_upd += n.value
+    print(json.dumps({"probe": "copy 3 x 256 MiB; integrate + raycast", "frames": frames,
+                      "copy_bytes_per_launch": nbytes, "n_upd_per_frame": n_upd / frames,
+                      "integrate_alg_bytes_per_frame": (8.0 * n_upd + 2.0 * W * H * frames) / frames}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,7 +186,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="short fixed workload for hardware-counter runs (no JSON benchmark line)")
     args = ap.parse_args()
+    if args.kernel_probe:
+        kernel_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the pipeline runs on a single MI355X "
               "(multi-GPU sharding is a design note, not built).", file=sys.stderr)
@@ -206,7 +260,7 @@ def main():
         "data": "synthetic (seeded analytic scene, Kinect noise model)",
         "config": {"workload": "512^3 TSDF (6 mm), 640x480 depth, 300-frame 6 m random walk with cyclic shifts",
                    "pipeline": "fused: device-resident pose, FP32-compensated ICP sums with FP64 solve, "
-                               "one HIP graph per frame; bilateral filter (denoiser not yet wired)",
+                               "one HIP graph per frame; bilateral filter",
                    "frames": N_FRAMES, "icp_iterations": [10, 5, 4]},
         "fused": {"frames_per_s": fps, "ate_m": ate_f, "shifts": v_f.shifts,
                   "extracted_points": int(v_f.extracted_points)},

@@ -59,6 +59,10 @@ int kfx_memcpy_dtoh(void* host_dst, const void* dev_src, size_t bytes);
 /* Asynchronous device-to-device copy and byte fill on the library stream. */
 int kfx_memcpy_dtod(void* dev_dst, const void* dev_src, size_t bytes);
 int kfx_fill_u8(void* dev_dst, uint8_t value, size_t bytes);
+/* Device-to-device copy by a streaming kernel (16-byte aligned pointers, size
+ * a multiple of 16): the achievable-bandwidth reference for the roofline and
+ * the calibration of the HBM traffic counters. */
+int kfx_copy_kernel(void* dev_dst, const void* dev_src, size_t bytes);
 
 /* Events on the library stream, for timing (replaces the OpenCL event timing
  * of opencl_helper::execute_kernel, common/src/opencl_helper.cpp:675-685). */
@@ -341,12 +345,13 @@ typedef struct kfx_pipeline_config {
     int32_t use_graph;            /* fused mode: replay each frame as one HIP graph */
     int32_t denoiser;             /* 1: learned denoiser instead of the bilateral filter */
     float fallback_rms_mm;        /* denoiser fallback threshold, 15 mm */
-    float shift_threshold;        /* cyclic shift trigger, 1.5 m from the cube centre */
+    float shift_threshold;        /* cyclic shift trigger: target point more than this from the cube centre, 1.5 m */
     float integrate_threshold;    /* integrate only after (angle + translation) / 2 >= this, 0.001 */
     double converge_rotation;     /* fused early exit per level (rad); 0 disables */
     double converge_translation;  /* fused early exit per level (m) */
     int32_t extract_on_shift;     /* extract the leaving slab before clearing it */
     int32_t reserved;
+    double shift_target_distance; /* target point = camera + this along the optical axis (m); 0: camera centre */
 } kfx_pipeline_config;
 
 typedef struct kfx_frame_info {
@@ -431,7 +436,7 @@ int kfx_synth_make_scene(uint64_t seed, int n_boxes, int n_spheres, const double
 int kfx_synth_orbit(const double center[3], double radius, double dy, double angle0, double dangle,
                     int n, kfx_pose* out);
 /* Seeded smooth random walk: `step` metres per frame from `start`, heading
- * wandering around +x, looking ahead and `pitch` radians down. */
+ * wandering within +-0.3 rad of +x, looking ahead and `pitch` radians down. */
 int kfx_synth_walk(uint64_t seed, const double start[3], double step, double pitch, int n, kfx_pose* out);
 /* Render one depth frame (uint16 mm, row-major cols x rows).  flags bit 0:
  * Kinect noise sigma(z) = 1.2 + 1.9 (z - 0.4)^2 mm, 1 mm quantisation, 2 %

@@ -18,7 +18,7 @@ def _run(kfx, pipe, depths):
 
 
 def _c1_run(kfx, mode, use_graph=0, frames=30, n=256, **kw):
-    # The C1 orbit keeps the camera 1.5 m from the cube centre: no shifts.
+    # The C1 orbit looks at the cube centre from 1.5 m: no shifts.
     intr = kfx.kinect_intrinsics()
     scene, poses = scenes.c1_scene(), scenes.c1_poses(frames)
     depths = np.stack([kfx.render(scene, poses[i], intr, scenes.C1_SEED, i, noise=True) for i in range(frames)])
@@ -68,13 +68,14 @@ def test_early_exit_reduces_iterations(gpu):
 @pytest.mark.parametrize("mode,use_graph", [(0, 0), (1, 1)])
 def test_cyclic_shift_keeps_tracking(gpu, mode, use_graph):
     # Random walk of 2 cm per frame (the 512^3 workload's trajectory) in a
-    # 256^3, 3.072 m cube with a 0.4 m shift threshold: several shifts in 80
-    # frames, tracking continues across them.
+    # 256^3, 3.072 m cube with a 0.4 m shift threshold on the target point:
+    # several shifts in 80 frames, tracking continues across them.
     kfx = gpu
     intr = kfx.kinect_intrinsics()
-    scene = kfx.make_scene(2, 10, 10, (-0.5, 7.5, -1.8, 1.8), 0.0)
+    scene = kfx.make_scene(2, 10, 10, (0.5, 9.0, -2.0, 2.0), 0.0)
     gt = kfx.walk(2, (0.0, -1.2, 0.0), 0.02, 0.35, 80)
-    origin = (gt[0].t[0] - 1.536, -2.5, gt[0].t[2] - 1.536)
+    R0, t0 = gt[0].matrices()
+    origin = t0 + 1.5 * R0[:, 2] - 1.536  # cube centred on the first target point
     depths = np.stack([kfx.render(scene, gt[i], intr, 2, i, noise=True) for i in range(80)])
     cfg = kfx.default_config(intr=intr, voxels=kfx.Point3i(256, 256, 256),
                              volume_size=kfx.Point3f(3.072, 3.072, 3.072), mode=mode, use_graph=use_graph,

@@ -75,7 +75,12 @@ def test_normal_map_matches_double_oracle(gpu, orc, c1_frame):
     ok = ~np.isnan(want[..., 0])
     assert ok.mean() > 0.5
     cosang = np.sum(got[ok] * want[ok], axis=-1)
-    assert np.all(cosang > 0.99), np.min(cosang)
+    # The orientation (toward the camera) is ill-conditioned where the normal
+    # is nearly perpendicular to the viewing ray: compare those up to sign.
+    view = vm[ok] / np.linalg.norm(vm[ok], axis=-1, keepdims=True)
+    grazing = np.abs(np.sum(want[ok] * view, axis=-1)) < 0.05
+    assert np.all(np.abs(cosang) > 0.99), np.min(np.abs(cosang))
+    assert np.all(cosang[~grazing] > 0.99), np.min(cosang[~grazing])
     assert np.mean(cosang > 1 - 1e-5) > 0.999  # within 4.5 mrad on 99.9 %
 
 
