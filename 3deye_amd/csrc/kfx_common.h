@@ -93,8 +93,14 @@ __host__ __device__ __forceinline__ uint32_t cyclic_address_in_range(int x, int 
 }
 
 // Voxel codec (raycast.cl:9-27): decode value / 32767, encode rtz then clamp.
+// The decode is the correctly rounded quotient, formed without a division:
+// q0 = v * RN(1/32767), then one FMA correction with the exact residual
+// (Markstein's theorem; checked for all 65536 int16 values).
 __device__ __forceinline__ float tsdf_decode(int16_t v) {
-    return (float)v / (float)KFX_MAX_DEPTH_VALUE;
+    constexpr float kInv = 1.0f / (float)KFX_MAX_DEPTH_VALUE;
+    const float a = (float)v;
+    const float q0 = a * kInv;
+    return fmaf(fmaf(-q0, (float)KFX_MAX_DEPTH_VALUE, a), kInv, q0);
 }
 __device__ __forceinline__ int16_t tsdf_encode(float tsdf) {
     // rtz conversion of a finite float; saturate before the int cast so the

@@ -9,14 +9,15 @@
 //    (GPU/opencl/transform_estimation.cl:4-53) reduces each entry with 256
 //    lanes; the host downloads the 27 floats and unpacks A and b.
 //
-// 2. The fused path (kfx_icp_iterate): one launch per ICP iteration.  Each
-//    workgroup accumulates its pixels' 27 products with compensated (TwoSum /
-//    Kahan) single-precision arithmetic and stores (sum, compensation) pairs;
-//    the last workgroup to finish (atomic ticket) adds every pair in double
-//    precision in a fixed order, solves the 6x6 system by Cholesky in double
-//    precision and composes the increment into the device-resident pose.  No
-//    host round trip between iterations, so a whole frame's 19 iterations can
-//    be enqueued (or captured in a graph) at once.
+// 2. The fused path (kfx_icp_iterate): one launch per ICP iteration.  The
+//    single-precision products of each pixel are summed in double precision
+//    per thread, then over the wave and the workgroup in a fixed order; the
+//    last workgroup to finish (atomic ticket) adds the workgroup totals in
+//    double precision in a fixed order, solves the 6x6 system by Cholesky in
+//    double precision and composes the increment into the device-resident
+//    pose.  No host round trip between iterations, so a whole frame's 19
+//    iterations can be enqueued (or captured in a graph) at once, and the
+//    result does not depend on the order in which workgroups finish.
 #include <math.h>
 
 #include "kfx_common.h"
@@ -47,34 +48,40 @@ struct IcpState {
     int pad;
 };
 
-// The 27 products: row[i] * row[j] for i <= j, j in [i, 6], row[6] = b
-// (unpacking order of opencl_wrappers.cpp:426-437).
-__device__ __forceinline__ void icp_row(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
-                                        const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev,
-                                        const float* Rc, const float* tc, const IcpFrame& f, int x, int y,
-                                        float row[7]) {
-    for (int k = 0; k < 7; ++k) row[k] = 0.0f;
-    const int i = y * f.cols + x;
-    const kfx_xyz n = ncurr[i];
-    if (isnan(n.x)) return;
-    const kfx_xyz v = vcurr[i];
-    if (isnan(v.x)) return;
+// Projective association of one pixel (current frame, camera -> volume by
+// Rc, tc; previous frame by rprev_inv, tprev): g = Rc v + tc, projected into
+// the previous frame and rounded to the nearest pixel j.  False when the
+// current vertex or normal is missing, the point is behind the previous
+// camera or it projects outside the image.
+__device__ __forceinline__ bool icp_project(const kfx_xyz& v, const kfx_xyz& n, const float* Rc, const float* tc,
+                                            const IcpFrame& f, float g[3], int& j) {
+    if (isnan(n.x) || isnan(v.x)) return false;
     const float gx = Rc[0] * v.x + Rc[1] * v.y + Rc[2] * v.z + tc[0];
     const float gy = Rc[3] * v.x + Rc[4] * v.y + Rc[5] * v.z + tc[1];
     const float gz = Rc[6] * v.x + Rc[7] * v.y + Rc[8] * v.z + tc[2];
+    g[0] = gx; g[1] = gy; g[2] = gz;
     const float dx = gx - f.tprev[0], dy = gy - f.tprev[1], dz = gz - f.tprev[2];
     const float px = f.rprev_inv[0] * dx + f.rprev_inv[1] * dy + f.rprev_inv[2] * dz;
     const float py = f.rprev_inv[3] * dx + f.rprev_inv[4] * dy + f.rprev_inv[5] * dz;
     const float pz = f.rprev_inv[6] * dx + f.rprev_inv[7] * dy + f.rprev_inv[8] * dz;
-    if (!(pz > 0.0f)) return;
+    if (!(pz > 0.0f)) return false;
     const float fu = rintf(px * f.fx / pz + f.cx);
     const float fv = rintf(py * f.fy / pz + f.cy);
-    if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)f.cols && fv < (float)f.rows)) return;
-    const int j = (int)fv * f.cols + (int)fu;
-    const kfx_xyz np = nprev[j];
-    if (isnan(np.x)) return;
-    const kfx_xyz vp = vprev[j];
-    if (isnan(vp.x)) return;
+    if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)f.cols && fv < (float)f.rows)) return false;
+    j = (int)fv * f.cols + (int)fu;
+    return true;
+}
+
+// Jacobian row and residual of an associated pair: zero unless the previous
+// vertex and normal exist, the points lie within dist_thr and the normals'
+// cross product is shorter than angle_thr (sine of the angle).  row[0..2] =
+// g x n_prev, row[3..5] = n_prev, row[6] = n_prev . (v_prev - g).
+__device__ __forceinline__ void icp_residual(const kfx_xyz& n, const float g[3], const kfx_xyz& np,
+                                             const kfx_xyz& vp, const float* Rc, const IcpFrame& f, bool ok,
+                                             float row[7]) {
+    for (int k = 0; k < 7; ++k) row[k] = 0.0f;
+    if (!ok || isnan(np.x) || isnan(vp.x)) return;
+    const float gx = g[0], gy = g[1], gz = g[2];
     const float ex = vp.x - gx, ey = vp.y - gy, ez = vp.z - gz;
     if (!(sqrtf(ex * ex + ey * ey + ez * ez) <= f.dist_thr)) return;
     const float cnx = Rc[0] * n.x + Rc[1] * n.y + Rc[2] * n.z;
@@ -89,6 +96,22 @@ __device__ __forceinline__ void icp_row(const kfx_xyz* __restrict__ vcurr, const
     row[4] = np.y;
     row[5] = np.z;
     row[6] = np.x * ex + np.y * ey + np.z * ez;
+}
+
+// The 27 products: row[i] * row[j] for i <= j, j in [i, 6], row[6] = b
+// (unpacking order of opencl_wrappers.cpp:426-437).
+__device__ __forceinline__ void icp_row(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
+                                        const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev,
+                                        const float* Rc, const float* tc, const IcpFrame& f, int x, int y,
+                                        float row[7]) {
+    const int i = y * f.cols + x;
+    const kfx_xyz n = ncurr[i], v = vcurr[i];
+    float g[3];
+    int j = 0;
+    const bool ok = icp_project(v, n, Rc, tc, f, g, j);
+    kfx_xyz np{NAN, NAN, NAN}, vp{NAN, NAN, NAN};
+    if (ok) { np = nprev[j]; vp = vprev[j]; }
+    icp_residual(n, g, np, vp, Rc, f, ok, row);
 }
 
 // ------------------------------------------------------------------------
@@ -160,73 +183,96 @@ __global__ void clean_buff_kernel(float* __restrict__ g, int width) {
 // Fused path.
 // ------------------------------------------------------------------------
 constexpr int kIcpThreads = 256;
-constexpr int kIcpPixelsPerThread = 4;
-constexpr int kIcpPixelsPerBlock = kIcpThreads * kIcpPixelsPerThread;
+constexpr int kIcpWaves = kIcpThreads / 64;
+constexpr int kIcpEntries = 27;
+constexpr unsigned kIcpTwoPixelMin = 150000;  // images at least this large take two pixels per thread
 
-// Error-free sum (Knuth TwoSum): s + e == a + b exactly.
-__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
-    s = a + b;
-    const float bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
+// Double-precision sums padded to 32 entries, spread over a wave by a
+// butterfly that halves the entries each step: after the step with xor offset
+// 32, 16, 8, 4, 2 a lane keeps the half selected by that bit of its lane id,
+// added to its partner's copy.  The last step (offset 1) adds the two lanes
+// left per entry.  Lane l ends with the wave total of entry l >> 1, in a fixed
+// association order.
+template <int kHalf>
+__device__ __forceinline__ void reduce_scatter_step(double (&v)[32], int lane) {
+    const bool upper = (lane & (2 * kHalf)) != 0;
+#pragma unroll
+    for (int i = 0; i < kHalf; ++i) {
+        const double send = upper ? v[i] : v[i + kHalf];
+        const double keep = upper ? v[i + kHalf] : v[i];
+        v[i] = keep + __shfl_xor(send, 2 * kHalf);
+    }
 }
 
-// Compensated add of a value into (sum, comp).
-__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
-    float e;
-    two_sum(s, v, s, e);
-    c += e;
+__device__ __forceinline__ double wave_reduce_scatter32(double (&v)[32], int lane) {
+    reduce_scatter_step<16>(v, lane);
+    reduce_scatter_step<8>(v, lane);
+    reduce_scatter_step<4>(v, lane);
+    reduce_scatter_step<2>(v, lane);
+    reduce_scatter_step<1>(v, lane);
+    return v[0] + __shfl_xor(v[0], 1);
 }
 
-// Compensated merge of two (sum, comp) pairs.
-__device__ __forceinline__ void pair_merge(float& s, float& c, float s2, float c2) {
-    float e;
-    two_sum(s, s2, s, e);
-    c = c + c2 + e;
-}
-
+// 6x6 Cholesky solve in double precision, then the pose update.  Every loop
+// has constant bounds, so the arrays stay in registers.
 __device__ void solve_and_update(IcpState* st, const double* sums, int tag, double conv_rot, double conv_trans) {
     double A[36], b[6];
-    int k = 0;
-    for (int i = 0; i < 6; ++i)
-        for (int j = i; j < 7; ++j, ++k) {
-            if (j == 6) b[i] = sums[k];
-            else A[j * 6 + i] = A[i * 6 + j] = sums[k];
-        }
+    {
+        int k = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = i; j < 7; ++j, ++k) {
+                if (j == 6) b[i] = sums[k];
+                else A[j * 6 + i] = A[i * 6 + j] = sums[k];
+            }
+    }
+#pragma unroll
     for (int i = 0; i < 36; ++i) st->A[i] = A[i];
+#pragma unroll
     for (int i = 0; i < 6; ++i) st->b[i] = b[i];
-    // Cholesky A = L L^T (lower, in place).
+    // Cholesky A = L L^T; a non-positive or non-finite pivot makes the
+    // system singular (the remaining entries are then not used).
     double L[36];
     bool ok = true;
-    for (int i = 0; i < 6 && ok; ++i) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
         for (int j = 0; j <= i; ++j) {
             double s = A[i * 6 + j];
+#pragma unroll
             for (int p = 0; p < j; ++p) s -= L[i * 6 + p] * L[j * 6 + p];
             if (i == j) {
-                if (!(s > 1e-300) || !isfinite(s)) { ok = false; break; }
-                L[i * 6 + i] = sqrt(s);
+                ok = ok && (s > 1e-300) && isfinite(s);
+                L[i * 6 + i] = sqrt(ok ? s : 1.0);
             } else {
                 L[i * 6 + j] = s / L[j * 6 + j];
             }
         }
-    }
     st->iterations += 1;
     if (!ok) {
         st->status = KFX_ERR_SINGULAR;
         st->singular_count += 1;
+#pragma unroll
         for (int i = 0; i < 6; ++i) st->x[i] = 0.0;
         return;
     }
     double yv[6], x[6];
+#pragma unroll
     for (int i = 0; i < 6; ++i) {
         double s = b[i];
+#pragma unroll
         for (int p = 0; p < i; ++p) s -= L[i * 6 + p] * yv[p];
         yv[i] = s / L[i * 6 + i];
     }
+#pragma unroll
     for (int i = 5; i >= 0; --i) {
         double s = yv[i];
+#pragma unroll
         for (int p = i + 1; p < 6; ++p) s -= L[p * 6 + i] * x[p];
         x[i] = s / L[i * 6 + i];
     }
+#pragma unroll
     for (int i = 0; i < 6; ++i) st->x[i] = x[i];
     st->status = KFX_OK;
     // Increment: R_inc = Rz(x2) Ry(x1) Rx(x0), t_inc = x[3..5];
@@ -235,26 +281,44 @@ __device__ void solve_and_update(IcpState* st, const double* sums, int tag, doub
     const double Ri[9] = {cg * cb, cg * sb * sa - sg * ca, cg * sb * ca + sg * sa,
                           sg * cb, sg * sb * sa + cg * ca, sg * sb * ca - cg * sa,
                           -sb, cb * sa, cb * ca};
-    double R[9], t[3];
+    double R0[9], t0[3], R[9], t[3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R0[i] = st->R[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) t0[i] = st->t[i];
+#pragma unroll
     for (int r = 0; r < 3; ++r) {
+#pragma unroll
         for (int c = 0; c < 3; ++c)
-            R[3 * r + c] = Ri[3 * r + 0] * st->R[c] + Ri[3 * r + 1] * st->R[3 + c] + Ri[3 * r + 2] * st->R[6 + c];
-        t[r] = Ri[3 * r + 0] * st->t[0] + Ri[3 * r + 1] * st->t[1] + Ri[3 * r + 2] * st->t[2] + x[3 + r];
+            R[3 * r + c] = Ri[3 * r + 0] * R0[c] + Ri[3 * r + 1] * R0[3 + c] + Ri[3 * r + 2] * R0[6 + c];
+        t[r] = Ri[3 * r + 0] * t0[0] + Ri[3 * r + 1] * t0[1] + Ri[3 * r + 2] * t0[2] + x[3 + r];
     }
+#pragma unroll
     for (int i = 0; i < 9; ++i) { st->R[i] = R[i]; st->Rf[i] = (float)R[i]; }
+#pragma unroll
     for (int i = 0; i < 3; ++i) { st->t[i] = t[i]; st->tf[i] = (float)t[i]; }
     const double rot = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
     const double tr = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
     if (conv_rot > 0.0 && rot < conv_rot && tr < conv_trans) st->converged_tag = tag;
 }
 
+// One ICP iteration.  Thread t of workgroup g handles pixels
+// g * 256 * P + q * 256 + t (q < P): the current-frame loads of all P pixels
+// are issued before any projection, and the previous-frame gathers of all P
+// before any residual, so a wave waits on memory twice per launch.  The
+// single-precision products of a pixel (icp_row's arithmetic) are summed in
+// double per thread, then over the wave (wave_reduce_scatter32) and the four
+// waves in a fixed order; the last workgroup to finish (atomic ticket) sums
+// the workgroup totals in a fixed order, solves and updates the pose.  The
+// sums do not depend on the order in which workgroups finish.
+template <int P>
 __global__ void __launch_bounds__(kIcpThreads)
 icp_fused_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
                  const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev, IcpFrame f,
-                 IcpState* __restrict__ st, float2* __restrict__ partials, int tag, double conv_rot,
+                 IcpState* __restrict__ st, double* __restrict__ partials, int tag, double conv_rot,
                  double conv_trans) {
-    __shared__ float2 wred[kIcpThreads / 64][27];
-    __shared__ double dred[kIcpThreads / 64][27];
+    __shared__ double wsum[kIcpWaves][32];
+    __shared__ double total[32];
     __shared__ bool last;
     if (tag >= 0 && st->converged_tag == tag) return;  // uniform across the grid
     if (f.prev_from_state) {
@@ -265,33 +329,52 @@ icp_fused_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ 
     for (int i = 0; i < 9; ++i) Rc[i] = st->Rf[i];
     for (int i = 0; i < 3; ++i) tc[i] = st->tf[i];
 
-    float s[27], c[27];
-    for (int k = 0; k < 27; ++k) { s[k] = 0.0f; c[k] = 0.0f; }
     const int npix = f.cols * f.rows;
-    const int base = blockIdx.x * kIcpPixelsPerBlock + threadIdx.x;
-    for (int q = 0; q < kIcpPixelsPerThread; ++q) {
+    const int base = blockIdx.x * (kIcpThreads * P) + threadIdx.x;
+    kfx_xyz v[P], n[P], vp[P], np[P];
+    float g[P][3];
+    bool ok[P];
+    int j[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
         const int p = base + q * kIcpThreads;
-        if (p >= npix) break;
+        const int ps = p < npix ? p : 0;
+        n[q] = ncurr[ps];
+        v[q] = vcurr[ps];
+    }
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int p = base + q * kIcpThreads;
+        ok[q] = (p < npix) && icp_project(v[q], n[q], Rc, tc, f, g[q], j[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int js = ok[q] ? j[q] : 0;
+        np[q] = nprev[js];
+        vp[q] = vprev[js];
+    }
+    double acc[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
         float row[7];
-        icp_row(vcurr, ncurr, vprev, nprev, Rc, tc, f, p % f.cols, p / f.cols, row);
+        icp_residual(n[q], g[q], np[q], vp[q], Rc, f, ok[q], row);
         int k = 0;
+#pragma unroll
         for (int i = 0; i < 6; ++i)
-            for (int j = i; j < 7; ++j, ++k) kahan_add(s[k], c[k], row[i] * row[j]);
+#pragma unroll
+            for (int jj = i; jj < 7; ++jj, ++k) acc[k] += (double)(row[i] * row[jj]);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int k = 0; k < 27; ++k) {
-        float a = s[k], e = c[k];
-        for (int off = 32; off > 0; off >>= 1) {
-            const float a2 = __shfl_xor(a, off), e2 = __shfl_xor(e, off);
-            pair_merge(a, e, a2, e2);
-        }
-        if (lane == 0) wred[wave][k] = make_float2(a, e);
-    }
+    const double wtot = wave_reduce_scatter32(acc, lane);
+    if ((lane & 1) == 0) wsum[wave][lane >> 1] = wtot;
     __syncthreads();
-    if (threadIdx.x < 27) {
-        float a = wred[0][threadIdx.x].x, e = wred[0][threadIdx.x].y;
-        for (int w = 1; w < kIcpThreads / 64; ++w) pair_merge(a, e, wred[w][threadIdx.x].x, wred[w][threadIdx.x].y);
-        partials[blockIdx.x * 27 + threadIdx.x] = make_float2(a, e);
+    if (threadIdx.x < kIcpEntries) {
+        double s = wsum[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < kIcpWaves; ++w) s += wsum[w][threadIdx.x];
+        partials[blockIdx.x * kIcpEntries + threadIdx.x] = s;
     }
     // Publish the partials, then take a ticket; the last workgroup reduces.
     __threadfence();
@@ -300,21 +383,32 @@ icp_fused_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ 
     __syncthreads();
     if (!last) return;
     __threadfence();
-    const int nblk = gridDim.x;
-    // Fixed-order double-precision sum: lane l of wave w handles entry k =
-    // w + 4 i over blocks l, l + 64, ...; then a shuffle tree per entry.
-    for (int k = wave; k < 27; k += kIcpThreads / 64) {
-        double acc = 0.0;
-        for (int b = lane; b < nblk; b += 64) {
-            const float2 v = partials[b * 27 + k];
-            acc += (double)v.x + (double)v.y;
-        }
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-        if (lane == 0) dred[0][k] = acc;
+    // Thread t sums the totals of workgroups t, t + 256, ... in order; each
+    // round issues the 27 loads of one workgroup's totals together, so the
+    // last workgroup waits on memory once per 256 workgroups.  Then the same
+    // fixed-order wave and workgroup reduction as above.
+#pragma unroll
+    for (int e = 0; e < 32; ++e) acc[e] = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += kIcpThreads) {
+        double t[kIcpEntries];
+#pragma unroll
+        for (int e = 0; e < kIcpEntries; ++e) t[e] = partials[b * kIcpEntries + e];
+#pragma unroll
+        for (int e = 0; e < kIcpEntries; ++e) acc[e] += t[e];
+    }
+    const double gtot = wave_reduce_scatter32(acc, lane);
+    __syncthreads();
+    if ((lane & 1) == 0) wsum[wave][lane >> 1] = gtot;
+    __syncthreads();
+    if (threadIdx.x < kIcpEntries) {
+        double s = wsum[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < kIcpWaves; ++w) s += wsum[w][threadIdx.x];
+        total[threadIdx.x] = s;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        solve_and_update(st, &dred[0][0], tag, conv_rot, conv_trans);
+        solve_and_update(st, total, tag, conv_rot, conv_trans);
         st->ticket = 0u;
         __threadfence();
     }
@@ -376,7 +470,7 @@ IcpFrame make_frame(const kfx_matrix_3f* rprev_inv, const kfx_point_3f* tprev, c
 
 struct kfx_icp {
     kfx::IcpState* state;
-    float2* partials;
+    double* partials;  // [workgroup][27] sums of the fused iteration
     int max_blocks;
 };
 
@@ -438,9 +532,9 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(out != nullptr && max_pixels > 0, "kfx_icp_create: bad arguments");
     kfx_icp* h = new kfx_icp();
-    h->max_blocks = (int)divup((unsigned)max_pixels, (unsigned)kIcpPixelsPerBlock);
+    h->max_blocks = (int)divup((unsigned)max_pixels, (unsigned)kIcpThreads);
     hipError_t e = hipMalloc((void**)&h->state, sizeof(IcpState));
-    if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(float2) * 27 * (size_t)h->max_blocks);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(double) * kIcpEntries * (size_t)h->max_blocks);
     if (e == hipSuccess) e = hipMemsetAsync(h->state, 0, sizeof(IcpState), stream());
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
     if (e != hipSuccess) {
@@ -482,15 +576,24 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
     KFX_REQUIRE((rotation_previous_inverse == nullptr) == (translation_previous == nullptr),
                 "kfx_icp_iterate: give both previous-pose arguments or neither");
     KFX_REQUIRE(intr->cols > 0 && intr->rows > 0, "kfx_icp_iterate: empty image");
-    const int nblk = (int)divup((unsigned)(intr->cols * intr->rows), (unsigned)kIcpPixelsPerBlock);
-    KFX_REQUIRE(nblk <= h->max_blocks, "kfx_icp_iterate: image of %d pixels exceeds the handle's capacity",
-                intr->cols * intr->rows);
+    const unsigned npix = (unsigned)(intr->cols * intr->rows);
+    KFX_REQUIRE(divup(npix, (unsigned)kIcpThreads) <= (unsigned)h->max_blocks,
+                "kfx_icp_iterate: image of %u pixels exceeds the handle's capacity", npix);
+    // Two pixels per thread on full-resolution images (fewer, fuller
+    // workgroups), one on the coarse levels (enough workgroups to spread).
+    const bool two = npix >= kIcpTwoPixelMin;
+    const int nblk = (int)divup(npix, (unsigned)(kIcpThreads * (two ? 2 : 1)));
     IcpFrame f = make_frame(rotation_previous_inverse, translation_previous, intr, distance_threshold,
                             angle_threshold);
     ProfScope prof("icp_fused");
-    icp_fused_kernel<<<nblk, kIcpThreads, 0, stream()>>>(vmap_current, nmap_current, vmap_previous, nmap_previous, f,
-                                                         h->state, h->partials, level_tag, converge_rotation,
-                                                         converge_translation);
+    if (two)
+        icp_fused_kernel<2><<<nblk, kIcpThreads, 0, stream()>>>(vmap_current, nmap_current, vmap_previous,
+                                                                nmap_previous, f, h->state, h->partials, level_tag,
+                                                                converge_rotation, converge_translation);
+    else
+        icp_fused_kernel<1><<<nblk, kIcpThreads, 0, stream()>>>(vmap_current, nmap_current, vmap_previous,
+                                                                nmap_previous, f, h->state, h->partials, level_tag,
+                                                                converge_rotation, converge_translation);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }

@@ -8,7 +8,7 @@
 //     iteration downloads the 27 sums (Estimate_Combined) and the host solves
 //     and updates the pose in double precision; fixed iteration counts.
 //   KFX_PIPELINE_FUSED — the pose lives on the GPU: one fused launch per ICP
-//     iteration (compensated FP32 block sums, FP64 reduction, Cholesky and
+//     iteration (FP64 sums of the FP32 products, Cholesky and
 //     pose update in the last workgroup), optional early exit per level,
 //     integration and raycast read the pose from device memory, and the whole
 //     frame can be replayed as one HIP graph; the host reads the pose once per

@@ -23,6 +23,7 @@ struct RaycastParams {
     float r[9];
     float tx, ty, tz;
     float csx, csy, csz;
+    float icsx, icsy, icsz;  // 1 / cell size, correctly rounded
     float fx, fy, cx, cy;
     int cols, rows;
     float time_step;
@@ -57,9 +58,23 @@ __device__ __forceinline__ float read_tsdf(const kfx_tsdf* __restrict__ vol, con
     return tsdf_decode(read_raw(vol, rb, x, y, z));
 }
 
+// p / cell, correctly rounded, without a division: q0 = p * (1/cell) is
+// within an ulp of the quotient, the residual p - q0 * cell is exact by FMA,
+// and one correction with the correctly rounded reciprocal gives the
+// correctly rounded quotient (Markstein's theorem; P. Markstein, "Computation
+// of elementary functions on the IBM RISC System/6000 processor", IBM J. Res.
+// Dev. 34(1), 1990).  The result equals IEEE p / cell bit for bit for all
+// operands away from underflow; near zero the sign, and so the floor below,
+// is still exact.
+__device__ __forceinline__ float div_cell(float p, float cell, float inv_cell) {
+    const float q0 = p * inv_cell;
+    const float r = fmaf(-q0, cell, p);
+    return fmaf(r, inv_cell, q0);
+}
+
 // floor(p / cell) with saturation (convert_int_sat_rtn).
-__device__ __forceinline__ int voxel_floor(float p, float cell) {
-    float q = floorf(p / cell);
+__device__ __forceinline__ int voxel_floor(float p, float cell, float inv_cell) {
+    float q = floorf(div_cell(p, cell, inv_cell));
     if (q != q) return 0;
     q = fminf(fmaxf(q, -2147483648.0f), 2147483520.0f);
     return (int)q;
@@ -68,9 +83,9 @@ __device__ __forceinline__ int voxel_floor(float p, float cell) {
 // interpolateTrilineary (raycast.cl:95-145): NaN outside [1, N-2].
 __device__ float interpolate_trilinear(const kfx_tsdf* __restrict__ vol, f3 pt, const RaycastParams& p) {
     const kfx_round_buffer& rb = p.rb;
-    int gx = voxel_floor(pt.x, p.csx);
-    int gy = voxel_floor(pt.y, p.csy);
-    int gz = voxel_floor(pt.z, p.csz);
+    int gx = voxel_floor(pt.x, p.csx, p.icsx);
+    int gy = voxel_floor(pt.y, p.csy, p.icsy);
+    int gz = voxel_floor(pt.z, p.csz, p.icsz);
     if (gx <= 0 || gx >= rb.voxels_size.x - 1) return NAN;
     if (gy <= 0 || gy >= rb.voxels_size.y - 1) return NAN;
     if (gz <= 0 || gz >= rb.voxels_size.z - 1) return NAN;
@@ -80,11 +95,11 @@ __device__ float interpolate_trilinear(const kfx_tsdf* __restrict__ vol, f3 pt, 
     gx = (pt.x < vx) ? (gx - 1) : gx;
     gy = (pt.y < vy) ? (gy - 1) : gy;
     gz = (pt.z < vz) ? (gz - 1) : gz;
-    float a = pt.x / p.csx - ((float)gx + 0.5f);
+    float a = div_cell(pt.x, p.csx, p.icsx) - ((float)gx + 0.5f);
     if (a < 0.0f) { gx--; a += 1.0f; }
-    float b = pt.y / p.csy - ((float)gy + 0.5f);
+    float b = div_cell(pt.y, p.csy, p.icsy) - ((float)gy + 0.5f);
     if (b < 0.0f) { gy--; b += 1.0f; }
-    float c = pt.z / p.csz - ((float)gz + 0.5f);
+    float c = div_cell(pt.z, p.csz, p.icsz) - ((float)gz + 0.5f);
     if (c < 0.0f) { gz--; c += 1.0f; }
     const float t000 = read_tsdf(vol, rb, gx + 0, gy + 0, gz + 0);
     const float t001 = read_tsdf(vol, rb, gx + 0, gy + 0, gz + 1);
@@ -152,7 +167,8 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
     const int nxv = rb.voxels_size.x, nyv = rb.voxels_size.y, nzv = rb.voxels_size.z;
     {
         const f3 q = ray_at(start, dir, time_curr);
-        int gx = voxel_floor(q.x, p.csx), gy = voxel_floor(q.y, p.csy), gz = voxel_floor(q.z, p.csz);
+        int gx = voxel_floor(q.x, p.csx, p.icsx), gy = voxel_floor(q.y, p.csy, p.icsy),
+            gz = voxel_floor(q.z, p.csz, p.icsz);
         gx = max(0, min(gx, nxv - 1));
         gy = max(0, min(gy, nyv - 1));
         gz = max(0, min(gz, nzv - 1));
@@ -167,8 +183,9 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
             const int tsdf_prev = tsdf;
             const float t_next = time_curr + step;
             const f3 q1 = ray_at(start, dir, t_next);
-            const int vx = voxel_floor(q1.x, p.csx), vy = voxel_floor(q1.y, p.csy), vz = voxel_floor(q1.z, p.csz);
-            if (!(vx >= 0 && vy >= 0 && vz >= 0 && vx < nxv && vy < nyv && vz < nzv)) break;
+            const int vx = voxel_floor(q1.x, p.csx, p.icsx), vy = voxel_floor(q1.y, p.csy, p.icsy),
+                      vz = voxel_floor(q1.z, p.csz, p.icsz);
+            if (!((unsigned)vx < (unsigned)nxv && (unsigned)vy < (unsigned)nyv && (unsigned)vz < (unsigned)nzv)) break;
             tsdf = read_raw(vol, rb, vx, vy, vz);
             if (tsdf_prev < 0 && tsdf > 0) break;
             if (tsdf_prev > 0 && tsdf < 0) {
@@ -182,7 +199,8 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
                 vout = kfx_xyz{vert.x, vert.y, vert.z};
 
                 const f3 q3 = ray_at(start, dir, time_curr);
-                const int hx = voxel_floor(q3.x, p.csx), hy = voxel_floor(q3.y, p.csy), hz = voxel_floor(q3.z, p.csz);
+                const int hx = voxel_floor(q3.x, p.csx, p.icsx), hy = voxel_floor(q3.y, p.csy, p.icsy),
+                          hz = voxel_floor(q3.z, p.csz, p.icsz);
                 if (hx > 1 && hy > 1 && hz > 1 && hx < nxv - 2 && hy < nyv - 2 && hz < nzv - 2) {
                     f3 t = vert;
                     t.x += p.csx;
@@ -237,6 +255,8 @@ static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* 
     }
     p.tx = translation->x; p.ty = translation->y; p.tz = translation->z;
     p.csx = cell_size->x; p.csy = cell_size->y; p.csz = cell_size->z;
+    KFX_REQUIRE(p.csx > 0.0f && p.csy > 0.0f && p.csz > 0.0f, "kfx_raycast: cell size must be positive");
+    p.icsx = 1.0f / p.csx; p.icsy = 1.0f / p.csy; p.icsz = 1.0f / p.csz;
     p.fx = intr->fx; p.fy = intr->fy; p.cx = intr->cx; p.cy = intr->cy;
     p.cols = intr->cols; p.rows = intr->rows;
     // time_step = Truncation_Distance * 0.8 (raycast.cl:209), the product

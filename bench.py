@@ -259,7 +259,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded analytic scene, Kinect noise model)",
         "config": {"workload": "512^3 TSDF (6 mm), 640x480 depth, 300-frame 6 m random walk with cyclic shifts",
-                   "pipeline": "fused: device-resident pose, FP32-compensated ICP sums with FP64 solve, "
+                   "pipeline": "fused: device-resident pose, FP64 ICP sums and solve, "
                                "one HIP graph per frame; bilateral filter",
                    "frames": N_FRAMES, "icp_iterations": [10, 5, 4]},
         "fused": {"frames_per_s": fps, "ate_m": ate_f, "shifts": v_f.shifts,

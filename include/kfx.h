@@ -216,8 +216,8 @@ int kfx_icp_create(int max_pixels, kfx_icp** out);
 int kfx_icp_destroy(kfx_icp* icp);
 /* Set the estimate (asynchronous, on the library stream). */
 int kfx_icp_reset(kfx_icp* icp, const kfx_pose* pose);
-/* One iteration: FP32 Jacobian rows, FP32 compensated (TwoSum) workgroup
- * sums, FP64 cross-workgroup sum in the last workgroup, FP64 6x6 Cholesky
+/* One iteration: FP32 Jacobian rows and products summed in FP64 (fixed
+ * order), FP64 cross-workgroup sum in the last workgroup, FP64 6x6 Cholesky
  * solve and pose composition on the GPU.  A singular system leaves the pose
  * unchanged and records KFX_ERR_SINGULAR.  When level_tag >= 0 and an
  * iteration's increment is below (converge_rotation rad, converge_translation

@@ -62,6 +62,24 @@ def test_pyramid_vertex_maps_bit_exact(gpu, orc, c1_frame):
         assert np.array_equal(got[ok], want[ok]), f"vertex map level {i}"
 
 
+def _eigen_gap(vm):
+    """Per pixel (lambda_1 - lambda_0) / lambda_2 of the 7x7 neighbourhood
+    covariance the normal is the smallest eigenvector of (window rows
+    [v-3, min(v+4, rows-1)), columns likewise), in double precision."""
+    H, W, _ = vm.shape
+    pad = np.full((H + 6, W + 6, 3), np.nan)
+    pad[3:H + 2, 3:W + 2] = vm[:H - 1, :W - 1]  # the last row/column is outside every window
+    win = [pad[3 + dy:3 + dy + H, 3 + dx:3 + dx + W] for dy in range(-3, 4) for dx in range(-3, 4)]
+    cnt = sum((~np.isnan(p[..., 0])).astype(np.float64) for p in win)
+    mean = sum(np.nan_to_num(p) for p in win) / np.maximum(cnt, 1)[..., None]
+    cov = np.zeros((H, W, 3, 3))
+    for p in win:
+        d = np.nan_to_num(p - mean)
+        cov += d[..., :, None] * d[..., None, :]
+    lam = np.linalg.eigvalsh(cov)
+    return (lam[..., 1] - lam[..., 0]) / np.maximum(lam[..., 2], 1e-300)
+
+
 def test_normal_map_matches_double_oracle(gpu, orc, c1_frame):
     kfx = gpu
     intr, d = c1_frame
@@ -75,13 +93,20 @@ def test_normal_map_matches_double_oracle(gpu, orc, c1_frame):
     ok = ~np.isnan(want[..., 0])
     assert ok.mean() > 0.5
     cosang = np.sum(got[ok] * want[ok], axis=-1)
-    # The orientation (toward the camera) is ill-conditioned where the normal
-    # is nearly perpendicular to the viewing ray: compare those up to sign.
+    # The direction is ill-conditioned where the two smallest eigenvalues of
+    # the neighbourhood covariance nearly coincide (points close to a line),
+    # and the orientation toward the camera where the normal is nearly
+    # perpendicular to the viewing ray: those pixels are compared in bulk only.
+    gap = _eigen_gap(vm)[ok]
     view = vm[ok] / np.linalg.norm(vm[ok], axis=-1, keepdims=True)
     grazing = np.abs(np.sum(want[ok] * view, axis=-1)) < 0.05
-    assert np.all(np.abs(cosang) > 0.99), np.min(np.abs(cosang))
-    assert np.all(cosang[~grazing] > 0.99), np.min(cosang[~grazing])
-    assert np.mean(cosang > 1 - 1e-5) > 0.999  # within 4.5 mrad on 99.9 %
+    wellcond = gap > 1e-2
+    assert wellcond.mean() > 0.95
+    assert np.all(np.abs(cosang[wellcond]) > 0.99), np.min(np.abs(cosang[wellcond]))
+    assert np.all(cosang[wellcond & ~grazing] > 0.99), np.min(cosang[wellcond & ~grazing])
+    # within 4.5 mrad on 99.9 % of the well-conditioned pixels and 95 % of all
+    assert np.mean(cosang[wellcond] > 1 - 1e-5) > 0.999, np.mean(cosang[wellcond] > 1 - 1e-5)
+    assert np.mean(cosang > 1 - 1e-5) > 0.95, np.mean(cosang > 1 - 1e-5)
 
 
 def test_transform_and_resize_bit_exact(gpu, orc, c1_frame):
