@@ -8,7 +8,18 @@
 
 #include "../../include/kfx.h"
 
+// Propagates a non-zero status.
+#define KFX_TRY(expr)                 \
+    do {                              \
+        int kfx_rc_ = (expr);         \
+        if (kfx_rc_) return kfx_rc_;  \
+    } while (0)
+
 namespace kfx {
+
+// Reserves the integration's depth-bound pyramid for images of cols x rows
+// (kfx_volume.hip), so that integrations inside a stream capture are culled.
+int integrate_reserve(int cols, int rows);
 
 // Library stream (created by kfx_init, default priority) on which every
 // entry point enqueues its work.

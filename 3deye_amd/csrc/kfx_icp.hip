@@ -9,16 +9,18 @@
 //    (GPU/opencl/transform_estimation.cl:4-53) reduces each entry with 256
 //    lanes; the host downloads the 27 floats and unpacks A and b.
 //
-// 2. The fused path (kfx_icp_iterate): one launch per ICP iteration.  The
-//    single-precision products of each pixel are summed in double precision
-//    per thread, then over the wave and the workgroup in a fixed order; the
-//    last workgroup to finish (atomic ticket) adds the workgroup totals in
-//    double precision in a fixed order, solves the 6x6 system by Cholesky in
-//    double precision and composes the increment into the device-resident
-//    pose.  No host round trip between iterations, so a whole frame's 19
+// 2. The fused path (kfx_icp_iterate): two launches per ICP iteration and
+//    no host round trip.  The single-precision products of each pixel are
+//    summed in double precision per thread, then over the wave and the
+//    workgroup in a fixed order; a one-workgroup launch adds the workgroup
+//    totals in double precision in a fixed order, solves the 6x6 system by
+//    Cholesky in double precision and composes the increment into the
+//    device-resident pose.  No host round trip between iterations, so a whole frame's 19
 //    iterations can be enqueued (or captured in a graph) at once, and the
 //    result does not depend on the order in which workgroups finish.
 #include <math.h>
+
+#include <type_traits>
 
 #include "kfx_common.h"
 
@@ -44,8 +46,7 @@ struct IcpState {
     int singular_count;
     int converged_tag;       // tag of the level that has converged (-1 none)
     int iterations;          // solves performed
-    unsigned int ticket;     // last-workgroup counter, reset by the last workgroup
-    int pad;
+    int pad[2];
 };
 
 // Projective association of one pixel (current frame, camera -> volume by
@@ -193,24 +194,75 @@ constexpr unsigned kIcpTwoPixelMin = 150000;  // images at least this large take
 // added to its partner's copy.  The last step (offset 1) adds the two lanes
 // left per entry.  Lane l ends with the wave total of entry l >> 1, in a fixed
 // association order.
-template <int kHalf>
-__device__ __forceinline__ void reduce_scatter_step(double (&v)[32], int lane) {
+template <int kHalf, typename T>
+__device__ __forceinline__ void reduce_scatter_step(T (&v)[32], int lane) {
     const bool upper = (lane & (2 * kHalf)) != 0;
 #pragma unroll
     for (int i = 0; i < kHalf; ++i) {
-        const double send = upper ? v[i] : v[i + kHalf];
-        const double keep = upper ? v[i + kHalf] : v[i];
+        const T send = upper ? v[i] : v[i + kHalf];
+        const T keep = upper ? v[i + kHalf] : v[i];
         v[i] = keep + __shfl_xor(send, 2 * kHalf);
     }
 }
 
-__device__ __forceinline__ double wave_reduce_scatter32(double (&v)[32], int lane) {
+template <typename T>
+__device__ __forceinline__ T wave_reduce_scatter32(T (&v)[32], int lane) {
     reduce_scatter_step<16>(v, lane);
     reduce_scatter_step<8>(v, lane);
     reduce_scatter_step<4>(v, lane);
     reduce_scatter_step<2>(v, lane);
     reduce_scatter_step<1>(v, lane);
     return v[0] + __shfl_xor(v[0], 1);
+}
+
+// Error-free sum (Knuth TwoSum): s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated (Kahan-style) accumulation of v into (s, c).
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    float e;
+    two_sum(s, v, s, e);
+    c += e;
+}
+
+// Merge of two compensated sums (s, c) += (s2, c2).  The TwoSum error term is
+// exact whichever operand comes first, so the merge is commutative.
+__device__ __forceinline__ void pair_merge(float& s, float& c, float s2, float c2) {
+    float e;
+    two_sum(s, s2, s, e);
+    c = c + c2 + e;
+}
+
+// The butterfly of wave_reduce_scatter32 on compensated single-precision
+// pairs: lane l ends with the wave total (s, c) of entry l >> 1.
+template <int kHalf>
+__device__ __forceinline__ void reduce_scatter_step(float (&s)[32], float (&c)[32], int lane) {
+    const bool upper = (lane & (2 * kHalf)) != 0;
+#pragma unroll
+    for (int i = 0; i < kHalf; ++i) {
+        const float send_s = upper ? s[i] : s[i + kHalf];
+        const float send_c = upper ? c[i] : c[i + kHalf];
+        float keep_s = upper ? s[i + kHalf] : s[i];
+        float keep_c = upper ? c[i + kHalf] : c[i];
+        pair_merge(keep_s, keep_c, __shfl_xor(send_s, 2 * kHalf), __shfl_xor(send_c, 2 * kHalf));
+        s[i] = keep_s;
+        c[i] = keep_c;
+    }
+}
+
+__device__ __forceinline__ float2 wave_reduce_scatter32(float (&s)[32], float (&c)[32], int lane) {
+    reduce_scatter_step<16>(s, c, lane);
+    reduce_scatter_step<8>(s, c, lane);
+    reduce_scatter_step<4>(s, c, lane);
+    reduce_scatter_step<2>(s, c, lane);
+    reduce_scatter_step<1>(s, c, lane);
+    float a = s[0], e = c[0];
+    pair_merge(a, e, __shfl_xor(s[0], 1), __shfl_xor(c[0], 1));
+    return make_float2(a, e);
 }
 
 // 6x6 Cholesky solve in double precision, then the pose update.  Every loop
@@ -302,24 +354,65 @@ __device__ void solve_and_update(IcpState* st, const double* sums, int tag, doub
     if (conv_rot > 0.0 && rot < conv_rot && tr < conv_trans) st->converged_tag = tag;
 }
 
-// One ICP iteration.  Thread t of workgroup g handles pixels
-// g * 256 * P + q * 256 + t (q < P): the current-frame loads of all P pixels
-// are issued before any projection, and the previous-frame gathers of all P
-// before any residual, so a wave waits on memory twice per launch.  The
-// single-precision products of a pixel (icp_row's arithmetic) are summed in
-// double per thread, then over the wave (wave_reduce_scatter32) and the four
-// waves in a fixed order; the last workgroup to finish (atomic ticket) sums
-// the workgroup totals in a fixed order, solves and updates the pose.  The
-// sums do not depend on the order in which workgroups finish.
-template <int P>
+// Precision of the per-workgroup sums (the cross-workgroup sum, the solve and
+// the pose update are double precision in every mode).
+enum IcpSumMode : int {
+    kSumKahan = KFX_ICP_SUM_FP32_KAHAN,  // FP32 products, compensated FP32 sums
+    kSumPlain = KFX_ICP_SUM_FP32,        // FP32 products, plain FP32 sums
+    kSumDouble = KFX_ICP_SUM_FP64,       // FP32 products summed in FP64
+};
+
+// Sum of 32 per-thread values over the workgroup in a fixed order: the
+// butterfly over each wave, then the four wave totals in wave order.  Thread
+// e < 27 returns entry e (the others return 0).
+template <typename T>
+__device__ __forceinline__ T workgroup_sum27(T (&acc)[32], T (*wsum)[32]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const T wtot = wave_reduce_scatter32(acc, lane);
+    if ((lane & 1) == 0) wsum[wave][lane >> 1] = wtot;
+    __syncthreads();
+    T s = 0;
+    if (threadIdx.x < kIcpEntries) {
+        s = wsum[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < kIcpWaves; ++w) s += wsum[w][threadIdx.x];
+    }
+    return s;
+}
+
+// The same for compensated single-precision pairs.
+__device__ __forceinline__ float2 workgroup_sum27(float (&s)[32], float (&c)[32], float2 (*wsum)[32]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float2 wtot = wave_reduce_scatter32(s, c, lane);
+    if ((lane & 1) == 0) wsum[wave][lane >> 1] = wtot;
+    __syncthreads();
+    float a = 0.0f, e = 0.0f;
+    if (threadIdx.x < kIcpEntries) {
+        a = wsum[0][threadIdx.x].x;
+        e = wsum[0][threadIdx.x].y;
+#pragma unroll
+        for (int w = 1; w < kIcpWaves; ++w) pair_merge(a, e, wsum[w][threadIdx.x].x, wsum[w][threadIdx.x].y);
+    }
+    return make_float2(a, e);
+}
+
+// One ICP iteration, first half: the 27 sums of each workgroup.  Thread t of
+// workgroup g handles pixels g * 256 * P + q * 256 + t (q < P): the
+// current-frame loads of all P pixels are issued before any projection, and
+// the previous-frame gathers of all P before any residual, so a wave waits on
+// memory twice.  The single-precision products of a pixel (icp_row's
+// arithmetic) are accumulated per thread (mode M), then over the workgroup in
+// a fixed order.  Workgroups never wait for each other: the cross-workgroup
+// sum is the next launch (icp_solve_kernel), so no workgroup needs a
+// device-wide fence or a shared counter.  Partials: [workgroup][27] float2
+// (sum, compensation) in the FP32 modes, double in the FP64 mode.
+template <int P, int M>
 __global__ void __launch_bounds__(kIcpThreads)
-icp_fused_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
-                 const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev, IcpFrame f,
-                 IcpState* __restrict__ st, double* __restrict__ partials, int tag, double conv_rot,
-                 double conv_trans) {
-    __shared__ double wsum[kIcpWaves][32];
-    __shared__ double total[32];
-    __shared__ bool last;
+icp_sums_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
+                const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev, IcpFrame f,
+                const IcpState* __restrict__ st, void* __restrict__ partials, int tag) {
+    __shared__ double wsum_d[kIcpWaves][32];
+    __shared__ float2 wsum_f[kIcpWaves][32];
     if (tag >= 0 && st->converged_tag == tag) return;  // uniform across the grid
     if (f.prev_from_state) {
         for (int i = 0; i < 9; ++i) f.rprev_inv[i] = st->Rpi[i];
@@ -353,65 +446,83 @@ icp_fused_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ 
         np[q] = nprev[js];
         vp[q] = vprev[js];
     }
+    float rows[P][7];
+#pragma unroll
+    for (int q = 0; q < P; ++q) icp_residual(n[q], g[q], np[q], vp[q], Rc, f, ok[q], rows[q]);
+    if constexpr (M == kSumDouble) {
+        double acc[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            int k = 0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+#pragma unroll
+                for (int jj = i; jj < 7; ++jj, ++k) acc[k] += (double)(rows[q][i] * rows[q][jj]);
+        }
+        const double s = workgroup_sum27(acc, wsum_d);
+        if (threadIdx.x < kIcpEntries) static_cast<double*>(partials)[blockIdx.x * kIcpEntries + threadIdx.x] = s;
+    } else {
+        float sum[32], comp[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k) { sum[k] = 0.0f; comp[k] = 0.0f; }
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            int k = 0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+#pragma unroll
+                for (int jj = i; jj < 7; ++jj, ++k) {
+                    const float prod = rows[q][i] * rows[q][jj];
+                    if constexpr (M == kSumKahan) kahan_add(sum[k], comp[k], prod);
+                    else sum[k] += prod;
+                }
+        }
+        float2 r;
+        if constexpr (M == kSumKahan) r = workgroup_sum27(sum, comp, wsum_f);
+        else r = make_float2(workgroup_sum27(sum, reinterpret_cast<float (*)[32]>(wsum_f)), 0.0f);
+        if (threadIdx.x < kIcpEntries) static_cast<float2*>(partials)[blockIdx.x * kIcpEntries + threadIdx.x] = r;
+    }
+}
+
+// One ICP iteration, second half (one workgroup): thread t adds the totals of
+// workgroups t, t + 256, ... in double precision in order, each round issuing
+// the 27 loads of one workgroup's totals together; then the fixed-order
+// workgroup sum in double precision, the Cholesky solve and the pose update.
+// The result does not depend on the order in which the first half's
+// workgroups ran.
+template <int M>
+__global__ void __launch_bounds__(kIcpThreads)
+icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials, int nblk, int tag,
+                 double conv_rot, double conv_trans) {
+    __shared__ double wsum[kIcpWaves][32];
+    __shared__ double total[32];
+    if (tag >= 0 && st->converged_tag == tag) return;
     double acc[32];
 #pragma unroll
-    for (int k = 0; k < 32; ++k) acc[k] = 0.0;
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-        float row[7];
-        icp_residual(n[q], g[q], np[q], vp[q], Rc, f, ok[q], row);
-        int k = 0;
-#pragma unroll
-        for (int i = 0; i < 6; ++i)
-#pragma unroll
-            for (int jj = i; jj < 7; ++jj, ++k) acc[k] += (double)(row[i] * row[jj]);
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const double wtot = wave_reduce_scatter32(acc, lane);
-    if ((lane & 1) == 0) wsum[wave][lane >> 1] = wtot;
-    __syncthreads();
-    if (threadIdx.x < kIcpEntries) {
-        double s = wsum[0][threadIdx.x];
-#pragma unroll
-        for (int w = 1; w < kIcpWaves; ++w) s += wsum[w][threadIdx.x];
-        partials[blockIdx.x * kIcpEntries + threadIdx.x] = s;
-    }
-    // Publish the partials, then take a ticket; the last workgroup reduces.
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = (atomicAdd(&st->ticket, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // Thread t sums the totals of workgroups t, t + 256, ... in order; each
-    // round issues the 27 loads of one workgroup's totals together, so the
-    // last workgroup waits on memory once per 256 workgroups.  Then the same
-    // fixed-order wave and workgroup reduction as above.
-#pragma unroll
     for (int e = 0; e < 32; ++e) acc[e] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += kIcpThreads) {
-        double t[kIcpEntries];
+    for (int b = threadIdx.x; b < nblk; b += kIcpThreads) {
+        if constexpr (M == kSumDouble) {
+            const double* pd = static_cast<const double*>(partials) + b * kIcpEntries;
+            double t[kIcpEntries];
 #pragma unroll
-        for (int e = 0; e < kIcpEntries; ++e) t[e] = partials[b * kIcpEntries + e];
+            for (int e = 0; e < kIcpEntries; ++e) t[e] = pd[e];
 #pragma unroll
-        for (int e = 0; e < kIcpEntries; ++e) acc[e] += t[e];
-    }
-    const double gtot = wave_reduce_scatter32(acc, lane);
-    __syncthreads();
-    if ((lane & 1) == 0) wsum[wave][lane >> 1] = gtot;
-    __syncthreads();
-    if (threadIdx.x < kIcpEntries) {
-        double s = wsum[0][threadIdx.x];
+            for (int e = 0; e < kIcpEntries; ++e) acc[e] += t[e];
+        } else {
+            const float2* pf = static_cast<const float2*>(partials) + b * kIcpEntries;
+            float2 t[kIcpEntries];
 #pragma unroll
-        for (int w = 1; w < kIcpWaves; ++w) s += wsum[w][threadIdx.x];
-        total[threadIdx.x] = s;
+            for (int e = 0; e < kIcpEntries; ++e) t[e] = pf[e];
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t[e].x + (double)t[e].y;
+        }
     }
+    const double s = workgroup_sum27(acc, wsum);
+    if (threadIdx.x < kIcpEntries) total[threadIdx.x] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        solve_and_update(st, total, tag, conv_rot, conv_trans);
-        st->ticket = 0u;
-        __threadfence();
-    }
+    if (threadIdx.x == 0) solve_and_update(st, total, tag, conv_rot, conv_trans);
 }
 
 // Start of a frame: the current estimate becomes the previous pose (as R^T,
@@ -444,7 +555,6 @@ __global__ void icp_reset_kernel(IcpState* st, kfx_pose pose) {
     st->singular_count = 0;
     st->converged_tag = -1;
     st->iterations = 0;
-    st->ticket = 0u;
 }
 
 IcpFrame make_frame(const kfx_matrix_3f* rprev_inv, const kfx_point_3f* tprev, const kfx_intrinsics* intr,
@@ -470,11 +580,22 @@ IcpFrame make_frame(const kfx_matrix_3f* rprev_inv, const kfx_point_3f* tprev, c
 
 struct kfx_icp {
     kfx::IcpState* state;
-    double* partials;  // [workgroup][27] sums of the fused iteration
+    void* partials;    // [workgroup][27] sums of the fused iteration (float2 or double)
+    int sum_mode;      // KFX_ICP_SUM_*
     int max_blocks;
 };
 
 using namespace kfx;
+
+template <int M>
+static void icp_iterate_launch(kfx_icp* h, const kfx_xyz* vc, const kfx_xyz* nc, const kfx_xyz* vp, const kfx_xyz* np,
+                               const IcpFrame& f, int nblk, bool two, int tag, double conv_rot, double conv_trans) {
+    if (two)
+        icp_sums_kernel<2, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
+    else
+        icp_sums_kernel<1, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
+    icp_solve_kernel<M><<<1, kIcpThreads, 0, stream()>>>(h->state, h->partials, nblk, tag, conv_rot, conv_trans);
+}
 
 extern "C" {
 
@@ -534,6 +655,7 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
     kfx_icp* h = new kfx_icp();
     h->max_blocks = (int)divup((unsigned)max_pixels, (unsigned)kIcpThreads);
     hipError_t e = hipMalloc((void**)&h->state, sizeof(IcpState));
+    h->sum_mode = KFX_ICP_SUM_FP32_KAHAN;
     if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(double) * kIcpEntries * (size_t)h->max_blocks);
     if (e == hipSuccess) e = hipMemsetAsync(h->state, 0, sizeof(IcpState), stream());
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
@@ -586,15 +708,24 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
     IcpFrame f = make_frame(rotation_previous_inverse, translation_previous, intr, distance_threshold,
                             angle_threshold);
     ProfScope prof("icp_fused");
-    if (two)
-        icp_fused_kernel<2><<<nblk, kIcpThreads, 0, stream()>>>(vmap_current, nmap_current, vmap_previous,
-                                                                nmap_previous, f, h->state, h->partials, level_tag,
-                                                                converge_rotation, converge_translation);
-    else
-        icp_fused_kernel<1><<<nblk, kIcpThreads, 0, stream()>>>(vmap_current, nmap_current, vmap_previous,
-                                                                nmap_previous, f, h->state, h->partials, level_tag,
-                                                                converge_rotation, converge_translation);
+    auto launch = [&](auto mode) {
+        icp_iterate_launch<decltype(mode)::value>(h, vmap_current, nmap_current, vmap_previous, nmap_previous, f,
+                                                  nblk, two, level_tag, converge_rotation, converge_translation);
+    };
+    switch (h->sum_mode) {
+        case KFX_ICP_SUM_FP32: launch(std::integral_constant<int, kSumPlain>{}); break;
+        case KFX_ICP_SUM_FP64: launch(std::integral_constant<int, kSumDouble>{}); break;
+        default: launch(std::integral_constant<int, kSumKahan>{}); break;
+    }
     KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_icp_set_sum_mode(kfx_icp* h, int mode) {
+    KFX_REQUIRE(h != nullptr, "kfx_icp_set_sum_mode: null handle");
+    KFX_REQUIRE(mode == KFX_ICP_SUM_FP32_KAHAN || mode == KFX_ICP_SUM_FP32 || mode == KFX_ICP_SUM_FP64,
+                "kfx_icp_set_sum_mode: unknown mode %d", mode);
+    h->sum_mode = mode;
     return KFX_OK;
 }
 

@@ -127,12 +127,6 @@ double motion(const double* R, const double* t, const double* Rp, const double* 
     return 0.5 * (ang + d);
 }
 
-#define KFX_TRY(expr)                 \
-    do {                              \
-        int kfx_rc_ = (expr);         \
-        if (kfx_rc_) return kfx_rc_;  \
-    } while (0)
-
 // The reference's execute() ends every kernel with clFinish; plain mode
 // mirrors that.
 int step_sync(const kfx_pipeline* p) {
@@ -464,6 +458,7 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     }
     int rc = kfx_icp_create((int)npix, &p->icp);
     if (!rc) rc = kfx_slicer_create(&rb, &p->slicer);
+    if (!rc) rc = integrate_reserve(p->cfg.intr.cols, p->cfg.intr.rows);
     if (!rc) rc = kfx_init_tsdf(p->vol, &rb);
     if (!rc) rc = kfx_synchronize();
     if (rc) {

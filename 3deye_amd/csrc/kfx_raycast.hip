@@ -58,6 +58,21 @@ __device__ __forceinline__ float read_tsdf(const kfx_tsdf* __restrict__ vol, con
     return tsdf_decode(read_raw(vol, rb, x, y, z));
 }
 
+// Samples of the march whose loads are issued together.
+constexpr int kMarchBatch = 4;
+
+// Linear index of in-range voxel coordinates after the cyclic offset: x + o
+// is below 2N, so min(x + o, x + o - N) in unsigned arithmetic is the wrap.
+__device__ __forceinline__ uint32_t march_address(uint32_t x, uint32_t y, uint32_t z, const kfx_round_buffer& rb) {
+    const uint32_t nx = (uint32_t)rb.voxels_size.x, ny = (uint32_t)rb.voxels_size.y, nz = (uint32_t)rb.voxels_size.z;
+    uint32_t rx = x + (uint32_t)rb.origin_GRID.x, ry = y + (uint32_t)rb.origin_GRID.y,
+             rz = z + (uint32_t)rb.origin_GRID.z;
+    rx = min(rx, rx - nx);
+    ry = min(ry, ry - ny);
+    rz = min(rz, rz - nz);
+    return (rz * ny + ry) * nx + rx;
+}
+
 // p / cell, correctly rounded, without a division: q0 = p * (1/cell) is
 // within an ulp of the quotient, the residual p - q0 * cell is exact by FMA,
 // and one correction with the correctly rounded reciprocal gives the
@@ -178,53 +193,97 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
         int tsdf = read_raw(vol, rb, gx, gy, gz);
         const float max_time = 3.0f * (vol_m.x + vol_m.y + vol_m.z);
         const float step = p.time_step;
+        const float fnx = (float)nxv, fny = (float)nyv, fnz = (float)nzv;
 
-        for (; time_curr < max_time; time_curr += step) {
-            const int tsdf_prev = tsdf;
-            const float t_next = time_curr + step;
-            const f3 q1 = ray_at(start, dir, t_next);
-            const int vx = voxel_floor(q1.x, p.csx, p.icsx), vy = voxel_floor(q1.y, p.csy, p.icsy),
-                      vz = voxel_floor(q1.z, p.csz, p.icsz);
-            if (!((unsigned)vx < (unsigned)nxv && (unsigned)vy < (unsigned)nyv && (unsigned)vz < (unsigned)nzv)) break;
-            tsdf = read_raw(vol, rb, vx, vy, vz);
-            if (tsdf_prev < 0 && tsdf > 0) break;
-            if (tsdf_prev > 0 && tsdf < 0) {
+        // The reference loop, for (; t < max_time; t += step) { sample at
+        // t + step; stop outside the volume; stop on - to +; refine on + to -
+        // }, taken kMarchBatch samples at a time: the positions and loads of a
+        // batch are issued first (loads stop at the first sample that leaves
+        // the volume or the time range), then the samples are tested in order.
+        // The sample times are the same running float sums as the reference's.
+        // March samples are finite (finite origin and direction, bounded t),
+        // so the floor is tested against [0, N) in float and converted only
+        // then, which equals the saturating convert_int_sat_rtn test.
+        bool found = false;
+        float hit_tc = 0.0f, hit_tn = 0.0f;
+        for (bool done = false; !done;) {
+            float tc[kMarchBatch], tn[kMarchBatch];
+            bool ok[kMarchBatch];
+            uint32_t addr[kMarchBatch];
+            float t = time_curr;
+            bool alive = true;
+#pragma unroll
+            for (int k = 0; k < kMarchBatch; ++k) {
+                tc[k] = t;
+                tn[k] = t + step;
+                const float fx = floorf(div_cell(start.x + dir.x * tn[k], p.csx, p.icsx));
+                const float fy = floorf(div_cell(start.y + dir.y * tn[k], p.csy, p.icsy));
+                const float fz = floorf(div_cell(start.z + dir.z * tn[k], p.csz, p.icsz));
+                alive = alive & (t < max_time) & (fx >= 0.0f) & (fx < fnx) & (fy >= 0.0f) & (fy < fny) &
+                        (fz >= 0.0f) & (fz < fnz);
+                ok[k] = alive;
+                addr[k] = alive ? march_address((uint32_t)(int)fx, (uint32_t)(int)fy, (uint32_t)(int)fz, rb) : 0u;
+                t = tn[k];
+            }
+            int16_t val[kMarchBatch];
+#pragma unroll
+            for (int k = 0; k < kMarchBatch; ++k) val[k] = ok[k] ? vol[addr[k]].value : (int16_t)0;
+#pragma unroll
+            for (int k = 0; k < kMarchBatch; ++k) {
+                // !ok: the time range ended or the sample left the volume.
+                if (!ok[k]) { done = true; break; }
+                const int tsdf_prev = tsdf;
+                tsdf = val[k];
+                if (tsdf_prev < 0 && tsdf > 0) { done = true; break; }
+                if (tsdf_prev > 0 && tsdf < 0) {
+                    found = true;
+                    hit_tc = tc[k];
+                    hit_tn = tn[k];
+                    done = true;
+                    break;
+                }
+            }
+            time_curr = t;
+        }
+        if (found) {
+            do {
+                const float t_cur = hit_tc;
+                const f3 q1 = ray_at(start, dir, hit_tn);
                 const float ftdt = interpolate_trilinear(vol, q1, p);
                 if (isnan(ftdt)) break;
-                const f3 q2 = ray_at(start, dir, time_curr);
+                const f3 q2 = ray_at(start, dir, t_cur);
                 const float ft = interpolate_trilinear(vol, q2, p);
                 if (isnan(ft)) break;
-                const float ts = time_curr - step * ft / (ftdt - ft);
+                const float ts = t_cur - step * ft / (ftdt - ft);
                 const f3 vert = ray_at(start, dir, ts);
                 vout = kfx_xyz{vert.x, vert.y, vert.z};
 
-                const f3 q3 = ray_at(start, dir, time_curr);
+                const f3 q3 = ray_at(start, dir, t_cur);
                 const int hx = voxel_floor(q3.x, p.csx, p.icsx), hy = voxel_floor(q3.y, p.csy, p.icsy),
                           hz = voxel_floor(q3.z, p.csz, p.icsz);
                 if (hx > 1 && hy > 1 && hz > 1 && hx < nxv - 2 && hy < nyv - 2 && hz < nzv - 2) {
-                    f3 t = vert;
-                    t.x += p.csx;
-                    const float fx1 = interpolate_trilinear(vol, t, p);
-                    t = vert;
-                    t.x -= p.csx;
-                    const float fx2 = interpolate_trilinear(vol, t, p);
-                    t = vert;
-                    t.y += p.csy;
-                    const float fy1 = interpolate_trilinear(vol, t, p);
-                    t = vert;
-                    t.y -= p.csy;
-                    const float fy2 = interpolate_trilinear(vol, t, p);
-                    t = vert;
-                    t.z += p.csz;
-                    const float fz1 = interpolate_trilinear(vol, t, p);
-                    t = vert;
-                    t.z -= p.csz;
-                    const float fz2 = interpolate_trilinear(vol, t, p);
+                    f3 tt = vert;
+                    tt.x += p.csx;
+                    const float fx1 = interpolate_trilinear(vol, tt, p);
+                    tt = vert;
+                    tt.x -= p.csx;
+                    const float fx2 = interpolate_trilinear(vol, tt, p);
+                    tt = vert;
+                    tt.y += p.csy;
+                    const float fy1 = interpolate_trilinear(vol, tt, p);
+                    tt = vert;
+                    tt.y -= p.csy;
+                    const float fy2 = interpolate_trilinear(vol, tt, p);
+                    tt = vert;
+                    tt.z += p.csz;
+                    const float fz1 = interpolate_trilinear(vol, tt, p);
+                    tt = vert;
+                    tt.z -= p.csz;
+                    const float fz2 = interpolate_trilinear(vol, tt, p);
                     const f3 n = normalized3(f3{fx1 - fx2, fy1 - fy2, fz1 - fz2});
                     nout = kfx_xyz{n.x, n.y, n.z};
                 }
-                break;
-            }
+            } while (false);
         }
     }
     vmap[idx] = vout;

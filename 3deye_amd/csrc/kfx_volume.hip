@@ -21,6 +21,9 @@ constexpr int kIntegrateBlockX = 64;
 constexpr int kIntegrateBlockY = 4;
 // Voxels of a run whose loads are issued before any of them is updated.
 constexpr int kIntegrateBatch = 8;
+// Coarsest level of the depth-bound pyramid (texels of 32 x 32 pixels).
+constexpr int kBoundLevels = 5;
+constexpr int kBoundTile = 1 << kBoundLevels;
 
 struct IntegrateParams {
     float r[9];          // rotation_inverse, row-major
@@ -36,6 +39,11 @@ struct IntegrateParams {
     // a voxel centre g (relative to the camera) can project into the image
     // only if dot(plane[k], g) >= 0 for every k (near, left, right, top, bottom).
     float plane[5][3];
+    // Depth-bound pyramid (null: no culling): level l in 1..kBoundLevels
+    // holds, per texel, the largest scaled depth over its 2^l x 2^l pixels,
+    // at bound + bound_off[l]; level 0 is the scaled depth itself.
+    const float* bound;
+    int bound_off[kBoundLevels + 1];
 };
 
 // Range [lo, hi] of z in [z0, z1) whose voxel centres of column (gx, gy) lie
@@ -142,6 +150,82 @@ __device__ __forceinline__ bool load_device_pose(IntegrateParams& p, const float
     return true;
 }
 
+// Depth-bound pyramid: one 32 x 32 pixel tile per workgroup, levels 1..5 by
+// repeated 2 x 2 maxima in LDS.  Pixels outside the image count as 0 (no
+// depth).
+__global__ void __launch_bounds__(kBoundTile * kBoundTile)
+depth_bound_kernel(const float* __restrict__ scaled, int cols, int rows, float* __restrict__ bound,
+                   IntegrateParams p) {
+    __shared__ float t[kBoundTile * kBoundTile];
+    const int tx = threadIdx.x % kBoundTile, ty = threadIdx.x / kBoundTile;
+    const int u = blockIdx.x * kBoundTile + tx, v = blockIdx.y * kBoundTile + ty;
+    t[threadIdx.x] = (u < cols && v < rows) ? scaled[v * cols + u] : 0.0f;
+    __syncthreads();
+    for (int l = 1; l <= kBoundLevels; ++l) {
+        const int n = kBoundTile >> l;  // texels per tile side at level l
+        float m = 0.0f;
+        if (tx < n && ty < n) {
+            const int s = kBoundTile >> (l - 1);  // side of the previous level's tile in t[]
+            m = fmaxf(fmaxf(t[(2 * ty) * s + 2 * tx], t[(2 * ty) * s + 2 * tx + 1]),
+                      fmaxf(t[(2 * ty + 1) * s + 2 * tx], t[(2 * ty + 1) * s + 2 * tx + 1]));
+        }
+        __syncthreads();
+        if (tx < n && ty < n) {
+            t[ty * n + tx] = m;
+            const int lw = (cols + (1 << l) - 1) >> l, lh = (rows + (1 << l) - 1) >> l;
+            const int a = blockIdx.x * n + tx, b = blockIdx.y * n + ty;
+            if (a < lw && b < lh) bound[p.bound_off[l] + b * lw + a] = m;
+        }
+        __syncthreads();
+    }
+}
+
+// True when no voxel of the column segment z in [za, zb] can be updated:
+// every voxel centre is behind the camera, or the segment projects (with a
+// one-pixel margin) only onto pixels without depth, or it lies farther from
+// the camera than the largest depth under its footprint plus the truncation
+// distance (the rule needs dist <= depth + trunc).  Conservative: false
+// whenever the test cannot decide.
+__device__ __forceinline__ bool segment_culled(const IntegrateParams& p, const float* __restrict__ scaled,
+                                               float gx, float gy, float cxy0, float cxy1, float cxy2, int za,
+                                               int zb) {
+    const float gza = ((float)za + 0.5f) * p.csz - p.tz, gzb = ((float)zb + 0.5f) * p.csz - p.tz;
+    const float ax = cxy0 + p.r[2] * gza, ay = cxy1 + p.r[5] * gza, az = cxy2 + p.r[8] * gza;
+    const float bx = cxy0 + p.r[2] * gzb, by = cxy1 + p.r[5] * gzb, bz = cxy2 + p.r[8] * gzb;
+    if (az <= 0.0f && bz <= 0.0f) return true;
+    if (!(az > 1e-4f && bz > 1e-4f)) return false;
+    const float ia = __builtin_amdgcn_rcpf(az), ib = __builtin_amdgcn_rcpf(bz);
+    const float ua = ax * p.fx * ia + p.cx, ub = bx * p.fx * ib + p.cx;
+    const float va = ay * p.fy * ia + p.cy, vb = by * p.fy * ib + p.cy;
+    const float fu0 = floorf(fminf(ua, ub)) - 1.0f, fu1 = ceilf(fmaxf(ua, ub)) + 1.0f;
+    const float fv0 = floorf(fminf(va, vb)) - 1.0f, fv1 = ceilf(fmaxf(va, vb)) + 1.0f;
+    if (fu1 < 0.0f || fv1 < 0.0f || fu0 > (float)(p.cols - 1) || fv0 > (float)(p.rows - 1)) return true;
+    const int u0 = (int)fmaxf(fu0, 0.0f), u1 = (int)fminf(fu1, (float)(p.cols - 1));
+    const int v0 = (int)fmaxf(fv0, 0.0f), v1 = (int)fminf(fv1, (float)(p.rows - 1));
+    const int span = max(u1 - u0, v1 - v0) + 1;
+    const int l = span <= 1 ? 0 : 32 - __clz(span - 1);  // 2^l >= span: at most 2 x 2 texels
+    if (l > kBoundLevels) return false;
+    float maxd;
+    if (l == 0) {
+        maxd = scaled[v0 * p.cols + u0];
+    } else {
+        const int lw = (p.cols + (1 << l) - 1) >> l;
+        int off = p.bound_off[1];
+        off = l == 2 ? p.bound_off[2] : off;
+        off = l == 3 ? p.bound_off[3] : off;
+        off = l == 4 ? p.bound_off[4] : off;
+        off = l == 5 ? p.bound_off[5] : off;
+        const float* b = p.bound + off;
+        const int a0 = u0 >> l, a1 = u1 >> l, b0 = v0 >> l, b1 = v1 >> l;
+        maxd = fmaxf(fmaxf(b[b0 * lw + a0], b[b0 * lw + a1]), fmaxf(b[b1 * lw + a0], b[b1 * lw + a1]));
+    }
+    if (maxd == 0.0f) return true;
+    // Nearest point of the segment to the camera, bounded below.
+    const float dz = (gza <= 0.0f && gzb >= 0.0f) ? 0.0f : fminf(fabsf(gza), fabsf(gzb));
+    const float dmin = sqrtf(gx * gx + gy * gy + dz * dz);
+    return dmin > (maxd + p.trunc) * 1.0001f + 1e-4f;
+}
+
 template <bool kDevicePose>
 __global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
 integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume, IntegrateParams p,
@@ -163,25 +247,63 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
     int lo, hi;
     clip_column(p, gx, gy, z_begin, z_end, lo, hi);
     if (!inside) { lo = z_end; hi = z_begin - 1; }
+    const int own_lo = lo, own_hi = hi;
     lo = wave_min(lo);
     hi = wave_max(hi);
     if (!inside) return;
 
+    // Column terms of the camera coordinates: the rule's dot products are
+    // evaluated as (r0 * gx + r1 * gy) + r2 * gz, so the first two terms are
+    // shared by the whole column with identical rounding.
+    const float cxy0 = p.r[0] * gx + p.r[1] * gy;
+    const float cxy1 = p.r[3] * gx + p.r[4] * gy;
+    const float cxy2 = p.r[6] * gx + p.r[7] * gy;
+    const float fcols = (float)p.cols, frows = (float)p.rows;
+    const uint32_t nz = (uint32_t)p.nz;
     for (int z0 = lo; z0 <= hi; z0 += kIntegrateBatch) {
+        // Skip the batch when no lane can update a voxel of it (every lane's
+        // part of the batch inside its own clipped range is culled).
+        if (p.bound != nullptr) {
+            const int za = max(z0, own_lo), zb = min(z0 + kIntegrateBatch - 1, own_hi);
+            const bool culled = za > zb || segment_culled(p, scaled, gx, gy, cxy0, cxy1, cxy2, za, zb);
+            if (__all(culled)) continue;
+        }
+        float dp[kIntegrateBatch], dist[kIntegrateBatch];
+        bool inimg[kIntegrateBatch];
+        // Phase 1 (no branches): projection and depth gather of the batch.
+        // Pixels outside the image read pixel 0 and are masked afterwards.
+#pragma unroll
+        for (int k = 0; k < kIntegrateBatch; ++k) {
+            const int z = z0 + k;
+            const float gz = ((float)z + 0.5f) * p.csz - p.tz;
+            const float camx = cxy0 + p.r[2] * gz;
+            const float camy = cxy1 + p.r[5] * gz;
+            const float camz = cxy2 + p.r[8] * gz;
+            const float inv_z = 1.0f / camz;
+            const float fu = rintf(camx * p.fx * inv_z + p.cx);
+            const float fv = rintf(camy * p.fy * inv_z + p.cy);
+            const bool in = (z <= hi) & (camz > 0.0f) & (fu >= 0.0f) & (fu < fcols) & (fv >= 0.0f) & (fv < frows);
+            const int pix = (int)(in ? fv : 0.0f) * p.cols + (int)(in ? fu : 0.0f);
+            dp[k] = scaled[pix];
+            dist[k] = sqrtf(gz * gz + part);
+            inimg[k] = in;
+        }
+        // Phase 2: the update test, and the loads of the voxels it selects.
         float tsdf_new[kIntegrateBatch];
         bool upd[kIntegrateBatch];
         kfx_tsdf prev[kIntegrateBatch];
         uint32_t addr[kIntegrateBatch];
-        // Phase 1: projection, depth gather and voxel loads for the batch.
 #pragma unroll
         for (int k = 0; k < kIntegrateBatch; ++k) {
-            const int z = z0 + k;
-            bool band = false;
-            upd[k] = (z <= hi) && integrate_voxel(p, gx, gy, part, z, scaled, tsdf_new[k], band);
-            addr[k] = col + p.slice * (uint32_t)wrap_once(z + p.oz, p.nz);
+            const float sdf = dp[k] - dist[k];
+            upd[k] = inimg[k] & (dp[k] != 0.0f) & (sdf >= -p.trunc);
+            tsdf_new[k] = fminf(1.0f, sdf * p.inv_trunc);
+            uint32_t rz = (uint32_t)(z0 + k) + (uint32_t)p.oz;
+            rz = min(rz, rz - nz);  // (z + oz) mod nz for z + oz < 2 nz
+            addr[k] = col + p.slice * rz;
             if (upd[k]) prev[k] = volume[addr[k]];
         }
-        // Phase 2: running weighted average, quantise, write what changed.
+        // Phase 3: running weighted average, quantise, write what changed.
 #pragma unroll
         for (int k = 0; k < kIntegrateBatch; ++k) {
             if (!upd[k]) continue;
@@ -283,6 +405,54 @@ int kfx_init_tsdf(kfx_tsdf* volume, const kfx_round_buffer* rb) {
 
 namespace kfx {
 
+// Library-owned depth-bound pyramid (one per process, grown on demand).
+static float* g_bound = nullptr;
+static size_t g_bound_cap = 0;  // floats
+
+static size_t bound_layout(int cols, int rows, int off[kBoundLevels + 1]) {
+    size_t total = 0;
+    off[0] = 0;
+    for (int l = 1; l <= kBoundLevels; ++l) {
+        off[l] = (int)total;
+        total += (size_t)((cols + (1 << l) - 1) >> l) * (size_t)((rows + (1 << l) - 1) >> l);
+    }
+    return total;
+}
+
+int integrate_reserve(int cols, int rows) {
+    int off[kBoundLevels + 1];
+    const size_t need = bound_layout(cols, rows, off);
+    if (need <= g_bound_cap) return KFX_OK;
+    if (g_bound) KFX_CHECK_HIP(hipFree(g_bound));
+    g_bound = nullptr;
+    g_bound_cap = 0;
+    KFX_CHECK_HIP(hipMalloc((void**)&g_bound, need * sizeof(float)));
+    g_bound_cap = need;
+    return KFX_OK;
+}
+
+// Builds the depth-bound pyramid of depth_scaled into the library buffer and
+// points p at it.  Inside a stream capture the buffer cannot be allocated:
+// without a large enough reservation the integration then runs unculled
+// (same result, more work).
+static int integrate_bound(const float* depth_scaled, const kfx_intrinsics* intr, IntegrateParams& p) {
+    p.bound = nullptr;
+    const size_t need = bound_layout(intr->cols, intr->rows, p.bound_off);
+    if (need > g_bound_cap) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
+        if (cs != hipStreamCaptureStatusNone) return KFX_OK;
+        KFX_TRY(integrate_reserve(intr->cols, intr->rows));
+    }
+    ProfScope prof("depth_bound");
+    dim3 grid(divup(intr->cols, kBoundTile), divup(intr->rows, kBoundTile));
+    depth_bound_kernel<<<grid, kBoundTile * kBoundTile, 0, stream()>>>(depth_scaled, intr->cols, intr->rows, g_bound,
+                                                                        p);
+    KFX_CHECK_LAUNCH();
+    p.bound = g_bound;
+    return KFX_OK;
+}
+
 static int integrate_setup(const char* who, const uint16_t* depth, float* depth_scaled,
                            const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
                            const kfx_point_3f* translation, float truncation_distance,
@@ -339,6 +509,7 @@ static int integrate_setup(const char* who, const uint16_t* depth, float* depth_
             p.plane[4][k] = (float)(-intr->fy * ri[1][k] + cb * ri[2][k]);
         }
     }
+    p.bound = nullptr;
     return KFX_OK;
 }
 
@@ -355,6 +526,7 @@ int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* vol
     int rc = integrate_setup("kfx_integrate_tsdf", depth, depth_scaled, intr, rotation_inverse, translation,
                              truncation_distance, cell_size, rb, p);
     if (rc) return rc;
+    KFX_TRY(integrate_bound(depth_scaled, intr, p));
     ProfScope prof("integrate");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
     dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
@@ -374,6 +546,7 @@ int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsd
     int rc = integrate_setup("kfx_integrate_tsdf_posed", depth, depth_scaled, intr, &ident, &zero,
                              truncation_distance, cell_size, rb, p);
     if (rc) return rc;
+    KFX_TRY(integrate_bound(depth_scaled, intr, p));
     ProfScope prof("integrate");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
     dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));

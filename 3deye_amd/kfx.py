@@ -110,6 +110,7 @@ class PipelineView(C.Structure):
 
 
 PLAIN, FUSED = 0, 1
+ICP_SUM_FP32_KAHAN, ICP_SUM_FP32, ICP_SUM_FP64 = 0, 1, 2
 
 assert C.sizeof(Intrinsics) == 24 and C.sizeof(RoundBuffer) == 72 and C.sizeof(Matrix3f) == 36
 
@@ -217,6 +218,7 @@ _PROTOS = {
                                    _P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "kfx_clean_buff": (_I, [_P, _I]),
     "kfx_icp_create": (_I, [_I, C.POINTER(_P)]),
+    "kfx_icp_set_sum_mode": (_I, [_P, _I]),
     "kfx_icp_destroy": (_I, [_P]),
     "kfx_icp_reset": (_I, [_P, C.POINTER(Pose)]),
     "kfx_icp_iterate": (_I, [_P, _P, _P, _P, _P, C.POINTER(Matrix3f), C.POINTER(Point3f), C.POINTER(Intrinsics),

@@ -199,8 +199,8 @@ int kfx_clean_buff(float* gbuf, int width);
 
 /* Fused ICP (no reference counterpart as one call: it replaces the
  * Estimate_Combined + host solve + pose update loop of the absent fusion
- * application).  The handle owns the device-resident pose (double), the
- * block partials and the last-workgroup ticket. */
+ * application).  The handle owns the device-resident pose (double) and the
+ * per-workgroup partial sums. */
 typedef struct kfx_icp kfx_icp;
 typedef struct kfx_icp_result {
     kfx_pose pose;         /* current estimate, camera -> volume */
@@ -212,12 +212,22 @@ typedef struct kfx_icp_result {
     int32_t converged_tag; /* level tag that met the convergence test, -1 none */
 } kfx_icp_result;
 
+/* Precision of the per-workgroup sums of the fused iteration; the
+ * cross-workgroup sum, the 6x6 solve and the pose update are FP64 in every
+ * mode.  FP32 Jacobian rows and products in every mode. */
+#define KFX_ICP_SUM_FP32_KAHAN 0 /* compensated (TwoSum) FP32 sums (default) */
+#define KFX_ICP_SUM_FP32 1       /* plain FP32 sums */
+#define KFX_ICP_SUM_FP64 2       /* FP64 sums */
+
 int kfx_icp_create(int max_pixels, kfx_icp** out);
 int kfx_icp_destroy(kfx_icp* icp);
 /* Set the estimate (asynchronous, on the library stream). */
 int kfx_icp_reset(kfx_icp* icp, const kfx_pose* pose);
-/* One iteration: FP32 Jacobian rows and products summed in FP64 (fixed
- * order), FP64 cross-workgroup sum in the last workgroup, FP64 6x6 Cholesky
+/* Selects KFX_ICP_SUM_* for later iterations (host-side setting). */
+int kfx_icp_set_sum_mode(kfx_icp* icp, int mode);
+/* One iteration (two launches): FP32 Jacobian rows and products, workgroup
+ * sums in the handle's KFX_ICP_SUM_* precision (fixed order), FP64
+ * cross-workgroup sum in one workgroup, FP64 6x6 Cholesky
  * solve and pose composition on the GPU.  A singular system leaves the pose
  * unchanged and records KFX_ERR_SINGULAR.  When level_tag >= 0 and an
  * iteration's increment is below (converge_rotation rad, converge_translation

@@ -1,6 +1,6 @@
 """GPU parity of the ICP paths against the oracle: the reference-style
-Estimate_Combined reduction bit for bit; the fused FP32-compensated / FP64
-path's normal equations, solve and pose update against double precision; a
+Estimate_Combined reduction bit for bit; the fused path's normal equations
+(compensated FP32, plain FP32 and FP64 workgroup sums), solve and pose update against double precision; a
 coarse-to-fine run within 1e-5 rad / 1e-5 m of the oracle's FP64 ICP; the
 singular-system and early-exit behaviour."""
 import ctypes as C
@@ -57,11 +57,17 @@ def _pose(kfx, R, t):
     return p
 
 
-def test_fused_iteration_matches_double(gpu, orc, pair):
+@pytest.mark.parametrize("mode,rtol", [(0, 1e-9), (2, 1e-9), (1, 1e-4)])
+def test_fused_iteration_matches_double(gpu, orc, pair, mode, rtol):
+    # mode 0: compensated FP32 workgroup sums (default), 2: FP64 sums, 1: plain
+    # FP32 sums.  All three are compared with the FP32 products summed in
+    # double precision; the compensated and FP64 sums agree to 1e-9, plain
+    # FP32 sums over ~1e5 products only to ~1e-5.
     kfx = gpu
     intr, maps, (Rp, tp), (Rc, tc) = pair
     bufs = _upload(kfx, maps)
     h = _icp_handle(kfx, intr.cols * intr.rows)
+    kfx.call("kfx_icp_set_sum_mode", h, mode)
     kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
     kfx.call("kfx_icp_iterate", h, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, C.byref(kfx.matrix3f(Rp.T)),
              C.byref(kfx.point3f(tp)), C.byref(intr), ii.DIST_THR, ii.ANGLE_THR, -1, 0.0, 0.0)
@@ -69,13 +75,21 @@ def test_fused_iteration_matches_double(gpu, orc, pair):
     kfx.call("kfx_icp_read", h, C.byref(res))
     s, n = orc.icp_sums(*maps, Rp, tp, Rp.T, tp, intr, ii.DIST_THR, ii.ANGLE_THR, precision=0)
     A, b = orc.unpack_sums(s)
-    np.testing.assert_allclose(np.array(res.A[:]).reshape(6, 6), A, rtol=1e-9, atol=1e-9 * np.abs(A).max())
-    np.testing.assert_allclose(np.array(res.b[:]), b, rtol=1e-9, atol=1e-9 * np.abs(b).max())
-    rc, R, t, x = orc.icp_solve_update(s, Rp, tp)
-    assert res.status == 0 and rc == 0
-    np.testing.assert_allclose(np.array(res.x[:]), x, atol=1e-9)
-    np.testing.assert_allclose(np.array(res.pose.R[:]).reshape(3, 3), R, atol=1e-9)
-    np.testing.assert_allclose(np.array(res.pose.t[:]), t, atol=1e-9)
+    np.testing.assert_allclose(np.array(res.A[:]).reshape(6, 6), A, rtol=rtol, atol=rtol * np.abs(A).max())
+    np.testing.assert_allclose(np.array(res.b[:]), b, rtol=rtol, atol=rtol * np.abs(b).max())
+    if mode != 1:
+        rc, R, t, x = orc.icp_solve_update(s, Rp, tp)
+        assert res.status == 0 and rc == 0
+        np.testing.assert_allclose(np.array(res.x[:]), x, atol=1e-9)
+        np.testing.assert_allclose(np.array(res.pose.R[:]).reshape(3, 3), R, atol=1e-9)
+        np.testing.assert_allclose(np.array(res.pose.t[:]), t, atol=1e-9)
+    kfx.call("kfx_icp_destroy", h)
+
+
+def test_icp_sum_mode_rejects_unknown(gpu):
+    kfx = gpu
+    h = _icp_handle(kfx, 1024)
+    assert kfx.lib.kfx_icp_set_sum_mode(h, 7) == 1
     kfx.call("kfx_icp_destroy", h)
 
 
