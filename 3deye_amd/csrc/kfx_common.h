@@ -20,6 +20,8 @@ namespace kfx {
 // Reserves the integration's depth-bound pyramid for images of cols x rows
 // (kfx_volume.hip), so that integrations inside a stream capture are culled.
 int integrate_reserve(int cols, int rows);
+// Reserves the depth-consistency gate's workgroup partials (kfx_denoise.hip).
+int consistency_reserve(int cols, int rows);
 
 // Library stream (created by kfx_init, default priority) on which every
 // entry point enqueues its work.

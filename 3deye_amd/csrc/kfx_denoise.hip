@@ -763,13 +763,20 @@ __global__ void __launch_bounds__(256) model_depth_kernel(const kfx_xyz* __restr
 // brought into the camera at the device pose and projected; where the frame
 // has depth at that pixel and the two differ by less than max_diff, the
 // squared difference is accumulated.  stats: [sum of squares (m^2), count].
+// Consistency of a depth frame with the raycast model, per 256-pixel
+// workgroup: the model vertices (volume coordinates) projected into the
+// camera at the device pose (12 floats: R camera -> volume, then t); where a
+// vertex lands on a pixel with depth, it counts towards n_model, and when the
+// depth is within max_diff of the vertex depth its squared difference and a
+// count are added too.  partials[g] = {sum e^2, count, n_model}.
 __global__ void __launch_bounds__(256) depth_consistency_kernel(const kfx_xyz* __restrict__ vmap,
                                                                 const float* __restrict__ dpose,
                                                                 const uint16_t* __restrict__ depth,
                                                                 kfx_intrinsics in, float max_diff,
-                                                                double* __restrict__ stats) {
+                                                                double* __restrict__ partials) {
+    __shared__ double red[4][3];
     const int p = blockIdx.x * 256 + threadIdx.x;
-    double s = 0.0, c = 0.0;
+    double s = 0.0, c = 0.0, m = 0.0;
     if (p < in.cols * in.rows) {
         const kfx_xyz v = vmap[p];
         if (!isnan(v.x)) {
@@ -782,6 +789,7 @@ __global__ void __launch_bounds__(256) depth_consistency_kernel(const kfx_xyz* _
                 if (fu >= 0.0f && fv >= 0.0f && fu < (float)in.cols && fv < (float)in.rows) {
                     const uint16_t d = depth[(int)fv * in.cols + (int)fu];
                     if (d) {
+                        m = 1.0;
                         const float e = (float)d * 0.001f - cz;
                         if (fabsf(e) < max_diff) {
                             s = (double)e * e;
@@ -795,25 +803,65 @@ __global__ void __launch_bounds__(256) depth_consistency_kernel(const kfx_xyz* _
     for (int off = 32; off > 0; off >>= 1) {
         s += __shfl_xor(s, off);
         c += __shfl_xor(c, off);
+        m += __shfl_xor(m, off);
     }
-    if ((threadIdx.x & 63) == 0 && c > 0.0) {
-        atomicAdd(&stats[0], s);
-        atomicAdd(&stats[1], c);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { red[wave][0] = s; red[wave][1] = c; red[wave][2] = m; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        partials[blockIdx.x * 3 + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
     }
 }
 
-__global__ void zero_doubles_kernel(double* p, int n) {
-    const int i = threadIdx.x;
-    if (i < n) p[i] = 0.0;
+// stats[k] = sum over workgroups of partials[g][k], in a fixed order (one
+// workgroup: thread t sums g = t, t + 256, ..., then a tree over the 256).
+__global__ void __launch_bounds__(256) consistency_reduce_kernel(const double* __restrict__ partials, int nblk,
+                                                                 double* __restrict__ stats) {
+    __shared__ double red[3][256];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int g = threadIdx.x; g < nblk; g += 256)
+        for (int k = 0; k < 3; ++k) a[k] += partials[g * 3 + k];
+    for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) stats[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Library-owned workgroup partials of the consistency gate.
+static double* g_cons = nullptr;
+static size_t g_cons_cap = 0;  // workgroups
+
+int consistency_reserve(int cols, int rows) {
+    const size_t need = divup((unsigned)(cols * rows), 256u);
+    if (need <= g_cons_cap) return KFX_OK;
+    if (g_cons) KFX_CHECK_HIP(hipFree(g_cons));
+    g_cons = nullptr;
+    g_cons_cap = 0;
+    KFX_CHECK_HIP(hipMalloc((void**)&g_cons, need * 3 * sizeof(double)));
+    g_cons_cap = need;
+    return KFX_OK;
 }
 
 int depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
                       const kfx_intrinsics* intr, float max_diff, double* dev_stats) {
-    zero_doubles_kernel<<<1, 64, 0, stream()>>>(dev_stats, 2);
-    KFX_CHECK_LAUNCH();
     const int P = intr->cols * intr->rows;
-    depth_consistency_kernel<<<divup(P, 256), 256, 0, stream()>>>(vmap_model, dev_pose, depth, *intr, max_diff,
-                                                                  dev_stats);
+    const int nblk = (int)divup((unsigned)P, 256u);
+    if ((size_t)nblk > g_cons_cap) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
+        KFX_REQUIRE(cs == hipStreamCaptureStatusNone,
+                    "kfx_depth_consistency: no workspace reserved for %dx%d frames inside a stream capture",
+                    intr->cols, intr->rows);
+        KFX_TRY(consistency_reserve(intr->cols, intr->rows));
+    }
+    depth_consistency_kernel<<<nblk, 256, 0, stream()>>>(vmap_model, dev_pose, depth, *intr, max_diff, g_cons);
+    KFX_CHECK_LAUNCH();
+    consistency_reduce_kernel<<<1, 256, 0, stream()>>>(g_cons, nblk, dev_stats);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }

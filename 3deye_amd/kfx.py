@@ -435,6 +435,12 @@ class Pipeline:
                                       (C.c_double * 3)(*[float(v) for v in volume_origin_world]), C.byref(self.h)))
         self.cfg = cfg
 
+    def set_denoiser(self, dn):
+        """Attach a Denoiser (or None); the config must have denoiser=1 for
+        it to replace the bilateral filter."""
+        self.dn = dn
+        check(lib.kfx_pipeline_set_denoiser(self.h, dn.h if dn is not None else None))
+
     def process(self, depth_dev_ptr):
         info = FrameInfo()
         check(lib.kfx_pipeline_process(self.h, depth_dev_ptr, C.byref(info)))
@@ -454,6 +460,50 @@ class Pipeline:
         try:
             if getattr(self, "h", None) is not None and self.h.value:
                 lib.kfx_pipeline_destroy(self.h)
+        except Exception:
+            pass
+
+
+DENOISER_PARAMS = 5089
+DENOISER_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "denoiser_weights.f32")
+
+
+class Denoiser:
+    """Owner of a kfx_denoiser handle (cols x rows frames)."""
+
+    def __init__(self, cols, rows, params=None):
+        self.h = C.c_void_p()
+        check(lib.kfx_denoiser_create(cols, rows, C.byref(self.h)))
+        if params is not None:
+            self.set_params(params)
+
+    @staticmethod
+    def load_params(path=DENOISER_WEIGHTS):
+        p = np.fromfile(path, dtype="<f4")
+        if p.size != DENOISER_PARAMS:
+            raise ValueError(f"{path}: {p.size} parameters, expected {DENOISER_PARAMS}")
+        return p.astype(np.float32)
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != DENOISER_PARAMS:
+            raise ValueError(f"{p.size} parameters, expected {DENOISER_PARAMS}")
+        check(lib.kfx_denoiser_set_params(self.h, p.ctypes.data_as(C.c_void_p)))
+
+    def get_params(self):
+        p = np.zeros(DENOISER_PARAMS, np.float32)
+        check(lib.kfx_denoiser_get_params(self.h, p.ctypes.data_as(C.c_void_p)))
+        return p
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            check(lib.kfx_denoiser_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and self.h.value:
+                lib.kfx_denoiser_destroy(self.h)
         except Exception:
             pass
 

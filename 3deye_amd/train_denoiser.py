@@ -2,11 +2,14 @@
 
 Training pairs are (raw depth frame, model depth) where the model depth is the
 pipeline's own raycast of its TSDF at the frame's estimated pose, produced by
-the bilateral-filter pipeline running on seeded random-walk sequences of the
-512^3 / 640x480 configuration (ground plane, 10 boxes, 10 spheres per scene,
-2 cm per frame).  Every pair contributes one forward + backward pass
-(kfx_denoiser_accumulate); Adam steps every `--batch` pairs.  The first
-`--skip` frames of a sequence are not used (the model is still thin there).
+the plain pipeline (the reference's control flow, bilateral filter) running
+on seeded random-walk sequences of the 512^3 / 640x480 workload family
+(workload.py: ground plane, 10 boxes, 10 spheres per scene, 2 cm per frame).
+Every pair contributes one forward + backward pass (kfx_denoiser_accumulate);
+Adam steps every `--batch` pairs.  The first `--skip` frames of a sequence
+are not used (the model is still thin there), nor are frames whose estimated
+position is more than 1 cm from the ground truth (a sequence stops once it
+is 3 cm off: its model no longer matches the frames).
 
 Held-out evaluation: on scenes with seeds not used for training, the RMSE of
 raw, bilateral-filtered and denoised depth against the renderer's exact depth,
@@ -28,21 +31,21 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 import kfx  # noqa: E402
+import workload as wl  # noqa: E402
 
-W, H = 640, 480
-N_VOX, VOL_M = 512, 3.072
-NP = 5089
+W, H = wl.W, wl.H
+NP = kfx.DENOISER_PARAMS
 TRAIN_SEED_BASE = 100
 HELDOUT_SEED_BASE = 5000
+KEEP_ERR_M = 0.01
+STOP_ERR_M = 0.03
 
 
 def sequence(seed, frames):
-    intr = kfx.kinect_intrinsics(W, H, 525.0)
-    scene = kfx.make_scene(seed, 10, 10, (0.5, 9.0, -2.0, 2.0), 0.0)
-    poses = kfx.walk(seed, (0.0, -1.2, 0.0), 0.02, 0.35, frames)
-    R, t = poses[0].matrices()
-    origin = t + 1.5 * R[:, 2] - VOL_M / 2  # cube centred on the first target point
-    return intr, scene, poses, origin
+    intr = wl.intrinsics()
+    scene = wl.scene(seed)
+    poses = wl.trajectory(seed, frames)
+    return intr, scene, poses, wl.cube_origin(poses[0])
 
 
 def render_seq(seed, frames, threads):
@@ -104,8 +107,7 @@ def main():
         p0 = load_params(args.init)
         kfx.call("kfx_denoiser_set_params", dn, p0.ctypes.data_as(C.c_void_p))
 
-    cfg = kfx.default_config(intr=kfx.kinect_intrinsics(W, H, 525.0), voxels=kfx.Point3i(N_VOX, N_VOX, N_VOX),
-                             volume_size=kfx.Point3f(VOL_M, VOL_M, VOL_M), mode=kfx.FUSED, use_graph=1)
+    cfg = wl.pipeline_config(mode=kfx.PLAIN)
     target = kfx.Buffer(W * H * 4)
     pool = cf.ThreadPoolExecutor(max_workers=1)
     seeds = [TRAIN_SEED_BASE + i for i in range(args.seeds)]
@@ -116,6 +118,7 @@ def main():
     batch_loss = 0.0
     t0 = time.time()
     total_pairs = args.seeds * (args.frames - args.skip)
+    skipped = stopped = 0
     for si, seed in enumerate(seeds):
         depths = fut.result()
         if si + 1 < len(seeds):
@@ -126,8 +129,13 @@ def main():
         fb = depths[0].nbytes
         for k in range(args.frames):
             dptr = C.c_void_p(dd.ptr.value + k * fb)
-            pipe.process(dptr)
-            if k < args.skip:
+            info = pipe.process(dptr)
+            err = float(np.linalg.norm(info.pose_world.matrices()[1] - poses[k].matrices()[1]))
+            if err > STOP_ERR_M:
+                stopped += 1
+                break
+            if k < args.skip or err > KEEP_ERR_M:
+                skipped += k >= args.skip
                 continue
             v = pipe.view()
             kfx.call("kfx_model_depth", C.c_void_p(v.vmap_model), C.byref(v.pose_volume), W, H, target.ptr)
@@ -157,6 +165,7 @@ def main():
     heldout = [HELDOUT_SEED_BASE + i for i in range(args.heldout)]
     rmse, n = heldout_eval(dn, heldout, 200, args.threads)
     summary = {"pairs": pairs, "seeds": args.seeds, "frames_per_seed": args.frames, "batch": args.batch,
+               "frames_skipped_pose_error": skipped, "sequences_stopped_tracking_lost": stopped,
                "lr": args.lr, "seconds": time.time() - t0, "loss_first": history[:5], "loss_last": history[-5:],
                "loss_history_every_10": history[::10], "heldout_seeds": heldout, "heldout_pixels": n,
                "heldout_rmse_mm": rmse}

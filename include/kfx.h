@@ -327,9 +327,10 @@ int kfx_denoiser_adam_step(kfx_denoiser* dn, float lr, int frames);
 int kfx_model_depth(const kfx_xyz* vmap_model, const kfx_pose* pose_volume, int cols, int rows, float* out_m);
 /* Fallback gate: raycast vertices (volume coordinates, previous frame)
  * projected into the camera at the device pose (12 floats, R camera ->
- * volume then t); where the depth frame has a value within max_diff_m of the
- * vertex depth, dev_stats[0] += squared difference, dev_stats[1] += 1
- * (dev_stats: 2 doubles on the device, cleared first). */
+ * volume then t).  dev_stats (3 doubles on the device) receives {sum of the
+ * squared depth differences over the pixels where the frame's depth is within
+ * max_diff_m of the vertex depth, the number of those pixels, the number of
+ * vertices landing on a pixel with depth}, summed in a fixed order. */
 int kfx_depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
                           const kfx_intrinsics* intr, float max_diff_m, double* dev_stats);
 

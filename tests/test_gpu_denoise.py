@@ -137,8 +137,13 @@ def test_model_depth_and_consistency(gpu):
     depth[0, :] = 0
     dd = kfx.Buffer.from_array(depth)
     dpose = kfx.Buffer.from_array(np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], np.float32))
-    st = kfx.Buffer(16)
+    st = kfx.Buffer(24)
     kfx.call("kfx_depth_consistency", dvm.ptr, dpose.ptr, dd.ptr, C.byref(intr), 0.1, st.ptr)
-    s = st.download(np.float64, 2)
-    assert s[1] == 64 * 48 - 1 - 64
+    s = st.download(np.float64, 3)
+    assert s[1] == 64 * 48 - 1 - 64 and s[2] == s[1]
     np.testing.assert_allclose(np.sqrt(s[0] / s[1]), 0.010, rtol=1e-3)
+    # 20 cm away: outside the 10 cm window, still on pixels with depth.
+    far = kfx.Buffer.from_array(np.full((48, 64), 2200, np.uint16))
+    kfx.call("kfx_depth_consistency", dvm.ptr, dpose.ptr, far.ptr, C.byref(intr), 0.1, st.ptr)
+    s = st.download(np.float64, 3)
+    assert s[0] == 0 and s[1] == 0 and s[2] == 64 * 48 - 1

@@ -92,3 +92,54 @@ def test_cyclic_shift_keeps_tracking(gpu, mode, use_graph):
     print(f"mode {mode}: {len(shifted)} shifts, ATE {ate * 1000:.2f} mm")
     assert ate < 0.02
     pipe.close()
+
+
+def _c1_learned(kfx, params, use_graph, frames=20, **kw):
+    intr = kfx.kinect_intrinsics()
+    scene, poses = scenes.c1_scene(), scenes.c1_poses(frames)
+    depths = np.stack([kfx.render(scene, poses[i], intr, scenes.C1_SEED, i, noise=True) for i in range(frames)])
+    cfg = kfx.default_config(intr=intr, voxels=kfx.Point3i(256, 256, 256), volume_size=kfx.Point3f(3.0, 3.0, 3.0),
+                             mode=1, use_graph=use_graph, denoiser=1, shift_threshold=100.0, **kw)
+    dn = kfx.Denoiser(intr.cols, intr.rows, params)
+    pipe = kfx.Pipeline(cfg, poses[0], (0.0, 0.0, 0.0))
+    pipe.set_denoiser(dn)
+    infos = _run(kfx, pipe, depths)
+    pipe.close()
+    dn.close()
+    return infos, poses
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_learned_pipeline_identity_denoiser_tracks(gpu, use_graph):
+    # All-zero parameters: the denoiser passes the raw depth through, which
+    # agrees with the model far inside the 15 mm gate, so no frame falls back.
+    kfx = gpu
+    infos, gt = _c1_learned(kfx, np.zeros(kfx.DENOISER_PARAMS, np.float32), use_graph)
+    assert all(i.fallback == 0 for i in infos[1:])
+    rms = [i.denoise_rms_mm for i in infos[1:]]
+    assert 0.5 < np.median(rms) < 10.0, rms
+    assert kfx.ate_rmse([i.pose_world for i in infos], gt) < 0.01
+
+
+def test_learned_pipeline_rms_gate(gpu):
+    # Output bias +5 cm with ICP switched off (the estimate stays at the
+    # previous pose, so the bias cannot be absorbed into the pose): the
+    # denoised frames disagree with the model by more than the 15 mm RMS gate.
+    kfx = gpu
+    params = np.zeros(kfx.DENOISER_PARAMS, np.float32)
+    params[-1] = 0.05
+    infos, _ = _c1_learned(kfx, params, 1, frames=8, icp_iterations=(0, 0, 0))
+    assert all(i.fallback == 1 and i.denoise_rms_mm > 15.0 for i in infos[1:]), [i.denoise_rms_mm for i in infos]
+
+
+def test_learned_pipeline_coverage_gate(gpu):
+    # Output bias +30 cm: no denoised pixel lies within 10 cm of the model, ICP
+    # finds no correspondences, and the coverage gate (fewer than half of the
+    # model pixels within 10 cm) falls back on every frame; the trajectory is
+    # the bilateral pipeline's.
+    kfx = gpu
+    params = np.zeros(kfx.DENOISER_PARAMS, np.float32)
+    params[-1] = 0.3
+    infos, gt = _c1_learned(kfx, params, 1)
+    assert all(i.fallback == 1 for i in infos[1:]), [i.denoise_rms_mm for i in infos]
+    assert kfx.ate_rmse([i.pose_world for i in infos], gt) < 0.01
