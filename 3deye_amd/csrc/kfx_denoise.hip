@@ -76,19 +76,39 @@ __global__ void dn_pack_kernel(const float* __restrict__ m, DnPacked* __restrict
 
 __device__ __forceinline__ float depth_m(uint16_t d) { return (float)d * 0.001f; }
 
+// Input normalisation: the depth of each tap relative to the output pixel's
+// depth, in units of the sensor noise at that depth (the Kinect model
+// sigma(z) = 1.2 + 1.9 (z - 0.4)^2 mm), clamped to +-kDnRelClip; 0 where the
+// tap or the centre has no depth.  The output is the centre depth plus
+// sigma times the network's residual, so the network is the same at every
+// distance.
+constexpr float kDnRelClip = 8.0f;
+__device__ __forceinline__ float noise_sigma_m(float z) {
+    const float dz = z - 0.4f;
+    return (1.2f + 1.9f * (dz * dz)) * 0.001f;
+}
+__device__ __forceinline__ float rel_depth(uint16_t dq, float zp, float inv_sigma) {
+    if (!dq) return 0.0f;
+    const float r = (depth_m(dq) - zp) * inv_sigma;
+    return fminf(fmaxf(r, -kDnRelClip), kDnRelClip);
+}
+
 // Layer 1 (2 -> 16), one lane per pixel.
 __global__ void __launch_bounds__(256) dn_l1_fwd(const uint16_t* __restrict__ depth, int W, int H,
                                                  const DnPacked* __restrict__ pk, uint16_t* __restrict__ a1) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= W || y >= H) return;
+    const uint16_t dc = depth[y * W + x];
+    const float zc = depth_m(dc);
+    const float inv_sigma = 1.0f / noise_sigma_m(zc);
     float in[2][9];
     for (int ky = 0; ky < 3; ++ky)
         for (int kx = 0; kx < 3; ++kx) {
             const int qx = x + kx - 1, qy = y + ky - 1;
             uint16_t d = 0;
             if (qx >= 0 && qy >= 0 && qx < W && qy < H) d = depth[qy * W + qx];
-            in[0][ky * 3 + kx] = d ? depth_m(d) : 0.0f;
+            in[0][ky * 3 + kx] = dc ? rel_depth(d, zc, inv_sigma) : 0.0f;
             in[1][ky * 3 + kx] = d ? 1.0f : 0.0f;
         }
     uint32_t packed[8];
@@ -187,7 +207,8 @@ __global__ void __launch_bounds__(256) dn_l4_fwd(const uint16_t* __restrict__ de
                 s += pk->w4[c * 9 + t] * bf2f(h);
             }
         }
-    const float y = d ? depth_m(d) + s : 0.0f;
+    const float zc = depth_m(d);
+    const float y = d ? zc + noise_sigma_m(zc) * s : 0.0f;
     if (ypred) ypred[p] = y;
     if (out) {
         int mm = 0;
@@ -200,9 +221,9 @@ __global__ void __launch_bounds__(256) dn_l4_fwd(const uint16_t* __restrict__ de
 // Training.
 // ---------------------------------------------------------------------------
 
-// Loss gradient of the output: dz4[p] = 2 (y - t) on pixels with a valid
-// input, a valid target and |x0 - t| < outlier; per-block sums of squared
-// error and counts (reduced by dn_loss_reduce).
+// Loss gradient of the output layer: dz4[p] = 2 (y - t) sigma(z) on pixels
+// with a valid input, a valid target and |z - t| < outlier (y = z + sigma(z)
+// s); per-block sums of squared error and counts (reduced by dn_loss_reduce).
 __global__ void __launch_bounds__(256) dn_loss_kernel(const uint16_t* __restrict__ depth,
                                                       const float* __restrict__ ypred,
                                                       const float* __restrict__ target, int P, float outlier,
@@ -217,7 +238,7 @@ __global__ void __launch_bounds__(256) dn_loss_kernel(const uint16_t* __restrict
         float g = 0.0f;
         if (d && t > 0.0f && fabsf(depth_m(d) - t) < outlier) {
             const float e = ypred[p] - t;
-            g = 2.0f * e;
+            g = (2.0f * e) * noise_sigma_m(depth_m(d));
             e2 = (double)e * e;
             c = 1;
         }
@@ -431,6 +452,9 @@ __global__ void __launch_bounds__(64) dn_l1_wgrad(const uint16_t* __restrict__ d
     for (int i = 0; i < 152; ++i) acc[i] = 0.0f;
     for (int x = seg * kSmallPixelsPerWave + lane; x < min(W, (seg + 1) * kSmallPixelsPerWave); x += 64) {
         const int p = y * W + x;
+        const uint16_t dc = depth[p];
+        const float zc = depth_m(dc);
+        const float inv_sigma = 1.0f / noise_sigma_m(zc);
         float g[8];
         bool any = false;
         for (int c = 0; c < 8; ++c) {
@@ -445,7 +469,7 @@ __global__ void __launch_bounds__(64) dn_l1_wgrad(const uint16_t* __restrict__ d
                 if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
                 const uint16_t d = depth[qy * W + qx];
                 if (!d) continue;
-                const float x0 = depth_m(d);
+                const float x0 = dc ? rel_depth(d, zc, inv_sigma) : 0.0f;
                 for (int c = 0; c < 8; ++c) {
                     acc[(c * 2 + 0) * 9 + ky * 3 + kx] += g[c] * x0;
                     acc[(c * 2 + 1) * 9 + ky * 3 + kx] += g[c];
@@ -728,7 +752,9 @@ int kfx_denoiser_adam_step(kfx_denoiser* d, float lr, int frames) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(d && frames > 0 && lr > 0.0f, "kfx_denoiser_adam_step: bad arguments");
     d->adam_t += 1;
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    // The loss is in m^2, so gradients are ~1e-6 and below: eps must sit far
+    // under sqrt(v) for Adam to stay scale-free.
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-12f;
     const float c1 = 1.0f - powf(b1, (float)d->adam_t), c2 = 1.0f - powf(b2, (float)d->adam_t);
     dn_adam_kernel<<<divup(kDnParams, 256), 256, 0, stream()>>>(d->master, d->grad, d->adam_m, d->adam_v, kDnParams,
                                                                 lr, b1, b2, eps, c1, c2, 1.0f / (float)frames);
@@ -765,21 +791,23 @@ __global__ void __launch_bounds__(256) model_depth_kernel(const kfx_xyz* __restr
 // squared difference is accumulated.  stats: [sum of squares (m^2), count].
 // Consistency of a depth frame with the raycast model, per 256-pixel
 // workgroup: the model vertices (volume coordinates) projected into the
-// camera at the device pose (12 floats: R camera -> volume, then t); where a
-// vertex lands on a pixel with depth, it counts towards n_model, and when the
-// depth is within max_diff of the vertex depth its squared difference and a
-// count are added too.  partials[g] = {sum e^2, count, n_model}.
+// camera at the device pose (12 floats: R camera -> volume, then t).  Every
+// valid vertex counts towards n_valid; where it lands on a pixel with depth,
+// towards n_depth, and when the depth is within max_diff of the vertex depth
+// its squared difference and a count are added too.
+// partials[g] = {sum e^2, count, n_depth, n_valid}.
 __global__ void __launch_bounds__(256) depth_consistency_kernel(const kfx_xyz* __restrict__ vmap,
                                                                 const float* __restrict__ dpose,
                                                                 const uint16_t* __restrict__ depth,
                                                                 kfx_intrinsics in, float max_diff,
                                                                 double* __restrict__ partials) {
-    __shared__ double red[4][3];
+    __shared__ double red[4][4];
     const int p = blockIdx.x * 256 + threadIdx.x;
-    double s = 0.0, c = 0.0, m = 0.0;
+    double s = 0.0, c = 0.0, m = 0.0, nv = 0.0;
     if (p < in.cols * in.rows) {
         const kfx_xyz v = vmap[p];
         if (!isnan(v.x)) {
+            nv = 1.0;
             const float dx = v.x - dpose[9], dy = v.y - dpose[10], dz = v.z - dpose[11];
             const float cx = dpose[0] * dx + dpose[3] * dy + dpose[6] * dz;
             const float cy = dpose[1] * dx + dpose[4] * dy + dpose[7] * dz;
@@ -804,13 +832,14 @@ __global__ void __launch_bounds__(256) depth_consistency_kernel(const kfx_xyz* _
         s += __shfl_xor(s, off);
         c += __shfl_xor(c, off);
         m += __shfl_xor(m, off);
+        nv += __shfl_xor(nv, off);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) { red[wave][0] = s; red[wave][1] = c; red[wave][2] = m; }
+    if (lane == 0) { red[wave][0] = s; red[wave][1] = c; red[wave][2] = m; red[wave][3] = nv; }
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if (threadIdx.x < 4) {
         const int k = threadIdx.x;
-        partials[blockIdx.x * 3 + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+        partials[blockIdx.x * 4 + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
     }
 }
 
@@ -818,18 +847,18 @@ __global__ void __launch_bounds__(256) depth_consistency_kernel(const kfx_xyz* _
 // workgroup: thread t sums g = t, t + 256, ..., then a tree over the 256).
 __global__ void __launch_bounds__(256) consistency_reduce_kernel(const double* __restrict__ partials, int nblk,
                                                                  double* __restrict__ stats) {
-    __shared__ double red[3][256];
-    double a[3] = {0.0, 0.0, 0.0};
+    __shared__ double red[4][256];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (int g = threadIdx.x; g < nblk; g += 256)
-        for (int k = 0; k < 3; ++k) a[k] += partials[g * 3 + k];
-    for (int k = 0; k < 3; ++k) red[k][threadIdx.x] = a[k];
+        for (int k = 0; k < 4; ++k) a[k] += partials[g * 4 + k];
+    for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = a[k];
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if (threadIdx.x < h)
-            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x < 3) stats[threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 4) stats[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // Library-owned workgroup partials of the consistency gate.
@@ -842,7 +871,7 @@ int consistency_reserve(int cols, int rows) {
     if (g_cons) KFX_CHECK_HIP(hipFree(g_cons));
     g_cons = nullptr;
     g_cons_cap = 0;
-    KFX_CHECK_HIP(hipMalloc((void**)&g_cons, need * 3 * sizeof(double)));
+    KFX_CHECK_HIP(hipMalloc((void**)&g_cons, need * 4 * sizeof(double)));
     g_cons_cap = need;
     return KFX_OK;
 }

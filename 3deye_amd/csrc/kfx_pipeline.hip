@@ -450,13 +450,14 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     p->ext_cap = 1u << 20;
     if (e == hipSuccess) e = hipMalloc((void**)&p->ext_pts, sizeof(kfx_xyzi) * p->ext_cap);
     if (e == hipSuccess) e = hipMalloc((void**)&p->ext_col, sizeof(kfx_color) * p->ext_cap);
-    if (e == hipSuccess) e = hipMalloc((void**)&p->dstats, sizeof(double) * 3);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&p->hstats, sizeof(double) * 3, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->dstats, sizeof(double) * 4);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&p->hstats, sizeof(double) * 4, hipHostMallocDefault);
     if (e != hipSuccess) {
         kfx_pipeline_destroy(p);
         return set_error(KFX_ERR_HIP, "kfx_pipeline_create: %s", hipGetErrorString(e));
     }
     int rc = kfx_icp_create((int)npix, &p->icp);
+    if (!rc) rc = kfx_icp_set_sum_mode(p->icp, p->cfg.icp_sum_mode);
     if (!rc) rc = kfx_slicer_create(&rb, &p->slicer);
     if (!rc) rc = integrate_reserve(p->cfg.intr.cols, p->cfg.intr.rows);
     if (!rc) rc = consistency_reserve(p->cfg.intr.cols, p->cfg.intr.rows);
@@ -506,15 +507,16 @@ int kfx_pipeline_process(kfx_pipeline* p, const uint16_t* depth, kfx_frame_info*
         // filter (redoing ICP from the previous pose) when the denoised frame
         // disagrees with the model by more than fallback_rms_mm.
         KFX_TRY(run_part(p, learned_part, &p->graph, &p->graph_exec));
-        KFX_CHECK_HIP(hipMemcpyAsync(p->hstats, p->dstats, sizeof(double) * 3, hipMemcpyDeviceToHost, stream()));
+        KFX_CHECK_HIP(hipMemcpyAsync(p->hstats, p->dstats, sizeof(double) * 4, hipMemcpyDeviceToHost, stream()));
         KFX_CHECK_HIP(hipStreamSynchronize(stream()));
-        const double cnt = p->hstats[1], n_model = p->hstats[2];
+        const double cnt = p->hstats[1], n_valid = p->hstats[3];
         const double rms_mm = cnt > 0 ? 1000.0 * sqrt(p->hstats[0] / cnt) : 0.0;
         // Fall back when the frame disagrees with the model: RMS above the
         // threshold over the pixels within 10 cm, or fewer than half of the
-        // model pixels within 10 cm at all.
-        const bool fallback = p->cfg.fallback_rms_mm > 0 && n_model >= 100 &&
-                              (rms_mm > p->cfg.fallback_rms_mm || cnt < 0.5 * n_model);
+        // model's vertices within 10 cm of the frame at all (a large error, or
+        // an ICP pose that lost the model).
+        const bool fallback = p->cfg.fallback_rms_mm > 0 && n_valid >= 100 &&
+                              (rms_mm > p->cfg.fallback_rms_mm || cnt < 0.5 * n_valid);
         if (info) {
             info->denoise_rms_mm = (float)rms_mm;
             info->fallback = fallback ? 1 : 0;

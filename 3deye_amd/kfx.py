@@ -74,6 +74,13 @@ class Pose(C.Structure):
         return np.array(self.R[:], dtype=np.float64).reshape(3, 3), np.array(self.t[:], dtype=np.float64)
 
 
+def pose_from(R, t):
+    p = Pose()
+    p.R[:] = [float(v) for v in np.asarray(R, np.float64).reshape(9)]
+    p.t[:] = [float(v) for v in np.asarray(t, np.float64).reshape(3)]
+    return p
+
+
 class IcpResult(C.Structure):
     _fields_ = [("pose", Pose), ("A", C.c_double * 36), ("b", C.c_double * 6), ("x", C.c_double * 6),
                 ("status", C.c_int32), ("singular_count", C.c_int32), ("iterations", C.c_int32),
@@ -91,7 +98,7 @@ class PipelineConfig(C.Structure):
                 ("use_graph", C.c_int32), ("denoiser", C.c_int32), ("fallback_rms_mm", C.c_float),
                 ("shift_threshold", C.c_float), ("integrate_threshold", C.c_float),
                 ("converge_rotation", C.c_double), ("converge_translation", C.c_double),
-                ("extract_on_shift", C.c_int32), ("reserved", C.c_int32), ("shift_target_distance", C.c_double)]
+                ("extract_on_shift", C.c_int32), ("icp_sum_mode", C.c_int32), ("shift_target_distance", C.c_double)]
 
 
 class FrameInfo(C.Structure):

@@ -8,8 +8,10 @@ on seeded random-walk sequences of the 512^3 / 640x480 workload family
 Every pair contributes one forward + backward pass (kfx_denoiser_accumulate);
 Adam steps every `--batch` pairs.  The first `--skip` frames of a sequence
 are not used (the model is still thin there), nor are frames whose estimated
-position is more than 1 cm from the ground truth (a sequence stops once it
-is 3 cm off: its model no longer matches the frames).
+position is more than 1 cm from the ground truth.  When a run drifts 3 cm
+from the ground truth, its model no longer matches the frames: the pipeline
+is restarted at that frame's ground-truth pose (a relocalisation) and the
+next `--skip` frames are again not used.
 
 Held-out evaluation: on scenes with seeds not used for training, the RMSE of
 raw, bilateral-filtered and denoised depth against the renderer's exact depth,
@@ -54,6 +56,19 @@ def render_seq(seed, frames, threads):
     return depths
 
 
+def init_params(rng):
+    """He-normal weights for the ReLU layers (fan-in 2 * 9 and 16 * 9), a
+    small output layer (the net starts close to the identity: output = raw
+    depth + residual), zero biases.  Layout: W1 [16][2][9], b1, W2 [16][16][9],
+    b2, W3, b3, W4 [16][9], b4."""
+    p = np.zeros(NP, np.float32)
+    p[0:288] = rng.normal(0.0, np.sqrt(2.0 / 18.0), 288)
+    p[304:2608] = rng.normal(0.0, np.sqrt(2.0 / 144.0), 2304)
+    p[2624:4928] = rng.normal(0.0, np.sqrt(2.0 / 144.0), 2304)
+    p[4944:5088] = rng.normal(0.0, 1e-3, 144)
+    return p
+
+
 def load_params(path):
     p = np.fromfile(path, dtype="<f4")
     if p.size != NP:
@@ -95,6 +110,7 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--heldout", type=int, default=10)
     ap.add_argument("--init", default=None, help="start from these parameters")
+    ap.add_argument("--init-seed", type=int, default=1, help="seed of the random initialisation")
     ap.add_argument("--out", default=os.path.join(HERE, "denoiser_weights.f32"))
     ap.add_argument("--log", default=None)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -103,9 +119,8 @@ def main():
     kfx.init(0)
     dn = C.c_void_p()
     kfx.call("kfx_denoiser_create", W, H, C.byref(dn))
-    if args.init:
-        p0 = load_params(args.init)
-        kfx.call("kfx_denoiser_set_params", dn, p0.ctypes.data_as(C.c_void_p))
+    p0 = load_params(args.init) if args.init else init_params(np.random.default_rng(args.init_seed))
+    kfx.call("kfx_denoiser_set_params", dn, p0.ctypes.data_as(C.c_void_p))
 
     cfg = wl.pipeline_config(mode=kfx.PLAIN)
     target = kfx.Buffer(W * H * 4)
@@ -127,15 +142,20 @@ def main():
         dd = kfx.Buffer.from_array(depths)
         pipe = kfx.Pipeline(cfg, poses[0], origin)
         fb = depths[0].nbytes
+        start = 0
         for k in range(args.frames):
             dptr = C.c_void_p(dd.ptr.value + k * fb)
             info = pipe.process(dptr)
             err = float(np.linalg.norm(info.pose_world.matrices()[1] - poses[k].matrices()[1]))
             if err > STOP_ERR_M:
                 stopped += 1
-                break
-            if k < args.skip or err > KEEP_ERR_M:
-                skipped += k >= args.skip
+                pipe.close()
+                pipe = kfx.Pipeline(cfg, poses[k], wl.cube_origin(poses[k]))
+                pipe.process(dptr)
+                start = k
+                continue
+            if k - start < args.skip or err > KEEP_ERR_M:
+                skipped += (k - start) >= args.skip
                 continue
             v = pipe.view()
             kfx.call("kfx_model_depth", C.c_void_p(v.vmap_model), C.byref(v.pose_volume), W, H, target.ptr)
@@ -165,7 +185,7 @@ def main():
     heldout = [HELDOUT_SEED_BASE + i for i in range(args.heldout)]
     rmse, n = heldout_eval(dn, heldout, 200, args.threads)
     summary = {"pairs": pairs, "seeds": args.seeds, "frames_per_seed": args.frames, "batch": args.batch,
-               "frames_skipped_pose_error": skipped, "sequences_stopped_tracking_lost": stopped,
+               "frames_skipped_pose_error": skipped, "relocalisations": stopped,
                "lr": args.lr, "seconds": time.time() - t0, "loss_first": history[:5], "loss_last": history[-5:],
                "loss_history_every_10": history[::10], "heldout_seeds": heldout, "heldout_pixels": n,
                "heldout_rmse_mm": rmse}

@@ -327,10 +327,11 @@ int kfx_denoiser_adam_step(kfx_denoiser* dn, float lr, int frames);
 int kfx_model_depth(const kfx_xyz* vmap_model, const kfx_pose* pose_volume, int cols, int rows, float* out_m);
 /* Fallback gate: raycast vertices (volume coordinates, previous frame)
  * projected into the camera at the device pose (12 floats, R camera ->
- * volume then t).  dev_stats (3 doubles on the device) receives {sum of the
+ * volume then t).  dev_stats (4 doubles on the device) receives {sum of the
  * squared depth differences over the pixels where the frame's depth is within
  * max_diff_m of the vertex depth, the number of those pixels, the number of
- * vertices landing on a pixel with depth}, summed in a fixed order. */
+ * vertices landing on a pixel with depth, the number of valid vertices},
+ * summed in a fixed order. */
 int kfx_depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
                           const kfx_intrinsics* intr, float max_diff_m, double* dev_stats);
 
@@ -361,7 +362,7 @@ typedef struct kfx_pipeline_config {
     double converge_rotation;     /* fused early exit per level (rad); 0 disables */
     double converge_translation;  /* fused early exit per level (m) */
     int32_t extract_on_shift;     /* extract the leaving slab before clearing it */
-    int32_t reserved;
+    int32_t icp_sum_mode;         /* fused mode: KFX_ICP_SUM_* workgroup sums (0: FP32 Kahan) */
     double shift_target_distance; /* target point = camera + this along the optical axis (m); 0: camera centre */
 } kfx_pipeline_config;
 

@@ -1,13 +1,17 @@
 // denoise.cpp — oracle for the learned depth denoiser: forward pass, loss and
 // backward pass in double precision.  Test infrastructure only (see oracle.h).
 //
-// Network (no reference counterpart): input channels x0 = depth in metres
-// (0 where invalid), x1 = valid mask; four 3x3 convolutions with zero padding,
-// 2 -> 16 -> 16 -> 16 -> 1 channels, ReLU after the first three; output
-// y = x0 + conv4 on valid pixels (residual), 0 elsewhere.  Parameters (5,089,
-// flat fp32): per layer the weights [co][ci][ky][kx] then the biases [co].
-// Loss: mean over pixels with a valid input, a valid target and
-// |x0 - target| < outlier of (y - target)^2.
+// Network (no reference counterpart): for output pixel p with depth z_p
+// (metres) and sensor noise sigma_p = (1.2 + 1.9 (z_p - 0.4)^2) mm, the
+// input channels at tap q are x0 = clamp((z_q - z_p) / sigma_p, -8, 8) (0
+// where q or p has no depth) and x1 = valid mask of q; four 3x3
+// convolutions with zero padding, 2 -> 16 -> 16 -> 16 -> 1 channels, ReLU
+// after the first three; output y = z_p + sigma_p * conv4 on valid pixels
+// (residual in noise units), 0 elsewhere.  The normalisation is evaluated in
+// single precision exactly as the GPU does.  Parameters (5,089, flat fp32):
+// per layer the weights [co][ci][ky][kx] then the biases [co].  Loss: mean
+// over pixels with a valid input, a valid target and |z_p - target| <
+// outlier of (y - target)^2.
 //
 // With bf16 != 0 the oracle rounds what the GPU stores in bfloat16 (weights
 // and the three hidden activations, round to nearest even) before using it,
@@ -45,20 +49,34 @@ double bf16_round(double v) {
     return f;
 }
 
+float noise_sigma_m(float z) {
+    const float dz = z - 0.4f;
+    return (1.2f + 1.9f * (dz * dz)) * 0.001f;
+}
+
 struct Net {
     int W, H;
     bool bf16;
     std::vector<double> w;                  // parameters (rounded if bf16)
-    std::vector<double> x;                  // [p][2]
+    std::vector<double> x;                  // [p][2]: depth (m), valid
+    std::vector<float> zf, inv_sigma, sigma; // single-precision depth, 1 / sigma, sigma per pixel
     std::vector<double> a[3];               // hidden activations [p][16]
     std::vector<double> y;                  // output
     std::vector<uint8_t> valid;
 };
 
-double in_val(const Net& n, int l, const std::vector<double>* src, int px, int py, int ci) {
+// Input of layer l at tap (px, py) for the output pixel c (layer 0 inputs
+// are relative to c).
+double in_val(const Net& n, int l, const std::vector<double>* src, int px, int py, int ci, int c) {
     if (px < 0 || py < 0 || px >= n.W || py >= n.H) return 0.0;
     const int p = py * n.W + px;
-    if (l == 0) return n.x[2 * p + ci];
+    if (l == 0) {
+        if (ci == 1) return n.x[2 * p + 1];
+        if (!n.valid[p] || !n.valid[c]) return 0.0;
+        float r = (n.zf[p] - n.zf[c]) * n.inv_sigma[c];
+        r = fminf(fmaxf(r, -8.0f), 8.0f);
+        return (double)r;
+    }
     return (*src)[(size_t)p * kC + ci];
 }
 
@@ -76,7 +94,7 @@ void forward(Net& n) {
                         for (int ky = 0; ky < 3; ++ky)
                             for (int kx = 0; kx < 3; ++kx)
                                 s += n.w[off + ((co * cin + ci) * 3 + ky) * 3 + kx] *
-                                     in_val(n, l, src, px + kx - 1, py + ky - 1, ci);
+                                     in_val(n, l, src, px + kx - 1, py + ky - 1, ci, py * n.W + px);
                     out[(size_t)(py * n.W + px) * cout + co] = s;
                 }
         if (l < 3) {
@@ -87,7 +105,7 @@ void forward(Net& n) {
             n.a[l] = out;
         } else {
             n.y.assign(P, 0.0);
-            for (int p = 0; p < P; ++p) n.y[p] = n.valid[p] ? n.x[2 * p] + out[p] : 0.0;
+            for (int p = 0; p < P; ++p) n.y[p] = n.valid[p] ? n.x[2 * p] + (double)n.sigma[p] * out[p] : 0.0;
         }
     }
 }
@@ -107,12 +125,19 @@ void setup(Net& n, const uint16_t* depth, int W, int H, const float* params, int
     const int P = W * H;
     n.x.assign(2 * (size_t)P, 0.0);
     n.valid.assign(P, 0);
-    for (int p = 0; p < P; ++p)
+    n.zf.assign(P, 0.0f);
+    n.sigma.assign(P, 0.0f);
+    n.inv_sigma.assign(P, 0.0f);
+    for (int p = 0; p < P; ++p) {
+        n.zf[p] = (float)depth[p] * 0.001f;
+        n.sigma[p] = noise_sigma_m(n.zf[p]);
+        n.inv_sigma[p] = 1.0f / n.sigma[p];
         if (depth[p]) {
             n.valid[p] = 1;
-            n.x[2 * p] = (double)((float)depth[p] * 0.001f);
+            n.x[2 * p] = (double)n.zf[p];
             n.x[2 * p + 1] = 1.0;
         }
+    }
 }
 
 }  // namespace
@@ -152,7 +177,8 @@ extern "C" double orc_dn_loss_grad(const uint16_t* depth, const float* target_m,
         if (!n.valid[p] || !(t > 0.0) || !(fabs(n.x[2 * p] - t) < outlier_m)) continue;
         const double e = n.y[p] - t;
         loss += e * e;
-        dy[p] = n.bf16 ? (double)(2.0f * (float)e) : 2.0 * e;
+        // dL/d(conv4) = 2 e sigma_p (y = z_p + sigma_p conv4)
+        dy[p] = n.bf16 ? (double)((2.0f * (float)e) * n.sigma[p]) : 2.0 * e * (double)n.sigma[p];
     }
     if (count) *count = cnt;
     if (cnt == 0) {
@@ -161,7 +187,7 @@ extern "C" double orc_dn_loss_grad(const uint16_t* depth, const float* target_m,
     }
     loss /= cnt;
     memset(grad, 0, sizeof(double) * layer_offset(4));
-    // dz of the output layer: dy on valid pixels (y = x0 + z).
+    // dz of the output layer: dy on valid pixels.
     std::vector<double> dz(dy);
     for (int p = 0; p < P; ++p)
         if (!n.valid[p]) dz[p] = 0.0;
@@ -179,7 +205,7 @@ extern "C" double orc_dn_loss_grad(const uint16_t* depth, const float* target_m,
                         for (int ky = 0; ky < 3; ++ky)
                             for (int kx = 0; kx < 3; ++kx)
                                 grad[off + ((co * cin + ci) * 3 + ky) * 3 + kx] +=
-                                    g * in_val(n, l, src, px + kx - 1, py + ky - 1, ci);
+                                    g * in_val(n, l, src, px + kx - 1, py + ky - 1, ci, py * W + px);
                 }
         if (l == 0) break;
         // input gradient, through the ReLU of the previous layer

@@ -137,13 +137,46 @@ def test_model_depth_and_consistency(gpu):
     depth[0, :] = 0
     dd = kfx.Buffer.from_array(depth)
     dpose = kfx.Buffer.from_array(np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], np.float32))
-    st = kfx.Buffer(24)
+    st = kfx.Buffer(32)
     kfx.call("kfx_depth_consistency", dvm.ptr, dpose.ptr, dd.ptr, C.byref(intr), 0.1, st.ptr)
-    s = st.download(np.float64, 3)
-    assert s[1] == 64 * 48 - 1 - 64 and s[2] == s[1]
+    s = st.download(np.float64, 4)
+    assert s[1] == 64 * 48 - 1 - 64 and s[2] == s[1] and s[3] == 64 * 48 - 1
     np.testing.assert_allclose(np.sqrt(s[0] / s[1]), 0.010, rtol=1e-3)
     # 20 cm away: outside the 10 cm window, still on pixels with depth.
     far = kfx.Buffer.from_array(np.full((48, 64), 2200, np.uint16))
     kfx.call("kfx_depth_consistency", dvm.ptr, dpose.ptr, far.ptr, C.byref(intr), 0.1, st.ptr)
-    s = st.download(np.float64, 3)
-    assert s[0] == 0 and s[1] == 0 and s[2] == 64 * 48 - 1
+    s = st.download(np.float64, 4)
+    assert s[0] == 0 and s[1] == 0 and s[2] == 64 * 48 - 1 and s[3] == 64 * 48 - 1
+
+
+def test_trained_denoiser_gains_over_bilateral(gpu):
+    # The committed network on frames of held-out scenes (seeds never used in
+    # training): depth RMSE against the renderer's exact depth, on pixels
+    # valid in the raw frame within 5 cm of the truth, for the raw frame, the
+    # bilateral filter (sigma_s 4.5 px, sigma_r 30 mm) and the network.
+    kfx = gpu
+    import workload as wl
+    W, H = wl.W, wl.H
+    dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params())
+    src, bil, den = kfx.Buffer(W * H * 2), kfx.Buffer(W * H * 2), kfx.Buffer(W * H * 2)
+    sums = {"raw": 0.0, "bilateral": 0.0, "denoised": 0.0}
+    count = 0
+    for seed in (5000, 5003, 5007):
+        poses = wl.trajectory(seed, 60)
+        sc = wl.scene(seed)
+        for i in (5, 30, 55):
+            raw, clean = kfx.render(sc, poses[i], wl.intrinsics(), seed, i, noise=True, want_clean=True)
+            src.upload(raw)
+            kfx.call("kfx_bilateral_filter", src.ptr, bil.ptr, W, H, 4.5, 30.0, 0xFFFFFFFF)
+            kfx.call("kfx_denoiser_forward", dn.h, src.ptr, den.ptr, None)
+            r = raw.astype(np.float64)
+            ok = (raw > 0) & (clean > 0) & (np.abs(r - clean * 1000) < 50)
+            gt = clean[ok] * 1000.0
+            sums["raw"] += np.sum((r[ok] - gt) ** 2)
+            sums["bilateral"] += np.sum((bil.download(np.uint16, (H, W)).astype(np.float64)[ok] - gt) ** 2)
+            sums["denoised"] += np.sum((den.download(np.uint16, (H, W)).astype(np.float64)[ok] - gt) ** 2)
+            count += int(ok.sum())
+    rmse = {k: float(np.sqrt(v / count)) for k, v in sums.items()}
+    print("held-out depth RMSE (mm):", rmse)
+    dn.close()
+    assert rmse["denoised"] < 0.6 * rmse["bilateral"] < 0.6 * rmse["raw"], rmse

@@ -122,24 +122,26 @@ def test_learned_pipeline_identity_denoiser_tracks(gpu, use_graph):
 
 
 def test_learned_pipeline_rms_gate(gpu):
-    # Output bias +5 cm with ICP switched off (the estimate stays at the
-    # previous pose, so the bias cannot be absorbed into the pose): the
-    # denoised frames disagree with the model by more than the 15 mm RMS gate.
+    # Output bias of 15 noise sigmas (about 5 cm at the C1 depths of 1.2-2 m)
+    # with ICP switched off (the estimate stays at the previous pose, so the
+    # bias cannot be absorbed into the pose): the denoised frames disagree
+    # with the model by more than the 15 mm RMS gate.
     kfx = gpu
     params = np.zeros(kfx.DENOISER_PARAMS, np.float32)
-    params[-1] = 0.05
+    params[-1] = 15.0
     infos, _ = _c1_learned(kfx, params, 1, frames=8, icp_iterations=(0, 0, 0))
     assert all(i.fallback == 1 and i.denoise_rms_mm > 15.0 for i in infos[1:]), [i.denoise_rms_mm for i in infos]
 
 
 def test_learned_pipeline_coverage_gate(gpu):
-    # Output bias +30 cm: no denoised pixel lies within 10 cm of the model, ICP
-    # finds no correspondences, and the coverage gate (fewer than half of the
-    # model pixels within 10 cm) falls back on every frame; the trajectory is
-    # the bilateral pipeline's.
+    # Output bias of 120 noise sigmas (over 30 cm at the C1 depths): no
+    # denoised pixel lies within 10 cm of the model, ICP finds no
+    # correspondences, and the coverage gate (fewer than half of the model
+    # pixels within 10 cm) falls back on every frame; the trajectory is the
+    # bilateral pipeline's.
     kfx = gpu
     params = np.zeros(kfx.DENOISER_PARAMS, np.float32)
-    params[-1] = 0.3
+    params[-1] = 120.0
     infos, gt = _c1_learned(kfx, params, 1)
     assert all(i.fallback == 1 for i in infos[1:]), [i.denoise_rms_mm for i in infos]
     assert kfx.ate_rmse([i.pose_world for i in infos], gt) < 0.01
