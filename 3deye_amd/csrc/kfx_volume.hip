@@ -46,17 +46,39 @@ struct IntegrateParams {
     int bound_off[kBoundLevels + 1];
 };
 
+// The pose-dependent terms of IntegrateParams, kept apart so that a kernel
+// taking its pose from device memory fills a small register-resident struct
+// instead of rewriting its (large) parameter block, which the compiler would
+// then place in scratch memory.
+struct PoseTerms {
+    float r[9];
+    float tx, ty, tz;
+    float plane[5][3];
+};
+
+__device__ __forceinline__ PoseTerms host_terms(const IntegrateParams& p) {
+    PoseTerms q;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) q.r[i] = p.r[i];
+    q.tx = p.tx; q.ty = p.ty; q.tz = p.tz;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) q.plane[k][j] = p.plane[k][j];
+    return q;
+}
+
 // Range [lo, hi] of z in [z0, z1) whose voxel centres of column (gx, gy) lie
 // inside every frustum plane, widened by one voxel; lo > hi when empty.
-__device__ __forceinline__ void clip_column(const IntegrateParams& p, float gx, float gy, int z0, int z1,
-                                            int& lo, int& hi) {
+__device__ __forceinline__ void clip_column(const IntegrateParams& p, const PoseTerms& q, float gx, float gy,
+                                            int z0, int z1, int& lo, int& hi) {
     float zl = (float)z0, zh = (float)(z1 - 1);
     bool empty = false;
-    const float base_z = 0.5f * p.csz - p.tz;
+    const float base_z = 0.5f * p.csz - q.tz;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        const float a = p.plane[k][0] * gx + p.plane[k][1] * gy + p.plane[k][2] * base_z;
-        const float b = p.plane[k][2] * p.csz;
+        const float a = q.plane[k][0] * gx + q.plane[k][1] * gy + q.plane[k][2] * base_z;
+        const float b = q.plane[k][2] * p.csz;
         if (b > 0.0f) zl = fmaxf(zl, -a * __builtin_amdgcn_rcpf(b));
         else if (b < 0.0f) zh = fminf(zh, -a * __builtin_amdgcn_rcpf(b));
         else if (a < 0.0f) empty = true;
@@ -95,13 +117,13 @@ __global__ void __launch_bounds__(256) scale_depth_kernel(const uint16_t* __rest
 
 // One voxel of the update rule.  Returns true when the voxel's stored value
 // changes and has to be written back.
-__device__ __forceinline__ bool integrate_voxel(const IntegrateParams& p, float gx, float gy,
+__device__ __forceinline__ bool integrate_voxel(const IntegrateParams& p, const PoseTerms& q, float gx, float gy,
                                                 float part, int z, const float* __restrict__ scaled,
                                                 float& out_dp, bool& in_band) {
-    const float gz = ((float)z + 0.5f) * p.csz - p.tz;
-    const float camx = p.r[0] * gx + p.r[1] * gy + p.r[2] * gz;
-    const float camy = p.r[3] * gx + p.r[4] * gy + p.r[5] * gz;
-    const float camz = p.r[6] * gx + p.r[7] * gy + p.r[8] * gz;
+    const float gz = ((float)z + 0.5f) * p.csz - q.tz;
+    const float camx = q.r[0] * gx + q.r[1] * gy + q.r[2] * gz;
+    const float camy = q.r[3] * gx + q.r[4] * gy + q.r[5] * gz;
+    const float camz = q.r[6] * gx + q.r[7] * gy + q.r[8] * gz;
     in_band = false;
     if (!(camz > 0.0f)) return false;
     const float inv_z = 1.0f / camz;
@@ -117,35 +139,42 @@ __device__ __forceinline__ bool integrate_voxel(const IntegrateParams& p, float 
     return true;
 }
 
-// Fills the pose-dependent fields of p from a device-resident pose (R
+// Fills the pose-dependent terms q from a device-resident pose (R
 // camera -> volume row-major, then t: 12 floats); false when the camera moved
 // less than motion_thr since the previous pose (R^T, t), the KinectFusion
 // integration metric (|rotation angle| + |translation|) / 2.
-__device__ __forceinline__ bool load_device_pose(IntegrateParams& p, const float* __restrict__ dpose,
+__device__ __forceinline__ bool load_device_pose(const IntegrateParams& p, PoseTerms& q, const float* __restrict__ dpose,
                                                  const float* __restrict__ dprev, float motion_thr) {
     float R[9], t[3];
+#pragma unroll
     for (int i = 0; i < 9; ++i) R[i] = dpose[i];
+#pragma unroll
     for (int i = 0; i < 3; ++i) t[i] = dpose[9 + i];
     if (dprev != nullptr && motion_thr > 0.0f) {
         float tr = 0.0f;
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j) tr += R[3 * i + j] * dprev[3 * j + i];
         const float ang = acosf(fminf(1.0f, fmaxf(-1.0f, 0.5f * (tr - 1.0f))));
         const float dx = t[0] - dprev[9], dy = t[1] - dprev[10], dz = t[2] - dprev[11];
         if (0.5f * (ang + sqrtf(dx * dx + dy * dy + dz * dz)) < motion_thr) return false;
     }
+#pragma unroll
     for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) p.r[3 * r + c] = R[3 * c + r];
-    p.tx = t[0]; p.ty = t[1]; p.tz = t[2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q.r[3 * r + c] = R[3 * c + r];
+    q.tx = t[0]; q.ty = t[1]; q.tz = t[2];
     const float m = kClipMarginPx;
     const float cl = p.cx + 0.5f + m, cr = (float)p.cols - 0.5f + m - p.cx;
     const float ct = p.cy + 0.5f + m, cb = (float)p.rows - 0.5f + m - p.cy;
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
-        p.plane[0][k] = p.r[6 + k];
-        p.plane[1][k] = p.fx * p.r[k] + cl * p.r[6 + k];
-        p.plane[2][k] = -p.fx * p.r[k] + cr * p.r[6 + k];
-        p.plane[3][k] = p.fy * p.r[3 + k] + ct * p.r[6 + k];
-        p.plane[4][k] = -p.fy * p.r[3 + k] + cb * p.r[6 + k];
+        q.plane[0][k] = q.r[6 + k];
+        q.plane[1][k] = p.fx * q.r[k] + cl * q.r[6 + k];
+        q.plane[2][k] = -p.fx * q.r[k] + cr * q.r[6 + k];
+        q.plane[3][k] = p.fy * q.r[3 + k] + ct * q.r[6 + k];
+        q.plane[4][k] = -p.fy * q.r[3 + k] + cb * q.r[6 + k];
     }
     return true;
 }
@@ -186,12 +215,13 @@ depth_bound_kernel(const float* __restrict__ scaled, int cols, int rows, float* 
 // the camera than the largest depth under its footprint plus the truncation
 // distance (the rule needs dist <= depth + trunc).  Conservative: false
 // whenever the test cannot decide.
-__device__ __forceinline__ bool segment_culled(const IntegrateParams& p, const float* __restrict__ scaled,
+__device__ __forceinline__ bool segment_culled(const IntegrateParams& p, const PoseTerms& q,
+                                               const float* __restrict__ scaled,
                                                float gx, float gy, float cxy0, float cxy1, float cxy2, int za,
                                                int zb) {
-    const float gza = ((float)za + 0.5f) * p.csz - p.tz, gzb = ((float)zb + 0.5f) * p.csz - p.tz;
-    const float ax = cxy0 + p.r[2] * gza, ay = cxy1 + p.r[5] * gza, az = cxy2 + p.r[8] * gza;
-    const float bx = cxy0 + p.r[2] * gzb, by = cxy1 + p.r[5] * gzb, bz = cxy2 + p.r[8] * gzb;
+    const float gza = ((float)za + 0.5f) * p.csz - q.tz, gzb = ((float)zb + 0.5f) * p.csz - q.tz;
+    const float ax = cxy0 + q.r[2] * gza, ay = cxy1 + q.r[5] * gza, az = cxy2 + q.r[8] * gza;
+    const float bx = cxy0 + q.r[2] * gzb, by = cxy1 + q.r[5] * gzb, bz = cxy2 + q.r[8] * gzb;
     if (az <= 0.0f && bz <= 0.0f) return true;
     if (!(az > 1e-4f && bz > 1e-4f)) return false;
     const float ia = __builtin_amdgcn_rcpf(az), ib = __builtin_amdgcn_rcpf(bz);
@@ -228,24 +258,29 @@ __device__ __forceinline__ bool segment_culled(const IntegrateParams& p, const f
 
 template <bool kDevicePose>
 __global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
-integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume, IntegrateParams p,
+integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume, const IntegrateParams p,
                  const float* __restrict__ dpose, const float* __restrict__ dprev, float motion_thr) {
-    if (kDevicePose && !load_device_pose(p, dpose, dprev, motion_thr)) return;
+    PoseTerms q;
+    if (kDevicePose) {
+        if (!load_device_pose(p, q, dpose, dprev, motion_thr)) return;
+    } else {
+        q = host_terms(p);
+    }
     const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
     const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
     const bool inside = x < p.nx && y < p.ny;
     const int z_begin = blockIdx.z * kIntegrateZRun;
     const int z_end = min(z_begin + kIntegrateZRun, p.nz);
 
-    const float gx = ((float)x + 0.5f) * p.csx - p.tx;
-    const float gy = ((float)y + 0.5f) * p.csy - p.ty;
+    const float gx = ((float)x + 0.5f) * p.csx - q.tx;
+    const float gy = ((float)y + 0.5f) * p.csy - q.ty;
     const float part = gx * gx + gy * gy;
     const uint32_t col = inside ? (uint32_t)wrap_once(x + p.ox, p.nx) + (uint32_t)p.nx * (uint32_t)wrap_once(y + p.oy, p.ny) : 0u;
 
     // Clip the run to the frustum; the wave walks the union of its lanes'
     // ranges so that every z step stays one contiguous 256-byte access.
     int lo, hi;
-    clip_column(p, gx, gy, z_begin, z_end, lo, hi);
+    clip_column(p, q, gx, gy, z_begin, z_end, lo, hi);
     if (!inside) { lo = z_end; hi = z_begin - 1; }
     const int own_lo = lo, own_hi = hi;
     lo = wave_min(lo);
@@ -255,9 +290,9 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
     // Column terms of the camera coordinates: the rule's dot products are
     // evaluated as (r0 * gx + r1 * gy) + r2 * gz, so the first two terms are
     // shared by the whole column with identical rounding.
-    const float cxy0 = p.r[0] * gx + p.r[1] * gy;
-    const float cxy1 = p.r[3] * gx + p.r[4] * gy;
-    const float cxy2 = p.r[6] * gx + p.r[7] * gy;
+    const float cxy0 = q.r[0] * gx + q.r[1] * gy;
+    const float cxy1 = q.r[3] * gx + q.r[4] * gy;
+    const float cxy2 = q.r[6] * gx + q.r[7] * gy;
     const float fcols = (float)p.cols, frows = (float)p.rows;
     const uint32_t nz = (uint32_t)p.nz;
     for (int z0 = lo; z0 <= hi; z0 += kIntegrateBatch) {
@@ -265,7 +300,7 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
         // part of the batch inside its own clipped range is culled).
         if (p.bound != nullptr) {
             const int za = max(z0, own_lo), zb = min(z0 + kIntegrateBatch - 1, own_hi);
-            const bool culled = za > zb || segment_culled(p, scaled, gx, gy, cxy0, cxy1, cxy2, za, zb);
+            const bool culled = za > zb || segment_culled(p, q, scaled, gx, gy, cxy0, cxy1, cxy2, za, zb);
             if (__all(culled)) continue;
         }
         float dp[kIntegrateBatch], dist[kIntegrateBatch];
@@ -275,10 +310,10 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
 #pragma unroll
         for (int k = 0; k < kIntegrateBatch; ++k) {
             const int z = z0 + k;
-            const float gz = ((float)z + 0.5f) * p.csz - p.tz;
-            const float camx = cxy0 + p.r[2] * gz;
-            const float camy = cxy1 + p.r[5] * gz;
-            const float camz = cxy2 + p.r[8] * gz;
+            const float gz = ((float)z + 0.5f) * p.csz - q.tz;
+            const float camx = cxy0 + q.r[2] * gz;
+            const float camy = cxy1 + q.r[5] * gz;
+            const float camz = cxy2 + q.r[8] * gz;
             const float inv_z = 1.0f / camz;
             const float fu = rintf(camx * p.fx * inv_z + p.cx);
             const float fv = rintf(camy * p.fy * inv_z + p.cy);
@@ -329,18 +364,19 @@ integrate_count_kernel(const float* __restrict__ scaled, IntegrateParams p, unsi
     const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
     const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
     unsigned n = 0;
+    const PoseTerms q = host_terms(p);
     if (x < p.nx && y < p.ny) {
         const int z_begin = blockIdx.z * kIntegrateZRun;
         const int z_end = min(z_begin + kIntegrateZRun, p.nz);
-        const float gx = ((float)x + 0.5f) * p.csx - p.tx;
-        const float gy = ((float)y + 0.5f) * p.csy - p.ty;
+        const float gx = ((float)x + 0.5f) * p.csx - q.tx;
+        const float gy = ((float)y + 0.5f) * p.csy - q.ty;
         const float part = gx * gx + gy * gy;
         int lo, hi;
-        clip_column(p, gx, gy, z_begin, z_end, lo, hi);
+        clip_column(p, q, gx, gy, z_begin, z_end, lo, hi);
         for (int z = lo; z <= hi; ++z) {
             float tn;
             bool band;
-            n += integrate_voxel(p, gx, gy, part, z, scaled, tn, band) ? 1u : 0u;
+            n += integrate_voxel(p, q, gx, gy, part, z, scaled, tn, band) ? 1u : 0u;
         }
     }
     for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
