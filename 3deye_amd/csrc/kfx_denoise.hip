@@ -76,21 +76,35 @@ __global__ void dn_pack_kernel(const float* __restrict__ m, DnPacked* __restrict
 
 __device__ __forceinline__ float depth_m(uint16_t d) { return (float)d * 0.001f; }
 
-// Input normalisation: the depth of each tap relative to the output pixel's
-// depth, in units of the sensor noise at that depth (the Kinect model
-// sigma(z) = 1.2 + 1.9 (z - 0.4)^2 mm), clamped to +-kDnRelClip; 0 where the
-// tap or the centre has no depth.  The output is the centre depth plus
-// sigma times the network's residual, so the network is the same at every
-// distance.
+// Input normalisation, in inverse depth w = 1/z (planes are affine in pixel
+// coordinates there, so a smoothing filter of w is unbiased on them, unlike
+// one of z): tap q enters as (w_q - w_p) / sigma_w(p), clamped to
+// +-kDnRelClip, where p is the output pixel and sigma_w = sigma_z / z^2 is
+// the inverse-depth noise of the Kinect model sigma_z = 1.2 + 1.9 (z - 0.4)^2
+// mm; 0 where the tap or the centre has no depth.  The output is
+// 1 / (w_p + sigma_w(p) s) for the network's residual s (the inverse depth
+// kept above w_p / 4), so the network is the same at every distance.
 constexpr float kDnRelClip = 8.0f;
 __device__ __forceinline__ float noise_sigma_m(float z) {
     const float dz = z - 0.4f;
     return (1.2f + 1.9f * (dz * dz)) * 0.001f;
 }
-__device__ __forceinline__ float rel_depth(uint16_t dq, float zp, float inv_sigma) {
+__device__ __forceinline__ float inv_depth(uint16_t d) { return 1000.0f / (float)d; }
+// sigma_w(p) = sigma_z(z_p) w_p^2.
+__device__ __forceinline__ float noise_sigma_inv(uint16_t d) {
+    const float w = inv_depth(d);
+    return noise_sigma_m(depth_m(d)) * (w * w);
+}
+__device__ __forceinline__ float rel_depth(uint16_t dq, float wp, float inv_sigma_w) {
     if (!dq) return 0.0f;
-    const float r = (depth_m(dq) - zp) * inv_sigma;
+    const float r = (inv_depth(dq) - wp) * inv_sigma_w;
     return fminf(fmaxf(r, -kDnRelClip), kDnRelClip);
+}
+// Output depth of a pixel with raw depth d > 0 and residual s.
+__device__ __forceinline__ float dn_output(uint16_t d, float s) {
+    const float wp = inv_depth(d);
+    const float w = fmaxf(wp + noise_sigma_inv(d) * s, 0.25f * wp);
+    return 1.0f / w;
 }
 
 // Layer 1 (2 -> 16), one lane per pixel.
@@ -100,15 +114,15 @@ __global__ void __launch_bounds__(256) dn_l1_fwd(const uint16_t* __restrict__ de
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= W || y >= H) return;
     const uint16_t dc = depth[y * W + x];
-    const float zc = depth_m(dc);
-    const float inv_sigma = 1.0f / noise_sigma_m(zc);
+    const float wc = dc ? inv_depth(dc) : 0.0f;
+    const float inv_sigma = dc ? 1.0f / noise_sigma_inv(dc) : 0.0f;
     float in[2][9];
     for (int ky = 0; ky < 3; ++ky)
         for (int kx = 0; kx < 3; ++kx) {
             const int qx = x + kx - 1, qy = y + ky - 1;
             uint16_t d = 0;
             if (qx >= 0 && qy >= 0 && qx < W && qy < H) d = depth[qy * W + qx];
-            in[0][ky * 3 + kx] = dc ? rel_depth(d, zc, inv_sigma) : 0.0f;
+            in[0][ky * 3 + kx] = dc ? rel_depth(d, wc, inv_sigma) : 0.0f;
             in[1][ky * 3 + kx] = d ? 1.0f : 0.0f;
         }
     uint32_t packed[8];
@@ -207,8 +221,7 @@ __global__ void __launch_bounds__(256) dn_l4_fwd(const uint16_t* __restrict__ de
                 s += pk->w4[c * 9 + t] * bf2f(h);
             }
         }
-    const float zc = depth_m(d);
-    const float y = d ? zc + noise_sigma_m(zc) * s : 0.0f;
+    const float y = d ? dn_output(d, s) : 0.0f;
     if (ypred) ypred[p] = y;
     if (out) {
         int mm = 0;
@@ -221,9 +234,11 @@ __global__ void __launch_bounds__(256) dn_l4_fwd(const uint16_t* __restrict__ de
 // Training.
 // ---------------------------------------------------------------------------
 
-// Loss gradient of the output layer: dz4[p] = 2 (y - t) sigma(z) on pixels
-// with a valid input, a valid target and |z - t| < outlier (y = z + sigma(z)
-// s); per-block sums of squared error and counts (reduced by dn_loss_reduce).
+// Loss gradient of the output layer: with y = 1 / (w_p + sigma_w s),
+// dz4[p] = dL/ds = -2 (y - t) y^2 sigma_w on pixels with a valid input, a
+// valid target and |z - t| < outlier (the clamp of the inverse depth is
+// passed straight through); per-block sums of squared error and counts
+// (reduced by dn_loss_reduce).
 __global__ void __launch_bounds__(256) dn_loss_kernel(const uint16_t* __restrict__ depth,
                                                       const float* __restrict__ ypred,
                                                       const float* __restrict__ target, int P, float outlier,
@@ -237,8 +252,9 @@ __global__ void __launch_bounds__(256) dn_loss_kernel(const uint16_t* __restrict
         const float t = target[p];
         float g = 0.0f;
         if (d && t > 0.0f && fabsf(depth_m(d) - t) < outlier) {
-            const float e = ypred[p] - t;
-            g = (2.0f * e) * noise_sigma_m(depth_m(d));
+            const float yv = ypred[p];
+            const float e = yv - t;
+            g = -(2.0f * e) * (yv * yv) * noise_sigma_inv(d);
             e2 = (double)e * e;
             c = 1;
         }
@@ -453,8 +469,8 @@ __global__ void __launch_bounds__(64) dn_l1_wgrad(const uint16_t* __restrict__ d
     for (int x = seg * kSmallPixelsPerWave + lane; x < min(W, (seg + 1) * kSmallPixelsPerWave); x += 64) {
         const int p = y * W + x;
         const uint16_t dc = depth[p];
-        const float zc = depth_m(dc);
-        const float inv_sigma = 1.0f / noise_sigma_m(zc);
+        const float wc = dc ? inv_depth(dc) : 0.0f;
+        const float inv_sigma = dc ? 1.0f / noise_sigma_inv(dc) : 0.0f;
         float g[8];
         bool any = false;
         for (int c = 0; c < 8; ++c) {
@@ -469,7 +485,7 @@ __global__ void __launch_bounds__(64) dn_l1_wgrad(const uint16_t* __restrict__ d
                 if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
                 const uint16_t d = depth[qy * W + qx];
                 if (!d) continue;
-                const float x0 = dc ? rel_depth(d, zc, inv_sigma) : 0.0f;
+                const float x0 = dc ? rel_depth(d, wc, inv_sigma) : 0.0f;
                 for (int c = 0; c < 8; ++c) {
                     acc[(c * 2 + 0) * 9 + ky * 3 + kx] += g[c] * x0;
                     acc[(c * 2 + 1) * 9 + ky * 3 + kx] += g[c];

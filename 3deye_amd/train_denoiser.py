@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--heldout", type=int, default=10)
     ap.add_argument("--init", default=None, help="start from these parameters")
     ap.add_argument("--init-seed", type=int, default=1, help="seed of the random initialisation")
+    ap.add_argument("--output-scale", type=float, default=1.0,
+                    help="scale of the trained residual (output layer weights and bias) in the saved network")
     ap.add_argument("--out", default=os.path.join(HERE, "denoiser_weights.f32"))
     ap.add_argument("--log", default=None)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -180,11 +182,17 @@ def main():
         kfx.call("kfx_denoiser_adam_step", dn, 1e-5, in_batch)
     params = np.zeros(NP, np.float32)
     kfx.call("kfx_denoiser_get_params", dn, params.ctypes.data_as(C.c_void_p))
+    # The residual is linear in the output layer: scaling its weights and bias
+    # scales the correction (in inverse-depth noise units) applied to the raw
+    # frame.
+    params[4944:5089] *= np.float32(args.output_scale)
+    kfx.call("kfx_denoiser_set_params", dn, params.ctypes.data_as(C.c_void_p))
     params.astype("<f4").tofile(args.out)
 
     heldout = [HELDOUT_SEED_BASE + i for i in range(args.heldout)]
     rmse, n = heldout_eval(dn, heldout, 200, args.threads)
     summary = {"pairs": pairs, "seeds": args.seeds, "frames_per_seed": args.frames, "batch": args.batch,
+               "output_scale": args.output_scale,
                "frames_skipped_pose_error": skipped, "relocalisations": stopped,
                "lr": args.lr, "seconds": time.time() - t0, "loss_first": history[:5], "loss_last": history[-5:],
                "loss_history_every_10": history[::10], "heldout_seeds": heldout, "heldout_pixels": n,

@@ -75,6 +75,19 @@ def run_pipeline(cfg, poses, origin, ddepth, frame_bytes, warmup, steps, profile
     return elapsed, infos, pipe
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "integrate_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per integrate launch from the committed rocprofv3 --pmc runs
+    (FETCH_SIZE and WRITE_SIZE of the --kernel-probe workload, each scaled by
+    the factor that maps a 256 MiB copy kernel's counter reading to 256 MiB)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        return json.load(f)
+
+
 def count_updates(intr, ddepth, frame_bytes, infos, first, rb):
     """N_upd of each integrated frame, from the pose it was integrated at."""
     scaled = kfx.Buffer(W * H * 4)
@@ -132,11 +145,13 @@ def cpu_baseline(intr, depths, poses, origin, frames=12):
                       f"19 ICP iterations, integrate, raycast), first {frames} frames of the 512^3 workload"}
 
 
-def kernel_probe(frames=20):
+def kernel_probe(frames=60, first=20):
     """Short, fixed workload for hardware-counter runs (rocprofv3 --pmc):
     three 256 MiB streaming copies (known traffic, for calibrating the
     FETCH_SIZE / WRITE_SIZE counters), then `frames` integrations of the
-    512^3 workload at the ground-truth poses, then their raycasts."""
+    512^3 workload at the ground-truth poses, each followed by its raycast.
+    Counters are read for the integrations from `first` on (a populated
+    volume, as in the timed benchmark)."""
     intr, scene, poses, origin, depths = workload_inputs(frames)
     kfx.init(0)
     nbytes = 256 << 20
@@ -152,26 +167,25 @@ def kernel_probe(frames=20):
     dd = kfx.Buffer.from_array(depths)
     scaled = kfx.Buffer(W * H * 4)
     vm, nm = kfx.Buffer(W * H * 12), kfx.Buffer(W * H * 12)
-    n_upd = 0
+    n_upd = []
     for i in range(frames):
         R, t = poses[i].matrices()
         tv = t - origin
         dp = C.c_void_p(dd.ptr.value + i * depths[0].nbytes)
+        if i >= first:
+            n = C.c_uint64()
+            kfx.call("kfx_integrate_count_updates", dp, scaled.ptr, C.byref(intr), C.byref(kfx.matrix3f(R.T)),
+                     C.byref(kfx.point3f(tv)), TRUNC, C.byref(cs), C.byref(rb), C.byref(n))
+            n_upd.append(n.value)
         kfx.call("kfx_integrate_tsdf", dp, scaled.ptr, vol.ptr, C.byref(intr), C.byref(kfx.matrix3f(R.T)),
                  C.byref(kfx.point3f(tv)), TRUNC, C.byref(cs), C.byref(rb))
         kfx.call("kfx_raycast", vol.ptr, vm.ptr, nm.ptr, C.byref(rb), C.byref(kfx.point3f(tv)),
                  C.byref(kfx.matrix3f(R)), TRUNC, C.byref(intr), C.byref(cs))
     kfx.synchronize()
-    for i in range(frames):
-        R, t = poses[i].matrices()
-        n = C.c_uint64()
-        dp = C.c_void_p(dd.ptr.value + i * depths[0].nbytes)
-        kfx.call("kfx_integrate_count_updates", dp, scaled.ptr, C.byref(intr), C.byref(kfx.matrix3f(R.T)),
-                 C.byref(kfx.point3f(t - origin)), TRUNC, C.byref(cs), C.byref(rb), C.byref(n))
-        n_upd += n.value
-    print(json.dumps({"probe": "copy 3 x 256 MiB; integrate + raycast", "frames": frames,
-                      "copy_bytes_per_launch": nbytes, "n_upd_per_frame": n_upd / frames,
-                      "integrate_alg_bytes_per_frame": (8.0 * n_upd + 2.0 * W * H * frames) / frames}))
+    n_avg = float(np.mean(n_upd))
+    print(json.dumps({"probe": "copy 3 x 256 MiB; integrate + raycast", "frames": frames, "counted_from": first,
+                      "copy_bytes_per_launch": nbytes, "n_upd_per_frame": n_avg,
+                      "integrate_alg_bytes_per_frame": 8.0 * n_avg + 2.0 * W * H}))
 
 
 def main():
@@ -251,8 +265,12 @@ def main():
         alg_bytes = (8.0 * n_upd + 2.0 * W * H * launches) / launches
         avg_s = int_ms / 1e3 / launches
         achieved = alg_bytes / avg_s / 1e9
+        traffic = measured_traffic()
         roofline = {"bound": "hbm", "kernel": "integrate", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_us": avg_s * 1e6,
+                    "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
+                    "traffic_source": traffic["source"] if traffic else None,
+                    "avg_launch_us": avg_s * 1e6,
                     "alg_bytes_per_launch": alg_bytes, "n_upd_per_launch": n_upd / launches,
                     "full_sweep_equiv_GBs": 8.0 * N_VOX ** 3 / avg_s / 1e9}
 

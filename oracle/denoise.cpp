@@ -2,12 +2,14 @@
 // backward pass in double precision.  Test infrastructure only (see oracle.h).
 //
 // Network (no reference counterpart): for output pixel p with depth z_p
-// (metres) and sensor noise sigma_p = (1.2 + 1.9 (z_p - 0.4)^2) mm, the
-// input channels at tap q are x0 = clamp((z_q - z_p) / sigma_p, -8, 8) (0
+// (metres), inverse depth w_p = 1000 / d_p and inverse-depth noise
+// sigma_w = sigma_z(z_p) w_p^2 (sigma_z = (1.2 + 1.9 (z - 0.4)^2) mm), the
+// input channels at tap q are x0 = clamp((w_q - w_p) / sigma_w, -8, 8) (0
 // where q or p has no depth) and x1 = valid mask of q; four 3x3
 // convolutions with zero padding, 2 -> 16 -> 16 -> 16 -> 1 channels, ReLU
-// after the first three; output y = z_p + sigma_p * conv4 on valid pixels
-// (residual in noise units), 0 elsewhere.  The normalisation is evaluated in
+// after the first three; output y = 1 / max(w_p + sigma_w conv4, w_p / 4) on
+// valid pixels (residual in inverse-depth noise units), 0 elsewhere; the
+// clamp's derivative is taken as 1 (as on the GPU).  The normalisation is evaluated in
 // single precision exactly as the GPU does.  Parameters (5,089, flat fp32):
 // per layer the weights [co][ci][ky][kx] then the biases [co].  Loss: mean
 // over pixels with a valid input, a valid target and |z_p - target| <
@@ -59,7 +61,7 @@ struct Net {
     bool bf16;
     std::vector<double> w;                  // parameters (rounded if bf16)
     std::vector<double> x;                  // [p][2]: depth (m), valid
-    std::vector<float> zf, inv_sigma, sigma; // single-precision depth, 1 / sigma, sigma per pixel
+    std::vector<float> wf, inv_sigma, sigma; // single-precision inverse depth, 1 / sigma_w, sigma_w per pixel
     std::vector<double> a[3];               // hidden activations [p][16]
     std::vector<double> y;                  // output
     std::vector<uint8_t> valid;
@@ -73,7 +75,7 @@ double in_val(const Net& n, int l, const std::vector<double>* src, int px, int p
     if (l == 0) {
         if (ci == 1) return n.x[2 * p + 1];
         if (!n.valid[p] || !n.valid[c]) return 0.0;
-        float r = (n.zf[p] - n.zf[c]) * n.inv_sigma[c];
+        float r = (n.wf[p] - n.wf[c]) * n.inv_sigma[c];
         r = fminf(fmaxf(r, -8.0f), 8.0f);
         return (double)r;
     }
@@ -105,7 +107,11 @@ void forward(Net& n) {
             n.a[l] = out;
         } else {
             n.y.assign(P, 0.0);
-            for (int p = 0; p < P; ++p) n.y[p] = n.valid[p] ? n.x[2 * p] + (double)n.sigma[p] * out[p] : 0.0;
+            for (int p = 0; p < P; ++p) {
+                if (!n.valid[p]) { n.y[p] = 0.0; continue; }
+                const double w = fmax((double)n.wf[p] + (double)n.sigma[p] * out[p], 0.25 * (double)n.wf[p]);
+                n.y[p] = 1.0 / w;
+            }
         }
     }
 }
@@ -125,18 +131,18 @@ void setup(Net& n, const uint16_t* depth, int W, int H, const float* params, int
     const int P = W * H;
     n.x.assign(2 * (size_t)P, 0.0);
     n.valid.assign(P, 0);
-    n.zf.assign(P, 0.0f);
+    n.wf.assign(P, 0.0f);
     n.sigma.assign(P, 0.0f);
     n.inv_sigma.assign(P, 0.0f);
     for (int p = 0; p < P; ++p) {
-        n.zf[p] = (float)depth[p] * 0.001f;
-        n.sigma[p] = noise_sigma_m(n.zf[p]);
+        if (!depth[p]) continue;
+        const float w = 1000.0f / (float)depth[p];
+        n.wf[p] = w;
+        n.sigma[p] = noise_sigma_m((float)depth[p] * 0.001f) * (w * w);
         n.inv_sigma[p] = 1.0f / n.sigma[p];
-        if (depth[p]) {
-            n.valid[p] = 1;
-            n.x[2 * p] = (double)n.zf[p];
-            n.x[2 * p + 1] = 1.0;
-        }
+        n.valid[p] = 1;
+        n.x[2 * p] = (double)((float)depth[p] * 0.001f);
+        n.x[2 * p + 1] = 1.0;
     }
 }
 
@@ -177,8 +183,13 @@ extern "C" double orc_dn_loss_grad(const uint16_t* depth, const float* target_m,
         if (!n.valid[p] || !(t > 0.0) || !(fabs(n.x[2 * p] - t) < outlier_m)) continue;
         const double e = n.y[p] - t;
         loss += e * e;
-        // dL/d(conv4) = 2 e sigma_p (y = z_p + sigma_p conv4)
-        dy[p] = n.bf16 ? (double)((2.0f * (float)e) * n.sigma[p]) : 2.0 * e * (double)n.sigma[p];
+        // dL/d(conv4) = -2 e y^2 sigma_w (y = 1 / (w_p + sigma_w conv4))
+        if (n.bf16) {
+            const float yf = (float)n.y[p], ef = (float)e;
+            dy[p] = (double)(-(2.0f * ef) * (yf * yf) * n.sigma[p]);
+        } else {
+            dy[p] = -2.0 * e * n.y[p] * n.y[p] * (double)n.sigma[p];
+        }
     }
     if (count) *count = cnt;
     if (cnt == 0) {

@@ -21,7 +21,11 @@ def test_zero_params_is_identity(orc):
     rng = np.random.default_rng(0)
     d, _ = _frame(rng)
     y = orc.dn_forward(d, np.zeros(orc.lib.orc_dn_param_count(), np.float32), bf16=False)
-    np.testing.assert_array_equal(y, np.where(d > 0, d.astype(np.float32) * np.float32(1e-3), 0.0))
+    # y = 1 / (1000 / d): the depth up to the rounding of the inverse, and
+    # exactly the input in millimetres.
+    assert np.array_equal(y > 0, d > 0)
+    np.testing.assert_allclose(y[d > 0], d[d > 0] * 1e-3, rtol=2e-7)
+    assert np.array_equal(np.rint(y * 1000.0).astype(np.int64), d.astype(np.int64))
 
 
 def test_param_count(orc):
