@@ -365,7 +365,7 @@ enum IcpSumMode : int {
 // Sum of 32 per-thread values over the workgroup in a fixed order: the
 // butterfly over each wave, then the four wave totals in wave order.  Thread
 // e < 27 returns entry e (the others return 0).
-template <typename T>
+template <int kWaves = kIcpWaves, typename T>
 __device__ __forceinline__ T workgroup_sum27(T (&acc)[32], T (*wsum)[32]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const T wtot = wave_reduce_scatter32(acc, lane);
@@ -375,7 +375,7 @@ __device__ __forceinline__ T workgroup_sum27(T (&acc)[32], T (*wsum)[32]) {
     if (threadIdx.x < kIcpEntries) {
         s = wsum[0][threadIdx.x];
 #pragma unroll
-        for (int w = 1; w < kIcpWaves; ++w) s += wsum[w][threadIdx.x];
+        for (int w = 1; w < kWaves; ++w) s += wsum[w][threadIdx.x];
     }
     return s;
 }
@@ -491,18 +491,20 @@ icp_sums_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
 // the 27 loads of one workgroup's totals together; then the fixed-order
 // workgroup sum in double precision, the Cholesky solve and the pose update.
 // The result does not depend on the order in which the first half's
-// workgroups ran.
+// workgroups ran.  (A 1024-thread variant that reads all partials in one
+// round measured slightly slower at 640x480.)
+constexpr int kIcpSolveThreads = 256;
 template <int M>
-__global__ void __launch_bounds__(kIcpThreads)
+__global__ void __launch_bounds__(kIcpSolveThreads)
 icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials, int nblk, int tag,
                  double conv_rot, double conv_trans) {
-    __shared__ double wsum[kIcpWaves][32];
+    __shared__ double wsum[kIcpSolveThreads / 64][32];
     __shared__ double total[32];
     if (tag >= 0 && st->converged_tag == tag) return;
     double acc[32];
 #pragma unroll
     for (int e = 0; e < 32; ++e) acc[e] = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += kIcpThreads) {
+    for (int b = threadIdx.x; b < nblk; b += kIcpSolveThreads) {
         if constexpr (M == kSumDouble) {
             const double* pd = static_cast<const double*>(partials) + b * kIcpEntries;
             double t[kIcpEntries];
@@ -519,7 +521,7 @@ icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials, i
             for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t[e].x + (double)t[e].y;
         }
     }
-    const double s = workgroup_sum27(acc, wsum);
+    const double s = workgroup_sum27<kIcpSolveThreads / 64>(acc, wsum);
     if (threadIdx.x < kIcpEntries) total[threadIdx.x] = s;
     __syncthreads();
     if (threadIdx.x == 0) solve_and_update(st, total, tag, conv_rot, conv_trans);
@@ -582,6 +584,7 @@ struct kfx_icp {
     kfx::IcpState* state;
     void* partials;    // [workgroup][27] sums of the fused iteration (float2 or double)
     int sum_mode;      // KFX_ICP_SUM_*
+    kfx::IcpState* host_state;  // pinned host copy for kfx_icp_read
     int max_blocks;
 };
 
@@ -594,7 +597,8 @@ static void icp_iterate_launch(kfx_icp* h, const kfx_xyz* vc, const kfx_xyz* nc,
         icp_sums_kernel<2, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
     else
         icp_sums_kernel<1, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
-    icp_solve_kernel<M><<<1, kIcpThreads, 0, stream()>>>(h->state, h->partials, nblk, tag, conv_rot, conv_trans);
+    icp_solve_kernel<M><<<1, kIcpSolveThreads, 0, stream()>>>(h->state, h->partials, nblk, tag, conv_rot,
+                                                                conv_trans);
 }
 
 extern "C" {
@@ -656,12 +660,15 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
     h->max_blocks = (int)divup((unsigned)max_pixels, (unsigned)kIcpThreads);
     hipError_t e = hipMalloc((void**)&h->state, sizeof(IcpState));
     h->sum_mode = KFX_ICP_SUM_FP32_KAHAN;
+    h->host_state = nullptr;
     if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(double) * kIcpEntries * (size_t)h->max_blocks);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&h->host_state, sizeof(IcpState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(h->state, 0, sizeof(IcpState), stream());
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
     if (e != hipSuccess) {
         (void)hipFree(h->state);
         (void)hipFree(h->partials);
+        if (h->host_state) (void)hipHostFree(h->host_state);
         delete h;
         return set_error(KFX_ERR_HIP, "kfx_icp_create: %s", hipGetErrorString(e));
     }
@@ -675,6 +682,7 @@ int kfx_icp_destroy(kfx_icp* h) {
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
     (void)hipFree(h->state);
     (void)hipFree(h->partials);
+    (void)hipHostFree(h->host_state);
     delete h;
     return KFX_OK;
 }
@@ -748,9 +756,9 @@ int kfx_icp_translate(kfx_icp* h, double dx, double dy, double dz) {
 int kfx_icp_read(kfx_icp* h, kfx_icp_result* out) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(h && out, "kfx_icp_read: null pointer");
-    IcpState s;
-    KFX_CHECK_HIP(hipMemcpyAsync(&s, h->state, sizeof(s), hipMemcpyDeviceToHost, stream()));
+    KFX_CHECK_HIP(hipMemcpyAsync(h->host_state, h->state, sizeof(IcpState), hipMemcpyDeviceToHost, stream()));
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
+    const IcpState& s = *h->host_state;
     for (int i = 0; i < 9; ++i) out->pose.R[i] = s.R[i];
     for (int i = 0; i < 3; ++i) out->pose.t[i] = s.t[i];
     for (int i = 0; i < 36; ++i) out->A[i] = s.A[i];
