@@ -266,7 +266,8 @@ __device__ __forceinline__ float2 wave_reduce_scatter32(float (&s)[32], float (&
 }
 
 // 6x6 Cholesky solve in double precision, then the pose update.  Every loop
-// has constant bounds, so the arrays stay in registers.
+// has constant bounds, so the arrays stay in registers; the six pivots are
+// inverted once and every other division is a multiplication.
 __device__ void solve_and_update(IcpState* st, const double* sums, int tag, double conv_rot, double conv_trans) {
     double A[36], b[6];
     {
@@ -285,7 +286,7 @@ __device__ void solve_and_update(IcpState* st, const double* sums, int tag, doub
     for (int i = 0; i < 6; ++i) st->b[i] = b[i];
     // Cholesky A = L L^T; a non-positive or non-finite pivot makes the
     // system singular (the remaining entries are then not used).
-    double L[36];
+    double L[36], inv_diag[6];
     bool ok = true;
 #pragma unroll
     for (int i = 0; i < 6; ++i)
@@ -297,8 +298,9 @@ __device__ void solve_and_update(IcpState* st, const double* sums, int tag, doub
             if (i == j) {
                 ok = ok && (s > 1e-300) && isfinite(s);
                 L[i * 6 + i] = sqrt(ok ? s : 1.0);
+                inv_diag[i] = 1.0 / L[i * 6 + i];
             } else {
-                L[i * 6 + j] = s / L[j * 6 + j];
+                L[i * 6 + j] = s * inv_diag[j];
             }
         }
     st->iterations += 1;
@@ -315,21 +317,24 @@ __device__ void solve_and_update(IcpState* st, const double* sums, int tag, doub
         double s = b[i];
 #pragma unroll
         for (int p = 0; p < i; ++p) s -= L[i * 6 + p] * yv[p];
-        yv[i] = s / L[i * 6 + i];
+        yv[i] = s * inv_diag[i];
     }
 #pragma unroll
     for (int i = 5; i >= 0; --i) {
         double s = yv[i];
 #pragma unroll
         for (int p = i + 1; p < 6; ++p) s -= L[p * 6 + i] * x[p];
-        x[i] = s / L[i * 6 + i];
+        x[i] = s * inv_diag[i];
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i) st->x[i] = x[i];
     st->status = KFX_OK;
     // Increment: R_inc = Rz(x2) Ry(x1) Rx(x0), t_inc = x[3..5];
     // t <- R_inc t + t_inc, R <- R_inc R.
-    const double ca = cos(x[0]), sa = sin(x[0]), cb = cos(x[1]), sb = sin(x[1]), cg = cos(x[2]), sg = sin(x[2]);
+    double sa, ca, sb, cb, sg, cg;
+    sincos(x[0], &sa, &ca);
+    sincos(x[1], &sb, &cb);
+    sincos(x[2], &sg, &cg);
     const double Ri[9] = {cg * cb, cg * sb * sa - sg * ca, cg * sb * ca + sg * sa,
                           sg * cb, sg * sb * sa + cg * ca, sg * sb * ca - cg * sa,
                           -sb, cb * sa, cb * ca};

@@ -182,34 +182,41 @@ normal_map_kernel(const kfx_xyz* __restrict__ vmap, kfx_xyz* __restrict__ nmap, 
     if (u >= cols || v >= rows) return;
     kfx_xyz out{NAN, NAN, NAN};
     const int lu = threadIdx.x + kNmR, lv = threadIdx.y + kNmR;
-    if (!isnan(sx[lv][lu])) {
-        const int ty = min(v + kNmR + 1, rows - 1), tx = min(u + kNmR + 1, cols - 1);
-        const int y0 = max(v - kNmR, 0), x0 = max(u - kNmR, 0);
-        float cxs = 0.0f, cys = 0.0f, czs = 0.0f;
-        int count = 0;
-        for (int cy = y0; cy < ty; ++cy)
-            for (int cx = x0; cx < tx; ++cx) {
-                const int ly = cy - by + kNmR, lx = cx - bx + kNmR;
-                const float px = sx[ly][lx];
-                if (!isnan(px)) { cxs += px; cys += sy[ly][lx]; czs += sz[ly][lx]; ++count; }
+    const float pcx = sx[lv][lu], pcy = sy[lv][lu], pcz = sz[lv][lu];
+    if (!isnan(pcx)) {
+        // Window rows [max(v-3, 0), min(v+4, rows-1)), columns likewise:
+        // out-of-image taps are NaN in the tile; the far-edge row and column
+        // are excluded by these limits.  One pass over the 49 taps, fully
+        // unrolled so the LDS reads are issued together, accumulating the
+        // count, the sum and the second moments of the taps relative to the
+        // centre vertex (small numbers, so the covariance
+        // S2 - S1 S1^T / n keeps its precision).
+        const int dymax = min(kNmR, rows - 2 - v), dxmax = min(kNmR, cols - 2 - u);
+        float n = 0.0f, s1x = 0.0f, s1y = 0.0f, s1z = 0.0f;
+        float sxx = 0.0f, sxy = 0.0f, sxz = 0.0f, syy = 0.0f, syz = 0.0f, szz = 0.0f;
+#pragma unroll 1
+        for (int dy = -kNmR; dy <= kNmR; ++dy)
+#pragma unroll
+            for (int dx = -kNmR; dx <= kNmR; ++dx) {
+                const float px = sx[lv + dy][lu + dx], py = sy[lv + dy][lu + dx], pz = sz[lv + dy][lu + dx];
+                const bool ok = (px == px) & (dy <= dymax) & (dx <= dxmax);
+                const float qx = ok ? px - pcx : 0.0f;
+                const float qy = ok ? py - pcy : 0.0f;
+                const float qz = ok ? pz - pcz : 0.0f;
+                n += ok ? 1.0f : 0.0f;
+                s1x += qx; s1y += qy; s1z += qz;
+                sxx += qx * qx; sxy += qx * qy; sxz += qx * qz;
+                syy += qy * qy; syz += qy * qz; szz += qz * qz;
             }
-        if (count >= (2 * kNmR + 1) * (2 * kNmR + 1) / 2) {
-            const float ic = 1.0f / (float)count;
-            cxs *= ic; cys *= ic; czs *= ic;
-            float cov[6] = {0, 0, 0, 0, 0, 0};
-            for (int cy = y0; cy < ty; ++cy)
-                for (int cx = x0; cx < tx; ++cx) {
-                    const int ly = cy - by + kNmR, lx = cx - bx + kNmR;
-                    const float px = sx[ly][lx];
-                    if (isnan(px)) continue;
-                    const float dx = px - cxs, dy = sy[ly][lx] - cys, dz = sz[ly][lx] - czs;
-                    cov[0] += dx * dx; cov[1] += dx * dy; cov[2] += dx * dz;
-                    cov[3] += dy * dy; cov[4] += dy * dz; cov[5] += dz * dz;
-                }
+        if (n >= (float)((2 * kNmR + 1) * (2 * kNmR + 1) / 2)) {
+            const float in = 1.0f / n;
+            const float mx = s1x * in, my = s1y * in, mz = s1z * in;
+            float cov[6] = {sxx - s1x * mx, sxy - s1x * my, sxz - s1x * mz,
+                            syy - s1y * my, syz - s1y * mz, szz - s1z * mz};
             float nx, ny, nz;
             smallest_eigvec(cov, nx, ny, nz);
             if (!isnan(nx)) {
-                if (nx * sx[lv][lu] + ny * sy[lv][lu] + nz * sz[lv][lu] > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+                if (nx * pcx + ny * pcy + nz * pcz > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
                 out = kfx_xyz{nx, ny, nz};
             }
         }

@@ -57,14 +57,17 @@ def _pose(kfx, R, t):
     return p
 
 
+@pytest.mark.parametrize("level", [0, 2])
 @pytest.mark.parametrize("mode,rtol", [(0, 1e-9), (2, 1e-9), (1, 1e-4)])
-def test_fused_iteration_matches_double(gpu, orc, pair, mode, rtol):
+def test_fused_iteration_matches_double(gpu, orc, pair, mode, rtol, level):
     # mode 0: compensated FP32 workgroup sums (default), 2: FP64 sums, 1: plain
     # FP32 sums.  All three are compared with the FP32 products summed in
     # double precision; the compensated and FP64 sums agree to 1e-9, plain
-    # FP32 sums over ~1e5 products only to ~1e-5.
+    # FP32 sums over ~1e5 products only to ~1e-5.  Level 0 (640x480) takes two
+    # pixels per thread, level 2 (160x120) one.
     kfx = gpu
-    intr, maps, (Rp, tp), (Rc, tc) = pair
+    intr0, maps0, (Rp, tp), (Rc, tc) = pair
+    intr, maps = intr0.level(level), ii.pyramid_maps(orc, maps0)[level]
     bufs = _upload(kfx, maps)
     h = _icp_handle(kfx, intr.cols * intr.rows)
     kfx.call("kfx_icp_set_sum_mode", h, mode)
