@@ -861,8 +861,13 @@ __global__ void __launch_bounds__(256) depth_consistency_kernel(const kfx_xyz* _
 
 // stats[k] = sum over workgroups of partials[g][k], in a fixed order (one
 // workgroup: thread t sums g = t, t + 256, ..., then a tree over the 256).
+// With a flag, also the fallback decision (1: fall back) for a gate of
+// fallback_m RMS: at least 100 valid model vertices, and an RMS difference
+// over the pixels within max_diff above fallback_m or fewer than half of the
+// valid vertices within max_diff.
 __global__ void __launch_bounds__(256) consistency_reduce_kernel(const double* __restrict__ partials, int nblk,
-                                                                 double* __restrict__ stats) {
+                                                                 double* __restrict__ stats, double fallback_m,
+                                                                 int32_t* __restrict__ flag) {
     __shared__ double red[4][256];
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (int g = threadIdx.x; g < nblk; g += 256)
@@ -875,6 +880,11 @@ __global__ void __launch_bounds__(256) consistency_reduce_kernel(const double* _
         __syncthreads();
     }
     if (threadIdx.x < 4) stats[threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x == 0 && flag != nullptr) {
+        const double sse = red[0][0], cnt = red[1][0], n_valid = red[3][0];
+        const double rms = cnt > 0.0 ? sqrt(sse / cnt) : 0.0;
+        *flag = (fallback_m > 0.0 && n_valid >= 100.0 && (rms > fallback_m || cnt < 0.5 * n_valid)) ? 1 : 0;
+    }
 }
 
 // Library-owned workgroup partials of the consistency gate.
@@ -893,7 +903,8 @@ int consistency_reserve(int cols, int rows) {
 }
 
 int depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
-                      const kfx_intrinsics* intr, float max_diff, double* dev_stats) {
+                      const kfx_intrinsics* intr, float max_diff, double* dev_stats, double fallback_m,
+                      int32_t* dev_flag) {
     const int P = intr->cols * intr->rows;
     const int nblk = (int)divup((unsigned)P, 256u);
     if ((size_t)nblk > g_cons_cap) {
@@ -906,7 +917,7 @@ int depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const ui
     }
     depth_consistency_kernel<<<nblk, 256, 0, stream()>>>(vmap_model, dev_pose, depth, *intr, max_diff, g_cons);
     KFX_CHECK_LAUNCH();
-    consistency_reduce_kernel<<<1, 256, 0, stream()>>>(g_cons, nblk, dev_stats);
+    consistency_reduce_kernel<<<1, 256, 0, stream()>>>(g_cons, nblk, dev_stats, fallback_m, dev_flag);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
@@ -931,7 +942,7 @@ int kfx_depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, cons
                           const kfx_intrinsics* intr, float max_diff_m, double* dev_stats) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(vmap_model && dev_pose && depth && intr && dev_stats, "kfx_depth_consistency: null pointer");
-    return depth_consistency(vmap_model, dev_pose, depth, intr, max_diff_m, dev_stats);
+    return depth_consistency(vmap_model, dev_pose, depth, intr, max_diff_m, dev_stats, 0.0, nullptr);
 }
 
 }  // extern "C"

@@ -22,7 +22,8 @@
 
 namespace kfx {
 int depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
-                      const kfx_intrinsics* intr, float max_diff, double* dev_stats);
+                      const kfx_intrinsics* intr, float max_diff, double* dev_stats, double fallback_m,
+                      int32_t* dev_flag);
 }
 
 
@@ -50,11 +51,11 @@ struct kfx_pipeline {
     long frame = 0;
     hipGraph_t graph = nullptr;           // whole frame (bilateral) or ICP part (denoiser)
     hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph_b = nullptr;         // integrate + raycast part (denoiser)
-    hipGraphExec_t graph_b_exec = nullptr;
     kfx_denoiser* denoiser = nullptr;
     double* dstats = nullptr;             // consistency sums (device)
     double* hstats = nullptr;             // pinned copy
+    int32_t* dgate = nullptr;             // fallback decision of the gate (device)
+    int32_t* hgate = nullptr;             // pinned copy
     uint64_t total_extracted = 0;
     int shifts = 0;
     int fallbacks = 0;
@@ -220,24 +221,34 @@ int icp_part(kfx_pipeline* p, bool begin) {
     return KFX_OK;
 }
 
-// Learned front end: denoiser, pyramid, maps, ICP, then the consistency of
-// the denoised frame with the model at the new pose (fallback gate).
-int learned_part(kfx_pipeline* p) {
+// Integration and raycast at the device pose, skipped on the device when
+// skip points at a non-zero flag.  Enqueues only.
+int map_part_skip(kfx_pipeline* p, const int32_t* skip) {
+    const float *dpose = nullptr, *dprev = nullptr;
+    KFX_TRY(kfx_icp_pose_device(p->icp, &dpose, &dprev));
+    KFX_TRY(kfx_integrate_tsdf_posed(p->raw, p->scaled, p->vol, &p->lvl[0], dpose, dprev, p->cfg.integrate_threshold,
+                                     p->cfg.truncation, &p->cell, &p->rb, skip));
+    KFX_TRY(kfx_raycast_posed(p->vol, p->vp[0], p->np[0], &p->rb, dpose, p->cfg.truncation, &p->lvl[0], &p->cell,
+                              skip));
+    return resize_prev(p);
+}
+
+int map_part(kfx_pipeline* p) { return map_part_skip(p, nullptr); }
+
+// The whole learned frame, enqueued at once: denoiser, pyramid, maps, ICP;
+// the consistency of the denoised frame with the model at the new pose
+// (fallback gate), decided on the device; integration and raycast, which
+// the device skips when the gate falls back (the host then redoes the frame
+// with the bilateral filter).  No host round trip inside the frame.
+int learned_frame(kfx_pipeline* p) {
     KFX_TRY(preprocess(p, true));
     KFX_TRY(icp_part(p, true));
     const float *dpose = nullptr, *dprev = nullptr;
     KFX_TRY(kfx_icp_pose_device(p->icp, &dpose, &dprev));
-    return depth_consistency(p->vp[0], dpose, p->dep[0], &p->lvl[0], 0.1f, p->dstats);
-}
-
-// Integration and raycast at the device pose.  Enqueues only.
-int map_part(kfx_pipeline* p) {
-    const float *dpose = nullptr, *dprev = nullptr;
-    KFX_TRY(kfx_icp_pose_device(p->icp, &dpose, &dprev));
-    KFX_TRY(kfx_integrate_tsdf_posed(p->raw, p->scaled, p->vol, &p->lvl[0], dpose, dprev, p->cfg.integrate_threshold,
-                                     p->cfg.truncation, &p->cell, &p->rb));
-    KFX_TRY(kfx_raycast_posed(p->vol, p->vp[0], p->np[0], &p->rb, dpose, p->cfg.truncation, &p->lvl[0], &p->cell));
-    return resize_prev(p);
+    KFX_TRY(depth_consistency(p->vp[0], dpose, p->dep[0], &p->lvl[0], 0.1f, p->dstats,
+                              1e-3 * (double)p->cfg.fallback_rms_mm, p->dgate));
+    // The raycast below overwrites vp[0]: the gate has read it already.
+    return map_part_skip(p, p->dgate);
 }
 
 // The whole fused frame with the bilateral filter.
@@ -250,12 +261,8 @@ int bilateral_frame(kfx_pipeline* p) {
 void drop_graph(kfx_pipeline* p) {
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    if (p->graph_b_exec) (void)hipGraphExecDestroy(p->graph_b_exec);
-    if (p->graph_b) (void)hipGraphDestroy(p->graph_b);
     p->graph_exec = nullptr;
     p->graph = nullptr;
-    p->graph_b_exec = nullptr;
-    p->graph_b = nullptr;
 }
 
 int capture(kfx_pipeline* p, int (*body)(kfx_pipeline*), hipGraph_t* graph, hipGraphExec_t* exec) {
@@ -399,6 +406,8 @@ int kfx_pipeline_destroy(kfx_pipeline* p) {
     (void)hipFree(p->ext_col);
     (void)hipFree(p->dstats);
     if (p->hstats) (void)hipHostFree(p->hstats);
+    (void)hipFree(p->dgate);
+    if (p->hgate) (void)hipHostFree(p->hgate);
     kfx_icp_destroy(p->icp);
     kfx_slicer_destroy(p->slicer);
     delete p;
@@ -452,6 +461,8 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     if (e == hipSuccess) e = hipMalloc((void**)&p->ext_col, sizeof(kfx_color) * p->ext_cap);
     if (e == hipSuccess) e = hipMalloc((void**)&p->dstats, sizeof(double) * 4);
     if (e == hipSuccess) e = hipHostMalloc((void**)&p->hstats, sizeof(double) * 4, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void**)&p->dgate, sizeof(int32_t));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&p->hgate, sizeof(int32_t), hipHostMallocDefault);
     if (e != hipSuccess) {
         kfx_pipeline_destroy(p);
         return set_error(KFX_ERR_HIP, "kfx_pipeline_create: %s", hipGetErrorString(e));
@@ -481,6 +492,8 @@ int kfx_pipeline_process(kfx_pipeline* p, const uint16_t* depth, kfx_frame_info*
     if (info) memset(info, 0, sizeof(*info));
     const size_t bytes = 2 * (size_t)p->cfg.intr.cols * p->cfg.intr.rows;
     if (depth != p->raw) KFX_TRY(kfx_memcpy_dtod(p->raw, depth, bytes));
+    kfx_icp_result r;
+    bool have_pose = false;  // r holds the fused estimate of this frame
 
     if (p->frame == 0) {
         KFX_TRY(preprocess(p, use_denoiser(p)));
@@ -503,20 +516,19 @@ int kfx_pipeline_process(kfx_pipeline* p, const uint16_t* depth, kfx_frame_info*
     } else if (!use_denoiser(p)) {
         KFX_TRY(run_part(p, bilateral_frame, &p->graph, &p->graph_exec));
     } else {
-        // Denoiser, ICP and the consistency gate; fall back to the bilateral
-        // filter (redoing ICP from the previous pose) when the denoised frame
-        // disagrees with the model by more than fallback_rms_mm.
-        KFX_TRY(run_part(p, learned_part, &p->graph, &p->graph_exec));
+        // The learned frame as one graph; the gate's statistics and decision
+        // come back with the pose read below (one round trip per frame).
+        // When the gate fell back, the device skipped integration and
+        // raycast: the frame is redone with the bilateral filter, ICP from
+        // the previous pose, then integration and raycast.
+        KFX_TRY(run_part(p, learned_frame, &p->graph, &p->graph_exec));
         KFX_CHECK_HIP(hipMemcpyAsync(p->hstats, p->dstats, sizeof(double) * 4, hipMemcpyDeviceToHost, stream()));
-        KFX_CHECK_HIP(hipStreamSynchronize(stream()));
-        const double cnt = p->hstats[1], n_valid = p->hstats[3];
+        KFX_CHECK_HIP(hipMemcpyAsync(p->hgate, p->dgate, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
+        KFX_TRY(kfx_icp_read(p->icp, &r));  // synchronises the stream
+        have_pose = true;
+        const double cnt = p->hstats[1];
         const double rms_mm = cnt > 0 ? 1000.0 * sqrt(p->hstats[0] / cnt) : 0.0;
-        // Fall back when the frame disagrees with the model: RMS above the
-        // threshold over the pixels within 10 cm, or fewer than half of the
-        // model's vertices within 10 cm of the frame at all (a large error, or
-        // an ICP pose that lost the model).
-        const bool fallback = p->cfg.fallback_rms_mm > 0 && n_valid >= 100 &&
-                              (rms_mm > p->cfg.fallback_rms_mm || cnt < 0.5 * n_valid);
+        const bool fallback = *p->hgate != 0;
         if (info) {
             info->denoise_rms_mm = (float)rms_mm;
             info->fallback = fallback ? 1 : 0;
@@ -528,13 +540,13 @@ int kfx_pipeline_process(kfx_pipeline* p, const uint16_t* depth, kfx_frame_info*
             KFX_TRY(kfx_icp_reset(p->icp, &prev));
             KFX_TRY(preprocess(p, false));
             KFX_TRY(icp_part(p, false));
+            KFX_TRY(map_part(p));
             p->fallbacks += 1;
+            have_pose = false;
         }
-        KFX_TRY(run_part(p, map_part, &p->graph_b, &p->graph_b_exec));
     }
     if (p->frame > 0 && p->cfg.mode == KFX_PIPELINE_FUSED) {
-        kfx_icp_result r;
-        KFX_TRY(kfx_icp_read(p->icp, &r));
+        if (!have_pose) KFX_TRY(kfx_icp_read(p->icp, &r));
         memcpy(p->R, r.pose.R, sizeof(p->R));
         memcpy(p->t, r.pose.t, sizeof(p->t));
         if (info) {

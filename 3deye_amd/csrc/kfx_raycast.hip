@@ -136,7 +136,9 @@ template <bool kDevicePose>
 __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict__ vol,
                                                       kfx_xyz* __restrict__ vmap,
                                                       kfx_xyz* __restrict__ nmap, RaycastParams p,
-                                                      const float* __restrict__ dpose) {
+                                                      const float* __restrict__ dpose,
+                                                      const int32_t* __restrict__ dskip) {
+    if (kDevicePose && dskip != nullptr && *dskip != 0) return;  // uniform across the grid
     if (kDevicePose) {
         for (int i = 0; i < 9; ++i) p.r[i] = dpose[i];
         p.tx = dpose[9]; p.ty = dpose[10]; p.tz = dpose[11];
@@ -299,7 +301,7 @@ using namespace kfx;
 static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
                           const kfx_round_buffer* rb, const kfx_point_3f* translation,
                           const kfx_matrix_3f* rotation, const float* dev_pose, float truncation_distance,
-                          const kfx_intrinsics* intr, const kfx_point_3f* cell_size) {
+                          const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(volume && vertex_map && normal_map && translation && rotation && intr && cell_size,
                 "kfx_raycast: null pointer");
@@ -326,9 +328,9 @@ static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* 
     dim3 block(64, 4);
     dim3 grid(divup(intr->cols, 64), divup(intr->rows, 4));
     if (dev_pose)
-        raycast_kernel<true><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, dev_pose);
+        raycast_kernel<true><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, dev_pose, dev_skip);
     else
-        raycast_kernel<false><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, nullptr);
+        raycast_kernel<false><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, nullptr, nullptr);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
@@ -338,15 +340,15 @@ extern "C" int kfx_raycast(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz*
                            const kfx_matrix_3f* rotation, float truncation_distance,
                            const kfx_intrinsics* intr, const kfx_point_3f* cell_size) {
     return raycast_launch(volume, vertex_map, normal_map, rb, translation, rotation, nullptr, truncation_distance,
-                          intr, cell_size);
+                          intr, cell_size, nullptr);
 }
 
 extern "C" int kfx_raycast_posed(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
                                  const kfx_round_buffer* rb, const float* dev_pose, float truncation_distance,
-                                 const kfx_intrinsics* intr, const kfx_point_3f* cell_size) {
+                                 const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip) {
     KFX_REQUIRE(dev_pose != nullptr, "kfx_raycast_posed: null pose");
     const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
     const kfx_point_3f zero = {0, 0, 0};
     return raycast_launch(volume, vertex_map, normal_map, rb, &zero, &ident, dev_pose, truncation_distance, intr,
-                          cell_size);
+                          cell_size, dev_skip);
 }

@@ -259,7 +259,9 @@ __device__ __forceinline__ bool segment_culled(const IntegrateParams& p, const P
 template <bool kDevicePose>
 __global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
 integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume, const IntegrateParams p,
-                 const float* __restrict__ dpose, const float* __restrict__ dprev, float motion_thr) {
+                 const float* __restrict__ dpose, const float* __restrict__ dprev, float motion_thr,
+                 const int32_t* __restrict__ dskip) {
+    if (kDevicePose && dskip != nullptr && *dskip != 0) return;  // uniform across the grid
     PoseTerms q;
     if (kDevicePose) {
         if (!load_device_pose(p, q, dpose, dprev, motion_thr)) return;
@@ -566,7 +568,7 @@ int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* vol
     ProfScope prof("integrate");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
     dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
-    integrate_kernel<false><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, nullptr, nullptr, 0.0f);
+    integrate_kernel<false><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, nullptr, nullptr, 0.0f, nullptr);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
@@ -574,7 +576,7 @@ int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* vol
 int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
                              const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
                              float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
-                             const kfx_round_buffer* rb) {
+                             const kfx_round_buffer* rb, const int32_t* dev_skip) {
     KFX_REQUIRE(volume != nullptr && dev_pose != nullptr, "kfx_integrate_tsdf_posed: null pointer");
     const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
     const kfx_point_3f zero = {0, 0, 0};
@@ -587,7 +589,7 @@ int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsd
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
     dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
     integrate_kernel<true><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, dev_pose, dev_prev_pose_inv,
-                                                         motion_threshold);
+                                                         motion_threshold, dev_skip);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }

@@ -205,9 +205,9 @@ _PROTOS = {
     "kfx_raycast": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), C.POINTER(Point3f), C.POINTER(Matrix3f), C.c_float,
                          C.POINTER(Intrinsics), C.POINTER(Point3f)]),
     "kfx_integrate_tsdf_posed": (_I, [_P, _P, _P, C.POINTER(Intrinsics), _P, _P, C.c_float, C.c_float,
-                                      C.POINTER(Point3f), C.POINTER(RoundBuffer)]),
+                                      C.POINTER(Point3f), C.POINTER(RoundBuffer), _P]),
     "kfx_raycast_posed": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), _P, C.c_float, C.POINTER(Intrinsics),
-                               C.POINTER(Point3f)]),
+                               C.POINTER(Point3f), _P]),
     "kfx_profile_enable": (_I, [_I]),
     "kfx_profile_reset": (_I, []),
     "kfx_profile_query": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),

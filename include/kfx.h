@@ -125,11 +125,13 @@ int kfx_integrate_count_updates(const uint16_t* depth, float* depth_scaled, cons
  * call can follow ICP without a host round trip (no reference counterpart).
  * When dev_prev_pose_inv (R^T, t of the previous frame) is given and
  * motion_threshold > 0, the update is skipped if the camera moved less than
- * the threshold ((rotation angle + translation) / 2, the KinectFusion rule). */
+ * the threshold ((rotation angle + translation) / 2, the KinectFusion rule).
+ * When dev_skip is given and *dev_skip != 0 at run time, the update is
+ * skipped (a decision taken earlier on the device, e.g. a fallback gate). */
 int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
                              const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
                              float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
-                             const kfx_round_buffer* rb);
+                             const kfx_round_buffer* rb, const int32_t* dev_skip);
 
 /* Raycast (common/inc/opencl_wrappers.h:23-33, GPU/opencl/raycast.cl:182-405):
  * per-pixel march through the cyclic TSDF; writes the surface vertex and the
@@ -138,10 +140,11 @@ int kfx_raycast(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map
                 const kfx_round_buffer* rb, const kfx_point_3f* translation,
                 const kfx_matrix_3f* rotation, float truncation_distance,
                 const kfx_intrinsics* intr, const kfx_point_3f* cell_size);
-/* kfx_raycast with the pose read from device memory (12 floats, as above). */
+/* kfx_raycast with the pose read from device memory (12 floats, as above);
+ * nothing is written when dev_skip is given and *dev_skip != 0. */
 int kfx_raycast_posed(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
                       const kfx_round_buffer* rb, const float* dev_pose, float truncation_distance,
-                      const kfx_intrinsics* intr, const kfx_point_3f* cell_size);
+                      const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip);
 
 
 /* ------------------------------------------------------------------------
