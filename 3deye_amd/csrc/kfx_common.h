@@ -22,6 +22,8 @@ namespace kfx {
 int integrate_reserve(int cols, int rows);
 // Reserves the depth-consistency gate's workgroup partials (kfx_denoise.hip).
 int consistency_reserve(int cols, int rows);
+// Reserves the colour integration's vertex-distance pyramid (kfx_volume.hip).
+int colors_reserve(int cols, int rows);
 
 // Library stream (created by kfx_init, default priority) on which every
 // entry point enqueues its work.

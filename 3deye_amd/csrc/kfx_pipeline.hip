@@ -34,6 +34,8 @@ struct kfx_pipeline {
     double origin_world[3];
     kfx_intrinsics lvl[KFX_ICP_LEVELS];
     kfx_tsdf* vol = nullptr;
+    kfx_color* cvol = nullptr;            // colour volume (cfg.colors)
+    kfx_color* rgb = nullptr;             // last colour frame (cfg.colors)
     uint16_t* raw = nullptr;
     uint16_t* dep[KFX_ICP_LEVELS] = {};
     float* scaled = nullptr;
@@ -189,6 +191,18 @@ int raycast_host_pose(kfx_pipeline* p) {
     return resize_prev(p);
 }
 
+// Colour integration at the host pose from the raycast just made (vp[0]).
+int colors_host_pose(kfx_pipeline* p) {
+    if (!p->cvol) return KFX_OK;
+    double Ri[9];
+    mat_T(p->R, Ri);
+    const kfx_matrix_3f m = to_m3f(Ri);
+    const kfx_point_3f tt = to_p3f(p->t);
+    KFX_TRY(kfx_integrate_colors(p->cvol, p->rgb, p->vp[0], &p->lvl[0], &m, &tt, p->cfg.truncation,
+                                 p->cfg.color_max_weight, &p->cell, &p->rb));
+    return step_sync(p);
+}
+
 int icp_plain(kfx_pipeline* p, kfx_frame_info* info) {
     double Rpi[9];
     mat_T(p->Rprev, Rpi);
@@ -230,6 +244,9 @@ int map_part_skip(kfx_pipeline* p, const int32_t* skip) {
                                      p->cfg.truncation, &p->cell, &p->rb, skip));
     KFX_TRY(kfx_raycast_posed(p->vol, p->vp[0], p->np[0], &p->rb, dpose, p->cfg.truncation, &p->lvl[0], &p->cell,
                               skip));
+    if (p->cvol)
+        KFX_TRY(kfx_integrate_colors_posed(p->cvol, p->rgb, p->vp[0], &p->lvl[0], dpose, p->cfg.truncation,
+                                           p->cfg.color_max_weight, &p->cell, &p->rb, skip));
     return resize_prev(p);
 }
 
@@ -317,22 +334,31 @@ int maybe_shift(kfx_pipeline* p, kfx_frame_info* info) {
     const kfx_point_3i pmn{mn[0], mn[1], mn[2]}, pmx{mx[0], mx[1], mx[2]};
     if (p->cfg.extract_on_shift) {
         uint64_t count = 0;
-        int rc = kfx_extract_slice(p->slicer, &pmn, &pmx, &p->rb, p->vol, nullptr, p->ext_pts, p->ext_col, p->ext_cap,
-                                   &count);
+        // Records and XYZI points are both 16 B: one buffer serves either.
+        static_assert(sizeof(kfx_slice_record) == sizeof(kfx_xyzi), "record size");
+        auto run = [&]() {
+            if (p->cfg.extract_records)
+                return kfx_extract_slice_records(p->slicer, &pmn, &pmx, &p->rb, p->vol, p->cvol,
+                                                 reinterpret_cast<kfx_slice_record*>(p->ext_pts), p->ext_col,
+                                                 p->ext_cap, &count);
+            return kfx_extract_slice(p->slicer, &pmn, &pmx, &p->rb, p->vol, p->cvol, p->ext_pts, p->ext_col,
+                                     p->ext_cap, &count);
+        };
+        int rc = run();
         if (rc == KFX_ERR_CAPACITY) {
             (void)hipFree(p->ext_pts);
             (void)hipFree(p->ext_col);
             p->ext_cap = count + count / 4 + 1024;
             KFX_CHECK_HIP(hipMalloc((void**)&p->ext_pts, sizeof(kfx_xyzi) * p->ext_cap));
             KFX_CHECK_HIP(hipMalloc((void**)&p->ext_col, sizeof(kfx_color) * p->ext_cap));
-            rc = kfx_extract_slice(p->slicer, &pmn, &pmx, &p->rb, p->vol, nullptr, p->ext_pts, p->ext_col, p->ext_cap,
-                                   &count);
+            rc = run();
         }
         KFX_TRY(rc);
         p->total_extracted += count;
         if (info) info->extracted_points = count;
     }
     KFX_TRY(kfx_clean_tsdf_slice(&pmn, &pmx, &p->rb, p->vol));
+    if (p->cvol) KFX_TRY(kfx_clean_color_slice(&pmn, &pmx, &p->rb, p->cvol));
     KFX_TRY(step_sync(p));
     int* og[3] = {&p->rb.origin_GRID.x, &p->rb.origin_GRID.y, &p->rb.origin_GRID.z};
     float* ogg[3] = {&p->rb.origin_GRID_global.x, &p->rb.origin_GRID_global.y, &p->rb.origin_GRID_global.z};
@@ -382,6 +408,8 @@ int kfx_pipeline_default_config(kfx_pipeline_config* c) {
     c->converge_rotation = 0.0;
     c->converge_translation = 0.0;
     c->extract_on_shift = 1;
+    c->colors = 0;
+    c->color_max_weight = KFX_DEFAULT_COLOR_MAX_WEIGHT;
     return KFX_OK;
 }
 
@@ -391,6 +419,8 @@ int kfx_pipeline_destroy(kfx_pipeline* p) {
     (void)hipStreamSynchronize(stream());
     drop_graph(p);
     (void)hipFree(p->vol);
+    (void)hipFree(p->cvol);
+    (void)hipFree(p->rgb);
     (void)hipFree(p->raw);
     for (int l = 0; l < KFX_ICP_LEVELS; ++l) {
         (void)hipFree(p->dep[l]);
@@ -425,6 +455,8 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
                     (int64_t)cfg->voxels.x * cfg->voxels.y * cfg->voxels.z < (int64_t)1 << 31,
                 "kfx_pipeline_create: voxel counts must be positive, below 2^31 in total");
     KFX_REQUIRE(!(cfg->mode == KFX_PIPELINE_PLAIN && cfg->use_graph), "kfx_pipeline_create: graphs need fused mode");
+    KFX_REQUIRE(!cfg->colors || (cfg->color_max_weight >= 1 && cfg->color_max_weight <= 255),
+                "kfx_pipeline_create: color_max_weight %d outside [1, 255]", cfg->color_max_weight);
     kfx_pipeline* p = new kfx_pipeline();
     p->cfg = *cfg;
     kfx_round_buffer& rb = p->rb;
@@ -445,6 +477,8 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     hipError_t e = hipMalloc((void**)&p->vol, sizeof(kfx_tsdf) * (size_t)rb.buffer_length);
     if (e == hipSuccess) e = hipMalloc((void**)&p->raw, 2 * npix);
     if (e == hipSuccess) e = hipMalloc((void**)&p->scaled, 4 * npix);
+    if (e == hipSuccess && cfg->colors) e = hipMalloc((void**)&p->cvol, sizeof(kfx_color) * (size_t)rb.buffer_length);
+    if (e == hipSuccess && cfg->colors) e = hipMalloc((void**)&p->rgb, sizeof(kfx_color) * npix);
     for (int l = 0; l < KFX_ICP_LEVELS && e == hipSuccess; ++l) {
         const size_t n = (size_t)p->lvl[l].cols * p->lvl[l].rows;
         e = hipMalloc((void**)&p->dep[l], 2 * n);
@@ -472,7 +506,9 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     if (!rc) rc = kfx_slicer_create(&rb, &p->slicer);
     if (!rc) rc = integrate_reserve(p->cfg.intr.cols, p->cfg.intr.rows);
     if (!rc) rc = consistency_reserve(p->cfg.intr.cols, p->cfg.intr.rows);
+    if (!rc && p->cvol) rc = colors_reserve(p->cfg.intr.cols, p->cfg.intr.rows);
     if (!rc) rc = kfx_init_tsdf(p->vol, &rb);
+    if (!rc && p->cvol) rc = kfx_init_colors(p->cvol, &rb);
     if (!rc) rc = kfx_synchronize();
     if (rc) {
         kfx_pipeline_destroy(p);
@@ -486,12 +522,11 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     return KFX_OK;
 }
 
-int kfx_pipeline_process(kfx_pipeline* p, const uint16_t* depth, kfx_frame_info* info) {
-    KFX_REQUIRE_INIT();
-    KFX_REQUIRE(p && depth, "kfx_pipeline_process: null pointer");
+static int process_frame(kfx_pipeline* p, const uint16_t* depth, const kfx_color* rgb, kfx_frame_info* info) {
     if (info) memset(info, 0, sizeof(*info));
-    const size_t bytes = 2 * (size_t)p->cfg.intr.cols * p->cfg.intr.rows;
-    if (depth != p->raw) KFX_TRY(kfx_memcpy_dtod(p->raw, depth, bytes));
+    const size_t npix = (size_t)p->cfg.intr.cols * p->cfg.intr.rows;
+    if (depth != p->raw) KFX_TRY(kfx_memcpy_dtod(p->raw, depth, 2 * npix));
+    if (rgb != nullptr && rgb != p->rgb) KFX_TRY(kfx_memcpy_dtod(p->rgb, rgb, sizeof(kfx_color) * npix));
     kfx_icp_result r;
     bool have_pose = false;  // r holds the fused estimate of this frame
 
@@ -499,6 +534,7 @@ int kfx_pipeline_process(kfx_pipeline* p, const uint16_t* depth, kfx_frame_info*
         KFX_TRY(preprocess(p, use_denoiser(p)));
         KFX_TRY(integrate_host_pose(p));
         KFX_TRY(raycast_host_pose(p));
+        KFX_TRY(colors_host_pose(p));
         if (p->cfg.mode == KFX_PIPELINE_FUSED) {
             kfx_pose pose;
             memcpy(pose.R, p->R, sizeof(p->R));
@@ -512,6 +548,7 @@ int kfx_pipeline_process(kfx_pipeline* p, const uint16_t* depth, kfx_frame_info*
         const bool integ = motion(p->R, p->t, p->Rprev, p->tprev) >= p->cfg.integrate_threshold;
         if (integ) KFX_TRY(integrate_host_pose(p));
         KFX_TRY(raycast_host_pose(p));
+        KFX_TRY(colors_host_pose(p));
         if (info) info->integrated = integ ? 1 : 0;
     } else if (!use_denoiser(p)) {
         KFX_TRY(run_part(p, bilateral_frame, &p->graph, &p->graph_exec));
@@ -570,6 +607,20 @@ int kfx_pipeline_process(kfx_pipeline* p, const uint16_t* depth, kfx_frame_info*
     return KFX_OK;
 }
 
+int kfx_pipeline_process(kfx_pipeline* p, const uint16_t* depth, kfx_frame_info* info) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(p && depth, "kfx_pipeline_process: null pointer");
+    KFX_REQUIRE(!p->cvol, "kfx_pipeline_process: the pipeline keeps a colour volume; use kfx_pipeline_process_rgbd");
+    return process_frame(p, depth, nullptr, info);
+}
+
+int kfx_pipeline_process_rgbd(kfx_pipeline* p, const uint16_t* depth, const kfx_color* rgb, kfx_frame_info* info) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(p && depth && rgb, "kfx_pipeline_process_rgbd: null pointer");
+    KFX_REQUIRE(p->cvol, "kfx_pipeline_process_rgbd: the pipeline was created without colours (cfg.colors = 0)");
+    return process_frame(p, depth, rgb, info);
+}
+
 int kfx_pipeline_set_denoiser(kfx_pipeline* p, kfx_denoiser* dn) {
     KFX_REQUIRE(p != nullptr, "kfx_pipeline_set_denoiser: null pipeline");
     KFX_REQUIRE(dn == nullptr || p->cfg.mode == KFX_PIPELINE_FUSED, "kfx_pipeline_set_denoiser: needs fused mode");
@@ -596,8 +647,11 @@ int kfx_pipeline_state(kfx_pipeline* p, kfx_pipeline_view* v) {
     v->shifts = p->shifts;
     v->fallbacks = p->fallbacks;
     v->extracted_points = p->total_extracted;
-    v->last_points = p->ext_pts;
+    v->last_points = p->cfg.extract_records ? nullptr : p->ext_pts;
+    v->last_records = p->cfg.extract_records ? reinterpret_cast<kfx_slice_record*>(p->ext_pts) : nullptr;
     v->last_colors = p->ext_col;
+    v->color_volume = p->cvol;
+    v->rgb = p->rgb;
     return KFX_OK;
 }
 

@@ -132,12 +132,16 @@ __device__ __forceinline__ f3 ray_at(f3 o, f3 d, float t) {
     return f3{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
 }
 
-template <bool kDevicePose>
+// kCount: also count the algorithm's voxel reads (first sample, march
+// samples inside the volume, 8 per in-bounds trilinear evaluation) into
+// *count, for the raycast's algorithmic bytes; results are unchanged.
+template <bool kDevicePose, bool kCount = false>
 __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict__ vol,
                                                       kfx_xyz* __restrict__ vmap,
                                                       kfx_xyz* __restrict__ nmap, RaycastParams p,
                                                       const float* __restrict__ dpose,
-                                                      const int32_t* __restrict__ dskip) {
+                                                      const int32_t* __restrict__ dskip,
+                                                      unsigned long long* __restrict__ count = nullptr) {
     if (kDevicePose && dskip != nullptr && *dskip != 0) return;  // uniform across the grid
     if (kDevicePose) {
         for (int i = 0; i < 9; ++i) p.r[i] = dpose[i];
@@ -193,6 +197,7 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
         // value / 32767 is the sign of the stored int16, so the samples stay
         // undecoded here.
         int tsdf = read_raw(vol, rb, gx, gy, gz);
+        uint32_t ns = 1;
         const float max_time = 3.0f * (vol_m.x + vol_m.y + vol_m.z);
         const float step = p.time_step;
         const float fnx = (float)nxv, fny = (float)nyv, fnz = (float)nzv;
@@ -234,6 +239,7 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
             for (int k = 0; k < kMarchBatch; ++k) {
                 // !ok: the time range ended or the sample left the volume.
                 if (!ok[k]) { done = true; break; }
+                if (kCount) ++ns;
                 const int tsdf_prev = tsdf;
                 tsdf = val[k];
                 if (tsdf_prev < 0 && tsdf > 0) { done = true; break; }
@@ -253,9 +259,11 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
                 const f3 q1 = ray_at(start, dir, hit_tn);
                 const float ftdt = interpolate_trilinear(vol, q1, p);
                 if (isnan(ftdt)) break;
+                if (kCount) ns += 8;
                 const f3 q2 = ray_at(start, dir, t_cur);
                 const float ft = interpolate_trilinear(vol, q2, p);
                 if (isnan(ft)) break;
+                if (kCount) ns += 8;
                 const float ts = t_cur - step * ft / (ftdt - ft);
                 const f3 vert = ray_at(start, dir, ts);
                 vout = kfx_xyz{vert.x, vert.y, vert.z};
@@ -282,11 +290,15 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
                     tt = vert;
                     tt.z -= p.csz;
                     const float fz2 = interpolate_trilinear(vol, tt, p);
+                    if (kCount)
+                        ns += 8u * (uint32_t)(!isnan(fx1) + !isnan(fx2) + !isnan(fy1) + !isnan(fy2) + !isnan(fz1) +
+                                              !isnan(fz2));
                     const f3 n = normalized3(f3{fx1 - fx2, fy1 - fy2, fz1 - fz2});
                     nout = kfx_xyz{n.x, n.y, n.z};
                 }
             } while (false);
         }
+        if (kCount) atomicAdd(count, (unsigned long long)ns);
     }
     vmap[idx] = vout;
     nmap[idx] = nout;
@@ -301,7 +313,8 @@ using namespace kfx;
 static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
                           const kfx_round_buffer* rb, const kfx_point_3f* translation,
                           const kfx_matrix_3f* rotation, const float* dev_pose, float truncation_distance,
-                          const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip) {
+                          const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip,
+                          unsigned long long* dev_count = nullptr) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(volume && vertex_map && normal_map && translation && rotation && intr && cell_size,
                 "kfx_raycast: null pointer");
@@ -327,7 +340,10 @@ static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* 
     ProfScope prof("raycast");
     dim3 block(64, 4);
     dim3 grid(divup(intr->cols, 64), divup(intr->rows, 4));
-    if (dev_pose)
+    if (dev_count)
+        raycast_kernel<false, true><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, nullptr, nullptr,
+                                                                  dev_count);
+    else if (dev_pose)
         raycast_kernel<true><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, dev_pose, dev_skip);
     else
         raycast_kernel<false><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, nullptr, nullptr);
@@ -351,4 +367,31 @@ extern "C" int kfx_raycast_posed(const kfx_tsdf* volume, kfx_xyz* vertex_map, kf
     const kfx_point_3f zero = {0, 0, 0};
     return raycast_launch(volume, vertex_map, normal_map, rb, &zero, &ident, dev_pose, truncation_distance, intr,
                           cell_size, dev_skip);
+}
+
+extern "C" int kfx_raycast_count_samples(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                                         const kfx_round_buffer* rb, const kfx_point_3f* translation,
+                                         const kfx_matrix_3f* rotation, float truncation_distance,
+                                         const kfx_intrinsics* intr, const kfx_point_3f* cell_size,
+                                         uint64_t* samples) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(samples != nullptr, "kfx_raycast_count_samples: null output");
+    unsigned long long* dcount = nullptr;
+    KFX_CHECK_HIP(hipMallocAsync((void**)&dcount, sizeof(unsigned long long), stream()));
+    hipError_t e = hipMemsetAsync(dcount, 0, sizeof(unsigned long long), stream());
+    int rc = KFX_OK;
+    if (e == hipSuccess)
+        rc = raycast_launch(volume, vertex_map, normal_map, rb, translation, rotation, nullptr, truncation_distance,
+                            intr, cell_size, nullptr, dcount);
+    unsigned long long host = 0;
+    if (e == hipSuccess && rc == KFX_OK)
+        e = hipMemcpyAsync(&host, dcount, sizeof(host), hipMemcpyDeviceToHost, stream());
+    hipError_t e2 = hipFreeAsync(dcount, stream());
+    hipError_t e3 = hipStreamSynchronize(stream());
+    if (rc) return rc;
+    KFX_CHECK_HIP(e);
+    KFX_CHECK_HIP(e2);
+    KFX_CHECK_HIP(e3);
+    *samples = host;
+    return KFX_OK;
 }

@@ -209,10 +209,13 @@ __global__ void __launch_bounds__(256) scan_seg_apply_kernel(uint32_t* __restric
     }
 }
 
+// Pass 3.  Writes XYZI points (out_pts) or records (out_rec, with the
+// global grid offset g0), and colours when out_col is given.
 __global__ void __launch_bounds__(256) slice_emit_kernel(const kfx_tsdf* __restrict__ vol,
                                                          const kfx_color* __restrict__ col, SliceParams p,
                                                          const uint32_t* __restrict__ offsets,
                                                          kfx_xyzi* __restrict__ out_pts,
+                                                         kfx_slice_record* __restrict__ out_rec, int3 g0,
                                                          kfx_color* __restrict__ out_col) {
     const int lane = threadIdx.x & 63;
     const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -236,10 +239,16 @@ __global__ void __launch_bounds__(256) slice_emit_kernel(const kfx_tsdf* __restr
                 if (bit) {
                     const uint32_t idx = base[k] + (uint32_t)__popcll(ballot & below);
                     const int px = x + (k == 0), py = y + (k == 1), pz = z + (k == 2);
-                    out_pts[idx] = kfx_xyzi{(float)px, (float)py, (float)pz, 1.0f};
-                    kfx_color c{0, 0, 0, 0};
-                    if (col != nullptr) c = col[cyclic_address_in_range(px, py, pz, p.rb)];
-                    out_col[idx] = c;
+                    if (out_pts != nullptr) out_pts[idx] = kfx_xyzi{(float)px, (float)py, (float)pz, 1.0f};
+                    if (out_rec != nullptr) {
+                        const kfx_tsdf v = vol[cyclic_address_in_range(px, py, pz, p.rb)];
+                        out_rec[idx] = kfx_slice_record{px + g0.x, py + g0.y, pz + g0.z, v.value, v.weight};
+                    }
+                    if (out_col != nullptr) {
+                        kfx_color c{0, 0, 0, 0};
+                        if (col != nullptr) c = col[cyclic_address_in_range(px, py, pz, p.rb)];
+                        out_col[idx] = c;
+                    }
                 }
                 base[k] += (uint32_t)__popcll(ballot);
             }
@@ -370,15 +379,16 @@ int kfx_slicer_destroy(kfx_slicer* s) {
     return KFX_OK;
 }
 
-int kfx_extract_slice(kfx_slicer* s, const kfx_point_3i* min_bounds, const kfx_point_3i* max_bounds,
-                      const kfx_round_buffer* rb, const kfx_tsdf* tsdf_volume, const kfx_color* color_volume,
-                      kfx_xyzi* points, kfx_color* colors, uint64_t capacity, uint64_t* count) {
+static int extract(const char* who, kfx_slicer* s, const kfx_point_3i* min_bounds, const kfx_point_3i* max_bounds,
+                   const kfx_round_buffer* rb, const kfx_tsdf* tsdf_volume, const kfx_color* color_volume,
+                   kfx_xyzi* points, kfx_slice_record* records, kfx_color* colors, uint64_t capacity,
+                   uint64_t* count) {
     SliceParams p;
-    int rc = slice_setup("kfx_extract_slice", min_bounds, max_bounds, rb, p);
+    int rc = slice_setup(who, min_bounds, max_bounds, rb, p);
     if (rc) return rc;
-    KFX_REQUIRE(s && tsdf_volume && count, "kfx_extract_slice: null pointer");
+    KFX_REQUIRE(s && tsdf_volume && count, "%s: null pointer", who);
     KFX_REQUIRE(s->rows == (long)rb->voxels_size.y * rb->voxels_size.z,
-                "kfx_extract_slice: slicer was created for another volume size");
+                "%s: slicer was created for another volume size", who);
     const long rows = s->rows;
     const unsigned row_blocks = (unsigned)((rows + 3) / 4);
     {
@@ -404,15 +414,32 @@ int kfx_extract_slice(kfx_slicer* s, const kfx_point_3i* min_bounds, const kfx_p
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
     const unsigned long long n = *s->host_total;
     *count = n;
-    KFX_REQUIRE(n < 0xFFFFFFFFull, "kfx_extract_slice: %llu points overflow the 32-bit index", n);
+    KFX_REQUIRE(n < 0xFFFFFFFFull, "%s: %llu points overflow the 32-bit index", who, n);
     if (n == 0) return KFX_OK;
-    if (n > capacity || points == nullptr || colors == nullptr)
-        return set_error(KFX_ERR_CAPACITY, "kfx_extract_slice: %llu points, capacity %llu", n,
-                         (unsigned long long)capacity);
+    if (n > capacity || (points == nullptr && records == nullptr) || (points != nullptr && colors == nullptr))
+        return set_error(KFX_ERR_CAPACITY, "%s: %llu points, capacity %llu", who, n, (unsigned long long)capacity);
+    const int3 g0 = make_int3((int)rintf(rb->origin_GRID_global.x), (int)rintf(rb->origin_GRID_global.y),
+                              (int)rintf(rb->origin_GRID_global.z));
     ProfScope prof("slice_emit");
-    slice_emit_kernel<<<row_blocks, 256, 0, stream()>>>(tsdf_volume, color_volume, p, s->counts, points, colors);
+    slice_emit_kernel<<<row_blocks, 256, 0, stream()>>>(tsdf_volume, color_volume, p, s->counts, points, records, g0,
+                                                        colors);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
+}
+
+int kfx_extract_slice(kfx_slicer* s, const kfx_point_3i* min_bounds, const kfx_point_3i* max_bounds,
+                      const kfx_round_buffer* rb, const kfx_tsdf* tsdf_volume, const kfx_color* color_volume,
+                      kfx_xyzi* points, kfx_color* colors, uint64_t capacity, uint64_t* count) {
+    return extract("kfx_extract_slice", s, min_bounds, max_bounds, rb, tsdf_volume, color_volume, points, nullptr,
+                   colors, capacity, count);
+}
+
+int kfx_extract_slice_records(kfx_slicer* s, const kfx_point_3i* min_bounds, const kfx_point_3i* max_bounds,
+                              const kfx_round_buffer* rb, const kfx_tsdf* tsdf_volume,
+                              const kfx_color* color_volume, kfx_slice_record* records, kfx_color* colors,
+                              uint64_t capacity, uint64_t* count) {
+    return extract("kfx_extract_slice_records", s, min_bounds, max_bounds, rb, tsdf_volume, color_volume, nullptr,
+                   records, colors, capacity, count);
 }
 
 }  // extern "C"

@@ -58,12 +58,15 @@ void look_rotation(const double f_in[3], double R[9]) {
 }
 
 // Smallest positive ray parameter s of o + s d against the scene (d has
-// camera z component 1, so s is the camera z-depth).  0 = no hit.
-double intersect(const kfx_synth_scene& sc, const double o[3], const double d[3]) {
+// camera z component 1, so s is the camera z-depth).  0 = no hit.  *hit_id
+// (optional) receives the primitive hit: the object index, -1 for the
+// ground, -2 for none.
+double intersect(const kfx_synth_scene& sc, const double o[3], const double d[3], int* hit_id = nullptr) {
     double best = INFINITY;
+    int id = -2;
     if (d[1] > 1e-12) {
         const double s = (sc.ground_y - o[1]) / d[1];
-        if (s > 0 && s < best) best = s;
+        if (s > 0 && s < best) { best = s; id = -1; }
     }
     for (int i = 0; i < sc.n_objects; ++i) {
         const kfx_synth_object& ob = sc.objects[i];
@@ -77,7 +80,7 @@ double intersect(const kfx_synth_scene& sc, const double o[3], const double d[3]
             const double sq = sqrt(disc);
             double s = (-b - sq) / a;
             if (s <= 0) s = (-b + sq) / a;
-            if (s > 0 && s < best) best = s;
+            if (s > 0 && s < best) { best = s; id = i; }
         } else {
             double t0 = -INFINITY, t1 = INFINITY;
             bool miss = false;
@@ -94,10 +97,41 @@ double intersect(const kfx_synth_scene& sc, const double o[3], const double d[3]
             }
             if (miss || t0 > t1) continue;
             const double s = t0 > 0 ? t0 : t1;
-            if (s > 0 && s < best) best = s;
+            if (s > 0 && s < best) { best = s; id = i; }
         }
     }
+    if (hit_id) *hit_id = std::isfinite(best) ? id : -2;
     return std::isfinite(best) ? best : 0.0;
+}
+
+// Albedo of a surface point: object i a fixed saturated colour (hue on the
+// golden-ratio sequence), the ground a 0.5 m checkerboard of two greys,
+// nothing black.  The w byte is 1 as the reference's capture sets it
+// (common/src/Capture.cpp:119-122).
+kfx_color albedo(int id, const double p[3]) {
+    kfx_color c{0, 0, 0, 1};
+    if (id == -1) {
+        const long k = (long)floor(p[0] / 0.5) + (long)floor(p[2] / 0.5);
+        const uint8_t g = (k & 1) ? 200 : 60;
+        c.r = c.g = c.b = g;
+    } else if (id >= 0) {
+        const double h = fmod(0.6180339887498949 * (id + 1), 1.0) * 6.0, s = 0.75, v = 0.9;
+        const int sector = (int)h;
+        const double f = h - sector, pp = v * (1 - s), q = v * (1 - s * f), t = v * (1 - s * (1 - f));
+        double rgb[3];
+        switch (sector % 6) {
+            case 0: rgb[0] = v; rgb[1] = t; rgb[2] = pp; break;
+            case 1: rgb[0] = q; rgb[1] = v; rgb[2] = pp; break;
+            case 2: rgb[0] = pp; rgb[1] = v; rgb[2] = t; break;
+            case 3: rgb[0] = pp; rgb[1] = q; rgb[2] = v; break;
+            case 4: rgb[0] = t; rgb[1] = pp; rgb[2] = v; break;
+            default: rgb[0] = v; rgb[1] = pp; rgb[2] = q; break;
+        }
+        c.r = (uint8_t)lrint(255.0 * rgb[0]);
+        c.g = (uint8_t)lrint(255.0 * rgb[1]);
+        c.b = (uint8_t)lrint(255.0 * rgb[2]);
+    }
+    return c;
 }
 
 }  // namespace
@@ -200,6 +234,36 @@ int kfx_synth_render(const kfx_synth_scene* sc, const kfx_pose* pose, const kfx_
                 uint16_t d = 0;
                 if (zm >= 0.4 && zm <= 6.0) d = (uint16_t)lrint(zm * 1000.0);
                 depth_mm[i] = d;
+            }
+    };
+    if (nt <= 1) {
+        rows(0, H);
+    } else {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < nt; ++k) pool.emplace_back(rows, H * k / nt, H * (k + 1) / nt);
+        for (auto& th : pool) th.join();
+    }
+    return KFX_OK;
+}
+
+int kfx_synth_render_rgb(const kfx_synth_scene* sc, const kfx_pose* pose, const kfx_intrinsics* in, int threads,
+                         kfx_color* rgb) {
+    if (!sc || !pose || !in || !rgb || in->cols <= 0 || in->rows <= 0) return KFX_ERR_INVALID_ARG;
+    const int W = in->cols, H = in->rows;
+    int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nt = std::min(nt, H);
+    auto rows = [&](int y0, int y1) {
+        for (int v = y0; v < y1; ++v)
+            for (int u = 0; u < W; ++u) {
+                const double dc[3] = {((double)u - in->cx) / in->fx, ((double)v - in->cy) / in->fy, 1.0};
+                const double* R = pose->R;
+                const double dw[3] = {R[0] * dc[0] + R[1] * dc[1] + R[2] * dc[2],
+                                      R[3] * dc[0] + R[4] * dc[1] + R[5] * dc[2],
+                                      R[6] * dc[0] + R[7] * dc[1] + R[8] * dc[2]};
+                int id = -2;
+                const double z = intersect(*sc, pose->t, dw, &id);
+                const double p[3] = {pose->t[0] + z * dw[0], pose->t[1] + z * dw[1], pose->t[2] + z * dw[2]};
+                rgb[v * W + u] = albedo(id, p);
             }
     };
     if (nt <= 1) {

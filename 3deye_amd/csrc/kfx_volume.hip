@@ -209,6 +209,55 @@ depth_bound_kernel(const float* __restrict__ scaled, int cols, int rows, float* 
     }
 }
 
+// Pixel footprint of the column segment z in [za, zb]: the box spanned by
+// the projections of its end voxels' centres, widened by one pixel, and the
+// smallest pyramid level l (texels of 2^l x 2^l pixels) whose 2 x 2 texels at
+// a0..a1, b0..b1 cover it.
+struct Footprint {
+    int l;
+    int u0, u1, v0, v1;  // pixels (level 0)
+    float gza, gzb;      // z of the end voxels' centres relative to the camera
+};
+enum { kSegmentOutside = 0, kSegmentUndecided = 1, kSegmentFootprint = 2 };
+
+// kSegmentOutside when no voxel centre of the segment can project into the
+// image (all behind the camera, or the footprint misses the image);
+// kSegmentUndecided when the test cannot tell (a centre too close to the
+// camera plane, or a footprint wider than the coarsest level).
+__device__ __forceinline__ int segment_footprint(const IntegrateParams& p, const PoseTerms& q, float cxy0,
+                                                 float cxy1, float cxy2, int za, int zb, Footprint& f) {
+    f.gza = ((float)za + 0.5f) * p.csz - q.tz;
+    f.gzb = ((float)zb + 0.5f) * p.csz - q.tz;
+    const float ax = cxy0 + q.r[2] * f.gza, ay = cxy1 + q.r[5] * f.gza, az = cxy2 + q.r[8] * f.gza;
+    const float bx = cxy0 + q.r[2] * f.gzb, by = cxy1 + q.r[5] * f.gzb, bz = cxy2 + q.r[8] * f.gzb;
+    if (az <= 0.0f && bz <= 0.0f) return kSegmentOutside;
+    if (!(az > 1e-4f && bz > 1e-4f)) return kSegmentUndecided;
+    const float ia = __builtin_amdgcn_rcpf(az), ib = __builtin_amdgcn_rcpf(bz);
+    const float ua = ax * p.fx * ia + p.cx, ub = bx * p.fx * ib + p.cx;
+    const float va = ay * p.fy * ia + p.cy, vb = by * p.fy * ib + p.cy;
+    const float fu0 = floorf(fminf(ua, ub)) - 1.0f, fu1 = ceilf(fmaxf(ua, ub)) + 1.0f;
+    const float fv0 = floorf(fminf(va, vb)) - 1.0f, fv1 = ceilf(fmaxf(va, vb)) + 1.0f;
+    if (fu1 < 0.0f || fv1 < 0.0f || fu0 > (float)(p.cols - 1) || fv0 > (float)(p.rows - 1)) return kSegmentOutside;
+    f.u0 = (int)fmaxf(fu0, 0.0f);
+    f.u1 = (int)fminf(fu1, (float)(p.cols - 1));
+    f.v0 = (int)fmaxf(fv0, 0.0f);
+    f.v1 = (int)fminf(fv1, (float)(p.rows - 1));
+    const int span = max(f.u1 - f.u0, f.v1 - f.v0) + 1;
+    f.l = span <= 1 ? 0 : 32 - __clz(span - 1);  // 2^l >= span: at most 2 x 2 texels
+    return f.l > kBoundLevels ? kSegmentUndecided : kSegmentFootprint;
+}
+
+// Offset of pyramid level l (1..kBoundLevels) without dynamic indexing of
+// the parameter block.
+__device__ __forceinline__ int level_offset(const IntegrateParams& p, int l) {
+    int off = p.bound_off[1];
+    off = l == 2 ? p.bound_off[2] : off;
+    off = l == 3 ? p.bound_off[3] : off;
+    off = l == 4 ? p.bound_off[4] : off;
+    off = l == 5 ? p.bound_off[5] : off;
+    return off;
+}
+
 // True when no voxel of the column segment z in [za, zb] can be updated:
 // every voxel centre is behind the camera, or the segment projects (with a
 // one-pixel margin) only onto pixels without depth, or it lies farther from
@@ -219,39 +268,21 @@ __device__ __forceinline__ bool segment_culled(const IntegrateParams& p, const P
                                                const float* __restrict__ scaled,
                                                float gx, float gy, float cxy0, float cxy1, float cxy2, int za,
                                                int zb) {
-    const float gza = ((float)za + 0.5f) * p.csz - q.tz, gzb = ((float)zb + 0.5f) * p.csz - q.tz;
-    const float ax = cxy0 + q.r[2] * gza, ay = cxy1 + q.r[5] * gza, az = cxy2 + q.r[8] * gza;
-    const float bx = cxy0 + q.r[2] * gzb, by = cxy1 + q.r[5] * gzb, bz = cxy2 + q.r[8] * gzb;
-    if (az <= 0.0f && bz <= 0.0f) return true;
-    if (!(az > 1e-4f && bz > 1e-4f)) return false;
-    const float ia = __builtin_amdgcn_rcpf(az), ib = __builtin_amdgcn_rcpf(bz);
-    const float ua = ax * p.fx * ia + p.cx, ub = bx * p.fx * ib + p.cx;
-    const float va = ay * p.fy * ia + p.cy, vb = by * p.fy * ib + p.cy;
-    const float fu0 = floorf(fminf(ua, ub)) - 1.0f, fu1 = ceilf(fmaxf(ua, ub)) + 1.0f;
-    const float fv0 = floorf(fminf(va, vb)) - 1.0f, fv1 = ceilf(fmaxf(va, vb)) + 1.0f;
-    if (fu1 < 0.0f || fv1 < 0.0f || fu0 > (float)(p.cols - 1) || fv0 > (float)(p.rows - 1)) return true;
-    const int u0 = (int)fmaxf(fu0, 0.0f), u1 = (int)fminf(fu1, (float)(p.cols - 1));
-    const int v0 = (int)fmaxf(fv0, 0.0f), v1 = (int)fminf(fv1, (float)(p.rows - 1));
-    const int span = max(u1 - u0, v1 - v0) + 1;
-    const int l = span <= 1 ? 0 : 32 - __clz(span - 1);  // 2^l >= span: at most 2 x 2 texels
-    if (l > kBoundLevels) return false;
+    Footprint f;
+    const int k = segment_footprint(p, q, cxy0, cxy1, cxy2, za, zb, f);
+    if (k != kSegmentFootprint) return k == kSegmentOutside;
     float maxd;
-    if (l == 0) {
-        maxd = scaled[v0 * p.cols + u0];
+    if (f.l == 0) {
+        maxd = scaled[f.v0 * p.cols + f.u0];
     } else {
-        const int lw = (p.cols + (1 << l) - 1) >> l;
-        int off = p.bound_off[1];
-        off = l == 2 ? p.bound_off[2] : off;
-        off = l == 3 ? p.bound_off[3] : off;
-        off = l == 4 ? p.bound_off[4] : off;
-        off = l == 5 ? p.bound_off[5] : off;
-        const float* b = p.bound + off;
-        const int a0 = u0 >> l, a1 = u1 >> l, b0 = v0 >> l, b1 = v1 >> l;
+        const int lw = (p.cols + (1 << f.l) - 1) >> f.l;
+        const float* b = p.bound + level_offset(p, f.l);
+        const int a0 = f.u0 >> f.l, a1 = f.u1 >> f.l, b0 = f.v0 >> f.l, b1 = f.v1 >> f.l;
         maxd = fmaxf(fmaxf(b[b0 * lw + a0], b[b0 * lw + a1]), fmaxf(b[b1 * lw + a0], b[b1 * lw + a1]));
     }
     if (maxd == 0.0f) return true;
     // Nearest point of the segment to the camera, bounded below.
-    const float dz = (gza <= 0.0f && gzb >= 0.0f) ? 0.0f : fminf(fabsf(gza), fabsf(gzb));
+    const float dz = (f.gza <= 0.0f && f.gzb >= 0.0f) ? 0.0f : fminf(fabsf(f.gza), fabsf(f.gzb));
     const float dmin = sqrtf(gx * gx + gy * gy + dz * dz);
     return dmin > (maxd + p.trunc) * 1.0001f + 1e-4f;
 }
@@ -385,6 +416,187 @@ integrate_count_kernel(const float* __restrict__ scaled, IntegrateParams p, unsi
     if ((threadIdx.x & 63) == 0 && n) atomicAdd(count, (unsigned long long)n);
 }
 
+// ---------------------------------------------------------------------------
+// Colour volume: Integrate_Colors (common/src/opencl_wrappers.cpp:181-214);
+// its kernel integrate_color_volume is absent from the reference, whose
+// arguments (RGB frame, raycast vertex map, colour volume, truncation,
+// maximum weight, R^-1, t, intrinsics, cell size, round buffer) are those of
+// the KinectFusion colour volume (PCL gpu/kinfu updateColorVolume).  Rule,
+// per voxel of the cyclic volume:
+//   voxel centre g = (i + 1/2) cell, camera point c = R^-1 (g - t), c.z > 0;
+//   pixel (u, v) = rint(f c.xy / c.z + c0) inside the image (the projection
+//   of the TSDF integration, so both pick the same pixel);
+//   raycast vertex p at that pixel valid (x not NaN) and |p - g| < trunc;
+//   then per channel rgb = rint_even((rgb * w + rgb_frame) / (w + 1)) in
+//   FP32 clamped to [0, 255], w = min(w + 1, max_weight).
+// The sweep is the TSDF integration's (z-major, frustum-clipped, batches of
+// kIntegrateBatch); batches are culled with a pyramid of the raycast
+// vertices' distances from the camera: an update needs
+// | |g - t| - |p - t| | <= |p - g| < trunc, so a segment whose distance
+// range misses [min |p - t| - trunc, max |p - t| + trunc] over its
+// footprint has nothing to update.
+
+// Colour-bound pyramid: one 32 x 32 pixel tile per workgroup; per texel of
+// levels 0..kBoundLevels the smallest and largest distance from the camera
+// to a valid raycast vertex under it (+inf and 0 when there is none).
+__global__ void __launch_bounds__(kBoundTile * kBoundTile)
+color_bound_kernel(const kfx_xyz* __restrict__ vmap, float2* __restrict__ bound, IntegrateParams p,
+                   const float* __restrict__ dpose, const int32_t* __restrict__ dskip) {
+    if (dskip != nullptr && *dskip != 0) return;  // uniform across the grid
+    float tx = p.tx, ty = p.ty, tz = p.tz;
+    if (dpose != nullptr) { tx = dpose[9]; ty = dpose[10]; tz = dpose[11]; }
+    __shared__ float2 t[kBoundTile * kBoundTile];
+    const int lx = threadIdx.x % kBoundTile, ly = threadIdx.x / kBoundTile;
+    const int u = blockIdx.x * kBoundTile + lx, v = blockIdx.y * kBoundTile + ly;
+    float2 m = make_float2(INFINITY, 0.0f);
+    if (u < p.cols && v < p.rows) {
+        const kfx_xyz q = vmap[v * p.cols + u];
+        if (!isnan(q.x)) {
+            const float dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+            m = make_float2(r, r);  // NaN when y or z is NaN: such a vertex never updates
+        }
+        bound[v * p.cols + u] = m;
+    }
+    t[threadIdx.x] = m;
+    __syncthreads();
+    for (int l = 1; l <= kBoundLevels; ++l) {
+        const int n = kBoundTile >> l;
+        float2 r = make_float2(INFINITY, 0.0f);
+        if (lx < n && ly < n) {
+            const int s = kBoundTile >> (l - 1);
+            const float2 a = t[(2 * ly) * s + 2 * lx], b = t[(2 * ly) * s + 2 * lx + 1];
+            const float2 c = t[(2 * ly + 1) * s + 2 * lx], d = t[(2 * ly + 1) * s + 2 * lx + 1];
+            r = make_float2(fminf(fminf(a.x, b.x), fminf(c.x, d.x)), fmaxf(fmaxf(a.y, b.y), fmaxf(c.y, d.y)));
+        }
+        __syncthreads();
+        if (lx < n && ly < n) {
+            t[ly * n + lx] = r;
+            const int lw = (p.cols + (1 << l) - 1) >> l, lh = (p.rows + (1 << l) - 1) >> l;
+            const int a = blockIdx.x * n + lx, b = blockIdx.y * n + ly;
+            if (a < lw && b < lh) bound[p.bound_off[l] + b * lw + a] = r;
+        }
+        __syncthreads();
+    }
+}
+
+// True when no voxel of the column segment z in [za, zb] can take a colour
+// (see the bound above).  Conservative: false whenever the test cannot
+// decide; a NaN bound never culls.
+__device__ __forceinline__ bool color_segment_culled(const IntegrateParams& p, const PoseTerms& q,
+                                                     const float2* __restrict__ cb, float gx, float gy, float cxy0,
+                                                     float cxy1, float cxy2, int za, int zb) {
+    Footprint f;
+    const int k = segment_footprint(p, q, cxy0, cxy1, cxy2, za, zb, f);
+    if (k != kSegmentFootprint) return k == kSegmentOutside;
+    const int lw = (p.cols + (1 << f.l) - 1) >> f.l;
+    const float2* b = cb + (f.l == 0 ? 0 : level_offset(p, f.l));
+    const int a0 = f.u0 >> f.l, a1 = f.u1 >> f.l, b0 = f.v0 >> f.l, b1 = f.v1 >> f.l;
+    const float2 e00 = b[b0 * lw + a0], e01 = b[b0 * lw + a1], e10 = b[b1 * lw + a0], e11 = b[b1 * lw + a1];
+    const float rmin = fminf(fminf(e00.x, e01.x), fminf(e10.x, e11.x));
+    const float rmax = fmaxf(fmaxf(e00.y, e01.y), fmaxf(e10.y, e11.y));
+    if (rmax == 0.0f) return true;  // no valid vertex under the footprint
+    const float part = gx * gx + gy * gy;
+    const float dz = (f.gza <= 0.0f && f.gzb >= 0.0f) ? 0.0f : fminf(fabsf(f.gza), fabsf(f.gzb));
+    const float dmin = sqrtf(part + dz * dz);
+    const float dmax = sqrtf(part + fmaxf(f.gza * f.gza, f.gzb * f.gzb));
+    const float slack = p.trunc * 1.0001f + 1e-4f;
+    return dmin > rmax * 1.0001f + slack || dmax < rmin * 0.9999f - slack;
+}
+
+template <bool kDevicePose>
+__global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
+integrate_colors_kernel(const kfx_color* __restrict__ rgb, const kfx_xyz* __restrict__ vmap,
+                        kfx_color* __restrict__ volume, const IntegrateParams p, const float2* __restrict__ cbound,
+                        int max_weight, const float* __restrict__ dpose, const int32_t* __restrict__ dskip) {
+    if (kDevicePose && dskip != nullptr && *dskip != 0) return;  // uniform across the grid
+    PoseTerms q;
+    if (kDevicePose) (void)load_device_pose(p, q, dpose, nullptr, 0.0f);
+    else q = host_terms(p);
+    const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
+    const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
+    const bool inside = x < p.nx && y < p.ny;
+    const int z_begin = blockIdx.z * kIntegrateZRun;
+    const int z_end = min(z_begin + kIntegrateZRun, p.nz);
+
+    const float vgx = ((float)x + 0.5f) * p.csx, vgy = ((float)y + 0.5f) * p.csy;
+    const float gx = vgx - q.tx, gy = vgy - q.ty;
+    const uint32_t col = inside ? (uint32_t)wrap_once(x + p.ox, p.nx) + (uint32_t)p.nx * (uint32_t)wrap_once(y + p.oy, p.ny) : 0u;
+    int lo, hi;
+    clip_column(p, q, gx, gy, z_begin, z_end, lo, hi);
+    if (!inside) { lo = z_end; hi = z_begin - 1; }
+    const int own_lo = lo, own_hi = hi;
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if (!inside) return;
+
+    const float cxy0 = q.r[0] * gx + q.r[1] * gy;
+    const float cxy1 = q.r[3] * gx + q.r[4] * gy;
+    const float cxy2 = q.r[6] * gx + q.r[7] * gy;
+    const float fcols = (float)p.cols, frows = (float)p.rows;
+    const uint32_t nz = (uint32_t)p.nz;
+    for (int z0 = lo; z0 <= hi; z0 += kIntegrateBatch) {
+        if (cbound != nullptr) {
+            const int za = max(z0, own_lo), zb = min(z0 + kIntegrateBatch - 1, own_hi);
+            const bool culled = za > zb || color_segment_culled(p, q, cbound, gx, gy, cxy0, cxy1, cxy2, za, zb);
+            if (__all(culled)) continue;
+        }
+        float px[kIntegrateBatch], py[kIntegrateBatch], pz[kIntegrateBatch];
+        int pix[kIntegrateBatch];
+        bool inimg[kIntegrateBatch];
+        // Phase 1 (no branches): projection and vertex gather.  Pixels
+        // outside the image read pixel 0 and are masked afterwards.
+#pragma unroll
+        for (int k = 0; k < kIntegrateBatch; ++k) {
+            const int z = z0 + k;
+            const float gz = ((float)z + 0.5f) * p.csz - q.tz;
+            const float camx = cxy0 + q.r[2] * gz;
+            const float camy = cxy1 + q.r[5] * gz;
+            const float camz = cxy2 + q.r[8] * gz;
+            const float inv_z = 1.0f / camz;
+            const float fu = rintf(camx * p.fx * inv_z + p.cx);
+            const float fv = rintf(camy * p.fy * inv_z + p.cy);
+            const bool in = (z <= hi) & (camz > 0.0f) & (fu >= 0.0f) & (fu < fcols) & (fv >= 0.0f) & (fv < frows);
+            pix[k] = (int)(in ? fv : 0.0f) * p.cols + (int)(in ? fu : 0.0f);
+            const kfx_xyz v = vmap[pix[k]];
+            px[k] = v.x; py[k] = v.y; pz[k] = v.z;
+            inimg[k] = in;
+        }
+        // Phase 2: the distance test, and the loads of what it selects.
+        bool upd[kIntegrateBatch];
+        kfx_color prev[kIntegrateBatch], cur[kIntegrateBatch];
+        uint32_t addr[kIntegrateBatch];
+#pragma unroll
+        for (int k = 0; k < kIntegrateBatch; ++k) {
+            const float vgz = ((float)(z0 + k) + 0.5f) * p.csz;
+            const float dx = px[k] - vgx, dy = py[k] - vgy, dz = pz[k] - vgz;
+            const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+            upd[k] = inimg[k] & !isnan(px[k]) & (dist < p.trunc);
+            uint32_t rz = (uint32_t)(z0 + k) + (uint32_t)p.oz;
+            rz = min(rz, rz - nz);
+            addr[k] = col + p.slice * rz;
+            if (upd[k]) {
+                prev[k] = volume[addr[k]];
+                cur[k] = rgb[pix[k]];
+            }
+        }
+        // Phase 3: running average per channel, write what changed.
+#pragma unroll
+        for (int k = 0; k < kIntegrateBatch; ++k) {
+            if (!upd[k]) continue;
+            const int w = prev[k].w;
+            const float den = (float)(w + 1);
+            kfx_color out;
+            out.r = (uint8_t)min(255, max(0, (int)rintf(((float)(prev[k].r * w) + (float)cur[k].r) / den)));
+            out.g = (uint8_t)min(255, max(0, (int)rintf(((float)(prev[k].g * w) + (float)cur[k].g) / den)));
+            out.b = (uint8_t)min(255, max(0, (int)rintf(((float)(prev[k].b * w) + (float)cur[k].b) / den)));
+            out.w = (uint8_t)min(max_weight, w + 1);
+            if (out.r != prev[k].r || out.g != prev[k].g || out.b != prev[k].b || out.w != prev[k].w)
+                volume[addr[k]] = out;
+        }
+    }
+}
+
 __global__ void cyclic_address_kernel(const int32_t* __restrict__ xyz, uint32_t* __restrict__ out,
                                       size_t n, kfx_round_buffer rb) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -491,26 +703,17 @@ static int integrate_bound(const float* depth_scaled, const kfx_intrinsics* intr
     return KFX_OK;
 }
 
-static int integrate_setup(const char* who, const uint16_t* depth, float* depth_scaled,
-                           const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
-                           const kfx_point_3f* translation, float truncation_distance,
-                           const kfx_point_3f* cell_size, const kfx_round_buffer* rb, IntegrateParams& p) {
+// Camera, volume and frustum terms of a column sweep (integration of depth or
+// colour): everything but the depth-bound pyramid.
+static int sweep_setup(const char* who, const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                       const kfx_point_3f* translation, float truncation_distance, const kfx_point_3f* cell_size,
+                       const kfx_round_buffer* rb, IntegrateParams& p) {
     KFX_REQUIRE_INIT();
-    KFX_REQUIRE(depth && depth_scaled && intr && rotation_inverse && translation && cell_size,
-                "%s: null pointer", who);
+    KFX_REQUIRE(intr && rotation_inverse && translation && cell_size, "%s: null pointer", who);
     int rc = check_round_buffer(rb, who);
     if (rc) return rc;
     KFX_REQUIRE(intr->cols > 0 && intr->rows > 0, "%s: empty image", who);
     KFX_REQUIRE(truncation_distance > 0.0f, "%s: truncation distance must be > 0", who);
-
-    {
-        ProfScope prof("scale_depth");
-        dim3 sblock(64, 4);
-        dim3 sgrid(divup(intr->cols, 64), divup(intr->rows, 4));
-        scale_depth_kernel<<<sgrid, sblock, 0, stream()>>>(depth, depth_scaled, *intr);
-        KFX_CHECK_LAUNCH();
-    }
-
     for (int r = 0; r < 3; ++r) {
         p.r[3 * r + 0] = rotation_inverse->data[r].x;
         p.r[3 * r + 1] = rotation_inverse->data[r].y;
@@ -548,6 +751,74 @@ static int integrate_setup(const char* who, const uint16_t* depth, float* depth_
         }
     }
     p.bound = nullptr;
+    return KFX_OK;
+}
+
+static int integrate_setup(const char* who, const uint16_t* depth, float* depth_scaled,
+                           const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                           const kfx_point_3f* translation, float truncation_distance,
+                           const kfx_point_3f* cell_size, const kfx_round_buffer* rb, IntegrateParams& p) {
+    KFX_REQUIRE(depth && depth_scaled, "%s: null pointer", who);
+    KFX_TRY(sweep_setup(who, intr, rotation_inverse, translation, truncation_distance, cell_size, rb, p));
+    ProfScope prof("scale_depth");
+    dim3 sblock(64, 4);
+    dim3 sgrid(divup(intr->cols, 64), divup(intr->rows, 4));
+    scale_depth_kernel<<<sgrid, sblock, 0, stream()>>>(depth, depth_scaled, *intr);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+// Library-owned colour-bound pyramid (float2 per texel, levels 0..5).
+static float2* g_cbound = nullptr;
+static size_t g_cbound_cap = 0;  // texels
+
+static size_t color_bound_layout(int cols, int rows, int off[kBoundLevels + 1]) {
+    size_t total = (size_t)cols * (size_t)rows;
+    off[0] = 0;
+    for (int l = 1; l <= kBoundLevels; ++l) {
+        off[l] = (int)total;
+        total += (size_t)((cols + (1 << l) - 1) >> l) * (size_t)((rows + (1 << l) - 1) >> l);
+    }
+    return total;
+}
+
+int colors_reserve(int cols, int rows) {
+    int off[kBoundLevels + 1];
+    const size_t need = color_bound_layout(cols, rows, off);
+    if (need <= g_cbound_cap) return KFX_OK;
+    if (g_cbound) KFX_CHECK_HIP(hipFree(g_cbound));
+    g_cbound = nullptr;
+    g_cbound_cap = 0;
+    KFX_CHECK_HIP(hipMalloc((void**)&g_cbound, need * sizeof(float2)));
+    g_cbound_cap = need;
+    return KFX_OK;
+}
+
+// Builds the colour-bound pyramid of vertex_map and returns it (null: run
+// unculled, inside a stream capture without a large enough reservation).
+static int colors_bound(const kfx_xyz* vertex_map, IntegrateParams& p, const float* dpose, const int32_t* dskip,
+                        const float2** out) {
+    *out = nullptr;
+    const size_t need = color_bound_layout(p.cols, p.rows, p.bound_off);
+    if (need > g_cbound_cap) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
+        if (cs != hipStreamCaptureStatusNone) return KFX_OK;
+        KFX_TRY(colors_reserve(p.cols, p.rows));
+    }
+    ProfScope prof("color_bound");
+    dim3 grid(divup(p.cols, kBoundTile), divup(p.rows, kBoundTile));
+    color_bound_kernel<<<grid, kBoundTile * kBoundTile, 0, stream()>>>(vertex_map, g_cbound, p, dpose, dskip);
+    KFX_CHECK_LAUNCH();
+    *out = g_cbound;
+    return KFX_OK;
+}
+
+static int colors_check(const char* who, kfx_color* color_volume, const kfx_color* rgb_frame,
+                        const kfx_xyz* vertex_map, int max_weight) {
+    KFX_REQUIRE(color_volume && rgb_frame && vertex_map, "%s: null pointer", who);
+    KFX_REQUIRE(max_weight >= 1 && max_weight <= 255, "%s: max_weight %d outside [1, 255] (8-bit weight)", who,
+                max_weight);
     return KFX_OK;
 }
 
@@ -620,6 +891,53 @@ int kfx_integrate_count_updates(const uint16_t* depth, float* depth_scaled, cons
     KFX_CHECK_HIP(e2);
     *count = host;
     return KFX_OK;
+}
+
+int kfx_integrate_colors(kfx_color* color_volume, const kfx_color* rgb_frame, const kfx_xyz* vertex_map,
+                         const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                         const kfx_point_3f* translation, float truncation_distance, int max_weight,
+                         const kfx_point_3f* cell_size, const kfx_round_buffer* rb) {
+    KFX_TRY(colors_check("kfx_integrate_colors", color_volume, rgb_frame, vertex_map, max_weight));
+    IntegrateParams p;
+    KFX_TRY(sweep_setup("kfx_integrate_colors", intr, rotation_inverse, translation, truncation_distance, cell_size,
+                        rb, p));
+    const float2* cb = nullptr;
+    KFX_TRY(colors_bound(vertex_map, p, nullptr, nullptr, &cb));
+    ProfScope prof("integrate_colors");
+    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
+    integrate_colors_kernel<false><<<grid, block, 0, stream()>>>(rgb_frame, vertex_map, color_volume, p, cb,
+                                                                 max_weight, nullptr, nullptr);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_integrate_colors_posed(kfx_color* color_volume, const kfx_color* rgb_frame, const kfx_xyz* vertex_map,
+                               const kfx_intrinsics* intr, const float* dev_pose, float truncation_distance,
+                               int max_weight, const kfx_point_3f* cell_size, const kfx_round_buffer* rb,
+                               const int32_t* dev_skip) {
+    KFX_TRY(colors_check("kfx_integrate_colors_posed", color_volume, rgb_frame, vertex_map, max_weight));
+    KFX_REQUIRE(dev_pose != nullptr, "kfx_integrate_colors_posed: null pose");
+    const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    const kfx_point_3f zero = {0, 0, 0};
+    IntegrateParams p;
+    KFX_TRY(sweep_setup("kfx_integrate_colors_posed", intr, &ident, &zero, truncation_distance, cell_size, rb, p));
+    const float2* cb = nullptr;
+    KFX_TRY(colors_bound(vertex_map, p, dev_pose, dev_skip, &cb));
+    ProfScope prof("integrate_colors");
+    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
+    integrate_colors_kernel<true><<<grid, block, 0, stream()>>>(rgb_frame, vertex_map, color_volume, p, cb,
+                                                                max_weight, dev_pose, dev_skip);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_init_colors(kfx_color* volume, const kfx_round_buffer* rb) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(volume && rb, "kfx_init_colors: null pointer");
+    KFX_REQUIRE(rb->buffer_length > 0, "kfx_init_colors: empty volume");
+    return kfx_fill_u8(volume, 0, (size_t)rb->buffer_length * sizeof(kfx_color));
 }
 
 }  // extern "C"

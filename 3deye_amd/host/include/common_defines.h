@@ -13,6 +13,7 @@
 #define DEFAULT_SIGMA_SPACE 4.5f
 #define DEFAULT_MAX_DISTANCE 300
 #define DEFAULT_TRUNCATION_DISTANCE 0.03f
+#define DEFAULT_MAX_WEIGHT 1 /* colour volume */
 #define DEFAULT_DISTANCE_THRESHOLD 0.1f
 #define DEFAULT_CAMERA_MOVEMENT_THRESHOLD 0.001f
 #define DEFAULT_VOLUME_SHIFTING_THRESHOLD 1.5f

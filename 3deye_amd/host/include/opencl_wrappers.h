@@ -45,6 +45,14 @@ void Transform_Maps(gpu_array_base* Vertex_Map_Src, gpu_array_base* Normal_Map_S
                     gpu_array_base* Normal_Map_Dst, matrix_3f& Rotation, vector_3f& Translation, intrinsics& Intr,
                     opencl_kernel* transform_maps_kernel);
 
+// Colour volume update (common/inc/opencl_wrappers.h:59-69); Vertex_Map is the
+// raycast of the current pose in volume coordinates, gRGB_Frame cols x rows
+// Color.  Max_Weight must lie in [1, 255].
+void Integrate_Colors(gpu_array_base* gTsdf_Color_Volume, gpu_array_base* gRGB_Frame, gpu_array_base* Vertex_Map,
+                      intrinsics& Intr, matrix_3f& Rotation_Inverse, vector_3f& Translation,
+                      float Truncation_Distance, int Max_Weight, point_3f Cell_Size, round_buffer* Round_Buffer,
+                      opencl_kernel* integrate_colors_kernel);
+
 // Trunc_Dist is accepted and ignored: the reference binds its bits to the
 // kernel's integer max-distance argument, where it never truncates
 // (opencl_wrappers.cpp:231 vs compute_bilateral_filtering.cl:13).

@@ -154,6 +154,19 @@ void Transform_Maps(gpu_array_base* Vertex_Map_Src, gpu_array_base* Normal_Map_S
           "Transform_Maps");
 }
 
+void Integrate_Colors(gpu_array_base* gTsdf_Color_Volume, gpu_array_base* gRGB_Frame, gpu_array_base* Vertex_Map,
+                      intrinsics& Intr, matrix_3f& Rotation_Inverse, vector_3f& Translation,
+                      float Truncation_Distance, int Max_Weight, point_3f Cell_Size, round_buffer* Round_Buffer,
+                      opencl_kernel*) {
+    const size_t n = npix(Intr);
+    check(kfx_integrate_colors(dev<kfx_color>(gTsdf_Color_Volume, (size_t)Round_Buffer->buffer_length,
+                                              "Integrate_Colors"),
+                               dev<kfx_color>(gRGB_Frame, n, "Integrate_Colors"),
+                               dev<kfx_xyz>(Vertex_Map, n, "Integrate_Colors"), K(Intr), M(Rotation_Inverse),
+                               P(Translation), Truncation_Distance, Max_Weight, P(Cell_Size), RB(Round_Buffer)),
+          "Integrate_Colors");
+}
+
 void Apply_Billateral_Filtration(gpu_array_base* gDepth, const intrinsics& Intr, float, opencl_kernel*,
                                  gpu_array_base* gDepth_Filtered) {
     const size_t n = npix(Intr);

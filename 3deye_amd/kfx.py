@@ -98,7 +98,8 @@ class PipelineConfig(C.Structure):
                 ("use_graph", C.c_int32), ("denoiser", C.c_int32), ("fallback_rms_mm", C.c_float),
                 ("shift_threshold", C.c_float), ("integrate_threshold", C.c_float),
                 ("converge_rotation", C.c_double), ("converge_translation", C.c_double),
-                ("extract_on_shift", C.c_int32), ("icp_sum_mode", C.c_int32), ("shift_target_distance", C.c_double)]
+                ("extract_on_shift", C.c_int32), ("icp_sum_mode", C.c_int32), ("shift_target_distance", C.c_double),
+                ("colors", C.c_int32), ("color_max_weight", C.c_int32), ("extract_records", C.c_int32)]
 
 
 class FrameInfo(C.Structure):
@@ -113,7 +114,8 @@ class PipelineView(C.Structure):
                 ("vmap_model", C.c_void_p), ("nmap_model", C.c_void_p), ("vmap_current", C.c_void_p),
                 ("nmap_current", C.c_void_p), ("pose_volume", Pose), ("frames", C.c_int64), ("shifts", C.c_int32),
                 ("fallbacks", C.c_int32), ("extracted_points", C.c_uint64), ("last_points", C.c_void_p),
-                ("last_colors", C.c_void_p)]
+                ("last_colors", C.c_void_p), ("color_volume", C.c_void_p), ("rgb", C.c_void_p),
+                ("last_records", C.c_void_p)]
 
 
 PLAIN, FUSED = 0, 1
@@ -123,6 +125,8 @@ assert C.sizeof(Intrinsics) == 24 and C.sizeof(RoundBuffer) == 72 and C.sizeof(M
 
 TSDF_DTYPE = np.dtype([("value", "<i2"), ("weight", "<i2")])
 XYZ_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4")])
+COLOR_DTYPE = np.dtype([("r", "u1"), ("g", "u1"), ("b", "u1"), ("w", "u1")])
+SLICE_RECORD_DTYPE = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("tsdf", "<i2"), ("weight", "<i2")])
 
 
 def point3f(v):
@@ -208,6 +212,14 @@ _PROTOS = {
                                       C.POINTER(Point3f), C.POINTER(RoundBuffer), _P]),
     "kfx_raycast_posed": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), _P, C.c_float, C.POINTER(Intrinsics),
                                C.POINTER(Point3f), _P]),
+    "kfx_raycast_count_samples": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), C.POINTER(Point3f), C.POINTER(Matrix3f),
+                                       C.c_float, C.POINTER(Intrinsics), C.POINTER(Point3f),
+                                       C.POINTER(C.c_uint64)]),
+    "kfx_integrate_colors": (_I, [_P, _P, _P, C.POINTER(Intrinsics), C.POINTER(Matrix3f), C.POINTER(Point3f),
+                                  C.c_float, _I, C.POINTER(Point3f), C.POINTER(RoundBuffer)]),
+    "kfx_integrate_colors_posed": (_I, [_P, _P, _P, C.POINTER(Intrinsics), _P, C.c_float, _I, C.POINTER(Point3f),
+                                        C.POINTER(RoundBuffer), _P]),
+    "kfx_init_colors": (_I, [_P, C.POINTER(RoundBuffer)]),
     "kfx_profile_enable": (_I, [_I]),
     "kfx_profile_reset": (_I, []),
     "kfx_profile_query": (_I, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -241,10 +253,13 @@ _PROTOS = {
     "kfx_slicer_destroy": (_I, [_P]),
     "kfx_extract_slice": (_I, [_P, C.POINTER(Point3i), C.POINTER(Point3i), C.POINTER(RoundBuffer), _P, _P, _P, _P,
                                C.c_uint64, C.POINTER(C.c_uint64)]),
+    "kfx_extract_slice_records": (_I, [_P, C.POINTER(Point3i), C.POINTER(Point3i), C.POINTER(RoundBuffer), _P, _P,
+                                       _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "kfx_pipeline_default_config": (_I, [C.POINTER(PipelineConfig)]),
     "kfx_pipeline_create": (_I, [C.POINTER(PipelineConfig), C.POINTER(Pose), C.POINTER(C.c_double), C.POINTER(_P)]),
     "kfx_pipeline_destroy": (_I, [_P]),
     "kfx_pipeline_process": (_I, [_P, _P, C.POINTER(FrameInfo)]),
+    "kfx_pipeline_process_rgbd": (_I, [_P, _P, _P, C.POINTER(FrameInfo)]),
     "kfx_pipeline_state": (_I, [_P, C.POINTER(PipelineView)]),
     "kfx_pipeline_set_denoiser": (_I, [_P, _P]),
     "kfx_denoiser_param_count": (_I, []),
@@ -268,6 +283,7 @@ _PROTOS = {
     "kfx_synth_orbit": (_I, [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.c_double, _I,
                              C.POINTER(Pose)]),
     "kfx_synth_walk": (_I, [C.c_uint64, C.POINTER(C.c_double), C.c_double, C.c_double, _I, C.POINTER(Pose)]),
+    "kfx_synth_render_rgb": (_I, [C.POINTER(SynthScene), C.POINTER(Pose), C.POINTER(Intrinsics), _I, _P]),
     "kfx_synth_render": (_I, [C.POINTER(SynthScene), C.POINTER(Pose), C.POINTER(Intrinsics), C.c_uint64, _I, _I, _I,
                               _P, _P]),
 }
@@ -363,6 +379,14 @@ class Buffer:
             pass
 
 
+def download(dev_ptr, dtype, shape):
+    """Copy device memory the library owns (e.g. a pipeline view's volume) to
+    a new host array."""
+    out = np.empty(shape, dtype=dtype)
+    check(lib.kfx_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), C.c_void_p(dev_ptr), out.nbytes))
+    return out
+
+
 class Timer:
     """HIP-event timer on the library stream."""
 
@@ -429,6 +453,14 @@ def render(scene, pose, intr, seed, frame, noise=True, threads=0, want_clean=Fal
     return (depth, clean) if want_clean else depth
 
 
+def render_rgb(scene, pose, intr, threads=0):
+    """Noise-free colour frame of the view (rows x cols COLOR_DTYPE)."""
+    rgb = np.zeros((intr.rows, intr.cols), dtype=COLOR_DTYPE)
+    check(lib.kfx_synth_render_rgb(C.byref(scene), C.byref(pose), C.byref(intr), threads,
+                                   rgb.ctypes.data_as(C.c_void_p)))
+    return rgb
+
+
 def kinect_intrinsics(cols=640, rows=480, f=525.0):
     return Intrinsics(f, f, (cols - 1) / 2.0, (rows - 1) / 2.0, cols, rows)
 
@@ -451,6 +483,11 @@ class Pipeline:
     def process(self, depth_dev_ptr):
         info = FrameInfo()
         check(lib.kfx_pipeline_process(self.h, depth_dev_ptr, C.byref(info)))
+        return info
+
+    def process_rgbd(self, depth_dev_ptr, rgb_dev_ptr):
+        info = FrameInfo()
+        check(lib.kfx_pipeline_process_rgbd(self.h, depth_dev_ptr, rgb_dev_ptr, C.byref(info)))
         return info
 
     def view(self):

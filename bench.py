@@ -105,6 +105,68 @@ def count_updates(intr, ddepth, frame_bytes, infos, first, rb):
     return total
 
 
+def count_raycast_samples(cfg, poses, origin, ddepth, frame_bytes, first, n):
+    """S (the raycast algorithm's voxel reads) of frames [first, n) of the
+    plain pipeline: an untimed replay of the same deterministic pipeline,
+    recounting each frame's raycast on its volume at its pose.  Frames
+    followed by a shift are left out (their volume has moved on)."""
+    intr = cfg.intr
+    pipe = kfx.Pipeline(cfg, poses[0], origin)
+    vm, nm = kfx.Buffer(W * H * 12), kfx.Buffer(W * H * 12)
+    total, frames = 0, 0
+    for i in range(n):
+        info = pipe.process(C.c_void_p(ddepth.ptr.value + i * frame_bytes))
+        if i < first or info.shifted:
+            continue
+        v = pipe.view()
+        R, t = info.pose_volume.matrices()
+        ns = C.c_uint64()
+        kfx.call("kfx_raycast_count_samples", C.c_void_p(v.volume), vm.ptr, nm.ptr, C.byref(v.round_buffer),
+                 C.byref(kfx.point3f(t)), C.byref(kfx.matrix3f(R)), TRUNC, C.byref(intr), C.byref(v.cell_size),
+                 C.byref(ns))
+        total += ns.value
+        frames += 1
+    pipe.close()
+    return total / max(frames, 1), frames
+
+
+def kernel_table(n_upd_per_launch, samples_per_launch):
+    """Per kernel of the plain pipeline: launches and time from the HIP
+    events of the timed run, algorithmic bytes per launch (the formulas of
+    the roofline model: bilateral 4 B/px; pyramid level i 2 (WH/4^(i-1)) +
+    2 (WH/4^i); vertex map 14 B/px; normal map 24 B/px; resize 12 B per input
+    and output pixel; ICP sums per iteration 48 B/px + 108 B per workgroup
+    partial, their reduction 108 B per partial; scale_depth 6 B/px;
+    integrate 8 N_upd + 2 WH; raycast 4 S + 24 WH), and the achieved GB/s
+    against the HBM peak."""
+    lv = [(W >> l, H >> l) for l in range(3)]
+    px = [w * h for w, h in lv]
+    nblk = [((w + 31) // 32) * ((h + 7) // 8) for w, h in lv]
+    iters = [10, 5, 4]
+    per_frame = {
+        "bilateral": (1, 4 * px[0]),
+        "pyr_down": (2, sum(2 * px[i - 1] + 2 * px[i] for i in (1, 2))),
+        "vertex_map": (3, sum(14 * p for p in px)),
+        "normal_map": (3, sum(24 * p for p in px)),
+        "resize_map": (4, sum(2 * (12 * px[l - 1] + 12 * px[l]) for l in (1, 2))),
+        "estimate_combined": (19, sum(iters[l] * (48 * px[l] + 108 * nblk[l]) for l in range(3))),
+        "transform_estimation": (19, sum(iters[l] * (108 * nblk[l] + 108) for l in range(3))),
+        "scale_depth": (1, 6 * px[0]),
+        "integrate": (1, 8 * n_upd_per_launch + 2 * px[0]),
+        "raycast": (1, 4 * samples_per_launch + 24 * px[0]),
+    }
+    out = {}
+    for name, (per, nbytes) in per_frame.items():
+        ms, cnt = kfx.profile_query(name)
+        if not cnt:
+            continue
+        b = nbytes / per
+        avg_s = ms / 1e3 / cnt
+        out[name] = {"launches": cnt, "avg_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": round(b),
+                     "GBs": round(b / avg_s / 1e9, 1), "frac_hbm_peak": round(b / avg_s / 1e9 / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def cpu_baseline(intr, depths, poses, origin, frames=12):
     """The oracle's single-precision restatement of the per-frame pipeline
     (bilateral, pyramid, vertex/normal maps, 19 ICP iterations, integrate,
@@ -250,14 +312,9 @@ def main():
         ate_p = kfx.ate_rmse([i.pose_world for i in inf_p], poses[:n])
         plain = {"frames_per_s": args.steps / el_p, "ms_per_frame": 1e3 * el_p / args.steps, "ate_m": ate_p}
     int_ms, int_n = kfx.profile_query("integrate")
-    kern = {}
-    for name in ("bilateral", "pyr_down", "vertex_map", "normal_map", "estimate_combined", "transform_estimation",
-                 "integrate", "raycast", "resize_map", "slice_count", "slice_emit", "clean_tsdf_slice"):
-        ms, cnt = kfx.profile_query(name)
-        if cnt:
-            kern[name] = round(1e3 * ms / cnt, 2)
 
     roofline = None
+    kern = None
     if int_n:
         rb = kfx.round_buffer(N_VOX, VOL_M)
         n_upd = count_updates(intr, ddepth, fb, inf_p, args.warmup, rb)
@@ -273,6 +330,10 @@ def main():
                     "avg_launch_us": avg_s * 1e6,
                     "alg_bytes_per_launch": alg_bytes, "n_upd_per_launch": n_upd / launches,
                     "full_sweep_equiv_GBs": 8.0 * N_VOX ** 3 / avg_s / 1e9}
+        s_avg, s_frames = count_raycast_samples(cfg_p, poses, origin, ddepth, fb, args.warmup, n)
+        kern = kernel_table(n_upd / launches, s_avg)
+        kern["raycast"]["samples_per_launch"] = round(s_avg)
+        kern["raycast"]["samples_counted_frames"] = s_frames
 
     base = None if args.no_cpu_baseline else cpu_baseline(intr, depths, poses, origin)
     head = learned if learned is not None else fused
@@ -302,7 +363,7 @@ def main():
         "speedup_vs_plain": (head["frames_per_s"] / plain["frames_per_s"]) if plain else None,
         "ate_le_plain": (head["ate_m"] <= plain["ate_m"]) if plain else None,
         "roofline": roofline,
-        "kernels_avg_us_plain": kern,
+        "kernels_plain": kern,
         "cpu_baseline": base,
     }
     print(json.dumps(out))

@@ -133,6 +133,41 @@ int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsd
                              float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
                              const kfx_round_buffer* rb, const int32_t* dev_skip);
 
+/* kfx_raycast that also returns S, the voxel reads its algorithm makes (first
+ * sample, every march sample inside the volume, 8 per in-bounds trilinear
+ * evaluation): the raycast's algorithmic bytes are 4 S + 24 W H.  Same
+ * outputs as kfx_raycast; blocking; for measurement (no reference
+ * counterpart). */
+int kfx_raycast_count_samples(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                              const kfx_round_buffer* rb, const kfx_point_3f* translation,
+                              const kfx_matrix_3f* rotation, float truncation_distance, const kfx_intrinsics* intr,
+                              const kfx_point_3f* cell_size, uint64_t* samples);
+
+/* Colour volume (Integrate_Colors, common/src/opencl_wrappers.cpp:181-214,
+ * declared in common/inc/opencl_wrappers.h:59-69; its kernel
+ * integrate_color_volume is absent from the reference).  Every voxel of the
+ * cyclic colour volume whose centre g projects (camera c = R^-1 (g - t),
+ * pixel rint(f c/c.z + c0), as in kfx_integrate_tsdf) onto a pixel whose
+ * raycast vertex p (volume coordinates, NaN = none) lies closer than
+ * truncation_distance to g takes the running average of the frame's colour:
+ * rgb = round_half_even((rgb * w + rgb_frame) / (w + 1)) per channel in FP32,
+ * w = min(w + 1, max_weight), max_weight in [1, 255] (the reference's
+ * default is 1, common/inc/common_defines.h:26).  rgb_frame: cols x rows
+ * kfx_color, the w byte ignored (common/src/Capture.cpp:115-123). */
+int kfx_integrate_colors(kfx_color* color_volume, const kfx_color* rgb_frame, const kfx_xyz* vertex_map,
+                         const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                         const kfx_point_3f* translation, float truncation_distance, int max_weight,
+                         const kfx_point_3f* cell_size, const kfx_round_buffer* rb);
+/* kfx_integrate_colors with the pose read from device memory (12 floats, R
+ * camera -> volume then t), skipped when dev_skip is given and non-zero at
+ * run time (no reference counterpart). */
+int kfx_integrate_colors_posed(kfx_color* color_volume, const kfx_color* rgb_frame, const kfx_xyz* vertex_map,
+                               const kfx_intrinsics* intr, const float* dev_pose, float truncation_distance,
+                               int max_weight, const kfx_point_3f* cell_size, const kfx_round_buffer* rb,
+                               const int32_t* dev_skip);
+/* Zero colour volume (colour 0, weight 0: "not set" for the slice scan). */
+int kfx_init_colors(kfx_color* volume, const kfx_round_buffer* rb);
+
 /* Raycast (common/inc/opencl_wrappers.h:23-33, GPU/opencl/raycast.cl:182-405):
  * per-pixel march through the cyclic TSDF; writes the surface vertex and the
  * normal (NaN where there is none) in volume coordinates, cols*rows XYZ each. */
@@ -291,6 +326,16 @@ int kfx_slicer_destroy(kfx_slicer* slicer);
 int kfx_extract_slice(kfx_slicer* slicer, const kfx_point_3i* min_bounds, const kfx_point_3i* max_bounds,
                       const kfx_round_buffer* rb, const kfx_tsdf* tsdf_volume, const kfx_color* color_volume,
                       kfx_xyzi* points, kfx_color* colors, uint64_t capacity, uint64_t* count);
+/* The same points as kfx_extract_slice, in the same order, as records: the
+ * point's voxel in global grid coordinates (local + round_buffer
+ * origin_GRID_global) with its stored tsdf value and weight, the
+ * (x, y, z, tsdf, weight) form of a world-model slice (no reference
+ * counterpart: the reference emits grid positions with w = 1).  colors may
+ * be NULL. */
+int kfx_extract_slice_records(kfx_slicer* slicer, const kfx_point_3i* min_bounds, const kfx_point_3i* max_bounds,
+                              const kfx_round_buffer* rb, const kfx_tsdf* tsdf_volume,
+                              const kfx_color* color_volume, kfx_slice_record* records, kfx_color* colors,
+                              uint64_t capacity, uint64_t* count);
 
 /* ------------------------------------------------------------------------
  * Learned depth denoiser (no reference counterpart; it stands in for the
@@ -367,6 +412,9 @@ typedef struct kfx_pipeline_config {
     int32_t extract_on_shift;     /* extract the leaving slab before clearing it */
     int32_t icp_sum_mode;         /* fused mode: KFX_ICP_SUM_* workgroup sums (0: FP32 Kahan) */
     double shift_target_distance; /* target point = camera + this along the optical axis (m); 0: camera centre */
+    int32_t colors;               /* 1: keep a colour volume; frames then come with RGB (kfx_pipeline_process_rgbd) */
+    int32_t color_max_weight;     /* colour running-average weight cap, 1..255 (reference default 1) */
+    int32_t extract_records;      /* 1: shifts extract kfx_slice_record (x, y, z, tsdf, weight); 0: XYZI points */
 } kfx_pipeline_config;
 
 typedef struct kfx_frame_info {
@@ -397,8 +445,11 @@ typedef struct kfx_pipeline_view {
     int32_t shifts;
     int32_t fallbacks;            /* frames that fell back from the denoiser to the bilateral filter */
     uint64_t extracted_points;    /* total over all shifts */
-    kfx_xyzi* last_points;        /* last shift's extracted points (device) */
+    kfx_xyzi* last_points;        /* last shift's extracted points (device; NULL with extract_records) */
     kfx_color* last_colors;
+    kfx_color* color_volume;      /* colour volume (device), NULL unless cfg.colors */
+    kfx_color* rgb;               /* last input colour frame (device), NULL unless cfg.colors */
+    kfx_slice_record* last_records; /* last shift's records (device), NULL unless cfg.extract_records */
 } kfx_pipeline_view;
 
 typedef struct kfx_pipeline kfx_pipeline;
@@ -413,6 +464,13 @@ int kfx_pipeline_destroy(kfx_pipeline* pipeline);
  * when the frame's pose is known on the host (one synchronisation per frame
  * in fused mode; one per step in plain mode). */
 int kfx_pipeline_process(kfx_pipeline* pipeline, const uint16_t* depth, kfx_frame_info* info);
+/* Process one depth frame with its registered colour frame (device pointer,
+ * cols x rows kfx_color): as kfx_pipeline_process, then the colour volume
+ * takes the frame's colours at the raycast of the new pose (the reference's
+ * Integrate_Colors step).  Needs cfg.colors; kfx_pipeline_process then
+ * fails for that pipeline. */
+int kfx_pipeline_process_rgbd(kfx_pipeline* pipeline, const uint16_t* depth, const kfx_color* rgb,
+                              kfx_frame_info* info);
 int kfx_pipeline_state(kfx_pipeline* pipeline, kfx_pipeline_view* view);
 /* Attach a trained denoiser (fused mode; used when cfg.denoiser is set).  The
  * pipeline does not own it. */
@@ -462,6 +520,11 @@ int kfx_synth_walk(uint64_t seed, const double start[3], double step, double pit
 int kfx_synth_render(const kfx_synth_scene* scene, const kfx_pose* pose, const kfx_intrinsics* intr,
                      uint64_t seed, int frame, int flags, int threads, uint16_t* depth_mm, float* clean_m);
 #define KFX_SYNTH_NOISE 1
+/* Colour frame of the same view (noise-free): per pixel the albedo of the
+ * surface hit (objects: a fixed colour each; ground: 0.5 m checkerboard of two
+ * greys; no hit: black), w = 1. */
+int kfx_synth_render_rgb(const kfx_synth_scene* scene, const kfx_pose* pose, const kfx_intrinsics* intr,
+                         int threads, kfx_color* rgb);
 
 /* Trajectory text file in the reference's format (write_isometry_to_file,
  * common/src/string_utils.cpp:26-74): per pose its index, the three rows of

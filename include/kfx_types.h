@@ -71,6 +71,13 @@ typedef struct kfx_xyz { float x, y, z; } kfx_xyz;
 /* Point of an extracted slice: local grid coordinates as floats, w = 1. */
 typedef struct kfx_xyzi { float x, y, z, w; } kfx_xyzi;
 
+/* Record of an extracted slice point: its voxel's global grid coordinates
+ * (local coordinates plus the cube's global voxel offset, origin_GRID_global,
+ * so records of successive shifts share one grid; world position = the
+ * volume's world origin at the first frame + (g + 1/2) * cell) and the
+ * voxel's stored tsdf value and weight. */
+typedef struct kfx_slice_record { int32_t x, y, z; int16_t tsdf, weight; } kfx_slice_record;
+
 /* Constants of the reference (common/inc/common_defines.h). */
 #define KFX_MAX_DEPTH_VALUE 32767          /* TSDF quantisation divisor */
 #define KFX_MAX_TSDF_WEIGHT 128            /* MAX_TSDF_WEIGHT (1<<7) */
@@ -79,6 +86,7 @@ typedef struct kfx_xyzi { float x, y, z, w; } kfx_xyzi;
 #define KFX_DEFAULT_TRUNCATION 0.03f       /* metres */
 #define KFX_DEFAULT_DIST_THRESHOLD 0.1f    /* ICP correspondence distance, metres */
 #define KFX_DEFAULT_SHIFT_THRESHOLD 1.5f   /* cyclic shift trigger, metres */
+#define KFX_DEFAULT_COLOR_MAX_WEIGHT 1     /* DEFAULT_MAX_WEIGHT, colour volume */
 #define KFX_ICP_LEVELS 3
 #define KFX_ICP_SUMS 27                    /* upper triangle of [A | b], 6x7 */
 
@@ -97,6 +105,7 @@ static_assert(sizeof(kfx_tsdf) == 4, "Tsdf layout");
 static_assert(sizeof(kfx_color) == 4, "Color layout");
 static_assert(sizeof(kfx_xyz) == 12, "XYZ layout");
 static_assert(sizeof(kfx_xyzi) == 16, "XYZI layout");
+static_assert(sizeof(kfx_slice_record) == 16, "slice record layout");
 #endif
 
 #endif /* KFX_TYPES_H */

@@ -54,8 +54,18 @@ uint64_t orc_integrate_f64(const double* scaled, kfx_tsdf* volume, const kfx_int
                            const double* rotation_inverse9, const double* translation3,
                            double trunc, const double* cell3, const kfx_round_buffer* rb);
 
-/* raycast.cl:182-405. */
-void orc_raycast_f32(const kfx_tsdf* volume, kfx_xyz* vmap, kfx_xyz* nmap, const kfx_round_buffer* rb,
+/* Colour volume update (Integrate_Colors, call site opencl_wrappers.cpp:181-214;
+ * kernel absent): the KinectFusion colour-volume rule.  Returns the number of
+ * voxels touched. */
+uint64_t orc_integrate_colors(kfx_color* volume, const kfx_color* rgb, const kfx_xyz* vmap,
+                              const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                              const kfx_point_3f* translation, float trunc, int max_weight,
+                              const kfx_point_3f* cell, const kfx_round_buffer* rb);
+
+/* raycast.cl:182-405.  Returns S, the voxel reads of the algorithm (first
+ * sample, march samples inside the volume, 8 per in-bounds trilinear
+ * evaluation). */
+uint64_t orc_raycast_f32(const kfx_tsdf* volume, kfx_xyz* vmap, kfx_xyz* nmap, const kfx_round_buffer* rb,
                      const kfx_point_3f* translation, const kfx_matrix_3f* rotation, float trunc,
                      const kfx_intrinsics* intr, const kfx_point_3f* cell);
 /* raycast.cl:95-145 (interpolateTrilineary), for direct checks. */

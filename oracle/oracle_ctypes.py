@@ -36,7 +36,8 @@ _PROTOS = {
     "orc_scale_depth_f64": (None, [_P, _P, IN]),
     "orc_integrate_f32": (C.c_uint64, [_P, _P, IN, M3, P3, C.c_float, P3, RB]),
     "orc_integrate_f64": (C.c_uint64, [_P, _P, IN, PD, PD, C.c_double, PD, RB]),
-    "orc_raycast_f32": (None, [_P, _P, _P, RB, P3, M3, C.c_float, IN, P3]),
+    "orc_integrate_colors": (C.c_uint64, [_P, _P, _P, IN, M3, P3, C.c_float, _I, P3, RB]),
+    "orc_raycast_f32": (C.c_uint64, [_P, _P, _P, RB, P3, M3, C.c_float, IN, P3]),
     "orc_interpolate_trilinear_f32": (C.c_float, [_P, C.c_float, C.c_float, C.c_float, P3, RB]),
     "orc_bilateral": (None, [_P, _P, _I, _I, C.c_float, C.c_float, C.c_uint32]),
     "orc_pyr_down": (None, [_P, _I, _I, _P, C.c_float]),
@@ -115,13 +116,27 @@ def integrate_f64(vol, depth, intr, R_inv, t, trunc, rb):
                                  float(trunc), cs.ctypes.data_as(PD), C.byref(rb))
 
 
-def raycast_f32(vol, rb, t, R, trunc, intr):
+def integrate_colors(cvol, rgb, vmap, intr, R_inv, t, trunc, max_weight, rb):
+    """In-place update of `cvol` (COLOR_DTYPE array of buffer_length) from
+    `rgb` (rows x cols COLOR_DTYPE) and the raycast `vmap` (rows x cols x 3
+    float32, volume coordinates); returns the number of voxels touched."""
+    cs = kfx.cell_size(rb)
+    rgb = np.ascontiguousarray(rgb)
+    vmap = np.ascontiguousarray(vmap, np.float32)
+    assert cvol.dtype == COLOR_DTYPE and cvol.size == rb.buffer_length
+    assert rgb.dtype == COLOR_DTYPE and rgb.size == intr.rows * intr.cols and vmap.size == 3 * rgb.size
+    return lib.orc_integrate_colors(ptr(cvol), ptr(rgb), ptr(vmap), C.byref(intr), C.byref(kfx.matrix3f(R_inv)),
+                                    C.byref(kfx.point3f(t)), float(trunc), int(max_weight), C.byref(cs),
+                                    C.byref(rb))
+
+
+def raycast_f32(vol, rb, t, R, trunc, intr, want_samples=False):
     vmap = np.zeros((intr.rows, intr.cols, 3), np.float32)
     nmap = np.zeros((intr.rows, intr.cols, 3), np.float32)
     cs = kfx.cell_size(rb)
-    lib.orc_raycast_f32(ptr(vol), ptr(vmap), ptr(nmap), C.byref(rb), C.byref(kfx.point3f(t)),
-                        C.byref(kfx.matrix3f(R)), float(trunc), C.byref(intr), C.byref(cs))
-    return vmap, nmap
+    samples = lib.orc_raycast_f32(ptr(vol), ptr(vmap), ptr(nmap), C.byref(rb), C.byref(kfx.point3f(t)),
+                                  C.byref(kfx.matrix3f(R)), float(trunc), C.byref(intr), C.byref(cs))
+    return (vmap, nmap, samples) if want_samples else (vmap, nmap)
 
 
 NO_MAX_DISTANCE = 0xFFFFFFFF

@@ -155,6 +155,63 @@ extern "C" uint64_t orc_integrate_f64(const double* scaled, kfx_tsdf* vol, const
     return touched;
 }
 
+// Colour volume update (Integrate_Colors, call site
+// opencl_wrappers.cpp:181-214; kernel integrate_color_volume absent).  The
+// published KinectFusion colour-volume rule (PCL gpu/kinfu,
+// color_volume.cu, updateColorVolume) with the projection of the TSDF
+// update above:
+//   g = (i + 1/2) * cell;  c = R^-1 (g - t);  skip unless c.z > 0
+//   (u, v) = rint(f * c.xy / c.z + c0);  skip unless inside the image
+//   p = vmap(u, v);  skip if p.x is NaN
+//   skip unless |p - g| < trunc
+//   per channel: rint((rgb * w + rgb_frame) / (w + 1)) in FP32, clamped to
+//   [0, 255];  w = min(w + 1, max_weight).
+// Returns the number of voxels the rule touched.
+extern "C" uint64_t orc_integrate_colors(kfx_color* vol, const kfx_color* rgb, const kfx_xyz* vmap,
+                                         const kfx_intrinsics* in, const kfx_matrix_3f* Ri, const kfx_point_3f* t,
+                                         float trunc, int max_weight, const kfx_point_3f* cs,
+                                         const kfx_round_buffer* rb) {
+    const int nx = rb->voxels_size.x, ny = rb->voxels_size.y, nz = rb->voxels_size.z;
+    const float r[9] = {Ri->data[0].x, Ri->data[0].y, Ri->data[0].z, Ri->data[1].x, Ri->data[1].y,
+                        Ri->data[1].z, Ri->data[2].x, Ri->data[2].y, Ri->data[2].z};
+    uint64_t touched = 0;
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : touched) num_threads(g_threads)
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            for (int z = 0; z < nz; ++z) {
+                const float vx = ((float)x + 0.5f) * cs->x, vy = ((float)y + 0.5f) * cs->y,
+                            vz = ((float)z + 0.5f) * cs->z;
+                const float gx = vx - t->x, gy = vy - t->y, gz = vz - t->z;
+                const float camx = r[0] * gx + r[1] * gy + r[2] * gz;
+                const float camy = r[3] * gx + r[4] * gy + r[5] * gz;
+                const float camz = r[6] * gx + r[7] * gy + r[8] * gz;
+                if (!(camz > 0.0f)) continue;
+                const float inv_z = 1.0f / camz;
+                const float fu = rintf(camx * in->fx * inv_z + in->cx);
+                const float fv = rintf(camy * in->fy * inv_z + in->cy);
+                if (!(fu >= 0.0f && fu < (float)in->cols && fv >= 0.0f && fv < (float)in->rows)) continue;
+                const int pix = (int)fv * in->cols + (int)fu;
+                const kfx_xyz p = vmap[pix];
+                if (isnan(p.x)) continue;
+                const float dx = p.x - vx, dy = p.y - vy, dz = p.z - vz;
+                if (!(sqrtf(dx * dx + dy * dy + dz * dz) < trunc)) continue;
+                kfx_color& cell = vol[orc_cyclic_address(x, y, z, rb)];
+                const kfx_color c = rgb[pix];
+                const int w = cell.w;
+                const float den = (float)(w + 1);
+                const float nr = ((float)(cell.r * w) + 1.0f * (float)c.r) / den;
+                const float ng = ((float)(cell.g * w) + 1.0f * (float)c.g) / den;
+                const float nb = ((float)(cell.b * w) + 1.0f * (float)c.b) / den;
+                cell.r = (uint8_t)std::min(255, std::max(0, (int)rintf(nr)));
+                cell.g = (uint8_t)std::min(255, std::max(0, (int)rintf(ng)));
+                cell.b = (uint8_t)std::min(255, std::max(0, (int)rintf(nb)));
+                cell.w = (uint8_t)std::min(max_weight, w + 1);
+                ++touched;
+            }
+        }
+    return touched;
+}
+
 // ---------------------------------------------------------------------------
 // Raycast, single precision in the order of raycast.cl.
 // ---------------------------------------------------------------------------
@@ -219,14 +276,18 @@ extern "C" float orc_interpolate_trilinear_f32(const kfx_tsdf* vol, float px, fl
                      read_t(vol, rb, gx + 1, gy + 1, gz + 1) * c));
 }
 
-extern "C" void orc_raycast_f32(const kfx_tsdf* vol, kfx_xyz* vmap, kfx_xyz* nmap, const kfx_round_buffer* rb,
-                                const kfx_point_3f* T, const kfx_matrix_3f* R, float trunc,
-                                const kfx_intrinsics* in, const kfx_point_3f* cs) {
+// Returns S, the TSDF voxel reads the algorithm makes: the first sample, every
+// march sample inside the volume, and 8 per trilinear evaluation that passes
+// its bounds test.
+extern "C" uint64_t orc_raycast_f32(const kfx_tsdf* vol, kfx_xyz* vmap, kfx_xyz* nmap, const kfx_round_buffer* rb,
+                                    const kfx_point_3f* T, const kfx_matrix_3f* R, float trunc,
+                                    const kfx_intrinsics* in, const kfx_point_3f* cs) {
     const int W = in->cols, H = in->rows;
     const V3 vm{rb->volume_size.x, rb->volume_size.y, rb->volume_size.z};
     const int NX = rb->voxels_size.x, NY = rb->voxels_size.y, NZ = rb->voxels_size.z;
     const float step = (float)((double)trunc * 0.8);  // raycast.cl:209
-#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads)
+    uint64_t samples = 0;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads) reduction(+ : samples)
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) {
             const int idx = W * y + x;
@@ -256,6 +317,7 @@ extern "C" void orc_raycast_f32(const kfx_tsdf* vol, kfx_xyz* vmap, kfx_xyz* nma
             int gy = std::max(0, std::min(vox_floor(q.y, cs->y), NY - 1));
             int gz = std::max(0, std::min(vox_floor(q.z, cs->z), NZ - 1));
             float tsdf = read_t(vol, rb, gx, gy, gz);
+            samples += 1;
             const float max_time = 3 * (vm.x + vm.y + vm.z);
             for (; tc < max_time; tc += step) {
                 const float prev = tsdf;
@@ -264,13 +326,16 @@ extern "C" void orc_raycast_f32(const kfx_tsdf* vol, kfx_xyz* vmap, kfx_xyz* nma
                 const int vx = vox_floor(q1.x, cs->x), vy = vox_floor(q1.y, cs->y), vz = vox_floor(q1.z, cs->z);
                 if (!(vx >= 0 && vy >= 0 && vz >= 0 && vx < NX && vy < NY && vz < NZ)) break;
                 tsdf = read_t(vol, rb, vx, vy, vz);
+                samples += 1;
                 if (prev < 0.f && tsdf > 0.f) break;
                 if (prev > 0.f && tsdf < 0.f) {
                     const float ftdt = orc_interpolate_trilinear_f32(vol, q1.x, q1.y, q1.z, cs, rb);
                     if (isnan(ftdt)) break;
+                    samples += 8;
                     const V3 q2 = at(start, d, tc);
                     const float ft = orc_interpolate_trilinear_f32(vol, q2.x, q2.y, q2.z, cs, rb);
                     if (isnan(ft)) break;
+                    samples += 8;
                     const float ts = tc - step * ft / (ftdt - ft);
                     const V3 v = at(start, d, ts);
                     vmap[idx] = kfx_xyz{v.x, v.y, v.z};
@@ -283,6 +348,8 @@ extern "C" void orc_raycast_f32(const kfx_tsdf* vol, kfx_xyz* vmap, kfx_xyz* nma
                         const float fy2 = orc_interpolate_trilinear_f32(vol, v.x, v.y - cs->y, v.z, cs, rb);
                         const float fz1 = orc_interpolate_trilinear_f32(vol, v.x, v.y, v.z + cs->z, cs, rb);
                         const float fz2 = orc_interpolate_trilinear_f32(vol, v.x, v.y, v.z - cs->z, cs, rb);
+                        samples += 8 * (uint64_t)(!isnan(fx1) + !isnan(fx2) + !isnan(fy1) + !isnan(fy2) +
+                                                  !isnan(fz1) + !isnan(fz2));
                         const V3 n = normalized3(V3{fx1 - fx2, fy1 - fy2, fz1 - fz2});
                         nmap[idx] = kfx_xyz{n.x, n.y, n.z};
                     }
@@ -290,4 +357,5 @@ extern "C" void orc_raycast_f32(const kfx_tsdf* vol, kfx_xyz* vmap, kfx_xyz* nma
                 }
             }
         }
+    return samples;
 }

@@ -6,7 +6,7 @@
 //   dropin_check <dir>   reads <dir>/in.bin, writes <dir>/*.bin
 //
 // in.bin: int32 W, H, N; float fx, fy, cx, cy, volume_m; float R0[9], t0[3]
-// (camera -> volume, frame 0); uint16 depth0[W*H], depth1[W*H].
+// (camera -> volume, frame 0); uint16 depth0[W*H], depth1[W*H]; Color rgb0[W*H].
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -44,7 +44,10 @@ int main(int argc, char** argv) {
     if (fread(hdr, 4, 3, f) != 3 || fread(cam, 4, 5, f) != 5 || fread(pose, 4, 12, f) != 12) return 2;
     const int W = hdr[0], H = hdr[1], N = hdr[2];
     std::vector<uint16_t> d0((size_t)W * H), d1((size_t)W * H);
-    if (fread(d0.data(), 2, d0.size(), f) != d0.size() || fread(d1.data(), 2, d1.size(), f) != d1.size()) return 2;
+    std::vector<Point_Color> rgb0((size_t)W * H);
+    if (fread(d0.data(), 2, d0.size(), f) != d0.size() || fread(d1.data(), 2, d1.size(), f) != d1.size() ||
+        fread(rgb0.data(), 4, rgb0.size(), f) != rgb0.size())
+        return 2;
     fclose(f);
 
     try {
@@ -54,7 +57,8 @@ int main(int argc, char** argv) {
                                           "resize_vertex_map",           "resize_normal_map",
                                           "estimate_combined",           "transform_estimation",
                                           "transform_map",               "clean_buff",
-                                          "clean_tsdf_slice"};
+                                          "clean_tsdf_slice",            "integrate_color_volume",
+                                          "clean_color_slice"};
         kernel_manager km("unused", names);
         intrinsics intr = {cam[0], cam[1], cam[2], cam[3], W, H};
         round_buffer rb(make_point_3i(N, N, N), make_point_3f(cam[4], cam[4], cam[4]));
@@ -87,6 +91,15 @@ int main(int argc, char** argv) {
             Resize_Vertex_Map(vp[l - 1], vp[l], li, km.Get_Kernel("resize_vertex_map"));
             Resize_Normal_Map(np_[l - 1], np_[l], li, km.Get_Kernel("resize_normal_map"));
         }
+        // Colours of frame 0 at its raycast, weight cap 1 (DEFAULT_MAX_WEIGHT).
+        gpu_array<Point_Color> rgb(dim3(W, H));
+        rgb.memcpy_from_host(rgb0.data());
+        gpu_array<Point_Color> colors(dim3(N, N, N));
+        colors.reset_gpu_part();
+        Integrate_Colors(&colors, &rgb, vp[0], intr, R0inv, t0, DEFAULT_TRUNCATION_DISTANCE, DEFAULT_MAX_WEIGHT, cell,
+                         &rb, km.Get_Kernel("integrate_color_volume"));
+        auto col0 = download<Point_Color>(&colors);
+        dump(dir, "colors0.bin", col0.data(), col0.size() * 4);
         auto vol0 = download<Point_Tsdf>(&volume);
         dump(dir, "volume0.bin", vol0.data(), vol0.size() * sizeof(Point_Tsdf));
         auto v0 = download<Point_XYZ>(vp[0]);
@@ -145,9 +158,13 @@ int main(int argc, char** argv) {
         point_3i mx = make_point_3i(rb.origin_GRID.x + 15, -1, -1);
         matrix<Point_XYZI> pts;
         matrix<Point_Color> cols;
-        Extract_Slice(mn, mx, &rb, &volume, nullptr, cell, &km, pts, cols);
+        Extract_Slice(mn, mx, &rb, &volume, &colors, cell, &km, pts, cols);
         dump(dir, "slice_points.bin", pts.data, pts.Get_Num_Of_Elem() * 16 * (pts.data != nullptr));
+        dump(dir, "slice_colors.bin", cols.data, cols.Get_Num_Of_Elem() * 4 * (cols.data != nullptr));
         Clean_Tsdf_Slice(mn, mx, &rb, &volume, km.Get_Kernel("clean_tsdf_slice"));
+        Clean_Color_Slice(mn, mx, &rb, &colors, km.Get_Kernel("clean_color_slice"));
+        auto col1 = download<Point_Color>(&colors);
+        dump(dir, "colors_cleaned.bin", col1.data(), col1.size() * 4);
         auto vol1 = download<Point_Tsdf>(&volume);
         dump(dir, "volume_cleaned.bin", vol1.data(), vol1.size() * sizeof(Point_Tsdf));
         for (auto* p : vp) delete p;

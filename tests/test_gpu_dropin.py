@@ -1,7 +1,8 @@
 """A C++ program written only against the reference host API
 (opencl_wrappers.h, gpu_array<>, kernel_manager, round_buffer, matrix<>)
-runs integrate, raycast, resize, bilateral, pyramid, vertex/normal maps,
-Estimate_Combined, Transform_Maps, Extract_Slice and Clean_Tsdf_Slice through
+runs integrate, raycast, Integrate_Colors, resize, bilateral, pyramid,
+vertex/normal maps, Estimate_Combined, Transform_Maps, Extract_Slice (with
+the colour volume), Clean_Tsdf_Slice and Clean_Color_Slice through
 libkfx_host.so; every dumped result is compared with the oracle."""
 import os
 import subprocess
@@ -32,12 +33,14 @@ def test_reference_api_program(gpu, orc, tmp_path):
     R0, t0 = poses[10].matrices()
     d0 = kfx.render(scene, poses[10], intr, 1, 10, noise=True)
     d1 = kfx.render(scene, poses[11], intr, 1, 11, noise=True)
+    rgb0 = kfx.render_rgb(scene, poses[10], intr)
     with open(tmp_path / "in.bin", "wb") as f:
         np.array([W, H, N], np.int32).tofile(f)
         np.array([intr.fx, intr.fy, intr.cx, intr.cy, 3.0], np.float32).tofile(f)
         np.concatenate([R0.reshape(9), t0]).astype(np.float32).tofile(f)
         d0.tofile(f)
         d1.tofile(f)
+        rgb0.tofile(f)
     r = subprocess.run([BIN, str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     d = str(tmp_path)
@@ -53,6 +56,10 @@ def test_reference_api_program(gpu, orc, tmp_path):
     assert np.array_equal(np.isnan(gv), np.isnan(rv))
     ok = ~np.isnan(rv)
     assert np.array_equal(gv[ok], rv[ok])
+    # frame 0 colours at that raycast (Integrate_Colors, weight cap 1)
+    col = np.zeros(rb.buffer_length, orc.COLOR_DTYPE)
+    assert orc.integrate_colors(col, rgb0, rv, intr, R0f.T, t0f, 0.03, 1, rb) > 1000
+    assert np.array_equal(_load(d, "colors0.bin", orc.COLOR_DTYPE), col)
     # frame 1: bilateral (within 1 mm), pyramid and vertex maps exact on the GPU's level 0
     g0 = _load(d, "depth1_l0.bin", np.uint16, (H, W))
     assert np.max(np.abs(g0.astype(int) - orc.bilateral(d1))) <= 1
@@ -84,8 +91,11 @@ def test_reference_api_program(gpu, orc, tmp_path):
     np.testing.assert_allclose(gt[ok], wv[ok], atol=1e-6)
     # Extract_Slice + Clean_Tsdf_Slice of a 16-voxel x slab
     mn, mx = (0, 1 << 30, 1 << 30), (15, -1, -1)
-    pts, _ = orc.extract_slice(vol, None, mn, mx, rb)
+    pts, pcol = orc.extract_slice(vol, col, mn, mx, rb)
     gp = np.fromfile(os.path.join(d, "slice_points.bin"), np.float32).reshape(-1, 4)
     assert len(pts) > 0 and np.array_equal(gp, pts)
+    assert np.array_equal(_load(d, "slice_colors.bin", np.uint8, (-1, 4)), pcol)
     orc.clean_slice(vol, mn, mx, rb)
+    orc.clean_slice(col, mn, mx, rb)
     assert np.array_equal(_load(d, "volume_cleaned.bin", kfx.TSDF_DTYPE), vol)
+    assert np.array_equal(_load(d, "colors_cleaned.bin", orc.COLOR_DTYPE), col)

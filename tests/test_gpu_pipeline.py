@@ -94,6 +94,44 @@ def test_cyclic_shift_keeps_tracking(gpu, mode, use_graph):
     pipe.close()
 
 
+def test_shift_extracts_records(gpu):
+    # The same walk with (x, y, z, tsdf, weight) records: the last shift's
+    # records lie in the slab that left the cube (outside the new cube's
+    # global grid range on some axis), carry observed voxels, and number as
+    # many as the points the XYZI run extracted at every shift.
+    kfx = gpu
+    intr = kfx.kinect_intrinsics()
+    scene = kfx.make_scene(2, 10, 10, (0.5, 9.0, -2.0, 2.0), 0.0)
+    gt = kfx.walk(2, (0.0, -1.2, 0.0), 0.02, 0.35, 60)
+    R0, t0 = gt[0].matrices()
+    origin = t0 + 1.5 * R0[:, 2] - 1.536
+    depths = np.stack([kfx.render(scene, gt[i], intr, 2, i, noise=True) for i in range(60)])
+    counts = {}
+    for rec in (0, 1):
+        cfg = kfx.default_config(intr=intr, voxels=kfx.Point3i(256, 256, 256),
+                                 volume_size=kfx.Point3f(3.072, 3.072, 3.072), mode=kfx.PLAIN,
+                                 shift_threshold=0.4, extract_records=rec)
+        pipe = kfx.Pipeline(cfg, gt[0], origin)
+        infos = _run(kfx, pipe, depths)
+        last = max(i for i, f in enumerate(infos) if f.shifted)
+        counts[rec] = [f.extracted_points for f in infos]
+        v = pipe.view()
+        if rec:
+            assert v.last_points is None and v.last_records
+            n = infos[last].extracted_points
+            r = kfx.download(v.last_records, kfx.SLICE_RECORD_DTYPE, n)
+            g = v.round_buffer.origin_GRID_global
+            assert n > 1000
+            assert np.all(r["weight"] > 0)
+            outside = np.zeros(n, bool)
+            for ax, g0 in (("x", g.x), ("y", g.y), ("z", g.z)):
+                g0 = int(round(g0))
+                outside |= (r[ax] < g0) | (r[ax] >= g0 + 256)
+            assert outside.all()
+        pipe.close()
+    assert counts[0] == counts[1]
+
+
 def _c1_learned(kfx, params, use_graph, frames=20, **kw):
     intr = kfx.kinect_intrinsics()
     scene, poses = scenes.c1_scene(), scenes.c1_poses(frames)

@@ -54,6 +54,38 @@ def test_extract_matches_oracle(gpu, orc, n, origin, mn, mx):
 
 
 @pytest.mark.parametrize("n,origin,mn,mx", CASES)
+def test_extract_records_match_points(gpu, orc, n, origin, mn, mx):
+    # (x, y, z, tsdf, weight) records: the oracle's points in the same order,
+    # shifted by the cube's global voxel offset, with the stored value and
+    # weight of each point's voxel.
+    kfx = gpu
+    rb, vol, col = si.random_volume(n, origin, seed=11 + n[1])
+    rb.origin_GRID_global = kfx.Point3f(1000.0, -7.0, 33.0)
+    dv, dc = kfx.Buffer.from_array(vol), kfx.Buffer.from_array(col)
+    wp, wc = orc.extract_slice(vol, col, mn, mx, rb)
+    s = C.c_void_p()
+    kfx.call("kfx_slicer_create", C.byref(rb), C.byref(s))
+    cap = max(1, len(wp))
+    drec, dcol = kfx.Buffer(16 * cap), kfx.Buffer(4 * cap)
+    cnt = C.c_uint64()
+    kfx.call("kfx_extract_slice_records", s, C.byref(kfx.point3i(mn)), C.byref(kfx.point3i(mx)), C.byref(rb), dv.ptr,
+             dc.ptr, drec.ptr, dcol.ptr, cap, C.byref(cnt))
+    kfx.call("kfx_slicer_destroy", s)
+    assert cnt.value == len(wp)
+    if not len(wp):
+        return
+    rec = drec.download(kfx.SLICE_RECORD_DTYPE, len(wp))
+    assert np.array_equal(dcol.download(np.uint8, (len(wp), 4)), wc)
+    local = wp[:, :3].astype(np.int64)
+    assert np.array_equal(rec["x"] - 1000, local[:, 0])
+    assert np.array_equal(rec["y"] + 7, local[:, 1])
+    assert np.array_equal(rec["z"] - 33, local[:, 2])
+    addr = [orc.lib.orc_cyclic_address(int(x), int(y), int(z), C.byref(rb)) for x, y, z in local]
+    assert np.array_equal(rec["tsdf"], vol["value"][addr])
+    assert np.array_equal(rec["weight"], vol["weight"][addr])
+
+
+@pytest.mark.parametrize("n,origin,mn,mx", CASES)
 def test_clean_matches_oracle(gpu, orc, n, origin, mn, mx):
     kfx = gpu
     rb, vol, col = si.random_volume(n, origin, seed=7)

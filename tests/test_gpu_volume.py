@@ -115,6 +115,14 @@ def test_raycast_matches_oracle_c1(gpu, orc, origin):
         nh = ~np.isnan(rn[..., 0])
         assert np.max(np.abs(nm[nh] - rn[nh])) <= 1e-4
         assert np.array_equal(vm[hit], rv[hit]), "raycast vertices not bit-identical"
+        # The counting variant: same maps, and the oracle's count of the
+        # algorithm's voxel reads (the raycast's algorithmic bytes 4 S + 24 W H).
+        ns = C.c_uint64()
+        kfx.call("kfx_raycast_count_samples", dv.ptr, dvm.ptr, dnm.ptr, C.byref(rb), C.byref(kfx.point3f(t)),
+                 C.byref(kfx.matrix3f(R)), scenes.TRUNC, C.byref(intr), C.byref(cs), C.byref(ns))
+        _, _, s_ref = orc.raycast_f32(ref, rb, t, R, scenes.TRUNC, intr, want_samples=True)
+        assert np.array_equal(dvm.download(np.float32, (intr.rows, intr.cols, 3)), vm, equal_nan=True)
+        assert ns.value == s_ref and s_ref > 10 * intr.cols * intr.rows
 
 
 def _random_rotation(rng):
