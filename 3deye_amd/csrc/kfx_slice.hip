@@ -255,6 +255,25 @@ __global__ void __launch_bounds__(256) slice_emit_kernel(const kfx_tsdf* __restr
         }
 }
 
+// Colour of the voxel under each point (Extract_Color_Volume,
+// common/src/opencl_wrappers.cpp:546-590; kernel extract_colors absent):
+// voxel floor(p / cell) of the local grid, its stored colour; (0,0,0,0)
+// when the point lies outside the cube or is not finite.
+__global__ void __launch_bounds__(256) extract_colors_kernel(const kfx_color* __restrict__ col,
+                                                             const kfx_xyzi* __restrict__ pts, uint64_t n,
+                                                             kfx_point_3f cell, kfx_round_buffer rb,
+                                                             kfx_color* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const kfx_xyzi p = pts[i];
+    const float fx = floorf(p.x / cell.x), fy = floorf(p.y / cell.y), fz = floorf(p.z / cell.z);
+    kfx_color c{0, 0, 0, 0};
+    if (fx >= 0.0f && fx < (float)rb.voxels_size.x && fy >= 0.0f && fy < (float)rb.voxels_size.y && fz >= 0.0f &&
+        fz < (float)rb.voxels_size.z)
+        c = col[cyclic_address_in_range((int)fx, (int)fy, (int)fz, rb)];
+    out[i] = c;
+}
+
 // clean_tsdf_slice (call site opencl_wrappers.cpp:835-854, kernel absent) and
 // clean_color_slice.cl:41-67: reset every black-zone voxel, one lane per
 // (x, y) column sweeping z, skipping the z range outside the zone for
@@ -440,6 +459,22 @@ int kfx_extract_slice_records(kfx_slicer* s, const kfx_point_3i* min_bounds, con
                               uint64_t capacity, uint64_t* count) {
     return extract("kfx_extract_slice_records", s, min_bounds, max_bounds, rb, tsdf_volume, color_volume, nullptr,
                    records, colors, capacity, count);
+}
+
+int kfx_extract_colors(const kfx_color* color_volume, const kfx_round_buffer* rb, const kfx_xyzi* points,
+                       uint64_t n, const kfx_point_3f* cell_size, kfx_color* colors) {
+    KFX_REQUIRE_INIT();
+    int rc = check_round_buffer(rb, "kfx_extract_colors");
+    if (rc) return rc;
+    KFX_REQUIRE(cell_size && cell_size->x > 0.0f && cell_size->y > 0.0f && cell_size->z > 0.0f,
+                "kfx_extract_colors: cell size must be positive");
+    if (n == 0) return KFX_OK;
+    KFX_REQUIRE(color_volume && points && colors, "kfx_extract_colors: null pointer");
+    ProfScope prof("extract_colors");
+    extract_colors_kernel<<<(unsigned)((n + 255) / 256), 256, 0, stream()>>>(color_volume, points, n, *cell_size,
+                                                                              *rb, colors);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
 }
 
 }  // extern "C"

@@ -1,6 +1,6 @@
 // matrix.h — host container of the reference (common/inc/matrix.h:13-79,
 // impl/matrix.hpp): the subset the hot path uses (allocation, element access,
-// extend_from for appending extracted slices).
+// extend_from for appending extracted slices, Save_To_File for writing them).
 #ifndef KFX_DROPIN_MATRIX
 #define KFX_DROPIN_MATRIX
 
@@ -8,8 +8,20 @@
 #include <string.h>
 
 #include <stdexcept>
+#include <string>
+#include <typeinfo>
 
+#include "base_point_types.h"
 #include "dim3.h"
+#include "kfx.h"
+
+// Element writers of Save_To_File (the slices are matrices of these).
+inline int kfx_save_matrix_text(const char* path, const Point_XYZI* d, size_t n) {
+    return kfx_save_points_text(path, reinterpret_cast<const kfx_xyzi*>(d), n);
+}
+inline int kfx_save_matrix_text(const char* path, const Point_Color* d, size_t n) {
+    return kfx_save_colors_text(path, reinterpret_cast<const kfx_color*>(d), n);
+}
 
 template <typename T>
 class matrix {
@@ -42,6 +54,23 @@ class matrix {
     void free_buffer() {
         free(data);
         data = nullptr;
+    }
+
+    // The compiler's name of the element storage type (matrix.hpp:443-447).
+    std::string Get_Matrix_Type() const { return typeid(data).name(); }
+
+    // matrix.hpp:455-476: text dump to custom_file_name, or to
+    // "<w>x<h>x<d>-<type>.tst" when it is empty.  1-D matrices of points or
+    // colours (the extracted slices); other shapes and element types have
+    // no writer here.
+    void Save_To_File(const std::string& custom_file_name = "") const {
+        if (sizes.height * sizes.depth != 1) throw std::runtime_error("matrix::Save_To_File: 1-D matrices only");
+        std::string name = custom_file_name;
+        if (name.empty())
+            name = std::to_string(sizes.width) + "x" + std::to_string(sizes.height) + "x" +
+                   std::to_string(sizes.depth) + "-" + Get_Matrix_Type() + ".tst";
+        if (kfx_save_matrix_text(name.c_str(), data, Get_Num_Of_Elem() * (data != nullptr)) != KFX_OK)
+            throw std::runtime_error("matrix::Save_To_File: cannot write " + name);
     }
 
     // Append another matrix's elements (the result is 1-D).

@@ -90,4 +90,17 @@ void Extract_Slice(point_3i Min_Bounds, point_3i Max_Bounds, round_buffer* Round
                    gpu_array_base* gColor_Volume, point_3f Cell_Size, kernel_manager* Gpu_Methods,
                    matrix<Point_XYZI>& Vertex_Slice, matrix<Point_Color>& Color_Slice);
 
+// Whole-volume cloud (common/inc/opencl_wrappers.h:124-129; kernels
+// scan_the_whole_array / extract_tsdf_cloud absent): Extract_Slice's zero
+// crossings over the entire cube, in its order, as metric volume coordinates
+// of the voxel centre ((g + 1/2) * cell), w = 1; appended to the cloud.
+void Extract_Tsdf_Volume(round_buffer* Round_Buffer, const point_3f& Cell_Size, gpu_array_base* gTsdf_Volume,
+                         kernel_manager* Gpu_Methods, matrix<Point_XYZI>& Resulting_Cloud_Parallel);
+// Colours of a metric cloud (common/inc/opencl_wrappers.h:131-138): the
+// colour of voxel floor(p / cell) for every point; the result has the
+// cloud's length.
+void Extract_Color_Volume(gpu_array_base* gColor_Volume, round_buffer* Round_Buffer, matrix<Point_XYZI>& Point_Cloud,
+                          point_3f Cell_Size, opencl_kernel* extract_colors_kernel,
+                          matrix<Point_Color>& Resulting_Color_Point_Cloud);
+
 #endif

@@ -287,3 +287,45 @@ void Extract_Slice(point_3i Min_Bounds, point_3i Max_Bounds, round_buffer* Round
     Vertex_Slice.extend_from(part_v);
     Color_Slice.extend_from(part_c);
 }
+
+void Extract_Tsdf_Volume(round_buffer* Round_Buffer, const point_3f& Cell_Size, gpu_array_base* gTsdf_Volume,
+                         kernel_manager* Gpu_Methods, matrix<Point_XYZI>& Resulting_Cloud_Parallel) {
+    // Every voxel is in the black zone when the x range spans the cube.
+    const point_3i mn = make_point_3i(Round_Buffer->origin_GRID.x, 1, 1);
+    const point_3i mx = make_point_3i(Round_Buffer->origin_GRID.x + Round_Buffer->voxels_size.x - 1, 0, 0);
+    matrix<Point_XYZI> grid;
+    matrix<Point_Color> unused;
+    Extract_Slice(mn, mx, Round_Buffer, gTsdf_Volume, nullptr, Cell_Size, Gpu_Methods, grid, unused);
+    const size_t n = grid.Get_Num_Of_Elem() * (grid.data != nullptr);
+    if (n == 0) return;
+    for (size_t i = 0; i < n; ++i) {
+        XYZI& p = grid.data[i].data;
+        p.x = (p.x + 0.5f) * Cell_Size.x;
+        p.y = (p.y + 0.5f) * Cell_Size.y;
+        p.z = (p.z + 0.5f) * Cell_Size.z;
+    }
+    Resulting_Cloud_Parallel.extend_from(grid);
+}
+
+void Extract_Color_Volume(gpu_array_base* gColor_Volume, round_buffer* Round_Buffer, matrix<Point_XYZI>& Point_Cloud,
+                          point_3f Cell_Size, opencl_kernel*, matrix<Point_Color>& Resulting_Color_Point_Cloud) {
+    const size_t n = Point_Cloud.Get_Num_Of_Elem() * (Point_Cloud.data != nullptr);
+    matrix<Point_Color> out(n);
+    if (n) {
+        void *dp = nullptr, *dc = nullptr;
+        int rc = kfx_malloc(&dp, sizeof(kfx_xyzi) * n);
+        if (rc == KFX_OK) rc = kfx_malloc(&dc, sizeof(kfx_color) * n);
+        if (rc == KFX_OK) rc = kfx_memcpy_htod(dp, Point_Cloud.data, sizeof(kfx_xyzi) * n);
+        if (rc == KFX_OK)
+            rc = kfx_extract_colors(dev<kfx_color>(gColor_Volume, (size_t)Round_Buffer->buffer_length,
+                                                   "Extract_Color_Volume"),
+                                    RB(Round_Buffer), static_cast<const kfx_xyzi*>(dp), n, P(Cell_Size),
+                                    static_cast<kfx_color*>(dc));
+        if (rc == KFX_OK) rc = kfx_memcpy_dtoh(out.data, dc, sizeof(kfx_color) * n);
+        kfx_free(dp);
+        kfx_free(dc);
+        check(rc, "Extract_Color_Volume");
+    }
+    Resulting_Color_Point_Cloud.free_buffer();
+    Resulting_Color_Point_Cloud.extend_from(out);
+}

@@ -283,6 +283,11 @@ _PROTOS = {
     "kfx_synth_orbit": (_I, [C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.c_double, _I,
                              C.POINTER(Pose)]),
     "kfx_synth_walk": (_I, [C.c_uint64, C.POINTER(C.c_double), C.c_double, C.c_double, _I, C.POINTER(Pose)]),
+    "kfx_extract_colors": (_I, [_P, C.POINTER(RoundBuffer), _P, C.c_uint64, C.POINTER(Point3f), _P]),
+    "kfx_save_points_text": (_I, [C.c_char_p, _P, C.c_uint64]),
+    "kfx_save_colors_text": (_I, [C.c_char_p, _P, C.c_uint64]),
+    "kfx_save_pcd": (_I, [C.c_char_p, _P, _P, C.c_uint64, _I]),
+    "kfx_save_records": (_I, [C.c_char_p, _P, C.c_uint64]),
     "kfx_synth_render_rgb": (_I, [C.POINTER(SynthScene), C.POINTER(Pose), C.POINTER(Intrinsics), _I, _P]),
     "kfx_synth_render": (_I, [C.POINTER(SynthScene), C.POINTER(Pose), C.POINTER(Intrinsics), C.c_uint64, _I, _I, _I,
                               _P, _P]),

@@ -337,6 +337,14 @@ int kfx_extract_slice_records(kfx_slicer* slicer, const kfx_point_3i* min_bounds
                               const kfx_color* color_volume, kfx_slice_record* records, kfx_color* colors,
                               uint64_t capacity, uint64_t* count);
 
+/* Extract_Color_Volume (common/inc/opencl_wrappers.h:131-138,
+ * common/src/opencl_wrappers.cpp:546-590; kernel extract_colors absent): for
+ * each point (volume coordinates in metres, device array), the colour of the
+ * voxel floor(p / cell) of the cyclic colour volume; (0,0,0,0) outside the
+ * cube.  Enqueues only. */
+int kfx_extract_colors(const kfx_color* color_volume, const kfx_round_buffer* rb, const kfx_xyzi* points,
+                       uint64_t n, const kfx_point_3f* cell_size, kfx_color* colors);
+
 /* ------------------------------------------------------------------------
  * Learned depth denoiser (no reference counterpart; it stands in for the
  * bilateral filter of Apply_Billateral_Filtration).  Residual network of four
@@ -525,6 +533,25 @@ int kfx_synth_render(const kfx_synth_scene* scene, const kfx_pose* pose, const k
  * greys; no hit: black), w = 1. */
 int kfx_synth_render_rgb(const kfx_synth_scene* scene, const kfx_pose* pose, const kfx_intrinsics* intr,
                          int threads, kfx_color* rgb);
+
+/* ------------------------------------------------------------------------
+ * World-model files (host arrays in, files out).
+ * ------------------------------------------------------------------------ */
+/* matrix<T>::Save_To_File text format (common/inc/impl/matrix.hpp:411-476)
+ * of an extracted slice as a 1-D matrix: "n 1 1 <type>", then every element
+ * and a space on one line; XYZI elements as x, y, z, w joined by 'x'
+ * (base_points.cpp:570-574), Color elements as "r.g.b.w " (base_points.cpp:278-282). */
+int kfx_save_points_text(const char* path, const kfx_xyzi* points, uint64_t n);
+int kfx_save_colors_text(const char* path, const kfx_color* colors, uint64_t n);
+/* PCD v0.7 file of the cloud the reference hands to PCL
+ * (convert_to_point_cloud, common/src/math_utils.cpp:216-328): x y z rgb
+ * (pcl::PointXYZRGB, the reference's packing b << 16 | g << 8 | r) when
+ * colors is given, else x y z intensity (pcl::PointXYZI, intensity = w);
+ * binary != 0 for DATA binary, else DATA ascii. */
+int kfx_save_pcd(const char* path, const kfx_xyzi* points, const kfx_color* colors, uint64_t n, int binary);
+/* Binary slice records: magic "KFXREC1\0", uint64 count, then the
+ * kfx_slice_record array (no reference counterpart). */
+int kfx_save_records(const char* path, const kfx_slice_record* records, uint64_t n);
 
 /* Trajectory text file in the reference's format (write_isometry_to_file,
  * common/src/string_utils.cpp:26-74): per pose its index, the three rows of

@@ -58,7 +58,7 @@ int main(int argc, char** argv) {
                                           "estimate_combined",           "transform_estimation",
                                           "transform_map",               "clean_buff",
                                           "clean_tsdf_slice",            "integrate_color_volume",
-                                          "clean_color_slice"};
+                                          "clean_color_slice",           "extract_colors"};
         kernel_manager km("unused", names);
         intrinsics intr = {cam[0], cam[1], cam[2], cam[3], W, H};
         round_buffer rb(make_point_3i(N, N, N), make_point_3f(cam[4], cam[4], cam[4]));
@@ -161,6 +161,16 @@ int main(int argc, char** argv) {
         Extract_Slice(mn, mx, &rb, &volume, &colors, cell, &km, pts, cols);
         dump(dir, "slice_points.bin", pts.data, pts.Get_Num_Of_Elem() * 16 * (pts.data != nullptr));
         dump(dir, "slice_colors.bin", cols.data, cols.Get_Num_Of_Elem() * 4 * (cols.data != nullptr));
+        pts.Save_To_File(dir + "/slice_points.tst");
+        cols.Save_To_File(dir + "/slice_colors.tst");
+        // The whole cube as a metric cloud, and the colours under it.
+        matrix<Point_XYZI> cloud;
+        matrix<Point_Color> cloud_colors;
+        Extract_Tsdf_Volume(&rb, cell, &volume, &km, cloud);
+        Extract_Color_Volume(&colors, &rb, cloud, cell, km.Get_Kernel("extract_colors"), cloud_colors);
+        dump(dir, "volume_cloud.bin", cloud.data, cloud.Get_Num_Of_Elem() * 16 * (cloud.data != nullptr));
+        dump(dir, "volume_cloud_colors.bin", cloud_colors.data,
+             cloud_colors.Get_Num_Of_Elem() * 4 * (cloud_colors.data != nullptr));
         Clean_Tsdf_Slice(mn, mx, &rb, &volume, km.Get_Kernel("clean_tsdf_slice"));
         Clean_Color_Slice(mn, mx, &rb, &colors, km.Get_Kernel("clean_color_slice"));
         auto col1 = download<Point_Color>(&colors);

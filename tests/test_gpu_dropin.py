@@ -95,6 +95,25 @@ def test_reference_api_program(gpu, orc, tmp_path):
     gp = np.fromfile(os.path.join(d, "slice_points.bin"), np.float32).reshape(-1, 4)
     assert len(pts) > 0 and np.array_equal(gp, pts)
     assert np.array_equal(_load(d, "slice_colors.bin", np.uint8, (-1, 4)), pcol)
+    # matrix::Save_To_File of both slices: the text holds the same values.
+    head, body = open(os.path.join(d, "slice_points.tst")).read().split("\n")[:2]
+    assert head == f"{len(pts)} 1 1 Point_XYZI" or head.startswith(f"{len(pts)} 1 1 ")
+    txt = np.array([[float(v) for v in e.split("x")] for e in body.split()], np.float32)
+    assert np.array_equal(txt, pts)  # grid coordinates print exactly under %g
+    chead, cbody = open(os.path.join(d, "slice_colors.tst")).read().split("\n")[:2]
+    assert chead.startswith(f"{len(pcol)} 1 1 ")
+    assert np.array_equal(np.array([[int(v) for v in e.split(".")] for e in cbody.split()], np.uint8), pcol)
+    # Extract_Tsdf_Volume: the whole cube's crossings, metric voxel centres;
+    # Extract_Color_Volume: the colour of the voxel under each point.
+    allp, _ = orc.extract_slice(vol, None, (0, 1, 1), (N - 1, 0, 0), rb)
+    cell = np.float32(3.0 / N)
+    want = allp.copy()
+    want[:, :3] = (allp[:, :3] + np.float32(0.5)) * cell
+    cloud = _load(d, "volume_cloud.bin", np.float32, (-1, 4))
+    assert len(cloud) > len(pts) and np.array_equal(cloud, want)
+    g = np.floor(cloud[:, :3] / cell).astype(np.int64)
+    addr = g[:, 0] + N * g[:, 1] + N * N * g[:, 2]  # origin 0
+    assert np.array_equal(_load(d, "volume_cloud_colors.bin", orc.COLOR_DTYPE), col[addr])
     orc.clean_slice(vol, mn, mx, rb)
     orc.clean_slice(col, mn, mx, rb)
     assert np.array_equal(_load(d, "volume_cleaned.bin", kfx.TSDF_DTYPE), vol)

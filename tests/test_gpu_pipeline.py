@@ -97,8 +97,9 @@ def test_cyclic_shift_keeps_tracking(gpu, mode, use_graph):
 def test_shift_extracts_records(gpu):
     # The same walk with (x, y, z, tsdf, weight) records: the last shift's
     # records lie in the slab that left the cube (outside the new cube's
-    # global grid range on some axis), carry observed voxels, and number as
-    # many as the points the XYZI run extracted at every shift.
+    # global grid range on some axis, or on its first layer: a crossing
+    # emits the +1 neighbour of a slab voxel), carry observed voxels, and
+    # number as many as the points the XYZI run extracted at every shift.
     kfx = gpu
     intr = kfx.kinect_intrinsics()
     scene = kfx.make_scene(2, 10, 10, (0.5, 9.0, -2.0, 2.0), 0.0)
@@ -126,7 +127,7 @@ def test_shift_extracts_records(gpu):
             outside = np.zeros(n, bool)
             for ax, g0 in (("x", g.x), ("y", g.y), ("z", g.z)):
                 g0 = int(round(g0))
-                outside |= (r[ax] < g0) | (r[ax] >= g0 + 256)
+                outside |= (r[ax] <= g0) | (r[ax] >= g0 + 256)
             assert outside.all()
         pipe.close()
     assert counts[0] == counts[1]
