@@ -98,10 +98,6 @@ __device__ __forceinline__ int wave_max(int v) {
     for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
     return v;
 }
-__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
-    for (int off = 32; off > 0; off >>= 1) v |= (uint32_t)__shfl_xor((int)v, off);
-    return v;
-}
 
 // scale_depth (opencl_wrappers.cpp:37-43; kernel absent from the reference,
 // KinectFusion convention): metres along the pixel's ray,
@@ -291,44 +287,6 @@ __device__ __forceinline__ bool segment_culled(const IntegrateParams& p, const P
     return dmin > (maxd + p.trunc) * 1.0001f + 1e-4f;
 }
 
-// segment_culled's decision without branches, negated: every lane issues its
-// four bound reads (at clamped, valid addresses) whatever the case, so the
-// tests of a whole run go out as one round of loads; the result is the
-// same.
-__device__ __forceinline__ bool segment_live(const IntegrateParams& p, const PoseTerms& q,
-                                             const float* __restrict__ scaled, float gx, float gy, float cxy0,
-                                             float cxy1, float cxy2, int za, int zb) {
-    const float gza = ((float)za + 0.5f) * p.csz - q.tz, gzb = ((float)zb + 0.5f) * p.csz - q.tz;
-    const float ax = cxy0 + q.r[2] * gza, ay = cxy1 + q.r[5] * gza, az = cxy2 + q.r[8] * gza;
-    const float bx = cxy0 + q.r[2] * gzb, by = cxy1 + q.r[5] * gzb, bz = cxy2 + q.r[8] * gzb;
-    const bool behind = az <= 0.0f && bz <= 0.0f;
-    const bool undecided = !(az > 1e-4f && bz > 1e-4f);
-    const float ia = __builtin_amdgcn_rcpf(az), ib = __builtin_amdgcn_rcpf(bz);
-    const float ua = ax * p.fx * ia + p.cx, ub = bx * p.fx * ib + p.cx;
-    const float va = ay * p.fy * ia + p.cy, vb = by * p.fy * ib + p.cy;
-    const float fu0 = floorf(fminf(ua, ub)) - 1.0f, fu1 = ceilf(fmaxf(ua, ub)) + 1.0f;
-    const float fv0 = floorf(fminf(va, vb)) - 1.0f, fv1 = ceilf(fmaxf(va, vb)) + 1.0f;
-    const float ucap = (float)(p.cols - 1), vcap = (float)(p.rows - 1);
-    const bool outside = fu1 < 0.0f || fv1 < 0.0f || fu0 > ucap || fv0 > vcap;
-    // fminf/fmaxf return the number when the other operand is NaN, so the
-    // clamped pixel range is always inside the image.
-    const int u0 = (int)fminf(fmaxf(fu0, 0.0f), ucap), u1 = (int)fminf(fmaxf(fu1, 0.0f), ucap);
-    const int v0 = (int)fminf(fmaxf(fv0, 0.0f), vcap), v1 = (int)fminf(fmaxf(fv1, 0.0f), vcap);
-    const int span = max(u1 - u0, v1 - v0) + 1;
-    const int l = span <= 1 ? 0 : 32 - __clz(span - 1);
-    const bool wide = l > kBoundLevels;
-    const int lc = min(l, kBoundLevels);
-    const float* base = lc == 0 ? scaled : p.bound + level_offset(p, lc);
-    const int lw = (p.cols + (1 << lc) - 1) >> lc;
-    const int a0 = u0 >> lc, a1 = u1 >> lc, b0 = v0 >> lc, b1 = v1 >> lc;
-    const float maxd = fmaxf(fmaxf(base[b0 * lw + a0], base[b0 * lw + a1]), fmaxf(base[b1 * lw + a0], base[b1 * lw + a1]));
-    const float dz = (gza <= 0.0f && gzb >= 0.0f) ? 0.0f : fminf(fabsf(gza), fabsf(gzb));
-    const float dmin = sqrtf(gx * gx + gy * gy + dz * dz);
-    const bool far = maxd == 0.0f || dmin > (maxd + p.trunc) * 1.0001f + 1e-4f;
-    const bool culled = behind || (!undecided && (outside || (!wide && far)));
-    return !culled;
-}
-
 template <bool kDevicePose>
 __global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
 integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume, const IntegrateParams p,
@@ -360,6 +318,7 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
     const int own_lo = lo, own_hi = hi;
     lo = wave_min(lo);
     hi = wave_max(hi);
+    if (!inside) return;
 
     // Column terms of the camera coordinates: the rule's dot products are
     // evaluated as (r0 * gx + r1 * gy) + r2 * gz, so the first two terms are
@@ -369,31 +328,18 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
     const float cxy2 = q.r[6] * gx + q.r[7] * gy;
     const float fcols = (float)p.cols, frows = (float)p.rows;
     const uint32_t nz = (uint32_t)p.nz;
-
-    // Culling of the run's batches, all at once: each lane bounds each of
-    // its 8-voxel segments (one round of pyramid reads for the whole run);
-    // the wave visits the batches any lane needs.
-    uint32_t live = 0;
-#pragma unroll
-    for (int b = 0; b < kIntegrateZRun / kIntegrateBatch; ++b) {
-        const int z0 = lo + b * kIntegrateBatch;
-        const int za = max(z0, own_lo), zb = min(z0 + kIntegrateBatch - 1, own_hi);
-        bool need = z0 <= hi;
-        if (p.bound != nullptr) need = need && za <= zb && segment_live(p, q, scaled, gx, gy, cxy0, cxy1, cxy2, za, zb);
-        live |= (uint32_t)need << b;
-    }
-    // Every lane takes part in the reduction (lanes outside the volume have
-    // an empty range and contribute nothing); then they leave.
-    live = wave_or(live);
-    if (!inside) return;
-    live = (uint32_t)__builtin_amdgcn_readfirstlane((int)live);
-    if (live == 0) return;
-
-    // Software pipeline over the live batches: the depth gather of the next
-    // batch is in flight while the current batch's voxels load and update.
-    // Phase 1 (no branches): projection and depth gather.  Pixels outside
-    // the image read pixel 0 and are masked afterwards.
-    auto gather = [&](int z0, float* dp, float* dist, bool* inimg) {
+    for (int z0 = lo; z0 <= hi; z0 += kIntegrateBatch) {
+        // Skip the batch when no lane can update a voxel of it (every lane's
+        // part of the batch inside its own clipped range is culled).
+        if (p.bound != nullptr) {
+            const int za = max(z0, own_lo), zb = min(z0 + kIntegrateBatch - 1, own_hi);
+            const bool culled = za > zb || segment_culled(p, q, scaled, gx, gy, cxy0, cxy1, cxy2, za, zb);
+            if (__all(culled)) continue;
+        }
+        float dp[kIntegrateBatch], dist[kIntegrateBatch];
+        bool inimg[kIntegrateBatch];
+        // Phase 1 (no branches): projection and depth gather of the batch.
+        // Pixels outside the image read pixel 0 and are masked afterwards.
 #pragma unroll
         for (int k = 0; k < kIntegrateBatch; ++k) {
             const int z = z0 + k;
@@ -410,14 +356,6 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
             dist[k] = sqrtf(gz * gz + part);
             inimg[k] = in;
         }
-    };
-    float dp[kIntegrateBatch], dist[kIntegrateBatch];
-    bool inimg[kIntegrateBatch];
-    int b = __builtin_ctz(live);
-    live &= live - 1;
-    gather(lo + b * kIntegrateBatch, dp, dist, inimg);
-    for (;;) {
-        const int z0 = lo + b * kIntegrateBatch;
         // Phase 2: the update test, and the loads of the voxels it selects.
         float tsdf_new[kIntegrateBatch];
         bool upd[kIntegrateBatch];
@@ -432,14 +370,6 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
             rz = min(rz, rz - nz);  // (z + oz) mod nz for z + oz < 2 nz
             addr[k] = col + p.slice * rz;
             if (upd[k]) prev[k] = volume[addr[k]];
-        }
-        // Next batch's phase 1, overlapping the loads above.
-        const bool more = live != 0;
-        int bn = 0;
-        if (more) {
-            bn = __builtin_ctz(live);
-            live &= live - 1;
-            gather(lo + bn * kIntegrateBatch, dp, dist, inimg);
         }
         // Phase 3: running weighted average, quantise, write what changed.
 #pragma unroll
@@ -457,8 +387,6 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
                 volume[addr[k]] = out;
             }
         }
-        if (!more) break;
-        b = bn;
     }
 }
 
