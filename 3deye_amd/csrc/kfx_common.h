@@ -107,6 +107,23 @@ __host__ __device__ __forceinline__ uint32_t cyclic_address_in_range(int x, int 
     return ((uint32_t)rb.voxels_size.y * rz + ry) * (uint32_t)rb.voxels_size.x + rx;
 }
 
+// Correctly rounded square root (IEEE sqrtf) without the library's range
+// scaling: the hardware v_sqrt_f32 (within one ulp) corrected by the signs of
+// its neighbours' residuals x - s' s, which picks the correctly rounded root
+// for every x in [2^-100, 2^100) whatever the starting value within an ulp
+// (checked exhaustively on the CPU by tests/cpp/exact_arith_check.c).  Other
+// x (zero, tiny, huge, NaN) take sqrtf.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    if (!(x >= 0x1p-100f && x < 0x1p100f)) return sqrtf(x);
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sdn = __uint_as_float(__float_as_uint(s) - 1u), sup = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rdn = fmaf(-sdn, s, x), rup = fmaf(-sup, s, x);
+    float r = s;
+    if (rdn <= 0.0f) r = sdn;
+    if (rup > 0.0f) r = sup;
+    return r;
+}
+
 // Voxel codec (raycast.cl:9-27): decode value / 32767, encode rtz then clamp.
 // The decode is the correctly rounded quotient, formed without a division:
 // q0 = v * RN(1/32767), then one FMA correction with the exact residual

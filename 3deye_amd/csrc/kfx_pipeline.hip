@@ -153,6 +153,12 @@ int preprocess(kfx_pipeline* p, bool learned) {
         KFX_TRY(kfx_pyr_down(p->dep[l - 1], p->lvl[l - 1].cols, p->lvl[l - 1].rows, p->dep[l], p->cfg.sigma_color));
         KFX_TRY(step_sync(p));
     }
+    if (p->cfg.mode == KFX_PIPELINE_FUSED) {
+        // Every level's vertex and normal maps in one launch (same values).
+        const uint16_t* dep[KFX_ICP_LEVELS];
+        for (int l = 0; l < KFX_ICP_LEVELS; ++l) dep[l] = p->dep[l];
+        return kfx_maps_pyramid(dep, p->vc, p->nc, p->lvl, KFX_ICP_LEVELS);
+    }
     for (int l = 0; l < KFX_ICP_LEVELS; ++l) {
         KFX_TRY(kfx_vertex_map(p->dep[l], p->vc[l], &p->lvl[l]));
         KFX_TRY(step_sync(p));
@@ -165,6 +171,10 @@ int preprocess(kfx_pipeline* p, bool learned) {
 }
 
 int resize_prev(kfx_pipeline* p) {
+    static_assert(KFX_ICP_LEVELS == 3, "the one-launch resize builds levels 1 and 2");
+    if (p->cfg.mode == KFX_PIPELINE_FUSED)
+        return kfx_resize_pyramid(p->vp[0], p->np[0], p->lvl[0].cols, p->lvl[0].rows, p->vp[1], p->np[1], p->vp[2],
+                                  p->np[2]);
     for (int l = 1; l < KFX_ICP_LEVELS; ++l) {
         KFX_TRY(kfx_resize_map(p->vp[l - 1], p->lvl[l - 1].cols, p->lvl[l - 1].rows, p->vp[l], 0));
         KFX_TRY(step_sync(p));

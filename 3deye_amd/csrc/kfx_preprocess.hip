@@ -7,6 +7,7 @@
 // NaN in x marking an invalid pixel.  The stencil kernels stage a tile plus
 // halo in LDS so each depth/vertex sample is read from HBM once per tile.
 #include <math.h>
+#include <string.h>
 
 #include "kfx_common.h"
 
@@ -165,23 +166,12 @@ __device__ void smallest_eigvec(const float* cov, float& nx, float& ny, float& n
     nx = vx * s; ny = vy * s; nz = vz * s;
 }
 
-__global__ void __launch_bounds__(kNmTX * kNmTY)
-normal_map_kernel(const kfx_xyz* __restrict__ vmap, kfx_xyz* __restrict__ nmap, int cols, int rows) {
-    __shared__ float sx[kNmLY][kNmLX], sy[kNmLY][kNmLX], sz[kNmLY][kNmLX];
-    const int bx = blockIdx.x * kNmTX, by = blockIdx.y * kNmTY;
-    const int tid = threadIdx.y * kNmTX + threadIdx.x;
-    for (int i = tid; i < kNmLX * kNmLY; i += kNmTX * kNmTY) {
-        const int ly = i / kNmLX, lx = i - ly * kNmLX;
-        const int gx = bx + lx - kNmR, gy = by + ly - kNmR;
-        kfx_xyz v{NAN, NAN, NAN};
-        if (gx >= 0 && gx < cols && gy >= 0 && gy < rows) v = vmap[gy * cols + gx];
-        sx[ly][lx] = v.x; sy[ly][lx] = v.y; sz[ly][lx] = v.z;
-    }
-    __syncthreads();
-    const int u = bx + threadIdx.x, v = by + threadIdx.y;
-    if (u >= cols || v >= rows) return;
+typedef float NmTile[kNmLY][kNmLX];
+
+// Normal of pixel (u, v) from the vertex tile in LDS (tile position lu, lv).
+__device__ __forceinline__ kfx_xyz tile_normal(const NmTile& sx, const NmTile& sy, const NmTile& sz, int lu, int lv,
+                                               int u, int v, int cols, int rows) {
     kfx_xyz out{NAN, NAN, NAN};
-    const int lu = threadIdx.x + kNmR, lv = threadIdx.y + kNmR;
     const float pcx = sx[lv][lu], pcy = sy[lv][lu], pcz = sz[lv][lu];
     if (!isnan(pcx)) {
         // Window rows [max(v-3, 0), min(v+4, rows-1)), columns likewise:
@@ -221,7 +211,71 @@ normal_map_kernel(const kfx_xyz* __restrict__ vmap, kfx_xyz* __restrict__ nmap, 
             }
         }
     }
-    nmap[v * cols + u] = out;
+    return out;
+}
+
+__global__ void __launch_bounds__(kNmTX * kNmTY)
+normal_map_kernel(const kfx_xyz* __restrict__ vmap, kfx_xyz* __restrict__ nmap, int cols, int rows) {
+    __shared__ NmTile sx, sy, sz;
+    const int bx = blockIdx.x * kNmTX, by = blockIdx.y * kNmTY;
+    const int tid = threadIdx.y * kNmTX + threadIdx.x;
+    for (int i = tid; i < kNmLX * kNmLY; i += kNmTX * kNmTY) {
+        const int ly = i / kNmLX, lx = i - ly * kNmLX;
+        const int gx = bx + lx - kNmR, gy = by + ly - kNmR;
+        kfx_xyz v{NAN, NAN, NAN};
+        if (gx >= 0 && gx < cols && gy >= 0 && gy < rows) v = vmap[gy * cols + gx];
+        sx[ly][lx] = v.x; sy[ly][lx] = v.y; sz[ly][lx] = v.z;
+    }
+    __syncthreads();
+    const int u = bx + threadIdx.x, v = by + threadIdx.y;
+    if (u >= cols || v >= rows) return;
+    nmap[v * cols + u] = tile_normal(sx, sy, sz, threadIdx.x + kNmR, threadIdx.y + kNmR, u, v, cols, rows);
+}
+
+// Vertex and normal maps of every pyramid level in one launch: each
+// workgroup owns one 32 x 8 tile of one level, back-projects its depth tile
+// and halo into LDS (vertex_map_kernel's arithmetic), writes the tile's
+// vertices and computes its normals from the same LDS tile (normal_map_kernel's
+// arithmetic): the per-level kernels' results, bit for bit.
+struct MapsLevel {
+    const uint16_t* depth;
+    kfx_xyz* vmap;
+    kfx_xyz* nmap;
+    kfx_intrinsics in;
+    float fx_inv, fy_inv;
+    int tiles_x, first_tile;
+};
+struct MapsParams {
+    MapsLevel lv[KFX_ICP_LEVELS];
+    int levels;
+};
+
+__global__ void __launch_bounds__(kNmTX * kNmTY) maps_kernel(const MapsParams P) {
+    __shared__ NmTile sx, sy, sz;
+    const int t = blockIdx.x;
+    int l = 0;
+    for (int k = 1; k < P.levels; ++k) l = t >= P.lv[k].first_tile ? k : l;
+    const MapsLevel& L = P.lv[l];
+    const int tl = t - L.first_tile;
+    const int bx = (tl % L.tiles_x) * kNmTX, by = (tl / L.tiles_x) * kNmTY;
+    const int cols = L.in.cols, rows = L.in.rows;
+    const int tid = threadIdx.y * kNmTX + threadIdx.x;
+    for (int i = tid; i < kNmLX * kNmLY; i += kNmTX * kNmTY) {
+        const int ly = i / kNmLX, lx = i - ly * kNmLX;
+        const int gx = bx + lx - kNmR, gy = by + ly - kNmR;
+        kfx_xyz v{NAN, NAN, NAN};
+        if (gx >= 0 && gx < cols && gy >= 0 && gy < rows) {
+            const float z = (float)L.depth[gy * cols + gx] / 1000.0f;
+            if (z != 0.0f) v = kfx_xyz{z * ((float)gx - L.in.cx) * L.fx_inv, z * ((float)gy - L.in.cy) * L.fy_inv, z};
+        }
+        sx[ly][lx] = v.x; sy[ly][lx] = v.y; sz[ly][lx] = v.z;
+    }
+    __syncthreads();
+    const int u = bx + threadIdx.x, v = by + threadIdx.y;
+    if (u >= cols || v >= rows) return;
+    const int lu = threadIdx.x + kNmR, lv = threadIdx.y + kNmR;
+    L.vmap[v * cols + u] = kfx_xyz{sx[lv][lu], sy[lv][lu], sz[lv][lu]};
+    L.nmap[v * cols + u] = tile_normal(sx, sy, sz, lu, lv, u, v, cols, rows);
 }
 
 // --------------------------------------------------------------------------
@@ -263,14 +317,7 @@ __global__ void __launch_bounds__(256) transform_maps_kernel(const kfx_xyz* __re
 // block at (2x, 2y) when all four are valid, NaN otherwise; normals are
 // renormalised.
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) resize_map_kernel(const kfx_xyz* __restrict__ in, int icols,
-                                                         kfx_xyz* __restrict__ out, int ocols, int orows,
-                                                         int normalize) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= ocols || y >= orows) return;
-    const kfx_xyz a = in[(2 * y) * icols + 2 * x], b = in[(2 * y) * icols + 2 * x + 1];
-    const kfx_xyz c = in[(2 * y + 1) * icols + 2 * x], d = in[(2 * y + 1) * icols + 2 * x + 1];
+__device__ __forceinline__ kfx_xyz resize4(kfx_xyz a, kfx_xyz b, kfx_xyz c, kfx_xyz d, bool normalize) {
     kfx_xyz o{NAN, NAN, NAN};
     if (!(isnan(a.x) || isnan(b.x) || isnan(c.x) || isnan(d.x))) {
         o.x = (a.x + b.x + c.x + d.x) / 4.0f;
@@ -281,7 +328,47 @@ __global__ void __launch_bounds__(256) resize_map_kernel(const kfx_xyz* __restri
             o.x *= s; o.y *= s; o.z *= s;
         }
     }
-    out[y * ocols + x] = o;
+    return o;
+}
+
+// Output pixel (x, y) of one 2x downsampling of `in` (icols wide).
+__device__ __forceinline__ kfx_xyz resize_at(const kfx_xyz* __restrict__ in, int icols, int x, int y, bool normalize) {
+    return resize4(in[(2 * y) * icols + 2 * x], in[(2 * y) * icols + 2 * x + 1], in[(2 * y + 1) * icols + 2 * x],
+                   in[(2 * y + 1) * icols + 2 * x + 1], normalize);
+}
+
+__global__ void __launch_bounds__(256) resize_map_kernel(const kfx_xyz* __restrict__ in, int icols,
+                                                         kfx_xyz* __restrict__ out, int ocols, int orows,
+                                                         int normalize) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= ocols || y >= orows) return;
+    out[y * ocols + x] = resize_at(in, icols, x, y, normalize != 0);
+}
+
+// Levels 1 and 2 of the vertex and normal maps from level 0 in one launch:
+// a level-2 pixel recomputes its four level-1 inputs from level 0 with the
+// same arithmetic, so the result is two resize_map_kernel passes bit for bit.
+__global__ void __launch_bounds__(256) resize_pyramid_kernel(const kfx_xyz* __restrict__ v0,
+                                                             const kfx_xyz* __restrict__ n0, int c0,
+                                                             kfx_xyz* __restrict__ v1, kfx_xyz* __restrict__ n1,
+                                                             int c1, int r1, kfx_xyz* __restrict__ v2,
+                                                             kfx_xyz* __restrict__ n2, int c2, int r2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p1 = c1 * r1;
+    if (i < p1) {
+        const int x = i % c1, y = i / c1;
+        v1[i] = resize_at(v0, c0, x, y, false);
+        n1[i] = resize_at(n0, c0, x, y, true);
+        return;
+    }
+    const int j = i - p1;
+    if (j >= c2 * r2) return;
+    const int x = j % c2, y = j / c2;
+    v2[j] = resize4(resize_at(v0, c0, 2 * x, 2 * y, false), resize_at(v0, c0, 2 * x + 1, 2 * y, false),
+                    resize_at(v0, c0, 2 * x, 2 * y + 1, false), resize_at(v0, c0, 2 * x + 1, 2 * y + 1, false), false);
+    n2[j] = resize4(resize_at(n0, c0, 2 * x, 2 * y, true), resize_at(n0, c0, 2 * x + 1, 2 * y, true),
+                    resize_at(n0, c0, 2 * x, 2 * y + 1, true), resize_at(n0, c0, 2 * x + 1, 2 * y + 1, true), true);
 }
 
 }  // namespace kfx
@@ -363,6 +450,52 @@ int kfx_resize_map(const kfx_xyz* in, int in_cols, int in_rows, kfx_xyz* out, in
     const int oc = in_cols / 2, orow = in_rows / 2;
     dim3 block(64, 4), grid(divup(oc, 64), divup(orow, 4));
     resize_map_kernel<<<grid, block, 0, stream()>>>(in, in_cols, out, oc, orow, normalize);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_maps_pyramid(const uint16_t* const* depth, kfx_xyz* const* vmap, kfx_xyz* const* nmap,
+                     const kfx_intrinsics* intr, int levels) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(depth && vmap && nmap && intr, "kfx_maps_pyramid: null pointer");
+    KFX_REQUIRE(levels >= 1 && levels <= KFX_ICP_LEVELS, "kfx_maps_pyramid: levels %d outside [1, %d]", levels,
+                KFX_ICP_LEVELS);
+    MapsParams P;
+    memset(&P, 0, sizeof(P));
+    P.levels = levels;
+    int tiles = 0;
+    for (int l = 0; l < levels; ++l) {
+        KFX_REQUIRE(depth[l] && vmap[l] && nmap[l], "kfx_maps_pyramid: null map at level %d", l);
+        KFX_REQUIRE(vmap[l] != nmap[l], "kfx_maps_pyramid: vertex and normal maps alias at level %d", l);
+        KFX_REQUIRE(intr[l].cols > 0 && intr[l].rows > 0, "kfx_maps_pyramid: empty level %d", l);
+        MapsLevel& L = P.lv[l];
+        L.depth = depth[l];
+        L.vmap = vmap[l];
+        L.nmap = nmap[l];
+        L.in = intr[l];
+        L.fx_inv = 1.0f / intr[l].fx;
+        L.fy_inv = 1.0f / intr[l].fy;
+        L.tiles_x = (int)divup(intr[l].cols, kNmTX);
+        L.first_tile = tiles;
+        tiles += L.tiles_x * (int)divup(intr[l].rows, kNmTY);
+    }
+    ProfScope prof("maps_pyramid");
+    maps_kernel<<<tiles, dim3(kNmTX, kNmTY), 0, stream()>>>(P);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_resize_pyramid(const kfx_xyz* vmap0, const kfx_xyz* nmap0, int cols, int rows, kfx_xyz* vmap1,
+                       kfx_xyz* nmap1, kfx_xyz* vmap2, kfx_xyz* nmap2) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(vmap0 && nmap0 && vmap1 && nmap1 && vmap2 && nmap2, "kfx_resize_pyramid: null pointer");
+    KFX_REQUIRE(cols >= 4 && rows >= 4 && cols % 4 == 0 && rows % 4 == 0,
+                "kfx_resize_pyramid: level 0 must be at least 4 x 4 and a multiple of 4");
+    const int c1 = cols / 2, r1 = rows / 2, c2 = cols / 4, r2 = rows / 4;
+    const int n = c1 * r1 + c2 * r2;
+    ProfScope prof("resize_pyramid");
+    resize_pyramid_kernel<<<(unsigned)divup(n, 256), 256, 0, stream()>>>(vmap0, nmap0, cols, vmap1, nmap1, c1, r1, vmap2,
+                                                                          nmap2, c2, r2);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }

@@ -12,7 +12,7 @@
 namespace kfx {
 
 // Voxels of one column that a thread handles; grid.z = Nz / kIntegrateZRun.
-constexpr int kIntegrateZRun = 64;
+constexpr int kIntegrateZRun = 32;
 // Frustum side planes are pushed out by this many pixels and the clipped z
 // range by one voxel on each side, so the clip never removes a voxel the
 // per-voxel projection test would accept.
@@ -283,7 +283,7 @@ __device__ __forceinline__ bool segment_culled(const IntegrateParams& p, const P
     if (maxd == 0.0f) return true;
     // Nearest point of the segment to the camera, bounded below.
     const float dz = (f.gza <= 0.0f && f.gzb >= 0.0f) ? 0.0f : fminf(fabsf(f.gza), fabsf(f.gzb));
-    const float dmin = sqrtf(gx * gx + gy * gy + dz * dz);
+    const float dmin = sqrt_rn(gx * gx + gy * gy + dz * dz);
     return dmin > (maxd + p.trunc) * 1.0001f + 1e-4f;
 }
 
@@ -353,7 +353,7 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
             const bool in = (z <= hi) & (camz > 0.0f) & (fu >= 0.0f) & (fu < fcols) & (fv >= 0.0f) & (fv < frows);
             const int pix = (int)(in ? fv : 0.0f) * p.cols + (int)(in ? fu : 0.0f);
             dp[k] = scaled[pix];
-            dist[k] = sqrtf(gz * gz + part);
+            dist[k] = sqrt_rn(gz * gz + part);
             inimg[k] = in;
         }
         // Phase 2: the update test, and the loads of the voxels it selects.
@@ -498,8 +498,8 @@ __device__ __forceinline__ bool color_segment_culled(const IntegrateParams& p, c
     if (rmax == 0.0f) return true;  // no valid vertex under the footprint
     const float part = gx * gx + gy * gy;
     const float dz = (f.gza <= 0.0f && f.gzb >= 0.0f) ? 0.0f : fminf(fabsf(f.gza), fabsf(f.gzb));
-    const float dmin = sqrtf(part + dz * dz);
-    const float dmax = sqrtf(part + fmaxf(f.gza * f.gza, f.gzb * f.gzb));
+    const float dmin = sqrt_rn(part + dz * dz);
+    const float dmax = sqrt_rn(part + fmaxf(f.gza * f.gza, f.gzb * f.gzb));
     const float slack = p.trunc * 1.0001f + 1e-4f;
     return dmin > rmax * 1.0001f + slack || dmax < rmin * 0.9999f - slack;
 }
@@ -570,7 +570,7 @@ integrate_colors_kernel(const kfx_color* __restrict__ rgb, const kfx_xyz* __rest
         for (int k = 0; k < kIntegrateBatch; ++k) {
             const float vgz = ((float)(z0 + k) + 0.5f) * p.csz;
             const float dx = px[k] - vgx, dy = py[k] - vgy, dz = pz[k] - vgz;
-            const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+            const float dist = sqrt_rn(dx * dx + dy * dy + dz * dz);
             upd[k] = inimg[k] & !isnan(px[k]) & (dist < p.trunc);
             uint32_t rz = (uint32_t)(z0 + k) + (uint32_t)p.oz;
             rz = min(rz, rz - nz);

@@ -215,6 +215,8 @@ _PROTOS = {
     "kfx_raycast_count_samples": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), C.POINTER(Point3f), C.POINTER(Matrix3f),
                                        C.c_float, C.POINTER(Intrinsics), C.POINTER(Point3f),
                                        C.POINTER(C.c_uint64)]),
+    "kfx_maps_pyramid": (_I, [_P, _P, _P, C.POINTER(Intrinsics), _I]),
+    "kfx_resize_pyramid": (_I, [_P, _P, _I, _I, _P, _P, _P, _P]),
     "kfx_integrate_colors": (_I, [_P, _P, _P, C.POINTER(Intrinsics), C.POINTER(Matrix3f), C.POINTER(Point3f),
                                   C.c_float, _I, C.POINTER(Point3f), C.POINTER(RoundBuffer)]),
     "kfx_integrate_colors_posed": (_I, [_P, _P, _P, C.POINTER(Intrinsics), _P, C.c_float, _I, C.POINTER(Point3f),

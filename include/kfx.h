@@ -210,6 +210,18 @@ int kfx_transform_maps(const kfx_xyz* vmap_src, const kfx_xyz* nmap_src, const k
  * (opencl_wrappers.h:37-47, opencl_wrappers.cpp:97-146): 2x2 mean, NaN if any
  * of the four is invalid; out is (in_cols/2) x (in_rows/2). */
 int kfx_resize_map(const kfx_xyz* in, int in_cols, int in_rows, kfx_xyz* out, int normalize);
+/* Vertex and normal maps of `levels` pyramid levels (at most 3) in one launch:
+ * for each level l, kfx_vertex_map(depth[l], vmap[l], &intr[l]) then
+ * kfx_normal_map(vmap[l], nmap[l], ...), bit for bit (no reference
+ * counterpart: one launch instead of the 2 x levels of Create_Vertex_Maps and
+ * Create_Normal_Maps). */
+int kfx_maps_pyramid(const uint16_t* const* depth, kfx_xyz* const* vmap, kfx_xyz* const* nmap,
+                     const kfx_intrinsics* intr, int levels);
+/* Levels 1 and 2 of a vertex map and a normal map from level 0 (cols x rows,
+ * multiples of 4) in one launch: kfx_resize_map twice per map, bit for bit
+ * (no reference counterpart). */
+int kfx_resize_pyramid(const kfx_xyz* vmap0, const kfx_xyz* nmap0, int cols, int rows, kfx_xyz* vmap1,
+                       kfx_xyz* nmap1, kfx_xyz* vmap2, kfx_xyz* nmap2);
 
 /* ------------------------------------------------------------------------
  * ICP normal equations and pose update.

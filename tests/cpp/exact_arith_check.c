@@ -1,0 +1,70 @@
+/* exact_arith_check.c — CPU proof of the division-free exact arithmetic in the
+ * HIP kernels: each fast sequence is compared, for every input of its range,
+ * with the IEEE operation it replaces (glibc's sqrtf and float division are
+ * correctly rounded).  The hardware approximations (v_sqrt_f32, one ulp) are
+ * modelled by every value within one ulp of the exact result.
+ *
+ *   sqrt_rn     (kfx_common.h): all x in [2^-100, 2^100), 3 starting values
+ *   tsdf_decode (kfx_common.h): all 65536 int16 values, v / 32767
+ *   div_cell    (kfx_raycast.hip): p / c for every float |p| in [2^-20, 2^8)
+ *               and each cell size c of the test and benchmark volumes
+ *
+ * Prints one line per check with its mismatch count; exit status 1 if any. */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+static float fb(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static uint32_t bf(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+static float sqrt_rn(float x, float s) {
+    const float sdn = fb(bf(s) - 1u), sup = fb(bf(s) + 1u);
+    const float rdn = fmaf(-sdn, s, x), rup = fmaf(-sup, s, x);
+    float r = s;
+    if (rdn <= 0.0f) r = sdn;
+    if (rup > 0.0f) r = sup;
+    return r;
+}
+
+int main(void) {
+    long bad_sqrt = 0, n_sqrt = 0;
+    const uint32_t lo = 27u << 23, hi = 227u << 23; /* [2^-100, 2^100) */
+#pragma omp parallel for reduction(+ : bad_sqrt, n_sqrt) schedule(static, 1 << 16)
+    for (long u = lo; u < (long)hi; ++u) {
+        const float x = fb((uint32_t)u), ref = sqrtf(x);
+        for (int d = -1; d <= 1; ++d) {
+            bad_sqrt += sqrt_rn(x, fb(bf(ref) + (uint32_t)d)) != ref;
+            ++n_sqrt;
+        }
+    }
+    printf("sqrt_rn: %ld cases, %ld mismatches\n", n_sqrt, bad_sqrt);
+
+    long bad_dec = 0;
+    const float y = 1.0f / 32767.0f;
+    for (int v = -32768; v <= 32767; ++v) {
+        const float a = (float)v, q0 = a * y, r = fmaf(-q0, 32767.0f, a), q = fmaf(r, y, q0);
+        bad_dec += q != a / 32767.0f;
+    }
+    printf("tsdf_decode: 65536 cases, %ld mismatches\n", bad_dec);
+
+    const float cells[] = {3.072f / 512.0f, 3.0f / 512.0f, 3.0f / 256.0f, 3.0f / 128.0f, 3.0f / 64.0f,
+                           1.0f / 64.0f,    1.0f / 32.0f,  1.28f / 64.0f, 1.2f / 64.0f,  1.2f / 32.0f,
+                           6.144f / 1024.0f};
+    long bad_div = 0, n_div = 0;
+    for (unsigned ci = 0; ci < sizeof(cells) / sizeof(cells[0]); ++ci) {
+        const float c = cells[ci], ic = 1.0f / c;
+        const uint32_t plo = 0x35800000u, phi = 0x43800000u; /* [2^-20, 2^8) */
+#pragma omp parallel for reduction(+ : bad_div, n_div) schedule(static, 1 << 16)
+        for (long u = plo; u < (long)phi; ++u) {
+            for (int s = 0; s < 2; ++s) {
+                const float p = s ? -fb((uint32_t)u) : fb((uint32_t)u);
+                const float q0 = p * ic, r = fmaf(-q0, c, p), q = fmaf(r, ic, q0);
+                bad_div += q != p / c;
+                ++n_div;
+            }
+        }
+    }
+    printf("div_cell: %ld cases, %ld mismatches\n", n_div, bad_div);
+    return (bad_sqrt || bad_dec || bad_div) ? 1 : 0;
+}

@@ -134,3 +134,56 @@ def test_transform_and_resize_bit_exact(gpu, orc, c1_frame):
         assert np.array_equal(np.isnan(got), np.isnan(want))
         ok = ~np.isnan(want)
         assert np.array_equal(got[ok], want[ok])
+
+
+def test_maps_pyramid_equals_per_level_kernels(gpu, orc, c1_frame):
+    # One launch for every level's vertex and normal maps: the same bits as
+    # kfx_vertex_map + kfx_normal_map level by level (vertices also against
+    # the oracle).
+    kfx = gpu
+    intr, d = c1_frame
+    lvl = [d]
+    for i in range(1, 3):
+        lvl.append(orc.pyr_down(lvl[-1]))
+    intrs = (kfx.Intrinsics * 3)(*[intr.level(i) for i in range(3)])
+    db = [kfx.Buffer.from_array(x) for x in lvl]
+    vb = [kfx.Buffer(x.size * 12) for x in lvl]
+    nb = [kfx.Buffer(x.size * 12) for x in lvl]
+    ptrs = lambda bufs: (C.c_void_p * 3)(*[b.ptr.value for b in bufs])
+    kfx.call("kfx_maps_pyramid", ptrs(db), ptrs(vb), ptrs(nb), intrs, 3)
+    for i in range(3):
+        shape = lvl[i].shape + (3,)
+        rv, rn = kfx.Buffer(lvl[i].size * 12), kfx.Buffer(lvl[i].size * 12)
+        kfx.call("kfx_vertex_map", db[i].ptr, rv.ptr, C.byref(intrs[i]))
+        kfx.call("kfx_normal_map", rv.ptr, rn.ptr, intrs[i].cols, intrs[i].rows)
+        for got_b, want_b in ((vb[i], rv), (nb[i], rn)):
+            got, want = got_b.download(np.float32, shape), want_b.download(np.float32, shape)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"level {i}"
+        want_v = orc.vertex_map(lvl[i], intrs[i])
+        got_v = vb[i].download(np.float32, shape)
+        assert np.array_equal(np.isnan(got_v), np.isnan(want_v))
+        ok = ~np.isnan(want_v)
+        assert np.array_equal(got_v[ok], want_v[ok])
+
+
+def test_resize_pyramid_equals_two_resizes(gpu, orc, c1_frame):
+    kfx = gpu
+    intr, d = c1_frame
+    vm = orc.vertex_map(d, intr)
+    nm = orc.normal_map_f64(vm)
+    W, H = intr.cols, intr.rows
+    vb, nb = kfx.Buffer.from_array(vm), kfx.Buffer.from_array(nm)
+    v1, n1 = kfx.Buffer(vm.nbytes // 4), kfx.Buffer(vm.nbytes // 4)
+    v2, n2 = kfx.Buffer(vm.nbytes // 16), kfx.Buffer(vm.nbytes // 16)
+    kfx.call("kfx_resize_pyramid", vb.ptr, nb.ptr, W, H, v1.ptr, n1.ptr, v2.ptr, n2.ptr)
+    for src, normalize, o1, o2, m in ((vb, 0, v1, v2, vm), (nb, 1, n1, n2, nm)):
+        a, b = kfx.Buffer(vm.nbytes // 4), kfx.Buffer(vm.nbytes // 16)
+        kfx.call("kfx_resize_map", src.ptr, W, H, a.ptr, normalize)
+        kfx.call("kfx_resize_map", a.ptr, W // 2, H // 2, b.ptr, normalize)
+        for got_b, want_b, shape in ((o1, a, (H // 2, W // 2, 3)), (o2, b, (H // 4, W // 4, 3))):
+            got, want = got_b.download(np.float32, shape), want_b.download(np.float32, shape)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        w2 = orc.resize_map(orc.resize_map(m, normalize), normalize)
+        g2 = o2.download(np.float32, (H // 4, W // 4, 3))
+        ok = ~np.isnan(w2)
+        assert np.array_equal(np.isnan(g2), np.isnan(w2)) and np.array_equal(g2[ok], w2[ok])
