@@ -231,6 +231,154 @@ __global__ void __launch_bounds__(256) dn_l4_fwd(const uint16_t* __restrict__ de
 }
 
 // ---------------------------------------------------------------------------
+// Inference: the four layers in one launch.  A workgroup owns a tile of
+// kDfTX x kDfTY output pixels and computes, in LDS, layer 1 on the tile and
+// a 3-pixel halo, layer 2 on a 2-pixel halo, layer 3 on a 1-pixel halo, then
+// layer 4 and the output depth; the hidden activations never leave the CU.
+// Each value is computed with the per-layer kernels' operations in their
+// order (the same MFMA sequence for the hidden layers), and positions
+// outside the image hold 0 (the zero padding), so the output equals the
+// four-launch forward bit for bit.  At 640x480 on MI355X: 33 us against
+// 49 us for the four launches with 32 x 16 tiles and 512 threads (52 KB of
+// LDS, three workgroups per CU, 600 workgroups); 64 x 16 tiles (96 KB, one
+// workgroup per CU) took 63 us, 256 threads per 32 x 16 tile 41 us.
+// ---------------------------------------------------------------------------
+constexpr int kDfTX = 32, kDfTY = 16, kDfThreads = 512;
+constexpr int kDfDX = kDfTX + 8, kDfDY = kDfTY + 8;   // depth tile (4-pixel halo)
+constexpr int kDfA1X = kDfTX + 6, kDfA1Y = kDfTY + 6; // layer-1 outputs
+constexpr int kDfA2X = kDfTX + 4, kDfA2Y = kDfTY + 4; // layer-2 outputs
+constexpr int kDfA3X = kDfTX + 2, kDfA3Y = kDfTY + 2; // layer-3 outputs (in the layer-1 buffer)
+static_assert(kDfA3X * kDfA3Y <= kDfA1X * kDfA1Y, "layer 3 reuses the layer-1 buffer");
+
+// One hidden layer on the tile: out (ox x oy positions, image origin gx0,
+// gy0) from in (ox + 2 wide), 16 consecutive positions of a row per MFMA
+// group, dn_mid_fwd's arithmetic.
+__device__ __forceinline__ void df_mid_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int ox,
+                                             int oy, int gx0, int gy0, int W, int H, const v4s (&wa)[9],
+                                             const float* __restrict__ b) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, kq = (lane >> 4) * 4;
+    const int segs_x = (ox + 15) / 16;
+    const int ix = ox + 2;
+    for (int g = wave; g < segs_x * oy; g += kDfThreads / 64) {
+        const int sy = g / segs_x, sx = (g % segs_x) * 16;
+        const int px = min(sx + n, ox - 1);  // the partial last group reads a valid position
+        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+            const uint2 u = *reinterpret_cast<const uint2*>(in + ((sy + ky) * ix + px + kx) * kDnC + kq);
+            v4s bb;
+            bb[0] = (short)(u.x & 0xffffu);
+            bb[1] = (short)(u.x >> 16);
+            bb[2] = (short)(u.y & 0xffffu);
+            bb[3] = (short)(u.y >> 16);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[tap], bb, acc, 0, 0, 0);
+        }
+        if (sx + n >= ox) continue;
+        const int gx = gx0 + sx + n, gy = gy0 + sy;
+        uint32_t lo = 0, hi = 0;
+        if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+            float v[4];
+            for (int i = 0; i < 4; ++i) {
+                const float z = acc[i] + b[kq + i];
+                v[i] = z > 0.0f ? z : 0.0f;
+            }
+            lo = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+            hi = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        }
+        *reinterpret_cast<uint2*>(out + (sy * ox + sx + n) * kDnC + kq) = make_uint2(lo, hi);
+    }
+}
+
+__global__ void __launch_bounds__(kDfThreads) dn_fused_fwd(const uint16_t* __restrict__ depth, int W, int H,
+                                                           const DnPacked* __restrict__ pk,
+                                                           float* __restrict__ ypred, uint16_t* __restrict__ out) {
+    __shared__ uint16_t sd[kDfDY * kDfDX];
+    __shared__ __align__(16) uint16_t s1[kDfA1Y * kDfA1X * kDnC];
+    __shared__ __align__(16) uint16_t s2[kDfA2Y * kDfA2X * kDnC];
+    const int x0 = blockIdx.x * kDfTX, y0 = blockIdx.y * kDfTY;
+    for (int i = threadIdx.x; i < kDfDX * kDfDY; i += kDfThreads) {
+        const int qx = x0 - 4 + i % kDfDX, qy = y0 - 4 + i / kDfDX;
+        sd[i] = (qx >= 0 && qy >= 0 && qx < W && qy < H) ? depth[qy * W + qx] : (uint16_t)0;
+    }
+    __syncthreads();
+    // Layer 1 (dn_l1_fwd per position; 0 outside the image).
+    for (int i = threadIdx.x; i < kDfA1X * kDfA1Y; i += kDfThreads) {
+        const int lx = i % kDfA1X, ly = i / kDfA1X;
+        const int x = x0 - 3 + lx, y = y0 - 3 + ly;
+        uint32_t packed[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (x >= 0 && y >= 0 && x < W && y < H) {
+            const uint16_t dc = sd[(ly + 1) * kDfDX + lx + 1];
+            const float wc = dc ? inv_depth(dc) : 0.0f;
+            const float inv_sigma = dc ? 1.0f / noise_sigma_inv(dc) : 0.0f;
+            float in[2][9];
+            for (int ky = 0; ky < 3; ++ky)
+                for (int kx = 0; kx < 3; ++kx) {
+                    const uint16_t d = sd[(ly + ky) * kDfDX + lx + kx];
+                    in[0][ky * 3 + kx] = dc ? rel_depth(d, wc, inv_sigma) : 0.0f;
+                    in[1][ky * 3 + kx] = d ? 1.0f : 0.0f;
+                }
+            for (int co = 0; co < 16; co += 2) {
+                float z[2];
+                for (int h = 0; h < 2; ++h) {
+                    float s = pk->b1[co + h];
+                    for (int ci = 0; ci < 2; ++ci)
+                        for (int t = 0; t < 9; ++t) s += pk->w1[((co + h) * 2 + ci) * 9 + t] * in[ci][t];
+                    z[h] = s > 0.0f ? s : 0.0f;
+                }
+                packed[co / 2] = (uint32_t)f2bf(z[0]) | ((uint32_t)f2bf(z[1]) << 16);
+            }
+        }
+        uint4* dst = reinterpret_cast<uint4*>(s1 + i * kDnC);
+        dst[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+        dst[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
+    }
+    // Hidden-layer weights of this lane (the A operands of its 9 MFMAs).
+    const int lane = threadIdx.x & 63;
+    const int n = lane & 15, kq = (lane >> 4) * 4;
+    v4s wa[9];
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) wa[tap] = wload4(&pk->w2[tap][n][kq]);
+    __syncthreads();
+    df_mid_layer(s1, s2, kDfA2X, kDfA2Y, x0 - 2, y0 - 2, W, H, wa, pk->b2);
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) wa[tap] = wload4(&pk->w3[tap][n][kq]);
+    __syncthreads();
+    df_mid_layer(s2, s1, kDfA3X, kDfA3Y, x0 - 1, y0 - 1, W, H, wa, pk->b3);
+    __syncthreads();
+    // Layer 4 and the output (dn_l4_fwd per pixel).
+    for (int i = threadIdx.x; i < kDfTX * kDfTY; i += kDfThreads) {
+        const int lx = i % kDfTX, ly = i / kDfTX;
+        const int x = x0 + lx, yy = y0 + ly;
+        if (x >= W || yy >= H) continue;
+        const int p = yy * W + x;
+        const uint16_t d = sd[(ly + 4) * kDfDX + lx + 4];
+        float s = pk->b4;
+        for (int ky = 0; ky < 3; ++ky)
+            for (int kx = 0; kx < 3; ++kx) {
+                const int qx = x + kx - 1, qy = yy + ky - 1;
+                if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
+                const uint4* src = reinterpret_cast<const uint4*>(s1 + ((ly + ky) * kDfA3X + lx + kx) * kDnC);
+                const uint4 u0 = src[0], u1 = src[1];
+                const uint32_t words[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+                const int t = ky * 3 + kx;
+                for (int c = 0; c < 16; ++c) {
+                    const uint16_t h = (uint16_t)((words[c >> 1] >> ((c & 1) * 16)) & 0xffffu);
+                    s += pk->w4[c * 9 + t] * bf2f(h);
+                }
+            }
+        const float y = d ? dn_output(d, s) : 0.0f;
+        if (ypred) ypred[p] = y;
+        if (out) {
+            int mm = 0;
+            if (d) mm = (int)fminf(fmaxf(rintf(y * 1000.0f), 1.0f), 65535.0f);
+            out[p] = (uint16_t)mm;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Training.
 // ---------------------------------------------------------------------------
 
@@ -608,6 +756,15 @@ int dn_forward(kfx_denoiser* d, const uint16_t* depth, uint16_t* out, float* ypr
     return KFX_OK;
 }
 
+// Inference forward in one launch (same output as dn_forward).
+int dn_forward_fused(kfx_denoiser* d, const uint16_t* depth, uint16_t* out, float* ypred) {
+    ProfScope prof("denoise_forward");
+    dim3 grid(divup(d->W, kDfTX), divup(d->H, kDfTY));
+    dn_fused_fwd<<<grid, kDfThreads, 0, stream()>>>(depth, d->W, d->H, d->pk, ypred, out);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
 int dn_pack(kfx_denoiser* d) {
     dn_pack_kernel<<<divup(2304, 256), 256, 0, stream()>>>(d->master, d->pk);
     KFX_CHECK_LAUNCH();
@@ -696,6 +853,12 @@ int kfx_denoiser_get_params(kfx_denoiser* d, float* host_params) {
 int kfx_denoiser_forward(kfx_denoiser* d, const uint16_t* depth, uint16_t* out_depth, float* out_metres) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(d && depth, "kfx_denoiser_forward: null pointer");
+    return dn_forward_fused(d, depth, out_depth, out_metres);
+}
+
+int kfx_denoiser_forward_layers(kfx_denoiser* d, const uint16_t* depth, uint16_t* out_depth, float* out_metres) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(d && depth, "kfx_denoiser_forward_layers: null pointer");
     return dn_forward(d, depth, out_depth, out_metres);
 }
 
@@ -867,7 +1030,8 @@ __global__ void __launch_bounds__(256) depth_consistency_kernel(const kfx_xyz* _
 // valid vertices within max_diff.
 __global__ void __launch_bounds__(256) consistency_reduce_kernel(const double* __restrict__ partials, int nblk,
                                                                  double* __restrict__ stats, double fallback_m,
-                                                                 int32_t* __restrict__ flag) {
+                                                                 int32_t* __restrict__ flag,
+                                                                 int32_t* __restrict__ host_flag) {
     __shared__ double red[4][256];
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (int g = threadIdx.x; g < nblk; g += 256)
@@ -883,7 +1047,9 @@ __global__ void __launch_bounds__(256) consistency_reduce_kernel(const double* _
     if (threadIdx.x == 0 && flag != nullptr) {
         const double sse = red[0][0], cnt = red[1][0], n_valid = red[3][0];
         const double rms = cnt > 0.0 ? sqrt(sse / cnt) : 0.0;
-        *flag = (fallback_m > 0.0 && n_valid >= 100.0 && (rms > fallback_m || cnt < 0.5 * n_valid)) ? 1 : 0;
+        const int32_t f = (fallback_m > 0.0 && n_valid >= 100.0 && (rms > fallback_m || cnt < 0.5 * n_valid)) ? 1 : 0;
+        *flag = f;
+        if (host_flag != nullptr) *host_flag = f;
     }
 }
 
@@ -904,7 +1070,7 @@ int consistency_reserve(int cols, int rows) {
 
 int depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
                       const kfx_intrinsics* intr, float max_diff, double* dev_stats, double fallback_m,
-                      int32_t* dev_flag) {
+                      int32_t* dev_flag, int32_t* host_flag) {
     const int P = intr->cols * intr->rows;
     const int nblk = (int)divup((unsigned)P, 256u);
     if ((size_t)nblk > g_cons_cap) {
@@ -917,7 +1083,7 @@ int depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const ui
     }
     depth_consistency_kernel<<<nblk, 256, 0, stream()>>>(vmap_model, dev_pose, depth, *intr, max_diff, g_cons);
     KFX_CHECK_LAUNCH();
-    consistency_reduce_kernel<<<1, 256, 0, stream()>>>(g_cons, nblk, dev_stats, fallback_m, dev_flag);
+    consistency_reduce_kernel<<<1, 256, 0, stream()>>>(g_cons, nblk, dev_stats, fallback_m, dev_flag, host_flag);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
@@ -942,7 +1108,7 @@ int kfx_depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, cons
                           const kfx_intrinsics* intr, float max_diff_m, double* dev_stats) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(vmap_model && dev_pose && depth && intr && dev_stats, "kfx_depth_consistency: null pointer");
-    return depth_consistency(vmap_model, dev_pose, depth, intr, max_diff_m, dev_stats, 0.0, nullptr);
+    return depth_consistency(vmap_model, dev_pose, depth, intr, max_diff_m, dev_stats, 0.0, nullptr, nullptr);
 }
 
 }  // extern "C"

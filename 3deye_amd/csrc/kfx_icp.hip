@@ -509,7 +509,23 @@ icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials, i
     double acc[32];
 #pragma unroll
     for (int e = 0; e < 32; ++e) acc[e] = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += kIcpSolveThreads) {
+    int b = threadIdx.x;
+    if constexpr (M != kSumDouble) {
+        // Two workgroups' totals per round, their loads issued together.
+        const float2* pf = static_cast<const float2*>(partials);
+        for (; b + kIcpSolveThreads < nblk; b += 2 * kIcpSolveThreads) {
+            float2 t0[kIcpEntries], t1[kIcpEntries];
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) t0[e] = pf[b * kIcpEntries + e];
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) t1[e] = pf[(b + kIcpSolveThreads) * kIcpEntries + e];
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t0[e].x + (double)t0[e].y;
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t1[e].x + (double)t1[e].y;
+        }
+    }
+    for (; b < nblk; b += kIcpSolveThreads) {
         if constexpr (M == kSumDouble) {
             const double* pd = static_cast<const double*>(partials) + b * kIcpEntries;
             double t[kIcpEntries];

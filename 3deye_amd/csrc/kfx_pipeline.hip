@@ -23,7 +23,7 @@
 namespace kfx {
 int depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
                       const kfx_intrinsics* intr, float max_diff, double* dev_stats, double fallback_m,
-                      int32_t* dev_flag);
+                      int32_t* dev_flag, int32_t* host_flag);
 }
 
 
@@ -54,10 +54,9 @@ struct kfx_pipeline {
     hipGraph_t graph = nullptr;           // whole frame (bilateral) or ICP part (denoiser)
     hipGraphExec_t graph_exec = nullptr;
     kfx_denoiser* denoiser = nullptr;
-    double* dstats = nullptr;             // consistency sums (device)
-    double* hstats = nullptr;             // pinned copy
+    double* hstats = nullptr;             // gate statistics (pinned host memory the gate kernel writes)
     int32_t* dgate = nullptr;             // fallback decision of the gate (device)
-    int32_t* hgate = nullptr;             // pinned copy
+    int32_t* hgate = nullptr;             // host mirror of dgate (pinned, written by the gate kernel)
     uint64_t total_extracted = 0;
     int shifts = 0;
     int fallbacks = 0;
@@ -272,8 +271,11 @@ int learned_frame(kfx_pipeline* p) {
     KFX_TRY(icp_part(p, true));
     const float *dpose = nullptr, *dprev = nullptr;
     KFX_TRY(kfx_icp_pose_device(p->icp, &dpose, &dprev));
-    KFX_TRY(depth_consistency(p->vp[0], dpose, p->dep[0], &p->lvl[0], 0.1f, p->dstats,
-                              1e-3 * (double)p->cfg.fallback_rms_mm, p->dgate));
+    // The gate's statistics go straight to pinned host memory and its flag
+    // both to the device (read by the skips below) and to the host: no copy
+    // commands after the graph.
+    KFX_TRY(depth_consistency(p->vp[0], dpose, p->dep[0], &p->lvl[0], 0.1f, p->hstats,
+                              1e-3 * (double)p->cfg.fallback_rms_mm, p->dgate, p->hgate));
     // The raycast below overwrites vp[0]: the gate has read it already.
     return map_part_skip(p, p->dgate);
 }
@@ -444,7 +446,6 @@ int kfx_pipeline_destroy(kfx_pipeline* p) {
     (void)hipFree(p->mbuf);
     (void)hipFree(p->ext_pts);
     (void)hipFree(p->ext_col);
-    (void)hipFree(p->dstats);
     if (p->hstats) (void)hipHostFree(p->hstats);
     (void)hipFree(p->dgate);
     if (p->hgate) (void)hipHostFree(p->hgate);
@@ -503,7 +504,6 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     p->ext_cap = 1u << 20;
     if (e == hipSuccess) e = hipMalloc((void**)&p->ext_pts, sizeof(kfx_xyzi) * p->ext_cap);
     if (e == hipSuccess) e = hipMalloc((void**)&p->ext_col, sizeof(kfx_color) * p->ext_cap);
-    if (e == hipSuccess) e = hipMalloc((void**)&p->dstats, sizeof(double) * 4);
     if (e == hipSuccess) e = hipHostMalloc((void**)&p->hstats, sizeof(double) * 4, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void**)&p->dgate, sizeof(int32_t));
     if (e == hipSuccess) e = hipHostMalloc((void**)&p->hgate, sizeof(int32_t), hipHostMallocDefault);
@@ -564,13 +564,12 @@ static int process_frame(kfx_pipeline* p, const uint16_t* depth, const kfx_color
         KFX_TRY(run_part(p, bilateral_frame, &p->graph, &p->graph_exec));
     } else {
         // The learned frame as one graph; the gate's statistics and decision
-        // come back with the pose read below (one round trip per frame).
+        // are in pinned host memory when the pose read below returns (one
+        // round trip per frame).
         // When the gate fell back, the device skipped integration and
         // raycast: the frame is redone with the bilateral filter, ICP from
         // the previous pose, then integration and raycast.
         KFX_TRY(run_part(p, learned_frame, &p->graph, &p->graph_exec));
-        KFX_CHECK_HIP(hipMemcpyAsync(p->hstats, p->dstats, sizeof(double) * 4, hipMemcpyDeviceToHost, stream()));
-        KFX_CHECK_HIP(hipMemcpyAsync(p->hgate, p->dgate, sizeof(int32_t), hipMemcpyDeviceToHost, stream()));
         KFX_TRY(kfx_icp_read(p->icp, &r));  // synchronises the stream
         have_pose = true;
         const double cnt = p->hstats[1];

@@ -270,6 +270,7 @@ _PROTOS = {
     "kfx_denoiser_set_params": (_I, [_P, _P]),
     "kfx_denoiser_get_params": (_I, [_P, _P]),
     "kfx_denoiser_forward": (_I, [_P, _P, _P, _P]),
+    "kfx_denoiser_forward_layers": (_I, [_P, _P, _P, _P]),
     "kfx_denoiser_accumulate": (_I, [_P, _P, _P, C.c_float, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "kfx_denoiser_get_grad": (_I, [_P, _P]),
     "kfx_denoiser_zero_grad": (_I, [_P]),

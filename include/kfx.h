@@ -379,6 +379,10 @@ int kfx_denoiser_get_params(kfx_denoiser* dn, float* host_params);
 /* Forward pass on a device depth frame: out_depth (uint16 mm) and/or
  * out_metres (float) may be NULL. */
 int kfx_denoiser_forward(kfx_denoiser* dn, const uint16_t* depth, uint16_t* out_depth, float* out_metres);
+/* The same forward pass layer by layer (four launches, the hidden
+ * activations through HBM, as training runs it): kfx_denoiser_forward's
+ * output bit for bit, for checking the one-launch inference kernel. */
+int kfx_denoiser_forward_layers(kfx_denoiser* dn, const uint16_t* depth, uint16_t* out_depth, float* out_metres);
 /* Forward + backward on one (raw depth, target depth in metres) pair: adds
  * the gradient of the frame's mean squared error (pixels with a valid input,
  * a target > 0 and |input - target| < outlier_m) to the gradient buffer.

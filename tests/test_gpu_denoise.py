@@ -64,6 +64,32 @@ def test_forward_matches_oracle(gpu, orc, dn):
     assert np.max(np.abs(mm[ok].astype(np.float64) - np.rint(yo[ok] * 1000))) <= 1
 
 
+@pytest.mark.parametrize("size", [(72, 40), (640, 480), (130, 33)])
+def test_fused_forward_equals_layer_by_layer(gpu, size):
+    # The one-launch inference kernel (tiles of 32 x 16 with halos in LDS)
+    # against the four-launch forward: identical bits, including the image
+    # borders and sizes that are not multiples of the tile.
+    kfx = gpu
+    W, H = size
+    rng = np.random.default_rng(W + H)
+    d, _ = _frame(rng, W, H)
+    h = C.c_void_p()
+    kfx.call("kfx_denoiser_create", W, H, C.byref(h))
+    for params in (_params(rng), kfx.Denoiser.load_params()):
+        kfx.call("kfx_denoiser_set_params", h, np.ascontiguousarray(params, np.float32).ctypes.data_as(C.c_void_p))
+        dd = kfx.Buffer.from_array(d)
+        outs = []
+        for name in ("kfx_denoiser_forward", "kfx_denoiser_forward_layers"):
+            ym, mm = kfx.Buffer(d.size * 4), kfx.Buffer(d.size * 2)
+            kfx.call(name, h, dd.ptr, mm.ptr, ym.ptr)
+            outs.append((ym.download(np.float32, d.shape), mm.download(np.uint16, d.shape)))
+        assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+        assert np.array_equal(outs[0][1], outs[1][1])
+        # The network ran: the output differs from the input depth in metres.
+        assert np.count_nonzero(np.abs(outs[0][0] - d.astype(np.float32) * 1e-3) > 1e-6) > 0
+    kfx.call("kfx_denoiser_destroy", h)
+
+
 def test_zero_params_identity(gpu, dn):
     kfx = gpu
     rng = np.random.default_rng(1)
