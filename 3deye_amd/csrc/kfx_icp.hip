@@ -183,7 +183,7 @@ __global__ void clean_buff_kernel(float* __restrict__ g, int width) {
 // ------------------------------------------------------------------------
 // Fused path.
 // ------------------------------------------------------------------------
-constexpr int kIcpThreads = 256;
+constexpr int kIcpThreads = 512;
 constexpr int kIcpWaves = kIcpThreads / 64;
 constexpr int kIcpEntries = 27;
 constexpr unsigned kIcpTwoPixelMin = 150000;  // images at least this large take two pixels per thread
@@ -265,10 +265,13 @@ __device__ __forceinline__ float2 wave_reduce_scatter32(float (&s)[32], float (&
     return make_float2(a, e);
 }
 
-// 6x6 Cholesky solve in double precision, then the pose update.  Every loop
-// has constant bounds, so the arrays stay in registers; the six pivots are
-// inverted once and every other division is a multiplication.
-__device__ void solve_and_update(IcpState* st, const double* sums, int tag, double conv_rot, double conv_trans) {
+// 6x6 Cholesky solve in double precision, then the pose update of the
+// estimate R0, t0 (st->R, st->t, which the caller loaded ahead, so their
+// latency hides behind the sums).  Every loop has constant bounds, so the
+// arrays stay in registers; the six pivots are inverted once and every other
+// division is a multiplication.
+__device__ void solve_and_update(IcpState* st, const double* sums, const double (&R0)[9], const double (&t0)[3],
+                                 int iterations, int tag, double conv_rot, double conv_trans) {
     double A[36], b[6];
     {
         int k = 0;
@@ -303,7 +306,7 @@ __device__ void solve_and_update(IcpState* st, const double* sums, int tag, doub
                 L[i * 6 + j] = s * inv_diag[j];
             }
         }
-    st->iterations += 1;
+    st->iterations = iterations + 1;
     if (!ok) {
         st->status = KFX_ERR_SINGULAR;
         st->singular_count += 1;
@@ -338,11 +341,7 @@ __device__ void solve_and_update(IcpState* st, const double* sums, int tag, doub
     const double Ri[9] = {cg * cb, cg * sb * sa - sg * ca, cg * sb * ca + sg * sa,
                           sg * cb, sg * sb * sa + cg * ca, sg * sb * ca - cg * sa,
                           -sb, cb * sa, cb * ca};
-    double R0[9], t0[3], R[9], t[3];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) R0[i] = st->R[i];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) t0[i] = st->t[i];
+    double R[9], t[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
@@ -401,38 +400,14 @@ __device__ __forceinline__ float2 workgroup_sum27(float (&s)[32], float (&c)[32]
     return make_float2(a, e);
 }
 
-// One ICP iteration, first half: the 27 sums of each workgroup.  Thread t of
-// workgroup g handles pixels g * 256 * P + q * 256 + t (q < P): the
-// current-frame loads of all P pixels are issued before any projection, and
-// the previous-frame gathers of all P before any residual, so a wave waits on
-// memory twice.  The single-precision products of a pixel (icp_row's
-// arithmetic) are accumulated per thread (mode M), then over the workgroup in
-// a fixed order.  Workgroups never wait for each other: the cross-workgroup
-// sum is the next launch (icp_solve_kernel), so no workgroup needs a
-// device-wide fence or a shared counter.  Partials: [workgroup][27] float2
-// (sum, compensation) in the FP32 modes, double in the FP64 mode.
-template <int P, int M>
-__global__ void __launch_bounds__(kIcpThreads)
-icp_sums_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
-                const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev, IcpFrame f,
-                const IcpState* __restrict__ st, void* __restrict__ partials, int tag) {
-    __shared__ double wsum_d[kIcpWaves][32];
-    __shared__ float2 wsum_f[kIcpWaves][32];
-    if (tag >= 0 && st->converged_tag == tag) return;  // uniform across the grid
-    if (f.prev_from_state) {
-        for (int i = 0; i < 9; ++i) f.rprev_inv[i] = st->Rpi[i];
-        for (int i = 0; i < 3; ++i) f.tprev[i] = st->tp[i];
-    }
-    float Rc[9], tc[3];
-    for (int i = 0; i < 9; ++i) Rc[i] = st->Rf[i];
-    for (int i = 0; i < 3; ++i) tc[i] = st->tf[i];
-
-    const int npix = f.cols * f.rows;
+// Current-frame loads of a workgroup's P pixels (they do not depend on the
+// pose): thread t of workgroup g takes pixels g * kIcpThreads * P +
+// q * kIcpThreads + t (q < P).
+template <int P>
+__device__ __forceinline__ void icp_load_current(const kfx_xyz* __restrict__ vcurr,
+                                                 const kfx_xyz* __restrict__ ncurr, int npix, kfx_xyz (&v)[P],
+                                                 kfx_xyz (&n)[P]) {
     const int base = blockIdx.x * (kIcpThreads * P) + threadIdx.x;
-    kfx_xyz v[P], n[P], vp[P], np[P];
-    float g[P][3];
-    bool ok[P];
-    int j[P];
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const int p = base + q * kIcpThreads;
@@ -440,6 +415,28 @@ icp_sums_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
         n[q] = ncurr[ps];
         v[q] = vcurr[ps];
     }
+}
+
+// The 27 sums of the workgroup's pixels at the pose (Rc, tc): the
+// projections of all P pixels, then the previous-frame gathers of all P
+// before any residual, so a wave waits on memory once more.  The
+// single-precision products of a pixel (icp_row's arithmetic) are
+// accumulated per thread (mode M), then over the workgroup in a fixed order.
+// Partials: [27][workgroup] float2 (sum, compensation) in the FP32 modes,
+// double in the FP64 mode; entry-major, so the cross-workgroup sum's loads of
+// one entry over consecutive workgroups coalesce.
+template <int P, int M>
+__device__ __forceinline__ void icp_block_sums(const kfx_xyz (&v)[P], const kfx_xyz (&n)[P],
+                                               const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev,
+                                               const IcpFrame& f, const float* Rc, const float* tc,
+                                               void* __restrict__ partials, double (*wsum_d)[32],
+                                               float2 (*wsum_f)[32]) {
+    const int npix = f.cols * f.rows;
+    const int base = blockIdx.x * (kIcpThreads * P) + threadIdx.x;
+    kfx_xyz vp[P], np[P];
+    float g[P][3];
+    bool ok[P];
+    int j[P];
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const int p = base + q * kIcpThreads;
@@ -467,7 +464,7 @@ icp_sums_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
                 for (int jj = i; jj < 7; ++jj, ++k) acc[k] += (double)(rows[q][i] * rows[q][jj]);
         }
         const double s = workgroup_sum27(acc, wsum_d);
-        if (threadIdx.x < kIcpEntries) static_cast<double*>(partials)[blockIdx.x * kIcpEntries + threadIdx.x] = s;
+        if (threadIdx.x < kIcpEntries) static_cast<double*>(partials)[threadIdx.x * gridDim.x + blockIdx.x] = s;
     } else {
         float sum[32], comp[32];
 #pragma unroll
@@ -487,65 +484,124 @@ icp_sums_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
         float2 r;
         if constexpr (M == kSumKahan) r = workgroup_sum27(sum, comp, wsum_f);
         else r = make_float2(workgroup_sum27(sum, reinterpret_cast<float (*)[32]>(wsum_f)), 0.0f);
-        if (threadIdx.x < kIcpEntries) static_cast<float2*>(partials)[blockIdx.x * kIcpEntries + threadIdx.x] = r;
+        if (threadIdx.x < kIcpEntries) static_cast<float2*>(partials)[threadIdx.x * gridDim.x + blockIdx.x] = r;
     }
 }
 
-// One ICP iteration, second half (one workgroup): thread t adds the totals of
-// workgroups t, t + 256, ... in double precision in order, each round issuing
-// the 27 loads of one workgroup's totals together; then the fixed-order
-// workgroup sum in double precision, the Cholesky solve and the pose update.
-// The result does not depend on the order in which the first half's
-// workgroups ran.  (A 1024-thread variant that reads all partials in one
-// round measured slightly slower at 640x480.)
-constexpr int kIcpSolveThreads = 256;
-template <int M>
-__global__ void __launch_bounds__(kIcpSolveThreads)
-icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials, int nblk, int tag,
-                 double conv_rot, double conv_trans) {
-    __shared__ double wsum[kIcpSolveThreads / 64][32];
-    __shared__ double total[32];
-    if (tag >= 0 && st->converged_tag == tag) return;
+__device__ __forceinline__ void icp_prev_from_state(IcpFrame& f, const IcpState* st) {
+    if (f.prev_from_state) {
+        for (int i = 0; i < 9; ++i) f.rprev_inv[i] = st->Rpi[i];
+        for (int i = 0; i < 3; ++i) f.tprev[i] = st->tp[i];
+    }
+}
+
+// One ICP iteration, first half: the 27 sums of each workgroup at the
+// state's pose.  Workgroups never wait for each other: the cross-workgroup
+// sum is the next launch (icp_solve_kernel), so no workgroup needs a
+// device-wide fence or a shared counter.
+template <int P, int M>
+__global__ void __launch_bounds__(kIcpThreads)
+icp_sums_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
+                const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev, IcpFrame f,
+                const IcpState* __restrict__ st, void* __restrict__ partials, int tag) {
+    __shared__ double wsum_d[kIcpWaves][32];
+    __shared__ float2 wsum_f[kIcpWaves][32];
+    if (tag >= 0 && st->converged_tag == tag) return;  // uniform across the grid
+    icp_prev_from_state(f, st);
+    float Rc[9], tc[3];
+    for (int i = 0; i < 9; ++i) Rc[i] = st->Rf[i];
+    for (int i = 0; i < 3; ++i) tc[i] = st->tf[i];
+    kfx_xyz v[P], n[P];
+    icp_load_current<P>(vcurr, ncurr, f.cols * f.rows, v, n);
+    icp_block_sums<P, M>(v, n, vprev, nprev, f, Rc, tc, partials, wsum_d, wsum_f);
+}
+
+// The cross-workgroup sum of nblk workgroups' partials into total[0..26]
+// (LDS), by a workgroup of kIcpThreads threads: thread t adds the totals of
+// workgroups t, t + kIcpThreads, ... in double precision in that order (up
+// to kRound workgroups' loads issued together per round), then the
+// fixed-order workgroup sum.  The result depends on nblk and the partials
+// only, not on kRound or on the order in which the workgroups ran.
+template <int M, int kRound>
+__device__ __forceinline__ void cross_block_sum(const void* __restrict__ partials, int nblk, double (*wsum)[32],
+                                                double* total) {
     double acc[32];
 #pragma unroll
     for (int e = 0; e < 32; ++e) acc[e] = 0.0;
     int b = threadIdx.x;
-    if constexpr (M != kSumDouble) {
-        // Two workgroups' totals per round, their loads issued together.
+    if constexpr (M != kSumDouble && kRound >= 3) {
         const float2* pf = static_cast<const float2*>(partials);
-        for (; b + kIcpSolveThreads < nblk; b += 2 * kIcpSolveThreads) {
+        for (; b + 2 * kIcpThreads < nblk; b += 3 * kIcpThreads) {
+            float2 t0[kIcpEntries], t1[kIcpEntries], t2[kIcpEntries];
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) t0[e] = pf[e * nblk + b];
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) t1[e] = pf[e * nblk + b + kIcpThreads];
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) t2[e] = pf[e * nblk + b + 2 * kIcpThreads];
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t0[e].x + (double)t0[e].y;
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t1[e].x + (double)t1[e].y;
+#pragma unroll
+            for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t2[e].x + (double)t2[e].y;
+        }
+    }
+    if constexpr (M != kSumDouble && kRound >= 2) {
+        const float2* pf = static_cast<const float2*>(partials);
+        for (; b + kIcpThreads < nblk; b += 2 * kIcpThreads) {
             float2 t0[kIcpEntries], t1[kIcpEntries];
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t0[e] = pf[b * kIcpEntries + e];
+            for (int e = 0; e < kIcpEntries; ++e) t0[e] = pf[e * nblk + b];
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t1[e] = pf[(b + kIcpSolveThreads) * kIcpEntries + e];
+            for (int e = 0; e < kIcpEntries; ++e) t1[e] = pf[e * nblk + b + kIcpThreads];
 #pragma unroll
             for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t0[e].x + (double)t0[e].y;
 #pragma unroll
             for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t1[e].x + (double)t1[e].y;
         }
     }
-    for (; b < nblk; b += kIcpSolveThreads) {
+    for (; b < nblk; b += kIcpThreads) {
         if constexpr (M == kSumDouble) {
-            const double* pd = static_cast<const double*>(partials) + b * kIcpEntries;
+            const double* pd = static_cast<const double*>(partials) + b;
             double t[kIcpEntries];
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t[e] = pd[e];
+            for (int e = 0; e < kIcpEntries; ++e) t[e] = pd[e * nblk];
 #pragma unroll
             for (int e = 0; e < kIcpEntries; ++e) acc[e] += t[e];
         } else {
-            const float2* pf = static_cast<const float2*>(partials) + b * kIcpEntries;
+            const float2* pf = static_cast<const float2*>(partials) + b;
             float2 t[kIcpEntries];
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t[e] = pf[e];
+            for (int e = 0; e < kIcpEntries; ++e) t[e] = pf[e * nblk];
 #pragma unroll
             for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t[e].x + (double)t[e].y;
         }
     }
-    const double s = workgroup_sum27<kIcpSolveThreads / 64>(acc, wsum);
+    const double s = workgroup_sum27<kIcpWaves>(acc, wsum);
     if (threadIdx.x < kIcpEntries) total[threadIdx.x] = s;
     __syncthreads();
-    if (threadIdx.x == 0) solve_and_update(st, total, tag, conv_rot, conv_trans);
+}
+
+// One ICP iteration, second half (one workgroup): the cross-workgroup sum
+// (three workgroups' loads per round: all of them in one round at 640x480),
+// then thread 0's Cholesky solve and pose update.  The estimate is loaded
+// before the sum so that its latency hides behind the partials' loads.
+template <int M>
+__global__ void __launch_bounds__(kIcpThreads)
+icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials, int nblk, int tag,
+                 double conv_rot, double conv_trans) {
+    __shared__ double wsum[kIcpWaves][32];
+    __shared__ double total[32];
+    if (tag >= 0 && st->converged_tag == tag) return;
+    double R0[9], t0[3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R0[i] = st->R[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) t0[i] = st->t[i];
+    const int iterations = st->iterations;
+    cross_block_sum<M, 3>(partials, nblk, wsum, total);
+    if (threadIdx.x == 0) solve_and_update(st, total, R0, t0, iterations, tag, conv_rot, conv_trans);
 }
 
 // Start of a frame: the current estimate becomes the previous pose (as R^T,
@@ -603,7 +659,7 @@ IcpFrame make_frame(const kfx_matrix_3f* rprev_inv, const kfx_point_3f* tprev, c
 
 struct kfx_icp {
     kfx::IcpState* state;
-    void* partials;    // [workgroup][27] sums of the fused iteration (float2 or double)
+    void* partials;    // [27][workgroup] sums of the fused iteration (float2 or double)
     int sum_mode;      // KFX_ICP_SUM_*
     kfx::IcpState* host_state;  // pinned host copy for kfx_icp_read
     int max_blocks;
@@ -618,7 +674,7 @@ static void icp_iterate_launch(kfx_icp* h, const kfx_xyz* vc, const kfx_xyz* nc,
         icp_sums_kernel<2, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
     else
         icp_sums_kernel<1, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
-    icp_solve_kernel<M><<<1, kIcpSolveThreads, 0, stream()>>>(h->state, h->partials, nblk, tag, conv_rot,
+    icp_solve_kernel<M><<<1, kIcpThreads, 0, stream()>>>(h->state, h->partials, nblk, tag, conv_rot,
                                                                 conv_trans);
 }
 
@@ -682,6 +738,7 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
     hipError_t e = hipMalloc((void**)&h->state, sizeof(IcpState));
     h->sum_mode = KFX_ICP_SUM_FP32_KAHAN;
     h->host_state = nullptr;
+    h->partials = nullptr;
     if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(double) * kIcpEntries * (size_t)h->max_blocks);
     if (e == hipSuccess) e = hipHostMalloc((void**)&h->host_state, sizeof(IcpState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(h->state, 0, sizeof(IcpState), stream());

@@ -146,3 +146,28 @@ def test_fused_singular_keeps_pose_and_early_exit(gpu, orc, pair):
     kfx.call("kfx_icp_read", h, C.byref(res))
     assert res.iterations == 1 and res.converged_tag == 7
     kfx.call("kfx_icp_destroy", h)
+
+
+@pytest.mark.parametrize("size", [(1280, 720), (1600, 1200)])
+@pytest.mark.parametrize("mode", [0, 2])
+def test_fused_iteration_large_images(gpu, orc, size, mode):
+    # 900 and 1875 workgroups of partial sums: the one-workgroup solve then
+    # loads two or three workgroups' partials per round before the remainder.
+    kfx = gpu
+    W, H = size
+    intr = kfx.kinect_intrinsics(W, H, 525.0 * W / 640)
+    maps, (Rp, tp), _ = ii.icp_pair(orc, intr, noise=True)
+    bufs = _upload(kfx, maps)
+    h = _icp_handle(kfx, W * H)
+    kfx.call("kfx_icp_set_sum_mode", h, mode)
+    kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+    kfx.call("kfx_icp_iterate", h, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, C.byref(kfx.matrix3f(Rp.T)),
+             C.byref(kfx.point3f(tp)), C.byref(intr), ii.DIST_THR, ii.ANGLE_THR, -1, 0.0, 0.0)
+    res = kfx.IcpResult()
+    kfx.call("kfx_icp_read", h, C.byref(res))
+    s, n = orc.icp_sums(*maps, Rp, tp, Rp.T, tp, intr, ii.DIST_THR, ii.ANGLE_THR, precision=0)
+    assert n > 10000
+    A, b = orc.unpack_sums(s)
+    np.testing.assert_allclose(np.array(res.A[:]).reshape(6, 6), A, rtol=1e-9, atol=1e-9 * np.abs(A).max())
+    np.testing.assert_allclose(np.array(res.b[:]), b, rtol=1e-9, atol=1e-9 * np.abs(b).max())
+    kfx.call("kfx_icp_destroy", h)
