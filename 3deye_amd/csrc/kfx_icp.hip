@@ -265,6 +265,22 @@ __device__ __forceinline__ float2 wave_reduce_scatter32(float (&s)[32], float (&
     return make_float2(a, e);
 }
 
+// sin and cos of an ICP increment angle.  For |x| <= 1/8 the Taylor series
+// (sin to x^9, cos to x^10; the first omitted terms are below 3e-17 relative)
+// in Horner form with fused multiply-adds; larger angles take sincos.
+__device__ __forceinline__ void small_sincos(double x, double& s, double& c) {
+    if (fabs(x) <= 0.125) {
+        const double z = x * x;
+        const double ps = fma(z, fma(z, fma(z, 1.0 / 362880.0, -1.0 / 5040.0), 1.0 / 120.0), -1.0 / 6.0);
+        s = fma(x * z, ps, x);
+        const double pc = fma(z, fma(z, fma(z, fma(z, -1.0 / 3628800.0, 1.0 / 40320.0), -1.0 / 720.0), 1.0 / 24.0),
+                              -0.5);
+        c = fma(z, pc, 1.0);
+    } else {
+        sincos(x, &s, &c);
+    }
+}
+
 // 6x6 Cholesky solve in double precision, then the pose update of the
 // estimate R0, t0 (st->R, st->t, which the caller loaded ahead, so their
 // latency hides behind the sums).  Every loop has constant bounds, so the
@@ -300,8 +316,14 @@ __device__ void solve_and_update(IcpState* st, const double* sums, const double 
             for (int p = 0; p < j; ++p) s -= L[i * 6 + p] * L[j * 6 + p];
             if (i == j) {
                 ok = ok && (s > 1e-300) && isfinite(s);
-                L[i * 6 + i] = sqrt(ok ? s : 1.0);
-                inv_diag[i] = 1.0 / L[i * 6 + i];
+                const double sv = ok ? s : 1.0;
+                double y = __builtin_amdgcn_rsq(sv);
+                for (int nr = 0; nr < 2; ++nr) {
+                    const double r = fma(-(sv * y), y, 1.0);
+                    y = fma(0.5 * y, r, y);
+                }
+                L[i * 6 + i] = sv * y;
+                inv_diag[i] = y;
             } else {
                 L[i * 6 + j] = s * inv_diag[j];
             }
@@ -335,9 +357,9 @@ __device__ void solve_and_update(IcpState* st, const double* sums, const double 
     // Increment: R_inc = Rz(x2) Ry(x1) Rx(x0), t_inc = x[3..5];
     // t <- R_inc t + t_inc, R <- R_inc R.
     double sa, ca, sb, cb, sg, cg;
-    sincos(x[0], &sa, &ca);
-    sincos(x[1], &sb, &cb);
-    sincos(x[2], &sg, &cg);
+    small_sincos(x[0], sa, ca);
+    small_sincos(x[1], sb, cb);
+    small_sincos(x[2], sg, cg);
     const double Ri[9] = {cg * cb, cg * sb * sa - sg * ca, cg * sb * ca + sg * sa,
                           sg * cb, sg * sb * sa + cg * ca, sg * sb * ca - cg * sa,
                           -sb, cb * sa, cb * ca};
@@ -506,13 +528,15 @@ icp_sums_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
                 const IcpState* __restrict__ st, void* __restrict__ partials, int tag) {
     __shared__ double wsum_d[kIcpWaves][32];
     __shared__ float2 wsum_f[kIcpWaves][32];
+    // The current-frame loads do not depend on the state: they are issued
+    // before the convergence test and the pose reads.
+    kfx_xyz v[P], n[P];
+    icp_load_current<P>(vcurr, ncurr, f.cols * f.rows, v, n);
     if (tag >= 0 && st->converged_tag == tag) return;  // uniform across the grid
     icp_prev_from_state(f, st);
     float Rc[9], tc[3];
     for (int i = 0; i < 9; ++i) Rc[i] = st->Rf[i];
     for (int i = 0; i < 3; ++i) tc[i] = st->tf[i];
-    kfx_xyz v[P], n[P];
-    icp_load_current<P>(vcurr, ncurr, f.cols * f.rows, v, n);
     icp_block_sums<P, M>(v, n, vprev, nprev, f, Rc, tc, partials, wsum_d, wsum_f);
 }
 

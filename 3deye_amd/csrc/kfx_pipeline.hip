@@ -236,11 +236,15 @@ int icp_plain(kfx_pipeline* p, kfx_frame_info* info) {
 // Fused ICP from the state's previous pose.  Enqueues only.
 int icp_part(kfx_pipeline* p, bool begin) {
     if (begin) KFX_TRY(kfx_icp_begin_frame(p->icp));
+    // Level tags only with the convergence test on: an untagged iteration
+    // skips the kernels' read of the convergence state (a memory round trip
+    // ahead of their loads).
+    const bool early_exit = p->cfg.converge_rotation > 0.0;
     for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l)
         for (int it = 0; it < p->cfg.icp_iterations[l]; ++it)
             KFX_TRY(kfx_icp_iterate(p->icp, p->vc[l], p->nc[l], p->vp[l], p->np[l], nullptr, nullptr, &p->lvl[l],
-                                    p->cfg.distance_threshold, p->cfg.angle_threshold, l, p->cfg.converge_rotation,
-                                    p->cfg.converge_translation));
+                                    p->cfg.distance_threshold, p->cfg.angle_threshold, early_exit ? l : -1,
+                                    p->cfg.converge_rotation, p->cfg.converge_translation));
     return KFX_OK;
 }
 
