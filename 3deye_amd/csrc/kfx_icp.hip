@@ -183,7 +183,7 @@ __global__ void clean_buff_kernel(float* __restrict__ g, int width) {
 // ------------------------------------------------------------------------
 // Fused path.
 // ------------------------------------------------------------------------
-constexpr int kIcpThreads = 512;
+constexpr int kIcpThreads = 256;
 constexpr int kIcpWaves = kIcpThreads / 64;
 constexpr int kIcpEntries = 27;
 constexpr unsigned kIcpTwoPixelMin = 150000;  // images at least this large take two pixels per thread
