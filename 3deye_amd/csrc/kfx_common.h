@@ -124,6 +124,23 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return r;
 }
 
+// The quotient a / (w + 1) of the TSDF running average, for weights w in
+// [0, 128], without a division: y = RN(1/b) from v_rcp_f32 (within one ulp)
+// and one Newton step with FMAs (correct from every starting value within an
+// ulp, for every b in [1, 129]), then Markstein's correction q0 = a y,
+// q = q0 + (a - q0 b) y with the exact FMA residual.  Equal to the IEEE
+// quotient for every |a| in [2^-100, 129] and b in [2, 129] (b = 1 is
+// exact); for |a| < 2^-100 both quotients are below 1/32767 in magnitude,
+// so tsdf_encode maps either to 0.  Checked exhaustively on the CPU by
+// tests/cpp/exact_arith_check.c.
+__device__ __forceinline__ float tsdf_average_quotient(float a, int b) {
+    const float fb = (float)b;
+    const float y0 = __builtin_amdgcn_rcpf(fb);
+    const float y = fmaf(fmaf(-fb, y0, 1.0f), y0, y0);
+    const float q0 = a * y;
+    return fmaf(fmaf(-q0, fb, a), y, q0);
+}
+
 // Voxel codec (raycast.cl:9-27): decode value / 32767, encode rtz then clamp.
 // The decode is the correctly rounded quotient, formed without a division:
 // q0 = v * RN(1/32767), then one FMA correction with the exact residual

@@ -377,7 +377,7 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
             if (!upd[k]) continue;
             const float tp = tsdf_decode(prev[k].value);
             const int w = prev[k].weight;
-            const float avg = (tp * (float)w + tsdf_new[k]) / (float)(w + 1);
+            const float avg = tsdf_average_quotient(tp * (float)w + tsdf_new[k], w + 1);
             const int w_new = min(w + 1, KFX_MAX_TSDF_WEIGHT);
             const int16_t v_new = tsdf_encode(avg);
             if (v_new != prev[k].value || w_new != w) {

@@ -8,6 +8,11 @@
  *   tsdf_decode (kfx_common.h): all 65536 int16 values, v / 32767
  *   div_cell    (kfx_raycast.hip): p / c for every float |p| in [2^-20, 2^8)
  *               and each cell size c of the test and benchmark volumes
+ *   rcp_weight  (tsdf_average_quotient, kfx_common.h): RN(1/b) for b in
+ *               [1, 129] from v_rcp_f32 (3 starting values) and one Newton step
+ *   avg_div     (tsdf_average_quotient): a / b for b in [2, 129] and every
+ *               float a in [2^-100, 129] (the quotient is odd in a, so
+ *               positive a suffice)
  *
  * Prints one line per check with its mismatch count; exit status 1 if any. */
 #include <math.h>
@@ -66,5 +71,29 @@ int main(void) {
         }
     }
     printf("div_cell: %ld cases, %ld mismatches\n", n_div, bad_div);
-    return (bad_sqrt || bad_dec || bad_div) ? 1 : 0;
+
+    long bad_rcp = 0;
+    for (int b = 1; b <= 129; ++b) {
+        const float x = (float)b, ref = 1.0f / x;
+        for (int d = -1; d <= 1; ++d) {
+            const float y0 = fb(bf(ref) + (uint32_t)d);
+            bad_rcp += fmaf(fmaf(-x, y0, 1.0f), y0, y0) != ref;
+        }
+    }
+    printf("rcp_weight: 387 cases, %ld mismatches\n", bad_rcp);
+
+    long bad_avg = 0, n_avg = 0;
+    for (int b = 2; b <= 129; ++b) {
+        const float fbv = (float)b, y = 1.0f / fbv;
+        const uint32_t alo = 27u << 23, ahi = 0x43010001u; /* [2^-100, 129] */
+#pragma omp parallel for reduction(+ : bad_avg, n_avg) schedule(static, 1 << 16)
+        for (long u = alo; u < (long)ahi; ++u) {
+            const float a = fb((uint32_t)u);
+            const float q0 = a * y, q = fmaf(fmaf(-q0, fbv, a), y, q0);
+            bad_avg += q != a / fbv;
+            ++n_avg;
+        }
+    }
+    printf("avg_div: %ld cases, %ld mismatches\n", n_avg, bad_avg);
+    return (bad_sqrt || bad_dec || bad_div || bad_rcp || bad_avg) ? 1 : 0;
 }
