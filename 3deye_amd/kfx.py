@@ -87,6 +87,22 @@ class IcpResult(C.Structure):
                 ("converged_tag", C.c_int32)]
 
 
+class RgbdParams(C.Structure):
+    _fields_ = [("max_depth_diff", C.c_float), ("min_gradient", C.c_float), ("levels", C.c_int32),
+                ("iterations", C.c_int32 * 4)]
+
+
+class RgbdResult(C.Structure):
+    _fields_ = [("pose", Pose), ("A", C.c_double * 36), ("b", C.c_double * 6), ("sigma", C.c_double),
+                ("correspondences", C.c_int32), ("solves", C.c_int32), ("ok", C.c_int32), ("reserved", C.c_int32)]
+
+
+def rgbd_params(max_depth_diff=0.07, min_gradient=10.0, levels=4, iterations=(7, 7, 7, 10)):
+    """Defaults of the OpenCV RGBD odometry the reference follows."""
+    it = list(iterations) + [0] * (4 - len(iterations))
+    return RgbdParams(max_depth_diff, min_gradient, levels, (C.c_int32 * 4)(*it))
+
+
 class Xyzi(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
 
@@ -245,6 +261,13 @@ _PROTOS = {
     "kfx_icp_iterate": (_I, [_P, _P, _P, _P, _P, C.POINTER(Matrix3f), C.POINTER(Point3f), C.POINTER(Intrinsics),
                              C.c_float, C.c_float, _I, C.c_double, C.c_double]),
     "kfx_icp_read": (_I, [_P, C.POINTER(IcpResult)]),
+    "kfx_rgbd_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
+    "kfx_rgbd_destroy": (_I, [_P]),
+    "kfx_rgbd_set_frame": (_I, [_P, _I, _P, _P]),
+    "kfx_rgbd_swap": (_I, [_P]),
+    "kfx_rgbd_download_level": (_I, [_P, _I, _I, _P, _P, _P, _P]),
+    "kfx_rgbd_estimate": (_I, [_P, C.POINTER(Intrinsics), C.POINTER(RgbdParams), C.POINTER(Pose),
+                               C.POINTER(RgbdResult)]),
     "kfx_icp_begin_frame": (_I, [_P]),
     "kfx_icp_translate": (_I, [_P, C.c_double, C.c_double, C.c_double]),
     "kfx_icp_pose_device": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),

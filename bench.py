@@ -207,6 +207,51 @@ def cpu_baseline(intr, depths, poses, origin, frames=12):
                       f"19 ICP iterations, integrate, raycast), first {frames} frames of the 512^3 workload"}
 
 
+def rgbd_odometry(intr, scene, poses, depths, pairs=60, warmup=5):
+    """RGBD photometric odometry (kfx_rgbd) on consecutive frames of the walk,
+    with rendered colour: pairs per second (frame preparation, 4 levels x
+    7/7/7/10 iterations from the identity, blocking result) and the error of
+    each pair's motion against the rendered motion."""
+    n = pairs + warmup + 1
+    rgb = kfx.Buffer.from_array(np.stack([kfx.render_rgb(scene, poses[k], intr) for k in range(n)]))
+    dep = kfx.Buffer.from_array(depths[:n])
+    cb, db = W * H * 4, W * H * 2
+    h = C.c_void_p()
+    kfx.check(kfx.lib.kfx_rgbd_create(W, H, 4, C.byref(h)))
+    prm = kfx.rgbd_params()
+    ident = kfx.pose_from(np.eye(3), np.zeros(3))
+    res = kfx.RgbdResult()
+
+    def frame(slot, k):
+        kfx.check(kfx.lib.kfx_rgbd_set_frame(h, slot, C.c_void_p(rgb.ptr.value + k * cb),
+                                             C.c_void_p(dep.ptr.value + k * db)))
+
+    frame(0, 0)
+    et, er, solves = [], [], []
+    t0 = None
+    for k in range(1, n):
+        if k == warmup + 1:
+            kfx.synchronize()
+            t0 = time.perf_counter()
+        frame(1, k)
+        kfx.check(kfx.lib.kfx_rgbd_estimate(h, C.byref(intr), C.byref(prm), C.byref(ident), C.byref(res)))
+        if k > warmup:
+            R0, tv0 = poses[k - 1].matrices()
+            R1, tv1 = poses[k].matrices()
+            Rt, tt = R1.T @ R0, R1.T @ (np.asarray(tv0) - np.asarray(tv1))
+            R = np.array(res.pose.R[:]).reshape(3, 3)
+            et.append(float(np.linalg.norm(np.array(res.pose.t[:]) - tt)))
+            er.append(float(np.arccos(np.clip((np.trace(R @ Rt.T) - 1.0) / 2.0, -1.0, 1.0))))
+            solves.append(res.solves)
+        kfx.check(kfx.lib.kfx_rgbd_swap(h))
+    el = time.perf_counter() - t0
+    kfx.check(kfx.lib.kfx_rgbd_destroy(h))
+    return {"pairs_per_s": pairs / el, "ms_per_pair": 1e3 * el / pairs, "pairs": pairs,
+            "trans_err_median_m": float(np.median(et)), "rot_err_median_rad": float(np.median(er)),
+            "solves_per_pair": float(np.mean(solves)),
+            "config": "640x480 walk frames 1-%d, 4 levels x 7/7/7/10, identity start" % (n - 1)}
+
+
 def kernel_probe(frames=60, first=20):
     """Short, fixed workload for hardware-counter runs (rocprofv3 --pmc):
     three 256 MiB streaming copies (known traffic, for calibrating the
@@ -258,6 +303,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-learned", action="store_true", help="no denoiser run (headline = fused bilateral)")
+    ap.add_argument("--skip-rgbd", action="store_true", help="no RGBD odometry measurement")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="short fixed workload for hardware-counter runs (no JSON benchmark line)")
     args = ap.parse_args()
@@ -346,6 +392,7 @@ def main():
         kern["raycast"]["samples_per_launch"] = round(s_avg)
         kern["raycast"]["samples_counted_frames"] = s_frames
 
+    rgbd = None if args.skip_rgbd else rgbd_odometry(intr, scene, poses, depths)
     base = None if args.no_cpu_baseline else cpu_baseline(intr, depths, poses, origin)
     head = learned if learned is not None else fused
     el_head = 1e-3 * head["ms_per_frame"] * args.steps
@@ -375,6 +422,7 @@ def main():
         "ate_le_plain": (head["ate_m"] <= plain["ate_m"]) if plain else None,
         "roofline": roofline,
         "kernels_plain": kern,
+        "rgbd_odometry": rgbd,
         "cpu_baseline": base,
     }
     print(json.dumps(out))

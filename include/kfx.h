@@ -304,6 +304,50 @@ int kfx_icp_read(kfx_icp* icp, kfx_icp_result* out);
 int kfx_icp_pose_device(kfx_icp* icp, const float** dev_pose, const float** dev_prev_pose_inv);
 
 /* ------------------------------------------------------------------------
+ * RGBD photometric odometry (Estimate_Isometry_By_RGBD,
+ * common/src/opencl_wrappers.cpp:1328-1466; kernels
+ * scan_for_interest_point.cl, scan_list_of_valid_correspondence.cl,
+ * compute_least_squares_equation.cl).  The reference's path is not
+ * functional; this is the dense RGBD odometry it follows (OpenCV rgbd
+ * module): Sobel-textured pixels of the current frame paired with the
+ * previous frame by projection and a depth test, weights 1/(sigma + |diff|),
+ * Sobel scale 1/8, normal equations in double precision, Cholesky solve with
+ * the |det| >= 1e-6 test, estimate <- exp(twist) * estimate, levels coarsest
+ * first.  Frames: RGB (kfx_color) and depth (uint16 mm) of cols x rows.
+ * ------------------------------------------------------------------------ */
+typedef struct kfx_rgbd kfx_rgbd;
+typedef struct kfx_rgbd_params {
+    float max_depth_diff;   /* m: RGBD_Depth_Threshold (0.07 in OpenCV) */
+    float min_gradient;     /* Sobel magnitude of a textured pixel (10) */
+    int32_t levels;         /* pyramid levels used (<= the handle's) */
+    int32_t iterations[4];  /* per level, finest first (7, 7, 7, 10) */
+} kfx_rgbd_params;
+typedef struct kfx_rgbd_result {
+    kfx_pose pose;            /* previous camera -> current camera coordinates */
+    double A[36], b[6];       /* last normal equations */
+    double sigma;             /* last residual RMS, intensity units */
+    int32_t correspondences;  /* of the last iteration */
+    int32_t solves;
+    int32_t ok;               /* a solve succeeded (the reference's return value) */
+    int32_t reserved;
+} kfx_rgbd_result;
+/* levels 1..4; cols and rows at least 8 << (levels - 1). */
+int kfx_rgbd_create(int cols, int rows, int levels, kfx_rgbd** out);
+int kfx_rgbd_destroy(kfx_rgbd* rgbd);
+/* Intensity, metric depth and Sobel pyramids of a frame into slot 0
+ * (previous) or 1 (current); enqueues only. */
+int kfx_rgbd_set_frame(kfx_rgbd* rgbd, int slot, const kfx_color* rgb, const uint16_t* depth);
+/* The current frame becomes the previous one (no copy). */
+int kfx_rgbd_swap(kfx_rgbd* rgbd);
+/* Blocking copy of a prepared level (any output may be NULL). */
+int kfx_rgbd_download_level(kfx_rgbd* rgbd, int slot, int level, float* intensity, float* depth, float* dx,
+                            float* dy);
+/* The motion from the previous to the current frame, starting at init;
+ * blocking.  out->ok == 0 when no level produced a solution. */
+int kfx_rgbd_estimate(kfx_rgbd* rgbd, const kfx_intrinsics* intr, const kfx_rgbd_params* params, const kfx_pose* init,
+                      kfx_rgbd_result* out);
+
+/* ------------------------------------------------------------------------
  * Cyclic shift: black-zone clean and slice extraction.  A local voxel
  * (x, y, z) is in the black zone when, on any axis, x + origin_GRID.x lies in
  * [min_bounds.x, max_bounds.x] (inclusive, unwrapped; clean_color_slice.cl:29-40).

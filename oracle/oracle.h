@@ -99,6 +99,34 @@ void orc_estimate_combined_f32(const kfx_xyz* vc, const kfx_xyz* nc, const kfx_x
 /* Cholesky solve + KinectFusion pose increment in double; 1 if singular. */
 int orc_icp_solve_update(const double sums[27], double* R, double* t, double x_out[6]);
 
+/* RGBD photometric odometry (Estimate_Isometry_By_RGBD,
+ * common/src/opencl_wrappers.cpp:1328-1466, as the dense RGBD odometry of
+ * the OpenCV rgbd module that the reference's kernels follow; the reference
+ * path itself is not functional).  Pinned by: the twist exponential against
+ * scipy's matrix exponential; identity and pure-translation correspondence
+ * cases; recovery of the rendered motion between two frames.
+ *   orc_rgbd_prepare: level arrays of one frame, per level l (cols >> l by
+ *     rows >> l, n_l pixels) packed as [I, D, dx, dy] one after the other.
+ *   orc_rgbd_corr: per previous-frame pixel of a level, the current-frame
+ *     pixel paired with it (-1 none), for the estimate's inverse (Rinv, tinv)
+ *     in single precision; returns the number of pairs.
+ *   orc_rgbd_sums: sigma and the 27 double normal-equation sums (21 upper A
+ *     entries row by row, then b) for the pairs and the estimate (R, t).
+ *   orc_rgbd_twist_exp: R (9) and t (3) of exp of the twist ksi (6).
+ *   orc_rgbd_estimate: the whole estimate; returns ok (a solve succeeded). */
+void orc_rgbd_prepare(const kfx_color* rgb, const uint16_t* depth, int cols, int rows, int levels, float* out);
+int orc_rgbd_corr(const float* i_prev, const float* d_prev, const float* d_curr, const float* dx_curr,
+                  const float* dy_curr, const kfx_intrinsics* intr, const float* Rinv, const float* tinv,
+                  float min_gradient_sq, float max_depth_diff, int32_t* corr);
+int orc_rgbd_sums(const float* i_prev, const float* d_prev, const float* i_curr, const float* dx_curr,
+                  const float* dy_curr, const kfx_intrinsics* intr, const int32_t* corr, const double* R,
+                  const double* t, double* sigma, double sums[27]);
+void orc_rgbd_twist_exp(const double ksi[6], double R[9], double t[3]);
+int orc_rgbd_estimate(const kfx_color* rgb_prev, const uint16_t* depth_prev, const kfx_color* rgb_curr,
+                      const uint16_t* depth_curr, const kfx_intrinsics* intr, int levels, const int* iterations,
+                      float min_gradient, float max_depth_diff, const double* R0, const double* t0, double* R,
+                      double* t, int* solves);
+
 /* clean_color_slice.cl:41-67 pattern (TSDF or colour volume, elem_bytes 4). */
 void orc_clean_slice(const kfx_point_3i* mn, const kfx_point_3i* mx, const kfx_round_buffer* rb, void* vol,
                      int elem_bytes);

@@ -48,6 +48,12 @@ _PROTOS = {
     "orc_icp_sums": (_I, [_P, _P, _P, _P, PD, PD, PD, PD, IN, C.c_double, C.c_double, _I, PD]),
     "orc_estimate_combined_f32": (None, [_P, _P, _P, _P, _P, _P, _P, _P, IN, C.c_float, C.c_float, _P]),
     "orc_icp_solve_update": (_I, [PD, PD, PD, PD]),
+    "orc_rgbd_prepare": (None, [_P, _P, _I, _I, _I, _P]),
+    "orc_rgbd_corr": (_I, [_P, _P, _P, _P, _P, IN, _P, _P, C.c_float, C.c_float, _P]),
+    "orc_rgbd_sums": (_I, [_P, _P, _P, _P, _P, IN, _P, PD, PD, PD, PD]),
+    "orc_rgbd_twist_exp": (None, [PD, PD, PD]),
+    "orc_rgbd_estimate": (_I, [_P, _P, _P, _P, IN, _I, C.POINTER(_I), C.c_float, C.c_float, PD, PD, PD, PD,
+                               C.POINTER(_I)]),
     "orc_dn_param_count": (_I, []),
     "orc_dn_forward": (None, [_P, _I, _I, _P, _I, _P, _P]),
     "orc_dn_loss_grad": (C.c_double, [_P, _P, _I, _I, _P, _I, C.c_double, _P, C.POINTER(_I)]),
@@ -269,3 +275,65 @@ def dn_loss_grad(depth, target, params, bf16=True, outlier=0.05):
     loss = lib.orc_dn_loss_grad(ptr(depth), ptr(target), W, H, ptr(params), int(bf16), float(outlier), ptr(g),
                                 C.byref(n))
     return loss, g, n.value
+
+
+def rgbd_prepare(rgb, depth, levels):
+    """Level arrays of one frame: a list of (I, D, dx, dy) per level."""
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    depth = np.ascontiguousarray(depth, np.uint16)
+    H, W = depth.shape
+    sizes = [(H >> l, W >> l) for l in range(levels)]
+    out = np.zeros(sum(4 * h * w for h, w in sizes), np.float32)
+    lib.orc_rgbd_prepare(ptr(rgb), ptr(depth), W, H, levels, ptr(out))
+    res, off = [], 0
+    for h, w in sizes:
+        arrs = [out[off + k * h * w: off + (k + 1) * h * w].reshape(h, w) for k in range(4)]
+        res.append(tuple(arrs))
+        off += 4 * h * w
+    return res
+
+
+def rgbd_corr(prev, curr, intr, Rinv, tinv, min_gradient, max_depth_diff):
+    """prev, curr: (I, D, dx, dy) of one level; returns the pair map (-1 none) and the pair count."""
+    H, W = prev[1].shape
+    corr = np.zeros(H * W, np.int32)
+    Ri = np.ascontiguousarray(Rinv, np.float32).reshape(9)
+    ti = np.ascontiguousarray(tinv, np.float32).reshape(3)
+    n = lib.orc_rgbd_corr(ptr(prev[0]), ptr(prev[1]), ptr(curr[1]), ptr(curr[2]), ptr(curr[3]), C.byref(intr),
+                          ptr(Ri), ptr(ti), float(min_gradient) ** 2, float(max_depth_diff), ptr(corr))
+    return corr.reshape(H, W), n
+
+
+def rgbd_sums(prev, curr, intr, corr, R, t):
+    Rd = np.ascontiguousarray(R, np.float64).reshape(9)
+    td = np.ascontiguousarray(t, np.float64).reshape(3)
+    sigma = C.c_double()
+    s = np.zeros(27, np.float64)
+    c = np.ascontiguousarray(corr, np.int32)
+    n = lib.orc_rgbd_sums(ptr(prev[0]), ptr(prev[1]), ptr(curr[0]), ptr(curr[2]), ptr(curr[3]), C.byref(intr),
+                          ptr(c), Rd.ctypes.data_as(PD), td.ctypes.data_as(PD), C.byref(sigma),
+                          s.ctypes.data_as(PD))
+    return s, sigma.value, n
+
+
+def rgbd_twist_exp(ksi):
+    k = np.ascontiguousarray(ksi, np.float64)
+    R = np.zeros(9)
+    t = np.zeros(3)
+    lib.orc_rgbd_twist_exp(k.ctypes.data_as(PD), R.ctypes.data_as(PD), t.ctypes.data_as(PD))
+    return R.reshape(3, 3), t
+
+
+def rgbd_estimate(rgb_prev, depth_prev, rgb_curr, depth_curr, intr, levels, iterations, min_gradient,
+                  max_depth_diff, R0, t0):
+    it = (C.c_int * 4)(*(list(iterations) + [0] * (4 - len(iterations))))
+    R0d = np.ascontiguousarray(R0, np.float64).reshape(9)
+    t0d = np.ascontiguousarray(t0, np.float64).reshape(3)
+    R = np.zeros(9)
+    t = np.zeros(3)
+    solves = C.c_int()
+    arrs = [np.ascontiguousarray(a) for a in (rgb_prev, depth_prev, rgb_curr, depth_curr)]
+    ok = lib.orc_rgbd_estimate(ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]), ptr(arrs[3]), C.byref(intr), levels, it,
+                               float(min_gradient), float(max_depth_diff), R0d.ctypes.data_as(PD),
+                               t0d.ctypes.data_as(PD), R.ctypes.data_as(PD), t.ctypes.data_as(PD), C.byref(solves))
+    return ok, R.reshape(3, 3), t, solves.value
