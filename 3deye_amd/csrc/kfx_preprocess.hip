@@ -68,28 +68,36 @@ bilateral_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, i
 // neighbourhood of src(2x, 2y) with separable weights {1,4,6,4,1}/16 over the
 // samples within 3 sigma_r (90 mm) of the centre, truncated to integer mm.
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) pyr_down_kernel(const uint16_t* __restrict__ src, int scols, int srows,
-                                                       uint16_t* __restrict__ dst, int dcols, int drows,
-                                                       float sigma_color) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    if (x >= dcols || y >= drows) return;
+// One output pixel (x, y) of the 2x downsample from a source of scols x srows
+// read through at(xx, yy).
+template <typename At>
+__device__ __forceinline__ uint16_t pyr_down_px(At at, int scols, int srows, int x, int y, float sigma_color) {
     const float wt[3] = {0.375f, 0.25f, 0.0625f};
-    const int center = src[(2 * y) * scols + 2 * x];
+    const int center = at(2 * x, 2 * y);
     const int x_lo = max(0, 2 * x - 2) - 2 * x, y_lo = max(0, 2 * y - 2) - 2 * y;
     const int x_hi = min(scols, 2 * x + 3) - 2 * x, y_hi = min(srows, 2 * y + 3) - 2 * y;
     const float thr = 3.0f * sigma_color;
     float sum = 0.0f, wall = 0.0f;
     for (int yi = y_lo; yi < y_hi; ++yi)
         for (int xi = x_lo; xi < x_hi; ++xi) {
-            const int val = src[(2 * y + yi) * scols + 2 * x + xi];
+            const int val = at(2 * x + xi, 2 * y + yi);
             if ((float)abs(val - center) < thr) {
                 const float w = wt[abs(xi)] * wt[abs(yi)];
                 sum += (float)val * w;
                 wall += w;
             }
         }
-    dst[y * dcols + x] = (uint16_t)(int)(sum / wall);
+    return (uint16_t)(int)(sum / wall);
+}
+
+__global__ void __launch_bounds__(256) pyr_down_kernel(const uint16_t* __restrict__ src, int scols, int srows,
+                                                       uint16_t* __restrict__ dst, int dcols, int drows,
+                                                       float sigma_color) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= dcols || y >= drows) return;
+    dst[y * dcols + x] = pyr_down_px([&](int xx, int yy) { return (int)src[yy * scols + xx]; }, scols, srows, x, y,
+                                     sigma_color);
 }
 
 // --------------------------------------------------------------------------

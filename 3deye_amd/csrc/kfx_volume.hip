@@ -115,6 +115,8 @@ __global__ void __launch_bounds__(256) scale_depth_kernel(const uint16_t* __rest
     scaled[idx] = (float)depth[idx] * lambda / 1000.0f;
 }
 
+int scale_depth_launch(const uint16_t* depth, float* depth_scaled, const kfx_intrinsics* intr);
+
 // One voxel of the update rule.  Returns true when the voxel's stored value
 // changes and has to be written back.
 __device__ __forceinline__ bool integrate_voxel(const IntegrateParams& p, const PoseTerms& q, float gx, float gy,
@@ -179,16 +181,32 @@ __device__ __forceinline__ bool load_device_pose(const IntegrateParams& p, PoseT
     return true;
 }
 
-// Depth-bound pyramid: one 32 x 32 pixel tile per workgroup, levels 1..5 by
-// repeated 2 x 2 maxima in LDS.  Pixels outside the image count as 0 (no
-// depth).
+// The scaled depth of one pixel (scale_depth_kernel's arithmetic).
+__device__ __forceinline__ float scaled_depth(const uint16_t* __restrict__ depth, const kfx_intrinsics& intr, int u,
+                                              int v) {
+    const float xl = ((float)u - intr.cx) / intr.fx;
+    const float yl = ((float)v - intr.cy) / intr.fy;
+    const float lambda = sqrtf(xl * xl + yl * yl + 1.0f);
+    return (float)depth[v * intr.cols + u] * lambda / 1000.0f;
+}
+
+// Scaled depth and depth-bound pyramid in one launch: one 32 x 32 pixel tile
+// per workgroup writes its pixels' scaled depth (every pixel belongs to one
+// tile) and builds levels 1..5 by repeated 2 x 2 maxima in LDS.  Pixels
+// outside the image count as 0 (no depth).
 __global__ void __launch_bounds__(kBoundTile * kBoundTile)
-depth_bound_kernel(const float* __restrict__ scaled, int cols, int rows, float* __restrict__ bound,
-                   IntegrateParams p) {
+depth_bound_kernel(const uint16_t* __restrict__ depth, float* __restrict__ scaled, kfx_intrinsics intr,
+                   float* __restrict__ bound, IntegrateParams p) {
     __shared__ float t[kBoundTile * kBoundTile];
+    const int cols = intr.cols, rows = intr.rows;
     const int tx = threadIdx.x % kBoundTile, ty = threadIdx.x / kBoundTile;
     const int u = blockIdx.x * kBoundTile + tx, v = blockIdx.y * kBoundTile + ty;
-    t[threadIdx.x] = (u < cols && v < rows) ? scaled[v * cols + u] : 0.0f;
+    float d = 0.0f;
+    if (u < cols && v < rows) {
+        d = scaled_depth(depth, intr, u, v);
+        scaled[v * cols + u] = d;
+    }
+    t[threadIdx.x] = d;
     __syncthreads();
     for (int l = 1; l <= kBoundLevels; ++l) {
         const int n = kBoundTile >> l;  // texels per tile side at level l
@@ -681,23 +699,24 @@ int integrate_reserve(int cols, int rows) {
     return KFX_OK;
 }
 
-// Builds the depth-bound pyramid of depth_scaled into the library buffer and
-// points p at it.  Inside a stream capture the buffer cannot be allocated:
-// without a large enough reservation the integration then runs unculled
-// (same result, more work).
-static int integrate_bound(const float* depth_scaled, const kfx_intrinsics* intr, IntegrateParams& p) {
+// Scales the depth into depth_scaled and builds its depth-bound pyramid into
+// the library buffer (one launch), pointing p at it.  Inside a stream
+// capture the buffer cannot be allocated: without a large enough reservation
+// only the depth is scaled and the integration runs unculled (same result,
+// more work).
+static int integrate_bound(const uint16_t* depth, float* depth_scaled, const kfx_intrinsics* intr,
+                           IntegrateParams& p) {
     p.bound = nullptr;
     const size_t need = bound_layout(intr->cols, intr->rows, p.bound_off);
     if (need > g_bound_cap) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
-        if (cs != hipStreamCaptureStatusNone) return KFX_OK;
+        if (cs != hipStreamCaptureStatusNone) return scale_depth_launch(depth, depth_scaled, intr);
         KFX_TRY(integrate_reserve(intr->cols, intr->rows));
     }
-    ProfScope prof("depth_bound");
+    ProfScope prof("scale_depth");
     dim3 grid(divup(intr->cols, kBoundTile), divup(intr->rows, kBoundTile));
-    depth_bound_kernel<<<grid, kBoundTile * kBoundTile, 0, stream()>>>(depth_scaled, intr->cols, intr->rows, g_bound,
-                                                                        p);
+    depth_bound_kernel<<<grid, kBoundTile * kBoundTile, 0, stream()>>>(depth, depth_scaled, *intr, g_bound, p);
     KFX_CHECK_LAUNCH();
     p.bound = g_bound;
     return KFX_OK;
@@ -754,18 +773,25 @@ static int sweep_setup(const char* who, const kfx_intrinsics* intr, const kfx_ma
     return KFX_OK;
 }
 
-static int integrate_setup(const char* who, const uint16_t* depth, float* depth_scaled,
-                           const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
-                           const kfx_point_3f* translation, float truncation_distance,
-                           const kfx_point_3f* cell_size, const kfx_round_buffer* rb, IntegrateParams& p) {
-    KFX_REQUIRE(depth && depth_scaled, "%s: null pointer", who);
-    KFX_TRY(sweep_setup(who, intr, rotation_inverse, translation, truncation_distance, cell_size, rb, p));
+int scale_depth_launch(const uint16_t* depth, float* depth_scaled, const kfx_intrinsics* intr) {
     ProfScope prof("scale_depth");
     dim3 sblock(64, 4);
     dim3 sgrid(divup(intr->cols, 64), divup(intr->rows, 4));
     scale_depth_kernel<<<sgrid, sblock, 0, stream()>>>(depth, depth_scaled, *intr);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
+}
+
+// Sweep terms; with scale, also the scaled depth (the integrations instead
+// scale it in integrate_bound's launch).
+static int integrate_setup(const char* who, const uint16_t* depth, float* depth_scaled,
+                           const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                           const kfx_point_3f* translation, float truncation_distance,
+                           const kfx_point_3f* cell_size, const kfx_round_buffer* rb, IntegrateParams& p,
+                           bool scale = true) {
+    KFX_REQUIRE(depth && depth_scaled, "%s: null pointer", who);
+    KFX_TRY(sweep_setup(who, intr, rotation_inverse, translation, truncation_distance, cell_size, rb, p));
+    return scale ? scale_depth_launch(depth, depth_scaled, intr) : KFX_OK;
 }
 
 // Library-owned colour-bound pyramid (float2 per texel, levels 0..5).
@@ -833,9 +859,9 @@ int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* vol
     KFX_REQUIRE(volume != nullptr, "kfx_integrate_tsdf: null volume");
     IntegrateParams p;
     int rc = integrate_setup("kfx_integrate_tsdf", depth, depth_scaled, intr, rotation_inverse, translation,
-                             truncation_distance, cell_size, rb, p);
+                             truncation_distance, cell_size, rb, p, false);
     if (rc) return rc;
-    KFX_TRY(integrate_bound(depth_scaled, intr, p));
+    KFX_TRY(integrate_bound(depth, depth_scaled, intr, p));
     ProfScope prof("integrate");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
     dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
@@ -853,9 +879,9 @@ int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsd
     const kfx_point_3f zero = {0, 0, 0};
     IntegrateParams p;
     int rc = integrate_setup("kfx_integrate_tsdf_posed", depth, depth_scaled, intr, &ident, &zero,
-                             truncation_distance, cell_size, rb, p);
+                             truncation_distance, cell_size, rb, p, false);
     if (rc) return rc;
-    KFX_TRY(integrate_bound(depth_scaled, intr, p));
+    KFX_TRY(integrate_bound(depth, depth_scaled, intr, p));
     ProfScope prof("integrate");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
     dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));

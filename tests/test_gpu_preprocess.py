@@ -187,3 +187,4 @@ def test_resize_pyramid_equals_two_resizes(gpu, orc, c1_frame):
         g2 = o2.download(np.float32, (H // 4, W // 4, 3))
         ok = ~np.isnan(w2)
         assert np.array_equal(np.isnan(g2), np.isnan(w2)) and np.array_equal(g2[ok], w2[ok])
+
