@@ -304,11 +304,25 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-learned", action="store_true", help="no denoiser run (headline = fused bilateral)")
     ap.add_argument("--skip-rgbd", action="store_true", help="no RGBD odometry measurement")
+    ap.add_argument("--profile-learned", type=int, default=0, metavar="N",
+                    help="only the learned pipeline, 10 warmup and N timed frames, for a kernel trace "
+                         "(prints frames/s, no JSON benchmark line)")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="short fixed workload for hardware-counter runs (no JSON benchmark line)")
     args = ap.parse_args()
     if args.kernel_probe:
         kernel_probe()
+        return
+    if args.profile_learned:
+        n_t = args.profile_learned
+        intr, scene, poses, origin, depths = workload_inputs(n_t + 10)
+        kfx.init(0)
+        dd = kfx.Buffer.from_array(depths)
+        dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params())
+        cfg = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1)
+        el, _, pipe = run_pipeline(cfg, poses, origin, dd, depths[0].nbytes, 10, n_t, denoiser=dn)
+        pipe.close()
+        print("learned pipeline: %d frames, %.1f frames/s" % (n_t, n_t / el))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the pipeline runs on a single MI355X "
