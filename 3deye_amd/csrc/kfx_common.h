@@ -124,6 +124,18 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return r;
 }
 
+// Correctly rounded 1/x for x in [2^-126, 2^126): v_rcp_f32 (not correctly
+// rounded by itself) and one Newton step with FMAs.  Equal to the IEEE
+// quotient for every float of that range on MI355X, checked exhaustively on
+// the device (kfx_check_reciprocal, run by the GPU tests); outside it (zero,
+// subnormals, the top binade, infinities) the result may differ, so callers
+// keep their operands inside.
+__device__ __forceinline__ float rcp_rn(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r, 1.0f), r, r);
+}
+constexpr float kRcpMin = 0x1p-126f, kRcpMax = 0x1p126f;
+
 // The quotient a / (w + 1) of the TSDF running average, for weights w in
 // [0, 128], without a division: y = RN(1/b) from v_rcp_f32 (within one ulp)
 // and one Newton step with FMAs (correct from every starting value within an

@@ -98,6 +98,18 @@ __global__ void __launch_bounds__(256) copy16_kernel(uint4* __restrict__ dst, co
 
 using namespace kfx;
 
+namespace kfx {
+// Every positive float with exponent in [e_lo, e_hi]: rcp_rn against the IEEE
+// quotient 1/x.
+__global__ void __launch_bounds__(256) check_reciprocal_kernel(int e_lo, unsigned long long total,
+                                                               unsigned long long* __restrict__ bad) {
+    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const float x = __uint_as_float(((uint32_t)(e_lo + 127 + (int)(i >> 23)) << 23) | (uint32_t)(i & 0x7FFFFFu));
+    if (__float_as_uint(rcp_rn(x)) != __float_as_uint(1.0f / x)) atomicAdd(bad, 1ull);
+}
+}  // namespace kfx
+
 extern "C" {
 
 const char* kfx_last_error(void) { return g_last_error; }
@@ -120,6 +132,27 @@ int kfx_init(int device) {
     }
     KFX_CHECK_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_device = device;
+    return KFX_OK;
+}
+
+int kfx_check_reciprocal(int exp_lo, int exp_hi, uint64_t* mismatches) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(mismatches && exp_lo >= -126 && exp_hi <= 127 && exp_lo <= exp_hi,
+                "kfx_check_reciprocal: exponents [%d, %d] outside [-126, 127]", exp_lo, exp_hi);
+    unsigned long long* d = nullptr;
+    KFX_CHECK_HIP(hipMallocAsync((void**)&d, sizeof(unsigned long long), stream()));
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(unsigned long long), stream());
+    const unsigned long long total = (unsigned long long)(exp_hi - exp_lo + 1) << 23;
+    if (e == hipSuccess) {
+        kfx::check_reciprocal_kernel<<<(unsigned)((total + 255) / 256), 256, 0, stream()>>>(exp_lo, total, d);
+        e = hipGetLastError();
+    }
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, stream());
+    (void)hipFreeAsync(d, stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    KFX_CHECK_HIP(e);
+    *mismatches = h;
     return KFX_OK;
 }
 

@@ -127,8 +127,8 @@ __device__ __forceinline__ bool integrate_voxel(const IntegrateParams& p, const 
     const float camy = q.r[3] * gx + q.r[4] * gy + q.r[5] * gz;
     const float camz = q.r[6] * gx + q.r[7] * gy + q.r[8] * gz;
     in_band = false;
-    if (!(camz > 0.0f)) return false;
-    const float inv_z = 1.0f / camz;
+    if (!(camz >= kRcpMin && camz < kRcpMax)) return false;
+    const float inv_z = rcp_rn(camz);
     const float fu = rintf(camx * p.fx * inv_z + p.cx);
     const float fv = rintf(camy * p.fy * inv_z + p.cy);
     if (!(fu >= 0.0f && fu < (float)p.cols && fv >= 0.0f && fv < (float)p.rows)) return false;
@@ -365,10 +365,13 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
             const float camx = cxy0 + q.r[2] * gz;
             const float camy = cxy1 + q.r[5] * gz;
             const float camz = cxy2 + q.r[8] * gz;
-            const float inv_z = 1.0f / camz;
+            // In front of the camera: camz in [2^-126, 2^126), where rcp_rn is
+            // the correctly rounded reciprocal.
+            const float inv_z = rcp_rn(camz);
             const float fu = rintf(camx * p.fx * inv_z + p.cx);
             const float fv = rintf(camy * p.fy * inv_z + p.cy);
-            const bool in = (z <= hi) & (camz > 0.0f) & (fu >= 0.0f) & (fu < fcols) & (fv >= 0.0f) & (fv < frows);
+            const bool in = (z <= hi) & (camz >= kRcpMin) & (camz < kRcpMax) & (fu >= 0.0f) & (fu < fcols) &
+                            (fv >= 0.0f) & (fv < frows);
             const int pix = (int)(in ? fv : 0.0f) * p.cols + (int)(in ? fu : 0.0f);
             dp[k] = scaled[pix];
             dist[k] = sqrt_rn(gz * gz + part);
@@ -571,10 +574,13 @@ integrate_colors_kernel(const kfx_color* __restrict__ rgb, const kfx_xyz* __rest
             const float camx = cxy0 + q.r[2] * gz;
             const float camy = cxy1 + q.r[5] * gz;
             const float camz = cxy2 + q.r[8] * gz;
-            const float inv_z = 1.0f / camz;
+            // In front of the camera: camz in [2^-126, 2^126), where rcp_rn is
+            // the correctly rounded reciprocal.
+            const float inv_z = rcp_rn(camz);
             const float fu = rintf(camx * p.fx * inv_z + p.cx);
             const float fv = rintf(camy * p.fy * inv_z + p.cy);
-            const bool in = (z <= hi) & (camz > 0.0f) & (fu >= 0.0f) & (fu < fcols) & (fv >= 0.0f) & (fv < frows);
+            const bool in = (z <= hi) & (camz >= kRcpMin) & (camz < kRcpMax) & (fu >= 0.0f) & (fu < fcols) &
+                            (fv >= 0.0f) & (fv < frows);
             pix[k] = (int)(in ? fv : 0.0f) * p.cols + (int)(in ? fu : 0.0f);
             const kfx_xyz v = vmap[pix[k]];
             px[k] = v.x; py[k] = v.y; pz[k] = v.z;

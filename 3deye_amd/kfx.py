@@ -260,6 +260,7 @@ _PROTOS = {
     "kfx_icp_reset": (_I, [_P, C.POINTER(Pose)]),
     "kfx_icp_iterate": (_I, [_P, _P, _P, _P, _P, C.POINTER(Matrix3f), C.POINTER(Point3f), C.POINTER(Intrinsics),
                              C.c_float, C.c_float, _I, C.c_double, C.c_double]),
+    "kfx_check_reciprocal": (_I, [_I, _I, C.POINTER(C.c_uint64)]),
     "kfx_icp_read": (_I, [_P, C.POINTER(IcpResult)]),
     "kfx_rgbd_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
     "kfx_rgbd_destroy": (_I, [_P]),

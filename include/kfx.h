@@ -48,6 +48,12 @@ const char* kfx_last_error(void);
 int kfx_device_count(int* count);
 /* Wait for all work enqueued on the library stream. */
 int kfx_synchronize(void);
+/* Diagnostics: the number of positive floats with exponent in [exp_lo,
+ * exp_hi] (within [-126, 127]) whose device reciprocal of the kernels'
+ * division-free form (v_rcp_f32 and one Newton step) differs from the IEEE
+ * quotient; 0 over [-126, 125] on MI355X.  Blocking.  No reference
+ * counterpart. */
+int kfx_check_reciprocal(int exp_lo, int exp_hi, uint64_t* mismatches);
 /* The library stream as a hipStream_t (for callers that interleave work). */
 int kfx_get_stream(void** hip_stream);
 

@@ -98,7 +98,9 @@ extern "C" uint64_t orc_integrate_f32(const float* scaled, kfx_tsdf* vol, const 
                 const float camx = r[0] * gx + r[1] * gy + r[2] * gz;
                 const float camy = r[3] * gx + r[4] * gy + r[5] * gz;
                 const float camz = r[6] * gx + r[7] * gy + r[8] * gz;
-                if (!(camz > 0.0f)) continue;
+                // In front: camz in [2^-126, 2^126) (the kernels' exact
+                // reciprocal range; wider than any volume).
+                if (!(camz >= 0x1p-126f && camz < 0x1p126f)) continue;
                 const float inv_z = 1.0f / camz;
                 const float fu = rintf(camx * in->fx * inv_z + in->cx);
                 const float fv = rintf(camy * in->fy * inv_z + in->cy);
@@ -185,7 +187,9 @@ extern "C" uint64_t orc_integrate_colors(kfx_color* vol, const kfx_color* rgb, c
                 const float camx = r[0] * gx + r[1] * gy + r[2] * gz;
                 const float camy = r[3] * gx + r[4] * gy + r[5] * gz;
                 const float camz = r[6] * gx + r[7] * gy + r[8] * gz;
-                if (!(camz > 0.0f)) continue;
+                // In front: camz in [2^-126, 2^126) (the kernels' exact
+                // reciprocal range; wider than any volume).
+                if (!(camz >= 0x1p-126f && camz < 0x1p126f)) continue;
                 const float inv_z = 1.0f / camz;
                 const float fu = rintf(camx * in->fx * inv_z + in->cx);
                 const float fv = rintf(camy * in->fy * inv_z + in->cy);

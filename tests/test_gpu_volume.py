@@ -160,3 +160,15 @@ def test_integrate_frustum_clip_random_poses(gpu, orc, seed):
     kfx.call("kfx_integrate_count_updates", dd.ptr, ds.ptr, C.byref(intr), C.byref(kfx.matrix3f(R.T)),
              C.byref(kfx.point3f(t)), scenes.TRUNC, C.byref(cs), C.byref(rb), C.byref(n))
     assert n.value == n_ref
+
+
+def test_device_reciprocal_is_correctly_rounded(gpu):
+    # The projections' division-free 1/z equals the IEEE quotient for every
+    # float in [2^-126, 2^126) on this device (the range the kernels keep z
+    # in); the top binade (subnormal results) is where it stops.
+    kfx = gpu
+    bad = C.c_uint64()
+    kfx.call("kfx_check_reciprocal", -126, 125, C.byref(bad))
+    assert bad.value == 0
+    kfx.call("kfx_check_reciprocal", 126, 126, C.byref(bad))
+    assert bad.value > 0
