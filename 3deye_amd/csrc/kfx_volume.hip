@@ -11,8 +11,11 @@
 
 namespace kfx {
 
-// Voxels of one column that a thread handles; grid.z = Nz / kIntegrateZRun.
+// Voxels of one column that a thread handles (grid.z = Nz / run): 32 up to
+// 512 slices, 64 beyond.
 constexpr int kIntegrateZRun = 32;
+constexpr int kIntegrateZRunLarge = 64;
+__host__ __device__ inline int integrate_zrun(int nz) { return nz > 512 ? kIntegrateZRunLarge : kIntegrateZRun; }
 // Frustum side planes are pushed out by this many pixels and the clipped z
 // range by one voxel on each side, so the clip never removes a voxel the
 // per-voxel projection test would accept.
@@ -34,6 +37,7 @@ struct IntegrateParams {
     float trunc, inv_trunc;
     int nx, ny, nz;
     int ox, oy, oz;      // cyclic origin, each in [0, n)
+    int zrun;            // voxels of a column per thread (integrate_zrun)
     uint32_t slice;      // nx * ny
     // Frustum planes through the camera centre in volume coordinates:
     // a voxel centre g (relative to the camera) can project into the image
@@ -320,8 +324,8 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
     const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
     const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
     const bool inside = x < p.nx && y < p.ny;
-    const int z_begin = blockIdx.z * kIntegrateZRun;
-    const int z_end = min(z_begin + kIntegrateZRun, p.nz);
+    const int z_begin = blockIdx.z * p.zrun;
+    const int z_end = min(z_begin + p.zrun, p.nz);
 
     const float gx = ((float)x + 0.5f) * p.csx - q.tx;
     const float gy = ((float)y + 0.5f) * p.csy - q.ty;
@@ -420,8 +424,8 @@ integrate_count_kernel(const float* __restrict__ scaled, IntegrateParams p, unsi
     unsigned n = 0;
     const PoseTerms q = host_terms(p);
     if (x < p.nx && y < p.ny) {
-        const int z_begin = blockIdx.z * kIntegrateZRun;
-        const int z_end = min(z_begin + kIntegrateZRun, p.nz);
+        const int z_begin = blockIdx.z * p.zrun;
+        const int z_end = min(z_begin + p.zrun, p.nz);
         const float gx = ((float)x + 0.5f) * p.csx - q.tx;
         const float gy = ((float)y + 0.5f) * p.csy - q.ty;
         const float part = gx * gx + gy * gy;
@@ -537,8 +541,8 @@ integrate_colors_kernel(const kfx_color* __restrict__ rgb, const kfx_xyz* __rest
     const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
     const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
     const bool inside = x < p.nx && y < p.ny;
-    const int z_begin = blockIdx.z * kIntegrateZRun;
-    const int z_end = min(z_begin + kIntegrateZRun, p.nz);
+    const int z_begin = blockIdx.z * p.zrun;
+    const int z_end = min(z_begin + p.zrun, p.nz);
 
     const float vgx = ((float)x + 0.5f) * p.csx, vgy = ((float)y + 0.5f) * p.csy;
     const float gx = vgx - q.tx, gy = vgy - q.ty;
@@ -753,6 +757,7 @@ static int sweep_setup(const char* who, const kfx_intrinsics* intr, const kfx_ma
     p.nx = rb->voxels_size.x; p.ny = rb->voxels_size.y; p.nz = rb->voxels_size.z;
     p.ox = rb->origin_GRID.x; p.oy = rb->origin_GRID.y; p.oz = rb->origin_GRID.z;
     p.slice = (uint32_t)p.nx * (uint32_t)p.ny;
+    p.zrun = integrate_zrun(p.nz);
     // Frustum planes: camera coordinates c = R^-1 g; a voxel can project
     // inside the image only if c.z > 0 and -0.5 - m <= f c.xy / c.z + c0 <
     // size - 0.5 + m.  In volume coordinates each plane normal is the
@@ -870,7 +875,7 @@ int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* vol
     KFX_TRY(integrate_bound(depth, depth_scaled, intr, p));
     ProfScope prof("integrate");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
     integrate_kernel<false><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, nullptr, nullptr, 0.0f, nullptr);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
@@ -890,7 +895,7 @@ int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsd
     KFX_TRY(integrate_bound(depth, depth_scaled, intr, p));
     ProfScope prof("integrate");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
     integrate_kernel<true><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, dev_pose, dev_prev_pose_inv,
                                                          motion_threshold, dev_skip);
     KFX_CHECK_LAUNCH();
@@ -911,7 +916,7 @@ int kfx_integrate_count_updates(const uint16_t* depth, float* depth_scaled, cons
     hipError_t e = hipMemsetAsync(dcount, 0, sizeof(unsigned long long), stream());
     if (e == hipSuccess) {
         dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-        dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
+        dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
         integrate_count_kernel<<<grid, block, 0, stream()>>>(depth_scaled, p, dcount);
         e = hipGetLastError();
     }
@@ -937,7 +942,7 @@ int kfx_integrate_colors(kfx_color* color_volume, const kfx_color* rgb_frame, co
     KFX_TRY(colors_bound(vertex_map, p, nullptr, nullptr, &cb));
     ProfScope prof("integrate_colors");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
     integrate_colors_kernel<false><<<grid, block, 0, stream()>>>(rgb_frame, vertex_map, color_volume, p, cb,
                                                                  max_weight, nullptr, nullptr);
     KFX_CHECK_LAUNCH();
@@ -958,7 +963,7 @@ int kfx_integrate_colors_posed(kfx_color* color_volume, const kfx_color* rgb_fra
     KFX_TRY(colors_bound(vertex_map, p, dev_pose, dev_skip, &cb));
     ProfScope prof("integrate_colors");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, kIntegrateZRun));
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
     integrate_colors_kernel<true><<<grid, block, 0, stream()>>>(rgb_frame, vertex_map, color_volume, p, cb,
                                                                 max_weight, dev_pose, dev_skip);
     KFX_CHECK_LAUNCH();
