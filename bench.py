@@ -207,6 +207,21 @@ def cpu_baseline(intr, depths, poses, origin, frames=12):
                       f"19 ICP iterations, integrate, raycast), first {frames} frames of the 512^3 workload"}
 
 
+def rgbd_cpu(intr, scene, poses, depths, pairs=4):
+    """The oracle's RGBD odometry (one host thread) on the first pairs of the
+    same frames: the CPU time per pair beside the GPU's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ctypes as orc
+
+    rgb = [kfx.render_rgb(scene, poses[k], intr).view(np.uint8) for k in range(pairs + 1)]
+    t0 = time.perf_counter()
+    for k in range(1, pairs + 1):
+        orc.rgbd_estimate(rgb[k - 1], depths[k - 1], rgb[k], depths[k], intr, 4, (7, 7, 7, 10), 10.0, 0.07,
+                          np.eye(3), np.zeros(3))
+    return {"ms_per_pair": 1e3 * (time.perf_counter() - t0) / pairs, "cores": 1, "kind": "port",
+            "sample": "oracle RGBD odometry, walk frames 0-%d" % pairs}
+
+
 def rgbd_odometry(intr, scene, poses, depths, pairs=60, warmup=5):
     """RGBD photometric odometry (kfx_rgbd) on consecutive frames of the walk,
     with rendered colour: pairs per second (frame preparation, 4 levels x
@@ -407,6 +422,8 @@ def main():
         kern["raycast"]["samples_counted_frames"] = s_frames
 
     rgbd = None if args.skip_rgbd else rgbd_odometry(intr, scene, poses, depths)
+    if rgbd is not None and not args.no_cpu_baseline:
+        rgbd["cpu_baseline"] = rgbd_cpu(intr, scene, poses, depths)
     base = None if args.no_cpu_baseline else cpu_baseline(intr, depths, poses, origin)
     head = learned if learned is not None else fused
     el_head = 1e-3 * head["ms_per_frame"] * args.steps
