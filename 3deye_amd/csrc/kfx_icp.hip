@@ -65,9 +65,16 @@ __device__ __forceinline__ bool icp_project(const kfx_xyz& v, const kfx_xyz& n, 
     const float px = f.rprev_inv[0] * dx + f.rprev_inv[1] * dy + f.rprev_inv[2] * dz;
     const float py = f.rprev_inv[3] * dx + f.rprev_inv[4] * dy + f.rprev_inv[5] * dz;
     const float pz = f.rprev_inv[6] * dx + f.rprev_inv[7] * dy + f.rprev_inv[8] * dz;
-    if (!(pz > 0.0f)) return false;
-    const float fu = rintf(px * f.fx / pz + f.cx);
-    const float fv = rintf(py * f.fy / pz + f.cy);
+    // In front: pz in [2^-126, 2^126), where rcp_rn is the correctly rounded
+    // reciprocal; each quotient (p f) / pz is then formed without a division
+    // (Markstein's correction with the exact FMA residual gives the correctly
+    // rounded quotient from RN(1/pz)).
+    if (!(pz >= kRcpMin && pz < kRcpMax)) return false;
+    const float y = rcp_rn(pz);
+    const float au = px * f.fx, av = py * f.fy;
+    const float qu0 = au * y, qv0 = av * y;
+    const float fu = rintf(fmaf(fmaf(-qu0, pz, au), y, qu0) + f.cx);
+    const float fv = rintf(fmaf(fmaf(-qv0, pz, av), y, qv0) + f.cy);
     if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)f.cols && fv < (float)f.rows)) return false;
     j = (int)fv * f.cols + (int)fu;
     return true;

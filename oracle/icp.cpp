@@ -30,7 +30,8 @@ bool row_f32(const kfx_xyz* vc, const kfx_xyz* nc, const kfx_xyz* vp, const kfx_
     const float px = Rpi[0] * dx + Rpi[1] * dy + Rpi[2] * dz;
     const float py = Rpi[3] * dx + Rpi[4] * dy + Rpi[5] * dz;
     const float pz = Rpi[6] * dx + Rpi[7] * dy + Rpi[8] * dz;
-    if (!(pz > 0.0f)) return false;
+    // In front: pz in [2^-126, 2^126) (the kernels' exact reciprocal range).
+    if (!(pz >= 0x1p-126f && pz < 0x1p126f)) return false;
     const float fu = rintf(px * in->fx / pz + in->cx);
     const float fv = rintf(py * in->fy / pz + in->cy);
     if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)in->cols && fv < (float)in->rows)) return false;

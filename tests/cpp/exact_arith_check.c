@@ -10,6 +10,10 @@
  *               and each cell size c of the test and benchmark volumes
  *   rcp_weight  (tsdf_average_quotient, kfx_common.h): RN(1/b) for b in
  *               [1, 129] from v_rcp_f32 (3 starting values) and one Newton step
+ *   quotient    (ICP projection, kfx_icp.hip): a / b as Markstein's
+ *               correction from RN(1/b), 2*10^8 random pairs (b in
+ *               [2^-8, 2^8), |a| in [2^-30, 2^14)); correct rounding for all
+ *               pairs is Markstein's theorem, this samples it
  *   avg_div     (tsdf_average_quotient): a / b for b in [2, 129] and every
  *               float a in [2^-100, 129] (the quotient is odd in a, so
  *               positive a suffice)
@@ -95,5 +99,19 @@ int main(void) {
         }
     }
     printf("avg_div: %ld cases, %ld mismatches\n", n_avg, bad_avg);
-    return (bad_sqrt || bad_dec || bad_div || bad_rcp || bad_avg) ? 1 : 0;
+
+    long bad_q = 0;
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    for (long i = 0; i < 200000000L; ++i) {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        const float b = fb(((uint32_t)(119 + (st % 16)) << 23) | (uint32_t)((st >> 8) & 0x7FFFFF));
+        const float a = fb(((uint32_t)(st >> 63) << 31) | ((uint32_t)(97 + ((st >> 32) % 44)) << 23) |
+                           (uint32_t)((st >> 40) & 0x7FFFFF));
+        const float y = 1.0f / b, q0 = a * y, q = fmaf(fmaf(-q0, b, a), y, q0);
+        bad_q += q != a / b;
+    }
+    printf("quotient: 200000000 cases, %ld mismatches\n", bad_q);
+    return (bad_sqrt || bad_dec || bad_div || bad_rcp || bad_avg || bad_q) ? 1 : 0;
 }
