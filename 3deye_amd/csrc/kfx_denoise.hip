@@ -89,7 +89,13 @@ __device__ __forceinline__ float noise_sigma_m(float z) {
     const float dz = z - 0.4f;
     return (1.2f + 1.9f * (dz * dz)) * 0.001f;
 }
-__device__ __forceinline__ float inv_depth(uint16_t d) { return 1000.0f / (float)d; }
+// 1000 / d, correctly rounded without a division: RN(1/d) from rcp_rn and
+// Markstein's correction (equal to the IEEE quotient for every d in
+// [1, 65535], checked on the CPU); d = 0 gives +inf as the quotient would.
+__device__ __forceinline__ float inv_depth(uint16_t d) {
+    const float b = (float)d, y = rcp_rn(b), q0 = 1000.0f * y;
+    return d ? fmaf(fmaf(-q0, b, 1000.0f), y, q0) : __builtin_inff();
+}
 // sigma_w(p) = sigma_z(z_p) w_p^2.
 __device__ __forceinline__ float noise_sigma_inv(uint16_t d) {
     const float w = inv_depth(d);
@@ -104,7 +110,7 @@ __device__ __forceinline__ float rel_depth(uint16_t dq, float wp, float inv_sigm
 __device__ __forceinline__ float dn_output(uint16_t d, float s) {
     const float wp = inv_depth(d);
     const float w = fmaxf(wp + noise_sigma_inv(d) * s, 0.25f * wp);
-    return 1.0f / w;
+    return rcp_rn(w);  // w >= wp / 4 >= 0.0038: inside rcp_rn's exact range
 }
 
 // Layer 1 (2 -> 16), one lane per pixel.
@@ -115,7 +121,7 @@ __global__ void __launch_bounds__(256) dn_l1_fwd(const uint16_t* __restrict__ de
     if (x >= W || y >= H) return;
     const uint16_t dc = depth[y * W + x];
     const float wc = dc ? inv_depth(dc) : 0.0f;
-    const float inv_sigma = dc ? 1.0f / noise_sigma_inv(dc) : 0.0f;
+    const float inv_sigma = dc ? rcp_rn(noise_sigma_inv(dc)) : 0.0f;
     float in[2][9];
     for (int ky = 0; ky < 3; ++ky)
         for (int kx = 0; kx < 3; ++kx) {
@@ -311,7 +317,7 @@ __global__ void __launch_bounds__(kDfThreads) dn_fused_fwd(const uint16_t* __res
         if (x >= 0 && y >= 0 && x < W && y < H) {
             const uint16_t dc = sd[(ly + 1) * kDfDX + lx + 1];
             const float wc = dc ? inv_depth(dc) : 0.0f;
-            const float inv_sigma = dc ? 1.0f / noise_sigma_inv(dc) : 0.0f;
+            const float inv_sigma = dc ? rcp_rn(noise_sigma_inv(dc)) : 0.0f;
             float in[2][9];
             for (int ky = 0; ky < 3; ++ky)
                 for (int kx = 0; kx < 3; ++kx) {
@@ -618,7 +624,7 @@ __global__ void __launch_bounds__(64) dn_l1_wgrad(const uint16_t* __restrict__ d
         const int p = y * W + x;
         const uint16_t dc = depth[p];
         const float wc = dc ? inv_depth(dc) : 0.0f;
-        const float inv_sigma = dc ? 1.0f / noise_sigma_inv(dc) : 0.0f;
+        const float inv_sigma = dc ? rcp_rn(noise_sigma_inv(dc)) : 0.0f;
         float g[8];
         bool any = false;
         for (int c = 0; c < 8; ++c) {

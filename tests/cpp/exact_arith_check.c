@@ -14,6 +14,7 @@
  *               correction from RN(1/b), 2*10^8 random pairs (b in
  *               [2^-8, 2^8), |a| in [2^-30, 2^14)); correct rounding for all
  *               pairs is Markstein's theorem, this samples it
+ *   inv_depth   (kfx_denoise.hip): 1000 / d for every d in [1, 65535]
  *   avg_div     (tsdf_average_quotient): a / b for b in [2, 129] and every
  *               float a in [2^-100, 129] (the quotient is odd in a, so
  *               positive a suffice)
@@ -113,5 +114,12 @@ int main(void) {
         bad_q += q != a / b;
     }
     printf("quotient: 200000000 cases, %ld mismatches\n", bad_q);
-    return (bad_sqrt || bad_dec || bad_div || bad_rcp || bad_avg || bad_q) ? 1 : 0;
+
+    long bad_inv = 0;
+    for (int d = 1; d <= 65535; ++d) {
+        const float b = (float)d, y = 1.0f / b, q0 = 1000.0f * y;
+        bad_inv += fmaf(fmaf(-q0, b, 1000.0f), y, q0) != 1000.0f / b;
+    }
+    printf("inv_depth: 65535 cases, %ld mismatches\n", bad_inv);
+    return (bad_sqrt || bad_dec || bad_div || bad_rcp || bad_avg || bad_q || bad_inv) ? 1 : 0;
 }

@@ -21,4 +21,4 @@ def test_exact_arithmetic_exhaustive(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count(" 0 mismatches") == 6
+    assert r.stdout.count(" 0 mismatches") == 7
