@@ -618,13 +618,26 @@ __device__ __forceinline__ void cross_block_sum(const void* __restrict__ partial
 // (three workgroups' loads per round: all of them in one round at 640x480),
 // then thread 0's Cholesky solve and pose update.  The estimate is loaded
 // before the sum so that its latency hides behind the partials' loads.
+// Thread 0 copies the state to host memory (8-byte stores).
+__device__ __forceinline__ void icp_publish(const IcpState* st, IcpState* mirror) {
+    static_assert(sizeof(IcpState) % 8 == 0, "state copy by 8-byte words");
+    const uint2* src = reinterpret_cast<const uint2*>(st);
+    uint2* dst = reinterpret_cast<uint2*>(mirror);
+    for (int i = 0; i < (int)(sizeof(IcpState) / 8); ++i) dst[i] = src[i];
+}
+
+// With mirror (pinned host memory), thread 0 also publishes the resulting
+// state there, so the host reads it after the stream without a copy command.
 template <int M>
 __global__ void __launch_bounds__(kIcpThreads)
 icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials, int nblk, int tag,
-                 double conv_rot, double conv_trans) {
+                 double conv_rot, double conv_trans, IcpState* __restrict__ mirror) {
     __shared__ double wsum[kIcpWaves][32];
     __shared__ double total[32];
-    if (tag >= 0 && st->converged_tag == tag) return;
+    if (tag >= 0 && st->converged_tag == tag) {
+        if (mirror != nullptr && threadIdx.x == 0) icp_publish(st, mirror);
+        return;
+    }
     double R0[9], t0[3];
 #pragma unroll
     for (int i = 0; i < 9; ++i) R0[i] = st->R[i];
@@ -632,7 +645,10 @@ icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials, i
     for (int i = 0; i < 3; ++i) t0[i] = st->t[i];
     const int iterations = st->iterations;
     cross_block_sum<M, 3>(partials, nblk, wsum, total);
-    if (threadIdx.x == 0) solve_and_update(st, total, R0, t0, iterations, tag, conv_rot, conv_trans);
+    if (threadIdx.x == 0) {
+        solve_and_update(st, total, R0, t0, iterations, tag, conv_rot, conv_trans);
+        if (mirror != nullptr) icp_publish(st, mirror);
+    }
 }
 
 // Start of a frame: the current estimate becomes the previous pose (as R^T,
@@ -694,6 +710,7 @@ struct kfx_icp {
     int sum_mode;      // KFX_ICP_SUM_*
     kfx::IcpState* host_state;  // pinned host copy for kfx_icp_read
     int max_blocks;
+    bool publish_next;  // the next iteration's solve writes host_state
 };
 
 using namespace kfx;
@@ -705,8 +722,10 @@ static void icp_iterate_launch(kfx_icp* h, const kfx_xyz* vc, const kfx_xyz* nc,
         icp_sums_kernel<2, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
     else
         icp_sums_kernel<1, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
+    IcpState* mirror = h->publish_next ? h->host_state : nullptr;
+    h->publish_next = false;
     icp_solve_kernel<M><<<1, kIcpThreads, 0, stream()>>>(h->state, h->partials, nblk, tag, conv_rot,
-                                                                conv_trans);
+                                                                conv_trans, mirror);
 }
 
 extern "C" {
@@ -866,6 +885,28 @@ int kfx_icp_read(kfx_icp* h, kfx_icp_result* out) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(h && out, "kfx_icp_read: null pointer");
     KFX_CHECK_HIP(hipMemcpyAsync(h->host_state, h->state, sizeof(IcpState), hipMemcpyDeviceToHost, stream()));
+    KFX_CHECK_HIP(hipStreamSynchronize(stream()));
+    const IcpState& s = *h->host_state;
+    for (int i = 0; i < 9; ++i) out->pose.R[i] = s.R[i];
+    for (int i = 0; i < 3; ++i) out->pose.t[i] = s.t[i];
+    for (int i = 0; i < 36; ++i) out->A[i] = s.A[i];
+    for (int i = 0; i < 6; ++i) { out->b[i] = s.b[i]; out->x[i] = s.x[i]; }
+    out->status = s.status;
+    out->singular_count = s.singular_count;
+    out->iterations = s.iterations;
+    out->converged_tag = s.converged_tag;
+    return KFX_OK;
+}
+
+int kfx_icp_publish_next(kfx_icp* h) {
+    KFX_REQUIRE(h != nullptr, "kfx_icp_publish_next: null handle");
+    h->publish_next = true;
+    return KFX_OK;
+}
+
+int kfx_icp_read_published(kfx_icp* h, kfx_icp_result* out) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(h && out, "kfx_icp_read_published: null pointer");
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
     const IcpState& s = *h->host_state;
     for (int i = 0; i < 9; ++i) out->pose.R[i] = s.R[i];

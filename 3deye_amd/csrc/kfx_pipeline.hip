@@ -240,11 +240,18 @@ int icp_part(kfx_pipeline* p, bool begin) {
     // skips the kernels' read of the convergence state (a memory round trip
     // ahead of their loads).
     const bool early_exit = p->cfg.converge_rotation > 0.0;
+    // The last iteration publishes the state to pinned host memory, which
+    // the host reads after the frame (kfx_icp_read_published).
+    int last_level = -1;
     for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l)
-        for (int it = 0; it < p->cfg.icp_iterations[l]; ++it)
+        if (p->cfg.icp_iterations[l] > 0) last_level = l;
+    for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l)
+        for (int it = 0; it < p->cfg.icp_iterations[l]; ++it) {
+            if (l == last_level && it == p->cfg.icp_iterations[l] - 1) KFX_TRY(kfx_icp_publish_next(p->icp));
             KFX_TRY(kfx_icp_iterate(p->icp, p->vc[l], p->nc[l], p->vp[l], p->np[l], nullptr, nullptr, &p->lvl[l],
                                     p->cfg.distance_threshold, p->cfg.angle_threshold, early_exit ? l : -1,
                                     p->cfg.converge_rotation, p->cfg.converge_translation));
+        }
     return KFX_OK;
 }
 
@@ -270,6 +277,13 @@ int map_part(kfx_pipeline* p) { return map_part_skip(p, nullptr); }
 // (fallback gate), decided on the device; integration and raycast, which
 // the device skips when the gate falls back (the host then redoes the frame
 // with the bilateral filter).  No host round trip inside the frame.
+// The state after a frame's ICP: the copy its last iteration published, or
+// a read when the configuration runs no ICP iteration.
+int icp_read_frame(kfx_pipeline* p, kfx_icp_result* r) {
+    const int n = p->cfg.icp_iterations[0] + p->cfg.icp_iterations[1] + p->cfg.icp_iterations[2];
+    return n > 0 ? kfx_icp_read_published(p->icp, r) : kfx_icp_read(p->icp, r);
+}
+
 int learned_frame(kfx_pipeline* p) {
     KFX_TRY(preprocess(p, true));
     KFX_TRY(icp_part(p, true));
@@ -574,7 +588,7 @@ static int process_frame(kfx_pipeline* p, const uint16_t* depth, const kfx_color
         // raycast: the frame is redone with the bilateral filter, ICP from
         // the previous pose, then integration and raycast.
         KFX_TRY(run_part(p, learned_frame, &p->graph, &p->graph_exec));
-        KFX_TRY(kfx_icp_read(p->icp, &r));  // synchronises the stream
+        KFX_TRY(icp_read_frame(p, &r));  // synchronises the stream
         have_pose = true;
         const double cnt = p->hstats[1];
         const double rms_mm = cnt > 0 ? 1000.0 * sqrt(p->hstats[0] / cnt) : 0.0;
@@ -596,7 +610,7 @@ static int process_frame(kfx_pipeline* p, const uint16_t* depth, const kfx_color
         }
     }
     if (p->frame > 0 && p->cfg.mode == KFX_PIPELINE_FUSED) {
-        if (!have_pose) KFX_TRY(kfx_icp_read(p->icp, &r));
+        if (!have_pose) KFX_TRY(icp_read_frame(p, &r));
         memcpy(p->R, r.pose.R, sizeof(p->R));
         memcpy(p->t, r.pose.t, sizeof(p->t));
         if (info) {

@@ -262,6 +262,8 @@ _PROTOS = {
                              C.c_float, C.c_float, _I, C.c_double, C.c_double]),
     "kfx_check_reciprocal": (_I, [_I, _I, C.POINTER(C.c_uint64)]),
     "kfx_icp_read": (_I, [_P, C.POINTER(IcpResult)]),
+    "kfx_icp_publish_next": (_I, [_P]),
+    "kfx_icp_read_published": (_I, [_P, C.POINTER(IcpResult)]),
     "kfx_rgbd_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
     "kfx_rgbd_destroy": (_I, [_P]),
     "kfx_rgbd_set_frame": (_I, [_P, _I, _P, _P]),

@@ -304,6 +304,15 @@ int kfx_icp_begin_frame(kfx_icp* icp);
 int kfx_icp_translate(kfx_icp* icp, double dx, double dy, double dz);
 /* Blocking read of the state. */
 int kfx_icp_read(kfx_icp* icp, kfx_icp_result* out);
+/* The next kfx_icp_iterate's solve also writes the resulting state to the
+ * handle's pinned host copy (also when that iteration is skipped by the
+ * early exit), in a graph replay too.  kfx_icp_read_published then
+ * synchronises the stream and reads that copy without a copy command; the
+ * caller guarantees that a publishing iteration was enqueued since the
+ * state last changed.  No reference counterpart: it saves the per-frame
+ * device-to-host copy of the fused pipeline. */
+int kfx_icp_publish_next(kfx_icp* icp);
+int kfx_icp_read_published(kfx_icp* icp, kfx_icp_result* out);
 /* Device addresses of the single-precision current pose (R row-major camera
  * -> volume, then t: 12 floats) and of the previous pose (R^T then t), for
  * kernels that consume the estimate without a host round trip. */
