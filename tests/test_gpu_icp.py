@@ -171,3 +171,27 @@ def test_fused_iteration_large_images(gpu, orc, size, mode):
     np.testing.assert_allclose(np.array(res.A[:]).reshape(6, 6), A, rtol=1e-9, atol=1e-9 * np.abs(A).max())
     np.testing.assert_allclose(np.array(res.b[:]), b, rtol=1e-9, atol=1e-9 * np.abs(b).max())
     kfx.call("kfx_icp_destroy", h)
+
+
+@pytest.mark.parametrize("conv", [0.0, 1.0])
+def test_published_state_equals_read(gpu, orc, pair, conv):
+    # The last solve's copy in pinned host memory equals a read of the device
+    # state, also when that iteration is skipped by the early exit (conv 1.0
+    # stops the level after one solve).
+    kfx = gpu
+    intr, maps, (Rp, tp), (Rc, tc) = pair
+    bufs = _upload(kfx, maps)
+    h = _icp_handle(kfx, intr.cols * intr.rows)
+    kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+    kfx.call("kfx_icp_begin_frame", h)
+    for it in range(4):
+        if it == 3:
+            kfx.call("kfx_icp_publish_next", h)
+        kfx.call("kfx_icp_iterate", h, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, None, None,
+                 C.byref(intr), ii.DIST_THR, ii.ANGLE_THR, 0, conv, conv)
+    a, b = kfx.IcpResult(), kfx.IcpResult()
+    kfx.call("kfx_icp_read_published", h, C.byref(a))
+    kfx.call("kfx_icp_read", h, C.byref(b))
+    assert bytes(a) == bytes(b)
+    assert a.iterations == (1 if conv > 0.0 else 4)
+    kfx.call("kfx_icp_destroy", h)
