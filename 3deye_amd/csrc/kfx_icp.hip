@@ -91,12 +91,14 @@ __device__ __forceinline__ void icp_residual(const kfx_xyz& n, const float g[3],
     if (!ok || isnan(np.x) || isnan(vp.x)) return;
     const float gx = g[0], gy = g[1], gz = g[2];
     const float ex = vp.x - gx, ey = vp.y - gy, ez = vp.z - gz;
-    if (!(sqrtf(ex * ex + ey * ey + ez * ez) <= f.dist_thr)) return;
+    // sqrt_rn: the correctly rounded root (sqrtf's value) without the
+    // library's range scaling.
+    if (!(sqrt_rn(ex * ex + ey * ey + ez * ez) <= f.dist_thr)) return;
     const float cnx = Rc[0] * n.x + Rc[1] * n.y + Rc[2] * n.z;
     const float cny = Rc[3] * n.x + Rc[4] * n.y + Rc[5] * n.z;
     const float cnz = Rc[6] * n.x + Rc[7] * n.y + Rc[8] * n.z;
     const float sx = cny * np.z - cnz * np.y, sy = cnz * np.x - cnx * np.z, sz = cnx * np.y - cny * np.x;
-    if (!(sqrtf(sx * sx + sy * sy + sz * sz) < f.angle_thr)) return;
+    if (!(sqrt_rn(sx * sx + sy * sy + sz * sz) < f.angle_thr)) return;
     row[0] = gy * np.z - gz * np.y;
     row[1] = gz * np.x - gx * np.z;
     row[2] = gx * np.y - gy * np.x;
