@@ -17,13 +17,58 @@
 
 namespace kfx {
 
-// Reserves the integration's depth-bound pyramid for images of cols x rows
-// (kfx_volume.hip), so that integrations inside a stream capture are culled.
-int integrate_reserve(int cols, int rows);
-// Reserves the depth-consistency gate's workgroup partials (kfx_denoise.hip).
-int consistency_reserve(int cols, int rows);
-// Reserves the colour integration's vertex-distance pyramid (kfx_volume.hip).
-int colors_reserve(int cols, int rows);
+// Device buffers that kernels use beyond their inputs and outputs.  A
+// pipeline owns one for its lifetime, so the kernel arguments its captured
+// graph holds stay valid; the standalone entry points share the library's
+// (library_workspace), which grows on demand and never frees a buffer it has
+// outgrown before the process ends, since a graph captured earlier may still
+// point at it.
+struct Workspace {
+    float* bound = nullptr;    // depth-bound pyramid, levels 1..5 (floats)
+    size_t bound_cap = 0;
+    float2* cbound = nullptr;  // colour-bound pyramid, levels 0..5 (texels)
+    size_t cbound_cap = 0;
+    double* cons = nullptr;    // depth-consistency gate: 4 doubles per 256-pixel workgroup
+    size_t cons_cap = 0;       // workgroups
+};
+// Sizes each buffer needs for images of cols x rows.
+size_t bound_floats(int cols, int rows);
+size_t cbound_texels(int cols, int rows);
+size_t cons_blocks(int cols, int rows);
+// Allocates the buffers of ws (the colour pyramid only with colors) for
+// images of cols x rows; a buffer already large enough is kept.  Buffers
+// replaced here are freed (owned workspaces are never captured mid-growth).
+int workspace_reserve(Workspace& ws, int cols, int rows, bool colors);
+void workspace_free(Workspace& ws);
+// Grows the library workspace to cols x rows; outgrown buffers are kept
+// (not freed) until the process ends.  Returns an error inside a stream
+// capture when it would have to allocate.
+int library_workspace_reserve(int cols, int rows, bool colors, bool gate);
+Workspace& library_workspace();
+
+// The integrations and the fallback gate over an explicit workspace (the
+// public entry points pass the library's, a pipeline its own).  A workspace
+// too small for the image runs the integrations unculled (same result).
+int integrate_tsdf_ws(const Workspace& ws, bool profile, const uint16_t* depth, float* depth_scaled,
+                      kfx_tsdf* volume, const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                      const kfx_point_3f* translation, float truncation_distance, const kfx_point_3f* cell_size,
+                      const kfx_round_buffer* rb);
+int integrate_tsdf_posed_ws(const Workspace& ws, const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
+                            const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
+                            float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
+                            const kfx_round_buffer* rb, const int32_t* dev_skip);
+int integrate_colors_ws(const Workspace& ws, kfx_color* color_volume, const kfx_color* rgb_frame,
+                        const kfx_xyz* vertex_map, const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                        const kfx_point_3f* translation, float truncation_distance, int max_weight,
+                        const kfx_point_3f* cell_size, const kfx_round_buffer* rb);
+int integrate_colors_posed_ws(const Workspace& ws, kfx_color* color_volume, const kfx_color* rgb_frame,
+                              const kfx_xyz* vertex_map, const kfx_intrinsics* intr, const float* dev_pose,
+                              float truncation_distance, int max_weight, const kfx_point_3f* cell_size,
+                              const kfx_round_buffer* rb, const int32_t* dev_skip);
+// The fallback gate (kfx_denoise.hip); ws.cons must hold cons_blocks().
+int depth_consistency(const Workspace& ws, const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
+                      const kfx_intrinsics* intr, float max_diff, double* dev_stats, double fallback_m,
+                      int32_t* dev_flag, int32_t* host_flag);
 
 // Library stream (created by kfx_init, default priority) on which every
 // entry point enqueues its work.
@@ -35,7 +80,7 @@ bool initialised();
 int set_error(int code, const char* fmt, ...);
 
 // Brackets a kernel launch with HIP events on the library stream while
-// kfx_profile_enable(1) is in force (no-op otherwise).
+// kfx_profile_enable(1) is in force (no-op otherwise, and for a null name).
 struct ProfScope {
     explicit ProfScope(const char* name);
     ~ProfScope();

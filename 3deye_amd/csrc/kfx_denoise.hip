@@ -1059,37 +1059,17 @@ __global__ void __launch_bounds__(256) consistency_reduce_kernel(const double* _
     }
 }
 
-// Library-owned workgroup partials of the consistency gate.
-static double* g_cons = nullptr;
-static size_t g_cons_cap = 0;  // workgroups
-
-int consistency_reserve(int cols, int rows) {
-    const size_t need = divup((unsigned)(cols * rows), 256u);
-    if (need <= g_cons_cap) return KFX_OK;
-    if (g_cons) KFX_CHECK_HIP(hipFree(g_cons));
-    g_cons = nullptr;
-    g_cons_cap = 0;
-    KFX_CHECK_HIP(hipMalloc((void**)&g_cons, need * 4 * sizeof(double)));
-    g_cons_cap = need;
-    return KFX_OK;
-}
-
-int depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
+int depth_consistency(const Workspace& ws, const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
                       const kfx_intrinsics* intr, float max_diff, double* dev_stats, double fallback_m,
                       int32_t* dev_flag, int32_t* host_flag) {
     const int P = intr->cols * intr->rows;
     const int nblk = (int)divup((unsigned)P, 256u);
-    if ((size_t)nblk > g_cons_cap) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
-        KFX_REQUIRE(cs == hipStreamCaptureStatusNone,
-                    "kfx_depth_consistency: no workspace reserved for %dx%d frames inside a stream capture",
-                    intr->cols, intr->rows);
-        KFX_TRY(consistency_reserve(intr->cols, intr->rows));
-    }
-    depth_consistency_kernel<<<nblk, 256, 0, stream()>>>(vmap_model, dev_pose, depth, *intr, max_diff, g_cons);
+    KFX_REQUIRE(ws.cons != nullptr && (size_t)nblk <= ws.cons_cap,
+                "kfx_depth_consistency: workspace holds %zu workgroup partials, %dx%d frames need %d", ws.cons_cap,
+                intr->cols, intr->rows, nblk);
+    depth_consistency_kernel<<<nblk, 256, 0, stream()>>>(vmap_model, dev_pose, depth, *intr, max_diff, ws.cons);
     KFX_CHECK_LAUNCH();
-    consistency_reduce_kernel<<<1, 256, 0, stream()>>>(g_cons, nblk, dev_stats, fallback_m, dev_flag, host_flag);
+    consistency_reduce_kernel<<<1, 256, 0, stream()>>>(ws.cons, nblk, dev_stats, fallback_m, dev_flag, host_flag);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
@@ -1114,7 +1094,9 @@ int kfx_depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, cons
                           const kfx_intrinsics* intr, float max_diff_m, double* dev_stats) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(vmap_model && dev_pose && depth && intr && dev_stats, "kfx_depth_consistency: null pointer");
-    return depth_consistency(vmap_model, dev_pose, depth, intr, max_diff_m, dev_stats, 0.0, nullptr, nullptr);
+    KFX_TRY(library_workspace_reserve(intr->cols, intr->rows, false, true));
+    return depth_consistency(library_workspace(), vmap_model, dev_pose, depth, intr, max_diff_m, dev_stats, 0.0,
+                             nullptr, nullptr);
 }
 
 }  // extern "C"

@@ -713,19 +713,20 @@ struct kfx_icp {
     kfx::IcpState* host_state;  // pinned host copy for kfx_icp_read
     int max_blocks;
     bool publish_next;  // the next iteration's solve writes host_state
+    bool published;     // a publishing solve was enqueued since the last reset or translate
 };
 
 using namespace kfx;
 
 template <int M>
 static void icp_iterate_launch(kfx_icp* h, const kfx_xyz* vc, const kfx_xyz* nc, const kfx_xyz* vp, const kfx_xyz* np,
-                               const IcpFrame& f, int nblk, bool two, int tag, double conv_rot, double conv_trans) {
+                               const IcpFrame& f, int nblk, bool two, int tag, double conv_rot, double conv_trans,
+                               bool publish) {
     if (two)
         icp_sums_kernel<2, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
     else
         icp_sums_kernel<1, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
-    IcpState* mirror = h->publish_next ? h->host_state : nullptr;
-    h->publish_next = false;
+    IcpState* mirror = publish ? h->host_state : nullptr;
     icp_solve_kernel<M><<<1, kIcpThreads, 0, stream()>>>(h->state, h->partials, nblk, tag, conv_rot,
                                                                 conv_trans, mirror);
 }
@@ -791,6 +792,8 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
     h->sum_mode = KFX_ICP_SUM_FP32_KAHAN;
     h->host_state = nullptr;
     h->partials = nullptr;
+    h->publish_next = false;
+    h->published = false;
     if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(double) * kIcpEntries * (size_t)h->max_blocks);
     if (e == hipSuccess) e = hipHostMalloc((void**)&h->host_state, sizeof(IcpState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(h->state, 0, sizeof(IcpState), stream());
@@ -822,6 +825,7 @@ int kfx_icp_reset(kfx_icp* h, const kfx_pose* pose) {
     KFX_REQUIRE(h && pose, "kfx_icp_reset: null pointer");
     icp_reset_kernel<<<1, 64, 0, stream()>>>(h->state, *pose);
     KFX_CHECK_LAUNCH();
+    h->published = false;
     return KFX_OK;
 }
 
@@ -830,6 +834,10 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
                     const kfx_matrix_3f* rotation_previous_inverse, const kfx_point_3f* translation_previous,
                     const kfx_intrinsics* intr, float distance_threshold, float angle_threshold, int level_tag,
                     double converge_rotation, double converge_translation) {
+    // A publish request applies to this call only, whether or not it
+    // passes its checks.
+    const bool publish = h != nullptr && h->publish_next;
+    if (h != nullptr) h->publish_next = false;
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(h && vmap_current && nmap_current && vmap_previous && nmap_previous && intr,
                 "kfx_icp_iterate: null pointer");
@@ -848,7 +856,8 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
     ProfScope prof("icp_fused");
     auto launch = [&](auto mode) {
         icp_iterate_launch<decltype(mode)::value>(h, vmap_current, nmap_current, vmap_previous, nmap_previous, f,
-                                                  nblk, two, level_tag, converge_rotation, converge_translation);
+                                                  nblk, two, level_tag, converge_rotation, converge_translation,
+                                                  publish);
     };
     switch (h->sum_mode) {
         case KFX_ICP_SUM_FP32: launch(std::integral_constant<int, kSumPlain>{}); break;
@@ -856,6 +865,7 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
         default: launch(std::integral_constant<int, kSumKahan>{}); break;
     }
     KFX_CHECK_LAUNCH();
+    if (publish) h->published = true;
     return KFX_OK;
 }
 
@@ -880,6 +890,7 @@ int kfx_icp_translate(kfx_icp* h, double dx, double dy, double dz) {
     KFX_REQUIRE(h != nullptr, "kfx_icp_translate: null handle");
     icp_translate_kernel<<<1, 64, 0, stream()>>>(h->state, dx, dy, dz);
     KFX_CHECK_LAUNCH();
+    h->published = false;
     return KFX_OK;
 }
 
@@ -909,6 +920,8 @@ int kfx_icp_publish_next(kfx_icp* h) {
 int kfx_icp_read_published(kfx_icp* h, kfx_icp_result* out) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(h && out, "kfx_icp_read_published: null pointer");
+    KFX_REQUIRE(h->published,
+                "kfx_icp_read_published: no publishing iteration was enqueued since the last reset or translate");
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
     const IcpState& s = *h->host_state;
     for (int i = 0; i < 9; ++i) out->pose.R[i] = s.R[i];

@@ -20,13 +20,6 @@
 
 #include "kfx_common.h"
 
-namespace kfx {
-int depth_consistency(const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
-                      const kfx_intrinsics* intr, float max_diff, double* dev_stats, double fallback_m,
-                      int32_t* dev_flag, int32_t* host_flag);
-}
-
-
 struct kfx_pipeline {
     kfx_pipeline_config cfg;
     kfx_round_buffer rb;
@@ -57,6 +50,8 @@ struct kfx_pipeline {
     double* hstats = nullptr;             // gate statistics (pinned host memory the gate kernel writes)
     int32_t* dgate = nullptr;             // fallback decision of the gate (device)
     int32_t* hgate = nullptr;             // host mirror of dgate (pinned, written by the gate kernel)
+    kfx::Workspace ws;                    // depth- and colour-bound pyramids, gate partials: this pipeline's own,
+                                          // so its captured graph never points at another's buffers
     uint64_t total_extracted = 0;
     int shifts = 0;
     int fallbacks = 0;
@@ -188,7 +183,8 @@ int integrate_host_pose(kfx_pipeline* p) {
     mat_T(p->R, Ri);
     const kfx_matrix_3f m = to_m3f(Ri);
     const kfx_point_3f tt = to_p3f(p->t);
-    KFX_TRY(kfx_integrate_tsdf(p->raw, p->scaled, p->vol, &p->lvl[0], &m, &tt, p->cfg.truncation, &p->cell, &p->rb));
+    KFX_TRY(integrate_tsdf_ws(p->ws, true, p->raw, p->scaled, p->vol, &p->lvl[0], &m, &tt, p->cfg.truncation,
+                              &p->cell, &p->rb));
     return step_sync(p);
 }
 
@@ -207,8 +203,8 @@ int colors_host_pose(kfx_pipeline* p) {
     mat_T(p->R, Ri);
     const kfx_matrix_3f m = to_m3f(Ri);
     const kfx_point_3f tt = to_p3f(p->t);
-    KFX_TRY(kfx_integrate_colors(p->cvol, p->rgb, p->vp[0], &p->lvl[0], &m, &tt, p->cfg.truncation,
-                                 p->cfg.color_max_weight, &p->cell, &p->rb));
+    KFX_TRY(integrate_colors_ws(p->ws, p->cvol, p->rgb, p->vp[0], &p->lvl[0], &m, &tt, p->cfg.truncation,
+                                p->cfg.color_max_weight, &p->cell, &p->rb));
     return step_sync(p);
 }
 
@@ -260,13 +256,13 @@ int icp_part(kfx_pipeline* p, bool begin) {
 int map_part_skip(kfx_pipeline* p, const int32_t* skip) {
     const float *dpose = nullptr, *dprev = nullptr;
     KFX_TRY(kfx_icp_pose_device(p->icp, &dpose, &dprev));
-    KFX_TRY(kfx_integrate_tsdf_posed(p->raw, p->scaled, p->vol, &p->lvl[0], dpose, dprev, p->cfg.integrate_threshold,
-                                     p->cfg.truncation, &p->cell, &p->rb, skip));
+    KFX_TRY(integrate_tsdf_posed_ws(p->ws, p->raw, p->scaled, p->vol, &p->lvl[0], dpose, dprev,
+                                    p->cfg.integrate_threshold, p->cfg.truncation, &p->cell, &p->rb, skip));
     KFX_TRY(kfx_raycast_posed(p->vol, p->vp[0], p->np[0], &p->rb, dpose, p->cfg.truncation, &p->lvl[0], &p->cell,
                               skip));
     if (p->cvol)
-        KFX_TRY(kfx_integrate_colors_posed(p->cvol, p->rgb, p->vp[0], &p->lvl[0], dpose, p->cfg.truncation,
-                                           p->cfg.color_max_weight, &p->cell, &p->rb, skip));
+        KFX_TRY(integrate_colors_posed_ws(p->ws, p->cvol, p->rgb, p->vp[0], &p->lvl[0], dpose, p->cfg.truncation,
+                                          p->cfg.color_max_weight, &p->cell, &p->rb, skip));
     return resize_prev(p);
 }
 
@@ -292,7 +288,7 @@ int learned_frame(kfx_pipeline* p) {
     // The gate's statistics go straight to pinned host memory and its flag
     // both to the device (read by the skips below) and to the host: no copy
     // commands after the graph.
-    KFX_TRY(depth_consistency(p->vp[0], dpose, p->dep[0], &p->lvl[0], 0.1f, p->hstats,
+    KFX_TRY(depth_consistency(p->ws, p->vp[0], dpose, p->dep[0], &p->lvl[0], 0.1f, p->hstats,
                               1e-3 * (double)p->cfg.fallback_rms_mm, p->dgate, p->hgate));
     // The raycast below overwrites vp[0]: the gate has read it already.
     return map_part_skip(p, p->dgate);
@@ -469,6 +465,7 @@ int kfx_pipeline_destroy(kfx_pipeline* p) {
     if (p->hgate) (void)hipHostFree(p->hgate);
     kfx_icp_destroy(p->icp);
     kfx_slicer_destroy(p->slicer);
+    workspace_free(p->ws);
     delete p;
     return KFX_OK;
 }
@@ -532,9 +529,7 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     int rc = kfx_icp_create((int)npix, &p->icp);
     if (!rc) rc = kfx_icp_set_sum_mode(p->icp, p->cfg.icp_sum_mode);
     if (!rc) rc = kfx_slicer_create(&rb, &p->slicer);
-    if (!rc) rc = integrate_reserve(p->cfg.intr.cols, p->cfg.intr.rows);
-    if (!rc) rc = consistency_reserve(p->cfg.intr.cols, p->cfg.intr.rows);
-    if (!rc && p->cvol) rc = colors_reserve(p->cfg.intr.cols, p->cfg.intr.rows);
+    if (!rc) rc = workspace_reserve(p->ws, p->cfg.intr.cols, p->cfg.intr.rows, p->cvol != nullptr);
     if (!rc) rc = kfx_init_tsdf(p->vol, &rb);
     if (!rc && p->cvol) rc = kfx_init_colors(p->cvol, &rb);
     if (!rc) rc = kfx_synchronize();

@@ -54,7 +54,7 @@ hipEvent_t pooled_event() {
 }  // namespace
 
 ProfScope::ProfScope(const char* name) : slot(-1) {
-    if (!g_prof_on) return;
+    if (!g_prof_on || name == nullptr) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(g_stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     std::lock_guard<std::mutex> lk(g_prof_mu);

@@ -7,6 +7,8 @@
 // one 256-byte contiguous segment; each lane sweeps a run of z for its
 // column ("z-major sweep"), the reference's (Nx, Ny) launch with the z loop
 // split over several workgroups so the chip has enough waves in flight.
+#include <vector>
+
 #include "kfx_common.h"
 
 namespace kfx {
@@ -401,11 +403,14 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
         for (int k = 0; k < kIntegrateBatch; ++k) {
             if (!upd[k]) continue;
             const float tp = tsdf_decode(prev[k].value);
-            const int w = prev[k].weight;
+            // Weights outside [0, KFX_MAX_TSDF_WEIGHT] (a volume not written
+            // by this rule) count as the nearest bound, which keeps w + 1
+            // inside tsdf_average_quotient's exact range.
+            const int w = max(0, min((int)prev[k].weight, KFX_MAX_TSDF_WEIGHT));
             const float avg = tsdf_average_quotient(tp * (float)w + tsdf_new[k], w + 1);
             const int w_new = min(w + 1, KFX_MAX_TSDF_WEIGHT);
             const int16_t v_new = tsdf_encode(avg);
-            if (v_new != prev[k].value || w_new != w) {
+            if (v_new != prev[k].value || w_new != prev[k].weight) {
                 kfx_tsdf out;
                 out.value = v_new;
                 out.weight = (int16_t)w_new;
@@ -683,10 +688,6 @@ int kfx_init_tsdf(kfx_tsdf* volume, const kfx_round_buffer* rb) {
 
 namespace kfx {
 
-// Library-owned depth-bound pyramid (one per process, grown on demand).
-static float* g_bound = nullptr;
-static size_t g_bound_cap = 0;  // floats
-
 static size_t bound_layout(int cols, int rows, int off[kBoundLevels + 1]) {
     size_t total = 0;
     off[0] = 0;
@@ -697,38 +698,26 @@ static size_t bound_layout(int cols, int rows, int off[kBoundLevels + 1]) {
     return total;
 }
 
-int integrate_reserve(int cols, int rows) {
+size_t bound_floats(int cols, int rows) {
     int off[kBoundLevels + 1];
-    const size_t need = bound_layout(cols, rows, off);
-    if (need <= g_bound_cap) return KFX_OK;
-    if (g_bound) KFX_CHECK_HIP(hipFree(g_bound));
-    g_bound = nullptr;
-    g_bound_cap = 0;
-    KFX_CHECK_HIP(hipMalloc((void**)&g_bound, need * sizeof(float)));
-    g_bound_cap = need;
-    return KFX_OK;
+    return bound_layout(cols, rows, off);
 }
 
 // Scales the depth into depth_scaled and builds its depth-bound pyramid into
-// the library buffer (one launch), pointing p at it.  Inside a stream
-// capture the buffer cannot be allocated: without a large enough reservation
-// only the depth is scaled and the integration runs unculled (same result,
+// ws.bound (one launch), pointing p at it.  A workspace too small for the
+// image (the library's, inside a stream capture, before any reservation)
+// only scales the depth, and the integration runs unculled (same result,
 // more work).
-static int integrate_bound(const uint16_t* depth, float* depth_scaled, const kfx_intrinsics* intr,
-                           IntegrateParams& p) {
+static int integrate_bound(const Workspace& ws, bool profile, const uint16_t* depth, float* depth_scaled,
+                           const kfx_intrinsics* intr, IntegrateParams& p) {
     p.bound = nullptr;
     const size_t need = bound_layout(intr->cols, intr->rows, p.bound_off);
-    if (need > g_bound_cap) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
-        if (cs != hipStreamCaptureStatusNone) return scale_depth_launch(depth, depth_scaled, intr);
-        KFX_TRY(integrate_reserve(intr->cols, intr->rows));
-    }
-    ProfScope prof("scale_depth");
+    if (ws.bound == nullptr || need > ws.bound_cap) return scale_depth_launch(depth, depth_scaled, intr);
+    ProfScope prof(profile ? "scale_depth" : nullptr);
     dim3 grid(divup(intr->cols, kBoundTile), divup(intr->rows, kBoundTile));
-    depth_bound_kernel<<<grid, kBoundTile * kBoundTile, 0, stream()>>>(depth, depth_scaled, *intr, g_bound, p);
+    depth_bound_kernel<<<grid, kBoundTile * kBoundTile, 0, stream()>>>(depth, depth_scaled, *intr, ws.bound, p);
     KFX_CHECK_LAUNCH();
-    p.bound = g_bound;
+    p.bound = ws.bound;
     return KFX_OK;
 }
 
@@ -805,10 +794,6 @@ static int integrate_setup(const char* who, const uint16_t* depth, float* depth_
     return scale ? scale_depth_launch(depth, depth_scaled, intr) : KFX_OK;
 }
 
-// Library-owned colour-bound pyramid (float2 per texel, levels 0..5).
-static float2* g_cbound = nullptr;
-static size_t g_cbound_cap = 0;  // texels
-
 static size_t color_bound_layout(int cols, int rows, int off[kBoundLevels + 1]) {
     size_t total = (size_t)cols * (size_t)rows;
     off[0] = 0;
@@ -819,35 +804,113 @@ static size_t color_bound_layout(int cols, int rows, int off[kBoundLevels + 1]) 
     return total;
 }
 
-int colors_reserve(int cols, int rows) {
+size_t cbound_texels(int cols, int rows) {
     int off[kBoundLevels + 1];
-    const size_t need = color_bound_layout(cols, rows, off);
-    if (need <= g_cbound_cap) return KFX_OK;
-    if (g_cbound) KFX_CHECK_HIP(hipFree(g_cbound));
-    g_cbound = nullptr;
-    g_cbound_cap = 0;
-    KFX_CHECK_HIP(hipMalloc((void**)&g_cbound, need * sizeof(float2)));
-    g_cbound_cap = need;
+    return color_bound_layout(cols, rows, off);
+}
+
+size_t cons_blocks(int cols, int rows) { return divup((unsigned)(cols * rows), 256u); }
+
+// Builds the colour-bound pyramid of vertex_map into ws.cbound and returns it
+// (null: the workspace is too small, run unculled).
+static int colors_bound(const Workspace& ws, bool profile, const kfx_xyz* vertex_map, IntegrateParams& p,
+                        const float* dpose, const int32_t* dskip, const float2** out) {
+    *out = nullptr;
+    const size_t need = color_bound_layout(p.cols, p.rows, p.bound_off);
+    if (ws.cbound == nullptr || need > ws.cbound_cap) return KFX_OK;
+    ProfScope prof(profile ? "color_bound" : nullptr);
+    dim3 grid(divup(p.cols, kBoundTile), divup(p.rows, kBoundTile));
+    color_bound_kernel<<<grid, kBoundTile * kBoundTile, 0, stream()>>>(vertex_map, ws.cbound, p, dpose, dskip);
+    KFX_CHECK_LAUNCH();
+    *out = ws.cbound;
     return KFX_OK;
 }
 
-// Builds the colour-bound pyramid of vertex_map and returns it (null: run
-// unculled, inside a stream capture without a large enough reservation).
-static int colors_bound(const kfx_xyz* vertex_map, IntegrateParams& p, const float* dpose, const int32_t* dskip,
-                        const float2** out) {
-    *out = nullptr;
-    const size_t need = color_bound_layout(p.cols, p.rows, p.bound_off);
-    if (need > g_cbound_cap) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
-        if (cs != hipStreamCaptureStatusNone) return KFX_OK;
-        KFX_TRY(colors_reserve(p.cols, p.rows));
+int workspace_reserve(Workspace& ws, int cols, int rows, bool colors) {
+    const size_t nb = bound_floats(cols, rows), nc = cbound_texels(cols, rows), ng = cons_blocks(cols, rows);
+    if (ws.bound_cap < nb) {
+        if (ws.bound) KFX_CHECK_HIP(hipFree(ws.bound));
+        ws.bound = nullptr;
+        ws.bound_cap = 0;
+        KFX_CHECK_HIP(hipMalloc((void**)&ws.bound, nb * sizeof(float)));
+        ws.bound_cap = nb;
     }
-    ProfScope prof("color_bound");
-    dim3 grid(divup(p.cols, kBoundTile), divup(p.rows, kBoundTile));
-    color_bound_kernel<<<grid, kBoundTile * kBoundTile, 0, stream()>>>(vertex_map, g_cbound, p, dpose, dskip);
-    KFX_CHECK_LAUNCH();
-    *out = g_cbound;
+    if (colors && ws.cbound_cap < nc) {
+        if (ws.cbound) KFX_CHECK_HIP(hipFree(ws.cbound));
+        ws.cbound = nullptr;
+        ws.cbound_cap = 0;
+        KFX_CHECK_HIP(hipMalloc((void**)&ws.cbound, nc * sizeof(float2)));
+        ws.cbound_cap = nc;
+    }
+    if (ws.cons_cap < ng) {
+        if (ws.cons) KFX_CHECK_HIP(hipFree(ws.cons));
+        ws.cons = nullptr;
+        ws.cons_cap = 0;
+        KFX_CHECK_HIP(hipMalloc((void**)&ws.cons, ng * 4 * sizeof(double)));
+        ws.cons_cap = ng;
+    }
+    return KFX_OK;
+}
+
+void workspace_free(Workspace& ws) {
+    if (ws.bound) (void)hipFree(ws.bound);
+    if (ws.cbound) (void)hipFree(ws.cbound);
+    if (ws.cons) (void)hipFree(ws.cons);
+    ws = Workspace();
+}
+
+namespace {
+Workspace g_lib_ws;
+std::vector<void*> g_outgrown;  // library buffers replaced by larger ones, kept until exit
+}  // namespace
+
+Workspace& library_workspace() { return g_lib_ws; }
+
+int library_workspace_reserve(int cols, int rows, bool colors, bool gate) {
+    const bool need_b = g_lib_ws.bound_cap < bound_floats(cols, rows);
+    const bool need_c = colors && g_lib_ws.cbound_cap < cbound_texels(cols, rows);
+    const bool need_g = gate && g_lib_ws.cons_cap < cons_blocks(cols, rows);
+    if (!need_b && !need_c && !need_g) return KFX_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return set_error(KFX_ERR_INVALID_ARG, "no library workspace reserved for %dx%d images inside a stream capture",
+                         cols, rows);
+    Workspace grown;
+    if (need_b) KFX_CHECK_HIP(hipMalloc((void**)&grown.bound, bound_floats(cols, rows) * sizeof(float)));
+    if (need_c) {
+        hipError_t e = hipMalloc((void**)&grown.cbound, cbound_texels(cols, rows) * sizeof(float2));
+        if (e != hipSuccess) { workspace_free(grown); KFX_CHECK_HIP(e); }
+    }
+    if (need_g) {
+        hipError_t e = hipMalloc((void**)&grown.cons, cons_blocks(cols, rows) * 4 * sizeof(double));
+        if (e != hipSuccess) { workspace_free(grown); KFX_CHECK_HIP(e); }
+    }
+    if (need_b) {
+        if (g_lib_ws.bound) g_outgrown.push_back(g_lib_ws.bound);
+        g_lib_ws.bound = grown.bound;
+        g_lib_ws.bound_cap = bound_floats(cols, rows);
+    }
+    if (need_c) {
+        if (g_lib_ws.cbound) g_outgrown.push_back(g_lib_ws.cbound);
+        g_lib_ws.cbound = grown.cbound;
+        g_lib_ws.cbound_cap = cbound_texels(cols, rows);
+    }
+    if (need_g) {
+        if (g_lib_ws.cons) g_outgrown.push_back(g_lib_ws.cons);
+        g_lib_ws.cons = grown.cons;
+        g_lib_ws.cons_cap = cons_blocks(cols, rows);
+    }
+    return KFX_OK;
+}
+
+// Library workspace for a standalone call: reserved outside a capture;
+// inside one, whatever was reserved before (too small: unculled).
+static int library_ws_for(int cols, int rows, bool colors, const Workspace** out) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
+    if (cs == hipStreamCaptureStatusNone) KFX_TRY(library_workspace_reserve(cols, rows, colors, false));
+    *out = &g_lib_ws;
     return KFX_OK;
 }
 
@@ -859,6 +922,86 @@ static int colors_check(const char* who, kfx_color* color_volume, const kfx_colo
     return KFX_OK;
 }
 
+
+int integrate_tsdf_ws(const Workspace& ws, bool profile, const uint16_t* depth, float* depth_scaled,
+                      kfx_tsdf* volume, const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                      const kfx_point_3f* translation, float truncation_distance, const kfx_point_3f* cell_size,
+                      const kfx_round_buffer* rb) {
+    KFX_REQUIRE(volume != nullptr, "kfx_integrate_tsdf: null volume");
+    IntegrateParams p;
+    int rc = integrate_setup("kfx_integrate_tsdf", depth, depth_scaled, intr, rotation_inverse, translation,
+                             truncation_distance, cell_size, rb, p, false);
+    if (rc) return rc;
+    KFX_TRY(integrate_bound(ws, profile, depth, depth_scaled, intr, p));
+    ProfScope prof(profile ? "integrate" : nullptr);
+    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
+    integrate_kernel<false><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, nullptr, nullptr, 0.0f, nullptr);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int integrate_tsdf_posed_ws(const Workspace& ws, const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
+                            const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
+                            float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
+                            const kfx_round_buffer* rb, const int32_t* dev_skip) {
+    KFX_REQUIRE(volume != nullptr && dev_pose != nullptr, "kfx_integrate_tsdf_posed: null pointer");
+    const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    const kfx_point_3f zero = {0, 0, 0};
+    IntegrateParams p;
+    int rc = integrate_setup("kfx_integrate_tsdf_posed", depth, depth_scaled, intr, &ident, &zero,
+                             truncation_distance, cell_size, rb, p, false);
+    if (rc) return rc;
+    KFX_TRY(integrate_bound(ws, true, depth, depth_scaled, intr, p));
+    ProfScope prof("integrate");
+    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
+    integrate_kernel<true><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, dev_pose, dev_prev_pose_inv,
+                                                         motion_threshold, dev_skip);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int integrate_colors_ws(const Workspace& ws, kfx_color* color_volume, const kfx_color* rgb_frame,
+                        const kfx_xyz* vertex_map, const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                        const kfx_point_3f* translation, float truncation_distance, int max_weight,
+                        const kfx_point_3f* cell_size, const kfx_round_buffer* rb) {
+    KFX_TRY(colors_check("kfx_integrate_colors", color_volume, rgb_frame, vertex_map, max_weight));
+    IntegrateParams p;
+    KFX_TRY(sweep_setup("kfx_integrate_colors", intr, rotation_inverse, translation, truncation_distance, cell_size,
+                        rb, p));
+    const float2* cb = nullptr;
+    KFX_TRY(colors_bound(ws, true, vertex_map, p, nullptr, nullptr, &cb));
+    ProfScope prof("integrate_colors");
+    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
+    integrate_colors_kernel<false><<<grid, block, 0, stream()>>>(rgb_frame, vertex_map, color_volume, p, cb,
+                                                                 max_weight, nullptr, nullptr);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int integrate_colors_posed_ws(const Workspace& ws, kfx_color* color_volume, const kfx_color* rgb_frame,
+                              const kfx_xyz* vertex_map, const kfx_intrinsics* intr, const float* dev_pose,
+                              float truncation_distance, int max_weight, const kfx_point_3f* cell_size,
+                              const kfx_round_buffer* rb, const int32_t* dev_skip) {
+    KFX_TRY(colors_check("kfx_integrate_colors_posed", color_volume, rgb_frame, vertex_map, max_weight));
+    KFX_REQUIRE(dev_pose != nullptr, "kfx_integrate_colors_posed: null pose");
+    const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    const kfx_point_3f zero = {0, 0, 0};
+    IntegrateParams p;
+    KFX_TRY(sweep_setup("kfx_integrate_colors_posed", intr, &ident, &zero, truncation_distance, cell_size, rb, p));
+    const float2* cb = nullptr;
+    KFX_TRY(colors_bound(ws, true, vertex_map, p, dev_pose, dev_skip, &cb));
+    ProfScope prof("integrate_colors");
+    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
+    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
+    integrate_colors_kernel<true><<<grid, block, 0, stream()>>>(rgb_frame, vertex_map, color_volume, p, cb,
+                                                                max_weight, dev_pose, dev_skip);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
 }  // namespace kfx
 
 extern "C" {
@@ -867,39 +1010,24 @@ int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* vol
                        const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
                        const kfx_point_3f* translation, float truncation_distance,
                        const kfx_point_3f* cell_size, const kfx_round_buffer* rb) {
-    KFX_REQUIRE(volume != nullptr, "kfx_integrate_tsdf: null volume");
-    IntegrateParams p;
-    int rc = integrate_setup("kfx_integrate_tsdf", depth, depth_scaled, intr, rotation_inverse, translation,
-                             truncation_distance, cell_size, rb, p, false);
-    if (rc) return rc;
-    KFX_TRY(integrate_bound(depth, depth_scaled, intr, p));
-    ProfScope prof("integrate");
-    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
-    integrate_kernel<false><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, nullptr, nullptr, 0.0f, nullptr);
-    KFX_CHECK_LAUNCH();
-    return KFX_OK;
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(intr != nullptr, "kfx_integrate_tsdf: null pointer");
+    const Workspace* ws = nullptr;
+    KFX_TRY(library_ws_for(intr->cols, intr->rows, false, &ws));
+    return integrate_tsdf_ws(*ws, true, depth, depth_scaled, volume, intr, rotation_inverse, translation,
+                             truncation_distance, cell_size, rb);
 }
 
 int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
                              const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
                              float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
                              const kfx_round_buffer* rb, const int32_t* dev_skip) {
-    KFX_REQUIRE(volume != nullptr && dev_pose != nullptr, "kfx_integrate_tsdf_posed: null pointer");
-    const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
-    const kfx_point_3f zero = {0, 0, 0};
-    IntegrateParams p;
-    int rc = integrate_setup("kfx_integrate_tsdf_posed", depth, depth_scaled, intr, &ident, &zero,
-                             truncation_distance, cell_size, rb, p, false);
-    if (rc) return rc;
-    KFX_TRY(integrate_bound(depth, depth_scaled, intr, p));
-    ProfScope prof("integrate");
-    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
-    integrate_kernel<true><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, dev_pose, dev_prev_pose_inv,
-                                                         motion_threshold, dev_skip);
-    KFX_CHECK_LAUNCH();
-    return KFX_OK;
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(intr != nullptr, "kfx_integrate_tsdf_posed: null pointer");
+    const Workspace* ws = nullptr;
+    KFX_TRY(library_ws_for(intr->cols, intr->rows, false, &ws));
+    return integrate_tsdf_posed_ws(*ws, depth, depth_scaled, volume, intr, dev_pose, dev_prev_pose_inv,
+                                   motion_threshold, truncation_distance, cell_size, rb, dev_skip);
 }
 
 int kfx_integrate_count_updates(const uint16_t* depth, float* depth_scaled, const kfx_intrinsics* intr,
@@ -934,40 +1062,24 @@ int kfx_integrate_colors(kfx_color* color_volume, const kfx_color* rgb_frame, co
                          const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
                          const kfx_point_3f* translation, float truncation_distance, int max_weight,
                          const kfx_point_3f* cell_size, const kfx_round_buffer* rb) {
-    KFX_TRY(colors_check("kfx_integrate_colors", color_volume, rgb_frame, vertex_map, max_weight));
-    IntegrateParams p;
-    KFX_TRY(sweep_setup("kfx_integrate_colors", intr, rotation_inverse, translation, truncation_distance, cell_size,
-                        rb, p));
-    const float2* cb = nullptr;
-    KFX_TRY(colors_bound(vertex_map, p, nullptr, nullptr, &cb));
-    ProfScope prof("integrate_colors");
-    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
-    integrate_colors_kernel<false><<<grid, block, 0, stream()>>>(rgb_frame, vertex_map, color_volume, p, cb,
-                                                                 max_weight, nullptr, nullptr);
-    KFX_CHECK_LAUNCH();
-    return KFX_OK;
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(intr != nullptr, "kfx_integrate_colors: null pointer");
+    const Workspace* ws = nullptr;
+    KFX_TRY(library_ws_for(intr->cols, intr->rows, true, &ws));
+    return integrate_colors_ws(*ws, color_volume, rgb_frame, vertex_map, intr, rotation_inverse, translation,
+                               truncation_distance, max_weight, cell_size, rb);
 }
 
 int kfx_integrate_colors_posed(kfx_color* color_volume, const kfx_color* rgb_frame, const kfx_xyz* vertex_map,
                                const kfx_intrinsics* intr, const float* dev_pose, float truncation_distance,
                                int max_weight, const kfx_point_3f* cell_size, const kfx_round_buffer* rb,
                                const int32_t* dev_skip) {
-    KFX_TRY(colors_check("kfx_integrate_colors_posed", color_volume, rgb_frame, vertex_map, max_weight));
-    KFX_REQUIRE(dev_pose != nullptr, "kfx_integrate_colors_posed: null pose");
-    const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
-    const kfx_point_3f zero = {0, 0, 0};
-    IntegrateParams p;
-    KFX_TRY(sweep_setup("kfx_integrate_colors_posed", intr, &ident, &zero, truncation_distance, cell_size, rb, p));
-    const float2* cb = nullptr;
-    KFX_TRY(colors_bound(vertex_map, p, dev_pose, dev_skip, &cb));
-    ProfScope prof("integrate_colors");
-    dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
-    integrate_colors_kernel<true><<<grid, block, 0, stream()>>>(rgb_frame, vertex_map, color_volume, p, cb,
-                                                                max_weight, dev_pose, dev_skip);
-    KFX_CHECK_LAUNCH();
-    return KFX_OK;
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(intr != nullptr, "kfx_integrate_colors_posed: null pointer");
+    const Workspace* ws = nullptr;
+    KFX_TRY(library_ws_for(intr->cols, intr->rows, true, &ws));
+    return integrate_colors_posed_ws(*ws, color_volume, rgb_frame, vertex_map, intr, dev_pose, truncation_distance,
+                                     max_weight, cell_size, rb, dev_skip);
 }
 
 int kfx_init_colors(kfx_color* volume, const kfx_round_buffer* rb) {

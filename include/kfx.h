@@ -307,9 +307,12 @@ int kfx_icp_read(kfx_icp* icp, kfx_icp_result* out);
 /* The next kfx_icp_iterate's solve also writes the resulting state to the
  * handle's pinned host copy (also when that iteration is skipped by the
  * early exit), in a graph replay too.  kfx_icp_read_published then
- * synchronises the stream and reads that copy without a copy command; the
- * caller guarantees that a publishing iteration was enqueued since the
- * state last changed.  No reference counterpart: it saves the per-frame
+ * synchronises the stream and reads that copy without a copy command.  It
+ * fails (KFX_ERR_INVALID_ARG) when no publishing iteration was enqueued
+ * since the last kfx_icp_reset or kfx_icp_translate; a graph replay of a
+ * captured publishing iteration counts as enqueued.  A publish request
+ * applies to the next kfx_icp_iterate call only, also when that call fails
+ * its argument checks.  No reference counterpart: it saves the per-frame
  * device-to-host copy of the fused pipeline. */
 int kfx_icp_publish_next(kfx_icp* icp);
 int kfx_icp_read_published(kfx_icp* icp, kfx_icp_result* out);

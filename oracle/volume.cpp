@@ -112,7 +112,9 @@ extern "C" uint64_t orc_integrate_f32(const float* scaled, kfx_tsdf* vol, const 
                 const float tsdf = std::min(1.0f, sdf * inv_trunc);
                 kfx_tsdf& cell = vol[orc_cyclic_address(x, y, z, rb)];
                 const float tp = orc_tsdf_decode(cell.value);
-                const int w = cell.weight;
+                // A stored weight outside [0, 128] (a volume not written by
+                // this rule) counts as the nearest of 0 and 128.
+                const int w = std::max(0, std::min((int)cell.weight, 128));
                 const float avg = (tp * (float)w + tsdf) / (float)(w + 1);
                 cell.value = orc_tsdf_encode(avg);
                 cell.weight = (int16_t)std::min(w + 1, 128);
@@ -148,7 +150,7 @@ extern "C" uint64_t orc_integrate_f64(const double* scaled, kfx_tsdf* vol, const
                 const double tsdf = std::min(1.0, sdf / trunc);
                 kfx_tsdf& cell = vol[orc_cyclic_address(x, y, z, rb)];
                 const double tp = (double)cell.value / 32767.0;
-                const int w = cell.weight;
+                const int w = std::max(0, std::min((int)cell.weight, 128));
                 cell.value = encode_f64((tp * w + tsdf) / (w + 1));
                 cell.weight = (int16_t)std::min(w + 1, 128);
                 ++touched;

@@ -195,3 +195,61 @@ def test_published_state_equals_read(gpu, orc, pair, conv):
     assert bytes(a) == bytes(b)
     assert a.iterations == (1 if conv > 0.0 else 4)
     kfx.call("kfx_icp_destroy", h)
+
+
+@pytest.mark.parametrize("width", [1200, 1201, 37])
+def test_clean_buff_zeroes_every_entry(gpu, width):
+    # Clean_Buff (clean_buff.cl:4-11): the 27 rows of `width` partials become
+    # +0.0; the words after the last row keep their contents.  1200 is the
+    # reference's stride, the others leave a partial last workgroup.
+    kfx = gpu
+    n = 27 * width
+    buf = kfx.Buffer((n + 64) * 4)
+    buf.fill(0xA5)
+    kfx.call("kfx_clean_buff", buf.ptr, width)
+    kfx.synchronize()
+    got = buf.download(np.uint32, n + 64)
+    assert not got[:n].any()
+    assert np.all(got[n:] == 0xA5A5A5A5)
+
+
+def test_clean_buff_rejects_bad_width(gpu):
+    kfx = gpu
+    buf = kfx.Buffer(27 * 4)
+    assert kfx.lib.kfx_clean_buff(buf.ptr, 0) != 0
+    assert kfx.lib.kfx_clean_buff(None, 4) != 0
+
+
+def test_read_published_tracks_what_was_enqueued(gpu, orc, pair):
+    # kfx_icp_read_published fails unless a publishing iteration was enqueued
+    # since the last reset or translate; a publish request is consumed by the
+    # next kfx_icp_iterate call even when that call fails its checks.
+    kfx = gpu
+    intr, maps, (Rp, tp), _ = pair
+    bufs = _upload(kfx, maps)
+    h = _icp_handle(kfx, intr.cols * intr.rows)
+    res = kfx.IcpResult()
+    kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+    assert kfx.lib.kfx_icp_read_published(h, C.byref(res)) != 0
+    assert b"publishing" in kfx.lib.kfx_last_error()
+
+    def iterate(vc):
+        return kfx.lib.kfx_icp_iterate(h, vc, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, C.byref(kfx.matrix3f(Rp.T)),
+                                       C.byref(kfx.point3f(tp)), C.byref(intr), ii.DIST_THR, ii.ANGLE_THR, -1, 0.0,
+                                       0.0)
+
+    kfx.call("kfx_icp_publish_next", h)
+    assert iterate(None) != 0                      # fails its checks, consumes the request
+    assert iterate(bufs[0].ptr) == 0               # no request: publishes nothing
+    assert kfx.lib.kfx_icp_read_published(h, C.byref(res)) != 0
+    kfx.call("kfx_icp_publish_next", h)
+    assert iterate(bufs[0].ptr) == 0
+    kfx.call("kfx_icp_read_published", h, C.byref(res))
+    full = kfx.IcpResult()
+    kfx.call("kfx_icp_read", h, C.byref(full))
+    assert res.iterations == full.iterations == 2
+    assert np.array_equal(np.array(res.pose.t[:]), np.array(full.pose.t[:]))
+    assert np.array_equal(np.array(res.pose.R[:]), np.array(full.pose.R[:]))
+    kfx.call("kfx_icp_translate", h, 0.01, 0.0, 0.0)
+    assert kfx.lib.kfx_icp_read_published(h, C.byref(res)) != 0
+    kfx.call("kfx_icp_destroy", h)

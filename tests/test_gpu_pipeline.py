@@ -184,3 +184,49 @@ def test_learned_pipeline_coverage_gate(gpu):
     infos, gt = _c1_learned(kfx, params, 1)
     assert all(i.fallback == 1 for i in infos[1:]), [i.denoise_rms_mm for i in infos]
     assert kfx.ate_rmse([i.pose_world for i in infos], gt) < 0.01
+
+
+def test_graph_replay_survives_a_larger_pipeline(gpu):
+    # Regression for per-pipeline workspaces: the captured graph of a
+    # 640x480 learned pipeline holds its depth-bound pyramid and gate
+    # partials.  A 1280x720 learned pipeline created and run in between
+    # needs larger ones; the first pipeline's later replays must still
+    # equal an uninterrupted run bit for bit.
+    kfx = gpu
+    intr = kfx.kinect_intrinsics()
+    scene, poses = scenes.c1_scene(), scenes.c1_poses(12)
+    depths = np.stack([kfx.render(scene, poses[i], intr, scenes.C1_SEED, i, noise=True) for i in range(12)])
+    zero = np.zeros(kfx.DENOISER_PARAMS, np.float32)
+
+    def learned(intr_, n):
+        cfg = kfx.default_config(intr=intr_, voxels=kfx.Point3i(n, n, n), volume_size=kfx.Point3f(3.0, 3.0, 3.0),
+                                 mode=kfx.FUSED, use_graph=1, denoiser=1, shift_threshold=100.0)
+        dn = kfx.Denoiser(intr_.cols, intr_.rows, zero)
+        pipe = kfx.Pipeline(cfg, poses[0], (0.0, 0.0, 0.0))
+        pipe.set_denoiser(dn)
+        return pipe, dn
+
+    dd = kfx.Buffer.from_array(depths)
+    fb = depths[0].nbytes
+    ref_pipe, ref_dn = learned(intr, 256)
+    ref = [ref_pipe.process(C.c_void_p(dd.ptr.value + i * fb)) for i in range(12)]
+    ref_pipe.close()
+    ref_dn.close()
+
+    pipe, dn = learned(intr, 256)
+    got = [pipe.process(C.c_void_p(dd.ptr.value + i * fb)) for i in range(6)]
+    big_intr = kfx.Intrinsics(900.0, 900.0, 639.5, 359.5, 1280, 720)
+    big_depths = np.stack([kfx.render(scene, poses[i], big_intr, scenes.C1_SEED, i, noise=True) for i in range(3)])
+    bd = kfx.Buffer.from_array(big_depths)
+    big, big_dn = learned(big_intr, 128)
+    big_infos = [big.process(C.c_void_p(bd.ptr.value + i * big_depths[0].nbytes)) for i in range(3)]
+    big.close()
+    big_dn.close()
+    assert all(i.integrated for i in big_infos)
+    got += [pipe.process(C.c_void_p(dd.ptr.value + i * fb)) for i in range(6, 12)]
+    pipe.close()
+    dn.close()
+    for a, b in zip(ref, got):
+        assert np.array_equal(np.array(a.pose_world.t[:]), np.array(b.pose_world.t[:]))
+        assert np.array_equal(np.array(a.pose_world.R[:]), np.array(b.pose_world.R[:]))
+        assert a.fallback == b.fallback and a.denoise_rms_mm == b.denoise_rms_mm

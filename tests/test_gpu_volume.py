@@ -172,3 +172,25 @@ def test_device_reciprocal_is_correctly_rounded(gpu):
     assert bad.value == 0
     kfx.call("kfx_check_reciprocal", 126, 126, C.byref(bad))
     assert bad.value > 0
+
+
+def test_integrate_clamps_out_of_range_weights(gpu, orc):
+    # A volume not written by the update rule (random values, weights from
+    # -300 to 32767): the kernel treats each weight outside [0, 128] as the
+    # nearest bound, as the oracle does, so the result is still bit-exact.
+    kfx = gpu
+    rng = np.random.default_rng(7)
+    intr = kfx.kinect_intrinsics(160, 120, 131.25)
+    rb = kfx.round_buffer(96, 1.92, origin=(5, 17, 60))
+    ref = scenes.empty_volume(rb)
+    ref["value"] = rng.integers(-32767, 32768, rb.buffer_length).astype(np.int16)
+    ref["weight"] = rng.integers(-300, 32768, rb.buffer_length).astype(np.int16)
+    R = np.eye(3)
+    t = np.array([0.96, 0.96, -0.2])
+    depth = rng.integers(600, 2500, size=(120, 160)).astype(np.uint16)
+    dv = kfx.Buffer.from_array(ref)
+    n = orc.integrate_f32(ref, depth, intr, R.T, t, scenes.TRUNC, rb)
+    _integrate_gpu(kfx, dv, depth, intr, R.T, t, scenes.TRUNC, rb)
+    got = dv.download(kfx.TSDF_DTYPE, rb.buffer_length)
+    assert n > 100000
+    assert np.array_equal(got, ref)

@@ -169,3 +169,30 @@ def test_trilinear_interpolation_reproduces_linear_field(kfx, orc):
         assert abs(got - f(p)) < 2e-4
     # Outside [1, N-2] voxels the interpolation is undefined (NaN).
     assert np.isnan(orc.lib.orc_interpolate_trilinear_f32(orc.ptr(vol), 0.05, 0.5, 0.5, C.byref(cs), C.byref(rb)))
+
+
+def test_oracle_clamps_out_of_range_weights(orc):
+    # Stored weights above 128 count as 128 and negative ones as 0: the
+    # update from such a voxel equals the update from the clamped voxel.
+    import kfx
+    rng = np.random.default_rng(3)
+    intr = kfx.kinect_intrinsics(80, 60, 65.625)
+    rb = kfx.round_buffer(48, 0.96)
+    depth = rng.integers(500, 1500, size=(60, 80)).astype(np.uint16)
+    t = np.array([0.48, 0.48, -0.1])
+    vals = rng.integers(-32767, 32768, rb.buffer_length).astype(np.int16)
+    out, counts = [], []
+    for w in (200, 128, -5, 0):
+        vol = np.zeros(rb.buffer_length, dtype=kfx.TSDF_DTYPE)
+        vol["value"] = vals
+        vol["weight"] = w
+        counts.append(orc.integrate_f32(vol, depth, intr, np.eye(3), t, 0.03, rb))
+        out.append(vol)
+    assert counts[0] > 1000 and len(set(counts)) == 1
+    # Values agree everywhere; an updated voxel ends at weight 128 (from 200
+    # or 128) or 1 (from -5 or 0), the others keep their weight.
+    assert np.array_equal(out[0]["value"], out[1]["value"])
+    assert np.array_equal(out[2]["value"], out[3]["value"])
+    assert np.count_nonzero(out[0]["weight"] == 128) == counts[0]
+    assert np.count_nonzero(out[2]["weight"] == 1) == counts[0]
+    assert np.array_equal(out[0]["weight"] == 128, out[2]["weight"] == 1)
