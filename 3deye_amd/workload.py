@@ -8,11 +8,11 @@ radii 0.1-0.5 m) in REGION, and a seeded random walk of STEP metres per frame
 at HEIGHT metres, looking PITCH radians down.  The pipeline tracks the
 camera's target point, 1.5 m along the optical axis, and re-centres the cube
 on it at every cyclic shift (when it is more than 1.5 m from the cube
-centre).  The walk's cube starts LEAD_M behind the first target point along
-the direction the target moves over the first LEAD_FRAMES frames, so that
-the first shift falls about LEAD_FRAMES frames in (frame 13 on the benchmark
-walk, then every 70-75 frames): a short run from the start still times a
-shift with its slab extraction.
+centre).  The cube starts centred on the first target point; the first
+shift falls at frame 77 of the benchmark walk, then every 73-75 frames.
+A timed run starts after PREROLL_FRAMES untimed frames that build the map,
+so that even a short timed window (from frame 65 on) holds a shift whose
+leaving slab was observed, with its points extracted.
 """
 import os
 import sys
@@ -33,8 +33,7 @@ PITCH = 0.6                         # look-down angle (radians)
 STEP = 0.02                         # metres per frame
 SCENE_SEED = 4                      # the benchmark's scene and trajectory
 TARGET_DISTANCE = 1.5               # the cube's centre is this far along the optical axis
-LEAD_M = 1.25                       # the first cube centre lies this far behind the first target point
-LEAD_FRAMES = 12                    # ... along the target's motion over this many frames
+PREROLL_FRAMES = 60                 # untimed frames before a timed run's warmup
 
 
 def intrinsics():
@@ -60,18 +59,6 @@ def cube_origin(pose, target_distance=TARGET_DISTANCE, vol_m=VOL_M):
     return target_point(pose, target_distance) - vol_m / 2
 
 
-def walk_cube_origin(poses, vol_m=VOL_M, lead_m=LEAD_M, lead_frames=LEAD_FRAMES):
-    """Corner of the walk's first cube: its centre lies lead_m behind the
-    first target point, opposite to the target's motion over the first
-    lead_frames frames, so the first cyclic shift comes early."""
-    t0 = target_point(poses[0])
-    u = target_point(poses[min(lead_frames, len(poses) - 1)]) - t0
-    n = np.linalg.norm(u)
-    if n == 0.0:
-        return t0 - vol_m / 2
-    return t0 - lead_m * u / n - vol_m / 2
-
-
 def pipeline_config(**kw):
     return kfx.default_config(intr=intrinsics(), voxels=kfx.Point3i(N_VOX, N_VOX, N_VOX),
                               volume_size=kfx.Point3f(VOL_M, VOL_M, VOL_M), **kw)
@@ -87,6 +74,6 @@ def sequence(seed, n_frames, threads=None, want_clean=False):
         frames = [kfx.render(sc, poses[i], intr, seed, i, noise=True, want_clean=True, **kw) for i in range(n_frames)]
         depths = np.stack([f[0] for f in frames])
         clean = np.stack([f[1] for f in frames])
-        return intr, sc, poses, walk_cube_origin(poses), depths, clean
+        return intr, sc, poses, cube_origin(poses[0]), depths, clean
     depths = np.stack([kfx.render(sc, poses[i], intr, seed, i, noise=True, **kw) for i in range(n_frames)])
-    return intr, sc, poses, walk_cube_origin(poses), depths
+    return intr, sc, poses, cube_origin(poses[0]), depths
