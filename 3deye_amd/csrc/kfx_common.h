@@ -70,6 +70,10 @@ int depth_consistency(const Workspace& ws, const kfx_xyz* vmap_model, const floa
                       const kfx_intrinsics* intr, float max_diff, double* dev_stats, double fallback_m,
                       int32_t* dev_flag, int32_t* host_flag);
 
+// Resolves, for the current device, the device address of the fixed scratch
+// array of kfx_integrate_tsdf_noalloc (kfx_volume.hip); called by kfx_init.
+int noalloc_setup(int device);
+
 // Library stream (created by kfx_init, default priority) on which every
 // entry point enqueues its work.
 hipStream_t stream();

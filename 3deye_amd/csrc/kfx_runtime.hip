@@ -132,7 +132,7 @@ int kfx_init(int device) {
     }
     KFX_CHECK_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_device = device;
-    return KFX_OK;
+    return noalloc_setup(device);
 }
 
 int kfx_check_reciprocal(int exp_lo, int exp_hi, uint64_t* mismatches) {

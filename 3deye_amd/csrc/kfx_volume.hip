@@ -1002,6 +1002,27 @@ int integrate_colors_posed_ws(const Workspace& ws, kfx_color* color_volume, cons
     return KFX_OK;
 }
 
+// Depth-bound pyramid of kfx_integrate_tsdf_noalloc: a fixed array in device
+// memory, one copy per device that loads the library, held until the
+// process ends.  Host code never passes the array's name to a kernel: it is
+// a host-side stand-in there.  noalloc_setup resolves each device's address
+// once (kfx_init, with the device current) and the entry point uses that.
+constexpr size_t kNoAllocBoundFloats = size_t(1) << 20;  // 4 MiB: levels 1..5 of images up to 2048 x 1536
+static __device__ __align__(256) float scratch[kNoAllocBoundFloats];
+constexpr int kNoAllocMaxDevices = 64;
+static float* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+int noalloc_setup(int device) {
+    KFX_REQUIRE(device >= 0 && device < kNoAllocMaxDevices, "kfx_init: device %d beyond the %d the library supports",
+                device, kNoAllocMaxDevices);
+    if (g_scratch_ptr[device] != nullptr) return KFX_OK;
+    float* ptr = nullptr;
+    KFX_CHECK_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    KFX_REQUIRE(ptr != nullptr, "kfx_init: no device address for the integration scratch array");
+    g_scratch_ptr[device] = ptr;
+    return KFX_OK;
+}
+
 }  // namespace kfx
 
 extern "C" {
@@ -1028,6 +1049,27 @@ int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsd
     KFX_TRY(library_ws_for(intr->cols, intr->rows, false, &ws));
     return integrate_tsdf_posed_ws(*ws, depth, depth_scaled, volume, intr, dev_pose, dev_prev_pose_inv,
                                    motion_threshold, truncation_distance, cell_size, rb, dev_skip);
+}
+
+int kfx_integrate_tsdf_noalloc(const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
+                               const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                               const kfx_point_3f* translation, float truncation_distance,
+                               const kfx_point_3f* cell_size, const kfx_round_buffer* rb) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(intr != nullptr, "kfx_integrate_tsdf_noalloc: null pointer");
+    KFX_REQUIRE(intr->cols > 0 && intr->rows > 0, "kfx_integrate_tsdf_noalloc: empty image");
+    const size_t need = bound_floats(intr->cols, intr->rows);
+    KFX_REQUIRE(need <= kNoAllocBoundFloats,
+                "kfx_integrate_tsdf_noalloc: a %dx%d image needs %zu floats of scratch, more than the %zu reserved",
+                intr->cols, intr->rows, need, kNoAllocBoundFloats);
+    const int dev = current_device();
+    KFX_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices && g_scratch_ptr[dev] != nullptr,
+                "kfx_integrate_tsdf_noalloc: scratch not set up for device %d", dev);
+    Workspace ws;
+    ws.bound = g_scratch_ptr[dev];
+    ws.bound_cap = kNoAllocBoundFloats;
+    return integrate_tsdf_ws(ws, false, depth, depth_scaled, volume, intr, rotation_inverse, translation,
+                             truncation_distance, cell_size, rb);
 }
 
 int kfx_integrate_count_updates(const uint16_t* depth, float* depth_scaled, const kfx_intrinsics* intr,

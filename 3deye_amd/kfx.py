@@ -222,6 +222,8 @@ _PROTOS = {
     "kfx_init_tsdf": (_I, [_P, C.POINTER(RoundBuffer)]),
     "kfx_integrate_tsdf": (_I, [_P, _P, _P, C.POINTER(Intrinsics), C.POINTER(Matrix3f), C.POINTER(Point3f),
                                 C.c_float, C.POINTER(Point3f), C.POINTER(RoundBuffer)]),
+    "kfx_integrate_tsdf_noalloc": (_I, [_P, _P, _P, C.POINTER(Intrinsics), C.POINTER(Matrix3f), C.POINTER(Point3f),
+                                        C.c_float, C.POINTER(Point3f), C.POINTER(RoundBuffer)]),
     "kfx_raycast": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), C.POINTER(Point3f), C.POINTER(Matrix3f), C.c_float,
                          C.POINTER(Intrinsics), C.POINTER(Point3f)]),
     "kfx_integrate_tsdf_posed": (_I, [_P, _P, _P, C.POINTER(Intrinsics), _P, _P, C.c_float, C.c_float,

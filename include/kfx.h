@@ -115,6 +115,20 @@ int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* vol
                        const kfx_point_3f* translation, float truncation_distance,
                        const kfx_point_3f* cell_size, const kfx_round_buffer* rb);
 
+/* kfx_integrate_tsdf that allocates nothing: no reference counterpart; for
+ * callers that cannot call hipMalloc (code under graph capture, tight
+ * serving loops, harnesses that forbid allocation).  Same arguments and the
+ * same result as kfx_integrate_tsdf.  Its depth-bound pyramid (the only
+ * device memory the integration uses beyond its arguments) lives in a fixed
+ * 4 MiB array per device, set up by kfx_init: images up to 2048 x 1536
+ * (levels 1..5 of the pyramid within 2^20 floats); larger images fail with
+ * KFX_ERR_INVALID_ARG.  Every call on a device shares that array, so calls
+ * that overlap on one device (from several threads) race; each call writes
+ * every texel it reads, so nothing carries over between calls. */
+int kfx_integrate_tsdf_noalloc(const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
+                               const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
+                               const kfx_point_3f* translation, float truncation_distance,
+                               const kfx_point_3f* cell_size, const kfx_round_buffer* rb);
 
 /* Number of voxels the update rule of kfx_integrate_tsdf changes for this
  * depth frame and pose (in-image projection, valid depth, sdf >= -trunc).
