@@ -30,6 +30,9 @@ struct Workspace {
     size_t cbound_cap = 0;
     double* cons = nullptr;    // depth-consistency gate: 4 doubles per 256-pixel workgroup
     size_t cons_cap = 0;       // workgroups
+    void* terms = nullptr;     // one frame's pose terms and skip flag (integration with a device pose)
+    uint8_t* flags = nullptr;  // one flag per integration workgroup (frustum test)
+    size_t flags_cap = 0;
 };
 // Sizes each buffer needs for images of cols x rows.
 size_t bound_floats(int cols, int rows);
@@ -40,10 +43,14 @@ size_t cons_blocks(int cols, int rows);
 // replaced here are freed (owned workspaces are never captured mid-growth).
 int workspace_reserve(Workspace& ws, int cols, int rows, bool colors);
 void workspace_free(Workspace& ws);
+// Integration workgroups of a volume (the flags the workspace must hold).
+size_t integrate_blocks(const kfx_point_3i& voxels);
+// Allocates the frame terms and the workgroup flags of ws for a volume.
+int workspace_reserve_volume(Workspace& ws, const kfx_point_3i& voxels);
 // Grows the library workspace to cols x rows; outgrown buffers are kept
 // (not freed) until the process ends.  Returns an error inside a stream
 // capture when it would have to allocate.
-int library_workspace_reserve(int cols, int rows, bool colors, bool gate);
+int library_workspace_reserve(int cols, int rows, bool colors, bool gate, const kfx_point_3i* voxels = nullptr);
 Workspace& library_workspace();
 
 // The integrations and the fallback gate over an explicit workspace (the
@@ -173,6 +180,23 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return r;
 }
 
+// sqrt_rn for arguments known to be non-negative and finite, without its
+// range branch: x is first clamped into [2^-100, 2^100).  The clamp changes
+// the root only for x < 2^-100 (a root below 2^-50) and x >= 2^100; the
+// integration subtracts the root from a scaled depth of at least 1 mm, where
+// a root below 2^-50 is below half an ulp either way, and its squared
+// distances stay far below 2^100.
+__device__ __forceinline__ float sqrt_rn_in_range(float x) {
+    const float xc = __builtin_amdgcn_fmed3f(x, 0x1p-100f, 0x1.fffffep99f);
+    const float s = __builtin_amdgcn_sqrtf(xc);
+    const float sdn = __uint_as_float(__float_as_uint(s) - 1u), sup = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rdn = fmaf(-sdn, s, xc), rup = fmaf(-sup, s, xc);
+    float r = s;
+    if (rdn <= 0.0f) r = sdn;
+    if (rup > 0.0f) r = sup;
+    return r;
+}
+
 // Correctly rounded 1/x for x in [2^-126, 2^126): v_rcp_f32 (not correctly
 // rounded by itself) and one Newton step with FMAs.  Equal to the IEEE
 // quotient for every float of that range on MI355X, checked exhaustively on
@@ -220,6 +244,14 @@ __device__ __forceinline__ int16_t tsdf_encode(float tsdf) {
     int fixedp = (int)s;  // truncation toward zero
     fixedp = max(-KFX_MAX_DEPTH_VALUE, min(KFX_MAX_DEPTH_VALUE, fixedp));
     return (int16_t)fixedp;
+}
+
+// tsdf_encode for |tsdf| < 2^16, the range of every running average (a
+// weighted mean of values within [-1, 1] and one sdf / trunc >= -1 - 2^-23):
+// there the product cannot reach the saturation that tsdf_encode guards.
+__device__ __forceinline__ int16_t tsdf_encode_unit(float tsdf) {
+    const int fixedp = (int)(tsdf * (float)KFX_MAX_DEPTH_VALUE);  // truncation toward zero
+    return (int16_t)max(-KFX_MAX_DEPTH_VALUE, min(KFX_MAX_DEPTH_VALUE, fixedp));
 }
 
 __host__ __device__ __forceinline__ unsigned divup(unsigned a, unsigned b) { return (a + b - 1) / b; }

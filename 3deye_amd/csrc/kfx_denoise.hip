@@ -1067,8 +1067,12 @@ int depth_consistency(const Workspace& ws, const kfx_xyz* vmap_model, const floa
     KFX_REQUIRE(ws.cons != nullptr && (size_t)nblk <= ws.cons_cap,
                 "kfx_depth_consistency: workspace holds %zu workgroup partials, %dx%d frames need %d", ws.cons_cap,
                 intr->cols, intr->rows, nblk);
-    depth_consistency_kernel<<<nblk, 256, 0, stream()>>>(vmap_model, dev_pose, depth, *intr, max_diff, ws.cons);
-    KFX_CHECK_LAUNCH();
+    {
+        ProfScope prof("consistency");
+        depth_consistency_kernel<<<nblk, 256, 0, stream()>>>(vmap_model, dev_pose, depth, *intr, max_diff, ws.cons);
+        KFX_CHECK_LAUNCH();
+    }
+    ProfScope prof("consistency_reduce");
     consistency_reduce_kernel<<<1, 256, 0, stream()>>>(ws.cons, nblk, dev_stats, fallback_m, dev_flag, host_flag);
     KFX_CHECK_LAUNCH();
     return KFX_OK;

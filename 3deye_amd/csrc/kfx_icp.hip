@@ -22,6 +22,8 @@
 
 #include <type_traits>
 
+#include <stdio.h>
+
 #include "kfx_common.h"
 
 namespace kfx {
@@ -722,11 +724,18 @@ template <int M>
 static void icp_iterate_launch(kfx_icp* h, const kfx_xyz* vc, const kfx_xyz* nc, const kfx_xyz* vp, const kfx_xyz* np,
                                const IcpFrame& f, int nblk, bool two, int tag, double conv_rot, double conv_trans,
                                bool publish) {
-    if (two)
-        icp_sums_kernel<2, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
-    else
-        icp_sums_kernel<1, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
+    {
+        // Named by image width, so a profile separates the pyramid levels.
+        char name[32];
+        snprintf(name, sizeof(name), "icp_sums@%d", f.cols);
+        ProfScope prof(name);
+        if (two)
+            icp_sums_kernel<2, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
+        else
+            icp_sums_kernel<1, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
+    }
     IcpState* mirror = publish ? h->host_state : nullptr;
+    ProfScope prof("icp_solve");
     icp_solve_kernel<M><<<1, kIcpThreads, 0, stream()>>>(h->state, h->partials, nblk, tag, conv_rot,
                                                                 conv_trans, mirror);
 }
@@ -853,7 +862,6 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
     const int nblk = (int)divup(npix, (unsigned)(kIcpThreads * (two ? 2 : 1)));
     IcpFrame f = make_frame(rotation_previous_inverse, translation_previous, intr, distance_threshold,
                             angle_threshold);
-    ProfScope prof("icp_fused");
     auto launch = [&](auto mode) {
         icp_iterate_launch<decltype(mode)::value>(h, vmap_current, nmap_current, vmap_previous, nmap_previous, f,
                                                   nblk, two, level_tag, converge_rotation, converge_translation,
@@ -880,6 +888,7 @@ int kfx_icp_set_sum_mode(kfx_icp* h, int mode) {
 int kfx_icp_begin_frame(kfx_icp* h) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(h != nullptr, "kfx_icp_begin_frame: null handle");
+    ProfScope prof("icp_begin_frame");
     icp_begin_frame_kernel<<<1, 64, 0, stream()>>>(h->state);
     KFX_CHECK_LAUNCH();
     return KFX_OK;

@@ -530,6 +530,7 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     if (!rc) rc = kfx_icp_set_sum_mode(p->icp, p->cfg.icp_sum_mode);
     if (!rc) rc = kfx_slicer_create(&rb, &p->slicer);
     if (!rc) rc = workspace_reserve(p->ws, p->cfg.intr.cols, p->cfg.intr.rows, p->cvol != nullptr);
+    if (!rc) rc = workspace_reserve_volume(p->ws, rb.voxels_size);
     if (!rc) rc = kfx_init_tsdf(p->vol, &rb);
     if (!rc && p->cvol) rc = kfx_init_colors(p->cvol, &rb);
     if (!rc) rc = kfx_synchronize();

@@ -30,6 +30,29 @@ constexpr int kIntegrateBatch = 8;
 constexpr int kBoundLevels = 5;
 constexpr int kBoundTile = 1 << kBoundLevels;
 
+// The sweep's workgroups: (bx, by, bz) covers the columns x in [64 bx,
+// 64 bx + 63], y in [4 by, 4 by + 3] over the z run bz.  The hardware deals
+// consecutive workgroups round-robin to the XCDs, so the launch grid takes
+// the y blocks fastest: each XCD then gets every eighth row band of every
+// region of the volume.  With x fastest, XCD k would get the x slabs
+// 64 (8 j + k) .. 64 (8 j + k) + 63 -- for a camera looking along x, a few
+// XCDs would hold the whole frustum.
+__host__ __device__ inline dim3 sweep_grid(int nx, int ny, int nz, int zrun) {
+    return dim3(divup((unsigned)ny, kIntegrateBlockY), divup((unsigned)nx, kIntegrateBlockX), divup((unsigned)nz, (unsigned)zrun));
+}
+struct SweepBlock {
+    int bx, by, bz;
+    uint32_t linear;  // bx + nbx (by + nby bz): the index of the pre-pass's workgroup flags
+};
+__device__ __forceinline__ SweepBlock sweep_block() {
+    SweepBlock b;
+    b.by = blockIdx.x;
+    b.bx = blockIdx.y;
+    b.bz = blockIdx.z;
+    b.linear = (uint32_t)b.bx + gridDim.y * ((uint32_t)b.by + gridDim.x * (uint32_t)b.bz);
+    return b;
+}
+
 struct IntegrateParams {
     float r[9];          // rotation_inverse, row-major
     float tx, ty, tz;    // camera position in volume coordinates
@@ -196,13 +219,57 @@ __device__ __forceinline__ float scaled_depth(const uint16_t* __restrict__ depth
     return (float)depth[v * intr.cols + u] * lambda / 1000.0f;
 }
 
-// Scaled depth and depth-bound pyramid in one launch: one 32 x 32 pixel tile
-// per workgroup writes its pixels' scaled depth (every pixel belongs to one
-// tile) and builds levels 1..5 by repeated 2 x 2 maxima in LDS.  Pixels
-// outside the image count as 0 (no depth).
+// Frustum test of the integration workgroup (bx, by, bz) -- voxel columns x
+// in [64 bx, 64 bx + 63], y in [4 by, 4 by + 3], z in one run of p.zrun:
+// 0 when for some plane every voxel centre of the box lies outside it (the
+// largest value over the box's corners, taken per axis, stays below the
+// plane), 2 when every centre lies inside every plane (the smallest value
+// reaches it), 1 otherwise.  Both decisions carry a tolerance far above the
+// rounding of these sums, so 0 is conservative (a voxel the update rule
+// accepts projects into the image, so it lies inside every plane, which
+// carry a margin of kClipMarginPx pixels) and 2 means that the per-column
+// clip would keep the whole run.
+__device__ __forceinline__ int block_frustum_class(const IntegrateParams& p, const PoseTerms& q, int bx, int by,
+                                                   int bz) {
+    const int x0 = bx * kIntegrateBlockX, x1 = min(x0 + kIntegrateBlockX, p.nx) - 1;
+    const int y0 = by * kIntegrateBlockY, y1 = min(y0 + kIntegrateBlockY, p.ny) - 1;
+    const int z0 = bz * p.zrun, z1 = min(z0 + p.zrun, p.nz) - 1;
+    const float gx0 = ((float)x0 + 0.5f) * p.csx - q.tx, gx1 = ((float)x1 + 0.5f) * p.csx - q.tx;
+    const float gy0 = ((float)y0 + 0.5f) * p.csy - q.ty, gy1 = ((float)y1 + 0.5f) * p.csy - q.ty;
+    const float gz0 = ((float)z0 + 0.5f) * p.csz - q.tz, gz1 = ((float)z1 + 0.5f) * p.csz - q.tz;
+    const float ex = fmaxf(fabsf(gx0), fabsf(gx1)), ey = fmaxf(fabsf(gy0), fabsf(gy1)), ez = fmaxf(fabsf(gz0), fabsf(gz1));
+    bool all_inside = true;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const float n0 = q.plane[k][0], n1 = q.plane[k][1], n2 = q.plane[k][2];
+        const float hi = fmaxf(n0 * gx0, n0 * gx1) + fmaxf(n1 * gy0, n1 * gy1) + fmaxf(n2 * gz0, n2 * gz1);
+        const float lo = fminf(n0 * gx0, n0 * gx1) + fminf(n1 * gy0, n1 * gy1) + fminf(n2 * gz0, n2 * gz1);
+        const float tol = 1e-4f * (fabsf(n0) * ex + fabsf(n1) * ey + fabsf(n2) * ez) +
+                          1e-3f * (fabsf(n0) + fabsf(n1) + fabsf(n2));
+        if (hi < -tol) return 0;
+        all_inside = all_inside && lo > tol;
+    }
+    return all_inside ? 2 : 1;
+}
+
+// Per-frame pre-pass of the integration, one launch:
+//  - scaled depth and the depth-bound pyramid: one 32 x 32 pixel tile per
+//    workgroup writes its pixels' scaled depth (every pixel belongs to one
+//    tile) and builds levels 1..5 by repeated 2 x 2 maxima in LDS; pixels
+//    outside the image count as 0 (no depth);
+//  - with a device pose, the pose terms of the frame and its skip decision
+//    (the gate's flag, or too little motion), written once by thread 0;
+//  - one flag per integration workgroup: 0 when the frame is skipped or the
+//    workgroup's box lies outside the frustum, so those workgroups exit at
+//    once; 2 when it lies inside, so its columns need no clip; 1 otherwise
+//    (threads take the workgroups in turn over the whole grid).
+template <bool kDevicePose>
 __global__ void __launch_bounds__(kBoundTile * kBoundTile)
-depth_bound_kernel(const uint16_t* __restrict__ depth, float* __restrict__ scaled, kfx_intrinsics intr,
-                   float* __restrict__ bound, IntegrateParams p) {
+integrate_prepass_kernel(const uint16_t* __restrict__ depth, float* __restrict__ scaled, kfx_intrinsics intr,
+                         float* __restrict__ bound, IntegrateParams p, const float* __restrict__ dpose,
+                         const float* __restrict__ dprev, float motion_thr, const int32_t* __restrict__ dskip,
+                         PoseTerms* __restrict__ terms, int32_t* __restrict__ frame_skip,
+                         uint8_t* __restrict__ flags, int nbx, int nby, int nbz) {
     __shared__ float t[kBoundTile * kBoundTile];
     const int cols = intr.cols, rows = intr.rows;
     const int tx = threadIdx.x % kBoundTile, ty = threadIdx.x / kBoundTile;
@@ -212,24 +279,46 @@ depth_bound_kernel(const uint16_t* __restrict__ depth, float* __restrict__ scale
         d = scaled_depth(depth, intr, u, v);
         scaled[v * cols + u] = d;
     }
-    t[threadIdx.x] = d;
-    __syncthreads();
-    for (int l = 1; l <= kBoundLevels; ++l) {
-        const int n = kBoundTile >> l;  // texels per tile side at level l
-        float m = 0.0f;
-        if (tx < n && ty < n) {
-            const int s = kBoundTile >> (l - 1);  // side of the previous level's tile in t[]
-            m = fmaxf(fmaxf(t[(2 * ty) * s + 2 * tx], t[(2 * ty) * s + 2 * tx + 1]),
-                      fmaxf(t[(2 * ty + 1) * s + 2 * tx], t[(2 * ty + 1) * s + 2 * tx + 1]));
-        }
+    if (bound != nullptr) {
+        t[threadIdx.x] = d;
         __syncthreads();
-        if (tx < n && ty < n) {
-            t[ty * n + tx] = m;
-            const int lw = (cols + (1 << l) - 1) >> l, lh = (rows + (1 << l) - 1) >> l;
-            const int a = blockIdx.x * n + tx, b = blockIdx.y * n + ty;
-            if (a < lw && b < lh) bound[p.bound_off[l] + b * lw + a] = m;
+        for (int l = 1; l <= kBoundLevels; ++l) {
+            const int n = kBoundTile >> l;  // texels per tile side at level l
+            float m = 0.0f;
+            if (tx < n && ty < n) {
+                const int sd = kBoundTile >> (l - 1);  // side of the previous level's tile in t[]
+                m = fmaxf(fmaxf(t[(2 * ty) * sd + 2 * tx], t[(2 * ty) * sd + 2 * tx + 1]),
+                          fmaxf(t[(2 * ty + 1) * sd + 2 * tx], t[(2 * ty + 1) * sd + 2 * tx + 1]));
+            }
+            __syncthreads();
+            if (tx < n && ty < n) {
+                t[ty * n + tx] = m;
+                const int lw = (cols + (1 << l) - 1) >> l, lh = (rows + (1 << l) - 1) >> l;
+                const int a = blockIdx.x * n + tx, b = blockIdx.y * n + ty;
+                if (a < lw && b < lh) bound[p.bound_off[l] + b * lw + a] = m;
+            }
+            __syncthreads();
         }
-        __syncthreads();
+    }
+    if (flags == nullptr) return;
+    const int nthreads = gridDim.x * gridDim.y * blockDim.x;
+    const int gid = (blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+    const int nblk = nbx * nby * nbz;
+    if (gid >= nblk) return;
+    PoseTerms q;
+    bool skip = false;
+    if (kDevicePose) {
+        skip = (dskip != nullptr && *dskip != 0) || !load_device_pose(p, q, dpose, dprev, motion_thr);
+        if (gid == 0) {
+            if (!skip) *terms = q;
+            *frame_skip = skip ? 1 : 0;
+        }
+    } else {
+        q = host_terms(p);
+    }
+    for (int g = gid; g < nblk; g += nthreads) {
+        const int bx = g % nbx, by = (g / nbx) % nby, bz = g / (nbx * nby);
+        flags[g] = skip ? 0 : (uint8_t)block_frustum_class(p, q, bx, by, bz);
     }
 }
 
@@ -282,95 +371,273 @@ __device__ __forceinline__ int level_offset(const IntegrateParams& p, int l) {
     return off;
 }
 
-// True when no voxel of the column segment z in [za, zb] can be updated:
-// every voxel centre is behind the camera, or the segment projects (with a
-// one-pixel margin) only onto pixels without depth, or it lies farther from
-// the camera than the largest depth under its footprint plus the truncation
-// distance (the rule needs dist <= depth + trunc).  Conservative: false
-// whenever the test cannot decide.
-__device__ __forceinline__ bool segment_culled(const IntegrateParams& p, const PoseTerms& q,
-                                               const float* __restrict__ scaled,
-                                               float gx, float gy, float cxy0, float cxy1, float cxy2, int za,
-                                               int zb) {
-    Footprint f;
-    const int k = segment_footprint(p, q, cxy0, cxy1, cxy2, za, zb, f);
-    if (k != kSegmentFootprint) return k == kSegmentOutside;
-    float maxd;
-    if (f.l == 0) {
-        maxd = scaled[f.v0 * p.cols + f.u0];
-    } else {
-        const int lw = (p.cols + (1 << f.l) - 1) >> f.l;
-        const float* b = p.bound + level_offset(p, f.l);
-        const int a0 = f.u0 >> f.l, a1 = f.u1 >> f.l, b0 = f.v0 >> f.l, b1 = f.v1 >> f.l;
-        maxd = fmaxf(fmaxf(b[b0 * lw + a0], b[b0 * lw + a1]), fmaxf(b[b1 * lw + a0], b[b1 * lw + a1]));
-    }
-    if (maxd == 0.0f) return true;
-    // Nearest point of the segment to the camera, bounded below.
-    const float dz = (f.gza <= 0.0f && f.gzb >= 0.0f) ? 0.0f : fminf(fabsf(f.gza), fabsf(f.gzb));
-    const float dmin = sqrt_rn(gx * gx + gy * gy + dz * dz);
-    return dmin > (maxd + p.trunc) * 1.0001f + 1e-4f;
+// Per z of a workgroup's run: the z terms of the camera coordinates,
+// r[2] gz, r[5] gz, r[8] gz, and gz^2, so that a voxel's camera coordinates
+// (r0 gx + r1 gy) + r2 gz and |g|^2 = gz^2 + (gx^2 + gy^2) are one add each
+// with the per-voxel evaluation's rounding; rows past the run's end keep the
+// batch loop's overhang in bounds.
+constexpr int kZTerms = kIntegrateZRunLarge + kIntegrateBatch;
+
+// Largest scaled depth under the pixel footprint of the lane's column
+// segment from camera point a to camera point b (its end voxels' centres):
+// the box spanned by their projections widened by one pixel, looked up as
+// 2 x 2 texels of the depth-bound pyramid level whose texels cover it.
+// Returns 0 when no voxel centre of the segment can project into the image
+// or no pixel under it has depth, a negative value when the test cannot
+// decide (a centre near the camera plane, a footprint wider than the
+// coarsest level).
+__device__ __forceinline__ float segment_max_depth(const IntegrateParams& p, const float* __restrict__ scaled,
+                                                   float ax, float ay, float az, float bx, float by, float bz) {
+    if (az <= 0.0f && bz <= 0.0f) return 0.0f;
+    if (!(az > 1e-4f && bz > 1e-4f)) return -1.0f;
+    const float ia = __builtin_amdgcn_rcpf(az), ib = __builtin_amdgcn_rcpf(bz);
+    const float ua = ax * p.fx * ia + p.cx, ub = bx * p.fx * ib + p.cx;
+    const float va = ay * p.fy * ia + p.cy, vb = by * p.fy * ib + p.cy;
+    const float fu0 = floorf(fminf(ua, ub)) - 1.0f, fu1 = ceilf(fmaxf(ua, ub)) + 1.0f;
+    const float fv0 = floorf(fminf(va, vb)) - 1.0f, fv1 = ceilf(fmaxf(va, vb)) + 1.0f;
+    if (fu1 < 0.0f || fv1 < 0.0f || fu0 > (float)(p.cols - 1) || fv0 > (float)(p.rows - 1)) return 0.0f;
+    const int u0 = (int)fmaxf(fu0, 0.0f), u1 = (int)fminf(fu1, (float)(p.cols - 1));
+    const int v0 = (int)fmaxf(fv0, 0.0f), v1 = (int)fminf(fv1, (float)(p.rows - 1));
+    const int span = max(u1 - u0, v1 - v0) + 1;
+    const int l = span <= 1 ? 0 : 32 - __clz(span - 1);  // 2^l >= span: at most 2 x 2 texels
+    if (l > kBoundLevels) return -1.0f;
+    if (l == 0) return scaled[v0 * p.cols + u0];
+    const int lw = (p.cols + (1 << l) - 1) >> l;
+    const float* b = p.bound + level_offset(p, l);
+    const int a0 = u0 >> l, a1 = u1 >> l, b0 = v0 >> l, b1 = v1 >> l;
+    return fmaxf(fmaxf(b[b0 * lw + a0], b[b0 * lw + a1]), fmaxf(b[b1 * lw + a0], b[b1 * lw + a1]));
 }
 
+// Distance bound of the update rule (dist <= depth + trunc), with a 1e-4
+// relative margin that dwarfs the rounding of the squares compared with it.
+__device__ __forceinline__ float reach(const IntegrateParams& p, float maxd) {
+    return (maxd + p.trunc) * 1.0001f + 1e-4f;
+}
+
+// Buffer resource over one slice of the volume (bytes [0, slice_bytes) from
+// base): the voxel loads and stores of a wave address one slice at a time
+// with a 32-bit offset inside it (raw buffer, no swizzle).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slice_rsrc(const char* base, uint32_t slice_bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), (short)0, (int)slice_bytes, 0x00020000);
+}
+
+// (w, w + 1, RN(1 / (w + 1)), min(w + 1, KFX_MAX_TSDF_WEIGHT) << 16) for the
+// running average from a voxel of weight w: the divisor's correctly rounded
+// reciprocal comes from v_rcp_f32 and one Newton step, exact for every
+// divisor up to 129.
+struct WeightTerms {
+    float fw, fb, y;
+    uint32_t w_new_hi;  // min(w + 1, KFX_MAX_TSDF_WEIGHT) << 16, the stored word's weight half
+};
+
+// True when no voxel of the workgroup's box can be updated because every
+// voxel centre lies farther from the camera than the largest depth under the
+// box's footprint plus the truncation: the footprint is the pixel box of
+// the projected corners (all in front of the camera), widened by one pixel,
+// looked up as level-5 texels of the depth-bound pyramid (one per lane, at
+// most 64), and the nearest point of the box comes from clamping the camera
+// position per axis.  The inputs are uniform across the workgroup, so its
+// waves agree.  Conservative: false whenever the test cannot decide.
+__device__ __forceinline__ bool block_beyond_surface(const IntegrateParams& p, const PoseTerms& q, int bx, int by,
+                                                     int bz) {
+    if (p.bound == nullptr) return false;
+    const int x0 = bx * kIntegrateBlockX, x1 = min(x0 + kIntegrateBlockX, p.nx) - 1;
+    const int y0 = by * kIntegrateBlockY, y1 = min(y0 + kIntegrateBlockY, p.ny) - 1;
+    const int z0 = bz * p.zrun, z1 = min(z0 + p.zrun, p.nz) - 1;
+    const float g[2][3] = {{((float)x0 + 0.5f) * p.csx - q.tx, ((float)y0 + 0.5f) * p.csy - q.ty,
+                            ((float)z0 + 0.5f) * p.csz - q.tz},
+                           {((float)x1 + 0.5f) * p.csx - q.tx, ((float)y1 + 0.5f) * p.csy - q.ty,
+                            ((float)z1 + 0.5f) * p.csz - q.tz}};
+    float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const float gx = g[c & 1][0], gy = g[(c >> 1) & 1][1], gz = g[c >> 2][2];
+        const float cx = q.r[0] * gx + q.r[1] * gy + q.r[2] * gz;
+        const float cy = q.r[3] * gx + q.r[4] * gy + q.r[5] * gz;
+        const float cz = q.r[6] * gx + q.r[7] * gy + q.r[8] * gz;
+        if (!(cz > 1e-3f)) return false;  // near or behind the camera plane: undecided
+        const float iz = __builtin_amdgcn_rcpf(cz);
+        const float u = cx * p.fx * iz + p.cx, v = cy * p.fy * iz + p.cy;
+        umin = fminf(umin, u); umax = fmaxf(umax, u);
+        vmin = fminf(vmin, v); vmax = fmaxf(vmax, v);
+    }
+    // The projection of the box lies in the hull of its corners' projections.
+    const float fu0 = floorf(umin) - 1.0f, fu1 = ceilf(umax) + 1.0f;
+    const float fv0 = floorf(vmin) - 1.0f, fv1 = ceilf(vmax) + 1.0f;
+    if (fu1 < 0.0f || fv1 < 0.0f || fu0 > (float)(p.cols - 1) || fv0 > (float)(p.rows - 1)) return true;
+    const int u0 = (int)fmaxf(fu0, 0.0f) >> kBoundLevels, u1 = (int)fminf(fu1, (float)(p.cols - 1)) >> kBoundLevels;
+    const int v0 = (int)fmaxf(fv0, 0.0f) >> kBoundLevels, v1 = (int)fminf(fv1, (float)(p.rows - 1)) >> kBoundLevels;
+    const int tw = u1 - u0 + 1, th = v1 - v0 + 1;
+    if (tw * th > 64) return false;
+    const int lane = threadIdx.x & 63;
+    const int lw = (p.cols + kBoundTile - 1) >> kBoundLevels;
+    float m = 0.0f;
+    if (lane < tw * th) m = p.bound[p.bound_off[kBoundLevels] + (v0 + lane / tw) * lw + (u0 + lane % tw)];
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if (m == 0.0f) return true;  // no depth under the footprint
+    float d2 = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float d = fmaxf(0.0f, fmaxf(g[0][a], -g[1][a]));
+        d2 += d * d;
+    }
+    const float r = reach(p, m);
+    return d2 > r * r;
+}
+
+// The TSDF sweep.  A workgroup is 64 x 4 columns over a run of p.zrun
+// slices; a wave holds 64 neighbouring x columns, so each slice it visits
+// is one 256-byte segment of the volume.
+//
+// Per lane: the column's run is clipped to the frustum (workgroups the
+// pre-pass found inside the frustum keep the whole run), then bounded by
+// distance: no voxel farther from the camera than the largest depth under
+// the run's footprint plus the truncation can be updated, and |g| is convex
+// along the column, so the remaining voxels form one z interval.  The wave
+// walks the union of its lanes' ranges in batches of kIntegrateBatch
+// slices.  Per batch: each lane bounds its part of the batch against the
+// pyramid in the same way and the wave skips the batch when every lane is
+// culled; otherwise every voxel of the batch is projected (the exact rule's
+// arithmetic: correctly rounded 1/z, rint) and its scaled depth gathered,
+// while the lanes that are not culled load their voxels of the batch; then
+// the rule is applied.
+//
+// The distance |g| = sqrt(gz^2 + gx^2 + gy^2) is needed only near the
+// surface: when |g|^2 < f |f| with f = 0.999999 dp - 1.01 trunc, then
+// sdf = dp - |g| exceeds 1.01 trunc - 2e-7 dp (the rounding of the square,
+// the root and the difference are each below 1.2e-7 of dp), so
+// min(1, sdf / trunc) = 1 without computing it; otherwise the correctly
+// rounded root decides, as in the oracle.
+//
+// The running average divides by w + 1 with its correctly rounded
+// reciprocal from an LDS table and Markstein's correction: the IEEE
+// quotient.
 template <bool kDevicePose>
 __global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
 integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume, const IntegrateParams p,
-                 const float* __restrict__ dpose, const float* __restrict__ dprev, float motion_thr,
-                 const int32_t* __restrict__ dskip) {
-    if (kDevicePose && dskip != nullptr && *dskip != 0) return;  // uniform across the grid
-    PoseTerms q;
-    if (kDevicePose) {
-        if (!load_device_pose(p, q, dpose, dprev, motion_thr)) return;
-    } else {
-        q = host_terms(p);
+                 const PoseTerms* __restrict__ dterms, const int32_t* __restrict__ frame_skip,
+                 const uint8_t* __restrict__ flags) {
+    __shared__ float4 zt[kZTerms];
+    __shared__ float gzt[kZTerms];
+    __shared__ WeightTerms qt[KFX_MAX_TSDF_WEIGHT + 1];
+    const SweepBlock blk = sweep_block();
+    int cls = 1;
+    if (flags != nullptr) {
+        cls = flags[blk.linear];
+        if (cls == 0) return;  // uniform: the frame is skipped, or the workgroup lies outside the frustum
     }
-    const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
-    const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
-    const bool inside = x < p.nx && y < p.ny;
-    const int z_begin = blockIdx.z * p.zrun;
+    if (kDevicePose && *frame_skip != 0) return;
+    PoseTerms q;
+    if (kDevicePose) q = *dterms;
+    else q = host_terms(p);
+    if (block_beyond_surface(p, q, blk.bx, blk.by, blk.bz)) return;  // uniform across the workgroup
+    const int tid = threadIdx.y * kIntegrateBlockX + threadIdx.x;
+    const int z_begin = blk.bz * p.zrun;
     const int z_end = min(z_begin + p.zrun, p.nz);
+    if (tid < kZTerms) {
+        const float gz = ((float)(z_begin + tid) + 0.5f) * p.csz - q.tz;
+        zt[tid] = make_float4(q.r[2] * gz, q.r[5] * gz, q.r[8] * gz, gz * gz);
+        gzt[tid] = gz;
+    }
+    if (tid <= KFX_MAX_TSDF_WEIGHT) {
+        const float fb = (float)(tid + 1);
+        const float y0 = __builtin_amdgcn_rcpf(fb);
+        WeightTerms t;
+        t.fw = (float)tid;
+        t.fb = fb;
+        t.y = fmaf(fmaf(-fb, y0, 1.0f), y0, y0);
+        t.w_new_hi = (uint32_t)min(tid + 1, KFX_MAX_TSDF_WEIGHT) << 16;
+        qt[tid] = t;
+    }
+    __syncthreads();
 
+    const int x = blk.bx * kIntegrateBlockX + threadIdx.x;
+    const int y = blk.by * kIntegrateBlockY + threadIdx.y;
+    const bool inside = x < p.nx && y < p.ny;
     const float gx = ((float)x + 0.5f) * p.csx - q.tx;
     const float gy = ((float)y + 0.5f) * p.csy - q.ty;
     const float part = gx * gx + gy * gy;
-    const uint32_t col = inside ? (uint32_t)wrap_once(x + p.ox, p.nx) + (uint32_t)p.nx * (uint32_t)wrap_once(y + p.oy, p.ny) : 0u;
-
-    // Clip the run to the frustum; the wave walks the union of its lanes'
-    // ranges so that every z step stays one contiguous 256-byte access.
-    int lo, hi;
-    clip_column(p, q, gx, gy, z_begin, z_end, lo, hi);
-    if (!inside) { lo = z_end; hi = z_begin - 1; }
-    const int own_lo = lo, own_hi = hi;
-    lo = wave_min(lo);
-    hi = wave_max(hi);
-    if (!inside) return;
-
-    // Column terms of the camera coordinates: the rule's dot products are
-    // evaluated as (r0 * gx + r1 * gy) + r2 * gz, so the first two terms are
-    // shared by the whole column with identical rounding.
+    // Column terms of the camera coordinates: (r0 gx + r1 gy) + r2 gz.
     const float cxy0 = q.r[0] * gx + q.r[1] * gy;
     const float cxy1 = q.r[3] * gx + q.r[4] * gy;
     const float cxy2 = q.r[6] * gx + q.r[7] * gy;
-    const float fcols = (float)p.cols, frows = (float)p.rows;
-    const uint32_t nz = (uint32_t)p.nz;
-    for (int z0 = lo; z0 <= hi; z0 += kIntegrateBatch) {
-        // Skip the batch when no lane can update a voxel of it (every lane's
-        // part of the batch inside its own clipped range is culled).
-        if (p.bound != nullptr) {
-            const int za = max(z0, own_lo), zb = min(z0 + kIntegrateBatch - 1, own_hi);
-            const bool culled = za > zb || segment_culled(p, q, scaled, gx, gy, cxy0, cxy1, cxy2, za, zb);
-            if (__all(culled)) continue;
+    int lo = z_begin, hi = z_end - 1;
+    if (cls != 2) clip_column(p, q, gx, gy, z_begin, z_end, lo, hi);
+    if (inside && lo <= hi && p.bound != nullptr) {
+        // Distance bound of the whole run.
+        const float4 ta = zt[lo - z_begin], tb = zt[hi - z_begin];
+        const float maxd = segment_max_depth(p, scaled, cxy0 + ta.x, cxy1 + ta.y, cxy2 + ta.z, cxy0 + tb.x,
+                                             cxy1 + tb.y, cxy2 + tb.z);
+        if (maxd == 0.0f) {
+            lo = z_end;
+            hi = z_begin - 1;
+        } else if (maxd > 0.0f) {
+            const float r = reach(p, maxd);
+            const float room = r * r - part;
+            if (!(room > 0.0f)) {
+                lo = z_end;
+                hi = z_begin - 1;
+            } else {
+                // |gz| <= sqrt(room): z in [(tz - s) / csz - 1/2, (tz + s) / csz - 1/2],
+                // widened by a voxel for the rounding of gz.
+                const float sq = sqrtf(room) * 1.0001f;
+                const float zl = (q.tz - sq) / p.csz - 0.5f, zh = (q.tz + sq) / p.csz - 0.5f;
+                lo = max(lo, (int)fmaxf(floorf(zl) - 1.0f, (float)z_begin - 1.0f));
+                hi = min(hi, (int)fminf(ceilf(zh) + 1.0f, (float)z_end));
+            }
         }
-        float dp[kIntegrateBatch], dist[kIntegrateBatch];
-        bool inimg[kIntegrateBatch];
-        // Phase 1 (no branches): projection and depth gather of the batch.
-        // Pixels outside the image read pixel 0 and are masked afterwards.
+    }
+    if (!inside) { lo = z_end; hi = z_begin - 1; }
+    const int own_lo = lo, own_hi = hi;
+    // The wave's range, in scalar registers: the batch loop, the slice
+    // addresses and the z tests are then scalar work.
+    lo = __builtin_amdgcn_readfirstlane(wave_min(lo));
+    hi = __builtin_amdgcn_readfirstlane(wave_max(hi));
+    if (!inside) return;
+
+    const float fcols = (float)p.cols, frows = (float)p.rows;
+    const float free_c = 1.01f * p.trunc;
+    const uint32_t col4 = 4u * ((uint32_t)wrap_once(x + p.ox, p.nx) + (uint32_t)p.nx * (uint32_t)wrap_once(y + p.oy, p.ny));
+    const uint64_t slice4 = 4ull * p.slice;
+    const uint32_t slice_bytes = 4u * p.slice;
+    const uint32_t nz = (uint32_t)p.nz;
+    const char* __restrict__ sbase = reinterpret_cast<const char*>(scaled);
+    char* __restrict__ vbase = reinterpret_cast<char*>(volume);
+    for (int z0 = lo; z0 <= hi; z0 += kIntegrateBatch) {
+        const int za = max(z0, own_lo), zb = min(z0 + kIntegrateBatch - 1, own_hi);
+        bool culled = za > zb;
+        if (!culled && p.bound != nullptr) {
+            const float4 ta = zt[za - z_begin], tb = zt[zb - z_begin];
+            const float maxd = segment_max_depth(p, scaled, cxy0 + ta.x, cxy1 + ta.y, cxy2 + ta.z, cxy0 + tb.x,
+                                                 cxy1 + tb.y, cxy2 + tb.z);
+            if (maxd == 0.0f) {
+                culled = true;
+            } else if (maxd > 0.0f) {
+                // Nearest point of the segment to the camera, squared.
+                const float gza = gzt[za - z_begin], gzb = gzt[zb - z_begin];
+                const float dz = (gza <= 0.0f && gzb >= 0.0f) ? 0.0f : fminf(fabsf(gza), fabsf(gzb));
+                const float r = reach(p, maxd);
+                culled = part + dz * dz > r * r;
+            }
+        }
+        if (__all(culled)) continue;
+        float dp[kIntegrateBatch], d2[kIntegrateBatch];
+        bool inimg[kIntegrateBatch], act[kIntegrateBatch];
+        uint32_t word[kIntegrateBatch];
+        const char* zbase[kIntegrateBatch];
+        // Phase 1: projection and depth gather of every voxel of the batch
+        // (outside the image: pixel 0, masked afterwards); the voxel loads
+        // of the lanes not culled.  The slice of each voxel is the same for
+        // the whole wave.
+        float4 tn = zt[z0 - z_begin];
 #pragma unroll
         for (int k = 0; k < kIntegrateBatch; ++k) {
             const int z = z0 + k;
-            const float gz = ((float)z + 0.5f) * p.csz - q.tz;
-            const float camx = cxy0 + q.r[2] * gz;
-            const float camy = cxy1 + q.r[5] * gz;
-            const float camz = cxy2 + q.r[8] * gz;
+            const float4 t = tn;
+            if (k + 1 < kIntegrateBatch) tn = zt[z + 1 - z_begin];
+            const float camx = cxy0 + t.x;
+            const float camy = cxy1 + t.y;
+            const float camz = cxy2 + t.z;
             // In front of the camera: camz in [2^-126, 2^126), where rcp_rn is
             // the correctly rounded reciprocal.
             const float inv_z = rcp_rn(camz);
@@ -378,44 +645,49 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
             const float fv = rintf(camy * p.fy * inv_z + p.cy);
             const bool in = (z <= hi) & (camz >= kRcpMin) & (camz < kRcpMax) & (fu >= 0.0f) & (fu < fcols) &
                             (fv >= 0.0f) & (fv < frows);
-            const int pix = (int)(in ? fv : 0.0f) * p.cols + (int)(in ? fu : 0.0f);
-            dp[k] = scaled[pix];
-            dist[k] = sqrt_rn(gz * gz + part);
+            // v cols + u is an integer below 2^24 inside the image, exact in
+            // one fma.
+            const float pixf = fmaf(fv, fcols, fu);
+            const uint32_t pix = in ? (uint32_t)pixf : 0u;
+            dp[k] = *reinterpret_cast<const float*>(sbase + 4u * pix);
+            d2[k] = t.w + part;
             inimg[k] = in;
+            uint32_t rz = (uint32_t)z + (uint32_t)p.oz;
+            rz = min(rz, rz - nz);  // (z + oz) mod nz for z + oz < 2 nz (uniform)
+            zbase[k] = vbase + slice4 * rz;
+            act[k] = !culled & (z >= own_lo) & (z <= own_hi);
+            word[k] = 0u;
+            if (act[k]) word[k] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(slice_rsrc(zbase[k], slice_bytes), col4, 0, 0);
         }
-        // Phase 2: the update test, and the loads of the voxels it selects.
-        float tsdf_new[kIntegrateBatch];
-        bool upd[kIntegrateBatch];
-        kfx_tsdf prev[kIntegrateBatch];
-        uint32_t addr[kIntegrateBatch];
+        // Phase 2: the update test, the running average, and the write of
+        // what changed.
 #pragma unroll
         for (int k = 0; k < kIntegrateBatch; ++k) {
-            const float sdf = dp[k] - dist[k];
-            upd[k] = inimg[k] & (dp[k] != 0.0f) & (sdf >= -p.trunc);
-            tsdf_new[k] = fminf(1.0f, sdf * p.inv_trunc);
-            uint32_t rz = (uint32_t)(z0 + k) + (uint32_t)p.oz;
-            rz = min(rz, rz - nz);  // (z + oz) mod nz for z + oz < 2 nz
-            addr[k] = col + p.slice * rz;
-            if (upd[k]) prev[k] = volume[addr[k]];
-        }
-        // Phase 3: running weighted average, quantise, write what changed.
-#pragma unroll
-        for (int k = 0; k < kIntegrateBatch; ++k) {
-            if (!upd[k]) continue;
-            const float tp = tsdf_decode(prev[k].value);
-            // Weights outside [0, KFX_MAX_TSDF_WEIGHT] (a volume not written
-            // by this rule) count as the nearest bound, which keeps w + 1
-            // inside tsdf_average_quotient's exact range.
-            const int w = max(0, min((int)prev[k].weight, KFX_MAX_TSDF_WEIGHT));
-            const float avg = tsdf_average_quotient(tp * (float)w + tsdf_new[k], w + 1);
-            const int w_new = min(w + 1, KFX_MAX_TSDF_WEIGHT);
-            const int16_t v_new = tsdf_encode(avg);
-            if (v_new != prev[k].value || w_new != prev[k].weight) {
-                kfx_tsdf out;
-                out.value = v_new;
-                out.weight = (int16_t)w_new;
-                volume[addr[k]] = out;
+            const bool valid = act[k] & inimg[k] & (dp[k] != 0.0f);
+            const float f = dp[k] * 0.999999f - free_c;
+            const bool free_space = d2[k] < f * fabsf(f);
+            float tsdf_new = 1.0f;
+            bool upd = valid & free_space;
+            const bool near = valid & !free_space;
+            if (__builtin_amdgcn_ballot_w64(near) != 0) {
+                // Near the surface: the rule with the correctly rounded root.
+                const float sdf = dp[k] - sqrt_rn_in_range(d2[k]);
+                if (near && sdf >= -p.trunc) {
+                    upd = true;
+                    tsdf_new = fminf(1.0f, sdf * p.inv_trunc);
+                }
             }
+            if (!upd) continue;
+            const int16_t v_old = (int16_t)(word[k] & 0xffffu);
+            // Weights outside [0, KFX_MAX_TSDF_WEIGHT] (a volume not written
+            // by this rule) count as the nearest bound.
+            const int w = max(0, min((int)(int16_t)(word[k] >> 16), KFX_MAX_TSDF_WEIGHT));
+            const WeightTerms bq = qt[w];
+            const float a = tsdf_decode(v_old) * bq.fw + tsdf_new;
+            const float q0 = a * bq.y;
+            const float avg = fmaf(fmaf(-q0, bq.fb, a), bq.y, q0);
+            const uint32_t out = bq.w_new_hi | (uint32_t)(uint16_t)tsdf_encode_unit(avg);
+            if (out != word[k]) __builtin_amdgcn_raw_buffer_store_b32(out, slice_rsrc(zbase[k], slice_bytes), col4, 0, 0);
         }
     }
 }
@@ -424,12 +696,13 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
 // updates (one 64-bit atomic per wave).
 __global__ void __launch_bounds__(kIntegrateBlockX * kIntegrateBlockY)
 integrate_count_kernel(const float* __restrict__ scaled, IntegrateParams p, unsigned long long* __restrict__ count) {
-    const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
-    const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
+    const SweepBlock blk = sweep_block();
+    const int x = blk.bx * kIntegrateBlockX + threadIdx.x;
+    const int y = blk.by * kIntegrateBlockY + threadIdx.y;
     unsigned n = 0;
     const PoseTerms q = host_terms(p);
     if (x < p.nx && y < p.ny) {
-        const int z_begin = blockIdx.z * p.zrun;
+        const int z_begin = blk.bz * p.zrun;
         const int z_end = min(z_begin + p.zrun, p.nz);
         const float gx = ((float)x + 0.5f) * p.csx - q.tx;
         const float gy = ((float)y + 0.5f) * p.csy - q.ty;
@@ -543,10 +816,11 @@ integrate_colors_kernel(const kfx_color* __restrict__ rgb, const kfx_xyz* __rest
     PoseTerms q;
     if (kDevicePose) (void)load_device_pose(p, q, dpose, nullptr, 0.0f);
     else q = host_terms(p);
-    const int x = blockIdx.x * kIntegrateBlockX + threadIdx.x;
-    const int y = blockIdx.y * kIntegrateBlockY + threadIdx.y;
+    const SweepBlock blk = sweep_block();
+    const int x = blk.bx * kIntegrateBlockX + threadIdx.x;
+    const int y = blk.by * kIntegrateBlockY + threadIdx.y;
     const bool inside = x < p.nx && y < p.ny;
-    const int z_begin = blockIdx.z * p.zrun;
+    const int z_begin = blk.bz * p.zrun;
     const int z_end = min(z_begin + p.zrun, p.nz);
 
     const float vgx = ((float)x + 0.5f) * p.csx, vgy = ((float)y + 0.5f) * p.csy;
@@ -703,21 +977,47 @@ size_t bound_floats(int cols, int rows) {
     return bound_layout(cols, rows, off);
 }
 
-// Scales the depth into depth_scaled and builds its depth-bound pyramid into
-// ws.bound (one launch), pointing p at it.  A workspace too small for the
-// image (the library's, inside a stream capture, before any reservation)
-// only scales the depth, and the integration runs unculled (same result,
-// more work).
-static int integrate_bound(const Workspace& ws, bool profile, const uint16_t* depth, float* depth_scaled,
-                           const kfx_intrinsics* intr, IntegrateParams& p) {
+// Frame terms buffer of a workspace: the PoseTerms of the frame, then its
+// skip flag.
+constexpr size_t kTermsSkipOffset = (sizeof(PoseTerms) + 15) / 16 * 16;
+constexpr size_t kTermsBytes = kTermsSkipOffset + 16;
+
+size_t integrate_blocks(const kfx_point_3i& n) {
+    return (size_t)divup((unsigned)n.x, kIntegrateBlockX) * divup((unsigned)n.y, kIntegrateBlockY) *
+           divup((unsigned)n.z, (unsigned)integrate_zrun(n.z));
+}
+
+// The integration pre-pass (one launch): the scaled depth into depth_scaled
+// and, as far as ws holds them, the depth-bound pyramid (p.bound points at
+// it) and the workgroup flags of the sweep grid igrid (*flags_out), with a
+// device pose also the frame's pose terms and skip flag.  Without pyramid or
+// flags (the library workspace inside a stream capture before any
+// reservation) the host-pose sweep runs unculled: same result, more work.
+template <bool kDevicePose>
+static int integrate_prepass(const Workspace& ws, bool profile, const uint16_t* depth, float* depth_scaled,
+                             const kfx_intrinsics* intr, IntegrateParams& p, dim3 igrid, const float* dpose,
+                             const float* dprev, float motion_thr, const int32_t* dskip, const uint8_t** flags_out) {
     p.bound = nullptr;
+    *flags_out = nullptr;
     const size_t need = bound_layout(intr->cols, intr->rows, p.bound_off);
-    if (ws.bound == nullptr || need > ws.bound_cap) return scale_depth_launch(depth, depth_scaled, intr);
-    ProfScope prof(profile ? "scale_depth" : nullptr);
+    const bool have_bound = ws.bound != nullptr && need <= ws.bound_cap;
+    const size_t nblk = (size_t)igrid.x * igrid.y * igrid.z;
+    const bool have_flags = ws.flags != nullptr && nblk <= ws.flags_cap && (!kDevicePose || ws.terms != nullptr);
+    KFX_REQUIRE(!kDevicePose || have_flags,
+                "kfx_integrate_tsdf_posed: no workspace for a %zu-workgroup sweep (reserve it outside a capture)",
+                nblk);
+    if (!have_bound && !have_flags) return scale_depth_launch(depth, depth_scaled, intr);
+    PoseTerms* terms = kDevicePose ? static_cast<PoseTerms*>(ws.terms) : nullptr;
+    int32_t* skip = kDevicePose ? reinterpret_cast<int32_t*>(static_cast<char*>(ws.terms) + kTermsSkipOffset) : nullptr;
+    ProfScope prof(profile ? "depth_bound" : nullptr);
     dim3 grid(divup(intr->cols, kBoundTile), divup(intr->rows, kBoundTile));
-    depth_bound_kernel<<<grid, kBoundTile * kBoundTile, 0, stream()>>>(depth, depth_scaled, *intr, ws.bound, p);
+    // igrid is sweep_grid's (y blocks, x blocks, z runs).
+    integrate_prepass_kernel<kDevicePose><<<grid, kBoundTile * kBoundTile, 0, stream()>>>(
+        depth, depth_scaled, *intr, have_bound ? ws.bound : nullptr, p, dpose, dprev, motion_thr, dskip, terms, skip,
+        have_flags ? ws.flags : nullptr, (int)igrid.y, (int)igrid.x, (int)igrid.z);
     KFX_CHECK_LAUNCH();
-    p.bound = ws.bound;
+    if (have_bound) p.bound = ws.bound;
+    if (have_flags) *flags_out = ws.flags;
     return KFX_OK;
 }
 
@@ -783,7 +1083,7 @@ int scale_depth_launch(const uint16_t* depth, float* depth_scaled, const kfx_int
 }
 
 // Sweep terms; with scale, also the scaled depth (the integrations instead
-// scale it in integrate_bound's launch).
+// scale it in the integration pre-pass).
 static int integrate_setup(const char* who, const uint16_t* depth, float* depth_scaled,
                            const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
                            const kfx_point_3f* translation, float truncation_distance,
@@ -852,10 +1152,25 @@ int workspace_reserve(Workspace& ws, int cols, int rows, bool colors) {
     return KFX_OK;
 }
 
+int workspace_reserve_volume(Workspace& ws, const kfx_point_3i& voxels) {
+    const size_t nb = integrate_blocks(voxels);
+    if (ws.terms == nullptr) KFX_CHECK_HIP(hipMalloc(&ws.terms, kTermsBytes));
+    if (ws.flags_cap < nb) {
+        if (ws.flags) KFX_CHECK_HIP(hipFree(ws.flags));
+        ws.flags = nullptr;
+        ws.flags_cap = 0;
+        KFX_CHECK_HIP(hipMalloc((void**)&ws.flags, nb));
+        ws.flags_cap = nb;
+    }
+    return KFX_OK;
+}
+
 void workspace_free(Workspace& ws) {
     if (ws.bound) (void)hipFree(ws.bound);
     if (ws.cbound) (void)hipFree(ws.cbound);
     if (ws.cons) (void)hipFree(ws.cons);
+    if (ws.terms) (void)hipFree(ws.terms);
+    if (ws.flags) (void)hipFree(ws.flags);
     ws = Workspace();
 }
 
@@ -866,11 +1181,13 @@ std::vector<void*> g_outgrown;  // library buffers replaced by larger ones, kept
 
 Workspace& library_workspace() { return g_lib_ws; }
 
-int library_workspace_reserve(int cols, int rows, bool colors, bool gate) {
+int library_workspace_reserve(int cols, int rows, bool colors, bool gate, const kfx_point_3i* voxels) {
     const bool need_b = g_lib_ws.bound_cap < bound_floats(cols, rows);
     const bool need_c = colors && g_lib_ws.cbound_cap < cbound_texels(cols, rows);
     const bool need_g = gate && g_lib_ws.cons_cap < cons_blocks(cols, rows);
-    if (!need_b && !need_c && !need_g) return KFX_OK;
+    const size_t nflags = voxels ? integrate_blocks(*voxels) : 0;
+    const bool need_f = voxels != nullptr && (g_lib_ws.flags_cap < nflags || g_lib_ws.terms == nullptr);
+    if (!need_b && !need_c && !need_g && !need_f) return KFX_OK;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
     if (cs != hipStreamCaptureStatusNone)
@@ -884,6 +1201,14 @@ int library_workspace_reserve(int cols, int rows, bool colors, bool gate) {
     }
     if (need_g) {
         hipError_t e = hipMalloc((void**)&grown.cons, cons_blocks(cols, rows) * 4 * sizeof(double));
+        if (e != hipSuccess) { workspace_free(grown); KFX_CHECK_HIP(e); }
+    }
+    if (need_f && g_lib_ws.flags_cap < nflags) {
+        hipError_t e = hipMalloc((void**)&grown.flags, nflags);
+        if (e != hipSuccess) { workspace_free(grown); KFX_CHECK_HIP(e); }
+    }
+    if (need_f && g_lib_ws.terms == nullptr) {
+        hipError_t e = hipMalloc(&grown.terms, kTermsBytes);
         if (e != hipSuccess) { workspace_free(grown); KFX_CHECK_HIP(e); }
     }
     if (need_b) {
@@ -901,15 +1226,22 @@ int library_workspace_reserve(int cols, int rows, bool colors, bool gate) {
         g_lib_ws.cons = grown.cons;
         g_lib_ws.cons_cap = cons_blocks(cols, rows);
     }
+    if (grown.flags) {
+        if (g_lib_ws.flags) g_outgrown.push_back(g_lib_ws.flags);
+        g_lib_ws.flags = grown.flags;
+        g_lib_ws.flags_cap = nflags;
+    }
+    if (grown.terms) g_lib_ws.terms = grown.terms;
     return KFX_OK;
 }
 
 // Library workspace for a standalone call: reserved outside a capture;
 // inside one, whatever was reserved before (too small: unculled).
-static int library_ws_for(int cols, int rows, bool colors, const Workspace** out) {
+static int library_ws_for(int cols, int rows, bool colors, const Workspace** out,
+                          const kfx_point_3i* voxels = nullptr) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     KFX_CHECK_HIP(hipStreamIsCapturing(stream(), &cs));
-    if (cs == hipStreamCaptureStatusNone) KFX_TRY(library_workspace_reserve(cols, rows, colors, false));
+    if (cs == hipStreamCaptureStatusNone) KFX_TRY(library_workspace_reserve(cols, rows, colors, false, voxels));
     *out = &g_lib_ws;
     return KFX_OK;
 }
@@ -932,11 +1264,13 @@ int integrate_tsdf_ws(const Workspace& ws, bool profile, const uint16_t* depth, 
     int rc = integrate_setup("kfx_integrate_tsdf", depth, depth_scaled, intr, rotation_inverse, translation,
                              truncation_distance, cell_size, rb, p, false);
     if (rc) return rc;
-    KFX_TRY(integrate_bound(ws, profile, depth, depth_scaled, intr, p));
-    ProfScope prof(profile ? "integrate" : nullptr);
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
-    integrate_kernel<false><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, nullptr, nullptr, 0.0f, nullptr);
+    const dim3 grid = sweep_grid(p.nx, p.ny, p.nz, p.zrun);
+    const uint8_t* flags = nullptr;
+    KFX_TRY(integrate_prepass<false>(ws, profile, depth, depth_scaled, intr, p, grid, nullptr, nullptr, 0.0f, nullptr,
+                                     &flags));
+    ProfScope prof(profile ? "integrate" : nullptr);
+    integrate_kernel<false><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, nullptr, nullptr, flags);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
@@ -952,12 +1286,15 @@ int integrate_tsdf_posed_ws(const Workspace& ws, const uint16_t* depth, float* d
     int rc = integrate_setup("kfx_integrate_tsdf_posed", depth, depth_scaled, intr, &ident, &zero,
                              truncation_distance, cell_size, rb, p, false);
     if (rc) return rc;
-    KFX_TRY(integrate_bound(ws, true, depth, depth_scaled, intr, p));
-    ProfScope prof("integrate");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
-    integrate_kernel<true><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, dev_pose, dev_prev_pose_inv,
-                                                         motion_threshold, dev_skip);
+    const dim3 grid = sweep_grid(p.nx, p.ny, p.nz, p.zrun);
+    const uint8_t* flags = nullptr;
+    KFX_TRY(integrate_prepass<true>(ws, true, depth, depth_scaled, intr, p, grid, dev_pose, dev_prev_pose_inv,
+                                    motion_threshold, dev_skip, &flags));
+    ProfScope prof("integrate");
+    const PoseTerms* terms = static_cast<const PoseTerms*>(ws.terms);
+    const int32_t* skip = reinterpret_cast<const int32_t*>(static_cast<const char*>(ws.terms) + kTermsSkipOffset);
+    integrate_kernel<true><<<grid, block, 0, stream()>>>(depth_scaled, volume, p, terms, skip, flags);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
@@ -974,7 +1311,7 @@ int integrate_colors_ws(const Workspace& ws, kfx_color* color_volume, const kfx_
     KFX_TRY(colors_bound(ws, true, vertex_map, p, nullptr, nullptr, &cb));
     ProfScope prof("integrate_colors");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
+    const dim3 grid = sweep_grid(p.nx, p.ny, p.nz, p.zrun);
     integrate_colors_kernel<false><<<grid, block, 0, stream()>>>(rgb_frame, vertex_map, color_volume, p, cb,
                                                                  max_weight, nullptr, nullptr);
     KFX_CHECK_LAUNCH();
@@ -995,7 +1332,7 @@ int integrate_colors_posed_ws(const Workspace& ws, kfx_color* color_volume, cons
     KFX_TRY(colors_bound(ws, true, vertex_map, p, dev_pose, dev_skip, &cb));
     ProfScope prof("integrate_colors");
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-    dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
+    const dim3 grid = sweep_grid(p.nx, p.ny, p.nz, p.zrun);
     integrate_colors_kernel<true><<<grid, block, 0, stream()>>>(rgb_frame, vertex_map, color_volume, p, cb,
                                                                 max_weight, dev_pose, dev_skip);
     KFX_CHECK_LAUNCH();
@@ -1032,9 +1369,9 @@ int kfx_integrate_tsdf(const uint16_t* depth, float* depth_scaled, kfx_tsdf* vol
                        const kfx_point_3f* translation, float truncation_distance,
                        const kfx_point_3f* cell_size, const kfx_round_buffer* rb) {
     KFX_REQUIRE_INIT();
-    KFX_REQUIRE(intr != nullptr, "kfx_integrate_tsdf: null pointer");
+    KFX_REQUIRE(intr != nullptr && rb != nullptr, "kfx_integrate_tsdf: null pointer");
     const Workspace* ws = nullptr;
-    KFX_TRY(library_ws_for(intr->cols, intr->rows, false, &ws));
+    KFX_TRY(library_ws_for(intr->cols, intr->rows, false, &ws, &rb->voxels_size));
     return integrate_tsdf_ws(*ws, true, depth, depth_scaled, volume, intr, rotation_inverse, translation,
                              truncation_distance, cell_size, rb);
 }
@@ -1044,9 +1381,9 @@ int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsd
                              float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
                              const kfx_round_buffer* rb, const int32_t* dev_skip) {
     KFX_REQUIRE_INIT();
-    KFX_REQUIRE(intr != nullptr, "kfx_integrate_tsdf_posed: null pointer");
+    KFX_REQUIRE(intr != nullptr && rb != nullptr, "kfx_integrate_tsdf_posed: null pointer");
     const Workspace* ws = nullptr;
-    KFX_TRY(library_ws_for(intr->cols, intr->rows, false, &ws));
+    KFX_TRY(library_ws_for(intr->cols, intr->rows, false, &ws, &rb->voxels_size));
     return integrate_tsdf_posed_ws(*ws, depth, depth_scaled, volume, intr, dev_pose, dev_prev_pose_inv,
                                    motion_threshold, truncation_distance, cell_size, rb, dev_skip);
 }
@@ -1086,7 +1423,7 @@ int kfx_integrate_count_updates(const uint16_t* depth, float* depth_scaled, cons
     hipError_t e = hipMemsetAsync(dcount, 0, sizeof(unsigned long long), stream());
     if (e == hipSuccess) {
         dim3 block(kIntegrateBlockX, kIntegrateBlockY);
-        dim3 grid(divup(p.nx, kIntegrateBlockX), divup(p.ny, kIntegrateBlockY), divup(p.nz, p.zrun));
+        const dim3 grid = sweep_grid(p.nx, p.ny, p.nz, p.zrun);
         integrate_count_kernel<<<grid, block, 0, stream()>>>(depth_scaled, p, dcount);
         e = hipGetLastError();
     }
