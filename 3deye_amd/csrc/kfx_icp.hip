@@ -458,7 +458,19 @@ __device__ __forceinline__ void icp_load_current(const kfx_xyz* __restrict__ vcu
 // Partials: [27][workgroup] float2 (sum, compensation) in the FP32 modes,
 // double in the FP64 mode; entry-major, so the cross-workgroup sum's loads of
 // one entry over consecutive workgroups coalesce.
-template <int P, int M>
+// Partial stores for a reader on another XCD within the same launch: a
+// relaxed atomic store at agent scope goes past the XCD's L2 to the device's
+// coherence point (no write-back of the L2 needed).
+__device__ __forceinline__ void store_coherent(void* dst, uint64_t bits) {
+    __hip_atomic_store(static_cast<unsigned long long*>(dst), (unsigned long long)bits, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t load_coherent(const void* src) {
+    return (uint64_t)__hip_atomic_load(static_cast<const unsigned long long*>(src), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int P, int M, bool kCoherent = false>
 __device__ __forceinline__ void icp_block_sums(const kfx_xyz (&v)[P], const kfx_xyz (&n)[P],
                                                const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev,
                                                const IcpFrame& f, const float* Rc, const float* tc,
@@ -497,7 +509,11 @@ __device__ __forceinline__ void icp_block_sums(const kfx_xyz (&v)[P], const kfx_
                 for (int jj = i; jj < 7; ++jj, ++k) acc[k] += (double)(rows[q][i] * rows[q][jj]);
         }
         const double s = workgroup_sum27(acc, wsum_d);
-        if (threadIdx.x < kIcpEntries) static_cast<double*>(partials)[threadIdx.x * gridDim.x + blockIdx.x] = s;
+        if (threadIdx.x < kIcpEntries) {
+            double* dst = static_cast<double*>(partials) + threadIdx.x * gridDim.x + blockIdx.x;
+            if constexpr (kCoherent) store_coherent(dst, __double_as_longlong(s));
+            else *dst = s;
+        }
     } else {
         float sum[32], comp[32];
 #pragma unroll
@@ -517,7 +533,12 @@ __device__ __forceinline__ void icp_block_sums(const kfx_xyz (&v)[P], const kfx_
         float2 r;
         if constexpr (M == kSumKahan) r = workgroup_sum27(sum, comp, wsum_f);
         else r = make_float2(workgroup_sum27(sum, reinterpret_cast<float (*)[32]>(wsum_f)), 0.0f);
-        if (threadIdx.x < kIcpEntries) static_cast<float2*>(partials)[threadIdx.x * gridDim.x + blockIdx.x] = r;
+        if (threadIdx.x < kIcpEntries) {
+            float2* dst = static_cast<float2*>(partials) + threadIdx.x * gridDim.x + blockIdx.x;
+            if constexpr (kCoherent)
+                store_coherent(dst, ((uint64_t)__float_as_uint(r.y) << 32) | __float_as_uint(r.x));
+            else *dst = r;
+        }
     }
 }
 
@@ -557,7 +578,22 @@ icp_sums_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
 // to kRound workgroups' loads issued together per round), then the
 // fixed-order workgroup sum.  The result depends on nblk and the partials
 // only, not on kRound or on the order in which the workgroups ran.
-template <int M, int kRound>
+template <bool kCoherent>
+__device__ __forceinline__ float2 partial_f2(const float2* p) {
+    if constexpr (kCoherent) {
+        const uint64_t b = load_coherent(p);
+        return make_float2(__uint_as_float((uint32_t)b), __uint_as_float((uint32_t)(b >> 32)));
+    } else {
+        return *p;
+    }
+}
+template <bool kCoherent>
+__device__ __forceinline__ double partial_d(const double* p) {
+    if constexpr (kCoherent) return __longlong_as_double((long long)load_coherent(p));
+    else return *p;
+}
+
+template <int M, int kRound, bool kCoherent = false>
 __device__ __forceinline__ void cross_block_sum(const void* __restrict__ partials, int nblk, double (*wsum)[32],
                                                 double* total) {
     double acc[32];
@@ -569,11 +605,11 @@ __device__ __forceinline__ void cross_block_sum(const void* __restrict__ partial
         for (; b + 2 * kIcpThreads < nblk; b += 3 * kIcpThreads) {
             float2 t0[kIcpEntries], t1[kIcpEntries], t2[kIcpEntries];
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t0[e] = pf[e * nblk + b];
+            for (int e = 0; e < kIcpEntries; ++e) t0[e] = partial_f2<kCoherent>(&pf[e * nblk + b]);
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t1[e] = pf[e * nblk + b + kIcpThreads];
+            for (int e = 0; e < kIcpEntries; ++e) t1[e] = partial_f2<kCoherent>(&pf[e * nblk + b + kIcpThreads]);
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t2[e] = pf[e * nblk + b + 2 * kIcpThreads];
+            for (int e = 0; e < kIcpEntries; ++e) t2[e] = partial_f2<kCoherent>(&pf[e * nblk + b + 2 * kIcpThreads]);
 #pragma unroll
             for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t0[e].x + (double)t0[e].y;
 #pragma unroll
@@ -587,9 +623,9 @@ __device__ __forceinline__ void cross_block_sum(const void* __restrict__ partial
         for (; b + kIcpThreads < nblk; b += 2 * kIcpThreads) {
             float2 t0[kIcpEntries], t1[kIcpEntries];
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t0[e] = pf[e * nblk + b];
+            for (int e = 0; e < kIcpEntries; ++e) t0[e] = partial_f2<kCoherent>(&pf[e * nblk + b]);
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t1[e] = pf[e * nblk + b + kIcpThreads];
+            for (int e = 0; e < kIcpEntries; ++e) t1[e] = partial_f2<kCoherent>(&pf[e * nblk + b + kIcpThreads]);
 #pragma unroll
             for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t0[e].x + (double)t0[e].y;
 #pragma unroll
@@ -601,14 +637,14 @@ __device__ __forceinline__ void cross_block_sum(const void* __restrict__ partial
             const double* pd = static_cast<const double*>(partials) + b;
             double t[kIcpEntries];
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t[e] = pd[e * nblk];
+            for (int e = 0; e < kIcpEntries; ++e) t[e] = partial_d<kCoherent>(&pd[e * nblk]);
 #pragma unroll
             for (int e = 0; e < kIcpEntries; ++e) acc[e] += t[e];
         } else {
             const float2* pf = static_cast<const float2*>(partials) + b;
             float2 t[kIcpEntries];
 #pragma unroll
-            for (int e = 0; e < kIcpEntries; ++e) t[e] = pf[e * nblk];
+            for (int e = 0; e < kIcpEntries; ++e) t[e] = partial_f2<kCoherent>(&pf[e * nblk]);
 #pragma unroll
             for (int e = 0; e < kIcpEntries; ++e) acc[e] += (double)t[e].x + (double)t[e].y;
         }
@@ -649,6 +685,96 @@ icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials, i
     for (int i = 0; i < 3; ++i) t0[i] = st->t[i];
     const int iterations = st->iterations;
     cross_block_sum<M, 3>(partials, nblk, wsum, total);
+    if (threadIdx.x == 0) {
+        solve_and_update(st, total, R0, t0, iterations, tag, conv_rot, conv_trans);
+        if (mirror != nullptr) icp_publish(st, mirror);
+    }
+}
+
+// cross_block_sum over coherent loads, in a register budget that leaves the
+// one-launch iteration four waves per SIMD: thread t adds the partials of
+// workgroups t, t + kIcpThreads, t + 2 kIcpThreads, ... per entry, nine
+// entries at a time (their loads issued together), in double precision and
+// in that order, then the fixed-order workgroup sum.  The same totals as
+// cross_block_sum: each entry's per-thread sum runs over the same
+// workgroups in the same order.
+template <int M>
+__device__ __forceinline__ void cross_block_sum_coherent(const void* __restrict__ partials, int nblk,
+                                                         double (*wsum)[32], double* total) {
+    constexpr int kGroup = 9;
+    static_assert(kIcpEntries % kGroup == 0, "entry groups");
+    double acc[32];
+#pragma unroll
+    for (int e = 0; e < 32; ++e) acc[e] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kIcpThreads) {
+#pragma unroll
+        for (int g = 0; g < kIcpEntries; g += kGroup) {
+            double t[kGroup];
+#pragma unroll
+            for (int e = 0; e < kGroup; ++e) {
+                if constexpr (M == kSumDouble) {
+                    t[e] = partial_d<true>(static_cast<const double*>(partials) + (size_t)(g + e) * nblk + b);
+                } else {
+                    const float2 v = partial_f2<true>(static_cast<const float2*>(partials) + (size_t)(g + e) * nblk + b);
+                    t[e] = (double)v.x + (double)v.y;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < kGroup; ++e) acc[g + e] += t[e];
+        }
+    }
+    const double s = workgroup_sum27<kIcpWaves>(acc, wsum);
+    if (threadIdx.x < kIcpEntries) total[threadIdx.x] = s;
+    __syncthreads();
+}
+
+// One ICP iteration in one launch: the 27 sums of each workgroup at the
+// state's pose, then the cross-workgroup sum, the solve and the pose update
+// by the workgroup that finishes last.  The partials go to the device's
+// coherence point (agent-scope atomic stores, no L2 write-back), each
+// workgroup waits until its stores have completed before taking a ticket
+// (one agent-scope atomic add), and the workgroup with the last ticket reads
+// the partials back with agent-scope atomic loads.  The sum is in workgroup
+// order whichever workgroup finishes last, so the state equals the
+// two-launch form's bit for bit.  The last workgroup resets the ticket for
+// the next launch, which the kernel boundary orders.
+constexpr int kWaitVmemZero = 0x0F70;  // s_waitcnt vmcnt(0): every load and store of the wave has completed
+
+template <int P, int M>
+__global__ void __launch_bounds__(kIcpThreads) __attribute__((amdgpu_waves_per_eu(4)))
+icp_iter_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ ncurr,
+                const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev, IcpFrame f,
+                IcpState* __restrict__ st, void* __restrict__ partials, unsigned* __restrict__ ticket, int tag,
+                double conv_rot, double conv_trans, IcpState* __restrict__ mirror) {
+    __shared__ double wsum_d[kIcpWaves][32];
+    __shared__ float2 wsum_f[kIcpWaves][32];
+    __shared__ double total[32];
+    __shared__ unsigned my_ticket;
+    kfx_xyz v[P], n[P];
+    icp_load_current<P>(vcurr, ncurr, f.cols * f.rows, v, n);
+    if (tag >= 0 && st->converged_tag == tag) {  // uniform across the grid
+        if (mirror != nullptr && blockIdx.x == 0 && threadIdx.x == 0) icp_publish(st, mirror);
+        return;
+    }
+    icp_prev_from_state(f, st);
+    float Rc[9], tc[3];
+    for (int i = 0; i < 9; ++i) Rc[i] = st->Rf[i];
+    for (int i = 0; i < 3; ++i) tc[i] = st->tf[i];
+    icp_block_sums<P, M, true>(v, n, vprev, nprev, f, Rc, tc, partials, wsum_d, wsum_f);
+    __builtin_amdgcn_s_waitcnt(kWaitVmemZero);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        my_ticket = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (my_ticket != gridDim.x - 1) return;  // uniform across the workgroup
+    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double R0[9], t0[3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R0[i] = st->R[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) t0[i] = st->t[i];
+    const int iterations = st->iterations;
+    cross_block_sum_coherent<M>(partials, gridDim.x, wsum_d, total);
     if (threadIdx.x == 0) {
         solve_and_update(st, total, R0, t0, iterations, tag, conv_rot, conv_trans);
         if (mirror != nullptr) icp_publish(st, mirror);
@@ -716,6 +842,8 @@ struct kfx_icp {
     int max_blocks;
     bool publish_next;  // the next iteration's solve writes host_state
     bool published;     // a publishing solve was enqueued since the last reset or translate
+    unsigned* ticket;   // workgroups done in the current one-launch iteration (zero between launches)
+    int launch_mode;    // KFX_ICP_LAUNCH_*
 };
 
 using namespace kfx;
@@ -724,6 +852,19 @@ template <int M>
 static void icp_iterate_launch(kfx_icp* h, const kfx_xyz* vc, const kfx_xyz* nc, const kfx_xyz* vp, const kfx_xyz* np,
                                const IcpFrame& f, int nblk, bool two, int tag, double conv_rot, double conv_trans,
                                bool publish) {
+    IcpState* mirror = publish ? h->host_state : nullptr;
+    if (h->launch_mode == KFX_ICP_LAUNCH_ONE) {
+        char name[32];
+        snprintf(name, sizeof(name), "icp_iter@%d", f.cols);
+        ProfScope prof(name);
+        if (two)
+            icp_iter_kernel<2, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials,
+                                                                      h->ticket, tag, conv_rot, conv_trans, mirror);
+        else
+            icp_iter_kernel<1, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials,
+                                                                      h->ticket, tag, conv_rot, conv_trans, mirror);
+        return;
+    }
     {
         // Named by image width, so a profile separates the pyramid levels.
         char name[32];
@@ -734,7 +875,6 @@ static void icp_iterate_launch(kfx_icp* h, const kfx_xyz* vc, const kfx_xyz* nc,
         else
             icp_sums_kernel<1, M><<<nblk, kIcpThreads, 0, stream()>>>(vc, nc, vp, np, f, h->state, h->partials, tag);
     }
-    IcpState* mirror = publish ? h->host_state : nullptr;
     ProfScope prof("icp_solve");
     icp_solve_kernel<M><<<1, kIcpThreads, 0, stream()>>>(h->state, h->partials, nblk, tag, conv_rot,
                                                                 conv_trans, mirror);
@@ -803,13 +943,18 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
     h->partials = nullptr;
     h->publish_next = false;
     h->published = false;
+    h->ticket = nullptr;
+    h->launch_mode = KFX_ICP_LAUNCH_TWO;
     if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(double) * kIcpEntries * (size_t)h->max_blocks);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->ticket, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemsetAsync(h->ticket, 0, sizeof(unsigned), stream());
     if (e == hipSuccess) e = hipHostMalloc((void**)&h->host_state, sizeof(IcpState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(h->state, 0, sizeof(IcpState), stream());
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
     if (e != hipSuccess) {
         (void)hipFree(h->state);
         (void)hipFree(h->partials);
+        (void)hipFree(h->ticket);
         if (h->host_state) (void)hipHostFree(h->host_state);
         delete h;
         return set_error(KFX_ERR_HIP, "kfx_icp_create: %s", hipGetErrorString(e));
@@ -824,6 +969,7 @@ int kfx_icp_destroy(kfx_icp* h) {
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
     (void)hipFree(h->state);
     (void)hipFree(h->partials);
+    (void)hipFree(h->ticket);
     (void)hipHostFree(h->host_state);
     delete h;
     return KFX_OK;
@@ -874,6 +1020,14 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
     }
     KFX_CHECK_LAUNCH();
     if (publish) h->published = true;
+    return KFX_OK;
+}
+
+int kfx_icp_set_launch_mode(kfx_icp* h, int mode) {
+    KFX_REQUIRE(h != nullptr, "kfx_icp_set_launch_mode: null handle");
+    KFX_REQUIRE(mode == KFX_ICP_LAUNCH_ONE || mode == KFX_ICP_LAUNCH_TWO, "kfx_icp_set_launch_mode: unknown mode %d",
+                mode);
+    h->launch_mode = mode;
     return KFX_OK;
 }
 

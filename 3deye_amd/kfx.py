@@ -136,6 +136,7 @@ class PipelineView(C.Structure):
 
 PLAIN, FUSED = 0, 1
 ICP_SUM_FP32_KAHAN, ICP_SUM_FP32, ICP_SUM_FP64 = 0, 1, 2
+ICP_LAUNCH_TWO, ICP_LAUNCH_ONE = 0, 1
 
 assert C.sizeof(Intrinsics) == 24 and C.sizeof(RoundBuffer) == 72 and C.sizeof(Matrix3f) == 36
 
@@ -265,6 +266,7 @@ _PROTOS = {
     "kfx_check_reciprocal": (_I, [_I, _I, C.POINTER(C.c_uint64)]),
     "kfx_icp_read": (_I, [_P, C.POINTER(IcpResult)]),
     "kfx_icp_publish_next": (_I, [_P]),
+    "kfx_icp_set_launch_mode": (_I, [_P, _I]),
     "kfx_icp_read_published": (_I, [_P, C.POINTER(IcpResult)]),
     "kfx_rgbd_create": (_I, [_I, _I, _I, C.POINTER(_P)]),
     "kfx_rgbd_destroy": (_I, [_P]),

@@ -9,25 +9,39 @@ scene (ground plane, 10 boxes, 10 spheres, sizes 0.1-0.5 m) along a
 quantisation, 2 % dropout; truncation 0.03 m; ICP 3 levels x 10/5/4; cyclic
 shift when the camera's target point (1.5 m along its optical axis) is more
 than 1.5 m from the cube centre, with the leaving slab extracted; the cube
-starts centred on the first frame's target point.  Depth frames are rendered
-on the host before timing and are resident in HBM when the timed region
-starts.
+starts centred on the first frame's target point, so the shifts fall at
+frames 77, 150, 225 and 297.  Depth frames are rendered on the host before
+timing and are resident in HBM when the timed region starts.
 
-One step = one frame through the pipeline (preprocess, ICP, integrate,
-raycast, shift).  The headline `value` is the learned pipeline (denoiser in
-place of the bilateral filter with the bilateral fallback, device-resident
-pose, FP32-compensated ICP sums, graph replay); the fused pipeline with the
-bilateral filter and the plain pipeline (the reference's control flow: a
-synchronisation after every kernel, the ICP normal equations downloaded and
-solved on the host every iteration) are timed on the same frames and reported
-beside it, with each trajectory's ATE against the ground truth.
+A run processes the walk's first PREROLL (60) frames untimed (they build the
+map), then W warmup frames, then times K frames: with the defaults W = 10,
+K = 230 the timed frames are 70-299 (four shifts); with W = 5, K = 20 they
+are 65-84 (the shift at frame 77).  One step = one frame through the
+pipeline (preprocess, ICP, integrate, raycast, shift with extraction).
 
-Prints one JSON line.
+The headline `value` is the learned pipeline (denoiser in place of the
+bilateral filter with the bilateral fallback, device-resident pose,
+FP32-compensated ICP sums, graph replay).  Timed on the same frames and
+reported beside it: the same with the ICP early exit; the fused pipeline
+with the bilateral filter, with and without early exit; and the plain
+pipeline (the reference's control flow: a synchronisation after every
+kernel, the ICP normal equations downloaded and solved on the host every
+iteration), each with its trajectory's ATE against the ground truth.
+Per-kernel tables: the plain pipeline's kernels and the learned pipeline's
+kernels (an eager replay of the same frames, identical results), each with
+algorithmic bytes or FLOPs per launch from HIP events around the launches.
+
+Prints one JSON line.  Other modes: --kernel-probe (the learned pipeline,
+eager, for hardware-counter runs), --summarise-pmc (turns two counter runs
+of the probe into profiles/integrate_traffic.json), --profile-learned N (the
+learned pipeline, graph replay, for a kernel trace).
 """
 import argparse
 import ctypes as C
+import glob
 import json
 import os
+import sqlite3
 import sys
 import time
 
@@ -39,13 +53,16 @@ import kfx  # noqa: E402
 import workload as wl  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 W, H = wl.W, wl.H
 N_VOX = wl.N_VOX
 VOL_M = wl.VOL_M
 TRUNC = wl.TRUNC
 SCENE_SEED = wl.SCENE_SEED
 N_FRAMES = 300
-cube_origin = wl.cube_origin
+PREROLL = wl.PREROLL_FRAMES
+DENOISER_FLOP_PER_PX = 10080
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "integrate_traffic.json")
 
 
 def workload_inputs(n_frames):
@@ -53,65 +70,74 @@ def workload_inputs(n_frames):
 
 
 def run_pipeline(cfg, poses, origin, ddepth, frame_bytes, warmup, steps, profile=False, denoiser=None):
-    """Frames [0, warmup) untimed, [warmup, warmup+steps) timed.  Returns
-    (elapsed seconds, frame infos, pipeline)."""
+    """Frames [0, PREROLL + warmup) untimed, then `steps` timed frames.
+    Returns (elapsed seconds, frame infos, pipeline, index of the first timed
+    frame)."""
     pipe = kfx.Pipeline(cfg, poses[0], origin)
     if denoiser is not None:
         pipe.set_denoiser(denoiser)
-    infos = []
-    for i in range(warmup):
-        infos.append(pipe.process(C.c_void_p(ddepth.ptr.value + i * frame_bytes)))
+    first = PREROLL + warmup
+    infos = [pipe.process(C.c_void_p(ddepth.ptr.value + i * frame_bytes)) for i in range(first)]
     kfx.synchronize()
     if profile:
         kfx.profile_reset()
         kfx.profile_enable(True)
     t0 = time.perf_counter()
-    for i in range(warmup, warmup + steps):
+    for i in range(first, first + steps):
         infos.append(pipe.process(C.c_void_p(ddepth.ptr.value + i * frame_bytes)))
     kfx.synchronize()
     elapsed = time.perf_counter() - t0
     if profile:
         kfx.profile_enable(False)
-    return elapsed, infos, pipe
+    return elapsed, infos, pipe, first
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "integrate_traffic.json")
-
-
-def measured_traffic():
-    """HBM bytes per integrate launch from the committed rocprofv3 --pmc runs
-    (FETCH_SIZE and WRITE_SIZE of the --kernel-probe workload, each scaled by
-    the factor that maps a 256 MiB copy kernel's counter reading to 256 MiB)."""
-    if not os.path.exists(TRAFFIC_FILE):
-        return None
-    with open(TRAFFIC_FILE) as f:
-        return json.load(f)
+def summary(elapsed, infos, pipe, first, steps, poses):
+    """frames/s of the timed frames, ATE over every frame processed, and what
+    the timed frames did (shifts, extracted points, fallbacks, ICP
+    iterations)."""
+    timed = infos[first:]
+    n = len(infos)
+    out = {"frames_per_s": steps / elapsed, "ms_per_frame": 1e3 * elapsed / steps,
+           "ate_m": kfx.ate_rmse([i.pose_world for i in infos], poses[:n]),
+           "frames_timed": [first, first + steps - 1],
+           "shifts_timed": int(sum(i.shifted for i in timed)),
+           "shift_frames_timed": [first + k for k, i in enumerate(timed) if i.shifted],
+           "extracted_points_timed": int(sum(i.extracted_points for i in timed)),
+           "fallbacks_timed": int(sum(i.fallback for i in timed)),
+           "icp_iterations_per_frame": float(np.mean([i.icp_iterations for i in timed]))}
+    pipe.close()
+    return out
 
 
 def count_updates(intr, ddepth, frame_bytes, infos, first, rb):
-    """N_upd of each integrated frame, from the pose it was integrated at."""
+    """N_upd of each frame from `first` on, from the pose it was integrated at
+    (0 for a frame the pipeline did not integrate)."""
     scaled = kfx.Buffer(W * H * 4)
     cs = kfx.cell_size(rb)
-    total = 0
+    out = []
     for k, info in enumerate(infos[first:], start=first):
         if not info.integrated:
+            out.append(0)
             continue
         R, t = info.pose_volume.matrices()
         n = C.c_uint64()
         kfx.call("kfx_integrate_count_updates", C.c_void_p(ddepth.ptr.value + k * frame_bytes), scaled.ptr,
                  C.byref(intr), C.byref(kfx.matrix3f(R.T)), C.byref(kfx.point3f(t)), TRUNC, C.byref(cs),
                  C.byref(rb), C.byref(n))
-        total += n.value
-    return total
+        out.append(n.value)
+    return out
 
 
-def count_raycast_samples(cfg, poses, origin, ddepth, frame_bytes, first, n):
-    """S (the raycast algorithm's voxel reads) of frames [first, n) of the
-    plain pipeline: an untimed replay of the same deterministic pipeline,
+def count_raycast_samples(cfg, poses, origin, ddepth, frame_bytes, first, n, denoiser=None):
+    """S (the raycast algorithm's voxel reads) averaged over frames [first, n)
+    of a pipeline: an untimed replay of the same deterministic pipeline,
     recounting each frame's raycast on its volume at its pose.  Frames
     followed by a shift are left out (their volume has moved on)."""
     intr = cfg.intr
     pipe = kfx.Pipeline(cfg, poses[0], origin)
+    if denoiser is not None:
+        pipe.set_denoiser(denoiser)
     vm, nm = kfx.Buffer(W * H * 12), kfx.Buffer(W * H * 12)
     total, frames = 0, 0
     for i in range(n):
@@ -130,17 +156,38 @@ def count_raycast_samples(cfg, poses, origin, ddepth, frame_bytes, first, n):
     return total / max(frames, 1), frames
 
 
-def kernel_table(n_upd_per_launch, samples_per_launch):
-    """Per kernel of the plain pipeline: launches and time from the HIP
-    events of the timed run, algorithmic bytes per launch (the formulas of
-    the roofline model: bilateral 4 B/px; pyramid level i 2 (WH/4^(i-1)) +
-    2 (WH/4^i); vertex map 14 B/px; normal map 24 B/px; resize 12 B per input
-    and output pixel; ICP sums per iteration 48 B/px + 108 B per workgroup
-    partial, their reduction 108 B per partial; scale_depth 6 B/px;
-    integrate 8 N_upd + 2 WH; raycast 4 S + 24 WH), and the achieved GB/s
-    against the HBM peak."""
+def level_sizes():
+    px = [(W >> l) * (H >> l) for l in range(3)]
+    return px
+
+
+def kernel_row(name, nbytes_per_launch=None, flop_per_launch=None):
+    """Launches, average duration from the HIP events of the profiled run,
+    and the achieved rate against the roofline of the kernel's bound."""
+    ms, cnt = kfx.profile_query(name)
+    if not cnt:
+        return None
+    avg_s = ms / 1e3 / cnt
+    row = {"launches": cnt, "avg_us": round(avg_s * 1e6, 2)}
+    if nbytes_per_launch is not None:
+        row["alg_bytes_per_launch"] = round(nbytes_per_launch)
+        row["GBs"] = round(nbytes_per_launch / avg_s / 1e9, 1)
+        row["frac_hbm_peak"] = round(nbytes_per_launch / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+    if flop_per_launch is not None:
+        row["alg_flop_per_launch"] = round(flop_per_launch)
+        row["TFLOPs"] = round(flop_per_launch / avg_s / 1e12, 2)
+        row["frac_mfma_bf16_peak"] = round(flop_per_launch / avg_s / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4)
+    return row
+
+
+def kernel_table_plain(n_upd_per_launch, samples_per_launch):
+    """The plain pipeline's kernels.  Algorithmic bytes per launch: bilateral
+    4 B/px; pyramid level i 2 (WH/4^(i-1)) + 2 (WH/4^i); vertex map 14 B/px;
+    normal map 24 B/px; resize 12 B per input and output pixel; ICP sums per
+    iteration 48 B/px + 108 B per workgroup partial, their reduction 108 B per
+    partial; scale_depth 6 B/px; integrate 8 N_upd + 2 WH; raycast 4 S + 24 WH."""
     lv = [(W >> l, H >> l) for l in range(3)]
-    px = [w * h for w, h in lv]
+    px = level_sizes()
     nblk = [((w + 31) // 32) * ((h + 7) // 8) for w, h in lv]
     iters = [10, 5, 4]
     per_frame = {
@@ -151,20 +198,69 @@ def kernel_table(n_upd_per_launch, samples_per_launch):
         "resize_map": (4, sum(2 * (12 * px[l - 1] + 12 * px[l]) for l in (1, 2))),
         "estimate_combined": (19, sum(iters[l] * (48 * px[l] + 108 * nblk[l]) for l in range(3))),
         "transform_estimation": (19, sum(iters[l] * (108 * nblk[l] + 108) for l in range(3))),
-        "scale_depth": (1, 6 * px[0]),
+        "depth_bound": (1, 6 * px[0]),
         "integrate": (1, 8 * n_upd_per_launch + 2 * px[0]),
         "raycast": (1, 4 * samples_per_launch + 24 * px[0]),
     }
     out = {}
     for name, (per, nbytes) in per_frame.items():
-        ms, cnt = kfx.profile_query(name)
-        if not cnt:
-            continue
-        b = nbytes / per
-        avg_s = ms / 1e3 / cnt
-        out[name] = {"launches": cnt, "avg_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": round(b),
-                     "GBs": round(b / avg_s / 1e9, 1), "frac_hbm_peak": round(b / avg_s / 1e9 / HBM_PEAK_GBS, 4)}
+        row = kernel_row(name, nbytes / per)
+        if row:
+            out[name] = row
     return out
+
+
+def kernel_table_learned(n_upd_per_launch, samples_per_launch):
+    """The learned pipeline's kernels (eager replay of the timed frames).
+    Algorithmic work per launch: denoiser 10,080 FLOP/px (bf16 MFMA hidden
+    layers, against 2.5 PFLOP/s dense) and 4 B/px; pyramid level i
+    2 (WH/4^(i-1)) + 2 (WH/4^i) B; vertex + normal maps of every level
+    26 B/px; ICP sums at level l 48 B/px + 216 B per workgroup (FP32-Kahan
+    partials), the solve 216 B per partial + the 600 B state; the gate 14 B/px
+    + 32 B per workgroup, its reduce 32 B per workgroup; the integration
+    pre-pass 2 + 4 + 4/3 B/px (depth in, scaled depth and pyramid out);
+    integrate 8 N_upd + 2 WH; raycast 4 S + 24 WH; model-map resize 24 B per
+    level-0 pixel read, 24 B per level-1 and level-2 pixel written."""
+    px = level_sizes()
+    nblk_icp = [(px[0] + 511) // 512, (px[1] + 255) // 256, (px[2] + 255) // 256]
+    nblk_gate = (px[0] + 255) // 256
+    rows = {
+        "denoise_forward": kernel_row("denoise_forward", 4 * px[0], DENOISER_FLOP_PER_PX * px[0]),
+        "pyr_down": kernel_row("pyr_down", sum(2 * px[i - 1] + 2 * px[i] for i in (1, 2)) / 2),
+        "maps_pyramid": kernel_row("maps_pyramid", sum(26 * p for p in px)),
+        "icp_begin_frame": kernel_row("icp_begin_frame"),
+        "depth_bound": kernel_row("depth_bound", (2 + 4 + 4 / 3) * px[0]),
+        "integrate": kernel_row("integrate", 8 * n_upd_per_launch + 2 * px[0]),
+        "raycast": kernel_row("raycast", 4 * samples_per_launch + 24 * px[0]),
+        "resize_pyramid": kernel_row("resize_pyramid", 24 * (px[0] + px[1] + px[2])),
+        "consistency": kernel_row("consistency", 14 * px[0] + 32 * nblk_gate),
+        "consistency_reduce": kernel_row("consistency_reduce", 32 * nblk_gate),
+    }
+    for l in range(3):
+        rows[f"icp_sums@{W >> l}"] = kernel_row(f"icp_sums@{W >> l}", 48 * px[l] + 216 * nblk_icp[l])
+    solve_bytes = sum(n_it * (216 * nb + 600) for n_it, nb in zip((10, 5, 4), nblk_icp)) / 19
+    rows["icp_solve"] = kernel_row("icp_solve", solve_bytes)
+    return {k: v for k, v in rows.items() if v}
+
+
+def measured_traffic(frames):
+    """HBM bytes per integrate launch over the given frames, from the
+    committed per-frame table of two rocprofv3 --pmc runs of --kernel-probe
+    (FETCH_SIZE and WRITE_SIZE of each integrate dispatch, scaled by the
+    factors that map a 256 MiB copy kernel's readings to 256 MiB), or None
+    when the table does not cover the frames."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    per = t.get("frames", {})
+    vals = [per[str(k)] for k in frames if str(k) in per]
+    if len(vals) != len(frames) or not vals:
+        return None
+    return {"traffic_bytes_per_launch": float(np.mean([v["fetch"] + v["write"] for v in vals])),
+            "fetch_bytes_per_launch": float(np.mean([v["fetch"] for v in vals])),
+            "write_bytes_per_launch": float(np.mean([v["write"] for v in vals])),
+            "source": os.path.relpath(TRAFFIC_FILE, ROOT) + " (" + t.get("command", "") + ")"}
 
 
 def cpu_baseline(intr, depths, poses, origin, frames=12):
@@ -267,13 +363,15 @@ def rgbd_odometry(intr, scene, poses, depths, pairs=60, warmup=5):
             "config": "640x480 walk frames 1-%d, 4 levels x 7/7/7/10, identity start" % (n - 1)}
 
 
-def kernel_probe(frames=60, first=20):
+def kernel_probe(frames=N_FRAMES):
     """Short, fixed workload for hardware-counter runs (rocprofv3 --pmc):
     three 256 MiB streaming copies (known traffic, for calibrating the
-    FETCH_SIZE / WRITE_SIZE counters), then `frames` integrations of the
-    512^3 workload at the ground-truth poses, each followed by its raycast.
-    Counters are read for the integrations from `first` on (a populated
-    volume, as in the timed benchmark)."""
+    FETCH_SIZE / WRITE_SIZE counters), then the learned pipeline of the
+    benchmark, eager (the same kernels and results as its graph replay), over
+    the walk's frames.  Prints, in order, the frame of every integration
+    dispatch with a device pose (frame 0 integrates at the host pose; a frame
+    that falls back to the bilateral filter dispatches twice, the first one
+    skipped on the device)."""
     intr, scene, poses, origin, depths = workload_inputs(frames)
     kfx.init(0)
     nbytes = 256 << 20
@@ -282,60 +380,101 @@ def kernel_probe(frames=60, first=20):
     for _ in range(3):
         kfx.call("kfx_copy_kernel", b.ptr, a.ptr, nbytes)
     kfx.synchronize()
-    rb = kfx.round_buffer(N_VOX, VOL_M)
-    cs = kfx.cell_size(rb)
-    vol = kfx.Buffer(rb.buffer_length * 4)
-    kfx.call("kfx_init_tsdf", vol.ptr, C.byref(rb))
+    a.free()
+    b.free()
     dd = kfx.Buffer.from_array(depths)
-    scaled = kfx.Buffer(W * H * 4)
-    vm, nm = kfx.Buffer(W * H * 12), kfx.Buffer(W * H * 12)
-    n_upd = []
+    dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params())
+    cfg = wl.pipeline_config(mode=kfx.FUSED, use_graph=0, denoiser=1)
+    pipe = kfx.Pipeline(cfg, poses[0], origin)
+    pipe.set_denoiser(dn)
+    dispatch_frames = []
     for i in range(frames):
-        R, t = poses[i].matrices()
-        tv = t - origin
-        dp = C.c_void_p(dd.ptr.value + i * depths[0].nbytes)
-        if i >= first:
-            n = C.c_uint64()
-            kfx.call("kfx_integrate_count_updates", dp, scaled.ptr, C.byref(intr), C.byref(kfx.matrix3f(R.T)),
-                     C.byref(kfx.point3f(tv)), TRUNC, C.byref(cs), C.byref(rb), C.byref(n))
-            n_upd.append(n.value)
-        kfx.call("kfx_integrate_tsdf", dp, scaled.ptr, vol.ptr, C.byref(intr), C.byref(kfx.matrix3f(R.T)),
-                 C.byref(kfx.point3f(tv)), TRUNC, C.byref(cs), C.byref(rb))
-        kfx.call("kfx_raycast", vol.ptr, vm.ptr, nm.ptr, C.byref(rb), C.byref(kfx.point3f(tv)),
-                 C.byref(kfx.matrix3f(R)), TRUNC, C.byref(intr), C.byref(cs))
+        info = pipe.process(C.c_void_p(dd.ptr.value + i * depths[0].nbytes))
+        if i > 0:
+            dispatch_frames += [i] * (2 if info.fallback else 1)
     kfx.synchronize()
-    n_avg = float(np.mean(n_upd))
-    print(json.dumps({"probe": "copy 3 x 256 MiB; integrate + raycast", "frames": frames, "counted_from": first,
-                      "copy_bytes_per_launch": nbytes, "n_upd_per_frame": n_avg,
-                      "integrate_alg_bytes_per_frame": 8.0 * n_avg + 2.0 * W * H}))
+    pipe.close()
+    dn.close()
+    print(json.dumps({"probe": "copy 3 x 256 MiB; learned pipeline (eager) over the walk", "frames": frames,
+                      "copy_bytes_per_launch": nbytes, "integrate_dispatch_frames": dispatch_frames}))
+
+
+def _pmc_values(db_dir, counter):
+    """(kernel name, counter value) per dispatch, in dispatch order, from a
+    rocprofv3 --pmc output directory."""
+    dbs = glob.glob(os.path.join(db_dir, "**", "*.db"), recursive=True)
+    if not dbs:
+        raise SystemExit(f"no rocprofv3 database under {db_dir}")
+    con = sqlite3.connect(dbs[0])
+    rows = con.execute("select dispatch_id, kernel_name, value from counters_collection where counter_name = ? "
+                       "order by dispatch_id", (counter,)).fetchall()
+    con.close()
+    agg = {}
+    for disp, name, value in rows:
+        k = agg.setdefault(disp, [name, 0.0])
+        k[1] += value
+    return [tuple(agg[d]) for d in sorted(agg)]
+
+
+def summarise_pmc(fetch_dir, write_dir, probe_log, out_json, command):
+    """Per-frame HBM traffic of the learned pipeline's integrate dispatches
+    from a FETCH_SIZE run and a WRITE_SIZE run of --kernel-probe, each counter
+    scaled by the factor that maps the 256 MiB copy kernel's reading to
+    256 MiB."""
+    with open(probe_log) as f:
+        probe = json.loads([ln for ln in f if ln.startswith("{")][-1])
+    frames = probe["integrate_dispatch_frames"]
+    copy_kib = probe["copy_bytes_per_launch"] / 1024.0
+    out = {"command": command, "frames": {}}
+    for label, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
+        vals = _pmc_values(d, counter)
+        copies = [v for n, v in vals if "copy16_kernel" in n]
+        scale = copy_kib / float(np.mean(copies)) if copies else 1.0
+        integ = [v for n, v in vals if "integrate_kernel<true>" in n]
+        if len(integ) != len(frames):
+            raise SystemExit(f"{counter}: {len(integ)} integrate dispatches, the probe made {len(frames)}")
+        out[f"{label}_calibration"] = scale
+        for fr, v in zip(frames, integ):
+            e = out["frames"].setdefault(str(fr), {"fetch": 0.0, "write": 0.0})
+            e[label] += v * 1024.0 * scale
+    with open(out_json, "w") as f:
+        json.dump(out, f, indent=0)
+    print(f"wrote {out_json}: {len(out['frames'])} frames; calibration fetch {out['fetch_calibration']:.4f} "
+          f"write {out['write_calibration']:.4f}")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=290)
+    ap.add_argument("--steps", type=int, default=230)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
-    ap.add_argument("--skip-learned", action="store_true", help="no denoiser run (headline = fused bilateral)")
     ap.add_argument("--skip-rgbd", action="store_true", help="no RGBD odometry measurement")
     ap.add_argument("--profile-learned", type=int, default=0, metavar="N",
-                    help="only the learned pipeline, 10 warmup and N timed frames, for a kernel trace "
-                         "(prints frames/s, no JSON benchmark line)")
+                    help="only the learned pipeline (graph replay): the pre-roll, 10 warmup and N timed frames, "
+                         "for a kernel trace (prints frames/s, no JSON benchmark line)")
     ap.add_argument("--kernel-probe", action="store_true",
-                    help="short fixed workload for hardware-counter runs (no JSON benchmark line)")
+                    help="the learned pipeline, eager, for hardware-counter runs (no JSON benchmark line)")
+    ap.add_argument("--summarise-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "PROBE_LOG"),
+                    help="write profiles/integrate_traffic.json from two --pmc runs of --kernel-probe")
+    ap.add_argument("--pmc-command", default="", help="the counter command, recorded with --summarise-pmc")
+    ap.add_argument("--traffic-out", default=TRAFFIC_FILE, help="where --summarise-pmc writes its table")
     args = ap.parse_args()
+    if args.summarise_pmc:
+        summarise_pmc(*args.summarise_pmc, args.traffic_out, args.pmc_command)
+        return
     if args.kernel_probe:
         kernel_probe()
         return
     if args.profile_learned:
         n_t = args.profile_learned
-        intr, scene, poses, origin, depths = workload_inputs(n_t + 10)
+        intr, scene, poses, origin, depths = workload_inputs(PREROLL + 10 + n_t)
         kfx.init(0)
         dd = kfx.Buffer.from_array(depths)
         dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params())
         cfg = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1)
-        el, _, pipe = run_pipeline(cfg, poses, origin, dd, depths[0].nbytes, 10, n_t, denoiser=dn)
+        el, _, pipe, _ = run_pipeline(cfg, poses, origin, dd, depths[0].nbytes, 10, n_t, denoiser=dn)
         pipe.close()
         print("learned pipeline: %d frames, %.1f frames/s" % (n_t, n_t / el))
         return
@@ -343,116 +482,114 @@ def main():
         print("bench.py measures one GPU only: the pipeline runs on a single MI355X "
               "(multi-GPU sharding is a design note, not built).", file=sys.stderr)
         sys.exit(2)
-    n = args.warmup + args.steps
-    if n > N_FRAMES or args.warmup < 1:
-        sys.exit(f"need warmup >= 1 and warmup + steps <= {N_FRAMES} (the trajectory length)")
+    n = PREROLL + args.warmup + args.steps
+    if n > N_FRAMES or args.warmup < 1 or args.steps < 1:
+        sys.exit(f"need warmup >= 1, steps >= 1 and {PREROLL} + warmup + steps <= {N_FRAMES} (the walk)")
 
     intr, scene, poses, origin, depths = workload_inputs(N_FRAMES)
     kfx.init(0)
     ddepth = kfx.Buffer.from_array(depths[:n])
     fb = depths[0].nbytes
+    first = PREROLL + args.warmup
 
+    dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params())
+    early = dict(converge_rotation=1e-5, converge_translation=1e-5)
     # Learned pipeline: headline.
-    learned = None
-    if not args.skip_learned:
-        dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params())
-        cfg_l = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1)
-        el_l, inf_l, pipe_l = run_pipeline(cfg_l, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=dn)
-        v_l = pipe_l.view()
-        pipe_l.close()
-        timed = inf_l[args.warmup:]
-        learned = {"frames_per_s": args.steps / el_l, "ms_per_frame": 1e3 * el_l / args.steps,
-                   "ate_m": kfx.ate_rmse([i.pose_world for i in inf_l], poses[:n]),
-                   "fallbacks": int(sum(i.fallback for i in timed)),
-                   "denoise_rms_mm_mean": float(np.mean([i.denoise_rms_mm for i in timed])),
-                   "shifts": v_l.shifts, "extracted_points": int(v_l.extracted_points)}
-        # Reported beside the headline, not as it: the same pipeline with the
-        # KinectFusion early exit (a level stops once an increment is below
-        # 1e-5 rad and 1e-5 m, the pose tolerance), at most 10/5/4 iterations.
-        cfg_e = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1, converge_rotation=1e-5,
-                                   converge_translation=1e-5)
-        el_e, inf_e, pipe_e = run_pipeline(cfg_e, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=dn)
-        pipe_e.close()
-        learned["early_exit_1e-5"] = {
-            "frames_per_s": args.steps / el_e,
-            "ate_m": kfx.ate_rmse([i.pose_world for i in inf_e], poses[:n]),
-            "icp_iterations_per_frame": float(np.mean([i.icp_iterations for i in inf_e[args.warmup:]]))}
-        dn.close()
-
-    # Fused pipeline with the bilateral filter.
+    cfg_l = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1)
+    run_l = run_pipeline(cfg_l, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=dn)
+    inf_l = run_l[1]
+    learned = summary(*run_l, args.steps, poses)
+    # Beside the headline, not as it: the KinectFusion early exit (a level
+    # stops once an increment is below 1e-5 rad and 1e-5 m, the pose
+    # tolerance), at most 10/5/4 iterations, for both filters.
+    cfg_le = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1, **early)
+    learned["early_exit_1e-5"] = summary(
+        *run_pipeline(cfg_le, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=dn), args.steps, poses)
     cfg_f = wl.pipeline_config(mode=kfx.FUSED, use_graph=1)
-    el_f, inf_f, pipe_f = run_pipeline(cfg_f, poses, origin, ddepth, fb, args.warmup, args.steps)
-    v_f = pipe_f.view()
-    pipe_f.close()
-    fused = {"frames_per_s": args.steps / el_f, "ms_per_frame": 1e3 * el_f / args.steps,
-             "ate_m": kfx.ate_rmse([i.pose_world for i in inf_f], poses[:n]), "shifts": v_f.shifts,
-             "extracted_points": int(v_f.extracted_points)}
+    fused = summary(*run_pipeline(cfg_f, poses, origin, ddepth, fb, args.warmup, args.steps), args.steps, poses)
+    cfg_fe = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, **early)
+    fused["early_exit_1e-5"] = summary(*run_pipeline(cfg_fe, poses, origin, ddepth, fb, args.warmup, args.steps),
+                                       args.steps, poses)
 
-    # Plain pipeline, with per-kernel HIP events for the integrate roofline.
-    plain = None
-    el_p, inf_p = None, None
+    # The learned pipeline again, eager, with HIP events around every launch:
+    # the per-kernel table and the integrate roofline of the headline.
+    cfg_lp = wl.pipeline_config(mode=kfx.FUSED, use_graph=0, denoiser=1)
+    el_lp, inf_lp, pipe_lp, _ = run_pipeline(cfg_lp, poses, origin, ddepth, fb, args.warmup, args.steps,
+                                             profile=True, denoiser=dn)
+    pipe_lp.close()
+    learned["eager_replay_identical"] = all(
+        a.pose_world.t[:] == b.pose_world.t[:] and a.pose_world.R[:] == b.pose_world.R[:]
+        for a, b in zip(inf_l, inf_lp))
+    int_ms, int_n = kfx.profile_query("integrate")
+    rb = kfx.round_buffer(N_VOX, VOL_M)
+    n_upd = count_updates(intr, ddepth, fb, inf_lp, first, rb)
+    s_avg, s_frames = count_raycast_samples(cfg_lp, poses, origin, ddepth, fb, first, n, denoiser=dn)
+    kern_l = kernel_table_learned(sum(n_upd) / int_n, s_avg)
+    kern_l["raycast"]["samples_per_launch"] = round(s_avg)
+    kern_l["raycast"]["samples_counted_frames"] = s_frames
+    alg_bytes = (8.0 * sum(n_upd) + 2.0 * W * H * int_n) / int_n
+    avg_s = int_ms / 1e3 / int_n
+    achieved = alg_bytes / avg_s / 1e9
+    traffic = measured_traffic(list(range(first, n)))
+    roofline = {"bound": "hbm", "kernel": "integrate (device pose, learned pipeline)", "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
+                "traffic_source": traffic["source"] if traffic else None,
+                "frames": [first, n - 1], "launches": int_n, "avg_launch_us": avg_s * 1e6,
+                "alg_bytes_per_launch": alg_bytes, "n_upd_per_launch": sum(n_upd) / int_n,
+                "full_sweep_equiv_GBs": 8.0 * N_VOX ** 3 / avg_s / 1e9}
+
+    # Plain pipeline, with per-kernel HIP events.
+    plain, kern_p = None, None
     if not args.skip_plain:
         cfg_p = wl.pipeline_config(mode=kfx.PLAIN)
-        el_p, inf_p, pipe_p = run_pipeline(cfg_p, poses, origin, ddepth, fb, args.warmup, args.steps, profile=True)
-        pipe_p.close()
-        ate_p = kfx.ate_rmse([i.pose_world for i in inf_p], poses[:n])
-        plain = {"frames_per_s": args.steps / el_p, "ms_per_frame": 1e3 * el_p / args.steps, "ate_m": ate_p}
-    int_ms, int_n = kfx.profile_query("integrate")
-
-    roofline = None
-    kern = None
-    if int_n:
-        rb = kfx.round_buffer(N_VOX, VOL_M)
-        n_upd = count_updates(intr, ddepth, fb, inf_p, args.warmup, rb)
-        launches = int_n
-        alg_bytes = (8.0 * n_upd + 2.0 * W * H * launches) / launches
-        avg_s = int_ms / 1e3 / launches
-        achieved = alg_bytes / avg_s / 1e9
-        traffic = measured_traffic()
-        roofline = {"bound": "hbm", "kernel": "integrate", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS,
-                    "traffic": traffic["traffic_bytes_per_launch"] if traffic else None,
-                    "traffic_source": traffic["source"] if traffic else None,
-                    "avg_launch_us": avg_s * 1e6,
-                    "alg_bytes_per_launch": alg_bytes, "n_upd_per_launch": n_upd / launches,
-                    "full_sweep_equiv_GBs": 8.0 * N_VOX ** 3 / avg_s / 1e9}
-        s_avg, s_frames = count_raycast_samples(cfg_p, poses, origin, ddepth, fb, args.warmup, n)
-        kern = kernel_table(n_upd / launches, s_avg)
-        kern["raycast"]["samples_per_launch"] = round(s_avg)
-        kern["raycast"]["samples_counted_frames"] = s_frames
+        el_p, inf_p, pipe_p, _ = run_pipeline(cfg_p, poses, origin, ddepth, fb, args.warmup, args.steps,
+                                              profile=True)
+        p_int_ms, p_int_n = kfx.profile_query("integrate")
+        p_upd = count_updates(intr, ddepth, fb, inf_p, first, rb)
+        ps_avg, ps_frames = count_raycast_samples(cfg_p, poses, origin, ddepth, fb, first, n)
+        kfx.profile_enable(False)
+        plain = summary(el_p, inf_p, pipe_p, first, args.steps, poses)
+        # kernel_table_plain reads the plain run's events, still recorded.
+        kern_p = kernel_table_plain(sum(p_upd) / max(p_int_n, 1), ps_avg)
+        kern_p["raycast"]["samples_per_launch"] = round(ps_avg)
+        kern_p["raycast"]["samples_counted_frames"] = ps_frames
+    dn.close()
 
     rgbd = None if args.skip_rgbd else rgbd_odometry(intr, scene, poses, depths)
     if rgbd is not None and not args.no_cpu_baseline:
         rgbd["cpu_baseline"] = rgbd_cpu(intr, scene, poses, depths)
     base = None if args.no_cpu_baseline else cpu_baseline(intr, depths, poses, origin)
-    head = learned if learned is not None else fused
-    el_head = 1e-3 * head["ms_per_frame"] * args.steps
     out = {
         "metric": "frames/s (512^3, 640x480) learned vs plain at <= plain ATE; integrate HBM GB/s",
-        "value": head["frames_per_s"],
+        "value": learned["frames_per_s"],
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * el_head / args.steps,
+        "ms_per_step": learned["ms_per_frame"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded analytic scene, Kinect noise model)",
-        "config": {"workload": "512^3 TSDF (6 mm), 640x480 depth, 300-frame 6 m random walk with cyclic shifts",
-                   "pipeline": ("learned: denoiser (bf16 MFMA) with bilateral fallback at 15 mm RMS, "
-                                if learned is not None else "fused: bilateral filter, ")
-                               + "device-resident pose, FP32-Kahan ICP workgroup sums with FP64 cross-workgroup "
-                                 "sum, Cholesky and pose update, HIP graph replay",
-                   "frames": N_FRAMES, "scene_seed": SCENE_SEED, "icp_iterations": [10, 5, 4]},
+        "config": {"workload": f"512^3 TSDF (6 mm), 640x480 depth, 300-frame 6 m random walk with cyclic shifts "
+                               f"at frames 77/150/225/297; timed frames {first}-{n - 1} after {PREROLL} pre-roll "
+                               f"and {args.warmup} warmup frames",
+                   "pipeline": "learned: denoiser (bf16 MFMA) with bilateral fallback at 15 mm RMS, device-resident "
+                               "pose, FP32-Kahan ICP workgroup sums with FP64 cross-workgroup sum, Cholesky and pose "
+                               "update, HIP graph replay",
+                   "frames_timed": [first, n - 1], "shifts_timed": learned["shifts_timed"],
+                   "extracted_points_timed": learned["extracted_points_timed"],
+                   "scene_seed": SCENE_SEED, "icp_iterations": [10, 5, 4]},
         "learned": learned,
         "fused": fused,
         "plain": plain,
-        "speedup_vs_plain": (head["frames_per_s"] / plain["frames_per_s"]) if plain else None,
-        "ate_le_plain": (head["ate_m"] <= plain["ate_m"]) if plain else None,
+        "speedup_vs_plain": (learned["frames_per_s"] / plain["frames_per_s"]) if plain else None,
+        "ate_le_plain": (learned["ate_m"] <= plain["ate_m"]) if plain else None,
         "roofline": roofline,
-        "kernels_plain": kern,
+        "kernels_learned": kern_l,
+        "kernels_plain": kern_p,
         "rgbd_odometry": rgbd,
         "cpu_baseline": base,
     }

@@ -295,9 +295,21 @@ int kfx_icp_destroy(kfx_icp* icp);
 int kfx_icp_reset(kfx_icp* icp, const kfx_pose* pose);
 /* Selects KFX_ICP_SUM_* for later iterations (host-side setting). */
 int kfx_icp_set_sum_mode(kfx_icp* icp, int mode);
-/* One iteration (two launches): FP32 Jacobian rows and products, workgroup
- * sums in the handle's KFX_ICP_SUM_* precision (fixed order), FP64
- * cross-workgroup sum in one workgroup, FP64 6x6 Cholesky
+/* How an iteration is launched (host-side setting; the states are
+ * bit-identical): KFX_ICP_LAUNCH_TWO (default) runs the cross-workgroup
+ * sum, solve and pose update as a second, one-workgroup launch after the
+ * sums; KFX_ICP_LAUNCH_ONE runs them in the sums launch, in the workgroup
+ * that finishes last.  One launch is the faster form for eager launches
+ * (320x240: 15.2 against 11.9 + 11.5 us, HIP events); under graph replay
+ * two launches are (the 512^3 walk: 1875 against 1751 frames/s).  No
+ * reference counterpart. */
+#define KFX_ICP_LAUNCH_TWO 0
+#define KFX_ICP_LAUNCH_ONE 1
+int kfx_icp_set_launch_mode(kfx_icp* icp, int mode);
+/* One iteration (one or two launches, kfx_icp_set_launch_mode): FP32
+ * Jacobian rows and products, workgroup sums in the handle's KFX_ICP_SUM_*
+ * precision (fixed order), FP64 cross-workgroup sum in one workgroup, FP64
+ * 6x6 Cholesky
  * solve and pose composition on the GPU.  A singular system leaves the pose
  * unchanged and records KFX_ERR_SINGULAR.  When level_tag >= 0 and an
  * iteration's increment is below (converge_rotation rad, converge_translation

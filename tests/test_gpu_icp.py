@@ -253,3 +253,60 @@ def test_read_published_tracks_what_was_enqueued(gpu, orc, pair):
     kfx.call("kfx_icp_translate", h, 0.01, 0.0, 0.0)
     assert kfx.lib.kfx_icp_read_published(h, C.byref(res)) != 0
     kfx.call("kfx_icp_destroy", h)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_one_launch_iteration_equals_two(gpu, orc, pair, mode, level):
+    # Five iterations from the same estimate with the solve in the last
+    # workgroup of the sums launch and with the solve as its own launch:
+    # the states (pose, normal equations, increment) are bit-identical in
+    # every sum mode, at 640x480 (two pixels per thread, 600 workgroups) and
+    # on the coarse levels.
+    kfx = gpu
+    intr0, maps0, (Rp, tp), _ = pair
+    intr, maps = intr0.level(level), ii.pyramid_maps(orc, maps0)[level]
+    bufs = _upload(kfx, maps)
+    out = []
+    for launch in (kfx.ICP_LAUNCH_ONE, kfx.ICP_LAUNCH_TWO):
+        h = _icp_handle(kfx, intr0.cols * intr0.rows)
+        kfx.call("kfx_icp_set_sum_mode", h, mode)
+        kfx.call("kfx_icp_set_launch_mode", h, launch)
+        kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+        for _ in range(5):
+            kfx.call("kfx_icp_iterate", h, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr,
+                     C.byref(kfx.matrix3f(Rp.T)), C.byref(kfx.point3f(tp)), C.byref(intr), ii.DIST_THR,
+                     ii.ANGLE_THR, -1, 0.0, 0.0)
+        res = kfx.IcpResult()
+        kfx.call("kfx_icp_read", h, C.byref(res))
+        out.append((np.array(res.pose.R[:]), np.array(res.pose.t[:]), np.array(res.A[:]), np.array(res.x[:]),
+                    res.iterations, res.status))
+        kfx.call("kfx_icp_destroy", h)
+    a, b = out
+    assert a[4] == b[4] == 5 and a[5] == b[5] == 0
+    for u, v in zip(a[:4], b[:4]):
+        assert np.array_equal(u, v)
+
+
+def test_one_launch_ticket_is_reset(gpu, orc, pair):
+    # Iterations on images of different sizes through one handle: each
+    # launch finds the ticket at zero (the last workgroup resets it), so the
+    # states still equal the two-launch form's.
+    kfx = gpu
+    intr0, maps0, (Rp, tp), _ = pair
+    pyr = ii.pyramid_maps(orc, maps0)
+    results = []
+    for launch in (kfx.ICP_LAUNCH_ONE, kfx.ICP_LAUNCH_TWO):
+        h = _icp_handle(kfx, intr0.cols * intr0.rows)
+        kfx.call("kfx_icp_set_launch_mode", h, launch)
+        kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+        for level in (2, 0, 1, 2, 0):
+            bufs = _upload(kfx, pyr[level])
+            kfx.call("kfx_icp_iterate", h, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr,
+                     C.byref(kfx.matrix3f(Rp.T)), C.byref(kfx.point3f(tp)), C.byref(intr0.level(level)),
+                     ii.DIST_THR, ii.ANGLE_THR, -1, 0.0, 0.0)
+        res = kfx.IcpResult()
+        kfx.call("kfx_icp_read", h, C.byref(res))
+        results.append((np.array(res.pose.R[:]), np.array(res.pose.t[:])))
+        kfx.call("kfx_icp_destroy", h)
+    assert np.array_equal(results[0][0], results[1][0]) and np.array_equal(results[0][1], results[1][1])
