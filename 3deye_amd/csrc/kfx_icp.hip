@@ -48,7 +48,8 @@ struct IcpState {
     int singular_count;
     int converged_tag;       // tag of the level that has converged (-1 none)
     int iterations;          // solves performed
-    int pad[2];
+    int fault;               // a barrier wait of the one-launch frame ran out: the state is not valid
+    int pad;
 };
 
 // Projective association of one pixel (current frame, camera -> volume by
@@ -399,48 +400,56 @@ enum IcpSumMode : int {
     kSumDouble = KFX_ICP_SUM_FP64,       // FP32 products summed in FP64
 };
 
-// Sum of 32 per-thread values over the workgroup in a fixed order: the
-// butterfly over each wave, then the four wave totals in wave order.  Thread
-// e < 27 returns entry e (the others return 0).
+// Sum of 32 per-thread values over a group of kIcpThreads threads (local
+// ids lt = 0 .. kIcpThreads - 1; the group is the workgroup, or a quarter of
+// a larger one) in a fixed order: the butterfly over each wave, then the
+// four wave totals in wave order.  Local thread e < 27 returns entry e (the
+// others return 0).  Every thread of the workgroup calls it (it holds a
+// workgroup barrier); wsum is the group's own.
 template <int kWaves = kIcpWaves, typename T>
-__device__ __forceinline__ T workgroup_sum27(T (&acc)[32], T (*wsum)[32]) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ __forceinline__ T group_sum27(T (&acc)[32], T (*wsum)[32], int lt) {
+    const int lane = lt & 63, wave = lt >> 6;
     const T wtot = wave_reduce_scatter32(acc, lane);
     if ((lane & 1) == 0) wsum[wave][lane >> 1] = wtot;
     __syncthreads();
     T s = 0;
-    if (threadIdx.x < kIcpEntries) {
-        s = wsum[0][threadIdx.x];
+    if (lt < kIcpEntries) {
+        s = wsum[0][lt];
 #pragma unroll
-        for (int w = 1; w < kWaves; ++w) s += wsum[w][threadIdx.x];
+        for (int w = 1; w < kWaves; ++w) s += wsum[w][lt];
     }
     return s;
 }
 
+template <int kWaves = kIcpWaves, typename T>
+__device__ __forceinline__ T workgroup_sum27(T (&acc)[32], T (*wsum)[32]) {
+    return group_sum27<kWaves>(acc, wsum, threadIdx.x);
+}
+
 // The same for compensated single-precision pairs.
-__device__ __forceinline__ float2 workgroup_sum27(float (&s)[32], float (&c)[32], float2 (*wsum)[32]) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ __forceinline__ float2 group_sum27(float (&s)[32], float (&c)[32], float2 (*wsum)[32], int lt) {
+    const int lane = lt & 63, wave = lt >> 6;
     const float2 wtot = wave_reduce_scatter32(s, c, lane);
     if ((lane & 1) == 0) wsum[wave][lane >> 1] = wtot;
     __syncthreads();
     float a = 0.0f, e = 0.0f;
-    if (threadIdx.x < kIcpEntries) {
-        a = wsum[0][threadIdx.x].x;
-        e = wsum[0][threadIdx.x].y;
+    if (lt < kIcpEntries) {
+        a = wsum[0][lt].x;
+        e = wsum[0][lt].y;
 #pragma unroll
-        for (int w = 1; w < kIcpWaves; ++w) pair_merge(a, e, wsum[w][threadIdx.x].x, wsum[w][threadIdx.x].y);
+        for (int w = 1; w < kIcpWaves; ++w) pair_merge(a, e, wsum[w][lt].x, wsum[w][lt].y);
     }
     return make_float2(a, e);
 }
 
-// Current-frame loads of a workgroup's P pixels (they do not depend on the
-// pose): thread t of workgroup g takes pixels g * kIcpThreads * P +
-// q * kIcpThreads + t (q < P).
+// Current-frame loads of block blk's P pixels per thread (they do not
+// depend on the pose): local thread lt takes pixels blk * kIcpThreads * P +
+// q * kIcpThreads + lt (q < P).
 template <int P>
 __device__ __forceinline__ void icp_load_current(const kfx_xyz* __restrict__ vcurr,
-                                                 const kfx_xyz* __restrict__ ncurr, int npix, kfx_xyz (&v)[P],
-                                                 kfx_xyz (&n)[P]) {
-    const int base = blockIdx.x * (kIcpThreads * P) + threadIdx.x;
+                                                 const kfx_xyz* __restrict__ ncurr, int npix, int blk, int lt,
+                                                 kfx_xyz (&v)[P], kfx_xyz (&n)[P]) {
+    const int base = blk * (kIcpThreads * P) + lt;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
         const int p = base + q * kIcpThreads;
@@ -450,14 +459,6 @@ __device__ __forceinline__ void icp_load_current(const kfx_xyz* __restrict__ vcu
     }
 }
 
-// The 27 sums of the workgroup's pixels at the pose (Rc, tc): the
-// projections of all P pixels, then the previous-frame gathers of all P
-// before any residual, so a wave waits on memory once more.  The
-// single-precision products of a pixel (icp_row's arithmetic) are
-// accumulated per thread (mode M), then over the workgroup in a fixed order.
-// Partials: [27][workgroup] float2 (sum, compensation) in the FP32 modes,
-// double in the FP64 mode; entry-major, so the cross-workgroup sum's loads of
-// one entry over consecutive workgroups coalesce.
 // Partial stores for a reader on another XCD within the same launch: a
 // relaxed atomic store at agent scope goes past the XCD's L2 to the device's
 // coherence point (no write-back of the L2 needed).
@@ -470,14 +471,25 @@ __device__ __forceinline__ uint64_t load_coherent(const void* src) {
                                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int P, int M, bool kCoherent = false>
-__device__ __forceinline__ void icp_block_sums(const kfx_xyz (&v)[P], const kfx_xyz (&n)[P],
-                                               const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev,
-                                               const IcpFrame& f, const float* Rc, const float* tc,
-                                               void* __restrict__ partials, double (*wsum_d)[32],
-                                               float2 (*wsum_f)[32]) {
+// (sum, compensation) of an entry in the FP32 modes (compensation 0 in the
+// plain mode), its sum in the FP64 mode.
+template <int M>
+using IcpGroupSum = std::conditional_t<M == kSumDouble, double, float2>;
+
+// The 27 sums of block blk's pixels by a group of kIcpThreads threads (local
+// id lt) at the pose (Rc, tc): the projections of all P pixels, then the
+// previous-frame gathers of all P before any residual, so a wave waits on
+// memory once more.  The single-precision products of a pixel (icp_row's
+// arithmetic) are accumulated per thread (mode M), then over the group in a
+// fixed order; local thread e < 27 returns entry e.
+template <int P, int M>
+__device__ __forceinline__ IcpGroupSum<M> icp_group_sums(const kfx_xyz (&v)[P], const kfx_xyz (&n)[P],
+                                                         const kfx_xyz* __restrict__ vprev,
+                                                         const kfx_xyz* __restrict__ nprev, const IcpFrame& f,
+                                                         const float* Rc, const float* tc, int blk, int lt,
+                                                         double (*wsum_d)[32], float2 (*wsum_f)[32]) {
     const int npix = f.cols * f.rows;
-    const int base = blockIdx.x * (kIcpThreads * P) + threadIdx.x;
+    const int base = blk * (kIcpThreads * P) + lt;
     kfx_xyz vp[P], np[P];
     float g[P][3];
     bool ok[P];
@@ -508,12 +520,7 @@ __device__ __forceinline__ void icp_block_sums(const kfx_xyz (&v)[P], const kfx_
 #pragma unroll
                 for (int jj = i; jj < 7; ++jj, ++k) acc[k] += (double)(rows[q][i] * rows[q][jj]);
         }
-        const double s = workgroup_sum27(acc, wsum_d);
-        if (threadIdx.x < kIcpEntries) {
-            double* dst = static_cast<double*>(partials) + threadIdx.x * gridDim.x + blockIdx.x;
-            if constexpr (kCoherent) store_coherent(dst, __double_as_longlong(s));
-            else *dst = s;
-        }
+        return group_sum27(acc, wsum_d, lt);
     } else {
         float sum[32], comp[32];
 #pragma unroll
@@ -530,10 +537,29 @@ __device__ __forceinline__ void icp_block_sums(const kfx_xyz (&v)[P], const kfx_
                     else sum[k] += prod;
                 }
         }
-        float2 r;
-        if constexpr (M == kSumKahan) r = workgroup_sum27(sum, comp, wsum_f);
-        else r = make_float2(workgroup_sum27(sum, reinterpret_cast<float (*)[32]>(wsum_f)), 0.0f);
-        if (threadIdx.x < kIcpEntries) {
+        if constexpr (M == kSumKahan) return group_sum27(sum, comp, wsum_f, lt);
+        else return make_float2(group_sum27(sum, reinterpret_cast<float (*)[32]>(wsum_f), lt), 0.0f);
+    }
+}
+
+// The workgroup's sums (icp_group_sums of block blockIdx.x) stored as the
+// partials: [27][workgroup] float2 (sum, compensation) in the FP32 modes,
+// double in the FP64 mode; entry-major, so the cross-workgroup sum's loads
+// of one entry over consecutive workgroups coalesce.
+template <int P, int M, bool kCoherent = false>
+__device__ __forceinline__ void icp_block_sums(const kfx_xyz (&v)[P], const kfx_xyz (&n)[P],
+                                               const kfx_xyz* __restrict__ vprev, const kfx_xyz* __restrict__ nprev,
+                                               const IcpFrame& f, const float* Rc, const float* tc,
+                                               void* __restrict__ partials, double (*wsum_d)[32],
+                                               float2 (*wsum_f)[32]) {
+    const IcpGroupSum<M> r =
+        icp_group_sums<P, M>(v, n, vprev, nprev, f, Rc, tc, blockIdx.x, threadIdx.x, wsum_d, wsum_f);
+    if (threadIdx.x < kIcpEntries) {
+        if constexpr (M == kSumDouble) {
+            double* dst = static_cast<double*>(partials) + threadIdx.x * gridDim.x + blockIdx.x;
+            if constexpr (kCoherent) store_coherent(dst, __double_as_longlong(r));
+            else *dst = r;
+        } else {
             float2* dst = static_cast<float2*>(partials) + threadIdx.x * gridDim.x + blockIdx.x;
             if constexpr (kCoherent)
                 store_coherent(dst, ((uint64_t)__float_as_uint(r.y) << 32) | __float_as_uint(r.x));
@@ -563,7 +589,7 @@ icp_sums_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
     // The current-frame loads do not depend on the state: they are issued
     // before the convergence test and the pose reads.
     kfx_xyz v[P], n[P];
-    icp_load_current<P>(vcurr, ncurr, f.cols * f.rows, v, n);
+    icp_load_current<P>(vcurr, ncurr, f.cols * f.rows, blockIdx.x, threadIdx.x, v, n);
     if (tag >= 0 && st->converged_tag == tag) return;  // uniform across the grid
     icp_prev_from_state(f, st);
     float Rc[9], tc[3];
@@ -751,7 +777,7 @@ icp_iter_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
     __shared__ double total[32];
     __shared__ unsigned my_ticket;
     kfx_xyz v[P], n[P];
-    icp_load_current<P>(vcurr, ncurr, f.cols * f.rows, v, n);
+    icp_load_current<P>(vcurr, ncurr, f.cols * f.rows, blockIdx.x, threadIdx.x, v, n);
     if (tag >= 0 && st->converged_tag == tag) {  // uniform across the grid
         if (mirror != nullptr && blockIdx.x == 0 && threadIdx.x == 0) icp_publish(st, mirror);
         return;
@@ -781,6 +807,223 @@ icp_iter_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
     }
 }
 
+// ------------------------------------------------------------------------
+// A frame's ICP in one launch.
+//
+// kFrameBlocks workgroups, all resident at once (the host checks the
+// occupancy first), run every iteration of every level.  Workgroup w of
+// kFrameGroups groups of kIcpThreads threads computes the sums of the
+// two-launch form's workgroups (blocks) w, w + kFrameBlocks, ... (its groups
+// take them in parallel, in rounds) and adds their totals in double
+// precision in that order: exactly what thread w of icp_solve_kernel's
+// cross-workgroup sum adds.  It stores that value (27 entries, agent-scope
+// stores, so they pass the XCD's L2).  Thread t of every workgroup's first
+// group then polls workgroup t's 27 entries until none holds the empty
+// marker; the workgroup runs the solve's fixed-order workgroup sum,
+// Cholesky solve and pose update itself (identical in every workgroup) and
+// goes on to the next iteration.  The states equal the begin-frame and
+// two-launch iterations' bit for bit.
+//
+// The entries are their own barrier (no counter, whose atomics serialise
+// at the coherence point, and no flag behind them, which would cost a
+// second round trip there).  Iterations use three buffers in turn,
+// numbered on from one launch to the next (the epoch, read by every
+// workgroup before its first store and advanced by workgroup 0 after its
+// last read).  In iteration k a workgroup first empties its entries of
+// buffer k + 1 (they hold iteration k - 2's values, which every workgroup
+// has read: each has written iteration k - 1's, which this one has seen),
+// and waits for those stores to complete before it stores iteration k's
+// values: whoever sees iteration k's values of a workgroup then finds its
+// entries of buffer k + 1 empty until iteration k + 1 fills them.  The
+// buffer of a launch's first iteration was emptied by the last iteration
+// of the launch before (or at creation).
+// ------------------------------------------------------------------------
+constexpr int kFrameGroups = 3;                                // three blocks of 512 pixels per round
+constexpr int kFrameThreads = kFrameGroups * kIcpThreads;
+constexpr int kFrameBlocks = kIcpThreads;                      // one workgroup per cross-sum thread
+constexpr int kFrameBuffers = 3;
+constexpr unsigned kPollLimit = 1u << 21;                      // about 0.1 s of polls
+// Empty-entry marker: a NaN whose payload no arithmetic produces (the
+// generated NaN is 0x7ff8000000000000, and a propagated single-precision
+// NaN has the low 29 bits clear).
+constexpr uint64_t kEmptyEntry = 0xfff4f1a600000001ull;
+
+struct IcpFrameLevel {
+    const kfx_xyz *vc, *nc, *vp, *np;
+    IcpFrame f;      // previous pose taken from the state
+    int iterations;
+    int nblk;        // the two-launch form's workgroups at this level
+    int two;         // two pixels per thread
+};
+
+struct IcpFrameArgs {
+    IcpFrameLevel lvl[KFX_ICP_LEVELS];
+    IcpState* st;
+    uint64_t* entries;   // [kFrameBuffers][27][kFrameBlocks] workgroup values (double bits) or kEmptyEntry
+    unsigned* epoch;     // iterations run by earlier launches (buffer numbering)
+    IcpState* mirror;    // pinned host copy of the final state, or null
+    int begin;           // start of a frame (icp_begin_frame_kernel's update first)
+    int tagged;          // level l's iterations carry tag l (early exit)
+    double conv_rot, conv_trans;
+};
+
+template <int M>
+__device__ __forceinline__ double group_total(const IcpGroupSum<M>& g) {
+    if constexpr (M == kSumDouble) return g;
+    else return (double)g.x + (double)g.y;
+}
+
+// State copy by 8-byte words, spread over the workgroup.
+__device__ __forceinline__ void copy_state(IcpState* dst, const IcpState* src) {
+    static_assert(sizeof(IcpState) % 8 == 0, "state copy by 8-byte words");
+    const uint2* s = reinterpret_cast<const uint2*>(src);
+    uint2* d = reinterpret_cast<uint2*>(dst);
+    for (int i = threadIdx.x; i < (int)(sizeof(IcpState) / 8); i += blockDim.x) d[i] = s[i];
+}
+
+template <int M>
+__global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a) {
+    __shared__ double wsum_d[kFrameGroups][kIcpWaves][32];
+    __shared__ float2 wsum_f[kFrameGroups][kIcpWaves][32];
+    __shared__ IcpGroupSum<M> red[kFrameGroups][32];
+    __shared__ double wsum_x[kIcpWaves][32];
+    __shared__ double total[32];
+    __shared__ IcpState s;
+    __shared__ int fault;
+    const int grp = threadIdx.x / kIcpThreads, lt = threadIdx.x % kIcpThreads;
+    const int w = blockIdx.x;
+    const unsigned epoch = __hip_atomic_load(a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    copy_state(&s, a.st);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (a.begin) {
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) s.Rpi[3 * r + c] = (float)s.R[3 * c + r];
+            for (int i = 0; i < 3; ++i) s.tp[i] = (float)s.t[i];
+            s.status = KFX_OK;
+            s.singular_count = 0;
+            s.converged_tag = -1;
+            s.iterations = 0;
+        }
+        fault = 0;
+    }
+    __syncthreads();
+    unsigned k = 0;  // iterations so far in this launch
+    const int per_round = kFrameBlocks * kFrameGroups;
+    auto buffer = [&](unsigned i) { return a.entries + (size_t)((epoch + i) % kFrameBuffers) * (kIcpEntries * kFrameBlocks); };
+    for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l) {
+        const IcpFrameLevel& L = a.lvl[l];
+        const int tag = a.tagged ? l : -1;
+        const int npix = L.f.cols * L.f.rows;
+        // With one round (every block of the level in one pass of the
+        // groups) a group's current-frame pixels stay in registers for all
+        // of the level's iterations.
+        const bool one_round = L.nblk <= per_round;
+        kfx_xyz v[2], n[2];
+        auto load = [&](int blk) {
+            if (L.two) icp_load_current<2>(L.vc, L.nc, npix, blk, lt, v, n);
+            else icp_load_current<1>(L.vc, L.nc, npix, blk, lt, reinterpret_cast<kfx_xyz(&)[1]>(v),
+                                     reinterpret_cast<kfx_xyz(&)[1]>(n));
+        };
+        if (one_round && L.iterations > 0) load(w + kFrameBlocks * grp);
+        for (int it = 0; it < L.iterations; ++it) {
+            if (tag >= 0 && s.converged_tag == tag) break;  // the same in every workgroup
+            uint64_t* cur = buffer(k);
+            if (threadIdx.x < kIcpEntries)
+                store_coherent(&buffer(k + 1)[threadIdx.x * kFrameBlocks + w], kEmptyEntry);
+            IcpFrame f = L.f;
+            icp_prev_from_state(f, &s);
+            float Rc[9], tc[3];
+            for (int i = 0; i < 9; ++i) Rc[i] = s.Rf[i];
+            for (int i = 0; i < 3; ++i) tc[i] = s.tf[i];
+            double acc = 0.0;  // threads 0..26: entry threadIdx.x over this workgroup's blocks
+            for (int r0 = 0; r0 < L.nblk; r0 += per_round) {
+                const int blk = w + r0 + kFrameBlocks * grp;
+                if (!one_round) load(blk);
+                const IcpGroupSum<M> g =
+                    L.two ? icp_group_sums<2, M>(v, n, L.vp, L.np, f, Rc, tc, blk, lt, wsum_d[grp], wsum_f[grp])
+                          : icp_group_sums<1, M>(reinterpret_cast<kfx_xyz(&)[1]>(v),
+                                                 reinterpret_cast<kfx_xyz(&)[1]>(n), L.vp, L.np, f, Rc, tc, blk,
+                                                 lt, wsum_d[grp], wsum_f[grp]);
+                if (lt < kIcpEntries) red[grp][lt] = g;
+                __syncthreads();
+                if (threadIdx.x < kIcpEntries) {
+#pragma unroll
+                    for (int q = 0; q < kFrameGroups; ++q)
+                        if (w + r0 + kFrameBlocks * q < L.nblk) acc += group_total<M>(red[q][threadIdx.x]);
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x < kIcpEntries) {
+                __builtin_amdgcn_s_waitcnt(kWaitVmemZero);  // the emptying stores above have completed
+                store_coherent(&cur[threadIdx.x * kFrameBlocks + w], (uint64_t)__double_as_longlong(acc));
+            }
+            // icp_solve_kernel's cross-workgroup sum: thread t of the first
+            // group holds workgroup t's values once they are all there, then
+            // the fixed-order sum.
+            if (grp == 0) {
+                uint64_t x[32];
+#pragma unroll
+                for (int e = 0; e < 32; ++e) x[e] = 0;  // +0.0
+                bool ready = false;
+                for (unsigned polls = 0; !ready; ++polls) {
+#pragma unroll
+                    for (int e = 0; e < kIcpEntries; ++e) x[e] = load_coherent(&cur[e * kFrameBlocks + lt]);
+                    ready = true;
+#pragma unroll
+                    for (int e = 0; e < kIcpEntries; ++e) ready = ready && x[e] != kEmptyEntry;
+                    if (!ready && polls + 1 == kPollLimit) {
+                        fault = 1;
+                        break;
+                    }
+                }
+                double xd[32];
+#pragma unroll
+                for (int e = 0; e < 32; ++e) xd[e] = __longlong_as_double((long long)x[e]);
+                const int lane = lt & 63, wave = lt >> 6;
+                const double wtot = wave_reduce_scatter32(xd, lane);
+                if ((lane & 1) == 0) wsum_x[wave][lane >> 1] = wtot;
+            }
+            __syncthreads();
+            if (threadIdx.x < kIcpEntries) {
+                double t = wsum_x[0][threadIdx.x];
+#pragma unroll
+                for (int q = 1; q < kIcpWaves; ++q) t += wsum_x[q][threadIdx.x];
+                total[threadIdx.x] = t;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double R0[9], t0[3];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) R0[i] = s.R[i];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) t0[i] = s.t[i];
+                solve_and_update(&s, total, R0, t0, s.iterations, tag, a.conv_rot, a.conv_trans);
+            }
+            __syncthreads();
+            ++k;
+        }
+    }
+    if (w == 0) {
+        if (threadIdx.x == 0) {
+            s.fault = fault;
+            // Every workgroup read the epoch before its first store, and
+            // workgroup 0 has read every workgroup's last values.
+            __hip_atomic_store(a.epoch, epoch + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        copy_state(a.st, &s);
+        if (a.mirror != nullptr) copy_state(a.mirror, &s);
+    }
+}
+
+// Every entry of the frame buffers empty (creation, reset).
+__global__ void icp_frame_empty_kernel(uint64_t* entries, unsigned* epoch) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kFrameBuffers * kIcpEntries * kFrameBlocks) entries[i] = kEmptyEntry;
+    if (i == 0) *epoch = 0;
+}
+
 // Start of a frame: the current estimate becomes the previous pose (as R^T,
 // t in single precision) and the per-frame counters reset.
 __global__ void icp_begin_frame_kernel(IcpState* st) {
@@ -792,6 +1035,7 @@ __global__ void icp_begin_frame_kernel(IcpState* st) {
     st->singular_count = 0;
     st->converged_tag = -1;
     st->iterations = 0;
+    st->fault = 0;
 }
 
 __global__ void icp_translate_kernel(IcpState* st, double dx, double dy, double dz) {
@@ -811,6 +1055,7 @@ __global__ void icp_reset_kernel(IcpState* st, kfx_pose pose) {
     st->singular_count = 0;
     st->converged_tag = -1;
     st->iterations = 0;
+    st->fault = 0;
 }
 
 IcpFrame make_frame(const kfx_matrix_3f* rprev_inv, const kfx_point_3f* tprev, const kfx_intrinsics* intr,
@@ -842,8 +1087,10 @@ struct kfx_icp {
     int max_blocks;
     bool publish_next;  // the next iteration's solve writes host_state
     bool published;     // a publishing solve was enqueued since the last reset or translate
-    unsigned* ticket;   // workgroups done in the current one-launch iteration (zero between launches)
+    unsigned* ticket;   // [0] one-launch iteration ticket (zero between launches), [1] frame buffer epoch
     int launch_mode;    // KFX_ICP_LAUNCH_*
+    uint64_t* frame_entries;  // [kFrameBuffers][27][kFrameBlocks] workgroup values of the one-launch frame
+    bool frame_resident;     // the device holds all kFrameBlocks workgroups of icp_frame_kernel at once
 };
 
 using namespace kfx;
@@ -878,6 +1125,25 @@ static void icp_iterate_launch(kfx_icp* h, const kfx_xyz* vc, const kfx_xyz* nc,
     ProfScope prof("icp_solve");
     icp_solve_kernel<M><<<1, kIcpThreads, 0, stream()>>>(h->state, h->partials, nblk, tag, conv_rot,
                                                                 conv_trans, mirror);
+}
+
+// Whether the device holds every workgroup of icp_frame_kernel at once (its
+// grid barrier needs them all resident): workgroups per CU at the kernel's
+// resources times the CUs.
+static bool frame_kernel_resident() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
+    int per_cu = 1 << 30;
+    const void* kernels[] = {reinterpret_cast<const void*>(&icp_frame_kernel<kSumKahan>),
+                             reinterpret_cast<const void*>(&icp_frame_kernel<kSumPlain>),
+                             reinterpret_cast<const void*>(&icp_frame_kernel<kSumDouble>)};
+    for (const void* k : kernels) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kFrameThreads, 0) != hipSuccess) return false;
+        per_cu = n < per_cu ? n : per_cu;
+    }
+    return (long long)per_cu * cus >= kFrameBlocks;
 }
 
 extern "C" {
@@ -944,16 +1210,26 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
     h->publish_next = false;
     h->published = false;
     h->ticket = nullptr;
-    h->launch_mode = KFX_ICP_LAUNCH_TWO;
+    h->launch_mode = KFX_ICP_LAUNCH_FRAME;
+    h->frame_entries = nullptr;
+    h->frame_resident = frame_kernel_resident();
     if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(double) * kIcpEntries * (size_t)h->max_blocks);
-    if (e == hipSuccess) e = hipMalloc((void**)&h->ticket, sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemsetAsync(h->ticket, 0, sizeof(unsigned), stream());
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&h->frame_entries, sizeof(uint64_t) * kFrameBuffers * kIcpEntries * kFrameBlocks);
+    if (e == hipSuccess) e = hipMalloc((void**)&h->ticket, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemsetAsync(h->ticket, 0, 2 * sizeof(unsigned), stream());
+    if (e == hipSuccess) {
+        icp_frame_empty_kernel<<<divup(kFrameBuffers * kIcpEntries * kFrameBlocks, 256), 256, 0, stream()>>>(
+            h->frame_entries, h->ticket + 1);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipHostMalloc((void**)&h->host_state, sizeof(IcpState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemsetAsync(h->state, 0, sizeof(IcpState), stream());
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
     if (e != hipSuccess) {
         (void)hipFree(h->state);
         (void)hipFree(h->partials);
+        (void)hipFree(h->frame_entries);
         (void)hipFree(h->ticket);
         if (h->host_state) (void)hipHostFree(h->host_state);
         delete h;
@@ -969,6 +1245,7 @@ int kfx_icp_destroy(kfx_icp* h) {
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
     (void)hipFree(h->state);
     (void)hipFree(h->partials);
+    (void)hipFree(h->frame_entries);
     (void)hipFree(h->ticket);
     (void)hipHostFree(h->host_state);
     delete h;
@@ -979,6 +1256,11 @@ int kfx_icp_reset(kfx_icp* h, const kfx_pose* pose) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(h && pose, "kfx_icp_reset: null pointer");
     icp_reset_kernel<<<1, 64, 0, stream()>>>(h->state, *pose);
+    KFX_CHECK_LAUNCH();
+    // The one-launch frame's buffers empty again (after a frame whose
+    // polls ran out they may not be).
+    icp_frame_empty_kernel<<<divup(kFrameBuffers * kIcpEntries * kFrameBlocks, 256), 256, 0, stream()>>>(
+        h->frame_entries, h->ticket + 1);
     KFX_CHECK_LAUNCH();
     h->published = false;
     return KFX_OK;
@@ -1025,9 +1307,90 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
 
 int kfx_icp_set_launch_mode(kfx_icp* h, int mode) {
     KFX_REQUIRE(h != nullptr, "kfx_icp_set_launch_mode: null handle");
-    KFX_REQUIRE(mode == KFX_ICP_LAUNCH_ONE || mode == KFX_ICP_LAUNCH_TWO, "kfx_icp_set_launch_mode: unknown mode %d",
-                mode);
+    KFX_REQUIRE(mode == KFX_ICP_LAUNCH_FRAME || mode == KFX_ICP_LAUNCH_ONE || mode == KFX_ICP_LAUNCH_TWO,
+                "kfx_icp_set_launch_mode: unknown mode %d", mode);
     h->launch_mode = mode;
+    return KFX_OK;
+}
+
+int kfx_icp_frame_resident(kfx_icp* h, int* resident) {
+    KFX_REQUIRE(h && resident, "kfx_icp_frame_resident: null pointer");
+    *resident = h->frame_resident ? 1 : 0;
+    return KFX_OK;
+}
+
+int kfx_icp_frame(kfx_icp* h, const kfx_icp_level* levels, int begin_frame, float distance_threshold,
+                  float angle_threshold, int tagged, double converge_rotation, double converge_translation) {
+    // A publish request applies to this call only, whether or not it
+    // passes its checks.
+    const bool publish = h != nullptr && h->publish_next;
+    if (h != nullptr) h->publish_next = false;
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(h && levels, "kfx_icp_frame: null pointer");
+    int last = -1;  // the finest level with iterations: its last one publishes
+    for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l) {
+        const kfx_icp_level& L = levels[l];
+        KFX_REQUIRE(L.iterations >= 0, "kfx_icp_frame: level %d: negative iteration count", l);
+        if (L.iterations == 0) continue;
+        last = l;
+        KFX_REQUIRE(L.vmap_current && L.nmap_current && L.vmap_previous && L.nmap_previous,
+                    "kfx_icp_frame: level %d: null map", l);
+        KFX_REQUIRE(L.intr.cols > 0 && L.intr.rows > 0, "kfx_icp_frame: level %d: empty image", l);
+        const unsigned npix = (unsigned)(L.intr.cols * L.intr.rows);
+        KFX_REQUIRE(divup(npix, (unsigned)kIcpThreads) <= (unsigned)h->max_blocks,
+                    "kfx_icp_frame: level %d: image of %u pixels exceeds the handle's capacity", l, npix);
+    }
+    if (h->launch_mode != KFX_ICP_LAUNCH_FRAME || !h->frame_resident) {
+        // The same frame as separate launches.
+        if (begin_frame) KFX_TRY(kfx_icp_begin_frame(h));
+        for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l)
+            for (int it = 0; it < levels[l].iterations; ++it) {
+                if (publish && l == last && it == levels[l].iterations - 1) h->publish_next = true;
+                KFX_TRY(kfx_icp_iterate(h, levels[l].vmap_current, levels[l].nmap_current, levels[l].vmap_previous,
+                                        levels[l].nmap_previous, nullptr, nullptr, &levels[l].intr,
+                                        distance_threshold, angle_threshold, tagged ? l : -1, converge_rotation,
+                                        converge_translation));
+            }
+        return KFX_OK;
+    }
+    IcpFrameArgs a;
+    for (int l = 0; l < KFX_ICP_LEVELS; ++l) {
+        const kfx_icp_level& L = levels[l];
+        IcpFrameLevel& d = a.lvl[l];
+        d.vc = L.vmap_current;
+        d.nc = L.nmap_current;
+        d.vp = L.vmap_previous;
+        d.np = L.nmap_previous;
+        d.iterations = L.iterations;
+        d.two = 0;
+        d.nblk = 0;
+        if (L.iterations > 0) {
+            d.f = make_frame(nullptr, nullptr, &L.intr, distance_threshold, angle_threshold);
+            const unsigned npix = (unsigned)(L.intr.cols * L.intr.rows);
+            d.two = npix >= kIcpTwoPixelMin ? 1 : 0;
+            d.nblk = (int)divup(npix, (unsigned)(kIcpThreads * (d.two ? 2 : 1)));
+        } else {
+            d.f = IcpFrame{};
+        }
+    }
+    a.st = h->state;
+    a.entries = h->frame_entries;
+    a.epoch = h->ticket + 1;
+    a.mirror = (publish && last >= 0) ? h->host_state : nullptr;
+    a.begin = begin_frame ? 1 : 0;
+    a.tagged = tagged ? 1 : 0;
+    a.conv_rot = converge_rotation;
+    a.conv_trans = converge_translation;
+    {
+        ProfScope prof("icp_frame");
+        switch (h->sum_mode) {
+            case KFX_ICP_SUM_FP32: icp_frame_kernel<kSumPlain><<<kFrameBlocks, kFrameThreads, 0, stream()>>>(a); break;
+            case KFX_ICP_SUM_FP64: icp_frame_kernel<kSumDouble><<<kFrameBlocks, kFrameThreads, 0, stream()>>>(a); break;
+            default: icp_frame_kernel<kSumKahan><<<kFrameBlocks, kFrameThreads, 0, stream()>>>(a); break;
+        }
+        KFX_CHECK_LAUNCH();
+    }
+    if (a.mirror != nullptr) h->published = true;
     return KFX_OK;
 }
 
@@ -1063,6 +1426,9 @@ int kfx_icp_read(kfx_icp* h, kfx_icp_result* out) {
     KFX_CHECK_HIP(hipMemcpyAsync(h->host_state, h->state, sizeof(IcpState), hipMemcpyDeviceToHost, stream()));
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
     const IcpState& s = *h->host_state;
+    if (s.fault != 0)
+        return set_error(KFX_ERR_HIP, "%s: a grid barrier wait of the one-launch frame ran out (the state is not valid)",
+                         __func__);
     for (int i = 0; i < 9; ++i) out->pose.R[i] = s.R[i];
     for (int i = 0; i < 3; ++i) out->pose.t[i] = s.t[i];
     for (int i = 0; i < 36; ++i) out->A[i] = s.A[i];
@@ -1087,6 +1453,9 @@ int kfx_icp_read_published(kfx_icp* h, kfx_icp_result* out) {
                 "kfx_icp_read_published: no publishing iteration was enqueued since the last reset or translate");
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
     const IcpState& s = *h->host_state;
+    if (s.fault != 0)
+        return set_error(KFX_ERR_HIP, "%s: a grid barrier wait of the one-launch frame ran out (the state is not valid)",
+                         __func__);
     for (int i = 0; i < 9; ++i) out->pose.R[i] = s.R[i];
     for (int i = 0; i < 3; ++i) out->pose.t[i] = s.t[i];
     for (int i = 0; i < 36; ++i) out->A[i] = s.A[i];

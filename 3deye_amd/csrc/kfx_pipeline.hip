@@ -229,26 +229,25 @@ int icp_plain(kfx_pipeline* p, kfx_frame_info* info) {
     return KFX_OK;
 }
 
-// Fused ICP from the state's previous pose.  Enqueues only.
+// Fused ICP from the state's previous pose (one launch where the device
+// allows it).  Enqueues only.  The last iteration publishes the state to
+// pinned host memory, which the host reads after the frame
+// (kfx_icp_read_published).  Level tags only with the convergence test on:
+// an untagged iteration skips the read of the convergence state.
 int icp_part(kfx_pipeline* p, bool begin) {
-    if (begin) KFX_TRY(kfx_icp_begin_frame(p->icp));
-    // Level tags only with the convergence test on: an untagged iteration
-    // skips the kernels' read of the convergence state (a memory round trip
-    // ahead of their loads).
-    const bool early_exit = p->cfg.converge_rotation > 0.0;
-    // The last iteration publishes the state to pinned host memory, which
-    // the host reads after the frame (kfx_icp_read_published).
-    int last_level = -1;
-    for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l)
-        if (p->cfg.icp_iterations[l] > 0) last_level = l;
-    for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l)
-        for (int it = 0; it < p->cfg.icp_iterations[l]; ++it) {
-            if (l == last_level && it == p->cfg.icp_iterations[l] - 1) KFX_TRY(kfx_icp_publish_next(p->icp));
-            KFX_TRY(kfx_icp_iterate(p->icp, p->vc[l], p->nc[l], p->vp[l], p->np[l], nullptr, nullptr, &p->lvl[l],
-                                    p->cfg.distance_threshold, p->cfg.angle_threshold, early_exit ? l : -1,
-                                    p->cfg.converge_rotation, p->cfg.converge_translation));
-        }
-    return KFX_OK;
+    kfx_icp_level lv[KFX_ICP_LEVELS];
+    for (int l = 0; l < KFX_ICP_LEVELS; ++l) {
+        lv[l].vmap_current = p->vc[l];
+        lv[l].nmap_current = p->nc[l];
+        lv[l].vmap_previous = p->vp[l];
+        lv[l].nmap_previous = p->np[l];
+        lv[l].intr = p->lvl[l];
+        lv[l].iterations = p->cfg.icp_iterations[l];
+    }
+    KFX_TRY(kfx_icp_publish_next(p->icp));
+    return kfx_icp_frame(p->icp, lv, begin ? 1 : 0, p->cfg.distance_threshold, p->cfg.angle_threshold,
+                         p->cfg.converge_rotation > 0.0 ? 1 : 0, p->cfg.converge_rotation,
+                         p->cfg.converge_translation);
 }
 
 // Integration and raycast at the device pose, skipped on the device when
@@ -436,6 +435,7 @@ int kfx_pipeline_default_config(kfx_pipeline_config* c) {
     c->extract_on_shift = 1;
     c->colors = 0;
     c->color_max_weight = KFX_DEFAULT_COLOR_MAX_WEIGHT;
+    c->icp_launch = KFX_ICP_LAUNCH_FRAME;
     return KFX_OK;
 }
 
@@ -528,6 +528,7 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     }
     int rc = kfx_icp_create((int)npix, &p->icp);
     if (!rc) rc = kfx_icp_set_sum_mode(p->icp, p->cfg.icp_sum_mode);
+    if (!rc) rc = kfx_icp_set_launch_mode(p->icp, p->cfg.icp_launch);
     if (!rc) rc = kfx_slicer_create(&rb, &p->slicer);
     if (!rc) rc = workspace_reserve(p->ws, p->cfg.intr.cols, p->cfg.intr.rows, p->cvol != nullptr);
     if (!rc) rc = workspace_reserve_volume(p->ws, rb.voxels_size);

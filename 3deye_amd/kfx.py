@@ -107,6 +107,11 @@ class Xyzi(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
 
 
+class IcpLevel(C.Structure):
+    _fields_ = [("vmap_current", C.c_void_p), ("nmap_current", C.c_void_p), ("vmap_previous", C.c_void_p),
+                ("nmap_previous", C.c_void_p), ("intr", Intrinsics), ("iterations", C.c_int32)]
+
+
 class PipelineConfig(C.Structure):
     _fields_ = [("intr", Intrinsics), ("voxels", Point3i), ("volume_size", Point3f), ("truncation", C.c_float),
                 ("sigma_space", C.c_float), ("sigma_color", C.c_float), ("distance_threshold", C.c_float),
@@ -115,7 +120,8 @@ class PipelineConfig(C.Structure):
                 ("shift_threshold", C.c_float), ("integrate_threshold", C.c_float),
                 ("converge_rotation", C.c_double), ("converge_translation", C.c_double),
                 ("extract_on_shift", C.c_int32), ("icp_sum_mode", C.c_int32), ("shift_target_distance", C.c_double),
-                ("colors", C.c_int32), ("color_max_weight", C.c_int32), ("extract_records", C.c_int32)]
+                ("colors", C.c_int32), ("color_max_weight", C.c_int32), ("extract_records", C.c_int32),
+                ("icp_launch", C.c_int32)]
 
 
 class FrameInfo(C.Structure):
@@ -136,7 +142,7 @@ class PipelineView(C.Structure):
 
 PLAIN, FUSED = 0, 1
 ICP_SUM_FP32_KAHAN, ICP_SUM_FP32, ICP_SUM_FP64 = 0, 1, 2
-ICP_LAUNCH_TWO, ICP_LAUNCH_ONE = 0, 1
+ICP_LAUNCH_FRAME, ICP_LAUNCH_TWO, ICP_LAUNCH_ONE = 0, 1, 2
 
 assert C.sizeof(Intrinsics) == 24 and C.sizeof(RoundBuffer) == 72 and C.sizeof(Matrix3f) == 36
 
@@ -276,6 +282,8 @@ _PROTOS = {
     "kfx_rgbd_estimate": (_I, [_P, C.POINTER(Intrinsics), C.POINTER(RgbdParams), C.POINTER(Pose),
                                C.POINTER(RgbdResult)]),
     "kfx_icp_begin_frame": (_I, [_P]),
+    "kfx_icp_frame": (_I, [_P, C.POINTER(IcpLevel), _I, C.c_float, C.c_float, _I, C.c_double, C.c_double]),
+    "kfx_icp_frame_resident": (_I, [_P, C.POINTER(_I)]),
     "kfx_icp_translate": (_I, [_P, C.c_double, C.c_double, C.c_double]),
     "kfx_icp_pose_device": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "kfx_determine_extract_step": (_I, [_I, C.POINTER(_I), C.POINTER(_I)]),

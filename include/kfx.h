@@ -295,16 +295,22 @@ int kfx_icp_destroy(kfx_icp* icp);
 int kfx_icp_reset(kfx_icp* icp, const kfx_pose* pose);
 /* Selects KFX_ICP_SUM_* for later iterations (host-side setting). */
 int kfx_icp_set_sum_mode(kfx_icp* icp, int mode);
-/* How an iteration is launched (host-side setting; the states are
- * bit-identical): KFX_ICP_LAUNCH_TWO (default) runs the cross-workgroup
- * sum, solve and pose update as a second, one-workgroup launch after the
- * sums; KFX_ICP_LAUNCH_ONE runs them in the sums launch, in the workgroup
- * that finishes last.  One launch is the faster form for eager launches
- * (320x240: 15.2 against 11.9 + 11.5 us, HIP events); under graph replay
- * two launches are (the 512^3 walk: 1875 against 1751 frames/s).  No
- * reference counterpart. */
-#define KFX_ICP_LAUNCH_TWO 0
-#define KFX_ICP_LAUNCH_ONE 1
+/* How iterations are launched (host-side setting; the states are
+ * bit-identical in every mode).  KFX_ICP_LAUNCH_FRAME (default):
+ * kfx_icp_frame runs a frame's iterations in one launch when the device
+ * holds all of that kernel's workgroups at once (kfx_icp_frame_resident),
+ * and kfx_icp_iterate runs as KFX_ICP_LAUNCH_TWO.  KFX_ICP_LAUNCH_TWO: an
+ * iteration is the sums launch, then the cross-workgroup sum, solve and
+ * pose update as a second, one-workgroup launch.  KFX_ICP_LAUNCH_ONE: the
+ * workgroup of the sums launch that finishes last does the second half
+ * (faster for eager launches: 320x240 15.2 against 11.9 + 11.5 us, HIP
+ * events; slower under graph replay: the 512^3 walk 1751 against 1875
+ * frames/s).  In the last two modes kfx_icp_frame enqueues the frame as
+ * kfx_icp_begin_frame and kfx_icp_iterate calls.  No reference
+ * counterpart. */
+#define KFX_ICP_LAUNCH_FRAME 0
+#define KFX_ICP_LAUNCH_TWO 1
+#define KFX_ICP_LAUNCH_ONE 2
 int kfx_icp_set_launch_mode(kfx_icp* icp, int mode);
 /* One iteration (one or two launches, kfx_icp_set_launch_mode): FP32
  * Jacobian rows and products, workgroup sums in the handle's KFX_ICP_SUM_*
@@ -325,6 +331,33 @@ int kfx_icp_iterate(kfx_icp* icp, const kfx_xyz* vmap_current, const kfx_xyz* nm
  * kfx_icp_begin_frame), so a frame's iterations need no host arguments. */
 /* Start a frame: the current estimate becomes the previous pose; counters reset. */
 int kfx_icp_begin_frame(kfx_icp* icp);
+/* One pyramid level of a frame's ICP: its maps, camera and iteration count. */
+typedef struct kfx_icp_level {
+    const kfx_xyz* vmap_current;
+    const kfx_xyz* nmap_current;
+    const kfx_xyz* vmap_previous;
+    const kfx_xyz* nmap_previous;
+    kfx_intrinsics intr;
+    int32_t iterations;
+} kfx_icp_level;
+/* A frame's ICP: with begin_frame, kfx_icp_begin_frame first; then
+ * levels[l].iterations iterations at each level l from KFX_ICP_LEVELS - 1
+ * down to 0, as kfx_icp_iterate with the previous pose from the state and
+ * level tag l when tagged (else -1).  The state equals that sequence's bit
+ * for bit.  In KFX_ICP_LAUNCH_FRAME mode on a device that holds all its
+ * workgroups at once it is one launch: 256 workgroups, each the sums of
+ * the two-launch form's workgroups w, w + 256, ..., joined by a grid
+ * barrier per iteration, after which every workgroup solves the same
+ * system; a barrier wait that runs out (workgroups not resident) makes the
+ * state invalid, which kfx_icp_read and kfx_icp_read_published report as
+ * KFX_ERR_HIP.  A pending kfx_icp_publish_next applies to the frame's last
+ * iteration.  The reference runs the frame as Estimate_Combined calls with
+ * a host solve per iteration (common/src/opencl_wrappers.cpp:364-438). */
+int kfx_icp_frame(kfx_icp* icp, const kfx_icp_level levels[KFX_ICP_LEVELS], int begin_frame,
+                  float distance_threshold, float angle_threshold, int tagged, double converge_rotation,
+                  double converge_translation);
+/* 1 when kfx_icp_frame runs as one launch on this device (KFX_ICP_LAUNCH_FRAME mode). */
+int kfx_icp_frame_resident(kfx_icp* icp, int* resident);
 /* Add (dx, dy, dz) to the estimate's translation (re-expression after a
  * cyclic shift of the volume). */
 int kfx_icp_translate(kfx_icp* icp, double dx, double dy, double dz);
@@ -527,6 +560,7 @@ typedef struct kfx_pipeline_config {
     int32_t colors;               /* 1: keep a colour volume; frames then come with RGB (kfx_pipeline_process_rgbd) */
     int32_t color_max_weight;     /* colour running-average weight cap, 1..255 (reference default 1) */
     int32_t extract_records;      /* 1: shifts extract kfx_slice_record (x, y, z, tsdf, weight); 0: XYZI points */
+    int32_t icp_launch;           /* fused mode: KFX_ICP_LAUNCH_* (0: a frame's ICP in one launch where possible) */
 } kfx_pipeline_config;
 
 typedef struct kfx_frame_info {

@@ -310,3 +310,109 @@ def test_one_launch_ticket_is_reset(gpu, orc, pair):
         results.append((np.array(res.pose.R[:]), np.array(res.pose.t[:])))
         kfx.call("kfx_icp_destroy", h)
     assert np.array_equal(results[0][0], results[1][0]) and np.array_equal(results[0][1], results[1][1])
+
+
+def _frame_levels(kfx, intr0, pyr_bufs, iters):
+    lv = (kfx.IcpLevel * 3)()
+    for l in range(3):
+        b = pyr_bufs[l]
+        lv[l] = kfx.IcpLevel(b[0].ptr.value, b[1].ptr.value, b[2].ptr.value, b[3].ptr.value, intr0.level(l), iters[l])
+    return lv
+
+
+def _state(kfx, h, published=False):
+    res = kfx.IcpResult()
+    kfx.call("kfx_icp_read_published" if published else "kfx_icp_read", h, C.byref(res))
+    return (np.array(res.pose.R[:]), np.array(res.pose.t[:]), np.array(res.A[:]), np.array(res.b[:]),
+            np.array(res.x[:]), res.iterations, res.status, res.singular_count, res.converged_tag)
+
+
+def _same_state(a, b):
+    for u, v in zip(a[:5], b[:5]):
+        assert np.array_equal(u, v)
+    assert a[5:] == b[5:]
+
+
+@pytest.fixture(scope="module")
+def pyr_bufs(gpu, orc, pair):
+    _, maps0, _, _ = pair
+    return [_upload(gpu, m) for m in ii.pyramid_maps(orc, maps0)]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("iters,tagged", [((10, 5, 4), 0), ((10, 5, 4), 1), ((3, 0, 2), 0)])
+@pytest.mark.parametrize("begin", [0, 1])
+def test_frame_one_launch_equals_iterations(gpu, pair, pyr_bufs, mode, iters, tagged, begin):
+    # A frame's ICP as one launch (256 resident workgroups, a grid barrier
+    # per iteration) and as begin-frame plus two-launch iterations: the
+    # states are bit-identical in every sum mode, with and without the early
+    # exit, with and without the begin-frame update, and the published copy
+    # equals the state.
+    kfx = gpu
+    intr0, _, (Rp, tp), _ = pair
+    lv = _frame_levels(kfx, intr0, pyr_bufs, iters)
+    conv = 1e-5 if tagged else 0.0
+    out = []
+    for launch in (kfx.ICP_LAUNCH_FRAME, kfx.ICP_LAUNCH_TWO):
+        h = _icp_handle(kfx, intr0.cols * intr0.rows)
+        resident = C.c_int()
+        kfx.call("kfx_icp_frame_resident", h, C.byref(resident))
+        if launch == kfx.ICP_LAUNCH_FRAME:
+            print(f"one-launch frame resident: {resident.value}")
+            assert resident.value == 1
+        kfx.call("kfx_icp_set_sum_mode", h, mode)
+        kfx.call("kfx_icp_set_launch_mode", h, launch)
+        kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+        kfx.call("kfx_icp_translate", h, 0.004, -0.003, 0.002)
+        kfx.call("kfx_icp_publish_next", h)
+        kfx.call("kfx_icp_frame", h, lv, begin, ii.DIST_THR, ii.ANGLE_THR, tagged, conv, conv)
+        st = _state(kfx, h)
+        _same_state(st, _state(kfx, h, published=True))
+        out.append(st)
+        kfx.call("kfx_icp_destroy", h)
+    a, b = out
+    _same_state(a, b)
+    assert a[6] == 0
+    if tagged:
+        assert a[5] < sum(iters)
+    else:
+        assert a[5] == sum(iters)
+
+
+def test_frame_one_launch_consecutive_frames(gpu, pair, pyr_bufs):
+    # Several frames through one handle, with different iteration counts:
+    # every launch finds the barrier counters at zero (the last workgroup
+    # out resets them), so the states still equal the separate launches'.
+    kfx = gpu
+    intr0, _, (Rp, tp), _ = pair
+    plans = [(10, 5, 4), (1, 1, 1), (0, 0, 3), (2, 0, 0), (10, 5, 4)]
+    out = []
+    for launch in (kfx.ICP_LAUNCH_FRAME, kfx.ICP_LAUNCH_TWO):
+        h = _icp_handle(kfx, intr0.cols * intr0.rows)
+        kfx.call("kfx_icp_set_launch_mode", h, launch)
+        kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, Rp, tp)))
+        states = []
+        for iters in plans:
+            kfx.call("kfx_icp_translate", h, 0.003, 0.0, -0.002)
+            kfx.call("kfx_icp_frame", h, _frame_levels(kfx, intr0, pyr_bufs, iters), 0, ii.DIST_THR, ii.ANGLE_THR,
+                     0, 0.0, 0.0)
+            states.append(_state(kfx, h))
+        out.append(states)
+        kfx.call("kfx_icp_destroy", h)
+    for a, b in zip(*out):
+        _same_state(a, b)
+
+
+def test_frame_argument_checks(gpu, pair, pyr_bufs):
+    kfx = gpu
+    intr0, _, _, _ = pair
+    h = _icp_handle(kfx, (intr0.cols * intr0.rows) // 4)
+    lv = _frame_levels(kfx, intr0, pyr_bufs, (10, 5, 4))
+    assert kfx.lib.kfx_icp_frame(h, lv, 1, ii.DIST_THR, ii.ANGLE_THR, 0, 0.0, 0.0) != 0
+    assert b"capacity" in kfx.lib.kfx_last_error()
+    lv = _frame_levels(kfx, intr0, pyr_bufs, (0, -1, 4))
+    assert kfx.lib.kfx_icp_frame(h, lv, 1, ii.DIST_THR, ii.ANGLE_THR, 0, 0.0, 0.0) != 0
+    lv = _frame_levels(kfx, intr0, pyr_bufs, (0, 0, 4))
+    lv[2].vmap_current = None
+    assert kfx.lib.kfx_icp_frame(h, lv, 1, ii.DIST_THR, ii.ANGLE_THR, 0, 0.0, 0.0) != 0
+    kfx.call("kfx_icp_destroy", h)

@@ -56,6 +56,23 @@ def test_fused_matches_plain_and_graph_is_exact(gpu):
         p.close()
 
 
+@pytest.mark.parametrize("early", [0.0, 1e-5])
+def test_one_launch_frame_icp_is_exact(gpu, early):
+    # The fused pipeline (graph replay) with each frame's ICP as one launch
+    # and as two launches per iteration: identical poses and iteration
+    # counts, with and without the early exit.
+    kfx = gpu
+    kw = dict(converge_rotation=early, converge_translation=early)
+    p1, i1, _ = _c1_run(kfx, 1, use_graph=1, frames=12, icp_launch=kfx.ICP_LAUNCH_FRAME, **kw)
+    p2, i2, _ = _c1_run(kfx, 1, use_graph=1, frames=12, icp_launch=kfx.ICP_LAUNCH_TWO, **kw)
+    for a, b in zip(i1, i2):
+        assert np.array_equal(np.array(a.pose_world.t[:]), np.array(b.pose_world.t[:]))
+        assert np.array_equal(np.array(a.pose_world.R[:]), np.array(b.pose_world.R[:]))
+        assert a.icp_iterations == b.icp_iterations
+    for p in (p1, p2):
+        p.close()
+
+
 def test_early_exit_reduces_iterations(gpu):
     kfx = gpu
     pipe, infos, gt = _c1_run(kfx, 1, frames=15, converge_rotation=1e-5, converge_translation=1e-5)
