@@ -816,32 +816,41 @@ icp_iter_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
 // two-launch form's workgroups (blocks) w, w + kFrameBlocks, ... (its groups
 // take them in parallel, in rounds) and adds their totals in double
 // precision in that order: exactly what thread w of icp_solve_kernel's
-// cross-workgroup sum adds.  It stores that value (27 entries, agent-scope
-// stores, so they pass the XCD's L2).  Thread t of every workgroup's first
-// group then polls workgroup t's 27 entries until none holds the empty
-// marker; the workgroup runs the solve's fixed-order workgroup sum,
-// Cholesky solve and pose update itself (identical in every workgroup) and
-// goes on to the next iteration.  The states equal the begin-frame and
-// two-launch iterations' bit for bit.
+// cross-workgroup sum adds.  The cross-workgroup sum then runs in two
+// stages over agent-scope stores and loads (they pass the XCDs' L2):
+//   1. every workgroup stores its 27 values; the first wave of workgroup q
+//      (q < 4), lane l holding workgroup 64 q + l's values, runs the
+//      butterfly of wave q of icp_solve_kernel's workgroup sum and stores
+//      the 27 wave totals;
+//   2. every workgroup reads the 4 x 27 wave totals and adds them in wave
+//      order, then runs the Cholesky solve and pose update itself
+//      (identical in every workgroup) and goes on to the next iteration.
+// The states equal the begin-frame and two-launch iterations' bit for bit.
 //
-// The entries are their own barrier (no counter, whose atomics serialise
-// at the coherence point, and no flag behind them, which would cost a
-// second round trip there).  Iterations use three buffers in turn,
-// numbered on from one launch to the next (the epoch, read by every
-// workgroup before its first store and advanced by workgroup 0 after its
-// last read).  In iteration k a workgroup first empties its entries of
-// buffer k + 1 (they hold iteration k - 2's values, which every workgroup
-// has read: each has written iteration k - 1's, which this one has seen),
-// and waits for those stores to complete before it stores iteration k's
-// values: whoever sees iteration k's values of a workgroup then finds its
-// entries of buffer k + 1 empty until iteration k + 1 fills them.  The
-// buffer of a launch's first iteration was emptied by the last iteration
-// of the launch before (or at creation).
+// The values are their own barrier: a reader polls them until none holds
+// the empty marker (no counter, whose atomics serialise at the coherence
+// point, and no flag behind the values, a second round trip there; the
+// two stages keep the polled bytes small).  Iterations use three buffers
+// of each stage in turn, numbered on from one launch to the next (the
+// epoch, read by every workgroup before its first store and advanced by
+// workgroup 0 after its last read).  In iteration k a writer first empties
+// its entries of buffer k + 1, which hold iteration k - 2's values: every
+// reader of those has finished with them, since this writer has seen
+// iteration k - 1's wave totals, which depend (through the leaders) on
+// every workgroup's iteration k - 1 values, each stored after its
+// iteration k - 2 reads.  The writer waits for the emptying stores to
+// complete before it stores iteration k's values, so a reader that has
+// seen those finds its entries of buffer k + 1 empty until iteration k + 1
+// fills them.  The buffers of a launch's first iteration were emptied by
+// the last iteration of the launch before (or at creation or reset).
 // ------------------------------------------------------------------------
 constexpr int kFrameGroups = 3;                                // three blocks of 512 pixels per round
 constexpr int kFrameThreads = kFrameGroups * kIcpThreads;
 constexpr int kFrameBlocks = kIcpThreads;                      // one workgroup per cross-sum thread
 constexpr int kFrameBuffers = 3;
+constexpr int kStage1Entries = kIcpEntries * kFrameBlocks;     // [27][kFrameBlocks] per buffer
+constexpr int kStage2Entries = kIcpWaves * 32;                 // [4 waves][32] per buffer
+constexpr int kFrameEntries = kFrameBuffers * (kStage1Entries + kStage2Entries);
 constexpr unsigned kPollLimit = 1u << 21;                      // about 0.1 s of polls
 // Empty-entry marker: a NaN whose payload no arithmetic produces (the
 // generated NaN is 0x7ff8000000000000, and a propagated single-precision
@@ -859,7 +868,8 @@ struct IcpFrameLevel {
 struct IcpFrameArgs {
     IcpFrameLevel lvl[KFX_ICP_LEVELS];
     IcpState* st;
-    uint64_t* entries;   // [kFrameBuffers][27][kFrameBlocks] workgroup values (double bits) or kEmptyEntry
+    uint64_t* entries;   // stage 1 [kFrameBuffers][27][kFrameBlocks], then stage 2 [kFrameBuffers][4][32]:
+                         // double bits or kEmptyEntry
     unsigned* epoch;     // iterations run by earlier launches (buffer numbering)
     IcpState* mirror;    // pinned host copy of the final state, or null
     int begin;           // start of a frame (icp_begin_frame_kernel's update first)
@@ -892,6 +902,7 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
     __shared__ int fault;
     const int grp = threadIdx.x / kIcpThreads, lt = threadIdx.x % kIcpThreads;
     const int w = blockIdx.x;
+    const bool leader = w < kIcpWaves;
     const unsigned epoch = __hip_atomic_load(a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     copy_state(&s, a.st);
     __syncthreads();
@@ -910,7 +921,10 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
     __syncthreads();
     unsigned k = 0;  // iterations so far in this launch
     const int per_round = kFrameBlocks * kFrameGroups;
-    auto buffer = [&](unsigned i) { return a.entries + (size_t)((epoch + i) % kFrameBuffers) * (kIcpEntries * kFrameBlocks); };
+    auto stage1 = [&](unsigned i) { return a.entries + (size_t)((epoch + i) % kFrameBuffers) * kStage1Entries; };
+    auto stage2 = [&](unsigned i) {
+        return a.entries + (size_t)kFrameBuffers * kStage1Entries + ((epoch + i) % kFrameBuffers) * kStage2Entries;
+    };
     for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l) {
         const IcpFrameLevel& L = a.lvl[l];
         const int tag = a.tagged ? l : -1;
@@ -928,9 +942,12 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
         if (one_round && L.iterations > 0) load(w + kFrameBlocks * grp);
         for (int it = 0; it < L.iterations; ++it) {
             if (tag >= 0 && s.converged_tag == tag) break;  // the same in every workgroup
-            uint64_t* cur = buffer(k);
-            if (threadIdx.x < kIcpEntries)
-                store_coherent(&buffer(k + 1)[threadIdx.x * kFrameBlocks + w], kEmptyEntry);
+            uint64_t* cur1 = stage1(k);
+            uint64_t* cur2 = stage2(k);
+            if (threadIdx.x < kIcpEntries) {
+                store_coherent(&stage1(k + 1)[threadIdx.x * kFrameBlocks + w], kEmptyEntry);
+                if (leader) store_coherent(&stage2(k + 1)[w * 32 + threadIdx.x], kEmptyEntry);
+            }
             IcpFrame f = L.f;
             icp_prev_from_state(f, &s);
             float Rc[9], tc[3];
@@ -956,19 +973,19 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
             }
             if (threadIdx.x < kIcpEntries) {
                 __builtin_amdgcn_s_waitcnt(kWaitVmemZero);  // the emptying stores above have completed
-                store_coherent(&cur[threadIdx.x * kFrameBlocks + w], (uint64_t)__double_as_longlong(acc));
+                store_coherent(&cur1[threadIdx.x * kFrameBlocks + w], (uint64_t)__double_as_longlong(acc));
             }
-            // icp_solve_kernel's cross-workgroup sum: thread t of the first
-            // group holds workgroup t's values once they are all there, then
-            // the fixed-order sum.
-            if (grp == 0) {
+            // Stage 1 (leaders' first wave): lane l polls workgroup
+            // 64 w + l's values, then the wave's butterfly.
+            if (leader && threadIdx.x < 64) {
+                const int src = w * 64 + threadIdx.x;
                 uint64_t x[32];
 #pragma unroll
                 for (int e = 0; e < 32; ++e) x[e] = 0;  // +0.0
                 bool ready = false;
                 for (unsigned polls = 0; !ready; ++polls) {
 #pragma unroll
-                    for (int e = 0; e < kIcpEntries; ++e) x[e] = load_coherent(&cur[e * kFrameBlocks + lt]);
+                    for (int e = 0; e < kIcpEntries; ++e) x[e] = load_coherent(&cur1[e * kFrameBlocks + src]);
                     ready = true;
 #pragma unroll
                     for (int e = 0; e < kIcpEntries; ++e) ready = ready && x[e] != kEmptyEntry;
@@ -980,9 +997,23 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
                 double xd[32];
 #pragma unroll
                 for (int e = 0; e < 32; ++e) xd[e] = __longlong_as_double((long long)x[e]);
-                const int lane = lt & 63, wave = lt >> 6;
-                const double wtot = wave_reduce_scatter32(xd, lane);
-                if ((lane & 1) == 0) wsum_x[wave][lane >> 1] = wtot;
+                const double wtot = wave_reduce_scatter32(xd, threadIdx.x);
+                if ((threadIdx.x & 1) == 0 && (threadIdx.x >> 1) < kIcpEntries)
+                    store_coherent(&cur2[w * 32 + (threadIdx.x >> 1)], (uint64_t)__double_as_longlong(wtot));
+            }
+            // Stage 2: thread q * 27 + e polls wave q's total of entry e.
+            if (threadIdx.x < kIcpWaves * kIcpEntries) {
+                const int q = threadIdx.x / kIcpEntries, e = threadIdx.x % kIcpEntries;
+                uint64_t x = load_coherent(&cur2[q * 32 + e]);
+                for (unsigned polls = 1; x == kEmptyEntry; ++polls) {
+                    if (polls == kPollLimit) {
+                        fault = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    x = load_coherent(&cur2[q * 32 + e]);
+                }
+                wsum_x[q][e] = __longlong_as_double((long long)x);
             }
             __syncthreads();
             if (threadIdx.x < kIcpEntries) {
@@ -1008,7 +1039,8 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
         if (threadIdx.x == 0) {
             s.fault = fault;
             // Every workgroup read the epoch before its first store, and
-            // workgroup 0 has read every workgroup's last values.
+            // the last wave totals (read by workgroup 0) depend on every
+            // workgroup's last values.
             __hip_atomic_store(a.epoch, epoch + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
@@ -1020,7 +1052,7 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
 // Every entry of the frame buffers empty (creation, reset).
 __global__ void icp_frame_empty_kernel(uint64_t* entries, unsigned* epoch) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < kFrameBuffers * kIcpEntries * kFrameBlocks) entries[i] = kEmptyEntry;
+    if (i < kFrameEntries) entries[i] = kEmptyEntry;
     if (i == 0) *epoch = 0;
 }
 
@@ -1089,7 +1121,7 @@ struct kfx_icp {
     bool published;     // a publishing solve was enqueued since the last reset or translate
     unsigned* ticket;   // [0] one-launch iteration ticket (zero between launches), [1] frame buffer epoch
     int launch_mode;    // KFX_ICP_LAUNCH_*
-    uint64_t* frame_entries;  // [kFrameBuffers][27][kFrameBlocks] workgroup values of the one-launch frame
+    uint64_t* frame_entries;  // the one-launch frame's stage 1 and stage 2 buffers (kFrameEntries words)
     bool frame_resident;     // the device holds all kFrameBlocks workgroups of icp_frame_kernel at once
 };
 
@@ -1215,11 +1247,11 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
     h->frame_resident = frame_kernel_resident();
     if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(double) * kIcpEntries * (size_t)h->max_blocks);
     if (e == hipSuccess)
-        e = hipMalloc((void**)&h->frame_entries, sizeof(uint64_t) * kFrameBuffers * kIcpEntries * kFrameBlocks);
+        e = hipMalloc((void**)&h->frame_entries, sizeof(uint64_t) * kFrameEntries);
     if (e == hipSuccess) e = hipMalloc((void**)&h->ticket, 2 * sizeof(unsigned));
     if (e == hipSuccess) e = hipMemsetAsync(h->ticket, 0, 2 * sizeof(unsigned), stream());
     if (e == hipSuccess) {
-        icp_frame_empty_kernel<<<divup(kFrameBuffers * kIcpEntries * kFrameBlocks, 256), 256, 0, stream()>>>(
+        icp_frame_empty_kernel<<<divup(kFrameEntries, 256), 256, 0, stream()>>>(
             h->frame_entries, h->ticket + 1);
         e = hipGetLastError();
     }
@@ -1259,7 +1291,7 @@ int kfx_icp_reset(kfx_icp* h, const kfx_pose* pose) {
     KFX_CHECK_LAUNCH();
     // The one-launch frame's buffers empty again (after a frame whose
     // polls ran out they may not be).
-    icp_frame_empty_kernel<<<divup(kFrameBuffers * kIcpEntries * kFrameBlocks, 256), 256, 0, stream()>>>(
+    icp_frame_empty_kernel<<<divup(kFrameEntries, 256), 256, 0, stream()>>>(
         h->frame_entries, h->ticket + 1);
     KFX_CHECK_LAUNCH();
     h->published = false;
