@@ -200,20 +200,84 @@ constexpr int kIcpWaves = kIcpThreads / 64;
 constexpr int kIcpEntries = 27;
 constexpr unsigned kIcpTwoPixelMin = 150000;  // images at least this large take two pixels per thread
 
-// Double-precision sums padded to 32 entries, spread over a wave by a
-// butterfly that halves the entries each step: after the step with xor offset
-// 32, 16, 8, 4, 2 a lane keeps the half selected by that bit of its lane id,
-// added to its partner's copy.  The last step (offset 1) adds the two lanes
-// left per entry.  Lane l ends with the wave total of entry l >> 1, in a fixed
-// association order.
+// Lane exchanges of the butterflies below.  Partner l ^ 32 and l ^ 16:
+// v_permlane32_swap / v_permlane16_swap of (a, b) return, in a lane whose
+// bit (5 or 4) is clear, (its own a, the partner's a), and where it is set
+// (the partner's b, its own b).  Partner l ^ 8, 4, 2, 1 (within rows of 16
+// lanes): DPP row_ror:8; quad_perm [3,2,1,0] then row_half_mirror
+// (l ^ 3 ^ 7); quad_perm [2,3,0,1]; quad_perm [1,0,3,2].  Every lane of the
+// wave takes part.
+template <int kXor>
+__device__ __forceinline__ unsigned dpp_xor_u32(unsigned v) {
+    int x = (int)v;
+    if constexpr (kXor == 8) {
+        x = __builtin_amdgcn_mov_dpp(x, 0x128, 0xf, 0xf, false);
+    } else if constexpr (kXor == 4) {
+        x = __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x1B, 0xf, 0xf, false), 0x141, 0xf, 0xf, false);
+    } else if constexpr (kXor == 2) {
+        x = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xf, 0xf, false);
+    } else {
+        static_assert(kXor == 1, "DPP exchange for xor 8, 4, 2, 1");
+        x = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xf, 0xf, false);
+    }
+    return (unsigned)x;
+}
+
+template <int kXor>
+__device__ __forceinline__ float dpp_xor(float v) {
+    return __uint_as_float(dpp_xor_u32<kXor>(__float_as_uint(v)));
+}
+template <int kXor>
+__device__ __forceinline__ double dpp_xor(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint64_t lo = dpp_xor_u32<kXor>((unsigned)b), hi = dpp_xor_u32<kXor>((unsigned)(b >> 32));
+    return __longlong_as_double((long long)((hi << 32) | lo));
+}
+
+// (kept, received) in either order for the partner l ^ (2 kHalf), kHalf =
+// 16 or 8, from a = the low-half value and b = the high-half value.
+template <int kHalf>
+__device__ __forceinline__ void swap_halves(float a, float b, float& x, float& y) {
+    static_assert(kHalf == 16 || kHalf == 8, "permlane swap distances");
+    const auto r = kHalf == 16 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false)
+                               : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    x = __uint_as_float(r[0]);
+    y = __uint_as_float(r[1]);
+}
+template <int kHalf>
+__device__ __forceinline__ void swap_halves(double a, double b, double& x, double& y) {
+    const uint64_t ua = (uint64_t)__double_as_longlong(a), ub = (uint64_t)__double_as_longlong(b);
+    float xl, yl, xh, yh;
+    swap_halves<kHalf>(__uint_as_float((unsigned)ua), __uint_as_float((unsigned)ub), xl, yl);
+    swap_halves<kHalf>(__uint_as_float((unsigned)(ua >> 32)), __uint_as_float((unsigned)(ub >> 32)), xh, yh);
+    x = __longlong_as_double((long long)(((uint64_t)__float_as_uint(xh) << 32) | __float_as_uint(xl)));
+    y = __longlong_as_double((long long)(((uint64_t)__float_as_uint(yh) << 32) | __float_as_uint(yl)));
+}
+
+// Sums padded to 32 entries, spread over a wave by a butterfly that halves
+// the entries each step: after the step with xor offset 32, 16, 8, 4, 2 a
+// lane keeps the half selected by that bit of its lane id, added to its
+// partner's copy.  The last step (offset 1) adds the two lanes left per
+// entry.  Lane l ends with the wave total of entry l >> 1, in a fixed
+// association order (each addition is commutative, so which lane of a pair
+// holds which operand does not matter).
 template <int kHalf, typename T>
 __device__ __forceinline__ void reduce_scatter_step(T (&v)[32], int lane) {
-    const bool upper = (lane & (2 * kHalf)) != 0;
+    if constexpr (kHalf >= 8) {
 #pragma unroll
-    for (int i = 0; i < kHalf; ++i) {
-        const T send = upper ? v[i] : v[i + kHalf];
-        const T keep = upper ? v[i + kHalf] : v[i];
-        v[i] = keep + __shfl_xor(send, 2 * kHalf);
+        for (int i = 0; i < kHalf; ++i) {
+            T x, y;
+            swap_halves<kHalf>(v[i], v[i + kHalf], x, y);
+            v[i] = x + y;
+        }
+    } else {
+        const bool upper = (lane & (2 * kHalf)) != 0;
+#pragma unroll
+        for (int i = 0; i < kHalf; ++i) {
+            const T send = upper ? v[i] : v[i + kHalf];
+            const T keep = upper ? v[i + kHalf] : v[i];
+            v[i] = keep + dpp_xor<2 * kHalf>(send);
+        }
     }
 }
 
@@ -224,7 +288,7 @@ __device__ __forceinline__ T wave_reduce_scatter32(T (&v)[32], int lane) {
     reduce_scatter_step<4>(v, lane);
     reduce_scatter_step<2>(v, lane);
     reduce_scatter_step<1>(v, lane);
-    return v[0] + __shfl_xor(v[0], 1);
+    return v[0] + dpp_xor<1>(v[0]);
 }
 
 // Error-free sum (Knuth TwoSum): s + e == a + b exactly.
@@ -253,16 +317,28 @@ __device__ __forceinline__ void pair_merge(float& s, float& c, float s2, float c
 // pairs: lane l ends with the wave total (s, c) of entry l >> 1.
 template <int kHalf>
 __device__ __forceinline__ void reduce_scatter_step(float (&s)[32], float (&c)[32], int lane) {
-    const bool upper = (lane & (2 * kHalf)) != 0;
+    if constexpr (kHalf >= 8) {
 #pragma unroll
-    for (int i = 0; i < kHalf; ++i) {
-        const float send_s = upper ? s[i] : s[i + kHalf];
-        const float send_c = upper ? c[i] : c[i + kHalf];
-        float keep_s = upper ? s[i + kHalf] : s[i];
-        float keep_c = upper ? c[i + kHalf] : c[i];
-        pair_merge(keep_s, keep_c, __shfl_xor(send_s, 2 * kHalf), __shfl_xor(send_c, 2 * kHalf));
-        s[i] = keep_s;
-        c[i] = keep_c;
+        for (int i = 0; i < kHalf; ++i) {
+            float xs, ys, xc, yc;
+            swap_halves<kHalf>(s[i], s[i + kHalf], xs, ys);
+            swap_halves<kHalf>(c[i], c[i + kHalf], xc, yc);
+            pair_merge(xs, xc, ys, yc);
+            s[i] = xs;
+            c[i] = xc;
+        }
+    } else {
+        const bool upper = (lane & (2 * kHalf)) != 0;
+#pragma unroll
+        for (int i = 0; i < kHalf; ++i) {
+            const float send_s = upper ? s[i] : s[i + kHalf];
+            const float send_c = upper ? c[i] : c[i + kHalf];
+            float keep_s = upper ? s[i + kHalf] : s[i];
+            float keep_c = upper ? c[i + kHalf] : c[i];
+            pair_merge(keep_s, keep_c, dpp_xor<2 * kHalf>(send_s), dpp_xor<2 * kHalf>(send_c));
+            s[i] = keep_s;
+            c[i] = keep_c;
+        }
     }
 }
 
@@ -273,7 +349,7 @@ __device__ __forceinline__ float2 wave_reduce_scatter32(float (&s)[32], float (&
     reduce_scatter_step<2>(s, c, lane);
     reduce_scatter_step<1>(s, c, lane);
     float a = s[0], e = c[0];
-    pair_merge(a, e, __shfl_xor(s[0], 1), __shfl_xor(c[0], 1));
+    pair_merge(a, e, dpp_xor<1>(s[0]), dpp_xor<1>(c[0]));
     return make_float2(a, e);
 }
 
