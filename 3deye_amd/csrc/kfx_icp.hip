@@ -584,6 +584,8 @@ __device__ __forceinline__ IcpGroupSum<M> icp_group_sums(const kfx_xyz (&v)[P], 
     float rows[P][7];
 #pragma unroll
     for (int q = 0; q < P; ++q) icp_residual(n[q], g[q], np[q], vp[q], Rc, f, ok[q], rows[q]);
+    // Accumulation from zero; the first pixel's step is written out: 0 + p
+    // is p + 0 (which turns -0 into +0), and TwoSum(0, p) has a zero error.
     if constexpr (M == kSumDouble) {
         double acc[32];
 #pragma unroll
@@ -594,7 +596,10 @@ __device__ __forceinline__ IcpGroupSum<M> icp_group_sums(const kfx_xyz (&v)[P], 
 #pragma unroll
             for (int i = 0; i < 6; ++i)
 #pragma unroll
-                for (int jj = i; jj < 7; ++jj, ++k) acc[k] += (double)(rows[q][i] * rows[q][jj]);
+                for (int jj = i; jj < 7; ++jj, ++k) {
+                    const double prod = (double)(rows[q][i] * rows[q][jj]);
+                    acc[k] = q == 0 ? prod + 0.0 : acc[k] + prod;
+                }
         }
         return group_sum27(acc, wsum_d, lt);
     } else {
@@ -609,7 +614,8 @@ __device__ __forceinline__ IcpGroupSum<M> icp_group_sums(const kfx_xyz (&v)[P], 
 #pragma unroll
                 for (int jj = i; jj < 7; ++jj, ++k) {
                     const float prod = rows[q][i] * rows[q][jj];
-                    if constexpr (M == kSumKahan) kahan_add(sum[k], comp[k], prod);
+                    if (q == 0) sum[k] = prod + 0.0f;
+                    else if constexpr (M == kSumKahan) kahan_add(sum[k], comp[k], prod);
                     else sum[k] += prod;
                 }
         }
