@@ -220,7 +220,8 @@ def kernel_table_learned(n_upd_per_launch, samples_per_launch):
     + 32 B per workgroup, its reduce 32 B per workgroup; the integration
     pre-pass 2 + 4 + 4/3 B/px (depth in, scaled depth and pyramid out);
     integrate 8 N_upd + 2 WH; raycast 4 S + 24 WH; model-map resize 24 B per
-    level-0 pixel read, 24 B per level-1 and level-2 pixel written."""
+    level-0 pixel read, 24 B per level-1 and level-2 pixel written; the
+    one-launch frame ICP 48 B/px per iteration and its exchange."""
     px = level_sizes()
     nblk_icp = [(px[0] + 511) // 512, (px[1] + 255) // 256, (px[2] + 255) // 256]
     nblk_gate = (px[0] + 255) // 256
@@ -240,6 +241,11 @@ def kernel_table_learned(n_upd_per_launch, samples_per_launch):
         rows[f"icp_sums@{W >> l}"] = kernel_row(f"icp_sums@{W >> l}", 48 * px[l] + 216 * nblk_icp[l])
     solve_bytes = sum(n_it * (216 * nb + 600) for n_it, nb in zip((10, 5, 4), nblk_icp)) / 19
     rows["icp_solve"] = kernel_row("icp_solve", solve_bytes)
+    # A frame's ICP in one launch: per iteration the pixels' 48 B and the
+    # exchange (27 values of each of 256 workgroups, written and read once by
+    # the 4 leaders; 4 x 27 wave totals read by every workgroup).
+    exch = 2 * 216 * 256 + 4 * 216 * 256
+    rows["icp_frame"] = kernel_row("icp_frame", sum(n_it * (48 * p + exch) for n_it, p in zip((10, 5, 4), px)))
     return {k: v for k, v in rows.items() if v}
 
 

@@ -87,7 +87,7 @@ extern "C" uint64_t orc_integrate_f32(const float* scaled, kfx_tsdf* vol, const 
                         Ri->data[1].z, Ri->data[2].x, Ri->data[2].y, Ri->data[2].z};
     const float inv_trunc = 1.0f / trunc;
     uint64_t touched = 0;
-#pragma omp parallel for collapse(2) schedule(static) reduction(+ : touched) num_threads(g_threads)
+#pragma omp parallel for collapse(2) schedule(dynamic, 64) reduction(+ : touched) num_threads(g_threads)
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
             const float gx = ((float)x + 0.5f) * cs->x - t->x;
@@ -129,7 +129,7 @@ extern "C" uint64_t orc_integrate_f64(const double* scaled, kfx_tsdf* vol, const
                                       const kfx_round_buffer* rb) {
     const int nx = rb->voxels_size.x, ny = rb->voxels_size.y, nz = rb->voxels_size.z;
     uint64_t touched = 0;
-#pragma omp parallel for collapse(2) schedule(static) reduction(+ : touched) num_threads(g_threads)
+#pragma omp parallel for collapse(2) schedule(dynamic, 64) reduction(+ : touched) num_threads(g_threads)
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
             const double gx = ((double)x + 0.5) * cs[0] - t[0];
@@ -140,6 +140,12 @@ extern "C" uint64_t orc_integrate_f64(const double* scaled, kfx_tsdf* vol, const
                 const double camy = r[3] * gx + r[4] * gy + r[5] * gz;
                 const double camz = r[6] * gx + r[7] * gy + r[8] * gz;
                 if (!(camz > 0.0)) continue;
+                // More than a pixel outside the image on either axis: skipped
+                // before the divisions (the test below rejects it as well).
+                const double ux = camx * in->fx, uy = camy * in->fy;
+                if (ux < -(in->cx + 1.0) * camz || ux > ((double)in->cols - in->cx + 1.0) * camz ||
+                    uy < -(in->cy + 1.0) * camz || uy > ((double)in->rows - in->cy + 1.0) * camz)
+                    continue;
                 const double fu = rint(camx * in->fx / camz + in->cx);
                 const double fv = rint(camy * in->fy / camz + in->cy);
                 if (!(fu >= 0.0 && fu < (double)in->cols && fv >= 0.0 && fv < (double)in->rows)) continue;
@@ -179,7 +185,7 @@ extern "C" uint64_t orc_integrate_colors(kfx_color* vol, const kfx_color* rgb, c
     const float r[9] = {Ri->data[0].x, Ri->data[0].y, Ri->data[0].z, Ri->data[1].x, Ri->data[1].y,
                         Ri->data[1].z, Ri->data[2].x, Ri->data[2].y, Ri->data[2].z};
     uint64_t touched = 0;
-#pragma omp parallel for collapse(2) schedule(static) reduction(+ : touched) num_threads(g_threads)
+#pragma omp parallel for collapse(2) schedule(dynamic, 64) reduction(+ : touched) num_threads(g_threads)
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
             for (int z = 0; z < nz; ++z) {
