@@ -36,6 +36,14 @@ TARGET_DISTANCE = 1.5               # the cube's centre is this far along the op
 PREROLL_FRAMES = 60                 # untimed frames before a timed run's warmup
 
 
+# The long walk (1024^3 / 1280x720 runs): the same scene family over 13.5 m
+# at the same object density, walked at LONG_STEP metres per frame (2000
+# frames cover 12 m).
+LONG_REGION = (0.5, 14.0, -1.5, 1.5)
+LONG_BOXES, LONG_SPHERES = 29, 29
+LONG_STEP = 0.006
+
+
 def intrinsics():
     return kfx.kinect_intrinsics(W, H, 525.0)
 
@@ -44,8 +52,12 @@ def scene(seed):
     return kfx.make_scene(seed, N_BOXES, N_SPHERES, REGION, 0.0)
 
 
-def trajectory(seed, n_frames):
-    return kfx.walk(seed, (0.0, -HEIGHT, 0.0), STEP, PITCH, n_frames)
+def trajectory(seed, n_frames, step=STEP):
+    return kfx.walk(seed, (0.0, -HEIGHT, 0.0), step, PITCH, n_frames)
+
+
+def long_scene(seed):
+    return kfx.make_scene(seed, LONG_BOXES, LONG_SPHERES, LONG_REGION, 0.0)
 
 
 def target_point(pose, target_distance=TARGET_DISTANCE):
