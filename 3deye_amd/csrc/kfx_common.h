@@ -72,6 +72,146 @@ int integrate_colors_posed_ws(const Workspace& ws, kfx_color* color_volume, cons
                               const kfx_xyz* vertex_map, const kfx_intrinsics* intr, const float* dev_pose,
                               float truncation_distance, int max_weight, const kfx_point_3f* cell_size,
                               const kfx_round_buffer* rb, const int32_t* dev_skip);
+// Lane exchanges of the wave butterflies (ICP sums, fallback gate).  Partner l ^ 32 and l ^ 16:
+// v_permlane32_swap / v_permlane16_swap of (a, b) return, in a lane whose
+// bit (5 or 4) is clear, (its own a, the partner's a), and where it is set
+// (the partner's b, its own b).  Partner l ^ 8, 4, 2, 1 (within rows of 16
+// lanes): DPP row_ror:8; quad_perm [3,2,1,0] then row_half_mirror
+// (l ^ 3 ^ 7); quad_perm [2,3,0,1]; quad_perm [1,0,3,2].  Every lane of the
+// wave takes part.
+template <int kXor>
+__device__ __forceinline__ unsigned dpp_xor_u32(unsigned v) {
+    int x = (int)v;
+    if constexpr (kXor == 8) {
+        x = __builtin_amdgcn_mov_dpp(x, 0x128, 0xf, 0xf, false);
+    } else if constexpr (kXor == 4) {
+        x = __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x1B, 0xf, 0xf, false), 0x141, 0xf, 0xf, false);
+    } else if constexpr (kXor == 2) {
+        x = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xf, 0xf, false);
+    } else {
+        static_assert(kXor == 1, "DPP exchange for xor 8, 4, 2, 1");
+        x = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xf, 0xf, false);
+    }
+    return (unsigned)x;
+}
+
+template <int kXor>
+__device__ __forceinline__ float dpp_xor(float v) {
+    return __uint_as_float(dpp_xor_u32<kXor>(__float_as_uint(v)));
+}
+template <int kXor>
+__device__ __forceinline__ double dpp_xor(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint64_t lo = dpp_xor_u32<kXor>((unsigned)b), hi = dpp_xor_u32<kXor>((unsigned)(b >> 32));
+    return __longlong_as_double((long long)((hi << 32) | lo));
+}
+
+// (kept, received) in either order for the partner l ^ (2 kHalf), kHalf =
+// 16 or 8, from a = the low-half value and b = the high-half value.
+template <int kHalf>
+__device__ __forceinline__ void swap_halves(float a, float b, float& x, float& y) {
+    static_assert(kHalf == 16 || kHalf == 8, "permlane swap distances");
+    const auto r = kHalf == 16 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false)
+                               : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    x = __uint_as_float(r[0]);
+    y = __uint_as_float(r[1]);
+}
+template <int kHalf>
+__device__ __forceinline__ void swap_halves(double a, double b, double& x, double& y) {
+    const uint64_t ua = (uint64_t)__double_as_longlong(a), ub = (uint64_t)__double_as_longlong(b);
+    float xl, yl, xh, yh;
+    swap_halves<kHalf>(__uint_as_float((unsigned)ua), __uint_as_float((unsigned)ub), xl, yl);
+    swap_halves<kHalf>(__uint_as_float((unsigned)(ua >> 32)), __uint_as_float((unsigned)(ub >> 32)), xh, yh);
+    x = __longlong_as_double((long long)(((uint64_t)__float_as_uint(xh) << 32) | __float_as_uint(xl)));
+    y = __longlong_as_double((long long)(((uint64_t)__float_as_uint(yh) << 32) | __float_as_uint(yl)));
+}
+
+// Sum over the wave by the xor butterfly (offsets 32, 16, 8, 4, 2, 1): each
+// lane adds its partner's value to its own at every step, so every lane ends
+// with the same total, in the association of `v += shfl_xor(v, off)`.  Lane
+// 0's total is also that of the last six levels of a halving tree (lane
+// t < h adds lane t + h, h = 32 .. 1): for t < h the partner t ^ h is t + h.
+__device__ __forceinline__ double wave_sum_xor(double v, int lane) {
+    double x, y;
+    swap_halves<16>(v, v, x, y);
+    v = v + ((lane & 32) ? x : y);
+    swap_halves<8>(v, v, x, y);
+    v = v + ((lane & 16) ? x : y);
+    v = v + dpp_xor<8>(v);
+    v = v + dpp_xor<4>(v);
+    v = v + dpp_xor<2>(v);
+    v = v + dpp_xor<1>(v);
+    return v;
+}
+
+// Per-pixel terms of the fallback gate: the model vertex of pixel p
+// (volume coordinates) seen from the camera at `pose` (single precision, R
+// row-major camera -> volume then t), projected to the nearest pixel of
+// `depth`; v = (squared depth difference if below max_diff, 1 if so,
+// 1 if the projection finds a depth, 1 if the vertex exists).  In two parts:
+// gate_project (v[3]; the pixel index idx the vertex projects to, -1 when
+// none, and its camera depth cz), then gate_depth with d = depth[idx].
+__device__ __forceinline__ void gate_project(const kfx_xyz& x, const float* pose, const kfx_intrinsics& in,
+                                             double (&v)[4], int& idx, float& cz) {
+    v[0] = v[1] = v[2] = v[3] = 0.0;
+    idx = -1;
+    cz = 0.0f;
+    if (isnan(x.x)) return;
+    v[3] = 1.0;
+    const float dx = x.x - pose[9], dy = x.y - pose[10], dz = x.z - pose[11];
+    const float cx = pose[0] * dx + pose[3] * dy + pose[6] * dz;
+    const float cy = pose[1] * dx + pose[4] * dy + pose[7] * dz;
+    cz = pose[2] * dx + pose[5] * dy + pose[8] * dz;
+    if (!(cz > 0.0f)) return;
+    const float fu = rintf(cx * in.fx / cz + in.cx), fv = rintf(cy * in.fy / cz + in.cy);
+    if (!(fu >= 0.0f && fv >= 0.0f && fu < (float)in.cols && fv < (float)in.rows)) return;
+    idx = (int)fv * in.cols + (int)fu;
+}
+__device__ __forceinline__ void gate_depth(uint16_t d, float cz, float max_diff, double (&v)[4]) {
+    if (!d) return;
+    v[2] = 1.0;
+    const float e = (float)d * 0.001f - cz;
+    if (fabsf(e) < max_diff) {
+        v[0] = (double)e * e;
+        v[1] = 1.0;
+    }
+}
+__device__ __forceinline__ void gate_terms(const kfx_xyz* __restrict__ vmap, const float* pose,
+                                           const uint16_t* __restrict__ depth, const kfx_intrinsics& in,
+                                           float max_diff, int p, double (&v)[4]) {
+    const kfx_xyz x = p < in.cols * in.rows ? vmap[p] : kfx_xyz{NAN, NAN, NAN};
+    int idx;
+    float cz;
+    gate_project(x, pose, in, v, idx, cz);
+    if (idx >= 0) gate_depth(depth[idx], cz, max_diff, v);
+}
+
+// The gate's decision from its totals: fall back with at least 100 valid
+// model vertices when the RMS difference of the matched pixels exceeds
+// fallback_m or fewer than half the vertices match.
+__device__ __forceinline__ int32_t gate_decision(double sse, double cnt, double n_valid, double fallback_m) {
+    const double rms = cnt > 0.0 ? sqrt(sse / cnt) : 0.0;
+    return (fallback_m > 0.0 && n_valid >= 100.0 && (rms > fallback_m || cnt < 0.5 * n_valid)) ? 1 : 0;
+}
+
+// The fallback gate of the learned pipeline (depth_consistency's inputs and
+// outputs), evaluated at the end of a one-launch frame ICP at its final pose.
+struct FrameGate {
+    const kfx_xyz* vmap_model;
+    const uint16_t* depth;
+    kfx_intrinsics intr;
+    float max_diff;
+    double fallback_m;
+    double* stats;        // [4]: squared error, count, matched, valid model vertices
+    int32_t* flag;        // 1: fall back
+    int32_t* host_flag;   // copy of the flag, or null
+};
+// kfx_icp_frame, with the gate (when not null) after the last iteration:
+// in the same launch when the frame runs as one, else as depth_consistency
+// after the separate launches; the results are the same either way.
+int icp_frame_gated(kfx_icp* h, const kfx_icp_level* levels, int begin_frame, float distance_threshold,
+                    float angle_threshold, int tagged, double converge_rotation, double converge_translation,
+                    const FrameGate* gate, const Workspace* gate_ws);
 // The fallback gate (kfx_denoise.hip); ws.cons must hold cons_blocks().
 int depth_consistency(const Workspace& ws, const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
                       const kfx_intrinsics* intr, float max_diff, double* dev_stats, double fallback_m,

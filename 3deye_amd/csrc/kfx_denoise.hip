@@ -8,14 +8,17 @@
 // biases [co]).  Weights are used in bfloat16, accumulation is fp32.
 //
 // Layout: hidden activations and their gradients are bfloat16 NHWC with 16
-// channels (32 bytes per pixel).  The 16 -> 16 layers are implicit GEMMs on
-// the matrix cores (v_mfma_f32_16x16x16_bf16): one wave computes 16 output
-// channels x 16 consecutive pixels, one MFMA per 3x3 tap; the B operand of a
-// lane is 4 channels of one pixel (one 8-byte load), the A operand 4 input
-// channels of one output channel's tap weights.  Weight gradients are GEMMs
+// channels (32 bytes per pixel).  Forward, every layer is an implicit GEMM on
+// the matrix cores, one wave computing 16 output channels x 16 consecutive
+// pixels: the 16 -> 16 layers and the 16 -> 1 layer (one useful output row)
+// with v_mfma_f32_16x16x32_bf16, two 3x3 taps per MFMA (K = 2 taps x 16
+// channels; five MFMAs, the last with one tap), the B operand of a lane 8
+// channels of one pixel (one 16-byte load); the 2 -> 16 layer with
+// v_mfma_f32_16x16x4_f32 on its single-precision inputs (K = 2 channels x 9
+// taps, five MFMAs).  Backward, the input gradients of the hidden layers use
+// v_mfma_f32_16x16x16_bf16, one MFMA per tap; weight gradients are GEMMs
 // over pixels (K = pixels) accumulated per wave, then reduced in a fixed
-// order.  The 2 -> 16 and 16 -> 1 layers (18 and 144 MACs per output) run on
-// the vector ALUs.
+// order.
 #include <math.h>
 
 #include <algorithm>
@@ -33,6 +36,7 @@ constexpr int kOffW4 = 4944, kOffB4 = 5088;
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 struct DnPacked {
     float w1[16 * 2 * 9];        // [co][ci][tap], bf16-rounded values
@@ -45,7 +49,13 @@ struct DnPacked {
     float b3[16];
     float w4[16 * 9];            // [ci][tap], bf16-rounded values
     float b4;
+    // Forward A operands for tap pairs (2 P, 2 P + 1), P < 5: [P][co][k] with
+    // k = 16 (tap - 2 P) + ci, bf16 bits, zero for tap 9.
+    alignas(16) uint16_t w2p[5][16][32];
+    alignas(16) uint16_t w3p[5][16][32];
+    alignas(16) uint16_t w4p[5][32];
 };
+constexpr int kDnPackThreads = 5 * 16 * 32;
 
 __host__ __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t b = __float_as_uint(f);
@@ -71,6 +81,15 @@ __global__ void dn_pack_kernel(const float* __restrict__ m, DnPacked* __restrict
         pk->w2t[tap][ci][co] = f2bf(m[kOffW2 + i]);
         pk->w3[tap][co][ci] = f2bf(m[kOffW3 + i]);
         pk->w3t[tap][ci][co] = f2bf(m[kOffW3 + i]);
+    }
+    if (i < kDnPackThreads) {
+        // i = (P * 16 + co) * 32 + k
+        const int k = i % 32, co = (i / 32) % 16, P = i / 512;
+        const int tap = 2 * P + k / 16, ci = k % 16;
+        const bool real = tap < 9;
+        pk->w2p[P][co][k] = real ? f2bf(m[kOffW2 + (co * 16 + ci) * 9 + tap]) : (uint16_t)0;
+        pk->w3p[P][co][k] = real ? f2bf(m[kOffW3 + (co * 16 + ci) * 9 + tap]) : (uint16_t)0;
+        if (co == 0) pk->w4p[P][k] = real ? f2bf(m[kOffW4 + ci * 9 + tap]) : (uint16_t)0;
     }
 }
 
@@ -113,42 +132,119 @@ __device__ __forceinline__ float dn_output(uint16_t d, float s) {
     return rcp_rn(w);  // w >= wp / 4 >= 0.0038: inside rcp_rn's exact range
 }
 
-// Layer 1 (2 -> 16), one lane per pixel.
-__global__ void __launch_bounds__(256) dn_l1_fwd(const uint16_t* __restrict__ depth, int W, int H,
-                                                 const DnPacked* __restrict__ pk, uint16_t* __restrict__ a1) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || y >= H) return;
-    const uint16_t dc = depth[y * W + x];
-    const float wc = dc ? inv_depth(dc) : 0.0f;
-    const float inv_sigma = dc ? rcp_rn(noise_sigma_inv(dc)) : 0.0f;
-    float in[2][9];
-    for (int ky = 0; ky < 3; ++ky)
-        for (int kx = 0; kx < 3; ++kx) {
-            const int qx = x + kx - 1, qy = y + ky - 1;
-            uint16_t d = 0;
-            if (qx >= 0 && qy >= 0 && qx < W && qy < H) d = depth[qy * W + qx];
-            in[0][ky * 3 + kx] = dc ? rel_depth(d, wc, inv_sigma) : 0.0f;
-            in[1][ky * 3 + kx] = d ? 1.0f : 0.0f;
+// The forward layers on groups of 16 positions: lane l (n = l % 16, q =
+// l / 16) supplies position n's operands and ends with the four output
+// channels 4 q .. 4 q + 3 of position n (the MFMA D layout).  The per-layer
+// kernels and the one-launch inference kernel both evaluate every layer
+// through these, so their outputs are equal bit for bit.
+
+// Per-pixel inputs of layer 1: the inverse depth w = 1000 / d (0 where the
+// pixel has no depth) and 1 / sigma_w (0 likewise).
+__device__ __forceinline__ float dn_w(uint16_t d) { return d ? inv_depth(d) : 0.0f; }
+__device__ __forceinline__ float dn_inv_sigma(uint16_t d) { return d ? rcp_rn(noise_sigma_inv(d)) : 0.0f; }
+
+// Layer 1 (2 -> 16): D[co][n] = sum_k W1[co][k] X[k][n], k = ci * 9 + tap,
+// with X[ci = 0] the tap's inverse depth relative to the centre's in units
+// of the centre's sigma_w, clamped (rel_depth; 0 where the tap or the
+// centre has no depth), and X[ci = 1] the tap's valid mask.  w_at(tap)
+// gives position n's tap inverse depths (dn_w; tap 4 the centre; 0 outside
+// the image), inv_sigma_c the centre's dn_inv_sigma.  Lane l supplies
+// k = 4 j + q of MFMA j.
+template <class WAt>
+__device__ __forceinline__ v4f l1_group(WAt w_at, float inv_sigma_c, const DnPacked* __restrict__ pk, int lane) {
+    const int n = lane & 15, q = lane >> 4;
+    const float wc = w_at(4);
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const int k = 4 * j + q;
+        float x = 0.0f, a = 0.0f;
+        if (k < 18) {
+            const float w = w_at(k % 9);
+            if (k < 9) {
+                const float r = (w - wc) * inv_sigma_c;
+                x = (w != 0.0f && wc != 0.0f) ? fminf(fmaxf(r, -kDnRelClip), kDnRelClip) : 0.0f;
+            } else {
+                x = w != 0.0f ? 1.0f : 0.0f;
+            }
+            a = pk->w1[n * 18 + k];
         }
-    uint32_t packed[8];
-    for (int co = 0; co < 16; co += 2) {
-        float z[2];
-        for (int h = 0; h < 2; ++h) {
-            float s = pk->b1[co + h];
-            for (int ci = 0; ci < 2; ++ci)
-                for (int t = 0; t < 9; ++t) s += pk->w1[((co + h) * 2 + ci) * 9 + t] * in[ci][t];
-            z[h] = s > 0.0f ? s : 0.0f;
-        }
-        packed[co / 2] = (uint32_t)f2bf(z[0]) | ((uint32_t)f2bf(z[1]) << 16);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x, acc, 0, 0, 0);
     }
-    uint4* dst = reinterpret_cast<uint4*>(a1 + (size_t)(y * W + x) * kDnC);
-    dst[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
-    dst[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
+    return acc;
 }
 
-// 4 bf16 channels [c0, c0+4) of pixel (qx, qy) as an MFMA operand; zero
-// outside the image.
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+typedef float v4fp __attribute__((ext_vector_type(4)));
+
+// Bias, ReLU and bfloat16 (round to nearest even, v_cvt_pk_bf16_f32) of a
+// group result: four channels 4 q .. 4 q + 3.
+__device__ __forceinline__ uint2 relu_pack(const v4f& acc, const float* __restrict__ b, int q) {
+    v4fp v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float z = acc[i] + b[4 * q + i];
+        v[i] = z > 0.0f ? z : 0.0f;
+    }
+    return __builtin_bit_cast(uint2, __builtin_convertvector(v, v4bf));
+}
+
+__device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
+
+// A 16 -> 16 layer or the 16 -> 1 layer: D[co][n] = sum over tap pairs P of
+// sum_k Wp[P][co][k] in[n + off(tap)][ci], k = 16 (tap - 2 P) + ci.
+// wa[P]: lane l's A operand (Wp[P][n][8 q .. 8 q + 7]); in_at(tap, c0): the
+// 8 channels c0 .. c0 + 7 of position n's input at the tap (zero outside
+// the image).
+template <class InAt>
+__device__ __forceinline__ v4f conv_group(InAt in_at, const v8bf (&wa)[5], int lane) {
+    const int q = lane >> 4;
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int P = 0; P < 5; ++P) {
+        const int tap = 2 * P + (q >> 1);
+        const uint4 u = tap < 9 ? in_at(tap, 8 * (q & 1)) : make_uint4(0, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[P], as_v8bf(u), acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// A operands of a 16 -> 16 layer (wp = w2p or w3p) or of the 16 -> 1 layer
+// (wp4 = w4p: row co = 0 only, the other rows zero).
+__device__ __forceinline__ void load_wa(const uint16_t (*wp)[16][32], int lane, v8bf (&wa)[5]) {
+    const int n = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int P = 0; P < 5; ++P) wa[P] = as_v8bf(*reinterpret_cast<const uint4*>(&wp[P][n][8 * q]));
+}
+__device__ __forceinline__ void load_wa4(const uint16_t (*wp4)[32], int lane, v8bf (&wa)[5]) {
+    const int n = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int P = 0; P < 5; ++P)
+        wa[P] = as_v8bf(n == 0 ? *reinterpret_cast<const uint4*>(&wp4[P][8 * q]) : make_uint4(0, 0, 0, 0));
+}
+
+// Output depth of the residual s of a pixel with raw depth d: metres
+// (ypred) and millimetres (out), both optional.
+__device__ __forceinline__ void dn_store_output(uint16_t d, float s, int p, float* __restrict__ ypred,
+                                                uint16_t* __restrict__ out) {
+    const float y = d ? dn_output(d, s) : 0.0f;
+    if (ypred) ypred[p] = y;
+    if (out) {
+        int mm = 0;
+        if (d) mm = (int)fminf(fmaxf(rintf(y * 1000.0f), 1.0f), 65535.0f);
+        out[p] = (uint16_t)mm;
+    }
+}
+
+// The 8 bf16 channels c0 .. c0 + 7 of pixel (qx, qy) of an NHWC activation;
+// zero outside the image.
+__device__ __forceinline__ uint4 load8(const uint16_t* __restrict__ act, int W, int H, int qx, int qy, int c0) {
+    if (qx < 0 || qy < 0 || qx >= W || qy >= H) return make_uint4(0, 0, 0, 0);
+    return *reinterpret_cast<const uint4*>(act + (size_t)(qy * W + qx) * kDnC + c0);
+}
+
+// 4 bf16 channels [c0, c0+4) of pixel (qx, qy) as a 16x16x16 MFMA operand
+// (the backward input gradients); zero outside the image.
 __device__ __forceinline__ v4s load4(const uint16_t* __restrict__ act, int W, int H, int qx, int qy, int c0) {
     v4s r = {0, 0, 0, 0};
     if (qx >= 0 && qy >= 0 && qx < W && qy < H) {
@@ -171,69 +267,58 @@ __device__ __forceinline__ v4s wload4(const uint16_t* w) {
     return r;
 }
 
-// Hidden layer (16 -> 16), forward: out[p][co] = relu(b[co] + sum_tap
-// sum_ci W[tap][co][ci] in[p + off(tap)][ci]).  Wave = 16 pixels of a row.
-// D[m = co][n = pixel]: lane l holds channels 4*(l/16)..+3 of pixel l%16.
-__global__ void __launch_bounds__(256) dn_mid_fwd(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int W,
-                                                  int H, const uint16_t* __restrict__ w, const float* __restrict__ b) {
+// Per-layer forward kernels (training): a wave computes 16 consecutive
+// pixels of a row, four waves per workgroup.
+__global__ void __launch_bounds__(256) dn_l1_fwd(const uint16_t* __restrict__ depth, int W, int H,
+                                                 const DnPacked* __restrict__ pk, uint16_t* __restrict__ a1) {
     const int lane = threadIdx.x & 63;
     const int x0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
     const int y = blockIdx.y;
     if (x0 >= W) return;
-    const int n = lane & 15, kq = (lane >> 4) * 4;
-    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap) {
-        const int ky = tap / 3, kx = tap % 3;
-        const v4s a = wload4(w + (tap * 16 + n) * 16 + kq);          // A[m = co = n][k = ci]
-        const v4s bb = load4(in, W, H, x0 + n + kx - 1, y + ky - 1, kq);  // B[k = ci][n = pixel]
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc, 0, 0, 0);
-    }
-    const int px = x0 + n;
-    if (px >= W) return;
-    uint32_t lo = 0, hi = 0;
-    float v[4];
-    for (int i = 0; i < 4; ++i) {
-        const float z = acc[i] + b[kq + i];
-        v[i] = z > 0.0f ? z : 0.0f;
-    }
-    const float v0 = v[0], v1 = v[1], v2 = v[2], v3 = v[3];
-    lo = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
-    hi = (uint32_t)f2bf(v2) | ((uint32_t)f2bf(v3) << 16);
-    *reinterpret_cast<uint2*>(out + (size_t)(y * W + px) * kDnC + kq) = make_uint2(lo, hi);
+    const int n = lane & 15, q = lane >> 4;
+    const int x = x0 + n;
+    auto w_at = [&](int t) -> float {
+        const int qx = x + t % 3 - 1, qy = y + t / 3 - 1;
+        return dn_w((qx >= 0 && qy >= 0 && qx < W && qy < H) ? depth[qy * W + qx] : (uint16_t)0);
+    };
+    const uint16_t dc = (x < W) ? depth[y * W + x] : (uint16_t)0;
+    const v4f acc = l1_group(w_at, dn_inv_sigma(dc), pk, lane);
+    if (x < W) *reinterpret_cast<uint2*>(a1 + (size_t)(y * W + x) * kDnC + 4 * q) = relu_pack(acc, pk->b1, q);
 }
 
-// Layer 4 (16 -> 1) and the residual: y = x0 + b4 + sum w4 a3 on valid
-// pixels.  Writes y (metres, optional) and the depth in mm (optional).
+// Hidden layer (16 -> 16): out[p][co] = relu(b[co] + sum_tap sum_ci
+// W[tap][co][ci] in[p + off(tap)][ci]).
+__global__ void __launch_bounds__(256) dn_mid_fwd(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int W,
+                                                  int H, const uint16_t (*wp)[16][32], const float* __restrict__ b) {
+    const int lane = threadIdx.x & 63;
+    const int x0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    const int y = blockIdx.y;
+    if (x0 >= W) return;
+    const int n = lane & 15, q = lane >> 4;
+    v8bf wa[5];
+    load_wa(wp, lane, wa);
+    auto in_at = [&](int t, int c0) { return load8(in, W, H, x0 + n + t % 3 - 1, y + t / 3 - 1, c0); };
+    const v4f acc = conv_group(in_at, wa, lane);
+    const int px = x0 + n;
+    if (px < W) *reinterpret_cast<uint2*>(out + (size_t)(y * W + px) * kDnC + 4 * q) = relu_pack(acc, b, q);
+}
+
+// Layer 4 (16 -> 1) and the output: s = b4 + sum w4 a3 (row 0 of the
+// group result, held by lanes 0..15).
 __global__ void __launch_bounds__(256) dn_l4_fwd(const uint16_t* __restrict__ depth, const uint16_t* __restrict__ a3,
                                                  int W, int H, const DnPacked* __restrict__ pk,
                                                  float* __restrict__ ypred, uint16_t* __restrict__ out) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int yy = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || yy >= H) return;
-    const int p = yy * W + x;
-    const uint16_t d = depth[p];
-    float s = pk->b4;
-    for (int ky = 0; ky < 3; ++ky)
-        for (int kx = 0; kx < 3; ++kx) {
-            const int qx = x + kx - 1, qy = yy + ky - 1;
-            if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
-            const uint4* src = reinterpret_cast<const uint4*>(a3 + (size_t)(qy * W + qx) * kDnC);
-            const uint4 u0 = src[0], u1 = src[1];
-            const uint32_t words[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
-            const int t = ky * 3 + kx;
-            for (int c = 0; c < 16; ++c) {
-                const uint16_t h = (uint16_t)((words[c >> 1] >> ((c & 1) * 16)) & 0xffffu);
-                s += pk->w4[c * 9 + t] * bf2f(h);
-            }
-        }
-    const float y = d ? dn_output(d, s) : 0.0f;
-    if (ypred) ypred[p] = y;
-    if (out) {
-        int mm = 0;
-        if (d) mm = (int)fminf(fmaxf(rintf(y * 1000.0f), 1.0f), 65535.0f);
-        out[p] = (uint16_t)mm;
-    }
+    const int lane = threadIdx.x & 63;
+    const int x0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    const int y = blockIdx.y;
+    if (x0 >= W) return;
+    const int n = lane & 15;
+    v8bf wa[5];
+    load_wa4(pk->w4p, lane, wa);
+    auto in_at = [&](int t, int c0) { return load8(a3, W, H, x0 + n + t % 3 - 1, y + t / 3 - 1, c0); };
+    const v4f acc = conv_group(in_at, wa, lane);
+    const int px = x0 + n;
+    if (lane < 16 && px < W) dn_store_output(depth[y * W + px], acc[0] + pk->b4, y * W + px, ypred, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -241,13 +326,11 @@ __global__ void __launch_bounds__(256) dn_l4_fwd(const uint16_t* __restrict__ de
 // kDfTX x kDfTY output pixels and computes, in LDS, layer 1 on the tile and
 // a 3-pixel halo, layer 2 on a 2-pixel halo, layer 3 on a 1-pixel halo, then
 // layer 4 and the output depth; the hidden activations never leave the CU.
-// Each value is computed with the per-layer kernels' operations in their
-// order (the same MFMA sequence for the hidden layers), and positions
-// outside the image hold 0 (the zero padding), so the output equals the
-// four-launch forward bit for bit.  At 640x480 on MI355X: 33 us against
-// 49 us for the four launches with 32 x 16 tiles and 512 threads (52 KB of
-// LDS, three workgroups per CU, 600 workgroups); 64 x 16 tiles (96 KB, one
-// workgroup per CU) took 63 us, 256 threads per 32 x 16 tile 41 us.
+// Every layer goes through the per-layer kernels' group functions, and
+// positions outside the image hold 0 (the zero padding), so the output
+// equals the four-launch forward bit for bit.  Tiles of 32 x 16 with 512
+// threads (52 KB of LDS, three workgroups per CU, 600 workgroups at
+// 640x480).
 // ---------------------------------------------------------------------------
 constexpr int kDfTX = 32, kDfTY = 16, kDfThreads = 512;
 constexpr int kDfDX = kDfTX + 8, kDfDY = kDfTY + 8;   // depth tile (4-pixel halo)
@@ -256,44 +339,26 @@ constexpr int kDfA2X = kDfTX + 4, kDfA2Y = kDfTY + 4; // layer-2 outputs
 constexpr int kDfA3X = kDfTX + 2, kDfA3Y = kDfTY + 2; // layer-3 outputs (in the layer-1 buffer)
 static_assert(kDfA3X * kDfA3Y <= kDfA1X * kDfA1Y, "layer 3 reuses the layer-1 buffer");
 
-// One hidden layer on the tile: out (ox x oy positions, image origin gx0,
-// gy0) from in (ox + 2 wide), 16 consecutive positions of a row per MFMA
-// group, dn_mid_fwd's arithmetic.
+// A 16 -> 16 layer on the tile: out (ox x oy positions, image origin gx0,
+// gy0) from in (ox + 2 wide), 16 consecutive positions of a row per group.
 __device__ __forceinline__ void df_mid_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int ox,
-                                             int oy, int gx0, int gy0, int W, int H, const v4s (&wa)[9],
+                                             int oy, int gx0, int gy0, int W, int H, const v8bf (&wa)[5],
                                              const float* __restrict__ b) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int n = lane & 15, kq = (lane >> 4) * 4;
+    const int n = lane & 15, q = lane >> 4;
     const int segs_x = (ox + 15) / 16;
     const int ix = ox + 2;
     for (int g = wave; g < segs_x * oy; g += kDfThreads / 64) {
         const int sy = g / segs_x, sx = (g % segs_x) * 16;
         const int px = min(sx + n, ox - 1);  // the partial last group reads a valid position
-        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int tap = 0; tap < 9; ++tap) {
-            const int ky = tap / 3, kx = tap % 3;
-            const uint2 u = *reinterpret_cast<const uint2*>(in + ((sy + ky) * ix + px + kx) * kDnC + kq);
-            v4s bb;
-            bb[0] = (short)(u.x & 0xffffu);
-            bb[1] = (short)(u.x >> 16);
-            bb[2] = (short)(u.y & 0xffffu);
-            bb[3] = (short)(u.y >> 16);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[tap], bb, acc, 0, 0, 0);
-        }
+        auto in_at = [&](int t, int c0) {
+            return *reinterpret_cast<const uint4*>(in + ((sy + t / 3) * ix + px + t % 3) * kDnC + c0);
+        };
+        const v4f acc = conv_group(in_at, wa, lane);
         if (sx + n >= ox) continue;
         const int gx = gx0 + sx + n, gy = gy0 + sy;
-        uint32_t lo = 0, hi = 0;
-        if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-            float v[4];
-            for (int i = 0; i < 4; ++i) {
-                const float z = acc[i] + b[kq + i];
-                v[i] = z > 0.0f ? z : 0.0f;
-            }
-            lo = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-            hi = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-        }
-        *reinterpret_cast<uint2*>(out + (sy * ox + sx + n) * kDnC + kq) = make_uint2(lo, hi);
+        const uint2 v = (gx >= 0 && gy >= 0 && gx < W && gy < H) ? relu_pack(acc, b, q) : make_uint2(0, 0);
+        *reinterpret_cast<uint2*>(out + (sy * ox + sx + n) * kDnC + 4 * q) = v;
     }
 }
 
@@ -304,83 +369,53 @@ __global__ void __launch_bounds__(kDfThreads) dn_fused_fwd(const uint16_t* __res
     __shared__ __align__(16) uint16_t s1[kDfA1Y * kDfA1X * kDnC];
     __shared__ __align__(16) uint16_t s2[kDfA2Y * kDfA2X * kDnC];
     const int x0 = blockIdx.x * kDfTX, y0 = blockIdx.y * kDfTY;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, q = lane >> 4;
+    // The depth tile, and per tile pixel dn_w and dn_inv_sigma in the
+    // layer-2 buffer (free until layer 2).
+    float* sw = reinterpret_cast<float*>(s2);
+    float* sis = sw + kDfDX * kDfDY;
+    static_assert(2 * kDfDX * kDfDY * sizeof(float) <= sizeof(s2), "layer-1 inputs fit the layer-2 buffer");
     for (int i = threadIdx.x; i < kDfDX * kDfDY; i += kDfThreads) {
         const int qx = x0 - 4 + i % kDfDX, qy = y0 - 4 + i / kDfDX;
-        sd[i] = (qx >= 0 && qy >= 0 && qx < W && qy < H) ? depth[qy * W + qx] : (uint16_t)0;
+        const uint16_t d = (qx >= 0 && qy >= 0 && qx < W && qy < H) ? depth[qy * W + qx] : (uint16_t)0;
+        sd[i] = d;
+        sw[i] = dn_w(d);
+        sis[i] = dn_inv_sigma(d);
     }
     __syncthreads();
-    // Layer 1 (dn_l1_fwd per position; 0 outside the image).
-    for (int i = threadIdx.x; i < kDfA1X * kDfA1Y; i += kDfThreads) {
+    // Layer 1 on groups of 16 consecutive positions of the flattened
+    // layer-1 tile (0 outside the image).
+    constexpr int kA1 = kDfA1X * kDfA1Y;
+    for (int g = wave; g < (kA1 + 15) / 16; g += kDfThreads / 64) {
+        const int i = min(16 * g + n, kA1 - 1);  // the partial last group reads a valid position
         const int lx = i % kDfA1X, ly = i / kDfA1X;
+        auto w_at = [&](int t) -> float { return sw[(ly + t / 3) * kDfDX + lx + t % 3]; };
+        const v4f acc = l1_group(w_at, sis[(ly + 1) * kDfDX + lx + 1], pk, lane);
+        if (16 * g + n >= kA1) continue;
         const int x = x0 - 3 + lx, y = y0 - 3 + ly;
-        uint32_t packed[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (x >= 0 && y >= 0 && x < W && y < H) {
-            const uint16_t dc = sd[(ly + 1) * kDfDX + lx + 1];
-            const float wc = dc ? inv_depth(dc) : 0.0f;
-            const float inv_sigma = dc ? rcp_rn(noise_sigma_inv(dc)) : 0.0f;
-            float in[2][9];
-            for (int ky = 0; ky < 3; ++ky)
-                for (int kx = 0; kx < 3; ++kx) {
-                    const uint16_t d = sd[(ly + ky) * kDfDX + lx + kx];
-                    in[0][ky * 3 + kx] = dc ? rel_depth(d, wc, inv_sigma) : 0.0f;
-                    in[1][ky * 3 + kx] = d ? 1.0f : 0.0f;
-                }
-            for (int co = 0; co < 16; co += 2) {
-                float z[2];
-                for (int h = 0; h < 2; ++h) {
-                    float s = pk->b1[co + h];
-                    for (int ci = 0; ci < 2; ++ci)
-                        for (int t = 0; t < 9; ++t) s += pk->w1[((co + h) * 2 + ci) * 9 + t] * in[ci][t];
-                    z[h] = s > 0.0f ? s : 0.0f;
-                }
-                packed[co / 2] = (uint32_t)f2bf(z[0]) | ((uint32_t)f2bf(z[1]) << 16);
-            }
-        }
-        uint4* dst = reinterpret_cast<uint4*>(s1 + i * kDnC);
-        dst[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
-        dst[1] = make_uint4(packed[4], packed[5], packed[6], packed[7]);
+        const uint2 v = (x >= 0 && y >= 0 && x < W && y < H) ? relu_pack(acc, pk->b1, q) : make_uint2(0, 0);
+        *reinterpret_cast<uint2*>(s1 + i * kDnC + 4 * q) = v;
     }
-    // Hidden-layer weights of this lane (the A operands of its 9 MFMAs).
-    const int lane = threadIdx.x & 63;
-    const int n = lane & 15, kq = (lane >> 4) * 4;
-    v4s wa[9];
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap) wa[tap] = wload4(&pk->w2[tap][n][kq]);
-    __syncthreads();
+    v8bf wa[5];
+    load_wa(pk->w2p, lane, wa);
+    __syncthreads();  // layer 1 done: s2 free for layer 2's outputs
     df_mid_layer(s1, s2, kDfA2X, kDfA2Y, x0 - 2, y0 - 2, W, H, wa, pk->b2);
-#pragma unroll
-    for (int tap = 0; tap < 9; ++tap) wa[tap] = wload4(&pk->w3[tap][n][kq]);
+    load_wa(pk->w3p, lane, wa);
     __syncthreads();
     df_mid_layer(s2, s1, kDfA3X, kDfA3Y, x0 - 1, y0 - 1, W, H, wa, pk->b3);
+    load_wa4(pk->w4p, lane, wa);
     __syncthreads();
-    // Layer 4 and the output (dn_l4_fwd per pixel).
-    for (int i = threadIdx.x; i < kDfTX * kDfTY; i += kDfThreads) {
-        const int lx = i % kDfTX, ly = i / kDfTX;
-        const int x = x0 + lx, yy = y0 + ly;
-        if (x >= W || yy >= H) continue;
-        const int p = yy * W + x;
-        const uint16_t d = sd[(ly + 4) * kDfDX + lx + 4];
-        float s = pk->b4;
-        for (int ky = 0; ky < 3; ++ky)
-            for (int kx = 0; kx < 3; ++kx) {
-                const int qx = x + kx - 1, qy = yy + ky - 1;
-                if (qx < 0 || qy < 0 || qx >= W || qy >= H) continue;
-                const uint4* src = reinterpret_cast<const uint4*>(s1 + ((ly + ky) * kDfA3X + lx + kx) * kDnC);
-                const uint4 u0 = src[0], u1 = src[1];
-                const uint32_t words[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
-                const int t = ky * 3 + kx;
-                for (int c = 0; c < 16; ++c) {
-                    const uint16_t h = (uint16_t)((words[c >> 1] >> ((c & 1) * 16)) & 0xffffu);
-                    s += pk->w4[c * 9 + t] * bf2f(h);
-                }
-            }
-        const float y = d ? dn_output(d, s) : 0.0f;
-        if (ypred) ypred[p] = y;
-        if (out) {
-            int mm = 0;
-            if (d) mm = (int)fminf(fmaxf(rintf(y * 1000.0f), 1.0f), 65535.0f);
-            out[p] = (uint16_t)mm;
-        }
+    // Layer 4 and the output, 16 pixels of a tile row per group.
+    for (int g = wave; g < kDfTY * (kDfTX / 16); g += kDfThreads / 64) {
+        const int ly = g / (kDfTX / 16), sx = (g % (kDfTX / 16)) * 16;
+        auto in_at = [&](int t, int c0) {
+            return *reinterpret_cast<const uint4*>(s1 + ((ly + t / 3) * kDfA3X + sx + n + t % 3) * kDnC + c0);
+        };
+        const v4f acc = conv_group(in_at, wa, lane);
+        const int x = x0 + sx + n, yy = y0 + ly;
+        if (lane < 16 && x < W && yy < H)
+            dn_store_output(sd[(ly + 4) * kDfDX + sx + n + 4], acc[0] + pk->b4, yy * W + x, ypred, out);
     }
 }
 
@@ -748,16 +783,15 @@ int small_waves(int W, int H) { return H * ((W + kSmallPixelsPerWave - 1) / kSma
 
 int dn_forward(kfx_denoiser* d, const uint16_t* depth, uint16_t* out, float* ypred) {
     const int W = d->W, H = d->H;
-    dim3 pix_grid(divup(W, 64), divup(H, 4));
     dim3 mid_grid(divup(W, 64), H);
     ProfScope prof("denoise_forward");
-    dn_l1_fwd<<<pix_grid, 256, 0, stream()>>>(depth, W, H, d->pk, d->a1);
+    dn_l1_fwd<<<mid_grid, 256, 0, stream()>>>(depth, W, H, d->pk, d->a1);
     KFX_CHECK_LAUNCH();
-    dn_mid_fwd<<<mid_grid, 256, 0, stream()>>>(d->a1, d->a2, W, H, &d->pk->w2[0][0][0], d->pk->b2);
+    dn_mid_fwd<<<mid_grid, 256, 0, stream()>>>(d->a1, d->a2, W, H, d->pk->w2p, d->pk->b2);
     KFX_CHECK_LAUNCH();
-    dn_mid_fwd<<<mid_grid, 256, 0, stream()>>>(d->a2, d->a3, W, H, &d->pk->w3[0][0][0], d->pk->b3);
+    dn_mid_fwd<<<mid_grid, 256, 0, stream()>>>(d->a2, d->a3, W, H, d->pk->w3p, d->pk->b3);
     KFX_CHECK_LAUNCH();
-    dn_l4_fwd<<<pix_grid, 256, 0, stream()>>>(depth, d->a3, W, H, d->pk, ypred, out);
+    dn_l4_fwd<<<mid_grid, 256, 0, stream()>>>(depth, d->a3, W, H, d->pk, ypred, out);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
@@ -772,7 +806,7 @@ int dn_forward_fused(kfx_denoiser* d, const uint16_t* depth, uint16_t* out, floa
 }
 
 int dn_pack(kfx_denoiser* d) {
-    dn_pack_kernel<<<divup(2304, 256), 256, 0, stream()>>>(d->master, d->pk);
+    dn_pack_kernel<<<divup(kDnPackThreads, 256), 256, 0, stream()>>>(d->master, d->pk);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }
@@ -987,40 +1021,12 @@ __global__ void __launch_bounds__(256) depth_consistency_kernel(const kfx_xyz* _
                                                                 kfx_intrinsics in, float max_diff,
                                                                 double* __restrict__ partials) {
     __shared__ double red[4][4];
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    double s = 0.0, c = 0.0, m = 0.0, nv = 0.0;
-    if (p < in.cols * in.rows) {
-        const kfx_xyz v = vmap[p];
-        if (!isnan(v.x)) {
-            nv = 1.0;
-            const float dx = v.x - dpose[9], dy = v.y - dpose[10], dz = v.z - dpose[11];
-            const float cx = dpose[0] * dx + dpose[3] * dy + dpose[6] * dz;
-            const float cy = dpose[1] * dx + dpose[4] * dy + dpose[7] * dz;
-            const float cz = dpose[2] * dx + dpose[5] * dy + dpose[8] * dz;
-            if (cz > 0.0f) {
-                const float fu = rintf(cx * in.fx / cz + in.cx), fv = rintf(cy * in.fy / cz + in.cy);
-                if (fu >= 0.0f && fv >= 0.0f && fu < (float)in.cols && fv < (float)in.rows) {
-                    const uint16_t d = depth[(int)fv * in.cols + (int)fu];
-                    if (d) {
-                        m = 1.0;
-                        const float e = (float)d * 0.001f - cz;
-                        if (fabsf(e) < max_diff) {
-                            s = (double)e * e;
-                            c = 1.0;
-                        }
-                    }
-                }
-            }
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        s += __shfl_xor(s, off);
-        c += __shfl_xor(c, off);
-        m += __shfl_xor(m, off);
-        nv += __shfl_xor(nv, off);
-    }
+    double v[4];
+    gate_terms(vmap, dpose, depth, in, max_diff, blockIdx.x * 256 + threadIdx.x, v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) { red[wave][0] = s; red[wave][1] = c; red[wave][2] = m; red[wave][3] = nv; }
+    for (int k = 0; k < 4; ++k) v[k] = wave_sum_xor(v[k], lane);
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) red[wave][k] = v[k];
     __syncthreads();
     if (threadIdx.x < 4) {
         const int k = threadIdx.x;
@@ -1051,9 +1057,7 @@ __global__ void __launch_bounds__(256) consistency_reduce_kernel(const double* _
     }
     if (threadIdx.x < 4) stats[threadIdx.x] = red[threadIdx.x][0];
     if (threadIdx.x == 0 && flag != nullptr) {
-        const double sse = red[0][0], cnt = red[1][0], n_valid = red[3][0];
-        const double rms = cnt > 0.0 ? sqrt(sse / cnt) : 0.0;
-        const int32_t f = (fallback_m > 0.0 && n_valid >= 100.0 && (rms > fallback_m || cnt < 0.5 * n_valid)) ? 1 : 0;
+        const int32_t f = gate_decision(red[0][0], red[1][0], red[3][0], fallback_m);
         *flag = f;
         if (host_flag != nullptr) *host_flag = f;
     }

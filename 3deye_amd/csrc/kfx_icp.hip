@@ -200,60 +200,6 @@ constexpr int kIcpWaves = kIcpThreads / 64;
 constexpr int kIcpEntries = 27;
 constexpr unsigned kIcpTwoPixelMin = 150000;  // images at least this large take two pixels per thread
 
-// Lane exchanges of the butterflies below.  Partner l ^ 32 and l ^ 16:
-// v_permlane32_swap / v_permlane16_swap of (a, b) return, in a lane whose
-// bit (5 or 4) is clear, (its own a, the partner's a), and where it is set
-// (the partner's b, its own b).  Partner l ^ 8, 4, 2, 1 (within rows of 16
-// lanes): DPP row_ror:8; quad_perm [3,2,1,0] then row_half_mirror
-// (l ^ 3 ^ 7); quad_perm [2,3,0,1]; quad_perm [1,0,3,2].  Every lane of the
-// wave takes part.
-template <int kXor>
-__device__ __forceinline__ unsigned dpp_xor_u32(unsigned v) {
-    int x = (int)v;
-    if constexpr (kXor == 8) {
-        x = __builtin_amdgcn_mov_dpp(x, 0x128, 0xf, 0xf, false);
-    } else if constexpr (kXor == 4) {
-        x = __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x1B, 0xf, 0xf, false), 0x141, 0xf, 0xf, false);
-    } else if constexpr (kXor == 2) {
-        x = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xf, 0xf, false);
-    } else {
-        static_assert(kXor == 1, "DPP exchange for xor 8, 4, 2, 1");
-        x = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xf, 0xf, false);
-    }
-    return (unsigned)x;
-}
-
-template <int kXor>
-__device__ __forceinline__ float dpp_xor(float v) {
-    return __uint_as_float(dpp_xor_u32<kXor>(__float_as_uint(v)));
-}
-template <int kXor>
-__device__ __forceinline__ double dpp_xor(double v) {
-    const uint64_t b = (uint64_t)__double_as_longlong(v);
-    const uint64_t lo = dpp_xor_u32<kXor>((unsigned)b), hi = dpp_xor_u32<kXor>((unsigned)(b >> 32));
-    return __longlong_as_double((long long)((hi << 32) | lo));
-}
-
-// (kept, received) in either order for the partner l ^ (2 kHalf), kHalf =
-// 16 or 8, from a = the low-half value and b = the high-half value.
-template <int kHalf>
-__device__ __forceinline__ void swap_halves(float a, float b, float& x, float& y) {
-    static_assert(kHalf == 16 || kHalf == 8, "permlane swap distances");
-    const auto r = kHalf == 16 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false)
-                               : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
-    x = __uint_as_float(r[0]);
-    y = __uint_as_float(r[1]);
-}
-template <int kHalf>
-__device__ __forceinline__ void swap_halves(double a, double b, double& x, double& y) {
-    const uint64_t ua = (uint64_t)__double_as_longlong(a), ub = (uint64_t)__double_as_longlong(b);
-    float xl, yl, xh, yh;
-    swap_halves<kHalf>(__uint_as_float((unsigned)ua), __uint_as_float((unsigned)ub), xl, yl);
-    swap_halves<kHalf>(__uint_as_float((unsigned)(ua >> 32)), __uint_as_float((unsigned)(ub >> 32)), xh, yh);
-    x = __longlong_as_double((long long)(((uint64_t)__float_as_uint(xh) << 32) | __float_as_uint(xl)));
-    y = __longlong_as_double((long long)(((uint64_t)__float_as_uint(yh) << 32) | __float_as_uint(yl)));
-}
-
 // Sums padded to 32 entries, spread over a wave by a butterfly that halves
 // the entries each step: after the step with xor offset 32, 16, 8, 4, 2 a
 // lane keeps the half selected by that bit of its lane id, added to its
@@ -932,7 +878,8 @@ constexpr int kFrameBlocks = kIcpThreads;                      // one workgroup 
 constexpr int kFrameBuffers = 3;
 constexpr int kStage1Entries = kIcpEntries * kFrameBlocks;     // [27][kFrameBlocks] per buffer
 constexpr int kStage2Entries = kIcpWaves * 32;                 // [4 waves][32] per buffer
-constexpr int kFrameEntries = kFrameBuffers * (kStage1Entries + kStage2Entries);
+constexpr int kGateEntries = 4 * kFrameBlocks;                 // the gate's workgroup sums
+constexpr int kFrameEntries = kFrameBuffers * (kStage1Entries + kStage2Entries) + kGateEntries;
 constexpr unsigned kPollLimit = 1u << 21;                      // about 0.1 s of polls
 // Empty-entry marker: a NaN whose payload no arithmetic produces (the
 // generated NaN is 0x7ff8000000000000, and a propagated single-precision
@@ -957,6 +904,8 @@ struct IcpFrameArgs {
     int begin;           // start of a frame (icp_begin_frame_kernel's update first)
     int tagged;          // level l's iterations carry tag l (early exit)
     double conv_rot, conv_trans;
+    FrameGate gate;          // the fallback gate (the kernel instance with the gate)
+    uint64_t* gate_entries;  // [kFrameBlocks][4] workgroup gate sums (double bits) or kEmptyEntry
 };
 
 template <int M>
@@ -973,7 +922,119 @@ __device__ __forceinline__ void copy_state(IcpState* dst, const IcpState* src) {
     for (int i = threadIdx.x; i < (int)(sizeof(IcpState) / 8); i += blockDim.x) d[i] = s[i];
 }
 
-template <int M>
+// The fallback gate at the final pose (depth_consistency_kernel and
+// consistency_reduce_kernel in one pass, with their sums in their order):
+// group g of workgroup w takes the gate's 256-pixel blocks w + kFrameBlocks
+// (3 r + g) (round r), sums each block as the gate kernel does (the wave
+// butterfly, then the waves in order) and the workgroup adds its blocks in
+// increasing order, as the reduce kernel's thread w does; workgroup 0 then
+// waits for every workgroup's 4 sums, runs the reduce kernel's tree over
+// them and writes the totals and the decision, and empties the entries for
+// the next launch (every workgroup has stored its own by then).  The model
+// vertices of the first kGateLdsRounds rounds may have been loaded into gv
+// already (gv_ready); rounds are taken in pairs, both depth gathers issued
+// before either is used.
+constexpr int kGateLdsRounds = 2;
+
+struct GateLds {
+    kfx_xyz v[kGateLdsRounds][kFrameThreads];
+};
+
+__device__ __forceinline__ int gate_pixel(int w, int r, int t, int nblk) {
+    const int blk = w + kFrameBlocks * (kFrameGroups * r + t / kIcpThreads);
+    return blk < nblk ? blk * 256 + t % kIcpThreads : 0x7fffffff;
+}
+
+// Model vertices of this workgroup's first rounds into LDS, by every thread
+// but the first wave's (which runs the solve meanwhile).
+__device__ __forceinline__ void gate_prefetch(const IcpFrameArgs& a, GateLds& gv, int w) {
+    if (threadIdx.x < 64) return;
+    const FrameGate& G = a.gate;
+    const int P = G.intr.cols * G.intr.rows, nblk = (P + 255) / 256;
+    for (int i = threadIdx.x - 64; i < kGateLdsRounds * kFrameThreads; i += kFrameThreads - 64) {
+        const int r = i / kFrameThreads, t = i % kFrameThreads;
+        const int p = gate_pixel(w, r, t, nblk);
+        gv.v[r][t] = p < P ? G.vmap_model[p] : kfx_xyz{NAN, NAN, NAN};
+    }
+}
+
+__device__ __forceinline__ void frame_gate(const IcpFrameArgs& a, const IcpState& s, const GateLds& gv,
+                                           bool gv_ready, int w, int grp, int lt, int& fault) {
+    __shared__ double gred[2][kFrameGroups][kIcpWaves][4];
+    __shared__ double gtree[4][kFrameBlocks];
+    const FrameGate& G = a.gate;
+    const int P = G.intr.cols * G.intr.rows, nblk = (P + 255) / 256;
+    const int rounds = (nblk + kFrameBlocks * kFrameGroups - 1) / (kFrameBlocks * kFrameGroups);
+    float pose[12];
+    for (int i = 0; i < 9; ++i) pose[i] = s.Rf[i];
+    for (int i = 0; i < 3; ++i) pose[9 + i] = s.tf[i];
+    const int lane = lt & 63, wave = lt >> 6;
+    double acc = 0.0;  // thread k < 4: entry k over this workgroup's blocks
+    for (int r0 = 0; r0 < rounds; r0 += 2) {
+        double v[2][4];
+        int idx[2];
+        float cz[2];
+        for (int h = 0; h < 2; ++h) {
+            const int r = r0 + h, p = gate_pixel(w, r, threadIdx.x, nblk);
+            const kfx_xyz x = (gv_ready && r < kGateLdsRounds) ? gv.v[r][threadIdx.x]
+                              : (r < rounds && p < P)           ? G.vmap_model[p]
+                                                                : kfx_xyz{NAN, NAN, NAN};
+            gate_project(x, pose, G.intr, v[h], idx[h], cz[h]);
+        }
+        uint16_t d[2];
+        for (int h = 0; h < 2; ++h) d[h] = idx[h] >= 0 ? G.depth[idx[h]] : (uint16_t)0;
+        for (int h = 0; h < 2; ++h) {
+            if (idx[h] >= 0) gate_depth(d[h], cz[h], G.max_diff, v[h]);
+            for (int k = 0; k < 4; ++k) v[h][k] = wave_sum_xor(v[h][k], lane);
+            if (lane == 0)
+                for (int k = 0; k < 4; ++k) gred[h][grp][wave][k] = v[h][k];
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            const int k = threadIdx.x;
+            for (int h = 0; h < 2; ++h)
+                for (int q = 0; q < kFrameGroups; ++q)
+                    if (w + kFrameBlocks * (kFrameGroups * (r0 + h) + q) < nblk)
+                        acc += ((gred[h][q][0][k] + gred[h][q][1][k]) + gred[h][q][2][k]) + gred[h][q][3][k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) store_coherent(&a.gate_entries[w * 4 + threadIdx.x], (uint64_t)__double_as_longlong(acc));
+    if (w != 0) return;
+    if (threadIdx.x < kFrameBlocks) {
+        uint64_t x[4];
+        bool ready = false;
+        for (unsigned polls = 0; !ready; ++polls) {
+            for (int k = 0; k < 4; ++k) x[k] = load_coherent(&a.gate_entries[threadIdx.x * 4 + k]);
+            ready = x[0] != kEmptyEntry && x[1] != kEmptyEntry && x[2] != kEmptyEntry && x[3] != kEmptyEntry;
+            if (!ready && polls + 1 == kPollLimit) {
+                fault = 1;
+                break;
+            }
+        }
+        for (int k = 0; k < 4; ++k) gtree[k][threadIdx.x] = __longlong_as_double((long long)x[k]);
+    }
+    // The reduce kernel's halving tree: h = 128 and 64 through LDS, then
+    // h = 32 .. 1 in the first wave.
+    __syncthreads();
+    if (threadIdx.x < 128)
+        for (int k = 0; k < 4; ++k) gtree[k][threadIdx.x] += gtree[k][threadIdx.x + 128];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        double t[4];
+        for (int k = 0; k < 4; ++k) t[k] = wave_sum_xor(gtree[k][threadIdx.x] + gtree[k][threadIdx.x + 64], lane);
+        if (threadIdx.x < 4) G.stats[threadIdx.x] = threadIdx.x == 0 ? t[0] : threadIdx.x == 1 ? t[1] : threadIdx.x == 2 ? t[2] : t[3];
+        if (threadIdx.x == 0) {
+            const int32_t f = gate_decision(t[0], t[1], t[3], G.fallback_m);
+            *G.flag = f;
+            if (G.host_flag != nullptr) *G.host_flag = f;
+        }
+    }
+    if (threadIdx.x < kFrameBlocks)
+        for (int k = 0; k < 4; ++k) store_coherent(&a.gate_entries[threadIdx.x * 4 + k], kEmptyEntry);
+}
+
+template <int M, bool kGate>
 __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a) {
     __shared__ double wsum_d[kFrameGroups][kIcpWaves][32];
     __shared__ float2 wsum_f[kFrameGroups][kIcpWaves][32];
@@ -982,6 +1043,8 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
     __shared__ double total[32];
     __shared__ IcpState s;
     __shared__ int fault;
+    __shared__ std::conditional_t<kGate, GateLds, char> gv;
+    bool gv_ready = false;
     const int grp = threadIdx.x / kIcpThreads, lt = threadIdx.x % kIcpThreads;
     const int w = blockIdx.x;
     const bool leader = w < kIcpWaves;
@@ -1113,10 +1176,19 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
                 for (int i = 0; i < 3; ++i) t0[i] = s.t[i];
                 solve_and_update(&s, total, R0, t0, s.iterations, tag, a.conv_rot, a.conv_trans);
             }
+            // During the frame's last solve the other waves load the gate's
+            // model vertices.
+            if constexpr (kGate) {
+                if (l == 0 && it == L.iterations - 1) {
+                    gate_prefetch(a, gv, w);
+                    gv_ready = true;
+                }
+            }
             __syncthreads();
             ++k;
         }
     }
+    if constexpr (kGate) frame_gate(a, s, gv, gv_ready, w, grp, lt, fault);
     if (w == 0) {
         if (threadIdx.x == 0) {
             s.fault = fault;
@@ -1249,15 +1321,110 @@ static bool frame_kernel_resident() {
     if (hipGetDevice(&dev) != hipSuccess) return false;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
     int per_cu = 1 << 30;
-    const void* kernels[] = {reinterpret_cast<const void*>(&icp_frame_kernel<kSumKahan>),
-                             reinterpret_cast<const void*>(&icp_frame_kernel<kSumPlain>),
-                             reinterpret_cast<const void*>(&icp_frame_kernel<kSumDouble>)};
+    const void* kernels[] = {reinterpret_cast<const void*>(&icp_frame_kernel<kSumKahan, false>),
+                             reinterpret_cast<const void*>(&icp_frame_kernel<kSumPlain, false>),
+                             reinterpret_cast<const void*>(&icp_frame_kernel<kSumDouble, false>),
+                             reinterpret_cast<const void*>(&icp_frame_kernel<kSumKahan, true>),
+                             reinterpret_cast<const void*>(&icp_frame_kernel<kSumPlain, true>),
+                             reinterpret_cast<const void*>(&icp_frame_kernel<kSumDouble, true>)};
     for (const void* k : kernels) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kFrameThreads, 0) != hipSuccess) return false;
         per_cu = n < per_cu ? n : per_cu;
     }
     return (long long)per_cu * cus >= kFrameBlocks;
+}
+
+int kfx::icp_frame_gated(kfx_icp* h, const kfx_icp_level* levels, int begin_frame, float distance_threshold,
+                         float angle_threshold, int tagged, double converge_rotation, double converge_translation,
+                         const FrameGate* gate, const Workspace* gate_ws) {
+    // A publish request applies to this call only, whether or not it
+    // passes its checks.
+    const bool publish = h != nullptr && h->publish_next;
+    if (h != nullptr) h->publish_next = false;
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(h && levels, "kfx_icp_frame: null pointer");
+    KFX_REQUIRE(gate == nullptr || (gate->vmap_model && gate->depth && gate->stats && gate->flag && gate_ws &&
+                                    gate->intr.cols > 0 && gate->intr.rows > 0),
+                "kfx_icp_frame: incomplete gate");
+    int last = -1;  // the finest level with iterations: its last one publishes
+    for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l) {
+        const kfx_icp_level& L = levels[l];
+        KFX_REQUIRE(L.iterations >= 0, "kfx_icp_frame: level %d: negative iteration count", l);
+        if (L.iterations == 0) continue;
+        last = l;
+        KFX_REQUIRE(L.vmap_current && L.nmap_current && L.vmap_previous && L.nmap_previous,
+                    "kfx_icp_frame: level %d: null map", l);
+        KFX_REQUIRE(L.intr.cols > 0 && L.intr.rows > 0, "kfx_icp_frame: level %d: empty image", l);
+        const unsigned npix = (unsigned)(L.intr.cols * L.intr.rows);
+        KFX_REQUIRE(divup(npix, (unsigned)kIcpThreads) <= (unsigned)h->max_blocks,
+                    "kfx_icp_frame: level %d: image of %u pixels exceeds the handle's capacity", l, npix);
+    }
+    if (h->launch_mode != KFX_ICP_LAUNCH_FRAME || !h->frame_resident) {
+        // The same frame as separate launches.
+        if (begin_frame) KFX_TRY(kfx_icp_begin_frame(h));
+        for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l)
+            for (int it = 0; it < levels[l].iterations; ++it) {
+                if (publish && l == last && it == levels[l].iterations - 1) h->publish_next = true;
+                KFX_TRY(kfx_icp_iterate(h, levels[l].vmap_current, levels[l].nmap_current, levels[l].vmap_previous,
+                                        levels[l].nmap_previous, nullptr, nullptr, &levels[l].intr,
+                                        distance_threshold, angle_threshold, tagged ? l : -1, converge_rotation,
+                                        converge_translation));
+            }
+        if (gate != nullptr)
+            KFX_TRY(depth_consistency(*gate_ws, gate->vmap_model, &h->state->Rf[0], gate->depth, &gate->intr,
+                                      gate->max_diff, gate->stats, gate->fallback_m, gate->flag, gate->host_flag));
+        return KFX_OK;
+    }
+    IcpFrameArgs a;
+    for (int l = 0; l < KFX_ICP_LEVELS; ++l) {
+        const kfx_icp_level& L = levels[l];
+        IcpFrameLevel& d = a.lvl[l];
+        d.vc = L.vmap_current;
+        d.nc = L.nmap_current;
+        d.vp = L.vmap_previous;
+        d.np = L.nmap_previous;
+        d.iterations = L.iterations;
+        d.two = 0;
+        d.nblk = 0;
+        if (L.iterations > 0) {
+            d.f = make_frame(nullptr, nullptr, &L.intr, distance_threshold, angle_threshold);
+            const unsigned npix = (unsigned)(L.intr.cols * L.intr.rows);
+            d.two = npix >= kIcpTwoPixelMin ? 1 : 0;
+            d.nblk = (int)divup(npix, (unsigned)(kIcpThreads * (d.two ? 2 : 1)));
+        } else {
+            d.f = IcpFrame{};
+        }
+    }
+    a.st = h->state;
+    a.entries = h->frame_entries;
+    a.epoch = h->ticket + 1;
+    a.mirror = (publish && last >= 0) ? h->host_state : nullptr;
+    a.begin = begin_frame ? 1 : 0;
+    a.tagged = tagged ? 1 : 0;
+    a.conv_rot = converge_rotation;
+    a.conv_trans = converge_translation;
+    a.gate = gate != nullptr ? *gate : FrameGate{};
+    a.gate_entries = h->frame_entries + kFrameBuffers * (kStage1Entries + kStage2Entries);
+    {
+        ProfScope prof("icp_frame");
+        auto launch = [&](auto mode, auto with_gate) {
+            icp_frame_kernel<decltype(mode)::value, decltype(with_gate)::value>
+                <<<kFrameBlocks, kFrameThreads, 0, stream()>>>(a);
+        };
+        auto launch_mode = [&](auto with_gate) {
+            switch (h->sum_mode) {
+                case KFX_ICP_SUM_FP32: launch(std::integral_constant<int, kSumPlain>{}, with_gate); break;
+                case KFX_ICP_SUM_FP64: launch(std::integral_constant<int, kSumDouble>{}, with_gate); break;
+                default: launch(std::integral_constant<int, kSumKahan>{}, with_gate); break;
+            }
+        };
+        if (gate != nullptr) launch_mode(std::true_type{});
+        else launch_mode(std::false_type{});
+        KFX_CHECK_LAUNCH();
+    }
+    if (a.mirror != nullptr) h->published = true;
+    return KFX_OK;
 }
 
 extern "C" {
@@ -1435,77 +1602,8 @@ int kfx_icp_frame_resident(kfx_icp* h, int* resident) {
 
 int kfx_icp_frame(kfx_icp* h, const kfx_icp_level* levels, int begin_frame, float distance_threshold,
                   float angle_threshold, int tagged, double converge_rotation, double converge_translation) {
-    // A publish request applies to this call only, whether or not it
-    // passes its checks.
-    const bool publish = h != nullptr && h->publish_next;
-    if (h != nullptr) h->publish_next = false;
-    KFX_REQUIRE_INIT();
-    KFX_REQUIRE(h && levels, "kfx_icp_frame: null pointer");
-    int last = -1;  // the finest level with iterations: its last one publishes
-    for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l) {
-        const kfx_icp_level& L = levels[l];
-        KFX_REQUIRE(L.iterations >= 0, "kfx_icp_frame: level %d: negative iteration count", l);
-        if (L.iterations == 0) continue;
-        last = l;
-        KFX_REQUIRE(L.vmap_current && L.nmap_current && L.vmap_previous && L.nmap_previous,
-                    "kfx_icp_frame: level %d: null map", l);
-        KFX_REQUIRE(L.intr.cols > 0 && L.intr.rows > 0, "kfx_icp_frame: level %d: empty image", l);
-        const unsigned npix = (unsigned)(L.intr.cols * L.intr.rows);
-        KFX_REQUIRE(divup(npix, (unsigned)kIcpThreads) <= (unsigned)h->max_blocks,
-                    "kfx_icp_frame: level %d: image of %u pixels exceeds the handle's capacity", l, npix);
-    }
-    if (h->launch_mode != KFX_ICP_LAUNCH_FRAME || !h->frame_resident) {
-        // The same frame as separate launches.
-        if (begin_frame) KFX_TRY(kfx_icp_begin_frame(h));
-        for (int l = KFX_ICP_LEVELS - 1; l >= 0; --l)
-            for (int it = 0; it < levels[l].iterations; ++it) {
-                if (publish && l == last && it == levels[l].iterations - 1) h->publish_next = true;
-                KFX_TRY(kfx_icp_iterate(h, levels[l].vmap_current, levels[l].nmap_current, levels[l].vmap_previous,
-                                        levels[l].nmap_previous, nullptr, nullptr, &levels[l].intr,
-                                        distance_threshold, angle_threshold, tagged ? l : -1, converge_rotation,
-                                        converge_translation));
-            }
-        return KFX_OK;
-    }
-    IcpFrameArgs a;
-    for (int l = 0; l < KFX_ICP_LEVELS; ++l) {
-        const kfx_icp_level& L = levels[l];
-        IcpFrameLevel& d = a.lvl[l];
-        d.vc = L.vmap_current;
-        d.nc = L.nmap_current;
-        d.vp = L.vmap_previous;
-        d.np = L.nmap_previous;
-        d.iterations = L.iterations;
-        d.two = 0;
-        d.nblk = 0;
-        if (L.iterations > 0) {
-            d.f = make_frame(nullptr, nullptr, &L.intr, distance_threshold, angle_threshold);
-            const unsigned npix = (unsigned)(L.intr.cols * L.intr.rows);
-            d.two = npix >= kIcpTwoPixelMin ? 1 : 0;
-            d.nblk = (int)divup(npix, (unsigned)(kIcpThreads * (d.two ? 2 : 1)));
-        } else {
-            d.f = IcpFrame{};
-        }
-    }
-    a.st = h->state;
-    a.entries = h->frame_entries;
-    a.epoch = h->ticket + 1;
-    a.mirror = (publish && last >= 0) ? h->host_state : nullptr;
-    a.begin = begin_frame ? 1 : 0;
-    a.tagged = tagged ? 1 : 0;
-    a.conv_rot = converge_rotation;
-    a.conv_trans = converge_translation;
-    {
-        ProfScope prof("icp_frame");
-        switch (h->sum_mode) {
-            case KFX_ICP_SUM_FP32: icp_frame_kernel<kSumPlain><<<kFrameBlocks, kFrameThreads, 0, stream()>>>(a); break;
-            case KFX_ICP_SUM_FP64: icp_frame_kernel<kSumDouble><<<kFrameBlocks, kFrameThreads, 0, stream()>>>(a); break;
-            default: icp_frame_kernel<kSumKahan><<<kFrameBlocks, kFrameThreads, 0, stream()>>>(a); break;
-        }
-        KFX_CHECK_LAUNCH();
-    }
-    if (a.mirror != nullptr) h->published = true;
-    return KFX_OK;
+    return kfx::icp_frame_gated(h, levels, begin_frame, distance_threshold, angle_threshold, tagged,
+                                converge_rotation, converge_translation, nullptr, nullptr);
 }
 
 int kfx_icp_set_sum_mode(kfx_icp* h, int mode) {

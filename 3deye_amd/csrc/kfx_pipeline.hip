@@ -234,7 +234,7 @@ int icp_plain(kfx_pipeline* p, kfx_frame_info* info) {
 // pinned host memory, which the host reads after the frame
 // (kfx_icp_read_published).  Level tags only with the convergence test on:
 // an untagged iteration skips the read of the convergence state.
-int icp_part(kfx_pipeline* p, bool begin) {
+int icp_part(kfx_pipeline* p, bool begin, const kfx::FrameGate* gate = nullptr) {
     kfx_icp_level lv[KFX_ICP_LEVELS];
     for (int l = 0; l < KFX_ICP_LEVELS; ++l) {
         lv[l].vmap_current = p->vc[l];
@@ -245,9 +245,9 @@ int icp_part(kfx_pipeline* p, bool begin) {
         lv[l].iterations = p->cfg.icp_iterations[l];
     }
     KFX_TRY(kfx_icp_publish_next(p->icp));
-    return kfx_icp_frame(p->icp, lv, begin ? 1 : 0, p->cfg.distance_threshold, p->cfg.angle_threshold,
-                         p->cfg.converge_rotation > 0.0 ? 1 : 0, p->cfg.converge_rotation,
-                         p->cfg.converge_translation);
+    return kfx::icp_frame_gated(p->icp, lv, begin ? 1 : 0, p->cfg.distance_threshold, p->cfg.angle_threshold,
+                                p->cfg.converge_rotation > 0.0 ? 1 : 0, p->cfg.converge_rotation,
+                                p->cfg.converge_translation, gate, &p->ws);
 }
 
 // Integration and raycast at the device pose, skipped on the device when
@@ -281,14 +281,20 @@ int icp_read_frame(kfx_pipeline* p, kfx_icp_result* r) {
 
 int learned_frame(kfx_pipeline* p) {
     KFX_TRY(preprocess(p, true));
-    KFX_TRY(icp_part(p, true));
-    const float *dpose = nullptr, *dprev = nullptr;
-    KFX_TRY(kfx_icp_pose_device(p->icp, &dpose, &dprev));
-    // The gate's statistics go straight to pinned host memory and its flag
-    // both to the device (read by the skips below) and to the host: no copy
-    // commands after the graph.
-    KFX_TRY(depth_consistency(p->ws, p->vp[0], dpose, p->dep[0], &p->lvl[0], 0.1f, p->hstats,
-                              1e-3 * (double)p->cfg.fallback_rms_mm, p->dgate, p->hgate));
+    // ICP, then the gate at the new pose (in the ICP launch where the frame
+    // runs as one).  The gate's statistics go straight to pinned host
+    // memory and its flag both to the device (read by the skips below) and
+    // to the host: no copy commands after the graph.
+    kfx::FrameGate gate;
+    gate.vmap_model = p->vp[0];
+    gate.depth = p->dep[0];
+    gate.intr = p->lvl[0];
+    gate.max_diff = 0.1f;
+    gate.fallback_m = 1e-3 * (double)p->cfg.fallback_rms_mm;
+    gate.stats = p->hstats;
+    gate.flag = p->dgate;
+    gate.host_flag = p->hgate;
+    KFX_TRY(icp_part(p, true, &gate));
     // The raycast below overwrites vp[0]: the gate has read it already.
     return map_part_skip(p, p->dgate);
 }

@@ -177,6 +177,28 @@ def test_learned_pipeline_identity_denoiser_tracks(gpu, use_graph):
     assert kfx.ate_rmse([i.pose_world for i in infos], gt) < 0.01
 
 
+@pytest.mark.parametrize("case", ["trained", "rms_gate"])
+def test_learned_gate_in_one_launch_frame_is_exact(gpu, case):
+    # The learned pipeline evaluates its fallback gate in the one-launch
+    # frame ICP; with separate launches (ICP iterations, then the two gate
+    # kernels) the poses, gate statistics and decisions are the same bits.
+    kfx = gpu
+    if case == "trained":
+        params, kw = kfx.Denoiser.load_params(), {}
+    else:
+        params = np.zeros(kfx.DENOISER_PARAMS, np.float32)
+        params[-1] = 15.0
+        kw = dict(icp_iterations=(0, 0, 0))
+    runs = [_c1_learned(kfx, params, 1, frames=10, icp_launch=launch, **kw)[0]
+            for launch in (kfx.ICP_LAUNCH_FRAME, kfx.ICP_LAUNCH_TWO)]
+    for a, b in zip(*runs):
+        assert np.array_equal(np.array(a.pose_world.t[:]), np.array(b.pose_world.t[:]))
+        assert np.array_equal(np.array(a.pose_world.R[:]), np.array(b.pose_world.R[:]))
+        assert a.fallback == b.fallback and a.denoise_rms_mm == b.denoise_rms_mm
+    if case == "rms_gate":
+        assert all(i.fallback == 1 for i in runs[0][1:])
+
+
 def test_learned_pipeline_rms_gate(gpu):
     # Output bias of 15 noise sigmas (about 5 cm at the C1 depths of 1.2-2 m)
     # with ICP switched off (the estimate stays at the previous pose, so the
