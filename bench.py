@@ -33,8 +33,8 @@ algorithmic bytes or FLOPs per launch from HIP events around the launches.
 
 Prints one JSON line.  Other modes: --kernel-probe (the learned pipeline,
 eager, for hardware-counter runs), --summarise-pmc (turns two counter runs
-of the probe into profiles/integrate_traffic.json), --profile-learned N (the
-learned pipeline, graph replay, for a kernel trace).
+of the probe into profiles/integrate_traffic.json), --profile-learned N and
+--profile-fused N (one pipeline, graph replay, for a kernel trace).
 """
 import argparse
 import ctypes as C
@@ -69,8 +69,10 @@ def workload_inputs(n_frames):
     return wl.sequence(SCENE_SEED, n_frames)
 
 
-def run_pipeline(cfg, poses, origin, ddepth, frame_bytes, warmup, steps, profile=False, denoiser=None):
-    """Frames [0, PREROLL + warmup) untimed, then `steps` timed frames.
+def run_pipeline(cfg, poses, origin, ddepth, frame_bytes, warmup, steps, profile=False, denoiser=None,
+                 rest=0):
+    """Frames [0, PREROLL + warmup) untimed, then `steps` timed frames, then
+    `rest` more frames untimed (the rest of the walk, for its ATE).
     Returns (elapsed seconds, frame infos, pipeline, index of the first timed
     frame)."""
     pipe = kfx.Pipeline(cfg, poses[0], origin)
@@ -89,17 +91,24 @@ def run_pipeline(cfg, poses, origin, ddepth, frame_bytes, warmup, steps, profile
     elapsed = time.perf_counter() - t0
     if profile:
         kfx.profile_enable(False)
+    for i in range(first + steps, first + steps + rest):
+        infos.append(pipe.process(C.c_void_p(ddepth.ptr.value + i * frame_bytes)))
+    kfx.synchronize()
     return elapsed, infos, pipe, first
 
 
 def summary(elapsed, infos, pipe, first, steps, poses):
-    """frames/s of the timed frames, ATE over every frame processed, and what
-    the timed frames did (shifts, extracted points, fallbacks, ICP
-    iterations)."""
-    timed = infos[first:]
+    """frames/s of the timed frames; ATE over every frame processed (the
+    whole walk when the run went on past the timed frames) and over the
+    frames up to the end of the timed ones; what the timed frames did
+    (shifts, extracted points, fallbacks, ICP iterations)."""
+    timed = infos[first:first + steps]
     n = len(infos)
+    est = [i.pose_world for i in infos]
     out = {"frames_per_s": steps / elapsed, "ms_per_frame": 1e3 * elapsed / steps,
-           "ate_m": kfx.ate_rmse([i.pose_world for i in infos], poses[:n]),
+           "ate_m": kfx.ate_rmse(est, poses[:n]),
+           "ate_frames": [0, n - 1],
+           "ate_m_to_timed_end": kfx.ate_rmse(est[:first + steps], poses[:first + steps]),
            "frames_timed": [first, first + steps - 1],
            "shifts_timed": int(sum(i.shifted for i in timed)),
            "shift_frames_timed": [first + k for k, i in enumerate(timed) if i.shifted],
@@ -449,6 +458,37 @@ def summarise_pmc(fetch_dir, write_dir, probe_log, out_json, command):
           f"write {out['write_calibration']:.4f}")
 
 
+def summarise_trace(db_dir, frames):
+    """Kernel table of a rocprofv3 --kernel-trace run of --profile-learned or
+    --profile-fused: per kernel, dispatches, mean and median span (us) and
+    microseconds per frame over the last `frames` frames (the timed ones: the
+    dispatches after the frame boundary that leaves `frames` pipeline
+    frames, one denoiser or bilateral launch opening each)."""
+    dbs = glob.glob(os.path.join(db_dir, "**", "*.db"), recursive=True)
+    if not dbs:
+        raise SystemExit(f"no rocprofv3 database under {db_dir}")
+    con = sqlite3.connect(dbs[0])
+    rows = con.execute("select S.display_name, K.start, K.end from rocpd_kernel_dispatch K join "
+                       "rocpd_info_kernel_symbol S on S.id = K.kernel_id and S.guid = K.guid "
+                       "order by K.start").fetchall()
+    con.close()
+    opens = [i for i, r in enumerate(rows) if "dn_fused_fwd" in r[0] or "bilateral_kernel" in r[0]]
+    if len(opens) < frames:
+        raise SystemExit(f"{len(opens)} frames in the trace, asked for {frames}")
+    rows = rows[opens[-frames]:]
+    per = {}
+    for name, a, b in rows:
+        per.setdefault(name, []).append((b - a) / 1e3)
+    wall = (rows[-1][2] - rows[0][1]) / 1e3 / frames
+    print(f"{frames} frames; first-dispatch to last-end wall time per frame {wall:.1f} us")
+    print(f"{'kernel':60s} {'calls':>6s} {'mean_us':>8s} {'med_us':>8s} {'us/frame':>9s}")
+    total = 0.0
+    for name, d in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        total += sum(d) / frames
+        print(f"{name[:60]:60s} {len(d):6d} {np.mean(d):8.2f} {np.median(d):8.2f} {sum(d) / frames:9.1f}")
+    print(f"{'sum of kernel spans per frame':60s} {'':6s} {'':8s} {'':8s} {total:9.1f}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -460,29 +500,37 @@ def main():
     ap.add_argument("--profile-learned", type=int, default=0, metavar="N",
                     help="only the learned pipeline (graph replay): the pre-roll, 10 warmup and N timed frames, "
                          "for a kernel trace (prints frames/s, no JSON benchmark line)")
+    ap.add_argument("--profile-fused", type=int, default=0, metavar="N",
+                    help="the same for the fused pipeline with the bilateral filter")
     ap.add_argument("--kernel-probe", action="store_true",
                     help="the learned pipeline, eager, for hardware-counter runs (no JSON benchmark line)")
     ap.add_argument("--summarise-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "PROBE_LOG"),
                     help="write profiles/integrate_traffic.json from two --pmc runs of --kernel-probe")
+    ap.add_argument("--summarise-trace", nargs=2, metavar=("TRACE_DIR", "FRAMES"),
+                    help="kernel table of the last FRAMES frames of a --kernel-trace run of --profile-learned/fused")
     ap.add_argument("--pmc-command", default="", help="the counter command, recorded with --summarise-pmc")
     ap.add_argument("--traffic-out", default=TRAFFIC_FILE, help="where --summarise-pmc writes its table")
     args = ap.parse_args()
+    if args.summarise_trace:
+        summarise_trace(args.summarise_trace[0], int(args.summarise_trace[1]))
+        return
     if args.summarise_pmc:
         summarise_pmc(*args.summarise_pmc, args.traffic_out, args.pmc_command)
         return
     if args.kernel_probe:
         kernel_probe()
         return
-    if args.profile_learned:
-        n_t = args.profile_learned
+    if args.profile_learned or args.profile_fused:
+        learned_run = args.profile_learned > 0
+        n_t = args.profile_learned if learned_run else args.profile_fused
         intr, scene, poses, origin, depths = workload_inputs(PREROLL + 10 + n_t)
         kfx.init(0)
         dd = kfx.Buffer.from_array(depths)
-        dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params())
-        cfg = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1)
+        dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params()) if learned_run else None
+        cfg = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1 if learned_run else 0)
         el, _, pipe, _ = run_pipeline(cfg, poses, origin, dd, depths[0].nbytes, 10, n_t, denoiser=dn)
         pipe.close()
-        print("learned pipeline: %d frames, %.1f frames/s" % (n_t, n_t / el))
+        print("%s pipeline: %d frames, %.1f frames/s" % ("learned" if learned_run else "fused", n_t, n_t / el))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the pipeline runs on a single MI355X "
@@ -494,15 +542,22 @@ def main():
 
     intr, scene, poses, origin, depths = workload_inputs(N_FRAMES)
     kfx.init(0)
-    ddepth = kfx.Buffer.from_array(depths[:n])
+    ddepth = kfx.Buffer.from_array(depths)
     fb = depths[0].nbytes
     first = PREROLL + args.warmup
+    rest = N_FRAMES - n  # frames after the timed ones: untimed, for the whole walk's ATE
 
     dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params())
     early = dict(converge_rotation=1e-5, converge_translation=1e-5)
     # Learned pipeline: headline.
     cfg_l = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1)
-    run_l = run_pipeline(cfg_l, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=dn)
+    # GPU warm-up before the first timed run: the learned pipeline over the
+    # pre-roll frames, repeated for half a second (an idle GPU starts a run
+    # at lower clocks than one that has been busy).
+    t_w = time.perf_counter()
+    while time.perf_counter() - t_w < 0.5:
+        run_pipeline(cfg_l, poses, origin, ddepth, fb, args.warmup, 1, denoiser=dn)[2].close()
+    run_l = run_pipeline(cfg_l, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=dn, rest=rest)
     inf_l = run_l[1]
     learned = summary(*run_l, args.steps, poses)
     # Beside the headline, not as it: the KinectFusion early exit (a level
@@ -510,12 +565,14 @@ def main():
     # tolerance), at most 10/5/4 iterations, for both filters.
     cfg_le = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1, **early)
     learned["early_exit_1e-5"] = summary(
-        *run_pipeline(cfg_le, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=dn), args.steps, poses)
+        *run_pipeline(cfg_le, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=dn, rest=rest), args.steps,
+        poses)
     cfg_f = wl.pipeline_config(mode=kfx.FUSED, use_graph=1)
-    fused = summary(*run_pipeline(cfg_f, poses, origin, ddepth, fb, args.warmup, args.steps), args.steps, poses)
+    fused = summary(*run_pipeline(cfg_f, poses, origin, ddepth, fb, args.warmup, args.steps, rest=rest), args.steps,
+                    poses)
     cfg_fe = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, **early)
-    fused["early_exit_1e-5"] = summary(*run_pipeline(cfg_fe, poses, origin, ddepth, fb, args.warmup, args.steps),
-                                       args.steps, poses)
+    fused["early_exit_1e-5"] = summary(*run_pipeline(cfg_fe, poses, origin, ddepth, fb, args.warmup, args.steps,
+                                                     rest=rest), args.steps, poses)
 
     # The learned pipeline again, eager, with HIP events around every launch:
     # the per-kernel table and the integrate roofline of the headline.
@@ -550,9 +607,9 @@ def main():
     if not args.skip_plain:
         cfg_p = wl.pipeline_config(mode=kfx.PLAIN)
         el_p, inf_p, pipe_p, _ = run_pipeline(cfg_p, poses, origin, ddepth, fb, args.warmup, args.steps,
-                                              profile=True)
+                                              profile=True, rest=rest)
         p_int_ms, p_int_n = kfx.profile_query("integrate")
-        p_upd = count_updates(intr, ddepth, fb, inf_p, first, rb)
+        p_upd = count_updates(intr, ddepth, fb, inf_p[:n], first, rb)
         ps_avg, ps_frames = count_raycast_samples(cfg_p, poses, origin, ddepth, fb, first, n)
         kfx.profile_enable(False)
         plain = summary(el_p, inf_p, pipe_p, first, args.steps, poses)
@@ -593,6 +650,8 @@ def main():
         "plain": plain,
         "speedup_vs_plain": (learned["frames_per_s"] / plain["frames_per_s"]) if plain else None,
         "ate_le_plain": (learned["ate_m"] <= plain["ate_m"]) if plain else None,
+        "ate_note": "ate_m: RMSE of camera positions over the whole 300-frame walk (the frames after the timed "
+                    "ones run untimed); ate_m_to_timed_end: over frames 0 to the last timed frame",
         "roofline": roofline,
         "kernels_learned": kern_l,
         "kernels_plain": kern_p,
