@@ -77,15 +77,29 @@ __device__ __forceinline__ uint16_t pyr_down_px(At at, int scols, int srows, int
     const int x_lo = max(0, 2 * x - 2) - 2 * x, y_lo = max(0, 2 * y - 2) - 2 * y;
     const int x_hi = min(scols, 2 * x + 3) - 2 * x, y_hi = min(srows, 2 * y + 3) - 2 * y;
     const float thr = 3.0f * sigma_color;
+    // The 25 taps' loads are issued together (taps outside the image read
+    // the centre and are masked), then added row by row in the window's
+    // order; a masked tap adds +0 to non-negative sums, which leaves them
+    // unchanged, so the sums are those of the loop over the clipped window.
+    int val[5][5];
+#pragma unroll
+    for (int yi = -2; yi <= 2; ++yi)
+#pragma unroll
+        for (int xi = -2; xi <= 2; ++xi) {
+            const bool in = yi >= y_lo && yi < y_hi && xi >= x_lo && xi < x_hi;
+            val[yi + 2][xi + 2] = at(in ? 2 * x + xi : 2 * x, in ? 2 * y + yi : 2 * y);
+        }
     float sum = 0.0f, wall = 0.0f;
-    for (int yi = y_lo; yi < y_hi; ++yi)
-        for (int xi = x_lo; xi < x_hi; ++xi) {
-            const int val = at(2 * x + xi, 2 * y + yi);
-            if ((float)abs(val - center) < thr) {
-                const float w = wt[abs(xi)] * wt[abs(yi)];
-                sum += (float)val * w;
-                wall += w;
-            }
+#pragma unroll
+    for (int yi = -2; yi <= 2; ++yi)
+#pragma unroll
+        for (int xi = -2; xi <= 2; ++xi) {
+            const int v = val[yi + 2][xi + 2];
+            const bool in = yi >= y_lo && yi < y_hi && xi >= x_lo && xi < x_hi;
+            const float w = wt[xi < 0 ? -xi : xi] * wt[yi < 0 ? -yi : yi];
+            const bool take = in && (float)abs(v - center) < thr;
+            sum += take ? (float)v * w : 0.0f;
+            wall += take ? w : 0.0f;
         }
     return (uint16_t)(int)(sum / wall);
 }
