@@ -59,11 +59,21 @@ Workspace& library_workspace();
 int integrate_tsdf_ws(const Workspace& ws, bool profile, const uint16_t* depth, float* depth_scaled,
                       kfx_tsdf* volume, const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
                       const kfx_point_3f* translation, float truncation_distance, const kfx_point_3f* cell_size,
-                      const kfx_round_buffer* rb);
+                      const kfx_round_buffer* rb, uint8_t* bricks = nullptr);
 int integrate_tsdf_posed_ws(const Workspace& ws, const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
                             const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
                             float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
-                            const kfx_round_buffer* rb, const int32_t* dev_skip);
+                            const kfx_round_buffer* rb, const int32_t* dev_skip, uint8_t* bricks = nullptr);
+// Brick flags of the raycast's skipping march (kfx_raycast.hip): bytes for
+// a volume (0 when an axis is not a multiple of 16, which turns the skipping
+// off); the exact flags of a volume (non-zero: the brick holds a negative
+// value); the raycast at a device pose with the skipping march.
+size_t brick_flag_bytes(const kfx_point_3i& voxels);
+int brick_flags_build(const kfx_tsdf* volume, const kfx_round_buffer* rb, uint8_t* bricks);
+int raycast_posed_bricks(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                         const kfx_round_buffer* rb, const float* dev_pose, float truncation_distance,
+                         const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip,
+                         const uint8_t* bricks);
 int integrate_colors_ws(const Workspace& ws, kfx_color* color_volume, const kfx_color* rgb_frame,
                         const kfx_xyz* vertex_map, const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
                         const kfx_point_3f* translation, float truncation_distance, int max_weight,

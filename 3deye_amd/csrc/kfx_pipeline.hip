@@ -50,6 +50,7 @@ struct kfx_pipeline {
     double* hstats = nullptr;             // gate statistics (pinned host memory the gate kernel writes)
     int32_t* dgate = nullptr;             // fallback decision of the gate (device)
     int32_t* hgate = nullptr;             // host mirror of dgate (pinned, written by the gate kernel)
+    uint8_t* bricks = nullptr;            // brick flags of the skipping raycast (fused mode; null: full march)
     kfx::Workspace ws;                    // depth- and colour-bound pyramids, gate partials: this pipeline's own,
                                           // so its captured graph never points at another's buffers
     uint64_t total_extracted = 0;
@@ -184,14 +185,18 @@ int integrate_host_pose(kfx_pipeline* p) {
     const kfx_matrix_3f m = to_m3f(Ri);
     const kfx_point_3f tt = to_p3f(p->t);
     KFX_TRY(integrate_tsdf_ws(p->ws, true, p->raw, p->scaled, p->vol, &p->lvl[0], &m, &tt, p->cfg.truncation,
-                              &p->cell, &p->rb));
+                              &p->cell, &p->rb, p->bricks));
     return step_sync(p);
 }
 
 int raycast_host_pose(kfx_pipeline* p) {
     const kfx_matrix_3f m = to_m3f(p->R);
     const kfx_point_3f tt = to_p3f(p->t);
-    KFX_TRY(kfx_raycast(p->vol, p->vp[0], p->np[0], &p->rb, &tt, &m, p->cfg.truncation, &p->lvl[0], &p->cell));
+    if (p->bricks)
+        KFX_TRY(kfx_raycast_bricks(p->vol, p->vp[0], p->np[0], &p->rb, &tt, &m, p->cfg.truncation, &p->lvl[0],
+                                   &p->cell, p->bricks));
+    else
+        KFX_TRY(kfx_raycast(p->vol, p->vp[0], p->np[0], &p->rb, &tt, &m, p->cfg.truncation, &p->lvl[0], &p->cell));
     KFX_TRY(step_sync(p));
     return resize_prev(p);
 }
@@ -256,9 +261,14 @@ int map_part_skip(kfx_pipeline* p, const int32_t* skip) {
     const float *dpose = nullptr, *dprev = nullptr;
     KFX_TRY(kfx_icp_pose_device(p->icp, &dpose, &dprev));
     KFX_TRY(integrate_tsdf_posed_ws(p->ws, p->raw, p->scaled, p->vol, &p->lvl[0], dpose, dprev,
-                                    p->cfg.integrate_threshold, p->cfg.truncation, &p->cell, &p->rb, skip));
-    KFX_TRY(kfx_raycast_posed(p->vol, p->vp[0], p->np[0], &p->rb, dpose, p->cfg.truncation, &p->lvl[0], &p->cell,
-                              skip));
+                                    p->cfg.integrate_threshold, p->cfg.truncation, &p->cell, &p->rb, skip,
+                                    p->bricks));
+    if (p->bricks)
+        KFX_TRY(raycast_posed_bricks(p->vol, p->vp[0], p->np[0], &p->rb, dpose, p->cfg.truncation, &p->lvl[0],
+                                     &p->cell, skip, p->bricks));
+    else
+        KFX_TRY(kfx_raycast_posed(p->vol, p->vp[0], p->np[0], &p->rb, dpose, p->cfg.truncation, &p->lvl[0],
+                                  &p->cell, skip));
     if (p->cvol)
         KFX_TRY(integrate_colors_posed_ws(p->ws, p->cvol, p->rgb, p->vp[0], &p->lvl[0], dpose, p->cfg.truncation,
                                           p->cfg.color_max_weight, &p->cell, &p->rb, skip));
@@ -403,6 +413,9 @@ int maybe_shift(kfx_pipeline* p, kfx_frame_info* info) {
     }
     if (p->cfg.mode == KFX_PIPELINE_FUSED)
         KFX_TRY(kfx_icp_translate(p->icp, -s[0] * cs[0], -s[1] * cs[1], -s[2] * cs[2]));
+    // The brick flags are in local voxel coordinates, which moved with the
+    // origin: rebuilt from the (cleaned) volume.
+    if (p->bricks) KFX_TRY(brick_flags_build(p->vol, &p->rb, p->bricks));
     // The previous-frame maps must be in the new volume coordinates.
     KFX_TRY(raycast_host_pose(p));
     drop_graph(p);
@@ -469,6 +482,7 @@ int kfx_pipeline_destroy(kfx_pipeline* p) {
     if (p->hstats) (void)hipHostFree(p->hstats);
     (void)hipFree(p->dgate);
     if (p->hgate) (void)hipHostFree(p->hgate);
+    (void)hipFree(p->bricks);
     kfx_icp_destroy(p->icp);
     kfx_slicer_destroy(p->slicer);
     workspace_free(p->ws);
@@ -528,6 +542,12 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
     if (e == hipSuccess) e = hipHostMalloc((void**)&p->hstats, sizeof(double) * 4, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void**)&p->dgate, sizeof(int32_t));
     if (e == hipSuccess) e = hipHostMalloc((void**)&p->hgate, sizeof(int32_t), hipHostMallocDefault);
+    // Brick flags for the skipping raycast in fused mode (plain mode marches
+    // every sample, as the reference does); the volume starts empty, so no
+    // brick holds a negative value.
+    const size_t nbricks = brick_flag_bytes(rb.voxels_size);
+    if (e == hipSuccess && cfg->mode == KFX_PIPELINE_FUSED && nbricks > 0) e = hipMalloc((void**)&p->bricks, nbricks);
+    if (e == hipSuccess && p->bricks) e = hipMemset(p->bricks, 0, nbricks);
     if (e != hipSuccess) {
         kfx_pipeline_destroy(p);
         return set_error(KFX_ERR_HIP, "kfx_pipeline_create: %s", hipGetErrorString(e));
@@ -677,6 +697,7 @@ int kfx_pipeline_state(kfx_pipeline* p, kfx_pipeline_view* v) {
     v->shifts = p->shifts;
     v->fallbacks = p->fallbacks;
     v->extracted_points = p->total_extracted;
+    v->brick_flags = p->bricks;
     v->last_points = p->cfg.extract_records ? nullptr : p->ext_pts;
     v->last_records = p->cfg.extract_records ? reinterpret_cast<kfx_slice_record*>(p->ext_pts) : nullptr;
     v->last_colors = p->ext_col;

@@ -28,6 +28,11 @@ struct RaycastParams {
     int cols, rows;
     float time_step;
     kfx_round_buffer rb;
+    // Brick flags for the skipping march (null: the full march): one byte per
+    // kRayBrick^3 brick of local voxel coordinates, x fastest (nbx x nby x
+    // nbz bricks), non-zero when the brick may hold a negative value.
+    const uint8_t* bricks;
+    int nbx, nby;
 };
 
 struct f3 { float x, y, z; };
@@ -95,15 +100,25 @@ __device__ __forceinline__ int voxel_floor(float p, float cell, float inv_cell) 
     return (int)q;
 }
 
-// interpolateTrilineary (raycast.cl:95-145): NaN outside [1, N-2].
-__device__ float interpolate_trilinear(const kfx_tsdf* __restrict__ vol, f3 pt, const RaycastParams& p) {
+// interpolateTrilineary (raycast.cl:95-145): NaN outside [1, N-2].  In three
+// parts, so that a caller can issue the loads of several evaluations before
+// using any: tri_setup (the cell and the weights; ok false where the
+// reference returns NaN), tri_load (the eight taps; address 0 when !ok) and
+// tri_eval.
+struct TriCell {
+    bool ok;
+    uint32_t x0, x1, r00, r01, r10, r11;  // tap addresses: r.. + x.
+    float a, b, c;
+};
+
+__device__ __forceinline__ TriCell tri_setup(f3 pt, const RaycastParams& p) {
     const kfx_round_buffer& rb = p.rb;
+    TriCell t;
     int gx = voxel_floor(pt.x, p.csx, p.icsx);
     int gy = voxel_floor(pt.y, p.csy, p.icsy);
     int gz = voxel_floor(pt.z, p.csz, p.icsz);
-    if (gx <= 0 || gx >= rb.voxels_size.x - 1) return NAN;
-    if (gy <= 0 || gy >= rb.voxels_size.y - 1) return NAN;
-    if (gz <= 0 || gz >= rb.voxels_size.z - 1) return NAN;
+    t.ok = !(gx <= 0 || gx >= rb.voxels_size.x - 1 || gy <= 0 || gy >= rb.voxels_size.y - 1 || gz <= 0 ||
+             gz >= rb.voxels_size.z - 1);
     const float vx = ((float)gx + 0.5f) * p.csx;
     const float vy = ((float)gy + 0.5f) * p.csy;
     const float vz = ((float)gz + 0.5f) * p.csz;
@@ -116,26 +131,246 @@ __device__ float interpolate_trilinear(const kfx_tsdf* __restrict__ vol, f3 pt, 
     if (b < 0.0f) { gy--; b += 1.0f; }
     float c = div_cell(pt.z, p.csz, p.icsz) - ((float)gz + 0.5f);
     if (c < 0.0f) { gz--; c += 1.0f; }
-    const float t000 = read_tsdf(vol, rb, gx + 0, gy + 0, gz + 0);
-    const float t001 = read_tsdf(vol, rb, gx + 0, gy + 0, gz + 1);
-    const float t010 = read_tsdf(vol, rb, gx + 0, gy + 1, gz + 0);
-    const float t011 = read_tsdf(vol, rb, gx + 0, gy + 1, gz + 1);
-    const float t100 = read_tsdf(vol, rb, gx + 1, gy + 0, gz + 0);
-    const float t101 = read_tsdf(vol, rb, gx + 1, gy + 0, gz + 1);
-    const float t110 = read_tsdf(vol, rb, gx + 1, gy + 1, gz + 0);
-    const float t111 = read_tsdf(vol, rb, gx + 1, gy + 1, gz + 1);
+    t.a = a; t.b = b; t.c = c;
+    // The taps gx .. gx + 1 etc. lie inside [0, N) when ok: their wrapped
+    // coordinates, with the wraps taken once per axis (cyclic_address).
+    const uint32_t nx = (uint32_t)rb.voxels_size.x, ny = (uint32_t)rb.voxels_size.y;
+    const uint32_t nz = (uint32_t)rb.voxels_size.z;
+    uint32_t x0 = (uint32_t)gx + (uint32_t)rb.origin_GRID.x, y0 = (uint32_t)gy + (uint32_t)rb.origin_GRID.y;
+    uint32_t z0 = (uint32_t)gz + (uint32_t)rb.origin_GRID.z;
+    x0 = min(x0, x0 - nx);
+    y0 = min(y0, y0 - ny);
+    z0 = min(z0, z0 - nz);
+    const uint32_t x1 = x0 + 1 == nx ? 0u : x0 + 1, y1 = y0 + 1 == ny ? 0u : y0 + 1, z1 = z0 + 1 == nz ? 0u : z0 + 1;
+    t.x0 = t.ok ? x0 : 0u;
+    t.x1 = t.ok ? x1 : 0u;
+    t.r00 = t.ok ? (z0 * ny + y0) * nx : 0u;
+    t.r01 = t.ok ? (z1 * ny + y0) * nx : 0u;
+    t.r10 = t.ok ? (z0 * ny + y1) * nx : 0u;
+    t.r11 = t.ok ? (z1 * ny + y1) * nx : 0u;
+    return t;
+}
+
+// v = {000, 001, 010, 011, 100, 101, 110, 111} (x, y, z bits).
+__device__ __forceinline__ void tri_load(const kfx_tsdf* __restrict__ vol, const TriCell& t, int16_t (&v)[8]) {
+    v[0] = vol[t.r00 + t.x0].value;
+    v[1] = vol[t.r01 + t.x0].value;
+    v[2] = vol[t.r10 + t.x0].value;
+    v[3] = vol[t.r11 + t.x0].value;
+    v[4] = vol[t.r00 + t.x1].value;
+    v[5] = vol[t.r01 + t.x1].value;
+    v[6] = vol[t.r10 + t.x1].value;
+    v[7] = vol[t.r11 + t.x1].value;
+}
+
+__device__ __forceinline__ float tri_eval(const TriCell& t, const int16_t (&v)[8]) {
+    if (!t.ok) return NAN;
+    const float a = t.a, b = t.b, c = t.c;
+    const float t000 = tsdf_decode(v[0]), t001 = tsdf_decode(v[1]), t010 = tsdf_decode(v[2]);
+    const float t011 = tsdf_decode(v[3]), t100 = tsdf_decode(v[4]), t101 = tsdf_decode(v[5]);
+    const float t110 = tsdf_decode(v[6]), t111 = tsdf_decode(v[7]);
     return (1.0f - a) * ((1.0f - b) * (t000 * (1.0f - c) + t001 * c) + b * (t010 * (1.0f - c) + t011 * c)) +
            a * ((1.0f - b) * (t100 * (1.0f - c) + t101 * c) + b * (t110 * (1.0f - c) + t111 * c));
+}
+
+__device__ float interpolate_trilinear(const kfx_tsdf* __restrict__ vol, f3 pt, const RaycastParams& p) {
+    const TriCell t = tri_setup(pt, p);
+    if (!t.ok) return NAN;
+    int16_t v[8];
+    tri_load(vol, t, v);
+    return tri_eval(t, v);
 }
 
 __device__ __forceinline__ f3 ray_at(f3 o, f3 d, float t) {
     return f3{o.x + d.x * t, o.y + d.y * t, o.z + d.z * t};
 }
 
+// The refinement of a + to - crossing between march times t_cur and t_next
+// (raycast.cl:291-331) and the normal (raycast.cl:332-404): vertex and normal
+// outputs, unchanged (NaN) where a trilinear sample leaves [1, N-2]; with
+// kCount, the reads made are added to ns.
+template <bool kCount>
+__device__ __forceinline__ void raycast_hit(const kfx_tsdf* __restrict__ vol, const RaycastParams& p, f3 start,
+                                            f3 dir, float hit_tc, float hit_tn, kfx_xyz& vout, kfx_xyz& nout,
+                                            uint32_t& ns) {
+    const kfx_round_buffer& rb = p.rb;
+    const float step = p.time_step;
+    const int nxv = rb.voxels_size.x, nyv = rb.voxels_size.y, nzv = rb.voxels_size.z;
+    const float t_cur = hit_tc;
+    // The two samples of the crossing: both cells' taps are loaded together.
+    const TriCell c1 = tri_setup(ray_at(start, dir, hit_tn), p);
+    const TriCell c2 = tri_setup(ray_at(start, dir, t_cur), p);
+    int16_t v1[8], v2[8];
+    tri_load(vol, c1, v1);
+    tri_load(vol, c2, v2);
+    const float ftdt = tri_eval(c1, v1);
+    if (isnan(ftdt)) return;
+    if (kCount) ns += 8;
+    const float ft = tri_eval(c2, v2);
+    if (isnan(ft)) return;
+    if (kCount) ns += 8;
+    const float ts = t_cur - step * ft / (ftdt - ft);
+    const f3 vert = ray_at(start, dir, ts);
+    vout = kfx_xyz{vert.x, vert.y, vert.z};
+
+    const f3 q3 = ray_at(start, dir, t_cur);
+    const int hx = voxel_floor(q3.x, p.csx, p.icsx), hy = voxel_floor(q3.y, p.csy, p.icsy),
+              hz = voxel_floor(q3.z, p.csz, p.icsz);
+    if (!(hx > 1 && hy > 1 && hz > 1 && hx < nxv - 2 && hy < nyv - 2 && hz < nzv - 2)) return;
+    // The six central-difference samples one cell apart: 48 taps loaded
+    // together.
+    TriCell cn[6];
+    {
+        f3 tt = vert; tt.x += p.csx; cn[0] = tri_setup(tt, p);
+        tt = vert; tt.x -= p.csx; cn[1] = tri_setup(tt, p);
+        tt = vert; tt.y += p.csy; cn[2] = tri_setup(tt, p);
+        tt = vert; tt.y -= p.csy; cn[3] = tri_setup(tt, p);
+        tt = vert; tt.z += p.csz; cn[4] = tri_setup(tt, p);
+        tt = vert; tt.z -= p.csz; cn[5] = tri_setup(tt, p);
+    }
+    int16_t vn[6][8];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) tri_load(vol, cn[i], vn[i]);
+    float f[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) f[i] = tri_eval(cn[i], vn[i]);
+    if (kCount)
+        ns += 8u * (uint32_t)(!isnan(f[0]) + !isnan(f[1]) + !isnan(f[2]) + !isnan(f[3]) + !isnan(f[4]) +
+                              !isnan(f[5]));
+    const f3 n = normalized3(f3{f[0] - f[1], f[2] - f[3], f[4] - f[5]});
+    nout = kfx_xyz{n.x, n.y, n.z};
+}
+
+// The skipping march: the reference's samples (the same times, the same
+// voxels) and the same outcome, without loading the samples that cannot
+// change it.  The march stops at the first pair of consecutive samples with
+// opposite signs (- to +: no hit; + to -: the crossing), so a sample that is
+// not negative can only matter next to a negative one.  In a brick whose
+// flag is clear (no negative value) no sample is loaded: the first sample
+// in it is located exactly (and loaded only after a negative sample: a
+// positive value ends the march), then every later sample whose time lies
+// in the brick shrunk by kRayBrickMargin metres on each side (certainly
+// inside it, whatever the rounding of its position) is stepped over with
+// the running time sum alone.  When a negative sample follows unloaded
+// samples, the last of them is loaded to decide the crossing.  In a flagged
+// brick the samples go kMarchBatch at a time, loaded together.
+constexpr int kRayBrickShift = 4;  // bricks of 16^3 voxels
+constexpr int kRayBrick = 1 << kRayBrickShift;
+constexpr float kRayBrickMargin = 1e-4f;
+
+__device__ __forceinline__ bool march_skip(const kfx_tsdf* __restrict__ vol, const RaycastParams& p, f3 start,
+                                           f3 dir, float t, int tsdf, float& hit_tc, float& hit_tn) {
+    const kfx_round_buffer& rb = p.rb;
+    const f3 vol_m{rb.volume_size.x, rb.volume_size.y, rb.volume_size.z};
+    const float max_time = 3.0f * (vol_m.x + vol_m.y + vol_m.z);
+    const float step = p.time_step;
+    const float fnx = (float)rb.voxels_size.x, fny = (float)rb.voxels_size.y, fnz = (float)rb.voxels_size.z;
+    const f3 idir{1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
+    const f3 bsz{(float)kRayBrick * p.csx, (float)kRayBrick * p.csy, (float)kRayBrick * p.csz};
+    bool known = true;  // tsdf holds the previous sample's value (else it is >= 0, at time t_unk)
+    float t_unk = 0.0f;
+    for (;;) {
+        if (!(t < max_time)) return false;
+        const float tn = t + step;
+        const float fx = floorf(div_cell(start.x + dir.x * tn, p.csx, p.icsx));
+        const float fy = floorf(div_cell(start.y + dir.y * tn, p.csy, p.icsy));
+        const float fz = floorf(div_cell(start.z + dir.z * tn, p.csz, p.icsz));
+        if (!((fx >= 0.0f) & (fx < fnx) & (fy >= 0.0f) & (fy < fny) & (fz >= 0.0f) & (fz < fnz))) return false;
+        const uint32_t gx = (uint32_t)(int)fx, gy = (uint32_t)(int)fy, gz = (uint32_t)(int)fz;
+        const uint32_t bx = gx >> kRayBrickShift, by = gy >> kRayBrickShift, bz = gz >> kRayBrickShift;
+        const uint32_t addr = march_address(gx, gy, gz, rb);
+        if (p.bricks[(bz * (uint32_t)p.nby + by) * (uint32_t)p.nbx + bx] == 0) {
+            // This sample is not negative; loaded only after a negative one
+            // (a positive value ends the march there).
+            if (known && tsdf < 0) {
+                const int cur = vol[addr].value;
+                if (cur > 0) return false;  // - to +
+                tsdf = cur;                 // 0
+            } else {
+                known = false;
+            }
+            // Samples certainly inside the brick: times in [t_lo, t_hi).
+            const float lx = (float)bx * bsz.x + kRayBrickMargin, hx = (float)(bx + 1) * bsz.x - kRayBrickMargin;
+            const float ly = (float)by * bsz.y + kRayBrickMargin, hy = (float)(by + 1) * bsz.y - kRayBrickMargin;
+            const float lz = (float)bz * bsz.z + kRayBrickMargin, hz = (float)(bz + 1) * bsz.z - kRayBrickMargin;
+            const float ax = (lx - start.x) * idir.x, cx = (hx - start.x) * idir.x;
+            const float ay = (ly - start.y) * idir.y, cy = (hy - start.y) * idir.y;
+            const float az = (lz - start.z) * idir.z, cz = (hz - start.z) * idir.z;
+            const float t_lo = fmaxf(fmaxf(fminf(ax, cx), fminf(ay, cy)), fminf(az, cz));
+            const float t_hi = fminf(fminf(fmaxf(ax, cx), fmaxf(ay, cy)), fmaxf(az, cz));
+            float tt = tn;
+            for (;;) {
+                const float tnn = tt + step;
+                if (!(tnn >= t_lo && tnn < t_hi)) break;
+                tt = tnn;
+                known = false;
+            }
+            if (!known) t_unk = tt;
+            t = tt;
+            continue;
+        }
+        // A flagged brick: this sample and the next kMarchBatch - 1 are
+        // loaded together (up to the first that leaves the volume or the
+        // time range), then tested in order, as the full march does.
+        float tcs[kMarchBatch], tns[kMarchBatch];
+        uint32_t addrs[kMarchBatch];
+        bool ok[kMarchBatch];
+        tcs[0] = t;
+        tns[0] = tn;
+        addrs[0] = addr;
+        ok[0] = true;
+        {
+            float tt = tn;
+            bool alive = true;
+#pragma unroll
+            for (int k = 1; k < kMarchBatch; ++k) {
+                tcs[k] = tt;
+                tns[k] = tt + step;
+                const float qx = floorf(div_cell(start.x + dir.x * tns[k], p.csx, p.icsx));
+                const float qy = floorf(div_cell(start.y + dir.y * tns[k], p.csy, p.icsy));
+                const float qz = floorf(div_cell(start.z + dir.z * tns[k], p.csz, p.icsz));
+                alive = alive & (tt < max_time) & (qx >= 0.0f) & (qx < fnx) & (qy >= 0.0f) & (qy < fny) &
+                        (qz >= 0.0f) & (qz < fnz);
+                ok[k] = alive;
+                addrs[k] = alive ? march_address((uint32_t)(int)qx, (uint32_t)(int)qy, (uint32_t)(int)qz, rb) : 0u;
+                tt = tns[k];
+            }
+        }
+        int val[kMarchBatch];
+#pragma unroll
+        for (int k = 0; k < kMarchBatch; ++k) val[k] = ok[k] ? vol[addrs[k]].value : 0;
+        if (!known) {
+            // The samples before were not loaded (not negative): the last of
+            // them is needed only when this one is negative.
+            tsdf = 0;
+            if (val[0] < 0) {
+                const uint32_t ux = (uint32_t)(int)floorf(div_cell(start.x + dir.x * t_unk, p.csx, p.icsx));
+                const uint32_t uy = (uint32_t)(int)floorf(div_cell(start.y + dir.y * t_unk, p.csy, p.icsy));
+                const uint32_t uz = (uint32_t)(int)floorf(div_cell(start.z + dir.z * t_unk, p.csz, p.icsz));
+                tsdf = vol[march_address(ux, uy, uz, rb)].value;
+            }
+            known = true;
+        }
+#pragma unroll
+        for (int k = 0; k < kMarchBatch; ++k) {
+            if (!ok[k]) return false;  // left the volume or the time range
+            const int prev = tsdf;
+            tsdf = val[k];
+            if (prev < 0 && tsdf > 0) return false;
+            if (prev > 0 && tsdf < 0) {
+                hit_tc = tcs[k];
+                hit_tn = tns[k];
+                return true;
+            }
+        }
+        t = tns[kMarchBatch - 1];
+    }
+}
+
 // kCount: also count the algorithm's voxel reads (first sample, march
 // samples inside the volume, 8 per in-bounds trilinear evaluation) into
 // *count, for the raycast's algorithmic bytes; results are unchanged.
-template <bool kDevicePose, bool kCount = false>
+template <bool kDevicePose, bool kCount = false, bool kSkip = false>
 __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict__ vol,
                                                       kfx_xyz* __restrict__ vmap,
                                                       kfx_xyz* __restrict__ nmap, RaycastParams p,
@@ -213,7 +448,8 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
         // then, which equals the saturating convert_int_sat_rtn test.
         bool found = false;
         float hit_tc = 0.0f, hit_tn = 0.0f;
-        for (bool done = false; !done;) {
+        if constexpr (kSkip) found = march_skip(vol, p, start, dir, time_curr, tsdf, hit_tc, hit_tn);
+        for (bool done = kSkip; !done;) {
             float tc[kMarchBatch], tn[kMarchBatch];
             bool ok[kMarchBatch];
             uint32_t addr[kMarchBatch];
@@ -253,51 +489,7 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
             }
             time_curr = t;
         }
-        if (found) {
-            do {
-                const float t_cur = hit_tc;
-                const f3 q1 = ray_at(start, dir, hit_tn);
-                const float ftdt = interpolate_trilinear(vol, q1, p);
-                if (isnan(ftdt)) break;
-                if (kCount) ns += 8;
-                const f3 q2 = ray_at(start, dir, t_cur);
-                const float ft = interpolate_trilinear(vol, q2, p);
-                if (isnan(ft)) break;
-                if (kCount) ns += 8;
-                const float ts = t_cur - step * ft / (ftdt - ft);
-                const f3 vert = ray_at(start, dir, ts);
-                vout = kfx_xyz{vert.x, vert.y, vert.z};
-
-                const f3 q3 = ray_at(start, dir, t_cur);
-                const int hx = voxel_floor(q3.x, p.csx, p.icsx), hy = voxel_floor(q3.y, p.csy, p.icsy),
-                          hz = voxel_floor(q3.z, p.csz, p.icsz);
-                if (hx > 1 && hy > 1 && hz > 1 && hx < nxv - 2 && hy < nyv - 2 && hz < nzv - 2) {
-                    f3 tt = vert;
-                    tt.x += p.csx;
-                    const float fx1 = interpolate_trilinear(vol, tt, p);
-                    tt = vert;
-                    tt.x -= p.csx;
-                    const float fx2 = interpolate_trilinear(vol, tt, p);
-                    tt = vert;
-                    tt.y += p.csy;
-                    const float fy1 = interpolate_trilinear(vol, tt, p);
-                    tt = vert;
-                    tt.y -= p.csy;
-                    const float fy2 = interpolate_trilinear(vol, tt, p);
-                    tt = vert;
-                    tt.z += p.csz;
-                    const float fz1 = interpolate_trilinear(vol, tt, p);
-                    tt = vert;
-                    tt.z -= p.csz;
-                    const float fz2 = interpolate_trilinear(vol, tt, p);
-                    if (kCount)
-                        ns += 8u * (uint32_t)(!isnan(fx1) + !isnan(fx2) + !isnan(fy1) + !isnan(fy2) + !isnan(fz1) +
-                                              !isnan(fz2));
-                    const f3 n = normalized3(f3{fx1 - fx2, fy1 - fy2, fz1 - fz2});
-                    nout = kfx_xyz{n.x, n.y, n.z};
-                }
-            } while (false);
-        }
+        if (found) raycast_hit<kCount>(vol, p, start, dir, hit_tc, hit_tn, vout, nout, ns);
         if (kCount) atomicAdd(count, (unsigned long long)ns);
     }
     vmap[idx] = vout;
@@ -305,6 +497,73 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
 }
 
 int check_round_buffer(const kfx_round_buffer* rb, const char* who);
+
+// Exact brick flags: a workgroup takes the 16 x 16 rows (y, z) of one
+// brick row (16 local y by 16 local z, every x), reading each row from
+// memory in order as 16-byte quads (a row is contiguous; the cyclic offset
+// only rotates it), and sets the bits of the x bricks whose voxels hold a
+// negative value; the workgroup's bits then become the bricks' flags.
+constexpr int kFlagThreads = 256;
+__global__ void __launch_bounds__(kFlagThreads) brick_flags_kernel(const kfx_tsdf* __restrict__ vol,
+                                                                   kfx_round_buffer rb,
+                                                                   uint8_t* __restrict__ bricks) {
+    __shared__ uint32_t words[kFlagThreads / 64][2];
+    const int nx = rb.voxels_size.x, ny = rb.voxels_size.y, nz = rb.voxels_size.z;
+    const int nbx = nx >> kRayBrickShift;  // at most 64 (the host checks nx <= 1024)
+    const int by = blockIdx.x, bz = blockIdx.y;
+    const int quads = nx / 4;
+    const uint32_t ox = (uint32_t)rb.origin_GRID.x;
+    uint64_t mask = 0;
+    for (int i = threadIdx.x; i < kRayBrick * kRayBrick * quads; i += kFlagThreads) {
+        const int q = i % quads, row = i / quads;
+        const int y = by * kRayBrick + row % kRayBrick, z = bz * kRayBrick + row / kRayBrick;
+        const uint32_t py = (uint32_t)wrap_once(y + rb.origin_GRID.y, ny), pz = (uint32_t)wrap_once(z + rb.origin_GRID.z, nz);
+        const uint4 u = *reinterpret_cast<const uint4*>(vol + ((size_t)pz * (size_t)ny + py) * (size_t)nx + 4 * q);
+        const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if ((int16_t)(w[k] & 0xffffu) < 0) {
+                // physical x 4 q + k is local x (4 q + k - ox) mod nx
+                uint32_t lx = (uint32_t)(4 * q + k) - ox;
+                lx = min(lx, lx + (uint32_t)nx);
+                mask |= 1ull << (lx >> kRayBrickShift);
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) mask |= __shfl_xor(mask, off);
+    if ((threadIdx.x & 63) == 0) {
+        words[threadIdx.x >> 6][0] = (uint32_t)mask;
+        words[threadIdx.x >> 6][1] = (uint32_t)(mask >> 32);
+    }
+    __syncthreads();
+    if (threadIdx.x < nbx) {
+        const int h = threadIdx.x >> 5, b = threadIdx.x & 31;
+        uint32_t m = 0;
+        for (int wv = 0; wv < kFlagThreads / 64; ++wv) m |= words[wv][h];
+        bricks[((size_t)bz * (size_t)(ny >> kRayBrickShift) + by) * (size_t)nbx + threadIdx.x] = (m >> b) & 1u;
+    }
+}
+
+size_t brick_flag_bytes(const kfx_point_3i& v) {
+    if (v.x <= 0 || v.y <= 0 || v.z <= 0 || v.x % kRayBrick || v.y % kRayBrick || v.z % kRayBrick) return 0;
+    return (size_t)(v.x / kRayBrick) * (size_t)(v.y / kRayBrick) * (size_t)(v.z / kRayBrick);
+}
+
+int brick_flags_build(const kfx_tsdf* volume, const kfx_round_buffer* rb, uint8_t* bricks) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(volume && bricks, "kfx_brick_flags_build: null pointer");
+    int rc = check_round_buffer(rb, "kfx_brick_flags_build");
+    if (rc) return rc;
+    const size_t n = brick_flag_bytes(rb->voxels_size);
+    KFX_REQUIRE(n > 0, "kfx_brick_flags_build: volume axes must be multiples of %d", kRayBrick);
+    KFX_REQUIRE(rb->voxels_size.x <= 64 * kRayBrick, "kfx_brick_flags_build: at most %d voxels along x",
+                64 * kRayBrick);
+    ProfScope prof("brick_flags");
+    dim3 grid(rb->voxels_size.y >> kRayBrickShift, rb->voxels_size.z >> kRayBrickShift);
+    brick_flags_kernel<<<grid, kFlagThreads, 0, stream()>>>(volume, *rb, bricks);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
 
 }  // namespace kfx
 
@@ -314,7 +573,7 @@ static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* 
                           const kfx_round_buffer* rb, const kfx_point_3f* translation,
                           const kfx_matrix_3f* rotation, const float* dev_pose, float truncation_distance,
                           const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip,
-                          unsigned long long* dev_count = nullptr) {
+                          unsigned long long* dev_count = nullptr, const uint8_t* bricks = nullptr) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(volume && vertex_map && normal_map && translation && rotation && intr && cell_size,
                 "kfx_raycast: null pointer");
@@ -337,10 +596,21 @@ static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* 
     // taken in double as a C compiler would with the double literal.
     p.time_step = (float)((double)truncation_distance * 0.8);
     p.rb = *rb;
+    p.bricks = bricks;
+    p.nbx = rb->voxels_size.x >> kRayBrickShift;
+    p.nby = rb->voxels_size.y >> kRayBrickShift;
+    KFX_REQUIRE(bricks == nullptr || brick_flag_bytes(rb->voxels_size) > 0,
+                "kfx_raycast: brick flags need volume axes that are multiples of %d", kRayBrick);
     ProfScope prof("raycast");
     dim3 block(64, 4);
     dim3 grid(divup(intr->cols, 64), divup(intr->rows, 4));
-    if (dev_count)
+    if (bricks != nullptr && dev_pose)
+        raycast_kernel<true, false, true><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, dev_pose,
+                                                                        dev_skip);
+    else if (bricks != nullptr)
+        raycast_kernel<false, false, true><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, nullptr,
+                                                                         nullptr);
+    else if (dev_count)
         raycast_kernel<false, true><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, nullptr, nullptr,
                                                                   dev_count);
     else if (dev_pose)
@@ -367,6 +637,36 @@ extern "C" int kfx_raycast_posed(const kfx_tsdf* volume, kfx_xyz* vertex_map, kf
     const kfx_point_3f zero = {0, 0, 0};
     return raycast_launch(volume, vertex_map, normal_map, rb, &zero, &ident, dev_pose, truncation_distance, intr,
                           cell_size, dev_skip);
+}
+
+int kfx::raycast_posed_bricks(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                              const kfx_round_buffer* rb, const float* dev_pose, float truncation_distance,
+                              const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip,
+                              const uint8_t* bricks) {
+    KFX_REQUIRE(dev_pose != nullptr, "kfx_raycast_posed: null pose");
+    const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    const kfx_point_3f zero = {0, 0, 0};
+    return raycast_launch(volume, vertex_map, normal_map, rb, &zero, &ident, dev_pose, truncation_distance, intr,
+                          cell_size, dev_skip, nullptr, bricks);
+}
+
+extern "C" int kfx_brick_flags_size(const kfx_round_buffer* rb, size_t* bytes) {
+    KFX_REQUIRE(rb && bytes, "kfx_brick_flags_size: null pointer");
+    *bytes = brick_flag_bytes(rb->voxels_size);
+    return KFX_OK;
+}
+
+extern "C" int kfx_brick_flags_build(const kfx_tsdf* volume, const kfx_round_buffer* rb, uint8_t* bricks) {
+    return brick_flags_build(volume, rb, bricks);
+}
+
+extern "C" int kfx_raycast_bricks(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                                  const kfx_round_buffer* rb, const kfx_point_3f* translation,
+                                  const kfx_matrix_3f* rotation, float truncation_distance, const kfx_intrinsics* intr,
+                                  const kfx_point_3f* cell_size, const uint8_t* bricks) {
+    KFX_REQUIRE(bricks != nullptr, "kfx_raycast_bricks: null brick flags");
+    return raycast_launch(volume, vertex_map, normal_map, rb, translation, rotation, nullptr, truncation_distance,
+                          intr, cell_size, nullptr, nullptr, bricks);
 }
 
 extern "C" int kfx_raycast_count_samples(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,

@@ -73,6 +73,11 @@ struct IntegrateParams {
     // at bound + bound_off[l]; level 0 is the scaled depth itself.
     const float* bound;
     int bound_off[kBoundLevels + 1];
+    // Brick flags the raycast's skipping march reads (null: none kept): one
+    // byte per 16^3 brick of local voxel coordinates (nbx x nby x ...),
+    // set when the rule writes a negative value into the brick.
+    uint8_t* bricks;
+    int nbx, nby;
 };
 
 // The pose-dependent terms of IntegrateParams, kept apart so that a kernel
@@ -686,8 +691,21 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
             const float a = tsdf_decode(v_old) * bq.fw + tsdf_new;
             const float q0 = a * bq.y;
             const float avg = fmaf(fmaf(-q0, bq.fb, a), bq.y, q0);
-            const uint32_t out = bq.w_new_hi | (uint32_t)(uint16_t)tsdf_encode_unit(avg);
+            const int16_t v_new = tsdf_encode_unit(avg);
+            const uint32_t out = bq.w_new_hi | (uint32_t)(uint16_t)v_new;
             if (out != word[k]) __builtin_amdgcn_raw_buffer_store_b32(out, slice_rsrc(zbase[k], slice_bytes), col4, 0, 0);
+            if (p.bricks != nullptr) {
+                // The first lane of each 16-column group (one brick) that
+                // writes a negative value sets the brick's flag.
+                const bool neg = v_new < 0;
+                const uint64_t m = __builtin_amdgcn_ballot_w64(neg);
+                const int lane = threadIdx.x & 63;
+                const uint32_t grp = (uint32_t)(m >> (lane & ~15)) & 0xffffu;
+                if (neg && (lane & 15) == __builtin_ctz(grp)) {
+                    const uint32_t z = (uint32_t)(z0 + k);
+                    p.bricks[((z >> 4) * (uint32_t)p.nby + ((uint32_t)y >> 4)) * (uint32_t)p.nbx + ((uint32_t)x >> 4)] = 1;
+                }
+            }
         }
     }
 }
@@ -1070,6 +1088,9 @@ static int sweep_setup(const char* who, const kfx_intrinsics* intr, const kfx_ma
         }
     }
     p.bound = nullptr;
+    p.bricks = nullptr;
+    p.nbx = p.nx >> 4;
+    p.nby = p.ny >> 4;
     return KFX_OK;
 }
 
@@ -1258,12 +1279,13 @@ static int colors_check(const char* who, kfx_color* color_volume, const kfx_colo
 int integrate_tsdf_ws(const Workspace& ws, bool profile, const uint16_t* depth, float* depth_scaled,
                       kfx_tsdf* volume, const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
                       const kfx_point_3f* translation, float truncation_distance, const kfx_point_3f* cell_size,
-                      const kfx_round_buffer* rb) {
+                      const kfx_round_buffer* rb, uint8_t* bricks) {
     KFX_REQUIRE(volume != nullptr, "kfx_integrate_tsdf: null volume");
     IntegrateParams p;
     int rc = integrate_setup("kfx_integrate_tsdf", depth, depth_scaled, intr, rotation_inverse, translation,
                              truncation_distance, cell_size, rb, p, false);
     if (rc) return rc;
+    p.bricks = bricks;
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
     const dim3 grid = sweep_grid(p.nx, p.ny, p.nz, p.zrun);
     const uint8_t* flags = nullptr;
@@ -1278,7 +1300,7 @@ int integrate_tsdf_ws(const Workspace& ws, bool profile, const uint16_t* depth, 
 int integrate_tsdf_posed_ws(const Workspace& ws, const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
                             const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
                             float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
-                            const kfx_round_buffer* rb, const int32_t* dev_skip) {
+                            const kfx_round_buffer* rb, const int32_t* dev_skip, uint8_t* bricks) {
     KFX_REQUIRE(volume != nullptr && dev_pose != nullptr, "kfx_integrate_tsdf_posed: null pointer");
     const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
     const kfx_point_3f zero = {0, 0, 0};
@@ -1286,6 +1308,7 @@ int integrate_tsdf_posed_ws(const Workspace& ws, const uint16_t* depth, float* d
     int rc = integrate_setup("kfx_integrate_tsdf_posed", depth, depth_scaled, intr, &ident, &zero,
                              truncation_distance, cell_size, rb, p, false);
     if (rc) return rc;
+    p.bricks = bricks;
     dim3 block(kIntegrateBlockX, kIntegrateBlockY);
     const dim3 grid = sweep_grid(p.nx, p.ny, p.nz, p.zrun);
     const uint8_t* flags = nullptr;

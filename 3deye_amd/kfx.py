@@ -137,7 +137,7 @@ class PipelineView(C.Structure):
                 ("nmap_current", C.c_void_p), ("pose_volume", Pose), ("frames", C.c_int64), ("shifts", C.c_int32),
                 ("fallbacks", C.c_int32), ("extracted_points", C.c_uint64), ("last_points", C.c_void_p),
                 ("last_colors", C.c_void_p), ("color_volume", C.c_void_p), ("rgb", C.c_void_p),
-                ("last_records", C.c_void_p)]
+                ("last_records", C.c_void_p), ("brick_flags", C.c_void_p)]
 
 
 PLAIN, FUSED = 0, 1
@@ -237,6 +237,10 @@ _PROTOS = {
                                       C.POINTER(Point3f), C.POINTER(RoundBuffer), _P]),
     "kfx_raycast_posed": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), _P, C.c_float, C.POINTER(Intrinsics),
                                C.POINTER(Point3f), _P]),
+    "kfx_brick_flags_size": (_I, [C.POINTER(RoundBuffer), C.POINTER(_SZ)]),
+    "kfx_brick_flags_build": (_I, [_P, C.POINTER(RoundBuffer), _P]),
+    "kfx_raycast_bricks": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), C.POINTER(Point3f), C.POINTER(Matrix3f),
+                                C.c_float, C.POINTER(Intrinsics), C.POINTER(Point3f), _P]),
     "kfx_raycast_count_samples": (_I, [_P, _P, _P, C.POINTER(RoundBuffer), C.POINTER(Point3f), C.POINTER(Matrix3f),
                                        C.c_float, C.POINTER(Intrinsics), C.POINTER(Point3f),
                                        C.POINTER(C.c_uint64)]),

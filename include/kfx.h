@@ -153,6 +153,24 @@ int kfx_integrate_tsdf_posed(const uint16_t* depth, float* depth_scaled, kfx_tsd
                              float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
                              const kfx_round_buffer* rb, const int32_t* dev_skip);
 
+/* Brick flags of the skipping raycast (no reference counterpart: an
+ * acceleration structure of this library).  One byte per 16^3 brick of
+ * local voxel coordinates, x fastest: non-zero when the brick may hold a
+ * negative TSDF value.  kfx_brick_flags_size gives the bytes for a volume
+ * (0 when an axis is not a multiple of 16: no skipping);
+ * kfx_brick_flags_build writes the exact flags of a volume (enqueued);
+ * kfx_raycast_bricks is kfx_raycast with the skipping march, whose outputs
+ * equal kfx_raycast's bit for bit whenever every brick holding a negative
+ * value is flagged (flags may over-state; they may not miss one).  A
+ * pipeline keeps its own flags: its integrations set them and every shift
+ * rebuilds them. */
+int kfx_brick_flags_size(const kfx_round_buffer* rb, size_t* bytes);
+int kfx_brick_flags_build(const kfx_tsdf* volume, const kfx_round_buffer* rb, uint8_t* bricks);
+int kfx_raycast_bricks(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
+                       const kfx_round_buffer* rb, const kfx_point_3f* translation, const kfx_matrix_3f* rotation,
+                       float truncation_distance, const kfx_intrinsics* intr, const kfx_point_3f* cell_size,
+                       const uint8_t* bricks);
+
 /* kfx_raycast that also returns S, the voxel reads its algorithm makes (first
  * sample, every march sample inside the volume, 8 per in-bounds trilinear
  * evaluation): the raycast's algorithmic bytes are 4 S + 24 W H.  Same
@@ -596,6 +614,8 @@ typedef struct kfx_pipeline_view {
     kfx_color* color_volume;      /* colour volume (device), NULL unless cfg.colors */
     kfx_color* rgb;               /* last input colour frame (device), NULL unless cfg.colors */
     kfx_slice_record* last_records; /* last shift's records (device), NULL unless cfg.extract_records */
+    uint8_t* brick_flags;         /* the skipping raycast's brick flags (device, kfx_brick_flags_size bytes),
+                                     NULL in plain mode or when an axis is not a multiple of 16 */
 } kfx_pipeline_view;
 
 typedef struct kfx_pipeline kfx_pipeline;

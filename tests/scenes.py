@@ -31,3 +31,15 @@ def pose_arrays(pose):
 
 def empty_volume(rb):
     return np.zeros(rb.buffer_length, dtype=kfx.TSDF_DTYPE)
+
+
+def brick_flags_cpu(vol, rb, brick=16):
+    """Exact brick flags of a volume on the CPU: 1 where a brick of local
+    voxel coordinates (x fastest) holds a negative value."""
+    n = rb.voxels_size.x
+    o = np.array([rb.origin_GRID.x, rb.origin_GRID.y, rb.origin_GRID.z])
+    v = vol["value"].reshape(n, n, n)  # [z][y][x] memory order
+    local = np.roll(v, shift=(-o[2], -o[1], -o[0]), axis=(0, 1, 2))
+    nb = n // brick
+    neg = (local < 0).reshape(nb, brick, nb, brick, nb, brick).any(axis=(1, 3, 5))
+    return neg.astype(np.uint8).reshape(-1)

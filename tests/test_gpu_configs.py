@@ -21,6 +21,7 @@ import pytest
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "3deye_amd"))
 import workload as wl  # noqa: E402
+import scenes  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -80,6 +81,11 @@ def test_c2_walk_window_across_a_shift(gpu, orc):
         prev_vmap = kfx.download(v.vmap_model, np.float32, (wl.H, wl.W, 3))
         prev_rb = _copy_rb(kfx, v.round_buffer)
         prev_info = info
+        # The integrations keep every brick holding a negative value flagged
+        # (the skipping raycast's condition for the full march's result).
+        flags = kfx.download(v.brick_flags, np.uint8, (wl.N_VOX // 16) ** 3)
+        exact = scenes.brick_flags_cpu(prev_vol, prev_rb)
+        assert not np.any((exact != 0) & (flags == 0))
     print(f"first shift at frame {k}")
     assert k is not None and k >= watch_from, "the walk's first shift must fall inside the watched window"
     assert all(i.fallback == 0 for i in infos[1:])
@@ -109,6 +115,9 @@ def test_c2_walk_window_across_a_shift(gpu, orc):
     got = kfx.download(v.volume, kfx.TSDF_DTYPE, n)
     same = got.view(np.uint32) == ref.view(np.uint32)
     assert same.all(), f"{np.count_nonzero(~same)} voxels differ after the shift"
+    # The shift rebuilt the brick flags of the moved volume exactly.
+    flags = kfx.download(v.brick_flags, np.uint8, (wl.N_VOX // 16) ** 3)
+    assert np.array_equal(flags != 0, scenes.brick_flags_cpu(got, _copy_rb(kfx, v.round_buffer)) != 0)
     assert v.shifts == 1
     ate = kfx.ate_rmse([i.pose_world for i in infos], poses[:k + 1])
     print(f"{len(want_pts)} points extracted at frame {k}; ATE {ate * 1000:.2f} mm over {k + 1} frames")

@@ -125,6 +125,50 @@ def test_raycast_matches_oracle_c1(gpu, orc, origin):
         assert ns.value == s_ref and s_ref > 10 * intr.cols * intr.rows
 
 
+@pytest.mark.parametrize("origin", [(0, 0, 0), (17, 101, 250)])
+def test_raycast_bricks_equal_full_march(gpu, orc, origin):
+    # The skipping march loads only samples in bricks flagged as holding a
+    # negative value; with exact flags, and with every brick flagged, its
+    # maps equal the full march's bit for bit.  The flags kernel equals a
+    # CPU evaluation of the same rule.
+    kfx = gpu
+    intr = kfx.kinect_intrinsics()
+    rb = kfx.round_buffer(256, 3.0, origin=origin)
+    scene, poses = scenes.c1_scene(), scenes.c1_poses(30)
+    ref = scenes.empty_volume(rb)
+    for f in range(0, 30, 3):
+        R, t = poses[f].matrices()
+        depth = kfx.render(scene, poses[f], intr, scenes.C1_SEED, f, noise=True)
+        orc.integrate_f32(ref, depth, intr, R.T, t, scenes.TRUNC, rb)
+    nbytes = C.c_size_t()
+    kfx.call("kfx_brick_flags_size", C.byref(rb), C.byref(nbytes))
+    assert nbytes.value == (256 // 16) ** 3
+    dv = kfx.Buffer.from_array(ref)
+    db = kfx.Buffer(nbytes.value)
+    kfx.call("kfx_brick_flags_build", dv.ptr, C.byref(rb), db.ptr)
+    kfx.synchronize()
+    flags = db.download(np.uint8, nbytes.value)
+    want = scenes.brick_flags_cpu(ref, rb)
+    assert np.array_equal(flags != 0, want != 0)
+    assert 0.01 < want.mean() < 0.5, want.mean()
+    ones = kfx.Buffer.from_array(np.ones(nbytes.value, np.uint8))
+    n = intr.cols * intr.rows
+    vm, nm, vb, nb = (kfx.Buffer(n * 12) for _ in range(4))
+    cs = kfx.cell_size(rb)
+    for f in (4, 11, 20, 29):
+        R, t = poses[f].matrices()
+        args = (C.byref(rb), C.byref(kfx.point3f(t)), C.byref(kfx.matrix3f(R)), scenes.TRUNC, C.byref(intr),
+                C.byref(cs))
+        kfx.call("kfx_raycast", dv.ptr, vm.ptr, nm.ptr, *args)
+        full_v = vm.download(np.uint32, n * 3)
+        full_n = nm.download(np.uint32, n * 3)
+        assert (~np.isnan(full_v.view(np.float32))).mean() > 0.3
+        for bricks in (db, ones):
+            kfx.call("kfx_raycast_bricks", dv.ptr, vb.ptr, nb.ptr, *args, bricks.ptr)
+            assert np.array_equal(vb.download(np.uint32, n * 3), full_v)
+            assert np.array_equal(nb.download(np.uint32, n * 3), full_n)
+
+
 def _random_rotation(rng):
     q = rng.normal(size=4)
     q /= np.linalg.norm(q)

@@ -196,3 +196,24 @@ def test_oracle_clamps_out_of_range_weights(orc):
     assert np.count_nonzero(out[0]["weight"] == 128) == counts[0]
     assert np.count_nonzero(out[2]["weight"] == 1) == counts[0]
     assert np.array_equal(out[0]["weight"] == 128, out[2]["weight"] == 1)
+
+
+def test_brick_flags_helper_brute_force(kfx):
+    # The CPU brick flags the raycast tests compare with, against a loop over
+    # local voxel coordinates through the cyclic address.
+    n, brick = 32, 16
+    rb = kfx.round_buffer(n, 1.0, origin=(5, 17, 30))
+    rng = np.random.default_rng(3)
+    vol = np.zeros(n ** 3, dtype=kfx.TSDF_DTYPE)
+    vol["value"] = rng.integers(0, 3, size=n ** 3)
+    for a in rng.integers(0, n ** 3, size=3):
+        vol["value"][a] = -7
+    want = np.zeros((n // brick) ** 3, np.uint8)
+    for z in range(n):
+        for y in range(n):
+            for x in range(n):
+                a = (((z + 30) % n) * n + (y + 17) % n) * n + (x + 5) % n  # the cyclic address
+                if vol["value"][a] < 0:
+                    want[((z // brick) * (n // brick) + y // brick) * (n // brick) + x // brick] = 1
+    assert want.sum() >= 1
+    assert np.array_equal(scenes.brick_flags_cpu(vol, rb), want)
