@@ -150,15 +150,25 @@ __device__ __forceinline__ float dn_inv_sigma(uint16_t d) { return d ? rcp_rn(no
 // gives position n's tap inverse depths (dn_w; tap 4 the centre; 0 outside
 // the image), inv_sigma_c the centre's dn_inv_sigma.  Lane l supplies
 // k = 4 j + q of MFMA j.
-template <class WAt>
-__device__ __forceinline__ v4f l1_group(WAt w_at, float inv_sigma_c, const DnPacked* __restrict__ pk, int lane) {
+// Lane l's A operands of layer 1 (W1[n][4 j + q], 0 past k = 17).
+__device__ __forceinline__ void load_w1(const DnPacked* __restrict__ pk, int lane, float (&a)[5]) {
     const int n = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const int k = 4 * j + q;
+        a[j] = k < 18 ? pk->w1[n * 18 + k] : 0.0f;
+    }
+}
+
+template <class WAt>
+__device__ __forceinline__ v4f l1_group(WAt w_at, float inv_sigma_c, const float (&a)[5], int lane) {
+    const int q = lane >> 4;
     const float wc = w_at(4);
     v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
         const int k = 4 * j + q;
-        float x = 0.0f, a = 0.0f;
+        float x = 0.0f;
         if (k < 18) {
             const float w = w_at(k % 9);
             if (k < 9) {
@@ -167,11 +177,17 @@ __device__ __forceinline__ v4f l1_group(WAt w_at, float inv_sigma_c, const DnPac
             } else {
                 x = w != 0.0f ? 1.0f : 0.0f;
             }
-            a = pk->w1[n * 18 + k];
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], x, acc, 0, 0, 0);
     }
     return acc;
+}
+
+template <class WAt>
+__device__ __forceinline__ v4f l1_group(WAt w_at, float inv_sigma_c, const DnPacked* __restrict__ pk, int lane) {
+    float a[5];
+    load_w1(pk, lane, a);
+    return l1_group(w_at, inv_sigma_c, a, lane);
 }
 
 typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
@@ -179,14 +195,23 @@ typedef float v4fp __attribute__((ext_vector_type(4)));
 
 // Bias, ReLU and bfloat16 (round to nearest even, v_cvt_pk_bf16_f32) of a
 // group result: four channels 4 q .. 4 q + 3.
-__device__ __forceinline__ uint2 relu_pack(const v4f& acc, const float* __restrict__ b, int q) {
+__device__ __forceinline__ uint2 relu_pack4(const v4f& acc, const float (&bq)[4]) {
     v4fp v;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float z = acc[i] + b[4 * q + i];
+        const float z = acc[i] + bq[i];
         v[i] = z > 0.0f ? z : 0.0f;
     }
     return __builtin_bit_cast(uint2, __builtin_convertvector(v, v4bf));
+}
+__device__ __forceinline__ void load_bias4(const float* __restrict__ b, int q, float (&bq)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bq[i] = b[4 * q + i];
+}
+__device__ __forceinline__ uint2 relu_pack(const v4f& acc, const float* __restrict__ b, int q) {
+    float bq[4];
+    load_bias4(b, q, bq);
+    return relu_pack4(acc, bq);
 }
 
 __device__ __forceinline__ v8bf as_v8bf(uint4 u) { return __builtin_bit_cast(v8bf, u); }
@@ -340,25 +365,27 @@ constexpr int kDfA3X = kDfTX + 2, kDfA3Y = kDfTY + 2; // layer-3 outputs (in the
 static_assert(kDfA3X * kDfA3Y <= kDfA1X * kDfA1Y, "layer 3 reuses the layer-1 buffer");
 
 // A 16 -> 16 layer on the tile: out (ox x oy positions, image origin gx0,
-// gy0) from in (ox + 2 wide), 16 consecutive positions of a row per group.
+// gy0) from in (ox + 2 wide), in groups of 16 consecutive positions of the
+// flattened ox x oy tile (a group may span two rows: each lane addresses its
+// own position).
 __device__ __forceinline__ void df_mid_layer(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int ox,
                                              int oy, int gx0, int gy0, int W, int H, const v8bf (&wa)[5],
-                                             const float* __restrict__ b) {
+                                             const float (&bq)[4]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = lane & 15, q = lane >> 4;
-    const int segs_x = (ox + 15) / 16;
+    const int npos = ox * oy;
     const int ix = ox + 2;
-    for (int g = wave; g < segs_x * oy; g += kDfThreads / 64) {
-        const int sy = g / segs_x, sx = (g % segs_x) * 16;
-        const int px = min(sx + n, ox - 1);  // the partial last group reads a valid position
+    for (int g = wave; g < (npos + 15) / 16; g += kDfThreads / 64) {
+        const int i = min(16 * g + n, npos - 1);  // the partial last group reads a valid position
+        const int ly = i / ox, lx = i - ly * ox;
         auto in_at = [&](int t, int c0) {
-            return *reinterpret_cast<const uint4*>(in + ((sy + t / 3) * ix + px + t % 3) * kDnC + c0);
+            return *reinterpret_cast<const uint4*>(in + ((ly + t / 3) * ix + lx + t % 3) * kDnC + c0);
         };
         const v4f acc = conv_group(in_at, wa, lane);
-        if (sx + n >= ox) continue;
-        const int gx = gx0 + sx + n, gy = gy0 + sy;
-        const uint2 v = (gx >= 0 && gy >= 0 && gx < W && gy < H) ? relu_pack(acc, b, q) : make_uint2(0, 0);
-        *reinterpret_cast<uint2*>(out + (sy * ox + sx + n) * kDnC + 4 * q) = v;
+        if (16 * g + n >= npos) continue;
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        const uint2 v = (gx >= 0 && gy >= 0 && gx < W && gy < H) ? relu_pack4(acc, bq) : make_uint2(0, 0);
+        *reinterpret_cast<uint2*>(out + i * kDnC + 4 * q) = v;
     }
 }
 
@@ -387,23 +414,31 @@ __global__ void __launch_bounds__(kDfThreads) dn_fused_fwd(const uint16_t* __res
     // Layer 1 on groups of 16 consecutive positions of the flattened
     // layer-1 tile (0 outside the image).
     constexpr int kA1 = kDfA1X * kDfA1Y;
-    for (int g = wave; g < (kA1 + 15) / 16; g += kDfThreads / 64) {
-        const int i = min(16 * g + n, kA1 - 1);  // the partial last group reads a valid position
-        const int lx = i % kDfA1X, ly = i / kDfA1X;
-        auto w_at = [&](int t) -> float { return sw[(ly + t / 3) * kDfDX + lx + t % 3]; };
-        const v4f acc = l1_group(w_at, sis[(ly + 1) * kDfDX + lx + 1], pk, lane);
-        if (16 * g + n >= kA1) continue;
-        const int x = x0 - 3 + lx, y = y0 - 3 + ly;
-        const uint2 v = (x >= 0 && y >= 0 && x < W && y < H) ? relu_pack(acc, pk->b1, q) : make_uint2(0, 0);
-        *reinterpret_cast<uint2*>(s1 + i * kDnC + 4 * q) = v;
+    {
+        float a1[5], b1[4];
+        load_w1(pk, lane, a1);
+        load_bias4(pk->b1, q, b1);
+        for (int g = wave; g < (kA1 + 15) / 16; g += kDfThreads / 64) {
+            const int i = min(16 * g + n, kA1 - 1);  // the partial last group reads a valid position
+            const int lx = i % kDfA1X, ly = i / kDfA1X;
+            auto w_at = [&](int t) -> float { return sw[(ly + t / 3) * kDfDX + lx + t % 3]; };
+            const v4f acc = l1_group(w_at, sis[(ly + 1) * kDfDX + lx + 1], a1, lane);
+            if (16 * g + n >= kA1) continue;
+            const int x = x0 - 3 + lx, y = y0 - 3 + ly;
+            const uint2 v = (x >= 0 && y >= 0 && x < W && y < H) ? relu_pack4(acc, b1) : make_uint2(0, 0);
+            *reinterpret_cast<uint2*>(s1 + i * kDnC + 4 * q) = v;
+        }
     }
     v8bf wa[5];
+    float bq[4];
     load_wa(pk->w2p, lane, wa);
+    load_bias4(pk->b2, q, bq);
     __syncthreads();  // layer 1 done: s2 free for layer 2's outputs
-    df_mid_layer(s1, s2, kDfA2X, kDfA2Y, x0 - 2, y0 - 2, W, H, wa, pk->b2);
+    df_mid_layer(s1, s2, kDfA2X, kDfA2Y, x0 - 2, y0 - 2, W, H, wa, bq);
     load_wa(pk->w3p, lane, wa);
+    load_bias4(pk->b3, q, bq);
     __syncthreads();
-    df_mid_layer(s2, s1, kDfA3X, kDfA3Y, x0 - 1, y0 - 1, W, H, wa, pk->b3);
+    df_mid_layer(s2, s1, kDfA3X, kDfA3Y, x0 - 1, y0 - 1, W, H, wa, bq);
     load_wa4(pk->w4p, lane, wa);
     __syncthreads();
     // Layer 4 and the output, 16 pixels of a tile row per group.

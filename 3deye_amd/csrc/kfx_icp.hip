@@ -930,36 +930,19 @@ __device__ __forceinline__ void copy_state(IcpState* dst, const IcpState* src) {
 // increasing order, as the reduce kernel's thread w does; workgroup 0 then
 // waits for every workgroup's 4 sums, runs the reduce kernel's tree over
 // them and writes the totals and the decision, and empties the entries for
-// the next launch (every workgroup has stored its own by then).  The model
-// vertices of the first kGateLdsRounds rounds may have been loaded into gv
-// already (gv_ready); rounds are taken in pairs, both depth gathers issued
-// before either is used.
-constexpr int kGateLdsRounds = 2;
-
-struct GateLds {
-    kfx_xyz v[kGateLdsRounds][kFrameThreads];
-};
+// the next launch (every workgroup has stored its own by then).  Rounds are
+// taken in pairs, both depth gathers issued before either is used.  (The
+// model vertices are loaded here, not ahead during the last solve: staging
+// them in LDS there cost more in the iterations' register pressure than it
+// saved.)
 
 __device__ __forceinline__ int gate_pixel(int w, int r, int t, int nblk) {
     const int blk = w + kFrameBlocks * (kFrameGroups * r + t / kIcpThreads);
     return blk < nblk ? blk * 256 + t % kIcpThreads : 0x7fffffff;
 }
 
-// Model vertices of this workgroup's first rounds into LDS, by every thread
-// but the first wave's (which runs the solve meanwhile).
-__device__ __forceinline__ void gate_prefetch(const IcpFrameArgs& a, GateLds& gv, int w) {
-    if (threadIdx.x < 64) return;
-    const FrameGate& G = a.gate;
-    const int P = G.intr.cols * G.intr.rows, nblk = (P + 255) / 256;
-    for (int i = threadIdx.x - 64; i < kGateLdsRounds * kFrameThreads; i += kFrameThreads - 64) {
-        const int r = i / kFrameThreads, t = i % kFrameThreads;
-        const int p = gate_pixel(w, r, t, nblk);
-        gv.v[r][t] = p < P ? G.vmap_model[p] : kfx_xyz{NAN, NAN, NAN};
-    }
-}
-
-__device__ __forceinline__ void frame_gate(const IcpFrameArgs& a, const IcpState& s, const GateLds& gv,
-                                           bool gv_ready, int w, int grp, int lt, int& fault) {
+__device__ __forceinline__ void frame_gate(const IcpFrameArgs& a, const IcpState& s, int w, int grp, int lt,
+                                           int& fault) {
     __shared__ double gred[2][kFrameGroups][kIcpWaves][4];
     __shared__ double gtree[4][kFrameBlocks];
     const FrameGate& G = a.gate;
@@ -976,9 +959,7 @@ __device__ __forceinline__ void frame_gate(const IcpFrameArgs& a, const IcpState
         float cz[2];
         for (int h = 0; h < 2; ++h) {
             const int r = r0 + h, p = gate_pixel(w, r, threadIdx.x, nblk);
-            const kfx_xyz x = (gv_ready && r < kGateLdsRounds) ? gv.v[r][threadIdx.x]
-                              : (r < rounds && p < P)           ? G.vmap_model[p]
-                                                                : kfx_xyz{NAN, NAN, NAN};
+            const kfx_xyz x = (r < rounds && p < P) ? G.vmap_model[p] : kfx_xyz{NAN, NAN, NAN};
             gate_project(x, pose, G.intr, v[h], idx[h], cz[h]);
         }
         uint16_t d[2];
@@ -1043,8 +1024,6 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
     __shared__ double total[32];
     __shared__ IcpState s;
     __shared__ int fault;
-    __shared__ std::conditional_t<kGate, GateLds, char> gv;
-    bool gv_ready = false;
     const int grp = threadIdx.x / kIcpThreads, lt = threadIdx.x % kIcpThreads;
     const int w = blockIdx.x;
     const bool leader = w < kIcpWaves;
@@ -1176,19 +1155,11 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
                 for (int i = 0; i < 3; ++i) t0[i] = s.t[i];
                 solve_and_update(&s, total, R0, t0, s.iterations, tag, a.conv_rot, a.conv_trans);
             }
-            // During the frame's last solve the other waves load the gate's
-            // model vertices.
-            if constexpr (kGate) {
-                if (l == 0 && it == L.iterations - 1) {
-                    gate_prefetch(a, gv, w);
-                    gv_ready = true;
-                }
-            }
             __syncthreads();
             ++k;
         }
     }
-    if constexpr (kGate) frame_gate(a, s, gv, gv_ready, w, grp, lt, fault);
+    if constexpr (kGate) frame_gate(a, s, w, grp, lt, fault);
     if (w == 0) {
         if (threadIdx.x == 0) {
             s.fault = fault;
