@@ -21,20 +21,26 @@ pipeline (preprocess, ICP, integrate, raycast, shift with extraction).
 
 The headline `value` is the learned pipeline (denoiser in place of the
 bilateral filter with the bilateral fallback, device-resident pose,
-FP32-compensated ICP sums, graph replay).  Timed on the same frames and
-reported beside it: the same with the ICP early exit; the fused pipeline
-with the bilateral filter, with and without early exit; and the plain
-pipeline (the reference's control flow: a synchronisation after every
-kernel, the ICP normal equations downloaded and solved on the host every
-iteration), each with its trajectory's ATE against the ground truth.
+one-launch frame ICP with FP32-compensated sums, raycast skipping bricks
+without negative values), its kernels launched in order on one stream with
+one host synchronisation per frame.  Every timed configuration first runs
+the same window once untimed.  Timed on the same frames and reported beside
+it: the same as one HIP graph per frame; with the ICP early exit; the fused
+pipeline with the bilateral filter (launched, as a graph, with early exit);
+and the plain pipeline (the reference's control flow: a synchronisation
+after every kernel, the ICP normal equations downloaded and solved on the
+host every iteration), each with its trajectory's ATE against the ground
+truth over the whole walk (the frames after the timed ones run untimed).
 Per-kernel tables: the plain pipeline's kernels and the learned pipeline's
-kernels (an eager replay of the same frames, identical results), each with
-algorithmic bytes or FLOPs per launch from HIP events around the launches.
+kernels (a replay of the same frames with HIP events around the launches,
+identical results), each with algorithmic bytes or FLOPs per launch.
 
 Prints one JSON line.  Other modes: --kernel-probe (the learned pipeline,
 eager, for hardware-counter runs), --summarise-pmc (turns two counter runs
-of the probe into profiles/integrate_traffic.json), --profile-learned N and
---profile-fused N (one pipeline, graph replay, for a kernel trace).
+of the probe into profiles/integrate_traffic.json), --summarise-counters
+(per-kernel counter means of a probe run), --profile-learned N and
+--profile-fused N (one pipeline, launched as in the headline, for a kernel
+trace), --summarise-trace (kernel table of such a trace).
 """
 import argparse
 import ctypes as C
@@ -382,7 +388,7 @@ def kernel_probe(frames=N_FRAMES):
     """Short, fixed workload for hardware-counter runs (rocprofv3 --pmc):
     three 256 MiB streaming copies (known traffic, for calibrating the
     FETCH_SIZE / WRITE_SIZE counters), then the learned pipeline of the
-    benchmark, eager (the same kernels and results as its graph replay), over
+    benchmark (the same kernels and results as the headline run), over
     the walk's frames.  Prints, in order, the frame of every integration
     dispatch with a device pose (frame 0 integrates at the host pose; a frame
     that falls back to the bilateral filter dispatches twice, the first one
@@ -489,6 +495,31 @@ def summarise_trace(db_dir, frames):
     print(f"{'sum of kernel spans per frame':60s} {'':6s} {'':8s} {'':8s} {total:9.1f}")
 
 
+def summarise_counters(db_dir, kernels):
+    """Mean per-dispatch value of every counter of a rocprofv3 --pmc run of
+    --kernel-probe for the named kernels (substring match), over their
+    dispatches after the first 20 (a populated volume)."""
+    dbs = glob.glob(os.path.join(db_dir, "**", "*.db"), recursive=True)
+    if not dbs:
+        raise SystemExit(f"no rocprofv3 database under {db_dir}")
+    con = sqlite3.connect(dbs[0])
+    rows = con.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection").fetchall()
+    con.close()
+    per = {}
+    for disp, name, counter, value in rows:
+        per.setdefault((disp, name), {}).setdefault(counter, 0.0)
+        per[(disp, name)][counter] += value
+    for want in kernels:
+        got = [m for (d, n), m in sorted(per.items()) if want in n]
+        if not got:
+            print(f"{want}: no dispatches")
+            continue
+        use = got[20:] if len(got) > 40 else got
+        print(f"{want}: {len(use)} dispatches")
+        for counter in sorted(use[0]):
+            print(f"  {counter:28s} {np.mean([m[counter] for m in use]):16.1f}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -498,7 +529,7 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-rgbd", action="store_true", help="no RGBD odometry measurement")
     ap.add_argument("--profile-learned", type=int, default=0, metavar="N",
-                    help="only the learned pipeline (graph replay): the pre-roll, 10 warmup and N timed frames, "
+                    help="only the learned pipeline (as in the headline): the pre-roll, 10 warmup and N timed frames, "
                          "for a kernel trace (prints frames/s, no JSON benchmark line)")
     ap.add_argument("--profile-fused", type=int, default=0, metavar="N",
                     help="the same for the fused pipeline with the bilateral filter")
@@ -508,9 +539,14 @@ def main():
                     help="write profiles/integrate_traffic.json from two --pmc runs of --kernel-probe")
     ap.add_argument("--summarise-trace", nargs=2, metavar=("TRACE_DIR", "FRAMES"),
                     help="kernel table of the last FRAMES frames of a --kernel-trace run of --profile-learned/fused")
+    ap.add_argument("--summarise-counters", nargs="+", metavar=("PMC_DIR", "KERNEL"),
+                    help="mean counter values per dispatch of the named kernels from a --pmc run of --kernel-probe")
     ap.add_argument("--pmc-command", default="", help="the counter command, recorded with --summarise-pmc")
     ap.add_argument("--traffic-out", default=TRAFFIC_FILE, help="where --summarise-pmc writes its table")
     args = ap.parse_args()
+    if args.summarise_counters:
+        summarise_counters(args.summarise_counters[0], args.summarise_counters[1:])
+        return
     if args.summarise_trace:
         summarise_trace(args.summarise_trace[0], int(args.summarise_trace[1]))
         return
@@ -527,7 +563,7 @@ def main():
         kfx.init(0)
         dd = kfx.Buffer.from_array(depths)
         dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params()) if learned_run else None
-        cfg = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1 if learned_run else 0)
+        cfg = wl.pipeline_config(mode=kfx.FUSED, use_graph=0, denoiser=1 if learned_run else 0)
         el, _, pipe, _ = run_pipeline(cfg, poses, origin, dd, depths[0].nbytes, 10, n_t, denoiser=dn)
         pipe.close()
         print("%s pipeline: %d frames, %.1f frames/s" % ("learned" if learned_run else "fused", n_t, n_t / el))
@@ -549,30 +585,31 @@ def main():
 
     dn = kfx.Denoiser(W, H, kfx.Denoiser.load_params())
     early = dict(converge_rotation=1e-5, converge_translation=1e-5)
-    # Learned pipeline: headline.
-    cfg_l = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1)
-    # GPU warm-up before the first timed run: the learned pipeline over the
-    # pre-roll frames, repeated for half a second (an idle GPU starts a run
-    # at lower clocks than one that has been busy).
-    t_w = time.perf_counter()
-    while time.perf_counter() - t_w < 0.5:
-        run_pipeline(cfg_l, poses, origin, ddepth, fb, args.warmup, 1, denoiser=dn)[2].close()
-    run_l = run_pipeline(cfg_l, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=dn, rest=rest)
+    def timed(cfg, den=None):
+        """A run of the timed window, after an identical untimed pass (the
+        first run of a configuration in a process starts slower)."""
+        run_pipeline(cfg, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=den)[2].close()
+        return run_pipeline(cfg, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=den, rest=rest)
+
+    # Learned pipeline, kernels launched in order on one stream: headline.
+    cfg_l = wl.pipeline_config(mode=kfx.FUSED, use_graph=0, denoiser=1)
+    run_l = timed(cfg_l, dn)
     inf_l = run_l[1]
     learned = summary(*run_l, args.steps, poses)
-    # Beside the headline, not as it: the KinectFusion early exit (a level
-    # stops once an increment is below 1e-5 rad and 1e-5 m, the pose
-    # tolerance), at most 10/5/4 iterations, for both filters.
-    cfg_le = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1, **early)
-    learned["early_exit_1e-5"] = summary(
-        *run_pipeline(cfg_le, poses, origin, ddepth, fb, args.warmup, args.steps, denoiser=dn, rest=rest), args.steps,
-        poses)
-    cfg_f = wl.pipeline_config(mode=kfx.FUSED, use_graph=1)
-    fused = summary(*run_pipeline(cfg_f, poses, origin, ddepth, fb, args.warmup, args.steps, rest=rest), args.steps,
-                    poses)
-    cfg_fe = wl.pipeline_config(mode=kfx.FUSED, use_graph=1, **early)
-    fused["early_exit_1e-5"] = summary(*run_pipeline(cfg_fe, poses, origin, ddepth, fb, args.warmup, args.steps,
-                                                     rest=rest), args.steps, poses)
+    # Beside the headline, not as it: the same frames as one HIP graph per
+    # frame (hipGraphLaunch costs more host time here than the frame's ten
+    # launches); the KinectFusion early exit (a level stops once an increment
+    # is below 1e-5 rad and 1e-5 m, the pose tolerance), at most 10/5/4
+    # iterations, for both filters.
+    learned["graph_replay"] = summary(*timed(wl.pipeline_config(mode=kfx.FUSED, use_graph=1, denoiser=1), dn),
+                                      args.steps, poses)
+    cfg_le = wl.pipeline_config(mode=kfx.FUSED, use_graph=0, denoiser=1, **early)
+    learned["early_exit_1e-5"] = summary(*timed(cfg_le, dn), args.steps, poses)
+    cfg_f = wl.pipeline_config(mode=kfx.FUSED, use_graph=0)
+    fused = summary(*timed(cfg_f), args.steps, poses)
+    fused["graph_replay"] = summary(*timed(wl.pipeline_config(mode=kfx.FUSED, use_graph=1)), args.steps, poses)
+    cfg_fe = wl.pipeline_config(mode=kfx.FUSED, use_graph=0, **early)
+    fused["early_exit_1e-5"] = summary(*timed(cfg_fe), args.steps, poses)
 
     # The learned pipeline again, eager, with HIP events around every launch:
     # the per-kernel table and the integrate roofline of the headline.
@@ -580,7 +617,7 @@ def main():
     el_lp, inf_lp, pipe_lp, _ = run_pipeline(cfg_lp, poses, origin, ddepth, fb, args.warmup, args.steps,
                                              profile=True, denoiser=dn)
     pipe_lp.close()
-    learned["eager_replay_identical"] = all(
+    learned["profiled_replay_identical"] = all(
         a.pose_world.t[:] == b.pose_world.t[:] and a.pose_world.R[:] == b.pose_world.R[:]
         for a, b in zip(inf_l, inf_lp))
     int_ms, int_n = kfx.profile_query("integrate")
@@ -640,8 +677,9 @@ def main():
                                f"at frames 77/150/225/297; timed frames {first}-{n - 1} after {PREROLL} pre-roll "
                                f"and {args.warmup} warmup frames",
                    "pipeline": "learned: denoiser (bf16 MFMA) with bilateral fallback at 15 mm RMS, device-resident "
-                               "pose, FP32-Kahan ICP workgroup sums with FP64 cross-workgroup sum, Cholesky and pose "
-                               "update, HIP graph replay",
+                               "pose, one-launch frame ICP with FP32-Kahan workgroup sums and FP64 cross-workgroup "
+                               "sum, Cholesky and pose update, raycast skipping bricks without negative values; "
+                               "kernels launched in order on one stream, one host synchronisation per frame",
                    "frames_timed": [first, n - 1], "shifts_timed": learned["shifts_timed"],
                    "extracted_points_timed": learned["extracted_points_timed"],
                    "scene_seed": SCENE_SEED, "icp_iterations": [10, 5, 4]},
