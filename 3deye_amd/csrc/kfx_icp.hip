@@ -13,8 +13,8 @@
 //    no host round trip.  The single-precision products of each pixel are
 //    summed in double precision per thread, then over the wave and the
 //    workgroup in a fixed order; a one-workgroup launch adds the workgroup
-//    totals in double precision in a fixed order, solves the 6x6 system by
-//    Cholesky in double precision and composes the increment into the
+//    totals in double precision in a fixed order, solves the 6x6 system in
+//    double precision over one wave and composes the increment into the
 //    device-resident pose.  No host round trip between iterations, so a whole frame's 19
 //    iterations can be enqueued (or captured in a graph) at once, and the
 //    result does not depend on the order in which workgroups finish.
@@ -299,119 +299,191 @@ __device__ __forceinline__ float2 wave_reduce_scatter32(float (&s)[32], float (&
     return make_float2(a, e);
 }
 
-// sin and cos of an ICP increment angle.  For |x| <= 1/8 the Taylor series
+// sin and cos of an ICP increment angle with |x| <= 1/8: the Taylor series
 // (sin to x^9, cos to x^10; the first omitted terms are below 3e-17 relative)
-// in Horner form with fused multiply-adds; larger angles take sincos.
-__device__ __forceinline__ void small_sincos(double x, double& s, double& c) {
-    if (fabs(x) <= 0.125) {
-        const double z = x * x;
-        const double ps = fma(z, fma(z, fma(z, 1.0 / 362880.0, -1.0 / 5040.0), 1.0 / 120.0), -1.0 / 6.0);
-        s = fma(x * z, ps, x);
-        const double pc = fma(z, fma(z, fma(z, fma(z, -1.0 / 3628800.0, 1.0 / 40320.0), -1.0 / 720.0), 1.0 / 24.0),
-                              -0.5);
-        c = fma(z, pc, 1.0);
-    } else {
-        sincos(x, &s, &c);
+// in Horner form with fused multiply-adds.
+__device__ __forceinline__ void sincos_series(double x, double& s, double& c) {
+    const double z = x * x;
+    const double ps = fma(z, fma(z, fma(z, 1.0 / 362880.0, -1.0 / 5040.0), 1.0 / 120.0), -1.0 / 6.0);
+    s = fma(x * z, ps, x);
+    const double pc = fma(z, fma(z, fma(z, fma(z, -1.0 / 3628800.0, 1.0 / 40320.0), -1.0 / 720.0), 1.0 / 24.0),
+                          -0.5);
+    c = fma(z, pc, 1.0);
+}
+
+// Increment angles beyond the series' range (rare: an ICP step of more than
+// 7 degrees) are brought into it and the series' results carried back, with
+// the series' own constants so the kernels that inline the solve need few
+// extra registers for it.  reduce_angle: past |x| = 3, r = x - k 2pi
+// (k = rint(x / 2pi), 2pi split into two doubles, each product subtracted
+// with one rounding); then r halved m <= 5 times to |r| <= 1/8 (a
+// non-finite x gives r = NaN).  double_angle: m steps (s, c) <- (2 s c,
+// (c - s)(c + s)), each adding a few ulps.
+__device__ __forceinline__ void reduce_angle(double& r, int& m) {
+    m = 0;
+    if (!isfinite(r)) {
+        r = r - r;
+        return;
+    }
+    if (fabs(r) > 3.0) {
+        constexpr double kInvTwoPi = 0x1.45f306dc9c883p-3;
+        constexpr double kTwoPi1 = 0x1.921fb54442d18p+2, kTwoPi2 = 0x1.1a62633145c07p-52;
+        const double k = rint(r * kInvTwoPi);
+        r = fma(-k, kTwoPi1, r);
+        r = fma(-k, kTwoPi2, r);
+    }
+    while (fabs(r) > 0.125 && m < 6) {
+        r *= 0.5;
+        ++m;
     }
 }
 
-// 6x6 Cholesky solve in double precision, then the pose update of the
-// estimate R0, t0 (st->R, st->t, which the caller loaded ahead, so their
-// latency hides behind the sums).  Every loop has constant bounds, so the
-// arrays stay in registers; the six pivots are inverted once and every other
-// division is a multiplication.
-__device__ void solve_and_update(IcpState* st, const double* sums, const double (&R0)[9], const double (&t0)[3],
-                                 int iterations, int tag, double conv_rot, double conv_trans) {
-    double A[36], b[6];
-    {
-        int k = 0;
-#pragma unroll
-        for (int i = 0; i < 6; ++i)
-#pragma unroll
-            for (int j = i; j < 7; ++j, ++k) {
-                if (j == 6) b[i] = sums[k];
-                else A[j * 6 + i] = A[i * 6 + j] = sums[k];
-            }
+__device__ __forceinline__ void double_angle(double& s, double& c, int m) {
+    for (int i = 0; i < m; ++i) {
+        const double s2 = 2.0 * s * c;
+        c = (c - s) * (c + s);
+        s = s2;
     }
+}
+
+// Lane l's value of v, in every lane (l a constant; every lane active).
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)u, l);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Index of entry (r, c), r <= c <= 6, among the 27 sums (row r of the upper
+// triangle of [A | b] after rows 0 .. r-1).
+__host__ __device__ constexpr int sum_index(int r, int c) { return 7 * r - r * (r - 1) / 2 + (c - r); }
+
+// The normal equations A x = b of the 27 sums solved in double precision,
+// then the pose update of the estimate R0, t0 (st->R, st->t: the state's
+// own, or a copy the caller loaded ahead; read after the elimination).
+// Called by every lane of one wave (lane 0 .. 63).
+//
+// Gauss-Jordan elimination on [A | b] without pivoting (A is symmetric
+// positive definite; its pivots are those of the LDL^T factorisation):
+// lane j <= 6 holds column j of [A | b] (lanes 7 .. 63 a copy of column 6);
+// step k broadcasts column k (readlane), every lane scales its row-k entry
+// by 1 / A_kk (reciprocal with two Newton steps) and eliminates entry k from
+// the other rows with fused multiply-adds.  A pivot that is not finite and
+// above 1e-300 makes the system singular: status KFX_ERR_SINGULAR, x = 0,
+// the pose unchanged.  After six steps column 6 holds x.  Each step's chain
+// is one broadcast, the reciprocal and two dependent operations, where a
+// one-lane Cholesky factorisation and two substitutions run several hundred
+// dependent double-precision operations.
+__device__ void solve_and_update(IcpState* st, const double* sums, const double* R0, const double* t0,
+                                 int iterations, int tag, double conv_rot, double conv_trans, int lane) {
+    if (lane < kIcpEntries) {  // the normal equations of the state
+        int r = 0;
 #pragma unroll
-    for (int i = 0; i < 36; ++i) st->A[i] = A[i];
+        for (int q = 1; q < 6; ++q) r += lane >= sum_index(q, q) ? 1 : 0;
+        const int c = lane - sum_index(r, r) + r;
+        const double v = sums[lane];
+        if (c == 6) {
+            st->b[r] = v;
+        } else {
+            st->A[r * 6 + c] = v;
+            st->A[c * 6 + r] = v;
+        }
+    }
+    const int j = lane < 6 ? lane : 6;
+    double col[6];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) st->b[i] = b[i];
-    // Cholesky A = L L^T; a non-positive or non-finite pivot makes the
-    // system singular (the remaining entries are then not used).
-    double L[36], inv_diag[6];
+    for (int i = 0; i < 6; ++i) col[i] = sums[i <= j ? sum_index(i, j) : sum_index(j, i)];
     bool ok = true;
 #pragma unroll
-    for (int i = 0; i < 6; ++i)
+    for (int k = 0; k < 6; ++k) {
+        double ck[6];
 #pragma unroll
-        for (int j = 0; j <= i; ++j) {
-            double s = A[i * 6 + j];
+        for (int i = 0; i < 6; ++i) ck[i] = lane_value(col[i], k);
+        const double d = ck[k];
+        ok = ok && (d > 1e-300) && isfinite(d);
+        const double dv = ok ? d : 1.0;
+        double y = __builtin_amdgcn_rcp(dv);
 #pragma unroll
-            for (int p = 0; p < j; ++p) s -= L[i * 6 + p] * L[j * 6 + p];
-            if (i == j) {
-                ok = ok && (s > 1e-300) && isfinite(s);
-                const double sv = ok ? s : 1.0;
-                double y = __builtin_amdgcn_rsq(sv);
-                for (int nr = 0; nr < 2; ++nr) {
-                    const double r = fma(-(sv * y), y, 1.0);
-                    y = fma(0.5 * y, r, y);
-                }
-                L[i * 6 + i] = sv * y;
-                inv_diag[i] = y;
-            } else {
-                L[i * 6 + j] = s * inv_diag[j];
-            }
+        for (int nr = 0; nr < 2; ++nr) y = fma(y, fma(-dv, y, 1.0), y);
+        const double pk = col[k] * y;
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+            if (i != k) col[i] = fma(-ck[i], pk, col[i]);
+        col[k] = pk;
+    }
+    if (lane == 0) st->iterations = iterations + 1;
+    if (!ok) {  // uniform
+        if (lane == 0) {
+            st->status = KFX_ERR_SINGULAR;
+            st->singular_count += 1;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) st->x[i] = 0.0;
         }
-    st->iterations = iterations + 1;
-    if (!ok) {
-        st->status = KFX_ERR_SINGULAR;
-        st->singular_count += 1;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) st->x[i] = 0.0;
         return;
     }
-    double yv[6], x[6];
+    double x[6];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        double s = b[i];
-#pragma unroll
-        for (int p = 0; p < i; ++p) s -= L[i * 6 + p] * yv[p];
-        yv[i] = s * inv_diag[i];
+    for (int i = 0; i < 6; ++i) x[i] = lane_value(col[i], 6);
+    // Increment: R_inc = Rz(x2) Ry(x1) Rx(x0), t_inc = x[3..5];
+    // t <- R_inc t + t_inc, R <- R_inc R.  Lane 3 r + c < 9 forms R[r][c],
+    // lane 9 + r forms t[r] (each with the same operations in the same order
+    // as one lane forming all twelve).
+    double r0 = x[0], r1 = x[1], r2 = x[2];
+    int m0 = 0, m1 = 0, m2 = 0;
+    const bool wide = !(fmax(fabs(x[0]), fmax(fabs(x[1]), fabs(x[2]))) <= 0.125);  // uniform
+    if (wide) {
+        reduce_angle(r0, m0);
+        reduce_angle(r1, m1);
+        reduce_angle(r2, m2);
     }
-#pragma unroll
-    for (int i = 5; i >= 0; --i) {
-        double s = yv[i];
-#pragma unroll
-        for (int p = i + 1; p < 6; ++p) s -= L[p * 6 + i] * x[p];
-        x[i] = s * inv_diag[i];
+    double sa, ca, sb, cb, sg, cg;  // the three series interleave
+    sincos_series(r0, sa, ca);
+    sincos_series(r1, sb, cb);
+    sincos_series(r2, sg, cg);
+    if (wide) {
+        double_angle(sa, ca, m0);
+        double_angle(sb, cb, m1);
+        double_angle(sg, cg, m2);
     }
+    const bool is_t = lane >= 9;
+    const int r = is_t ? lane - 9 : lane / 3;  // row of R_inc (lanes >= 12: unused)
+    const int c = lane - 3 * (lane / 3);        // column of R0 (R lanes)
+    double ri0, ri1, ri2;
+    if (r == 0) {
+        ri0 = cg * cb; ri1 = cg * sb * sa - sg * ca; ri2 = cg * sb * ca + sg * sa;
+    } else if (r == 1) {
+        ri0 = sg * cb; ri1 = sg * sb * sa + cg * ca; ri2 = sg * sb * ca - cg * sa;
+    } else {
+        ri0 = -sb; ri1 = cb * sa; ri2 = cb * ca;
+    }
+    double o0, o1, o2;
+    if (is_t) {
+        o0 = t0[0]; o1 = t0[1]; o2 = t0[2];
+    } else if (c == 0) {
+        o0 = R0[0]; o1 = R0[3]; o2 = R0[6];
+    } else if (c == 1) {
+        o0 = R0[1]; o1 = R0[4]; o2 = R0[7];
+    } else {
+        o0 = R0[2]; o1 = R0[5]; o2 = R0[8];
+    }
+    double v = ri0 * o0 + ri1 * o1 + ri2 * o2;
+    if (is_t) v = v + (r == 0 ? x[3] : r == 1 ? x[4] : x[5]);
+    if (lane < 9) {
+        st->R[lane] = v;
+        st->Rf[lane] = (float)v;
+    } else if (lane < 12) {
+        st->t[r] = v;
+        st->tf[r] = (float)v;
+    }
+    if (lane != 0) return;
 #pragma unroll
     for (int i = 0; i < 6; ++i) st->x[i] = x[i];
     st->status = KFX_OK;
-    // Increment: R_inc = Rz(x2) Ry(x1) Rx(x0), t_inc = x[3..5];
-    // t <- R_inc t + t_inc, R <- R_inc R.
-    double sa, ca, sb, cb, sg, cg;
-    small_sincos(x[0], sa, ca);
-    small_sincos(x[1], sb, cb);
-    small_sincos(x[2], sg, cg);
-    const double Ri[9] = {cg * cb, cg * sb * sa - sg * ca, cg * sb * ca + sg * sa,
-                          sg * cb, sg * sb * sa + cg * ca, sg * sb * ca - cg * sa,
-                          -sb, cb * sa, cb * ca};
-    double R[9], t[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            R[3 * r + c] = Ri[3 * r + 0] * R0[c] + Ri[3 * r + 1] * R0[3 + c] + Ri[3 * r + 2] * R0[6 + c];
-        t[r] = Ri[3 * r + 0] * t0[0] + Ri[3 * r + 1] * t0[1] + Ri[3 * r + 2] * t0[2] + x[3 + r];
+    if (conv_rot > 0.0) {
+        const double rot = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        const double tr = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+        if (rot < conv_rot && tr < conv_trans) st->converged_tag = tag;
     }
-#pragma unroll
-    for (int i = 0; i < 9; ++i) { st->R[i] = R[i]; st->Rf[i] = (float)R[i]; }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) { st->t[i] = t[i]; st->tf[i] = (float)t[i]; }
-    const double rot = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-    const double tr = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
-    if (conv_rot > 0.0 && rot < conv_rot && tr < conv_trans) st->converged_tag = tag;
 }
 
 // Precision of the per-workgroup sums (the cross-workgroup sum, the solve and
@@ -427,11 +499,14 @@ enum IcpSumMode : int {
 // a larger one) in a fixed order: the butterfly over each wave, then the
 // four wave totals in wave order.  Local thread e < 27 returns entry e (the
 // others return 0).  Every thread of the workgroup calls it (it holds a
-// workgroup barrier); wsum is the group's own.
+// workgroup barrier); wsum is the group's own.  A wave with live == false
+// (uniform: every value of every lane is +0) skips its butterfly, whose
+// totals would all be +0.
 template <int kWaves = kIcpWaves, typename T>
-__device__ __forceinline__ T group_sum27(T (&acc)[32], T (*wsum)[32], int lt) {
+__device__ __forceinline__ T group_sum27(T (&acc)[32], T (*wsum)[32], int lt, bool live = true) {
     const int lane = lt & 63, wave = lt >> 6;
-    const T wtot = wave_reduce_scatter32(acc, lane);
+    T wtot = 0;
+    if (live) wtot = wave_reduce_scatter32(acc, lane);
     if ((lane & 1) == 0) wsum[wave][lane >> 1] = wtot;
     __syncthreads();
     T s = 0;
@@ -449,9 +524,11 @@ __device__ __forceinline__ T workgroup_sum27(T (&acc)[32], T (*wsum)[32]) {
 }
 
 // The same for compensated single-precision pairs.
-__device__ __forceinline__ float2 group_sum27(float (&s)[32], float (&c)[32], float2 (*wsum)[32], int lt) {
+__device__ __forceinline__ float2 group_sum27(float (&s)[32], float (&c)[32], float2 (*wsum)[32], int lt,
+                                             bool live = true) {
     const int lane = lt & 63, wave = lt >> 6;
-    const float2 wtot = wave_reduce_scatter32(s, c, lane);
+    float2 wtot = make_float2(0.0f, 0.0f);
+    if (live) wtot = wave_reduce_scatter32(s, c, lane);
     if ((lane & 1) == 0) wsum[wave][lane >> 1] = wtot;
     __syncthreads();
     float a = 0.0f, e = 0.0f;
@@ -530,6 +607,12 @@ __device__ __forceinline__ IcpGroupSum<M> icp_group_sums(const kfx_xyz (&v)[P], 
     float rows[P][7];
 #pragma unroll
     for (int q = 0; q < P; ++q) icp_residual(n[q], g[q], np[q], vp[q], Rc, f, ok[q], rows[q]);
+    // A wave none of whose pixels is associated sums only +0 (the
+    // accumulation below starts with p + 0).
+    bool any_ok = false;
+#pragma unroll
+    for (int q = 0; q < P; ++q) any_ok = any_ok || ok[q];
+    const bool live = __any(any_ok);
     // Accumulation from zero; the first pixel's step is written out: 0 + p
     // is p + 0 (which turns -0 into +0), and TwoSum(0, p) has a zero error.
     if constexpr (M == kSumDouble) {
@@ -547,7 +630,7 @@ __device__ __forceinline__ IcpGroupSum<M> icp_group_sums(const kfx_xyz (&v)[P], 
                     acc[k] = q == 0 ? prod + 0.0 : acc[k] + prod;
                 }
         }
-        return group_sum27(acc, wsum_d, lt);
+        return group_sum27(acc, wsum_d, lt, live);
     } else {
         float sum[32], comp[32];
 #pragma unroll
@@ -565,8 +648,8 @@ __device__ __forceinline__ IcpGroupSum<M> icp_group_sums(const kfx_xyz (&v)[P], 
                     else sum[k] += prod;
                 }
         }
-        if constexpr (M == kSumKahan) return group_sum27(sum, comp, wsum_f, lt);
-        else return make_float2(group_sum27(sum, reinterpret_cast<float (*)[32]>(wsum_f), lt), 0.0f);
+        if constexpr (M == kSumKahan) return group_sum27(sum, comp, wsum_f, lt, live);
+        else return make_float2(group_sum27(sum, reinterpret_cast<float (*)[32]>(wsum_f), lt, live), 0.0f);
     }
 }
 
@@ -710,7 +793,7 @@ __device__ __forceinline__ void cross_block_sum(const void* __restrict__ partial
 
 // One ICP iteration, second half (one workgroup): the cross-workgroup sum
 // (three workgroups' loads per round: all of them in one round at 640x480),
-// then thread 0's Cholesky solve and pose update.  The estimate is loaded
+// then the first wave's solve and pose update.  The estimate is loaded
 // before the sum so that its latency hides behind the partials' loads.
 // Thread 0 copies the state to host memory (8-byte stores).
 __device__ __forceinline__ void icp_publish(const IcpState* st, IcpState* mirror) {
@@ -739,10 +822,10 @@ icp_solve_kernel(IcpState* __restrict__ st, const void* __restrict__ partials, i
     for (int i = 0; i < 3; ++i) t0[i] = st->t[i];
     const int iterations = st->iterations;
     cross_block_sum<M, 3>(partials, nblk, wsum, total);
-    if (threadIdx.x == 0) {
-        solve_and_update(st, total, R0, t0, iterations, tag, conv_rot, conv_trans);
-        if (mirror != nullptr) icp_publish(st, mirror);
-    }
+    if (threadIdx.x < 64) solve_and_update(st, total, R0, t0, iterations, tag, conv_rot, conv_trans, threadIdx.x);
+    if (mirror == nullptr) return;  // uniform
+    __syncthreads();  // the state's stores by the first wave's lanes are visible
+    if (threadIdx.x == 0) icp_publish(st, mirror);
 }
 
 // cross_block_sum over coherent loads, in a register budget that leaves the
@@ -829,10 +912,10 @@ icp_iter_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
     for (int i = 0; i < 3; ++i) t0[i] = st->t[i];
     const int iterations = st->iterations;
     cross_block_sum_coherent<M>(partials, gridDim.x, wsum_d, total);
-    if (threadIdx.x == 0) {
-        solve_and_update(st, total, R0, t0, iterations, tag, conv_rot, conv_trans);
-        if (mirror != nullptr) icp_publish(st, mirror);
-    }
+    if (threadIdx.x < 64) solve_and_update(st, total, R0, t0, iterations, tag, conv_rot, conv_trans, threadIdx.x);
+    if (mirror == nullptr) return;  // uniform
+    __syncthreads();  // the state's stores by the first wave's lanes are visible
+    if (threadIdx.x == 0) icp_publish(st, mirror);
 }
 
 // ------------------------------------------------------------------------
@@ -851,7 +934,7 @@ icp_iter_kernel(const kfx_xyz* __restrict__ vcurr, const kfx_xyz* __restrict__ n
 //      butterfly of wave q of icp_solve_kernel's workgroup sum and stores
 //      the 27 wave totals;
 //   2. every workgroup reads the 4 x 27 wave totals and adds them in wave
-//      order, then runs the Cholesky solve and pose update itself
+//      order, then runs the solve and pose update itself
 //      (identical in every workgroup) and goes on to the next iteration.
 // The states equal the begin-frame and two-launch iterations' bit for bit.
 //
@@ -1147,14 +1230,8 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
                 total[threadIdx.x] = t;
             }
             __syncthreads();
-            if (threadIdx.x == 0) {
-                double R0[9], t0[3];
-#pragma unroll
-                for (int i = 0; i < 9; ++i) R0[i] = s.R[i];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) t0[i] = s.t[i];
-                solve_and_update(&s, total, R0, t0, s.iterations, tag, a.conv_rot, a.conv_trans);
-            }
+            if (threadIdx.x < 64)
+                solve_and_update(&s, total, s.R, s.t, s.iterations, tag, a.conv_rot, a.conv_trans, threadIdx.x);
             __syncthreads();
             ++k;
         }
@@ -1193,6 +1270,22 @@ __global__ void icp_begin_frame_kernel(IcpState* st) {
     st->converged_tag = -1;
     st->iterations = 0;
     st->fault = 0;
+}
+
+// The solve and pose update of an iteration applied to 27 sums in device
+// memory (one wave).
+__global__ void __launch_bounds__(64) icp_solve_sums_kernel(IcpState* __restrict__ st,
+                                                             const double* __restrict__ sums) {
+    __shared__ double total[32];
+    if (threadIdx.x < kIcpEntries) total[threadIdx.x] = sums[threadIdx.x];
+    double R0[9], t0[3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R0[i] = st->R[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) t0[i] = st->t[i];
+    const int iterations = st->iterations;
+    __syncthreads();
+    solve_and_update(st, total, R0, t0, iterations, -1, 0.0, 0.0, threadIdx.x);
 }
 
 __global__ void icp_translate_kernel(IcpState* st, double dx, double dy, double dz) {
@@ -1456,7 +1549,7 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
     kfx_icp* h = new kfx_icp();
     h->max_blocks = (int)divup((unsigned)max_pixels, (unsigned)kIcpThreads);
     hipError_t e = hipMalloc((void**)&h->state, sizeof(IcpState));
-    h->sum_mode = KFX_ICP_SUM_FP32_KAHAN;
+    h->sum_mode = KFX_ICP_SUM_FP64;
     h->host_state = nullptr;
     h->partials = nullptr;
     h->publish_next = false;
@@ -1591,6 +1684,18 @@ int kfx_icp_begin_frame(kfx_icp* h) {
     ProfScope prof("icp_begin_frame");
     icp_begin_frame_kernel<<<1, 64, 0, stream()>>>(h->state);
     KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_icp_solve_sums(kfx_icp* h, const double sums[27]) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(h && sums, "kfx_icp_solve_sums: null pointer");
+    // The partial-sum buffer holds at least 27 doubles; the copy and the
+    // solve are ordered on the stream after any iteration enqueued before.
+    KFX_CHECK_HIP(hipMemcpyAsync(h->partials, sums, sizeof(double) * kIcpEntries, hipMemcpyHostToDevice, stream()));
+    icp_solve_sums_kernel<<<1, 64, 0, stream()>>>(h->state, static_cast<const double*>(h->partials));
+    KFX_CHECK_LAUNCH();
+    h->published = false;
     return KFX_OK;
 }
 

@@ -8,8 +8,8 @@
 //     iteration downloads the 27 sums (Estimate_Combined) and the host solves
 //     and updates the pose in double precision; fixed iteration counts.
 //   KFX_PIPELINE_FUSED — the pose lives on the GPU: one fused launch per ICP
-//     iteration (FP64 sums of the FP32 products, Cholesky and
-//     pose update in the last workgroup), optional early exit per level,
+//     iteration (FP64 sums of the FP32 products, solve and
+//     pose update on the GPU), optional early exit per level,
 //     integration and raycast read the pose from device memory, and the whole
 //     frame can be replayed as one HIP graph; the host reads the pose once per
 //     frame to decide the cyclic shift.
@@ -455,6 +455,7 @@ int kfx_pipeline_default_config(kfx_pipeline_config* c) {
     c->colors = 0;
     c->color_max_weight = KFX_DEFAULT_COLOR_MAX_WEIGHT;
     c->icp_launch = KFX_ICP_LAUNCH_FRAME;
+    c->icp_sum_mode = KFX_ICP_SUM_FP64;
     return KFX_OK;
 }
 

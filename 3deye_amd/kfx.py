@@ -288,6 +288,7 @@ _PROTOS = {
     "kfx_icp_begin_frame": (_I, [_P]),
     "kfx_icp_frame": (_I, [_P, C.POINTER(IcpLevel), _I, C.c_float, C.c_float, _I, C.c_double, C.c_double]),
     "kfx_icp_frame_resident": (_I, [_P, C.POINTER(_I)]),
+    "kfx_icp_solve_sums": (_I, [_P, _P]),
     "kfx_icp_translate": (_I, [_P, C.c_double, C.c_double, C.c_double]),
     "kfx_icp_pose_device": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "kfx_determine_extract_step": (_I, [_I, C.POINTER(_I), C.POINTER(_I)]),

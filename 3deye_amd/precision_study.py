@@ -1,5 +1,5 @@
 """ICP precision study on the GPU: the fused pipeline with FP32-Kahan, plain
-FP32 and FP64 workgroup sums (cross-workgroup sum, Cholesky solve and pose
+FP32 and FP64 workgroup sums (cross-workgroup sum, 6x6 solve and pose
 composition in FP64 in every mode) over a long seeded trajectory.
 
 Workload: 512^3 volume (3.072 m cube), 640x480 depth of a workload-family

@@ -230,7 +230,7 @@ def kernel_table_learned(n_upd_per_launch, samples_per_launch):
     Algorithmic work per launch: denoiser 10,080 FLOP/px (bf16 MFMA hidden
     layers, against 2.5 PFLOP/s dense) and 4 B/px; pyramid level i
     2 (WH/4^(i-1)) + 2 (WH/4^i) B; vertex + normal maps of every level
-    26 B/px; ICP sums at level l 48 B/px + 216 B per workgroup (FP32-Kahan
+    26 B/px; ICP sums at level l 48 B/px + 216 B per workgroup (27 FP64
     partials), the solve 216 B per partial + the 600 B state; the gate 14 B/px
     + 32 B per workgroup, its reduce 32 B per workgroup; the integration
     pre-pass 2 + 4 + 4/3 B/px (depth in, scaled depth and pyramid out);
@@ -677,8 +677,8 @@ def main():
                                f"at frames 77/150/225/297; timed frames {first}-{n - 1} after {PREROLL} pre-roll "
                                f"and {args.warmup} warmup frames",
                    "pipeline": "learned: denoiser (bf16 MFMA) with bilateral fallback at 15 mm RMS, device-resident "
-                               "pose, one-launch frame ICP with FP32-Kahan workgroup sums and FP64 cross-workgroup "
-                               "sum, Cholesky and pose update, raycast skipping bricks without negative values; "
+                               "pose, one-launch frame ICP with FP64 workgroup and cross-workgroup sums, "
+                               "one-wave Gauss-Jordan solve and pose update, raycast skipping bricks without negative values; "
                                "kernels launched in order on one stream, one host synchronisation per frame",
                    "frames_timed": [first, n - 1], "shifts_timed": learned["shifts_timed"],
                    "extracted_points_timed": learned["extracted_points_timed"],

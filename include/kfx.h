@@ -302,10 +302,13 @@ typedef struct kfx_icp_result {
 
 /* Precision of the per-workgroup sums of the fused iteration; the
  * cross-workgroup sum, the 6x6 solve and the pose update are FP64 in every
- * mode.  FP32 Jacobian rows and products in every mode. */
-#define KFX_ICP_SUM_FP32_KAHAN 0 /* compensated (TwoSum) FP32 sums (default) */
+ * mode.  FP32 Jacobian rows and products in every mode.  FP64 is the default
+ * (of kfx_icp_create and kfx_pipeline_default_config): the most accurate and,
+ * on gfx950, the fastest (a wave's FP64 sum tree costs one half-rate add per
+ * step where a compensated FP32 pair merge costs about ten FP32 operations). */
+#define KFX_ICP_SUM_FP32_KAHAN 0 /* compensated (TwoSum) FP32 sums */
 #define KFX_ICP_SUM_FP32 1       /* plain FP32 sums */
-#define KFX_ICP_SUM_FP64 2       /* FP64 sums */
+#define KFX_ICP_SUM_FP64 2       /* FP64 sums (default) */
 
 int kfx_icp_create(int max_pixels, kfx_icp** out);
 int kfx_icp_destroy(kfx_icp* icp);
@@ -333,8 +336,7 @@ int kfx_icp_set_launch_mode(kfx_icp* icp, int mode);
 /* One iteration (one or two launches, kfx_icp_set_launch_mode): FP32
  * Jacobian rows and products, workgroup sums in the handle's KFX_ICP_SUM_*
  * precision (fixed order), FP64 cross-workgroup sum in one workgroup, FP64
- * 6x6 Cholesky
- * solve and pose composition on the GPU.  A singular system leaves the pose
+ * 6x6 solve (Gauss-Jordan over one wave) and pose composition on the GPU.  A singular system leaves the pose
  * unchanged and records KFX_ERR_SINGULAR.  When level_tag >= 0 and an
  * iteration's increment is below (converge_rotation rad, converge_translation
  * m), later iterations with the same tag return immediately (early exit
@@ -376,6 +378,15 @@ int kfx_icp_frame(kfx_icp* icp, const kfx_icp_level levels[KFX_ICP_LEVELS], int 
                   double converge_translation);
 /* 1 when kfx_icp_frame runs as one launch on this device (KFX_ICP_LAUNCH_FRAME mode). */
 int kfx_icp_frame_resident(kfx_icp* icp, int* resident);
+/* One solve and pose update from 27 sums given by the caller (host memory,
+ * in the unpacking order of common/src/opencl_wrappers.cpp:422-437: row k of
+ * the upper triangle of [A | b]), as an iteration applies them: x from
+ * A x = b, R <- R_inc(x) R, t <- R_inc(x) t + x[3..5]; a system that is not
+ * positive definite leaves the pose and counts as singular.  No reference
+ * counterpart: the reference hands A and b to its caller
+ * (opencl_wrappers.cpp:364-438, kfx_estimate_combined here), which solves on
+ * the host; this applies the GPU's solve to them.  Enqueued on the stream. */
+int kfx_icp_solve_sums(kfx_icp* icp, const double sums[27]);
 /* Add (dx, dy, dz) to the estimate's translation (re-expression after a
  * cyclic shift of the volume). */
 int kfx_icp_translate(kfx_icp* icp, double dx, double dy, double dz);
@@ -573,7 +584,7 @@ typedef struct kfx_pipeline_config {
     double converge_rotation;     /* fused early exit per level (rad); 0 disables */
     double converge_translation;  /* fused early exit per level (m) */
     int32_t extract_on_shift;     /* extract the leaving slab before clearing it */
-    int32_t icp_sum_mode;         /* fused mode: KFX_ICP_SUM_* workgroup sums (0: FP32 Kahan) */
+    int32_t icp_sum_mode;         /* fused mode: KFX_ICP_SUM_* workgroup sums (default FP64) */
     double shift_target_distance; /* target point = camera + this along the optical axis (m); 0: camera centre */
     int32_t colors;               /* 1: keep a colour volume; frames then come with RGB (kfx_pipeline_process_rgbd) */
     int32_t color_max_weight;     /* colour running-average weight cap, 1..255 (reference default 1) */

@@ -416,3 +416,72 @@ def test_frame_argument_checks(gpu, pair, pyr_bufs):
     lv[2].vmap_current = None
     assert kfx.lib.kfx_icp_frame(h, lv, 1, ii.DIST_THR, ii.ANGLE_THR, 0, 0.0, 0.0) != 0
     kfx.call("kfx_icp_destroy", h)
+
+
+def _pack_sums(A, b):
+    """[A | b] -> the 27 sums (row k of its upper triangle; opencl_wrappers.cpp:426-437)."""
+    return np.array([A[i, j] if j < 6 else b[i] for i in range(6) for j in range(i, 7)], np.float64)
+
+
+@pytest.mark.parametrize("angles", [(1e-3, -2e-3, 5e-4),      # the series
+                                    (0.125, -0.1, 0.05),       # the series' boundary
+                                    (0.3, -1.2, 2.9),          # halving and double-angle steps
+                                    (3.5, -7.5, 100.0),        # reduction by 2 pi first
+                                    (-0.2, 0.01, -1000.0)])
+def test_solve_sums_matches_oracle(gpu, orc, angles):
+    # The GPU's solve (Gauss-Jordan over one wave) and pose update, against
+    # the oracle's Cholesky solve with libm sin/cos, on a positive definite
+    # system whose solution holds the given angles; every increment angle
+    # range of the update is reached (series, halving, reduction by 2 pi).
+    from scipy.spatial.transform import Rotation
+    kfx = gpu
+    rng = np.random.default_rng(abs(hash(angles)) % 2**32)
+    Q, _ = np.linalg.qr(rng.normal(size=(6, 6)))
+    A = (Q * np.geomspace(1.0, 1e3, 6)) @ Q.T * 1e4
+    A = (A + A.T) / 2
+    x_true = np.array(list(angles) + list(rng.normal(scale=0.05, size=3)))
+    sums = _pack_sums(A, A @ x_true)
+    R0 = Rotation.from_rotvec(rng.normal(scale=0.5, size=3)).as_matrix()
+    t0 = rng.normal(size=3)
+    h = _icp_handle(kfx, 1024)
+    kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, R0, t0)))
+    kfx.call("kfx_icp_solve_sums", h, sums.ctypes.data_as(C.c_void_p))
+    res = kfx.IcpResult()
+    kfx.call("kfx_icp_read", h, C.byref(res))
+    rc, R, t, x = orc.icp_solve_update(sums, R0, t0)
+    assert rc == 0 and res.status == 0 and res.iterations == 1
+    # Both solves are backward stable: they differ by about
+    # cond(A) eps max|x| (cond(A) = 1e3), the poses by as much.
+    tol = 1e3 * 1e-15 * max(1.0, np.abs(x_true).max())
+    np.testing.assert_allclose(np.array(res.x[:]), x, rtol=0, atol=tol)
+    np.testing.assert_allclose(x, x_true, rtol=0, atol=tol)
+    np.testing.assert_allclose(np.array(res.pose.R[:]).reshape(3, 3), R, atol=tol)
+    np.testing.assert_allclose(np.array(res.pose.t[:]), t, atol=tol)
+    Rg = np.array(res.pose.R[:]).reshape(3, 3)
+    np.testing.assert_allclose(Rg @ Rg.T, np.eye(3), atol=1e-13)
+    np.testing.assert_array_equal(np.array(res.A[:]).reshape(6, 6), orc.unpack_sums(sums)[0])
+    kfx.call("kfx_icp_destroy", h)
+
+
+@pytest.mark.parametrize("bad", ["zero", "nan", "indefinite"])
+def test_solve_sums_singular_keeps_pose(gpu, orc, bad):
+    kfx = gpu
+    A = np.diag([4.0, 3.0, 2.0, 5.0, 6.0, 7.0])
+    if bad == "zero":
+        A[:] = 0.0
+    elif bad == "nan":
+        A[2, 2] = np.nan
+    else:
+        A[4, 4] = -1.0
+    sums = _pack_sums(A, np.ones(6))
+    R0, t0 = np.eye(3), np.array([0.1, 0.2, 0.3])
+    h = _icp_handle(kfx, 1024)
+    kfx.call("kfx_icp_reset", h, C.byref(_pose(kfx, R0, t0)))
+    kfx.call("kfx_icp_solve_sums", h, sums.ctypes.data_as(C.c_void_p))
+    res = kfx.IcpResult()
+    kfx.call("kfx_icp_read", h, C.byref(res))
+    rc, *_ = orc.icp_solve_update(sums, R0, t0)
+    assert rc == 1 and res.status == 5 and res.singular_count == 1 and res.iterations == 1
+    assert np.array_equal(np.array(res.pose.R[:]), R0.reshape(9)) and np.array_equal(np.array(res.pose.t[:]), t0)
+    assert np.array_equal(np.array(res.x[:]), np.zeros(6))
+    kfx.call("kfx_icp_destroy", h)
