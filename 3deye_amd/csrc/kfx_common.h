@@ -222,6 +222,16 @@ struct FrameGate {
 int icp_frame_gated(kfx_icp* h, const kfx_icp_level* levels, int begin_frame, float distance_threshold,
                     float angle_threshold, int tagged, double converge_rotation, double converge_translation,
                     const FrameGate* gate, const Workspace* gate_ws);
+// A publishing frame launch (kfx_icp_frame as one launch after
+// kfx_icp_publish_next) raises a pinned completion flag right after its host
+// copy of the state and the gate's host outputs.  icp_clear_done lowers it
+// (a host store; call it before enqueueing or replaying such a launch, with
+// none in flight).  icp_read_published_early returns the published state as
+// soon as the flag is up, without waiting for the work enqueued after the
+// launch; when the last publishing launch was not a frame launch it is
+// kfx_icp_read_published.
+void icp_clear_done(kfx_icp* h);
+int icp_read_published_early(kfx_icp* h, kfx_icp_result* out);
 // The fallback gate (kfx_denoise.hip); ws.cons must hold cons_blocks().
 int depth_consistency(const Workspace& ws, const kfx_xyz* vmap_model, const float* dev_pose, const uint16_t* depth,
                       const kfx_intrinsics* intr, float max_diff, double* dev_stats, double fallback_m,

@@ -984,6 +984,7 @@ struct IcpFrameArgs {
                          // double bits or kEmptyEntry
     unsigned* epoch;     // iterations run by earlier launches (buffer numbering)
     IcpState* mirror;    // pinned host copy of the final state, or null
+    int32_t* host_done;  // pinned flag raised after the host copy (with mirror), or null
     int begin;           // start of a frame (icp_begin_frame_kernel's update first)
     int tagged;          // level l's iterations carry tag l (early exit)
     double conv_rot, conv_trans;
@@ -997,12 +998,14 @@ __device__ __forceinline__ double group_total(const IcpGroupSum<M>& g) {
     else return (double)g.x + (double)g.y;
 }
 
-// State copy by 8-byte words, spread over the workgroup.
-__device__ __forceinline__ void copy_state(IcpState* dst, const IcpState* src) {
+// State copy by 8-byte words, spread over the workgroup (over the first
+// wave: first_wave = true).
+__device__ __forceinline__ void copy_state(IcpState* dst, const IcpState* src, bool first_wave = false) {
     static_assert(sizeof(IcpState) % 8 == 0, "state copy by 8-byte words");
     const uint2* s = reinterpret_cast<const uint2*>(src);
     uint2* d = reinterpret_cast<uint2*>(dst);
-    for (int i = threadIdx.x; i < (int)(sizeof(IcpState) / 8); i += blockDim.x) d[i] = s[i];
+    const int step = first_wave ? 64 : (int)blockDim.x;
+    for (int i = threadIdx.x; i < (int)(sizeof(IcpState) / 8); i += step) d[i] = s[i];
 }
 
 // The fallback gate at the final pose (depth_consistency_kernel and
@@ -1247,7 +1250,15 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
         }
         __syncthreads();
         copy_state(a.st, &s);
-        if (a.mirror != nullptr) copy_state(a.mirror, &s);
+        if (a.mirror != nullptr && threadIdx.x < 64) {
+            // The host copy by the first wave, which wrote the gate's host
+            // outputs too; then the host's completion flag, released at
+            // system scope after all of them (the host reads the frame's
+            // result from there while later work runs on).
+            copy_state(a.mirror, &s, true);
+            if (a.host_done != nullptr && threadIdx.x == 0)
+                __hip_atomic_store(a.host_done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -1341,6 +1352,8 @@ struct kfx_icp {
     int launch_mode;    // KFX_ICP_LAUNCH_*
     uint64_t* frame_entries;  // the one-launch frame's stage 1 and stage 2 buffers (kFrameEntries words)
     bool frame_resident;     // the device holds all kFrameBlocks workgroups of icp_frame_kernel at once
+    int32_t* host_done;      // pinned flag the publishing frame launch raises after the host copy
+    bool done_raised;        // the last publishing launch was a frame launch (it raises host_done)
 };
 
 using namespace kfx;
@@ -1406,6 +1419,7 @@ int kfx::icp_frame_gated(kfx_icp* h, const kfx_icp_level* levels, int begin_fram
     // passes its checks.
     const bool publish = h != nullptr && h->publish_next;
     if (h != nullptr) h->publish_next = false;
+    if (publish) h->done_raised = false;  // until a frame launch raises the flag
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(h && levels, "kfx_icp_frame: null pointer");
     KFX_REQUIRE(gate == nullptr || (gate->vmap_model && gate->depth && gate->stats && gate->flag && gate_ws &&
@@ -1464,6 +1478,7 @@ int kfx::icp_frame_gated(kfx_icp* h, const kfx_icp_level* levels, int begin_fram
     a.entries = h->frame_entries;
     a.epoch = h->ticket + 1;
     a.mirror = (publish && last >= 0) ? h->host_state : nullptr;
+    a.host_done = a.mirror != nullptr ? h->host_done : nullptr;
     a.begin = begin_frame ? 1 : 0;
     a.tagged = tagged ? 1 : 0;
     a.conv_rot = converge_rotation;
@@ -1487,8 +1502,50 @@ int kfx::icp_frame_gated(kfx_icp* h, const kfx_icp_level* levels, int begin_fram
         else launch_mode(std::false_type{});
         KFX_CHECK_LAUNCH();
     }
-    if (a.mirror != nullptr) h->published = true;
+    if (a.mirror != nullptr) {
+        h->published = true;
+        h->done_raised = true;
+    }
     return KFX_OK;
+}
+
+// The host copy of the state as a result (after the copy is complete).
+static int read_host_state(kfx_icp* h, kfx_icp_result* out) {
+    const IcpState& s = *h->host_state;
+    if (s.fault != 0)
+        return set_error(KFX_ERR_HIP, "%s: a grid barrier wait of the one-launch frame ran out (the state is not valid)",
+                         __func__);
+    for (int i = 0; i < 9; ++i) out->pose.R[i] = s.R[i];
+    for (int i = 0; i < 3; ++i) out->pose.t[i] = s.t[i];
+    for (int i = 0; i < 36; ++i) out->A[i] = s.A[i];
+    for (int i = 0; i < 6; ++i) { out->b[i] = s.b[i]; out->x[i] = s.x[i]; }
+    out->status = s.status;
+    out->singular_count = s.singular_count;
+    out->iterations = s.iterations;
+    out->converged_tag = s.converged_tag;
+    return KFX_OK;
+}
+
+void kfx::icp_clear_done(kfx_icp* h) { __atomic_store_n(h->host_done, 0, __ATOMIC_RELEASE); }
+
+int kfx::icp_read_published_early(kfx_icp* h, kfx_icp_result* out) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(h && out, "icp_read_published_early: null pointer");
+    if (!h->done_raised || !h->published) return kfx_icp_read_published(h, out);
+    // Spin on the flag; every 256 polls ask whether the stream has ended
+    // (then the flag must be up) or failed, so a failed launch cannot hold
+    // the host here.
+    for (unsigned polls = 1; __atomic_load_n(h->host_done, __ATOMIC_ACQUIRE) == 0; ++polls) {
+        if (polls % 256 != 0) continue;
+        const hipError_t q = hipStreamQuery(stream());
+        if (q == hipErrorNotReady) continue;
+        KFX_CHECK_HIP(q);
+        if (__atomic_load_n(h->host_done, __ATOMIC_ACQUIRE) == 0)
+            return set_error(KFX_ERR_INVALID_ARG, "%s: the stream ended without the frame launch's completion flag",
+                             __func__);
+        break;
+    }
+    return read_host_state(h, out);
 }
 
 extern "C" {
@@ -1558,6 +1615,8 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
     h->launch_mode = KFX_ICP_LAUNCH_FRAME;
     h->frame_entries = nullptr;
     h->frame_resident = frame_kernel_resident();
+    h->host_done = nullptr;
+    h->done_raised = false;
     if (e == hipSuccess) e = hipMalloc((void**)&h->partials, sizeof(double) * kIcpEntries * (size_t)h->max_blocks);
     if (e == hipSuccess)
         e = hipMalloc((void**)&h->frame_entries, sizeof(uint64_t) * kFrameEntries);
@@ -1569,6 +1628,8 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipHostMalloc((void**)&h->host_state, sizeof(IcpState), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&h->host_done, 64, hipHostMallocDefault);
+    if (e == hipSuccess) *h->host_done = 0;
     if (e == hipSuccess) e = hipMemsetAsync(h->state, 0, sizeof(IcpState), stream());
     if (e == hipSuccess) e = hipStreamSynchronize(stream());
     if (e != hipSuccess) {
@@ -1577,6 +1638,7 @@ int kfx_icp_create(int max_pixels, kfx_icp** out) {
         (void)hipFree(h->frame_entries);
         (void)hipFree(h->ticket);
         if (h->host_state) (void)hipHostFree(h->host_state);
+        if (h->host_done) (void)hipHostFree(h->host_done);
         delete h;
         return set_error(KFX_ERR_HIP, "kfx_icp_create: %s", hipGetErrorString(e));
     }
@@ -1593,6 +1655,7 @@ int kfx_icp_destroy(kfx_icp* h) {
     (void)hipFree(h->frame_entries);
     (void)hipFree(h->ticket);
     (void)hipHostFree(h->host_state);
+    (void)hipHostFree(h->host_done);
     delete h;
     return KFX_OK;
 }
@@ -1620,6 +1683,7 @@ int kfx_icp_iterate(kfx_icp* h, const kfx_xyz* vmap_current, const kfx_xyz* nmap
     // passes its checks.
     const bool publish = h != nullptr && h->publish_next;
     if (h != nullptr) h->publish_next = false;
+    if (publish) h->done_raised = false;  // until a frame launch raises the flag
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(h && vmap_current && nmap_current && vmap_previous && nmap_previous && intr,
                 "kfx_icp_iterate: null pointer");
@@ -1713,19 +1777,7 @@ int kfx_icp_read(kfx_icp* h, kfx_icp_result* out) {
     KFX_REQUIRE(h && out, "kfx_icp_read: null pointer");
     KFX_CHECK_HIP(hipMemcpyAsync(h->host_state, h->state, sizeof(IcpState), hipMemcpyDeviceToHost, stream()));
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
-    const IcpState& s = *h->host_state;
-    if (s.fault != 0)
-        return set_error(KFX_ERR_HIP, "%s: a grid barrier wait of the one-launch frame ran out (the state is not valid)",
-                         __func__);
-    for (int i = 0; i < 9; ++i) out->pose.R[i] = s.R[i];
-    for (int i = 0; i < 3; ++i) out->pose.t[i] = s.t[i];
-    for (int i = 0; i < 36; ++i) out->A[i] = s.A[i];
-    for (int i = 0; i < 6; ++i) { out->b[i] = s.b[i]; out->x[i] = s.x[i]; }
-    out->status = s.status;
-    out->singular_count = s.singular_count;
-    out->iterations = s.iterations;
-    out->converged_tag = s.converged_tag;
-    return KFX_OK;
+    return read_host_state(h, out);
 }
 
 int kfx_icp_publish_next(kfx_icp* h) {
@@ -1740,19 +1792,7 @@ int kfx_icp_read_published(kfx_icp* h, kfx_icp_result* out) {
     KFX_REQUIRE(h->published,
                 "kfx_icp_read_published: no publishing iteration was enqueued since the last reset or translate");
     KFX_CHECK_HIP(hipStreamSynchronize(stream()));
-    const IcpState& s = *h->host_state;
-    if (s.fault != 0)
-        return set_error(KFX_ERR_HIP, "%s: a grid barrier wait of the one-launch frame ran out (the state is not valid)",
-                         __func__);
-    for (int i = 0; i < 9; ++i) out->pose.R[i] = s.R[i];
-    for (int i = 0; i < 3; ++i) out->pose.t[i] = s.t[i];
-    for (int i = 0; i < 36; ++i) out->A[i] = s.A[i];
-    for (int i = 0; i < 6; ++i) { out->b[i] = s.b[i]; out->x[i] = s.x[i]; }
-    out->status = s.status;
-    out->singular_count = s.singular_count;
-    out->iterations = s.iterations;
-    out->converged_tag = s.converged_tag;
-    return KFX_OK;
+    return read_host_state(h, out);
 }
 
 int kfx_icp_pose_device(kfx_icp* h, const float** dev_pose, const float** dev_prev_pose_inv) {

@@ -56,6 +56,7 @@ struct kfx_pipeline {
     uint64_t total_extracted = 0;
     int shifts = 0;
     int fallbacks = 0;
+    bool icp_in_flight = false;           // a publishing ICP launch was enqueued and its result not yet read
 };
 
 using namespace kfx;
@@ -286,7 +287,20 @@ int map_part(kfx_pipeline* p) { return map_part_skip(p, nullptr); }
 // a read when the configuration runs no ICP iteration.
 int icp_read_frame(kfx_pipeline* p, kfx_icp_result* r) {
     const int n = p->cfg.icp_iterations[0] + p->cfg.icp_iterations[1] + p->cfg.icp_iterations[2];
-    return n > 0 ? kfx_icp_read_published(p->icp, r) : kfx_icp_read(p->icp, r);
+    KFX_TRY(n > 0 ? icp_read_published_early(p->icp, r) : kfx_icp_read(p->icp, r));
+    p->icp_in_flight = false;
+    return KFX_OK;
+}
+
+// Before a fused frame's ICP is enqueued (or replayed): the completion flag
+// of its launch lowered, once no earlier publishing launch can still raise
+// it (after a frame that ended in an error before its result was read, the
+// stream is drained first).
+int arm_icp_read(kfx_pipeline* p) {
+    if (p->icp_in_flight) KFX_CHECK_HIP(hipStreamSynchronize(stream()));
+    icp_clear_done(p->icp);
+    p->icp_in_flight = true;
+    return KFX_OK;
 }
 
 int learned_frame(kfx_pipeline* p) {
@@ -603,6 +617,7 @@ static int process_frame(kfx_pipeline* p, const uint16_t* depth, const kfx_color
         KFX_TRY(colors_host_pose(p));
         if (info) info->integrated = integ ? 1 : 0;
     } else if (!use_denoiser(p)) {
+        KFX_TRY(arm_icp_read(p));
         KFX_TRY(run_part(p, bilateral_frame, &p->graph, &p->graph_exec));
     } else {
         // The learned frame as one graph; the gate's statistics and decision
@@ -611,8 +626,9 @@ static int process_frame(kfx_pipeline* p, const uint16_t* depth, const kfx_color
         // When the gate fell back, the device skipped integration and
         // raycast: the frame is redone with the bilateral filter, ICP from
         // the previous pose, then integration and raycast.
+        KFX_TRY(arm_icp_read(p));
         KFX_TRY(run_part(p, learned_frame, &p->graph, &p->graph_exec));
-        KFX_TRY(icp_read_frame(p, &r));  // synchronises the stream
+        KFX_TRY(icp_read_frame(p, &r));  // waits for the ICP launch only
         have_pose = true;
         const double cnt = p->hstats[1];
         const double rms_mm = cnt > 0 ? 1000.0 * sqrt(p->hstats[0] / cnt) : 0.0;
@@ -627,6 +643,7 @@ static int process_frame(kfx_pipeline* p, const uint16_t* depth, const kfx_color
             memcpy(prev.t, p->tprev, sizeof(prev.t));
             KFX_TRY(kfx_icp_reset(p->icp, &prev));
             KFX_TRY(preprocess(p, false));
+            KFX_TRY(arm_icp_read(p));
             KFX_TRY(icp_part(p, false));
             KFX_TRY(map_part(p));
             p->fallbacks += 1;
@@ -682,6 +699,8 @@ int kfx_pipeline_set_denoiser(kfx_pipeline* p, kfx_denoiser* dn) {
 
 int kfx_pipeline_state(kfx_pipeline* p, kfx_pipeline_view* v) {
     KFX_REQUIRE(p && v, "kfx_pipeline_state: null pointer");
+    KFX_REQUIRE_INIT();
+    KFX_CHECK_HIP(hipStreamSynchronize(stream()));
     v->volume = p->vol;
     v->round_buffer = p->rb;
     v->cell_size = p->cell;

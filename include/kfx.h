@@ -638,8 +638,14 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
                         const double volume_origin_world[3], kfx_pipeline** out);
 int kfx_pipeline_destroy(kfx_pipeline* pipeline);
 /* Process one depth frame (device pointer, cols x rows uint16 mm).  Returns
- * when the frame's pose is known on the host (one synchronisation per frame
- * in fused mode; one per step in plain mode). */
+ * when the frame's pose is known on the host: in fused mode as soon as the
+ * frame's ICP launch has published it (the host waits on a flag that launch
+ * raises), while the frame's integration, raycast and map update may still
+ * run on the library stream; every later library call on that stream is
+ * ordered after them (kfx_synchronize waits for them; kfx_pipeline_state
+ * does too).  The depth buffer is read by a copy on the library stream:
+ * overwrite it through the library (kfx_memcpy_htod) or after
+ * kfx_synchronize.  One synchronisation per step in plain mode. */
 int kfx_pipeline_process(kfx_pipeline* pipeline, const uint16_t* depth, kfx_frame_info* info);
 /* Process one depth frame with its registered colour frame (device pointer,
  * cols x rows kfx_color): as kfx_pipeline_process, then the colour volume
@@ -648,6 +654,8 @@ int kfx_pipeline_process(kfx_pipeline* pipeline, const uint16_t* depth, kfx_fram
  * fails for that pipeline. */
 int kfx_pipeline_process_rgbd(kfx_pipeline* pipeline, const uint16_t* depth, const kfx_color* rgb,
                               kfx_frame_info* info);
+/* The pipeline's buffers and counters, after the work enqueued so far has
+ * completed (it synchronises the library stream). */
 int kfx_pipeline_state(kfx_pipeline* pipeline, kfx_pipeline_view* view);
 /* Attach a trained denoiser (fused mode; used when cfg.denoiser is set).  The
  * pipeline does not own it. */
