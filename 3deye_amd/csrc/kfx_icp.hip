@@ -1211,30 +1211,39 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
                 if ((threadIdx.x & 1) == 0 && (threadIdx.x >> 1) < kIcpEntries)
                     store_coherent(&cur2[w * 32 + (threadIdx.x >> 1)], (uint64_t)__double_as_longlong(wtot));
             }
-            // Stage 2: thread q * 27 + e polls wave q's total of entry e.
-            if (threadIdx.x < kIcpWaves * kIcpEntries) {
-                const int q = threadIdx.x / kIcpEntries, e = threadIdx.x % kIcpEntries;
-                uint64_t x = load_coherent(&cur2[q * 32 + e]);
-                for (unsigned polls = 1; x == kEmptyEntry; ++polls) {
-                    if (polls == kPollLimit) {
-                        fault = 1;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    x = load_coherent(&cur2[q * 32 + e]);
-                }
-                wsum_x[q][e] = __longlong_as_double((long long)x);
-            }
-            __syncthreads();
-            if (threadIdx.x < kIcpEntries) {
-                double t = wsum_x[0][threadIdx.x];
+            // Stage 2 and the solve in the first wave: lane e < 27 polls
+            // the four wave totals of entry e and adds them in wave order,
+            // then the wave solves; one barrier after it.
+            if (threadIdx.x < 64) {
+                if (threadIdx.x < kIcpEntries) {
+                    const int e = threadIdx.x;
+                    uint64_t x[kIcpWaves];
+                    for (unsigned polls = 1;; ++polls) {
+                        bool ready = true;
 #pragma unroll
-                for (int q = 1; q < kIcpWaves; ++q) t += wsum_x[q][threadIdx.x];
-                total[threadIdx.x] = t;
-            }
-            __syncthreads();
-            if (threadIdx.x < 64)
+                        for (int q = 0; q < kIcpWaves; ++q) {
+                            x[q] = load_coherent(&cur2[q * 32 + e]);
+                            ready = ready && x[q] != kEmptyEntry;
+                        }
+                        if (ready) break;
+                        if (polls == kPollLimit) {
+                            fault = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    double t = __longlong_as_double((long long)x[0]);
+#pragma unroll
+                    for (int q = 1; q < kIcpWaves; ++q) t += __longlong_as_double((long long)x[q]);
+                    total[e] = t;
+                }
+                // The other lanes read total[] after these stores: one wave,
+                // whose LDS operations complete in order.
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 solve_and_update(&s, total, s.R, s.t, s.iterations, tag, a.conv_rot, a.conv_trans, threadIdx.x);
+            }
             __syncthreads();
             ++k;
         }
