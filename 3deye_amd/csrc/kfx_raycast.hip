@@ -267,6 +267,17 @@ __device__ __forceinline__ bool march_skip(const kfx_tsdf* __restrict__ vol, con
     const float fnx = (float)rb.voxels_size.x, fny = (float)rb.voxels_size.y, fnz = (float)rb.voxels_size.z;
     const f3 idir{1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     const f3 bsz{(float)kRayBrick * p.csx, (float)kRayBrick * p.csy, (float)kRayBrick * p.csz};
+    // Brick b's shrunk box along an axis is entered at (b bsz + e - start) /
+    // dir and left at (b bsz + x - start) / dir, with the faces taken by the
+    // sign of dir: e = m or bsz - m, x = bsz - m or m (m the margin).
+    const auto offsets = [&](float d, float bs, float st, float& e, float& x) {
+        e = (d > 0.0f ? kRayBrickMargin : bs - kRayBrickMargin) - st;
+        x = (d > 0.0f ? bs - kRayBrickMargin : kRayBrickMargin) - st;
+    };
+    f3 oe, ox;
+    offsets(dir.x, bsz.x, start.x, oe.x, ox.x);
+    offsets(dir.y, bsz.y, start.y, oe.y, ox.y);
+    offsets(dir.z, bsz.z, start.z, oe.z, ox.z);
     bool known = true;  // tsdf holds the previous sample's value (else it is >= 0, at time t_unk)
     float t_unk = 0.0f;
     for (;;) {
@@ -278,26 +289,24 @@ __device__ __forceinline__ bool march_skip(const kfx_tsdf* __restrict__ vol, con
         if (!((fx >= 0.0f) & (fx < fnx) & (fy >= 0.0f) & (fy < fny) & (fz >= 0.0f) & (fz < fnz))) return false;
         const uint32_t gx = (uint32_t)(int)fx, gy = (uint32_t)(int)fy, gz = (uint32_t)(int)fz;
         const uint32_t bx = gx >> kRayBrickShift, by = gy >> kRayBrickShift, bz = gz >> kRayBrickShift;
-        const uint32_t addr = march_address(gx, gy, gz, rb);
         if (p.bricks[(bz * (uint32_t)p.nby + by) * (uint32_t)p.nbx + bx] == 0) {
             // This sample is not negative; loaded only after a negative one
             // (a positive value ends the march there).
             if (known && tsdf < 0) {
-                const int cur = vol[addr].value;
+                const int cur = vol[march_address(gx, gy, gz, rb)].value;
                 if (cur > 0) return false;  // - to +
                 tsdf = cur;                 // 0
             } else {
                 known = false;
             }
-            // Samples certainly inside the brick: times in [t_lo, t_hi).
-            const float lx = (float)bx * bsz.x + kRayBrickMargin, hx = (float)(bx + 1) * bsz.x - kRayBrickMargin;
-            const float ly = (float)by * bsz.y + kRayBrickMargin, hy = (float)(by + 1) * bsz.y - kRayBrickMargin;
-            const float lz = (float)bz * bsz.z + kRayBrickMargin, hz = (float)(bz + 1) * bsz.z - kRayBrickMargin;
-            const float ax = (lx - start.x) * idir.x, cx = (hx - start.x) * idir.x;
-            const float ay = (ly - start.y) * idir.y, cy = (hy - start.y) * idir.y;
-            const float az = (lz - start.z) * idir.z, cz = (hz - start.z) * idir.z;
-            const float t_lo = fmaxf(fmaxf(fminf(ax, cx), fminf(ay, cy)), fminf(az, cz));
-            const float t_hi = fminf(fminf(fmaxf(ax, cx), fmaxf(ay, cy)), fmaxf(az, cz));
+            // Samples certainly inside the brick: times in [t_lo, t_hi) (the
+            // box shrunk by the margin, which covers the rounding of these
+            // few operations).
+            const float fbx = (float)bx, fby = (float)by, fbz = (float)bz;
+            const float t_lo = fmaxf(fmaxf(fmaf(fbx, bsz.x, oe.x) * idir.x, fmaf(fby, bsz.y, oe.y) * idir.y),
+                                     fmaf(fbz, bsz.z, oe.z) * idir.z);
+            const float t_hi = fminf(fminf(fmaf(fbx, bsz.x, ox.x) * idir.x, fmaf(fby, bsz.y, ox.y) * idir.y),
+                                     fmaf(fbz, bsz.z, ox.z) * idir.z);
             float tt = tn;
             for (;;) {
                 const float tnn = tt + step;
@@ -317,7 +326,7 @@ __device__ __forceinline__ bool march_skip(const kfx_tsdf* __restrict__ vol, con
         bool ok[kMarchBatch];
         tcs[0] = t;
         tns[0] = tn;
-        addrs[0] = addr;
+        addrs[0] = march_address(gx, gy, gz, rb);
         ok[0] = true;
         {
             float tt = tn;
@@ -382,8 +391,12 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
         for (int i = 0; i < 9; ++i) p.r[i] = dpose[i];
         p.tx = dpose[9]; p.ty = dpose[10]; p.tz = dpose[11];
     }
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    // A workgroup takes a 16x16 pixel tile, each wave an 8x8 quarter of it:
+    // rays of one wave stay close, so their marches diverge less than a
+    // 64-pixel row's.
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x = blockIdx.x * 16 + 8 * (wave & 1) + (lane & 7);
+    const int y = blockIdx.y * 16 + 8 * (wave >> 1) + (lane >> 3);
     if (x >= p.cols || y >= p.rows) return;
     const int idx = p.cols * y + x;
     kfx_xyz vout{NAN, NAN, NAN};
@@ -602,8 +615,8 @@ static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* 
     KFX_REQUIRE(bricks == nullptr || brick_flag_bytes(rb->voxels_size) > 0,
                 "kfx_raycast: brick flags need volume axes that are multiples of %d", kRayBrick);
     ProfScope prof("raycast");
-    dim3 block(64, 4);
-    dim3 grid(divup(intr->cols, 64), divup(intr->rows, 4));
+    dim3 block(64, 4);  // 16x16 pixel tiles (the kernel's mapping)
+    dim3 grid(divup(intr->cols, 16), divup(intr->rows, 16));
     if (bricks != nullptr && dev_pose)
         raycast_kernel<true, false, true><<<grid, block, 0, stream()>>>(volume, vertex_map, normal_map, p, dev_pose,
                                                                         dev_skip);
