@@ -556,8 +556,13 @@ integrate_kernel(const float* __restrict__ scaled, kfx_tsdf* __restrict__ volume
     }
     __syncthreads();
 
-    const int x = blk.bx * kIntegrateBlockX + threadIdx.x;
-    const int y = blk.by * kIntegrateBlockY + threadIdx.y;
+    // Wave w of the workgroup takes its 16 x 4 columns x0 + 16 w .. + 15,
+    // y0 .. y0 + 3 (not one row of 64): neighbouring columns' z ranges
+    // differ less, so the range the wave walks (the union of its lanes')
+    // is shorter.  A 16-lane group is one brick row.
+    const int lane = tid & 63, wave = tid >> 6;
+    const int x = blk.bx * kIntegrateBlockX + 16 * wave + (lane & 15);
+    const int y = blk.by * kIntegrateBlockY + (lane >> 4);
     const bool inside = x < p.nx && y < p.ny;
     const float gx = ((float)x + 0.5f) * p.csx - q.tx;
     const float gy = ((float)y + 0.5f) * p.csy - q.ty;
