@@ -57,6 +57,7 @@ struct kfx_pipeline {
     int shifts = 0;
     int fallbacks = 0;
     bool icp_in_flight = false;           // a publishing ICP launch was enqueued and its result not yet read
+    const uint16_t* in = nullptr;         // the frame's raw depth: the caller's buffer, or raw (graph replay)
 };
 
 using namespace kfx;
@@ -140,9 +141,9 @@ bool use_denoiser(const kfx_pipeline* p) { return p->cfg.denoiser && p->denoiser
 int preprocess(kfx_pipeline* p, bool learned) {
     const kfx_intrinsics& in = p->lvl[0];
     if (learned)
-        KFX_TRY(kfx_denoiser_forward(p->denoiser, p->raw, p->dep[0], nullptr));
+        KFX_TRY(kfx_denoiser_forward(p->denoiser, p->in, p->dep[0], nullptr));
     else
-        KFX_TRY(kfx_bilateral_filter(p->raw, p->dep[0], in.cols, in.rows, p->cfg.sigma_space, p->cfg.sigma_color,
+        KFX_TRY(kfx_bilateral_filter(p->in, p->dep[0], in.cols, in.rows, p->cfg.sigma_space, p->cfg.sigma_color,
                                      0xFFFFFFFFu));
     KFX_TRY(step_sync(p));
     for (int l = 1; l < KFX_ICP_LEVELS; ++l) {
@@ -185,7 +186,7 @@ int integrate_host_pose(kfx_pipeline* p) {
     mat_T(p->R, Ri);
     const kfx_matrix_3f m = to_m3f(Ri);
     const kfx_point_3f tt = to_p3f(p->t);
-    KFX_TRY(integrate_tsdf_ws(p->ws, true, p->raw, p->scaled, p->vol, &p->lvl[0], &m, &tt, p->cfg.truncation,
+    KFX_TRY(integrate_tsdf_ws(p->ws, true, p->in, p->scaled, p->vol, &p->lvl[0], &m, &tt, p->cfg.truncation,
                               &p->cell, &p->rb, p->bricks));
     return step_sync(p);
 }
@@ -261,7 +262,7 @@ int icp_part(kfx_pipeline* p, bool begin, const kfx::FrameGate* gate = nullptr) 
 int map_part_skip(kfx_pipeline* p, const int32_t* skip) {
     const float *dpose = nullptr, *dprev = nullptr;
     KFX_TRY(kfx_icp_pose_device(p->icp, &dpose, &dprev));
-    KFX_TRY(integrate_tsdf_posed_ws(p->ws, p->raw, p->scaled, p->vol, &p->lvl[0], dpose, dprev,
+    KFX_TRY(integrate_tsdf_posed_ws(p->ws, p->in, p->scaled, p->vol, &p->lvl[0], dpose, dprev,
                                     p->cfg.integrate_threshold, p->cfg.truncation, &p->cell, &p->rb, skip,
                                     p->bricks));
     if (p->bricks)
@@ -591,7 +592,15 @@ int kfx_pipeline_create(const kfx_pipeline_config* cfg, const kfx_pose* initial_
 static int process_frame(kfx_pipeline* p, const uint16_t* depth, const kfx_color* rgb, kfx_frame_info* info) {
     if (info) memset(info, 0, sizeof(*info));
     const size_t npix = (size_t)p->cfg.intr.cols * p->cfg.intr.rows;
-    if (depth != p->raw) KFX_TRY(kfx_memcpy_dtod(p->raw, depth, 2 * npix));
+    // The frame's kernels read the caller's depth buffer where they are
+    // launched afresh each frame; a captured graph reads the pipeline's own
+    // copy (its kernels' arguments are fixed).
+    if (p->cfg.use_graph && depth != p->raw) {
+        KFX_TRY(kfx_memcpy_dtod(p->raw, depth, 2 * npix));
+        p->in = p->raw;
+    } else {
+        p->in = depth;
+    }
     if (rgb != nullptr && rgb != p->rgb) KFX_TRY(kfx_memcpy_dtod(p->rgb, rgb, sizeof(kfx_color) * npix));
     kfx_icp_result r;
     bool have_pose = false;  // r holds the fused estimate of this frame
@@ -706,7 +715,7 @@ int kfx_pipeline_state(kfx_pipeline* p, kfx_pipeline_view* v) {
     v->cell_size = p->cell;
     for (int k = 0; k < 3; ++k) v->origin_world[k] = p->origin_world[k];
     v->depth_filtered = p->dep[0];
-    v->depth_raw = p->raw;
+    v->depth_raw = p->in != nullptr ? p->in : p->raw;
     v->vmap_model = p->vp[0];
     v->nmap_model = p->np[0];
     v->vmap_current = p->vc[0];

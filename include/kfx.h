@@ -609,7 +609,8 @@ typedef struct kfx_pipeline_view {
     kfx_round_buffer round_buffer;
     kfx_point_3f cell_size;
     double origin_world[3];       /* world position of local voxel (0,0,0)'s corner */
-    uint16_t* depth_raw;          /* last input frame (device) */
+    const uint16_t* depth_raw;    /* last input frame as read (device: the caller's buffer, or the pipeline's
+                                     copy with cfg.use_graph) */
     uint16_t* depth_filtered;     /* level-0 filtered depth (device) */
     kfx_xyz* vmap_model;          /* last raycast, volume coordinates (device) */
     kfx_xyz* nmap_model;
@@ -643,9 +644,10 @@ int kfx_pipeline_destroy(kfx_pipeline* pipeline);
  * raises), while the frame's integration, raycast and map update may still
  * run on the library stream; every later library call on that stream is
  * ordered after them (kfx_synchronize waits for them; kfx_pipeline_state
- * does too).  The depth buffer is read by a copy on the library stream:
- * overwrite it through the library (kfx_memcpy_htod) or after
- * kfx_synchronize.  One synchronisation per step in plain mode. */
+ * does too).  The depth buffer is read on the library stream (by the
+ * frame's kernels, or by a copy into the pipeline's own buffer when
+ * cfg.use_graph is set): overwrite it through the library (kfx_memcpy_htod)
+ * or after kfx_synchronize.  One synchronisation per step in plain mode. */
 int kfx_pipeline_process(kfx_pipeline* pipeline, const uint16_t* depth, kfx_frame_info* info);
 /* Process one depth frame with its registered colour frame (device pointer,
  * cols x rows kfx_color): as kfx_pipeline_process, then the colour volume
