@@ -353,11 +353,12 @@ __global__ void __launch_bounds__(256) dn_l4_fwd(const uint16_t* __restrict__ de
 // layer 4 and the output depth; the hidden activations never leave the CU.
 // Every layer goes through the per-layer kernels' group functions, and
 // positions outside the image hold 0 (the zero padding), so the output
-// equals the four-launch forward bit for bit.  Tiles of 32 x 16 with 512
-// threads (52 KB of LDS, three workgroups per CU, 600 workgroups at
-// 640x480).
+// equals the four-launch forward bit for bit.  Tiles of 32 x 20 with 512
+// threads (61 KB of LDS, two workgroups per CU): 480 workgroups at 640x480,
+// so no CU holds more than two (32 x 16 tiles made 600, and the 88 CUs that
+// held three set the time).
 // ---------------------------------------------------------------------------
-constexpr int kDfTX = 32, kDfTY = 16, kDfThreads = 512;
+constexpr int kDfTX = 32, kDfTY = 20, kDfThreads = 512;
 constexpr int kDfDX = kDfTX + 8, kDfDY = kDfTY + 8;   // depth tile (4-pixel halo)
 constexpr int kDfA1X = kDfTX + 6, kDfA1Y = kDfTY + 6; // layer-1 outputs
 constexpr int kDfA2X = kDfTX + 4, kDfA2Y = kDfTY + 4; // layer-2 outputs
