@@ -217,8 +217,11 @@ __device__ __forceinline__ kfx_xyz tile_normal(const NmTile& sx, const NmTile& s
                 const float qz = ok ? pz - pcz : 0.0f;
                 n += ok ? 1.0f : 0.0f;
                 s1x += qx; s1y += qy; s1z += qz;
-                sxx += qx * qx; sxy += qx * qy; sxz += qx * qz;
-                syy += qy * qy; syz += qy * qz; szz += qz * qz;
+                // Second moments with fused multiply-adds (the normals are
+                // compared with the double-precision oracle within a
+                // tolerance, not bit for bit).
+                sxx = fmaf(qx, qx, sxx); sxy = fmaf(qx, qy, sxy); sxz = fmaf(qx, qz, sxz);
+                syy = fmaf(qy, qy, syy); syz = fmaf(qy, qz, syz); szz = fmaf(qz, qz, szz);
             }
         if (n >= (float)((2 * kNmR + 1) * (2 * kNmR + 1) / 2)) {
             const float in = 1.0f / n;
