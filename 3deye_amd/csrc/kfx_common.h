@@ -67,13 +67,15 @@ int integrate_tsdf_posed_ws(const Workspace& ws, const uint16_t* depth, float* d
 // Brick flags of the raycast's skipping march (kfx_raycast.hip): bytes for
 // a volume (0 when an axis is not a multiple of 16, which turns the skipping
 // off); the exact flags of a volume (non-zero: the brick holds a negative
-// value); the raycast at a device pose with the skipping march.
+// value); the raycast at a device pose with the skipping march, which also
+// builds levels 1 and 2 of the maps when pyramid = {v1, n1, v2, n2} (the
+// separate resize's results, bit for bit).
 size_t brick_flag_bytes(const kfx_point_3i& voxels);
 int brick_flags_build(const kfx_tsdf* volume, const kfx_round_buffer* rb, uint8_t* bricks);
 int raycast_posed_bricks(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
                          const kfx_round_buffer* rb, const float* dev_pose, float truncation_distance,
                          const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip,
-                         const uint8_t* bricks);
+                         const uint8_t* bricks, kfx_xyz* const* pyramid = nullptr);
 int integrate_colors_ws(const Workspace& ws, kfx_color* color_volume, const kfx_color* rgb_frame,
                         const kfx_xyz* vertex_map, const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
                         const kfx_point_3f* translation, float truncation_distance, int max_weight,
@@ -289,6 +291,22 @@ struct kfx_pose_f {
     float R[9];
     float t[3];
 };
+
+// Resize of a vertex or normal map (KinectFusion resizeMap): the mean of a
+// 2x2 block when all four are valid, NaN otherwise; normals renormalised.
+__device__ __forceinline__ kfx_xyz resize4(kfx_xyz a, kfx_xyz b, kfx_xyz c, kfx_xyz d, bool normalize) {
+    kfx_xyz o{NAN, NAN, NAN};
+    if (!(isnan(a.x) || isnan(b.x) || isnan(c.x) || isnan(d.x))) {
+        o.x = (a.x + b.x + c.x + d.x) / 4.0f;
+        o.y = (a.y + b.y + c.y + d.y) / 4.0f;
+        o.z = (a.z + b.z + c.z + d.z) / 4.0f;
+        if (normalize) {
+            const float s = 1.0f / sqrtf(o.x * o.x + o.y * o.y + o.z * o.z);
+            o.x *= s; o.y *= s; o.z *= s;
+        }
+    }
+    return o;
+}
 
 // ---------------------------------------------------------------------------
 // Cyclic volume addressing (reference: GPU/opencl/raycast.cl:29-47).

@@ -265,16 +265,21 @@ int map_part_skip(kfx_pipeline* p, const int32_t* skip) {
     KFX_TRY(integrate_tsdf_posed_ws(p->ws, p->in, p->scaled, p->vol, &p->lvl[0], dpose, dprev,
                                     p->cfg.integrate_threshold, p->cfg.truncation, &p->cell, &p->rb, skip,
                                     p->bricks));
+    // The skipping raycast builds levels 1 and 2 itself (no resize launch).
+    kfx_xyz* const pyramid[4] = {p->vp[1], p->np[1], p->vp[2], p->np[2]};
+    const bool fused_pyramid = p->bricks != nullptr && p->lvl[1].cols == p->lvl[0].cols / 2 &&
+                               p->lvl[1].rows == p->lvl[0].rows / 2 && p->lvl[2].cols == p->lvl[1].cols / 2 &&
+                               p->lvl[2].rows == p->lvl[1].rows / 2;
     if (p->bricks)
         KFX_TRY(raycast_posed_bricks(p->vol, p->vp[0], p->np[0], &p->rb, dpose, p->cfg.truncation, &p->lvl[0],
-                                     &p->cell, skip, p->bricks));
+                                     &p->cell, skip, p->bricks, fused_pyramid ? pyramid : nullptr));
     else
         KFX_TRY(kfx_raycast_posed(p->vol, p->vp[0], p->np[0], &p->rb, dpose, p->cfg.truncation, &p->lvl[0],
                                   &p->cell, skip));
     if (p->cvol)
         KFX_TRY(integrate_colors_posed_ws(p->ws, p->cvol, p->rgb, p->vp[0], &p->lvl[0], dpose, p->cfg.truncation,
                                           p->cfg.color_max_weight, &p->cell, &p->rb, skip));
-    return resize_prev(p);
+    return fused_pyramid ? KFX_OK : resize_prev(p);
 }
 
 int map_part(kfx_pipeline* p) { return map_part_skip(p, nullptr); }
@@ -727,6 +732,10 @@ int kfx_pipeline_state(kfx_pipeline* p, kfx_pipeline_view* v) {
     v->fallbacks = p->fallbacks;
     v->extracted_points = p->total_extracted;
     v->brick_flags = p->bricks;
+    for (int l = 0; l < KFX_ICP_LEVELS; ++l) {
+        v->vmap_model_levels[l] = p->vp[l];
+        v->nmap_model_levels[l] = p->np[l];
+    }
     v->last_points = p->cfg.extract_records ? nullptr : p->ext_pts;
     v->last_records = p->cfg.extract_records ? reinterpret_cast<kfx_slice_record*>(p->ext_pts) : nullptr;
     v->last_colors = p->ext_col;

@@ -342,19 +342,8 @@ __global__ void __launch_bounds__(256) transform_maps_kernel(const kfx_xyz* __re
 // block at (2x, 2y) when all four are valid, NaN otherwise; normals are
 // renormalised.
 // --------------------------------------------------------------------------
-__device__ __forceinline__ kfx_xyz resize4(kfx_xyz a, kfx_xyz b, kfx_xyz c, kfx_xyz d, bool normalize) {
-    kfx_xyz o{NAN, NAN, NAN};
-    if (!(isnan(a.x) || isnan(b.x) || isnan(c.x) || isnan(d.x))) {
-        o.x = (a.x + b.x + c.x + d.x) / 4.0f;
-        o.y = (a.y + b.y + c.y + d.y) / 4.0f;
-        o.z = (a.z + b.z + c.z + d.z) / 4.0f;
-        if (normalize) {
-            const float s = 1.0f / sqrtf(o.x * o.x + o.y * o.y + o.z * o.z);
-            o.x *= s; o.y *= s; o.z *= s;
-        }
-    }
-    return o;
-}
+// (resize4, the mean of a 2x2 block, is in kfx_common.h: the raycast's
+// fused pyramid uses it too.)
 
 // Output pixel (x, y) of one 2x downsampling of `in` (icols wide).
 __device__ __forceinline__ kfx_xyz resize_at(const kfx_xyz* __restrict__ in, int icols, int x, int y, bool normalize) {

@@ -33,6 +33,11 @@ struct RaycastParams {
     // nbz bricks), non-zero when the brick may hold a negative value.
     const uint8_t* bricks;
     int nbx, nby;
+    // Levels 1 and 2 of the output maps, built by each workgroup from its
+    // tile (null: not built): c1 x r1 = (cols / 2) x (rows / 2), c2 x r2
+    // half of that again.
+    kfx_xyz *v1, *n1, *v2, *n2;
+    int c1, r1, c2, r2;
 };
 
 struct f3 { float x, y, z; };
@@ -379,29 +384,12 @@ __device__ __forceinline__ bool march_skip(const kfx_tsdf* __restrict__ vol, con
 // kCount: also count the algorithm's voxel reads (first sample, march
 // samples inside the volume, 8 per in-bounds trilinear evaluation) into
 // *count, for the raycast's algorithmic bytes; results are unchanged.
-template <bool kDevicePose, bool kCount = false, bool kSkip = false>
-__global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict__ vol,
-                                                      kfx_xyz* __restrict__ vmap,
-                                                      kfx_xyz* __restrict__ nmap, RaycastParams p,
-                                                      const float* __restrict__ dpose,
-                                                      const int32_t* __restrict__ dskip,
-                                                      unsigned long long* __restrict__ count = nullptr) {
-    if (kDevicePose && dskip != nullptr && *dskip != 0) return;  // uniform across the grid
-    if (kDevicePose) {
-        for (int i = 0; i < 9; ++i) p.r[i] = dpose[i];
-        p.tx = dpose[9]; p.ty = dpose[10]; p.tz = dpose[11];
-    }
-    // A workgroup takes a 16x16 pixel tile, each wave an 8x8 quarter of it:
-    // rays of one wave stay close, so their marches diverge less than a
-    // 64-pixel row's.
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int x = blockIdx.x * 16 + 8 * (wave & 1) + (lane & 7);
-    const int y = blockIdx.y * 16 + 8 * (wave >> 1) + (lane >> 3);
-    if (x >= p.cols || y >= p.rows) return;
-    const int idx = p.cols * y + x;
-    kfx_xyz vout{NAN, NAN, NAN};
-    kfx_xyz nout{NAN, NAN, NAN};
-
+// The ray of pixel (x, y): its vertex and normal (NaN when it hits
+// nothing).
+template <bool kCount, bool kSkip>
+__device__ __forceinline__ void raycast_pixel(const kfx_tsdf* __restrict__ vol, const RaycastParams& p, int x, int y,
+                                              unsigned long long* __restrict__ count, kfx_xyz& vout,
+                                              kfx_xyz& nout) {
     const kfx_round_buffer& rb = p.rb;
     const f3 vol_m{rb.volume_size.x, rb.volume_size.y, rb.volume_size.z};
     const f3 start{p.tx, p.ty, p.tz};
@@ -426,11 +414,7 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
     const float t_exit = fminf(fminf(txmax, tymax), tzmax);
 
     t_enter = fmaxf(t_enter, 0.0f);
-    if (!(t_enter < t_exit)) {
-        vmap[idx] = vout;
-        nmap[idx] = nout;
-        return;
-    }
+    if (!(t_enter < t_exit)) return;
 
     float time_curr = t_enter;
     const int nxv = rb.voxels_size.x, nyv = rb.voxels_size.y, nzv = rb.voxels_size.z;
@@ -505,8 +489,72 @@ __global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict
         if (found) raycast_hit<kCount>(vol, p, start, dir, hit_tc, hit_tn, vout, nout, ns);
         if (kCount) atomicAdd(count, (unsigned long long)ns);
     }
-    vmap[idx] = vout;
-    nmap[idx] = nout;
+}
+
+// Levels 1 and 2 of the maps from the workgroup's 16x16 tile (tile position
+// lx, ly of this thread's pixel, its vertex v and normal n): the same
+// resize4 arithmetic, in the same order, as the separate resize of the whole
+// level 0, so the results are those bit for bit.  Every thread of the
+// workgroup calls it.
+__device__ __forceinline__ void raycast_pyramid(const RaycastParams& p, int tid, int lx, int ly, const kfx_xyz& v,
+                                                const kfx_xyz& n) {
+    __shared__ kfx_xyz sv[16][16], sn[16][16], s1v[8][8], s1n[8][8];
+    sv[ly][lx] = v;
+    sn[ly][lx] = n;
+    __syncthreads();
+    if (tid < 64) {
+        const int ax = tid & 7, ay = tid >> 3;
+        const kfx_xyz a = resize4(sv[2 * ay][2 * ax], sv[2 * ay][2 * ax + 1], sv[2 * ay + 1][2 * ax],
+                                  sv[2 * ay + 1][2 * ax + 1], false);
+        const kfx_xyz b = resize4(sn[2 * ay][2 * ax], sn[2 * ay][2 * ax + 1], sn[2 * ay + 1][2 * ax],
+                                  sn[2 * ay + 1][2 * ax + 1], true);
+        s1v[ay][ax] = a;
+        s1n[ay][ax] = b;
+        const int x1 = (int)blockIdx.x * 8 + ax, y1 = (int)blockIdx.y * 8 + ay;
+        if (x1 < p.c1 && y1 < p.r1) {
+            p.v1[y1 * p.c1 + x1] = a;
+            p.n1[y1 * p.c1 + x1] = b;
+        }
+    }
+    __syncthreads();
+    if (tid < 16) {
+        const int ax = tid & 3, ay = tid >> 2;
+        const int x2 = (int)blockIdx.x * 4 + ax, y2 = (int)blockIdx.y * 4 + ay;
+        if (x2 < p.c2 && y2 < p.r2) {
+            p.v2[y2 * p.c2 + x2] = resize4(s1v[2 * ay][2 * ax], s1v[2 * ay][2 * ax + 1], s1v[2 * ay + 1][2 * ax],
+                                           s1v[2 * ay + 1][2 * ax + 1], false);
+            p.n2[y2 * p.c2 + x2] = resize4(s1n[2 * ay][2 * ax], s1n[2 * ay][2 * ax + 1], s1n[2 * ay + 1][2 * ax],
+                                           s1n[2 * ay + 1][2 * ax + 1], true);
+        }
+    }
+}
+
+template <bool kDevicePose, bool kCount = false, bool kSkip = false>
+__global__ void __launch_bounds__(256) raycast_kernel(const kfx_tsdf* __restrict__ vol,
+                                                      kfx_xyz* __restrict__ vmap,
+                                                      kfx_xyz* __restrict__ nmap, RaycastParams p,
+                                                      const float* __restrict__ dpose,
+                                                      const int32_t* __restrict__ dskip,
+                                                      unsigned long long* __restrict__ count = nullptr) {
+    if (kDevicePose && dskip != nullptr && *dskip != 0) return;  // uniform across the grid
+    if (kDevicePose) {
+        for (int i = 0; i < 9; ++i) p.r[i] = dpose[i];
+        p.tx = dpose[9]; p.ty = dpose[10]; p.tz = dpose[11];
+    }
+    // A workgroup takes a 16x16 pixel tile, each wave an 8x8 quarter of it:
+    // rays of one wave stay close, so their marches diverge less than a
+    // 64-pixel row's.
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lx = 8 * (wave & 1) + (lane & 7), ly = 8 * (wave >> 1) + (lane >> 3);
+    const int x = (int)blockIdx.x * 16 + lx, y = (int)blockIdx.y * 16 + ly;
+    kfx_xyz vout{NAN, NAN, NAN};
+    kfx_xyz nout{NAN, NAN, NAN};
+    if (x < p.cols && y < p.rows) {
+        raycast_pixel<kCount, kSkip>(vol, p, x, y, count, vout, nout);
+        vmap[p.cols * y + x] = vout;
+        nmap[p.cols * y + x] = nout;
+    }
+    if (p.v1 != nullptr) raycast_pyramid(p, tid, lx, ly, vout, nout);  // uniform
 }
 
 int check_round_buffer(const kfx_round_buffer* rb, const char* who);
@@ -586,7 +634,8 @@ static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* 
                           const kfx_round_buffer* rb, const kfx_point_3f* translation,
                           const kfx_matrix_3f* rotation, const float* dev_pose, float truncation_distance,
                           const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip,
-                          unsigned long long* dev_count = nullptr, const uint8_t* bricks = nullptr) {
+                          unsigned long long* dev_count = nullptr, const uint8_t* bricks = nullptr,
+                          kfx_xyz* const* pyramid = nullptr) {
     KFX_REQUIRE_INIT();
     KFX_REQUIRE(volume && vertex_map && normal_map && translation && rotation && intr && cell_size,
                 "kfx_raycast: null pointer");
@@ -612,6 +661,13 @@ static int raycast_launch(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* 
     p.bricks = bricks;
     p.nbx = rb->voxels_size.x >> kRayBrickShift;
     p.nby = rb->voxels_size.y >> kRayBrickShift;
+    p.v1 = p.n1 = p.v2 = p.n2 = nullptr;
+    p.c1 = intr->cols / 2; p.r1 = intr->rows / 2;
+    p.c2 = p.c1 / 2; p.r2 = p.r1 / 2;
+    if (pyramid != nullptr) {
+        KFX_REQUIRE(pyramid[0] && pyramid[1] && pyramid[2] && pyramid[3], "kfx_raycast: null pyramid map");
+        p.v1 = pyramid[0]; p.n1 = pyramid[1]; p.v2 = pyramid[2]; p.n2 = pyramid[3];
+    }
     KFX_REQUIRE(bricks == nullptr || brick_flag_bytes(rb->voxels_size) > 0,
                 "kfx_raycast: brick flags need volume axes that are multiples of %d", kRayBrick);
     ProfScope prof("raycast");
@@ -655,12 +711,12 @@ extern "C" int kfx_raycast_posed(const kfx_tsdf* volume, kfx_xyz* vertex_map, kf
 int kfx::raycast_posed_bricks(const kfx_tsdf* volume, kfx_xyz* vertex_map, kfx_xyz* normal_map,
                               const kfx_round_buffer* rb, const float* dev_pose, float truncation_distance,
                               const kfx_intrinsics* intr, const kfx_point_3f* cell_size, const int32_t* dev_skip,
-                              const uint8_t* bricks) {
+                              const uint8_t* bricks, kfx_xyz* const* pyramid) {
     KFX_REQUIRE(dev_pose != nullptr, "kfx_raycast_posed: null pose");
     const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
     const kfx_point_3f zero = {0, 0, 0};
     return raycast_launch(volume, vertex_map, normal_map, rb, &zero, &ident, dev_pose, truncation_distance, intr,
-                          cell_size, dev_skip, nullptr, bricks);
+                          cell_size, dev_skip, nullptr, bricks, pyramid);
 }
 
 extern "C" int kfx_brick_flags_size(const kfx_round_buffer* rb, size_t* bytes) {

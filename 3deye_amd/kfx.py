@@ -137,7 +137,8 @@ class PipelineView(C.Structure):
                 ("nmap_current", C.c_void_p), ("pose_volume", Pose), ("frames", C.c_int64), ("shifts", C.c_int32),
                 ("fallbacks", C.c_int32), ("extracted_points", C.c_uint64), ("last_points", C.c_void_p),
                 ("last_colors", C.c_void_p), ("color_volume", C.c_void_p), ("rgb", C.c_void_p),
-                ("last_records", C.c_void_p), ("brick_flags", C.c_void_p)]
+                ("last_records", C.c_void_p), ("brick_flags", C.c_void_p),
+                ("vmap_model_levels", C.c_void_p * 3), ("nmap_model_levels", C.c_void_p * 3)]
 
 
 PLAIN, FUSED = 0, 1

@@ -628,6 +628,8 @@ typedef struct kfx_pipeline_view {
     kfx_slice_record* last_records; /* last shift's records (device), NULL unless cfg.extract_records */
     uint8_t* brick_flags;         /* the skipping raycast's brick flags (device, kfx_brick_flags_size bytes),
                                      NULL in plain mode or when an axis is not a multiple of 16 */
+    kfx_xyz* vmap_model_levels[KFX_ICP_LEVELS]; /* the model maps of every ICP level (device; [0] is vmap_model) */
+    kfx_xyz* nmap_model_levels[KFX_ICP_LEVELS];
 } kfx_pipeline_view;
 
 typedef struct kfx_pipeline kfx_pipeline;

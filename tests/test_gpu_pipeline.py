@@ -269,3 +269,27 @@ def test_graph_replay_survives_a_larger_pipeline(gpu):
         assert np.array_equal(np.array(a.pose_world.t[:]), np.array(b.pose_world.t[:]))
         assert np.array_equal(np.array(a.pose_world.R[:]), np.array(b.pose_world.R[:]))
         assert a.fallback == b.fallback and a.denoise_rms_mm == b.denoise_rms_mm
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_fused_raycast_builds_the_map_pyramid(gpu, use_graph):
+    # The fused pipeline's skipping raycast writes levels 1 and 2 of the
+    # model maps from its own tiles; they equal a separate resize of its
+    # level 0 (kfx_resize_pyramid) bit for bit, NaNs included.
+    kfx = gpu
+    pipe, infos, _gt = _c1_run(kfx, 1, use_graph=use_graph, frames=12)
+    v = pipe.view()
+    assert v.brick_flags  # the skipping raycast ran
+    intr = kfx.kinect_intrinsics()
+    W, H = intr.cols, intr.rows
+    got = [kfx.download(v.vmap_model_levels[l], np.float32, (H >> l, W >> l, 3)) for l in range(3)]
+    got_n = [kfx.download(v.nmap_model_levels[l], np.float32, (H >> l, W >> l, 3)) for l in range(3)]
+    assert v.vmap_model_levels[0] == v.vmap_model and v.nmap_model_levels[0] == v.nmap_model
+    out = [kfx.Buffer((W >> l) * (H >> l) * 12) for l in (1, 1, 2, 2)]
+    kfx.call("kfx_resize_pyramid", C.c_void_p(v.vmap_model), C.c_void_p(v.nmap_model), W, H,
+             out[0].ptr, out[1].ptr, out[2].ptr, out[3].ptr)
+    want = [o.download(np.float32, (H >> l, W >> l, 3)) for o, l in zip(out, (1, 1, 2, 2))]
+    assert np.isfinite(got[1][..., 0]).mean() > 0.3
+    for a, b in zip((got[1], got_n[1], got[2], got_n[2]), want):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    pipe.close()
