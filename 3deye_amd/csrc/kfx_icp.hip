@@ -1106,7 +1106,6 @@ __global__ void __launch_bounds__(kFrameThreads) icp_frame_kernel(IcpFrameArgs a
     __shared__ double wsum_d[kFrameGroups][kIcpWaves][32];
     __shared__ float2 wsum_f[kFrameGroups][kIcpWaves][32];
     __shared__ IcpGroupSum<M> red[kFrameGroups][32];
-    __shared__ double wsum_x[kIcpWaves][32];
     __shared__ double total[32];
     __shared__ IcpState s;
     __shared__ int fault;
