@@ -146,15 +146,15 @@ int preprocess(kfx_pipeline* p, bool learned) {
         KFX_TRY(kfx_bilateral_filter(p->in, p->dep[0], in.cols, in.rows, p->cfg.sigma_space, p->cfg.sigma_color,
                                      0xFFFFFFFFu));
     KFX_TRY(step_sync(p));
+    if (p->cfg.mode == KFX_PIPELINE_FUSED) {
+        // The depth pyramid and every level's vertex and normal maps in one
+        // launch (same values as the steps below; levels halve exactly).
+        return kfx_maps_pyramid_from_depth(p->dep[0], p->dep, p->vc, p->nc, p->lvl, KFX_ICP_LEVELS,
+                                           p->cfg.sigma_color);
+    }
     for (int l = 1; l < KFX_ICP_LEVELS; ++l) {
         KFX_TRY(kfx_pyr_down(p->dep[l - 1], p->lvl[l - 1].cols, p->lvl[l - 1].rows, p->dep[l], p->cfg.sigma_color));
         KFX_TRY(step_sync(p));
-    }
-    if (p->cfg.mode == KFX_PIPELINE_FUSED) {
-        // Every level's vertex and normal maps in one launch (same values).
-        const uint16_t* dep[KFX_ICP_LEVELS];
-        for (int l = 0; l < KFX_ICP_LEVELS; ++l) dep[l] = p->dep[l];
-        return kfx_maps_pyramid(dep, p->vc, p->nc, p->lvl, KFX_ICP_LEVELS);
     }
     for (int l = 0; l < KFX_ICP_LEVELS; ++l) {
         KFX_TRY(kfx_vertex_map(p->dep[l], p->vc[l], &p->lvl[l]));

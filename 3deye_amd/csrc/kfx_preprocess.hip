@@ -68,19 +68,29 @@ bilateral_kernel(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst, i
 // neighbourhood of src(2x, 2y) with separable weights {1,4,6,4,1}/16 over the
 // samples within 3 sigma_r (90 mm) of the centre, truncated to integer mm.
 // --------------------------------------------------------------------------
+// The range test |d - centre| < 3 sigma_r on integer differences d (|d| <
+// 2^16, exact in float) as |d| < thr with thr = ceil(3 sigma_r) clamped to
+// [0, 65536]; 0 (no tap passes) for a NaN or non-positive sigma_r.
+__device__ __forceinline__ int pyr_threshold(float sigma_color) {
+    const float c = ceilf(3.0f * sigma_color);
+    return !(c > 0.0f) ? 0 : (c >= 65536.0f ? 65536 : (int)c);
+}
+
 // One output pixel (x, y) of the 2x downsample from a source of scols x srows
-// read through at(xx, yy).
+// read through at(xx, yy), thr from pyr_threshold.  The weights are the
+// integers 16 wt[|xi|] x 16 wt[|yi|] of {6, 4, 1} x {6, 4, 1}: every term of
+// the float sums sum(d w) and sum(w) is a multiple of 1/256 and every partial
+// sum is below 2^16 (the weights add up to 1), so those float sums are exact
+// and equal the integer sums / 256, and their quotient is the correctly
+// rounded s / wsum of the integers.
 template <typename At>
-__device__ __forceinline__ uint16_t pyr_down_px(At at, int scols, int srows, int x, int y, float sigma_color) {
-    const float wt[3] = {0.375f, 0.25f, 0.0625f};
+__device__ __forceinline__ uint16_t pyr_down_px(At at, int scols, int srows, int x, int y, int thr) {
+    constexpr int kW[3] = {6, 4, 1};
     const int center = at(2 * x, 2 * y);
     const int x_lo = max(0, 2 * x - 2) - 2 * x, y_lo = max(0, 2 * y - 2) - 2 * y;
     const int x_hi = min(scols, 2 * x + 3) - 2 * x, y_hi = min(srows, 2 * y + 3) - 2 * y;
-    const float thr = 3.0f * sigma_color;
     // The 25 taps' loads are issued together (taps outside the image read
-    // the centre and are masked), then added row by row in the window's
-    // order; a masked tap adds +0 to non-negative sums, which leaves them
-    // unchanged, so the sums are those of the loop over the clipped window.
+    // the centre and are masked).
     int val[5][5];
 #pragma unroll
     for (int yi = -2; yi <= 2; ++yi)
@@ -89,19 +99,19 @@ __device__ __forceinline__ uint16_t pyr_down_px(At at, int scols, int srows, int
             const bool in = yi >= y_lo && yi < y_hi && xi >= x_lo && xi < x_hi;
             val[yi + 2][xi + 2] = at(in ? 2 * x + xi : 2 * x, in ? 2 * y + yi : 2 * y);
         }
-    float sum = 0.0f, wall = 0.0f;
+    int s = 0, wsum = 0;
 #pragma unroll
     for (int yi = -2; yi <= 2; ++yi)
 #pragma unroll
         for (int xi = -2; xi <= 2; ++xi) {
             const int v = val[yi + 2][xi + 2];
             const bool in = yi >= y_lo && yi < y_hi && xi >= x_lo && xi < x_hi;
-            const float w = wt[xi < 0 ? -xi : xi] * wt[yi < 0 ? -yi : yi];
-            const bool take = in && (float)abs(v - center) < thr;
-            sum += take ? (float)v * w : 0.0f;
-            wall += take ? w : 0.0f;
+            const int w = kW[xi < 0 ? -xi : xi] * kW[yi < 0 ? -yi : yi];
+            const int wt = in && abs(v - center) < thr ? w : 0;
+            s += v * wt;
+            wsum += wt;
         }
-    return (uint16_t)(int)(sum / wall);
+    return (uint16_t)(int)((float)s / (float)wsum);
 }
 
 __global__ void __launch_bounds__(256) pyr_down_kernel(const uint16_t* __restrict__ src, int scols, int srows,
@@ -111,7 +121,7 @@ __global__ void __launch_bounds__(256) pyr_down_kernel(const uint16_t* __restric
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= dcols || y >= drows) return;
     dst[y * dcols + x] = pyr_down_px([&](int xx, int yy) { return (int)src[yy * scols + xx]; }, scols, srows, x, y,
-                                     sigma_color);
+                                     pyr_threshold(sigma_color));
 }
 
 // --------------------------------------------------------------------------
@@ -301,6 +311,140 @@ __global__ void __launch_bounds__(kNmTX * kNmTY) maps_kernel(const MapsParams P)
     const int lu = threadIdx.x + kNmR, lv = threadIdx.y + kNmR;
     L.vmap[v * cols + u] = kfx_xyz{sx[lv][lu], sy[lv][lu], sz[lv][lu]};
     L.nmap[v * cols + u] = tile_normal(sx, sy, sz, lu, lv, u, v, cols, rows);
+}
+
+// The same maps from level-0 depth alone: a tile of level l >= 1 derives its
+// depth halo with pyr_down_px from an LDS region of level l - 1, and level
+// 2's region of level 1 is itself derived from a staged region of level 0,
+// so no depth pyramid is written beforehand.  A halo of hx x kNmLY pixels
+// (hx = tile width + 7) reads (2 hx + 3) x (2 kNmLY + 3) pixels of the level
+// above (5x5 taps about every second pixel), and that region reads twice as
+// much again of the level above it.  Level-2 tiles, whose two derivations
+// make them the costliest, are kL2TX = 8 pixels wide (a quarter of the other
+// levels' 32) so that their work spreads over more workgroups, and they come
+// first in the grid.
+constexpr int kL2TX = 8;
+template <int TW>
+struct DeriveGeom {
+    static constexpr int kHX = TW + 2 * kNmR + 1;                 // halo width
+    static constexpr int k1X = 2 * kHX + 3, k1Y = 2 * kNmLY + 3;  // region of level l - 1
+    static constexpr int k0X = 2 * k1X + 3, k0Y = 2 * k1Y + 3;    // its region of level l - 2
+};
+static_assert(KFX_ICP_LEVELS <= 3, "maps_derive_kernel derives at most two levels");
+static_assert(DeriveGeom<kL2TX>::kHX <= kNmLX, "a level-2 halo fits the vertex tile");
+
+struct MapsDeriveParams {
+    MapsLevel lv[KFX_ICP_LEVELS];  // lv[l].depth unused for l >= 1
+    uint16_t* depth_out[KFX_ICP_LEVELS];  // derived depth of level l >= 1 (tile cores), or null
+    int levels;
+    float sigma_color;
+};
+
+// Copies the W x H region of src (scols x srows) whose corner is (ox, oy)
+// into dst (row stride W).  Out-of-image positions hold a clamped in-image
+// pixel and are never read, since pyr_down_px reads only in-image taps.  The
+// loop has a fixed trip count and no branch (the last pass's spare lanes
+// repeat the final element), so its loads are issued together rather than
+// one latency per pass.
+template <int W, int H>
+__device__ __forceinline__ void stage_depth(const uint16_t* __restrict__ src, int scols, int srows, int ox, int oy,
+                                            uint16_t* dst, int tid) {
+    constexpr int kN = W * H, kThreads = kNmTX * kNmTY, kPasses = (kN + kThreads - 1) / kThreads;
+    uint16_t val[kPasses];
+#pragma unroll
+    for (int k = 0; k < kPasses; ++k) {
+        const int i = min(tid + k * kThreads, kN - 1);
+        const int ry = i / W, rx = i - ry * W;
+        const int x = min(max(ox + rx, 0), scols - 1), y = min(max(oy + ry, 0), srows - 1);
+        val[k] = src[y * scols + x];
+    }
+#pragma unroll
+    for (int k = 0; k < kPasses; ++k) dst[min(tid + k * kThreads, kN - 1)] = val[k];
+}
+
+// Downsampled pixel (x, y) of a scols x srows level held in LDS as the
+// region s (row stride w, corner (ox, oy)) that covers its 5x5 window.
+__device__ __forceinline__ int pyr_down_lds(const uint16_t* s, int w, int ox, int oy, int scols, int srows, int x,
+                                            int y, int thr) {
+    return pyr_down_px([&](int xx, int yy) { return (int)s[(yy - oy) * w + (xx - ox)]; }, scols, srows, x, y, thr);
+}
+
+// One tile (tile index tl within level l) of tile width TW.
+template <int TW>
+__device__ __forceinline__ void maps_derive_tile(const MapsDeriveParams& P, int l, int tl, int tid, uint16_t* d0,
+                                                 NmTile& vx, NmTile& vy, NmTile& vz, uint16_t* d1) {
+    using G = DeriveGeom<TW>;
+    const MapsLevel& L = P.lv[l];
+    const int bx = (tl % L.tiles_x) * TW, by = (tl / L.tiles_x) * kNmTY;
+    const int hx = bx - kNmR, hy = by - kNmR;  // halo corner
+    const int cols = L.in.cols, rows = L.in.rows;
+    const int thr = pyr_threshold(P.sigma_color);
+    const int sox = 2 * hx - 2, soy = 2 * hy - 2;  // corner of the level l - 1 region
+    if (l == 2) {
+        const kfx_intrinsics& i0 = P.lv[0].in;
+        const kfx_intrinsics& i1 = P.lv[1].in;
+        const int o0x = 2 * sox - 2, o0y = 2 * soy - 2;
+        stage_depth<G::k0X, G::k0Y>(P.lv[0].depth, i0.cols, i0.rows, o0x, o0y, d0, tid);
+        __syncthreads();
+        for (int i = tid; i < G::k1X * G::k1Y; i += kNmTX * kNmTY) {
+            const int ry = i / G::k1X, rx = i - ry * G::k1X;
+            const int x = sox + rx, y = soy + ry;
+            if (x >= 0 && x < i1.cols && y >= 0 && y < i1.rows)
+                d1[i] = (uint16_t)pyr_down_lds(d0, G::k0X, o0x, o0y, i0.cols, i0.rows, x, y, thr);
+        }
+        __syncthreads();
+    } else if (l == 1) {
+        stage_depth<G::k1X, G::k1Y>(P.lv[0].depth, P.lv[0].in.cols, P.lv[0].in.rows, sox, soy, d1, tid);
+        __syncthreads();
+    }
+    const kfx_intrinsics& is = P.lv[l > 0 ? l - 1 : 0].in;  // source level of a derived tile
+    uint16_t* dout = l > 0 ? P.depth_out[l] : nullptr;
+    for (int i = tid; i < G::kHX * kNmLY; i += kNmTX * kNmTY) {
+        const int ly = i / G::kHX, lx = i - ly * G::kHX;
+        const int gx = hx + lx, gy = hy + ly;
+        kfx_xyz v{NAN, NAN, NAN};
+        if (gx >= 0 && gx < cols && gy >= 0 && gy < rows) {
+            int d;
+            if (l == 0) {
+                d = L.depth[gy * cols + gx];
+            } else {
+                d = pyr_down_lds(d1, G::k1X, sox, soy, is.cols, is.rows, gx, gy, thr);
+                if (dout && lx >= kNmR && lx < kNmR + TW && ly >= kNmR && ly < kNmR + kNmTY)
+                    dout[gy * cols + gx] = (uint16_t)d;
+            }
+            const float z = (float)(uint16_t)d / 1000.0f;
+            if (z != 0.0f) v = kfx_xyz{z * ((float)gx - L.in.cx) * L.fx_inv, z * ((float)gy - L.in.cy) * L.fy_inv, z};
+        }
+        vx[ly][lx] = v.x; vy[ly][lx] = v.y; vz[ly][lx] = v.z;
+    }
+    __syncthreads();
+    const int u = bx + threadIdx.x, v = by + threadIdx.y;
+    if (threadIdx.x >= TW || u >= cols || v >= rows) return;
+    const int lu = threadIdx.x + kNmR, lv = threadIdx.y + kNmR;
+    L.vmap[v * cols + u] = kfx_xyz{vx[lv][lu], vy[lv][lu], vz[lv][lu]};
+    L.nmap[v * cols + u] = tile_normal(vx, vy, vz, lu, lv, u, v, cols, rows);
+}
+
+__global__ void __launch_bounds__(kNmTX * kNmTY) maps_derive_kernel(const MapsDeriveParams P) {
+    using G2 = DeriveGeom<kL2TX>;
+    using G1 = DeriveGeom<kNmTX>;
+    __shared__ union {
+        uint16_t d0[G2::k0X * G2::k0Y];  // level 0 under a level-2 tile's level-1 region
+        struct {
+            NmTile x, y, z;
+        } v;  // the tile's vertices (written once d0 is consumed)
+    } lds;
+    __shared__ uint16_t d1[G1::k1X * G1::k1Y > G2::k1X * G2::k1Y ? G1::k1X * G1::k1Y : G2::k1X * G2::k1Y];
+    // Tiles run from the deepest level to level 0.
+    const int t = blockIdx.x;
+    int l = P.levels - 1;
+    for (int k = P.levels - 2; k >= 0; --k) l = t >= P.lv[k].first_tile ? k : l;
+    const int tl = t - P.lv[l].first_tile;
+    const int tid = threadIdx.y * kNmTX + threadIdx.x;
+    if (l == 2)
+        maps_derive_tile<kL2TX>(P, 2, tl, tid, lds.d0, lds.v.x, lds.v.y, lds.v.z, d1);
+    else
+        maps_derive_tile<kNmTX>(P, l, tl, tid, lds.d0, lds.v.x, lds.v.y, lds.v.z, d1);
 }
 
 // --------------------------------------------------------------------------
@@ -495,6 +639,47 @@ int kfx_maps_pyramid(const uint16_t* const* depth, kfx_xyz* const* vmap, kfx_xyz
     }
     ProfScope prof("maps_pyramid");
     maps_kernel<<<tiles, dim3(kNmTX, kNmTY), 0, stream()>>>(P);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
+
+int kfx_maps_pyramid_from_depth(const uint16_t* depth, uint16_t* const* depth_out, kfx_xyz* const* vmap,
+                                kfx_xyz* const* nmap, const kfx_intrinsics* intr, int levels, float sigma_color) {
+    KFX_REQUIRE_INIT();
+    KFX_REQUIRE(depth && vmap && nmap && intr, "kfx_maps_pyramid_from_depth: null pointer");
+    KFX_REQUIRE(levels >= 1 && levels <= KFX_ICP_LEVELS, "kfx_maps_pyramid_from_depth: levels %d outside [1, %d]",
+                levels, KFX_ICP_LEVELS);
+    MapsDeriveParams P;
+    memset(&P, 0, sizeof(P));
+    P.levels = levels;
+    P.sigma_color = sigma_color;
+    P.lv[0].depth = depth;
+    for (int l = 0; l < levels; ++l) {
+        KFX_REQUIRE(vmap[l] && nmap[l], "kfx_maps_pyramid_from_depth: null map at level %d", l);
+        KFX_REQUIRE(vmap[l] != nmap[l], "kfx_maps_pyramid_from_depth: vertex and normal maps alias at level %d", l);
+        KFX_REQUIRE(intr[l].cols > 0 && intr[l].rows > 0, "kfx_maps_pyramid_from_depth: empty level %d", l);
+        if (l > 0) {
+            KFX_REQUIRE(intr[l].cols == intr[l - 1].cols / 2 && intr[l].rows == intr[l - 1].rows / 2,
+                        "kfx_maps_pyramid_from_depth: level %d is %dx%d, not half of level %d (%dx%d)", l,
+                        intr[l].cols, intr[l].rows, l - 1, intr[l - 1].cols, intr[l - 1].rows);
+            P.depth_out[l] = depth_out ? depth_out[l] : nullptr;
+            KFX_REQUIRE(P.depth_out[l] != depth, "kfx_maps_pyramid_from_depth: level %d depth overwrites level 0", l);
+        }
+        MapsLevel& L = P.lv[l];
+        L.vmap = vmap[l];
+        L.nmap = nmap[l];
+        L.in = intr[l];
+        L.fx_inv = 1.0f / intr[l].fx;
+        L.fy_inv = 1.0f / intr[l].fy;
+        L.tiles_x = (int)divup(intr[l].cols, l == 2 ? kL2TX : kNmTX);
+    }
+    int tiles = 0;
+    for (int l = levels - 1; l >= 0; --l) {
+        P.lv[l].first_tile = tiles;
+        tiles += P.lv[l].tiles_x * (int)divup(intr[l].rows, kNmTY);
+    }
+    ProfScope prof("maps_pyramid");
+    maps_derive_kernel<<<tiles, dim3(kNmTX, kNmTY), 0, stream()>>>(P);
     KFX_CHECK_LAUNCH();
     return KFX_OK;
 }

@@ -246,6 +246,7 @@ _PROTOS = {
                                        C.c_float, C.POINTER(Intrinsics), C.POINTER(Point3f),
                                        C.POINTER(C.c_uint64)]),
     "kfx_maps_pyramid": (_I, [_P, _P, _P, C.POINTER(Intrinsics), _I]),
+    "kfx_maps_pyramid_from_depth": (_I, [_P, _P, _P, _P, C.POINTER(Intrinsics), _I, C.c_float]),
     "kfx_resize_pyramid": (_I, [_P, _P, _I, _I, _P, _P, _P, _P]),
     "kfx_integrate_colors": (_I, [_P, _P, _P, C.POINTER(Intrinsics), C.POINTER(Matrix3f), C.POINTER(Point3f),
                                   C.c_float, _I, C.POINTER(Point3f), C.POINTER(RoundBuffer)]),

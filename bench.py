@@ -228,9 +228,9 @@ def kernel_table_plain(n_upd_per_launch, samples_per_launch):
 def kernel_table_learned(n_upd_per_launch, samples_per_launch):
     """The learned pipeline's kernels (eager replay of the timed frames).
     Algorithmic work per launch: denoiser 10,080 FLOP/px (bf16 MFMA hidden
-    layers, against 2.5 PFLOP/s dense) and 4 B/px; pyramid level i
-    2 (WH/4^(i-1)) + 2 (WH/4^i) B; vertex + normal maps of every level
-    26 B/px; ICP sums at level l 48 B/px + 216 B per workgroup (27 FP64
+    layers, against 2.5 PFLOP/s dense) and 4 B/px; the depth pyramid with the
+    vertex + normal maps of every level 26 B/px (level-0 depth in, the other
+    levels' depth out, 24 B of maps); ICP sums at level l 48 B/px + 216 B per workgroup (27 FP64
     partials), the solve 216 B per partial + the 600 B state; the gate 14 B/px
     + 32 B per workgroup, its reduce 32 B per workgroup; the integration
     pre-pass 2 + 4 + 4/3 B/px (depth in, scaled depth and pyramid out);
@@ -242,7 +242,6 @@ def kernel_table_learned(n_upd_per_launch, samples_per_launch):
     nblk_gate = (px[0] + 255) // 256
     rows = {
         "denoise_forward": kernel_row("denoise_forward", 4 * px[0], DENOISER_FLOP_PER_PX * px[0]),
-        "pyr_down": kernel_row("pyr_down", sum(2 * px[i - 1] + 2 * px[i] for i in (1, 2)) / 2),
         "maps_pyramid": kernel_row("maps_pyramid", sum(26 * p for p in px)),
         "icp_begin_frame": kernel_row("icp_begin_frame"),
         "depth_bound": kernel_row("depth_bound", (2 + 4 + 4 / 3) * px[0]),

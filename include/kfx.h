@@ -255,6 +255,16 @@ int kfx_resize_map(const kfx_xyz* in, int in_cols, int in_rows, kfx_xyz* out, in
  * Create_Normal_Maps). */
 int kfx_maps_pyramid(const uint16_t* const* depth, kfx_xyz* const* vmap, kfx_xyz* const* nmap,
                      const kfx_intrinsics* intr, int levels);
+/* kfx_maps_pyramid from level-0 depth alone: level l >= 1's depth is
+ * kfx_pyr_down of level l - 1 (sigma_color as there), derived inside the same
+ * launch, and every level's maps equal those of kfx_pyr_down followed by
+ * kfx_maps_pyramid, bit for bit.  Level l must be exactly half of level
+ * l - 1 (intr[l].cols == intr[l-1].cols / 2, rows likewise).  depth_out may
+ * be NULL; otherwise depth_out[l] (l >= 1; depth_out[0] is not read) receives
+ * level l's depth, or is NULL to skip it (no reference counterpart: one
+ * launch instead of compute_depth_pyramid's levels plus the maps). */
+int kfx_maps_pyramid_from_depth(const uint16_t* depth, uint16_t* const* depth_out, kfx_xyz* const* vmap,
+                                kfx_xyz* const* nmap, const kfx_intrinsics* intr, int levels, float sigma_color);
 /* Levels 1 and 2 of a vertex map and a normal map from level 0 (cols x rows,
  * multiples of 4) in one launch: kfx_resize_map twice per map, bit for bit
  * (no reference counterpart). */

@@ -166,6 +166,87 @@ def test_maps_pyramid_equals_per_level_kernels(gpu, orc, c1_frame):
         assert np.array_equal(got_v[ok], want_v[ok])
 
 
+@pytest.mark.parametrize("sigma", [30.0, 29.7, 1.0 / 3.0, 5.5, 1e6])
+def test_pyr_down_range_threshold_and_full_range_depth(gpu, orc, c1_frame, sigma):
+    # The kernel's integer range test and sums against the oracle's float
+    # ones: non-integer, exactly integer (3 sigma = 1) and unbounded
+    # thresholds, on the frame with a block of full-range uint16 values
+    # (sums up to 65535 x 256, the largest the exact-sum argument allows).
+    kfx = gpu
+    intr, d = c1_frame
+    d = d.copy()
+    rng = np.random.default_rng(7)
+    d[100:220, 200:360] = rng.integers(0, 65536, size=(120, 160), dtype=np.uint16)
+    d[300:340, 40:120] = 65535
+    d[400:410, 500:620] = rng.integers(0, 3, size=(10, 120), dtype=np.uint16)
+    h, w = d.shape
+    src = kfx.Buffer.from_array(d)
+    dst = kfx.Buffer(2 * (h // 2) * (w // 2))
+    kfx.call("kfx_pyr_down", src.ptr, w, h, dst.ptr, C.c_float(sigma))
+    got = dst.download(np.uint16, (h // 2, w // 2))
+    want = orc.pyr_down(d, float(np.float32(sigma)))
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("crop,levels", [((480, 640), 3), ((478, 634), 3), ((101, 203), 3), ((480, 640), 2),
+                                         ((480, 640), 1)])
+def test_maps_pyramid_from_depth_equals_pyr_down_then_maps(gpu, orc, c1_frame, crop, levels):
+    # Level 1 and 2 depth derived inside the maps launch: the same bits as
+    # kfx_pyr_down level by level then kfx_maps_pyramid, on the full frame,
+    # on crops whose levels are ragged against the 32 x 8 tiles, and with
+    # fewer levels; the derived depth also against the oracle's pyramid.
+    kfx = gpu
+    intr0, d = c1_frame
+    d = np.ascontiguousarray(d[:crop[0], :crop[1]])
+    base = kfx.Intrinsics(intr0.fx, intr0.fy, intr0.cx, intr0.cy, crop[1], crop[0])
+    intrs = (kfx.Intrinsics * 3)(*[base.level(i) for i in range(3)])
+    sc = 30.0
+    shapes = [(intrs[i].rows, intrs[i].cols) for i in range(3)]
+    d0 = kfx.Buffer.from_array(d)
+    db = [d0] + [kfx.Buffer(2 * r * c) for r, c in shapes[1:]]
+    for i in range(1, levels):
+        kfx.call("kfx_pyr_down", db[i - 1].ptr, intrs[i - 1].cols, intrs[i - 1].rows, db[i].ptr, C.c_float(sc))
+    ptrs = lambda bufs: (C.c_void_p * 3)(*[b.ptr.value if b is not None else None for b in bufs])
+    mk = lambda: [kfx.Buffer(12 * r * c) for r, c in shapes]
+    vw, nw, vg, ng = mk(), mk(), mk(), mk()
+    kfx.call("kfx_maps_pyramid", ptrs(db), ptrs(vw), ptrs(nw), intrs, levels)
+    dout = [None] + [kfx.Buffer(2 * r * c) for r, c in shapes[1:]]
+    kfx.call("kfx_maps_pyramid_from_depth", d0.ptr, ptrs(dout), ptrs(vg), ptrs(ng), intrs, levels, C.c_float(sc))
+    want_d = [d]
+    for i in range(1, levels):
+        want_d.append(orc.pyr_down(want_d[-1], sc))
+        got = dout[i].download(np.uint16, shapes[i])
+        assert np.array_equal(got, db[i].download(np.uint16, shapes[i])), f"depth level {i}"
+        assert np.array_equal(got, want_d[i]), f"depth level {i} vs oracle"
+    for i in range(levels):
+        for got_b, want_b in ((vg[i], vw[i]), (ng[i], nw[i])):
+            got = got_b.download(np.float32, shapes[i] + (3,))
+            want = want_b.download(np.float32, shapes[i] + (3,))
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"level {i} {crop}"
+    # Without depth outputs the maps are the same.
+    vg2, ng2 = mk(), mk()
+    kfx.call("kfx_maps_pyramid_from_depth", d0.ptr, None, ptrs(vg2), ptrs(ng2), intrs, levels, C.c_float(sc))
+    for i in range(levels):
+        for a, b in ((vg[i], vg2[i]), (ng[i], ng2[i])):
+            x = a.download(np.float32, shapes[i] + (3,))
+            y = b.download(np.float32, shapes[i] + (3,))
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def test_maps_pyramid_from_depth_rejects_unhalved_levels(gpu, c1_frame):
+    kfx = gpu
+    intr, d = c1_frame
+    lv = [intr.level(i) for i in range(3)]
+    lv[2] = kfx.Intrinsics(lv[2].fx, lv[2].fy, lv[2].cx, lv[2].cy, lv[2].cols + 1, lv[2].rows)
+    intrs = (kfx.Intrinsics * 3)(*lv)
+    d0 = kfx.Buffer.from_array(d)
+    bufs = [kfx.Buffer(12 * x.cols * x.rows) for x in lv for _ in range(2)]
+    vm = (C.c_void_p * 3)(*[b.ptr.value for b in bufs[0::2]])
+    nm = (C.c_void_p * 3)(*[b.ptr.value for b in bufs[1::2]])
+    with pytest.raises(kfx.KfxError, match="not half"):
+        kfx.call("kfx_maps_pyramid_from_depth", d0.ptr, None, vm, nm, intrs, 3, C.c_float(30.0))
+
+
 def test_resize_pyramid_equals_two_resizes(gpu, orc, c1_frame):
     kfx = gpu
     intr, d = c1_frame
