@@ -320,10 +320,11 @@ __global__ void __launch_bounds__(kNmTX * kNmTY) maps_kernel(const MapsParams P)
 // (hx = tile width + 7) reads (2 hx + 3) x (2 kNmLY + 3) pixels of the level
 // above (5x5 taps about every second pixel), and that region reads twice as
 // much again of the level above it.  Level-2 tiles, whose two derivations
-// make them the costliest, are kL2TX = 8 pixels wide (a quarter of the other
-// levels' 32) so that their work spreads over more workgroups, and they come
-// first in the grid.
-constexpr int kL2TX = 8;
+// make them the costliest, are kL2TX = 16 pixels wide (half the other
+// levels' 32) so that their work spreads over more workgroups (8 wide
+// repeats more of the level-1 derivation, 32 wide leaves the launch waiting
+// on 75 long tiles), and they come first in the grid.
+constexpr int kL2TX = 16;
 template <int TW>
 struct DeriveGeom {
     static constexpr int kHX = TW + 2 * kNmR + 1;                 // halo width
