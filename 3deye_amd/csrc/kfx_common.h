@@ -60,10 +60,38 @@ int integrate_tsdf_ws(const Workspace& ws, bool profile, const uint16_t* depth, 
                       kfx_tsdf* volume, const kfx_intrinsics* intr, const kfx_matrix_3f* rotation_inverse,
                       const kfx_point_3f* translation, float truncation_distance, const kfx_point_3f* cell_size,
                       const kfx_round_buffer* rb, uint8_t* bricks = nullptr);
+// bound_ready: depth_scaled and ws's depth-bound pyramid already hold this
+// frame's values (built by maps_pyramid_launch with depth_bound_job's job),
+// so the pre-pass computes only the pose terms and workgroup flags.
 int integrate_tsdf_posed_ws(const Workspace& ws, const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
                             const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
                             float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
-                            const kfx_round_buffer* rb, const int32_t* dev_skip, uint8_t* bricks = nullptr);
+                            const kfx_round_buffer* rb, const int32_t* dev_skip, uint8_t* bricks = nullptr,
+                            bool bound_ready = false);
+
+// Coarsest level of the integration's depth-bound pyramid (texels of 32 x 32
+// pixels).
+constexpr int kBoundLevels = 5;
+constexpr int kBoundTile = 1 << kBoundLevels;
+// The integration pre-pass's pose-independent half -- the scaled depth of
+// every pixel and the depth-bound pyramid's levels 1..5, one 32 x 32 tile per
+// workgroup -- as extra workgroups of the maps launch.
+struct DepthBoundJob {
+    const uint16_t* depth;
+    float* scaled;
+    float* bound;
+    kfx_intrinsics intr;
+    int bound_off[kBoundLevels + 1];
+    int tiles_x, tiles;
+};
+// The job for images of *intr into ws's pyramid; false when ws holds no
+// pyramid of that size (the pre-pass then builds both halves itself).
+bool depth_bound_job(const Workspace& ws, const uint16_t* depth, float* depth_scaled, const kfx_intrinsics* intr,
+                     DepthBoundJob* job);
+// kfx_maps_pyramid_from_depth with an optional depth-bound job in the same
+// launch.
+int maps_pyramid_launch(const uint16_t* depth, uint16_t* const* depth_out, kfx_xyz* const* vmap, kfx_xyz* const* nmap,
+                        const kfx_intrinsics* intr, int levels, float sigma_color, const DepthBoundJob* job);
 // Brick flags of the raycast's skipping march (kfx_raycast.hip): bytes for
 // a volume (0 when an axis is not a multiple of 16, which turns the skipping
 // off); the exact flags of a volume (non-zero: the brick holds a negative
@@ -294,6 +322,16 @@ struct kfx_pose_f {
 
 // Resize of a vertex or normal map (KinectFusion resizeMap): the mean of a
 // 2x2 block when all four are valid, NaN otherwise; normals renormalised.
+// The scaled depth of one pixel (scale_depth_kernel's arithmetic): depth in
+// metres times the length of the pixel's ray at unit depth.
+__device__ __forceinline__ float scaled_depth(const uint16_t* __restrict__ depth, const kfx_intrinsics& intr, int u,
+                                              int v) {
+    const float xl = ((float)u - intr.cx) / intr.fx;
+    const float yl = ((float)v - intr.cy) / intr.fy;
+    const float lambda = sqrtf(xl * xl + yl * yl + 1.0f);
+    return (float)depth[v * intr.cols + u] * lambda / 1000.0f;
+}
+
 __device__ __forceinline__ kfx_xyz resize4(kfx_xyz a, kfx_xyz b, kfx_xyz c, kfx_xyz d, bool normalize) {
     kfx_xyz o{NAN, NAN, NAN};
     if (!(isnan(a.x) || isnan(b.x) || isnan(c.x) || isnan(d.x))) {

@@ -57,6 +57,7 @@ struct kfx_pipeline {
     int shifts = 0;
     int fallbacks = 0;
     bool icp_in_flight = false;           // a publishing ICP launch was enqueued and its result not yet read
+    bool bound_ready = false;             // the maps launch built this frame's scaled depth and depth-bound pyramid
     const uint16_t* in = nullptr;         // the frame's raw depth: the caller's buffer, or raw (graph replay)
 };
 
@@ -146,11 +147,18 @@ int preprocess(kfx_pipeline* p, bool learned) {
         KFX_TRY(kfx_bilateral_filter(p->in, p->dep[0], in.cols, in.rows, p->cfg.sigma_space, p->cfg.sigma_color,
                                      0xFFFFFFFFu));
     KFX_TRY(step_sync(p));
+    p->bound_ready = false;
     if (p->cfg.mode == KFX_PIPELINE_FUSED) {
         // The depth pyramid and every level's vertex and normal maps in one
-        // launch (same values as the steps below; levels halve exactly).
-        return kfx_maps_pyramid_from_depth(p->dep[0], p->dep, p->vc, p->nc, p->lvl, KFX_ICP_LEVELS,
-                                           p->cfg.sigma_color);
+        // launch (same values as the steps below; levels halve exactly),
+        // with the integration pre-pass's scaled depth and depth-bound
+        // pyramid of the raw frame as extra workgroups.
+        DepthBoundJob job;
+        const bool built = depth_bound_job(p->ws, p->in, p->scaled, &p->lvl[0], &job);
+        KFX_TRY(maps_pyramid_launch(p->dep[0], p->dep, p->vc, p->nc, p->lvl, KFX_ICP_LEVELS, p->cfg.sigma_color,
+                                    built ? &job : nullptr));
+        p->bound_ready = built;
+        return KFX_OK;
     }
     for (int l = 1; l < KFX_ICP_LEVELS; ++l) {
         KFX_TRY(kfx_pyr_down(p->dep[l - 1], p->lvl[l - 1].cols, p->lvl[l - 1].rows, p->dep[l], p->cfg.sigma_color));
@@ -264,7 +272,7 @@ int map_part_skip(kfx_pipeline* p, const int32_t* skip) {
     KFX_TRY(kfx_icp_pose_device(p->icp, &dpose, &dprev));
     KFX_TRY(integrate_tsdf_posed_ws(p->ws, p->in, p->scaled, p->vol, &p->lvl[0], dpose, dprev,
                                     p->cfg.integrate_threshold, p->cfg.truncation, &p->cell, &p->rb, skip,
-                                    p->bricks));
+                                    p->bricks, p->bound_ready));
     // The skipping raycast builds levels 1 and 2 itself (no resize launch).
     kfx_xyz* const pyramid[4] = {p->vp[1], p->np[1], p->vp[2], p->np[2]};
     const bool fused_pyramid = p->bricks != nullptr && p->lvl[1].cols == p->lvl[0].cols / 2 &&

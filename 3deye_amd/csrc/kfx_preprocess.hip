@@ -339,7 +339,48 @@ struct MapsDeriveParams {
     uint16_t* depth_out[KFX_ICP_LEVELS];  // derived depth of level l >= 1 (tile cores), or null
     int levels;
     float sigma_color;
+    DepthBoundJob job;  // the integration's depth-bound tiles, the grid's first job.tiles workgroups
 };
+
+// One 32 x 32 tile of the integration pre-pass's pose-independent half
+// (integrate_prepass_kernel's arithmetic, by 256 threads): the tile's scaled
+// depth (0 outside the image), then levels 1..5 of the depth-bound pyramid
+// by repeated 2 x 2 maxima in LDS (t, 1024 floats).
+__device__ __forceinline__ void depth_bound_tile(const DepthBoundJob& J, int tb, int tid, float* t) {
+    constexpr int kThreads = kNmTX * kNmTY;
+    const int cols = J.intr.cols, rows = J.intr.rows;
+    const int bx = tb % J.tiles_x, by = tb / J.tiles_x;
+#pragma unroll
+    for (int k = 0; k < kBoundTile * kBoundTile / kThreads; ++k) {
+        const int i = tid + k * kThreads;
+        const int u = bx * kBoundTile + i % kBoundTile, v = by * kBoundTile + i / kBoundTile;
+        float d = 0.0f;
+        if (u < cols && v < rows) {
+            d = scaled_depth(J.depth, J.intr, u, v);
+            J.scaled[v * cols + u] = d;
+        }
+        t[i] = d;
+    }
+    __syncthreads();
+    for (int l = 1; l <= kBoundLevels; ++l) {
+        const int n = kBoundTile >> l;        // texels per tile side at level l (n * n <= 256)
+        const int sd = kBoundTile >> (l - 1);  // side of the previous level's tile in t[]
+        const int tx = tid % n, ty = tid / n;
+        const bool own = tid < n * n;
+        float m = 0.0f;
+        if (own)
+            m = fmaxf(fmaxf(t[(2 * ty) * sd + 2 * tx], t[(2 * ty) * sd + 2 * tx + 1]),
+                      fmaxf(t[(2 * ty + 1) * sd + 2 * tx], t[(2 * ty + 1) * sd + 2 * tx + 1]));
+        __syncthreads();
+        if (own) {
+            t[ty * n + tx] = m;
+            const int lw = (cols + (1 << l) - 1) >> l, lh = (rows + (1 << l) - 1) >> l;
+            const int a = bx * n + tx, b = by * n + ty;
+            if (a < lw && b < lh) J.bound[J.bound_off[l] + b * lw + a] = m;
+        }
+        __syncthreads();
+    }
+}
 
 // Copies the W x H region of src (scols x srows) whose corner is (ox, oy)
 // into dst (row stride W).  Out-of-image positions hold a clamped in-image
@@ -436,12 +477,17 @@ __global__ void __launch_bounds__(kNmTX * kNmTY) maps_derive_kernel(const MapsDe
         } v;  // the tile's vertices (written once d0 is consumed)
     } lds;
     __shared__ uint16_t d1[G1::k1X * G1::k1Y > G2::k1X * G2::k1Y ? G1::k1X * G1::k1Y : G2::k1X * G2::k1Y];
+    const int tid = threadIdx.y * kNmTX + threadIdx.x;
+    if ((int)blockIdx.x < P.job.tiles) {
+        static_assert(sizeof(lds.d0) >= kBoundTile * kBoundTile * sizeof(float), "a bound tile fits in d0");
+        depth_bound_tile(P.job, blockIdx.x, tid, reinterpret_cast<float*>(lds.d0));
+        return;
+    }
     // Tiles run from the deepest level to level 0.
-    const int t = blockIdx.x;
+    const int t = blockIdx.x - P.job.tiles;
     int l = P.levels - 1;
     for (int k = P.levels - 2; k >= 0; --k) l = t >= P.lv[k].first_tile ? k : l;
     const int tl = t - P.lv[l].first_tile;
-    const int tid = threadIdx.y * kNmTX + threadIdx.x;
     if (l == 2)
         maps_derive_tile<kL2TX>(P, 2, tl, tid, lds.d0, lds.v.x, lds.v.y, lds.v.z, d1);
     else
@@ -533,6 +579,48 @@ __global__ void __launch_bounds__(256) resize_pyramid_kernel(const kfx_xyz* __re
 }  // namespace kfx
 
 using namespace kfx;
+
+int kfx::maps_pyramid_launch(const uint16_t* depth, uint16_t* const* depth_out, kfx_xyz* const* vmap,
+                             kfx_xyz* const* nmap, const kfx_intrinsics* intr, int levels, float sigma_color,
+                             const DepthBoundJob* job) {
+    KFX_REQUIRE(depth && vmap && nmap && intr, "kfx_maps_pyramid_from_depth: null pointer");
+    KFX_REQUIRE(levels >= 1 && levels <= KFX_ICP_LEVELS, "kfx_maps_pyramid_from_depth: levels %d outside [1, %d]",
+                levels, KFX_ICP_LEVELS);
+    MapsDeriveParams P;
+    memset(&P, 0, sizeof(P));
+    P.levels = levels;
+    P.sigma_color = sigma_color;
+    P.lv[0].depth = depth;
+    for (int l = 0; l < levels; ++l) {
+        KFX_REQUIRE(vmap[l] && nmap[l], "kfx_maps_pyramid_from_depth: null map at level %d", l);
+        KFX_REQUIRE(vmap[l] != nmap[l], "kfx_maps_pyramid_from_depth: vertex and normal maps alias at level %d", l);
+        KFX_REQUIRE(intr[l].cols > 0 && intr[l].rows > 0, "kfx_maps_pyramid_from_depth: empty level %d", l);
+        if (l > 0) {
+            KFX_REQUIRE(intr[l].cols == intr[l - 1].cols / 2 && intr[l].rows == intr[l - 1].rows / 2,
+                        "kfx_maps_pyramid_from_depth: level %d is %dx%d, not half of level %d (%dx%d)", l,
+                        intr[l].cols, intr[l].rows, l - 1, intr[l - 1].cols, intr[l - 1].rows);
+            P.depth_out[l] = depth_out ? depth_out[l] : nullptr;
+            KFX_REQUIRE(P.depth_out[l] != depth, "kfx_maps_pyramid_from_depth: level %d depth overwrites level 0", l);
+        }
+        MapsLevel& L = P.lv[l];
+        L.vmap = vmap[l];
+        L.nmap = nmap[l];
+        L.in = intr[l];
+        L.fx_inv = 1.0f / intr[l].fx;
+        L.fy_inv = 1.0f / intr[l].fy;
+        L.tiles_x = (int)divup(intr[l].cols, l == 2 ? kL2TX : kNmTX);
+    }
+    int tiles = 0;
+    for (int l = levels - 1; l >= 0; --l) {
+        P.lv[l].first_tile = tiles;
+        tiles += P.lv[l].tiles_x * (int)divup(intr[l].rows, kNmTY);
+    }
+    if (job != nullptr) P.job = *job;
+    ProfScope prof("maps_pyramid");
+    maps_derive_kernel<<<tiles + P.job.tiles, dim3(kNmTX, kNmTY), 0, stream()>>>(P);
+    KFX_CHECK_LAUNCH();
+    return KFX_OK;
+}
 
 extern "C" {
 
@@ -647,42 +735,7 @@ int kfx_maps_pyramid(const uint16_t* const* depth, kfx_xyz* const* vmap, kfx_xyz
 int kfx_maps_pyramid_from_depth(const uint16_t* depth, uint16_t* const* depth_out, kfx_xyz* const* vmap,
                                 kfx_xyz* const* nmap, const kfx_intrinsics* intr, int levels, float sigma_color) {
     KFX_REQUIRE_INIT();
-    KFX_REQUIRE(depth && vmap && nmap && intr, "kfx_maps_pyramid_from_depth: null pointer");
-    KFX_REQUIRE(levels >= 1 && levels <= KFX_ICP_LEVELS, "kfx_maps_pyramid_from_depth: levels %d outside [1, %d]",
-                levels, KFX_ICP_LEVELS);
-    MapsDeriveParams P;
-    memset(&P, 0, sizeof(P));
-    P.levels = levels;
-    P.sigma_color = sigma_color;
-    P.lv[0].depth = depth;
-    for (int l = 0; l < levels; ++l) {
-        KFX_REQUIRE(vmap[l] && nmap[l], "kfx_maps_pyramid_from_depth: null map at level %d", l);
-        KFX_REQUIRE(vmap[l] != nmap[l], "kfx_maps_pyramid_from_depth: vertex and normal maps alias at level %d", l);
-        KFX_REQUIRE(intr[l].cols > 0 && intr[l].rows > 0, "kfx_maps_pyramid_from_depth: empty level %d", l);
-        if (l > 0) {
-            KFX_REQUIRE(intr[l].cols == intr[l - 1].cols / 2 && intr[l].rows == intr[l - 1].rows / 2,
-                        "kfx_maps_pyramid_from_depth: level %d is %dx%d, not half of level %d (%dx%d)", l,
-                        intr[l].cols, intr[l].rows, l - 1, intr[l - 1].cols, intr[l - 1].rows);
-            P.depth_out[l] = depth_out ? depth_out[l] : nullptr;
-            KFX_REQUIRE(P.depth_out[l] != depth, "kfx_maps_pyramid_from_depth: level %d depth overwrites level 0", l);
-        }
-        MapsLevel& L = P.lv[l];
-        L.vmap = vmap[l];
-        L.nmap = nmap[l];
-        L.in = intr[l];
-        L.fx_inv = 1.0f / intr[l].fx;
-        L.fy_inv = 1.0f / intr[l].fy;
-        L.tiles_x = (int)divup(intr[l].cols, l == 2 ? kL2TX : kNmTX);
-    }
-    int tiles = 0;
-    for (int l = levels - 1; l >= 0; --l) {
-        P.lv[l].first_tile = tiles;
-        tiles += P.lv[l].tiles_x * (int)divup(intr[l].rows, kNmTY);
-    }
-    ProfScope prof("maps_pyramid");
-    maps_derive_kernel<<<tiles, dim3(kNmTX, kNmTY), 0, stream()>>>(P);
-    KFX_CHECK_LAUNCH();
-    return KFX_OK;
+    return kfx::maps_pyramid_launch(depth, depth_out, vmap, nmap, intr, levels, sigma_color, nullptr);
 }
 
 int kfx_resize_pyramid(const kfx_xyz* vmap0, const kfx_xyz* nmap0, int cols, int rows, kfx_xyz* vmap1,

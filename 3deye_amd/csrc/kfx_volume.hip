@@ -26,9 +26,6 @@ constexpr int kIntegrateBlockX = 64;
 constexpr int kIntegrateBlockY = 4;
 // Voxels of a run whose loads are issued before any of them is updated.
 constexpr int kIntegrateBatch = 8;
-// Coarsest level of the depth-bound pyramid (texels of 32 x 32 pixels).
-constexpr int kBoundLevels = 5;
-constexpr int kBoundTile = 1 << kBoundLevels;
 
 // The sweep's workgroups: (bx, by, bz) covers the columns x in [64 bx,
 // 64 bx + 63], y in [4 by, 4 by + 3] over the z run bz.  The hardware deals
@@ -215,15 +212,6 @@ __device__ __forceinline__ bool load_device_pose(const IntegrateParams& p, PoseT
     return true;
 }
 
-// The scaled depth of one pixel (scale_depth_kernel's arithmetic).
-__device__ __forceinline__ float scaled_depth(const uint16_t* __restrict__ depth, const kfx_intrinsics& intr, int u,
-                                              int v) {
-    const float xl = ((float)u - intr.cx) / intr.fx;
-    const float yl = ((float)v - intr.cy) / intr.fy;
-    const float lambda = sqrtf(xl * xl + yl * yl + 1.0f);
-    return (float)depth[v * intr.cols + u] * lambda / 1000.0f;
-}
-
 // Frustum test of the integration workgroup (bx, by, bz) -- voxel columns x
 // in [64 bx, 64 bx + 63], y in [4 by, 4 by + 3], z in one run of p.zrun:
 // 0 when for some plane every voxel centre of the box lies outside it (the
@@ -274,17 +262,17 @@ integrate_prepass_kernel(const uint16_t* __restrict__ depth, float* __restrict__
                          float* __restrict__ bound, IntegrateParams p, const float* __restrict__ dpose,
                          const float* __restrict__ dprev, float motion_thr, const int32_t* __restrict__ dskip,
                          PoseTerms* __restrict__ terms, int32_t* __restrict__ frame_skip,
-                         uint8_t* __restrict__ flags, int nbx, int nby, int nbz) {
+                         uint8_t* __restrict__ flags, int nbx, int nby, int nbz, int pixels) {
     __shared__ float t[kBoundTile * kBoundTile];
     const int cols = intr.cols, rows = intr.rows;
     const int tx = threadIdx.x % kBoundTile, ty = threadIdx.x / kBoundTile;
     const int u = blockIdx.x * kBoundTile + tx, v = blockIdx.y * kBoundTile + ty;
     float d = 0.0f;
-    if (u < cols && v < rows) {
+    if (pixels && u < cols && v < rows) {
         d = scaled_depth(depth, intr, u, v);
         scaled[v * cols + u] = d;
     }
-    if (bound != nullptr) {
+    if (pixels && bound != nullptr) {
         t[threadIdx.x] = d;
         __syncthreads();
         for (int l = 1; l <= kBoundLevels; ++l) {
@@ -1016,10 +1004,15 @@ size_t integrate_blocks(const kfx_point_3i& n) {
 // device pose also the frame's pose terms and skip flag.  Without pyramid or
 // flags (the library workspace inside a stream capture before any
 // reservation) the host-pose sweep runs unculled: same result, more work.
+//
+// bound_ready: depth_scaled and the pyramid were built by the maps launch
+// (depth_bound_job); the launch then covers the flags only, one thread per
+// sweep workgroup.
 template <bool kDevicePose>
 static int integrate_prepass(const Workspace& ws, bool profile, const uint16_t* depth, float* depth_scaled,
                              const kfx_intrinsics* intr, IntegrateParams& p, dim3 igrid, const float* dpose,
-                             const float* dprev, float motion_thr, const int32_t* dskip, const uint8_t** flags_out) {
+                             const float* dprev, float motion_thr, const int32_t* dskip, const uint8_t** flags_out,
+                             bool bound_ready = false) {
     p.bound = nullptr;
     *flags_out = nullptr;
     const size_t need = bound_layout(intr->cols, intr->rows, p.bound_off);
@@ -1033,15 +1026,30 @@ static int integrate_prepass(const Workspace& ws, bool profile, const uint16_t* 
     PoseTerms* terms = kDevicePose ? static_cast<PoseTerms*>(ws.terms) : nullptr;
     int32_t* skip = kDevicePose ? reinterpret_cast<int32_t*>(static_cast<char*>(ws.terms) + kTermsSkipOffset) : nullptr;
     ProfScope prof(profile ? "depth_bound" : nullptr);
-    dim3 grid(divup(intr->cols, kBoundTile), divup(intr->rows, kBoundTile));
+    const bool pixels = !(bound_ready && have_bound && have_flags);
+    const dim3 grid = pixels ? dim3(divup(intr->cols, kBoundTile), divup(intr->rows, kBoundTile))
+                             : dim3((unsigned)divup(nblk, (size_t)(kBoundTile * kBoundTile)));
     // igrid is sweep_grid's (y blocks, x blocks, z runs).
     integrate_prepass_kernel<kDevicePose><<<grid, kBoundTile * kBoundTile, 0, stream()>>>(
         depth, depth_scaled, *intr, have_bound ? ws.bound : nullptr, p, dpose, dprev, motion_thr, dskip, terms, skip,
-        have_flags ? ws.flags : nullptr, (int)igrid.y, (int)igrid.x, (int)igrid.z);
+        have_flags ? ws.flags : nullptr, (int)igrid.y, (int)igrid.x, (int)igrid.z, pixels ? 1 : 0);
     KFX_CHECK_LAUNCH();
     if (have_bound) p.bound = ws.bound;
     if (have_flags) *flags_out = ws.flags;
     return KFX_OK;
+}
+
+bool depth_bound_job(const Workspace& ws, const uint16_t* depth, float* depth_scaled, const kfx_intrinsics* intr,
+                     DepthBoundJob* job) {
+    const size_t need = bound_layout(intr->cols, intr->rows, job->bound_off);
+    if (ws.bound == nullptr || need > ws.bound_cap || ws.flags == nullptr || ws.terms == nullptr) return false;
+    job->depth = depth;
+    job->scaled = depth_scaled;
+    job->bound = ws.bound;
+    job->intr = *intr;
+    job->tiles_x = (int)divup(intr->cols, kBoundTile);
+    job->tiles = job->tiles_x * (int)divup(intr->rows, kBoundTile);
+    return true;
 }
 
 // Camera, volume and frustum terms of a column sweep (integration of depth or
@@ -1305,7 +1313,7 @@ int integrate_tsdf_ws(const Workspace& ws, bool profile, const uint16_t* depth, 
 int integrate_tsdf_posed_ws(const Workspace& ws, const uint16_t* depth, float* depth_scaled, kfx_tsdf* volume,
                             const kfx_intrinsics* intr, const float* dev_pose, const float* dev_prev_pose_inv,
                             float motion_threshold, float truncation_distance, const kfx_point_3f* cell_size,
-                            const kfx_round_buffer* rb, const int32_t* dev_skip, uint8_t* bricks) {
+                            const kfx_round_buffer* rb, const int32_t* dev_skip, uint8_t* bricks, bool bound_ready) {
     KFX_REQUIRE(volume != nullptr && dev_pose != nullptr, "kfx_integrate_tsdf_posed: null pointer");
     const kfx_matrix_3f ident = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
     const kfx_point_3f zero = {0, 0, 0};
@@ -1318,7 +1326,7 @@ int integrate_tsdf_posed_ws(const Workspace& ws, const uint16_t* depth, float* d
     const dim3 grid = sweep_grid(p.nx, p.ny, p.nz, p.zrun);
     const uint8_t* flags = nullptr;
     KFX_TRY(integrate_prepass<true>(ws, true, depth, depth_scaled, intr, p, grid, dev_pose, dev_prev_pose_inv,
-                                    motion_threshold, dev_skip, &flags));
+                                    motion_threshold, dev_skip, &flags, bound_ready));
     ProfScope prof("integrate");
     const PoseTerms* terms = static_cast<const PoseTerms*>(ws.terms);
     const int32_t* skip = reinterpret_cast<const int32_t*>(static_cast<const char*>(ws.terms) + kTermsSkipOffset);
