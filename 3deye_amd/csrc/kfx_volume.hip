@@ -1027,10 +1027,14 @@ static int integrate_prepass(const Workspace& ws, bool profile, const uint16_t* 
     int32_t* skip = kDevicePose ? reinterpret_cast<int32_t*>(static_cast<char*>(ws.terms) + kTermsSkipOffset) : nullptr;
     ProfScope prof(profile ? "depth_bound" : nullptr);
     const bool pixels = !(bound_ready && have_bound && have_flags);
+    // The flags alone: one wave per workgroup, so that each wave's serial
+    // chain (pose terms, frustum test) has a SIMD to itself.
+    constexpr unsigned kFlagsBlock = 64;
     const dim3 grid = pixels ? dim3(divup(intr->cols, kBoundTile), divup(intr->rows, kBoundTile))
-                             : dim3((unsigned)divup(nblk, (size_t)(kBoundTile * kBoundTile)));
+                             : dim3((unsigned)divup(nblk, (size_t)kFlagsBlock));
+    const unsigned block = pixels ? kBoundTile * kBoundTile : kFlagsBlock;
     // igrid is sweep_grid's (y blocks, x blocks, z runs).
-    integrate_prepass_kernel<kDevicePose><<<grid, kBoundTile * kBoundTile, 0, stream()>>>(
+    integrate_prepass_kernel<kDevicePose><<<grid, block, 0, stream()>>>(
         depth, depth_scaled, *intr, have_bound ? ws.bound : nullptr, p, dpose, dprev, motion_thr, dskip, terms, skip,
         have_flags ? ws.flags : nullptr, (int)igrid.y, (int)igrid.x, (int)igrid.z, pixels ? 1 : 0);
     KFX_CHECK_LAUNCH();
