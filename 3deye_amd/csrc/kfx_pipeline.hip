@@ -744,6 +744,8 @@ int kfx_pipeline_state(kfx_pipeline* p, kfx_pipeline_view* v) {
         v->vmap_model_levels[l] = p->vp[l];
         v->nmap_model_levels[l] = p->np[l];
     }
+    v->depth_scaled = p->scaled;
+    v->depth_bound = p->ws.bound;
     v->last_points = p->cfg.extract_records ? nullptr : p->ext_pts;
     v->last_records = p->cfg.extract_records ? reinterpret_cast<kfx_slice_record*>(p->ext_pts) : nullptr;
     v->last_colors = p->ext_col;

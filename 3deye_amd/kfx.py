@@ -138,7 +138,8 @@ class PipelineView(C.Structure):
                 ("fallbacks", C.c_int32), ("extracted_points", C.c_uint64), ("last_points", C.c_void_p),
                 ("last_colors", C.c_void_p), ("color_volume", C.c_void_p), ("rgb", C.c_void_p),
                 ("last_records", C.c_void_p), ("brick_flags", C.c_void_p),
-                ("vmap_model_levels", C.c_void_p * 3), ("nmap_model_levels", C.c_void_p * 3)]
+                ("vmap_model_levels", C.c_void_p * 3), ("nmap_model_levels", C.c_void_p * 3),
+                ("depth_scaled", C.c_void_p), ("depth_bound", C.c_void_p)]
 
 
 PLAIN, FUSED = 0, 1

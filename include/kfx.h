@@ -640,6 +640,10 @@ typedef struct kfx_pipeline_view {
                                      NULL in plain mode or when an axis is not a multiple of 16 */
     kfx_xyz* vmap_model_levels[KFX_ICP_LEVELS]; /* the model maps of every ICP level (device; [0] is vmap_model) */
     kfx_xyz* nmap_model_levels[KFX_ICP_LEVELS];
+    float* depth_scaled;          /* the last frame's scaled depth (device, cols*rows floats) */
+    float* depth_bound;           /* the integration's depth-bound pyramid (device): levels 1..5 in order, level l
+                                     ceil(cols / 2^l) x ceil(rows / 2^l) texels, each the largest scaled depth
+                                     of its 2^l x 2^l pixels; NULL when the workspace holds none */
 } kfx_pipeline_view;
 
 typedef struct kfx_pipeline kfx_pipeline;

@@ -293,3 +293,35 @@ def test_fused_raycast_builds_the_map_pyramid(gpu, use_graph):
     for a, b in zip((got[1], got_n[1], got[2], got_n[2]), want):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     pipe.close()
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_fused_maps_launch_builds_scaled_depth_and_bound_pyramid(gpu, orc, use_graph):
+    # The fused pipeline builds the integration's scaled depth and depth-bound
+    # pyramid of the raw frame inside its maps launch (the pre-pass after ICP
+    # then computes flags only): the scaled depth equals the oracle's
+    # scale_depth of the last frame bit for bit, and every pyramid texel of
+    # levels 1..5 is the maximum of its 2^l x 2^l pixels (0 outside the image).
+    kfx = gpu
+    frames = 3
+    pipe, _infos, _gt = _c1_run(kfx, 1, use_graph=use_graph, frames=frames)
+    v = pipe.view()
+    assert v.depth_bound
+    intr = kfx.kinect_intrinsics()
+    W, H = intr.cols, intr.rows
+    d = kfx.render(scenes.c1_scene(), scenes.c1_poses(frames)[frames - 1], intr, scenes.C1_SEED, frames - 1,
+                   noise=True)
+    got = kfx.download(v.depth_scaled, np.float32, (H, W))
+    want = orc.scale_depth_f32(d, intr).astype(np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    sizes = [((W + (1 << l) - 1) >> l, (H + (1 << l) - 1) >> l) for l in range(1, 6)]
+    pyr = kfx.download(v.depth_bound, np.float32, (sum(w * h for w, h in sizes),))
+    pad = np.zeros(((H + 31) // 32 * 32, (W + 31) // 32 * 32), np.float32)
+    pad[:H, :W] = want
+    off = 0
+    for l, (w, h) in enumerate(sizes, start=1):
+        s = 1 << l
+        m = pad.reshape(pad.shape[0] // s, s, pad.shape[1] // s, s).max(axis=(1, 3))[:h, :w]
+        assert np.array_equal(pyr[off:off + w * h].reshape(h, w), m), f"level {l}"
+        off += w * h
+    pipe.close()
